@@ -1,0 +1,181 @@
+/*
+ * sqbem.h -- C ABI of libsqbem, the MI355X (gfx950) squirmer-in-tube BEM library.
+ *
+ * Plain C types only: host or device pointers, sizes, status codes. Every function
+ * returns 0 on success or a nonzero status, with a message from sqbem_last_error().
+ * Messages reuse the reference's error texts ("dgetrf failed", "dgesv failed",
+ * "gemm: dimension mismatch") so the C++ headers can rethrow them unchanged.
+ *
+ * Matrices are column-major with LAPACK leading dimensions. Device pointers are HIP
+ * device memory of the current device; device work is ordered on the library stream
+ * (sqbem_set_stream) and host-returning calls synchronise it.
+ *
+ * Reference citations are file:line in byjesh1998/squirmer-bem-3dtube.
+ */
+#ifndef SQBEM_H
+#define SQBEM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define SQBEM_OK 0
+#define SQBEM_ERR_ARG 1      /* bad argument or dimension mismatch */
+#define SQBEM_ERR_HIP 2      /* HIP runtime error (no device, launch failure, ...) */
+#define SQBEM_ERR_SINGULAR 3 /* zero pivot: LAPACK info > 0 */
+#define SQBEM_ERR_NOMEM 4    /* device allocation failed */
+#define SQBEM_ERR_STATE 5    /* object used in a state that does not allow the call */
+#define SQBEM_ERR_CONVERGE 6 /* iterative refinement did not reach its tolerance */
+
+/* arithmetic of the dense factorisations (operator assembly is FP64 in every mode) */
+#define SQBEM_FP64 0    /* FP64 everywhere: the drop-in path */
+#define SQBEM_FP32_IR 1 /* FP32 factors (FP32 MFMA updates), FP64 iterative refinement */
+#define SQBEM_FP16_IR 2 /* FP32 factors, FP16 MFMA trailing updates with per-block scaling */
+#define SQBEM_BF16_IR 3 /* FP32 factors, BF16 MFMA trailing updates */
+
+/* ------------------------------------------------------------------ runtime */
+const char* sqbem_last_error(void);
+int sqbem_init(int device);
+int sqbem_set_stream(void* hip_stream); /* NULL: the library's own stream */
+int sqbem_sync(void);
+int sqbem_device_info(char* name, int name_len, int* cu_count, double* hbm_gb);
+
+/* ------------------------------------------------------------------ meshes (host)
+ * Generators with the reference's node/panel order, src/utils/mesh.hpp:45-247. */
+typedef struct sqbem_mesh sqbem_mesh;
+/* swimmer_tube_mesh, src/utils/mesh.hpp:194-197 */
+sqbem_mesh* sqbem_mesh_swimmer_tube(double R, double L, double a, int n_theta, int n_r);
+/* tube_mesh over explicit axial nodes, src/utils/mesh.hpp:149-190 */
+sqbem_mesh* sqbem_mesh_tube(double R, const double* z, int nz_nodes, int n_theta, int n_r);
+/* tube_mesh(R, uniform_z(0, L, n_z), ...) as DuctFlow builds it, src/duct_flow.hpp:32-36 */
+sqbem_mesh* sqbem_mesh_duct(double R, double L, int n_theta, int n_z, int n_r);
+/* icosphere, src/utils/mesh.hpp:208-245 */
+sqbem_mesh* sqbem_mesh_icosphere(double a, int level, double cx, double cy, double cz);
+/* Mesh{nodes, tris, tags} + geometry(), src/utils/mesh.hpp:50-72 */
+sqbem_mesh* sqbem_mesh_from_arrays(const double* nodes, int n_nodes, const int* tris,
+                                   const int* tags, int n_panels);
+/* merge, src/utils/mesh.hpp:95-103 */
+sqbem_mesh* sqbem_mesh_merge(const sqbem_mesh* a, const sqbem_mesh* b);
+/* Mesh::translate, src/utils/mesh.hpp:82-85 */
+int sqbem_mesh_translate(sqbem_mesh* m, double dx, double dy, double dz);
+int sqbem_mesh_sizes(const sqbem_mesh* m, int* n_nodes, int* n_panels);
+/* copies out nodes[3nn], tris[3np], tags[np], V[9np], normal[3np], centroid[3np],
+ * area[np], h[np]; any pointer may be NULL */
+int sqbem_mesh_get(const sqbem_mesh* m, double* nodes, int* tris, int* tags, double* V,
+                   double* normal, double* centroid, double* area, double* h);
+void sqbem_mesh_free(sqbem_mesh* m);
+
+/* ------------------------------------------------------------------ dense algebra
+ * Device pointers; replace the OpenBLAS calls of src/utils/linalg.hpp:17-26. */
+/* C = alpha A B + beta C, dgemm_('N','N'), src/utils/linalg.hpp:42-48 */
+int sqbem_dgemm(int m, int n, int k, double alpha, const double* A, int lda, const double* B,
+                int ldb, double beta, double* C, int ldc);
+/* FP32 counterpart (FP32 MFMA) used by the mixed-precision factorisations */
+int sqbem_sgemm(int m, int n, int k, float alpha, const float* A, int lda, const float* B,
+                int ldb, float beta, float* C, int ldc);
+/* FP32 in/out with 16-bit MFMA products (kind SQBEM_FP16_IR or SQBEM_BF16_IR) and FP32
+ * accumulation; FP16 operands are scaled per row of A and per column of B by powers of
+ * two, undone exactly in the epilogue */
+int sqbem_gemm16(int kind, int m, int n, int k, float alpha, const float* A, int lda,
+                 const float* B, int ldb, float beta, float* C, int ldc);
+/* P A = L U with partial pivoting, dgetrf_, src/utils/linalg.hpp:65-71. ipiv is a device
+ * array of n 1-based pivot rows; *info (host) is 0 or the first zero-pivot column. */
+int sqbem_dgetrf(int n, double* A, int lda, int* ipiv, int* info);
+/* B <- A^-1 B from the dgetrf factors, dgetrs_('N'), src/utils/linalg.hpp:72-77 */
+int sqbem_dgetrs(int n, int nrhs, const double* LU, int lda, const int* ipiv, double* B, int ldb);
+/* B <- A^-T B from the dgetrf factors (dgetrs_('T')); used by the adjoint solves */
+int sqbem_dgetrs_trans(int n, int nrhs, const double* LU, int lda, const int* ipiv, double* B,
+                       int ldb);
+/* solve A X = B overwriting A with its factors, dgesv_, src/utils/linalg.hpp:53-58 */
+int sqbem_dgesv(int n, int nrhs, double* A, int lda, int* ipiv, double* B, int ldb, int* info);
+/* FP32 factorisation; kind selects the trailing-update MFMA (SQBEM_FP32_IR/FP16_IR/BF16_IR) */
+int sqbem_sgetrf(int kind, int n, float* A, int lda, int* ipiv, int* info);
+int sqbem_sgetrs(int n, int nrhs, const float* LU, int lda, const int* ipiv, float* B, int ldb);
+
+/* Host-array forms of the same calls, for the C++ linalg.hpp API (Mat lives in host
+ * memory): the library stages the operands through HBM. */
+/* gemm(alpha, A, B, beta, C), src/utils/linalg.hpp:42-48 */
+int sqbem_host_dgemm(int m, int n, int k, double alpha, const double* A, int lda, const double* B,
+                     int ldb, double beta, double* C, int ldc);
+/* solve_dense(A, B): B <- A^-1 B, A unchanged, src/utils/linalg.hpp:53-58 */
+int sqbem_host_dgesv(int n, int nrhs, const double* A, int lda, double* B, int ldb);
+/* LUFactor (src/utils/linalg.hpp:62-78) with the factors kept in HBM */
+typedef struct sqbem_lu sqbem_lu;
+/* LUFactor::factor: copies A (host) to the device and factors it; *info as dgetrf_ */
+sqbem_lu* sqbem_lu_create(int n, const double* A, int lda, int* info);
+/* LUFactor::solve: B (host, n x nrhs) <- A^-1 B */
+int sqbem_lu_solve(const sqbem_lu* lu, int nrhs, double* B, int ldb);
+/* copies the packed L\U factors (n x n, column-major) and 1-based pivots to the host */
+int sqbem_lu_get(const sqbem_lu* lu, double* LU, int* piv);
+int sqbem_lu_size(const sqbem_lu* lu);
+void sqbem_lu_free(sqbem_lu* lu);
+
+/* ------------------------------------------------------------------ panel integrals
+ * Device copy of a mesh's panel geometry, read by the assembly kernels. */
+typedef struct sqbem_dmesh sqbem_dmesh;
+sqbem_dmesh* sqbem_dmesh_upload(const sqbem_mesh* m);
+void sqbem_dmesh_free(sqbem_dmesh* d);
+/* The loops over panel_integrals (src/utils/kernels.hpp:67-82) of src/swimmer.hpp:52-57,
+ * 188-221 and src/duct_flow.hpp:90-102 as one device pass. For target i (x0: device,
+ * 3*nt doubles, xyz interleaved) and panel p0+p of src:
+ *   SL(3i+a, 3p+b) = SL_ab / sl_div,  DL(3i+a, 3p+b) = -DL_ab / dl_div,
+ *   dlsum[9i+k] += DL_k (raw stresslet integral, summed over the panels).
+ * self_src (device, nt ints, or NULL) names the panel containing target i (Duffy rule);
+ * SL, DL, dlsum are device pointers or NULL. */
+int sqbem_assemble(const sqbem_dmesh* src, int p0, int np, const double* x0, int nt,
+                   const int* self_src, double sl_div, double* SL, long ld_sl, double dl_div,
+                   double* DL, long ld_dl, double* dlsum);
+/* one pair on the device, host in/out: panel_integrals, src/utils/kernels.hpp:67-82 */
+int sqbem_panel_integrals(const sqbem_mesh* m, int p, const double* x0, int self, double* SL,
+                          double* DL);
+
+/* ------------------------------------------------------------------ solvers */
+typedef struct sqbem_tube sqbem_tube;
+typedef struct sqbem_swimmer sqbem_swimmer;
+typedef struct sqbem_duct sqbem_duct;
+
+/* Tube::Tube + Tube::factor (src/swimmer.hpp:40-60): assembles T = SL/(8 pi mu) with the
+ * rank-one deflation (src/swimmer.hpp:64-74) on the device and factors it there. */
+sqbem_tube* sqbem_tube_create(const sqbem_mesh* mesh, double mu, int precision);
+int sqbem_tube_info(const sqbem_tube* t, int* Nt, double* t_assembly, double* t_factor);
+/* the tube's factor (Tube::lu, src/swimmer.hpp:38): a view valid while the tube lives
+ * (FP64 mode); in the refined modes it solves with the FP32 factors */
+const sqbem_lu* sqbem_tube_lu(const sqbem_tube* t);
+void sqbem_tube_free(sqbem_tube* t);
+
+/* SwimmerSolver::SwimmerSolver + build (src/swimmer.hpp:92-97, 180-251): coupling
+ * assembly, rigid-body map, Schur complement against the tube factors. tube may be NULL
+ * (unbounded fluid). The sphere mesh is copied. */
+sqbem_swimmer* sqbem_swimmer_create(const sqbem_tube* tube, const sqbem_mesh* sphere,
+                                    const double* center, double mu, int precision);
+/* solve_free (src/swimmer.hpp:107-139), one solution per slip field. slips: host,
+ * nslip*Ns*3; outputs host: U[3*nslip], Om[3*nslip], f_sphere[nslip*Ns*3],
+ * f_tube[nslip*Nt*3] (outputs may be NULL) */
+int sqbem_swimmer_solve_free(sqbem_swimmer* s, int nslip, const double* slips, double* U,
+                             double* Om, double* f_sphere, double* f_tube);
+/* solve_prescribed (src/swimmer.hpp:142-153): towed rigid sphere, no slip */
+int sqbem_swimmer_solve_prescribed(sqbem_swimmer* s, const double* U, const double* Om,
+                                   double* f_sphere, double* f_tube);
+/* t_assembly / t_schur (src/swimmer.hpp:90, 181, 235, 250), seconds; iterations and final
+ * relative residual of the last refined solve (0 and 0 in FP64 mode) */
+int sqbem_swimmer_times(const sqbem_swimmer* s, double* t_assembly, double* t_schur,
+                        int* ir_iters, double* ir_residual);
+void sqbem_swimmer_free(sqbem_swimmer* s);
+
+/* velocity_at (src/utils/field.hpp:18-34) on the device: u, f host [3*np]; pts host
+ * [3*npts]; out host [3*npts] */
+int sqbem_velocity_at(const sqbem_mesh* m, const double* u, const double* f, double mu,
+                      const double* pts, int npts, double* out);
+
+/* DuctFlow (src/duct_flow.hpp:32-36, 82-108): mixed-BC assembly and factorisation */
+sqbem_duct* sqbem_duct_create(const sqbem_mesh* mesh, double mu);
+/* DuctFlow::solve (src/duct_flow.hpp:41-63): u, f host [3*np], *Q host */
+int sqbem_duct_solve(const sqbem_duct* d, double p_in, double p_out, double* u, double* f,
+                     double* Q);
+void sqbem_duct_free(sqbem_duct* d);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SQBEM_H */

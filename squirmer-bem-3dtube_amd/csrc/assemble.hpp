@@ -1,0 +1,58 @@
+// assemble.hpp -- internal interface of the panel-integral kernels (assemble.hip).
+#pragma once
+#include "../../include/utils/mesh.hpp"
+#include "common.hpp"
+
+namespace sq {
+
+constexpr int MODE_BLOCKS = 0;  // write SL / DL blocks and DL row sums
+constexpr int MODE_FIELD = 1;   // fold pairs into a velocity (representation formula)
+
+// Device copy of a mesh's panel geometry in structure-of-arrays form, plus the seven
+// far-field quadrature points of every panel (x, y, z, weight * area).
+struct DMesh {
+    int np = 0;
+    double total_area = 0;
+    DBuf<double> V;        // 9 x np: vertex k coordinate c at row 3k + c
+    DBuf<double> nrm;      // 3 x np
+    DBuf<double> cen;      // 3 x np
+    DBuf<double> area;     // np
+    DBuf<double> h;        // np
+    DBuf<double> q7;       // 28 x np
+    DBuf<double> cen_xyz;  // 3 np, xyz interleaved (targets of collocation)
+    DBuf<double> nrm_xyz;  // 3 np, xyz interleaved
+    DMesh() = default;
+    explicit DMesh(const bem::Mesh& m);
+    void upload(const bem::Mesh& m);
+};
+
+struct AsmArgs {
+    int nt;
+    const double* x0;
+    const int* self_src;
+    int p0, np, src_np;
+    const double *V, *nrm, *cen, *area, *h, *q7;
+    double sl_div, dl_div;
+    double* SL;
+    long ld_sl;
+    double* DL;
+    long ld_dl;
+    double* dls_part;
+    double defl_sc;
+    const double* defl_tn;
+    const double* fld_u;
+    const double* fld_f;
+    double* vel_part;
+};
+
+// SL(3i+a, 3p+b) = SL_ab/sl_div (+ defl_sc tn_i[a] n_p[b] A_p), DL(...) = -DL_ab/dl_div,
+// dlsum[9i+k] += DL_k. Device pointers; any output may be null.
+void assemble(const DMesh& src, int p0, int np, const double* x0, int nt, const int* self_src, double sl_div,
+              double* SL, long ld_sl, double dl_div, double* DL, long ld_dl, double* dlsum, double defl_sc,
+              const double* defl_tn);
+
+// velocity at npts points (device, xyz interleaved) from panel velocities u and
+// tractions f (device, 3 per panel): out (device, 3 npts)
+void field(const DMesh& src, const double* pts, int npts, const double* u, const double* f, double mu, double* out);
+
+}  // namespace sq

@@ -1,0 +1,133 @@
+// common.hpp -- libsqbem internals: error state, device context, RAII device buffers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/sqbem.h"
+
+namespace sq {
+
+// Error carried from the internals to the C ABI boundary.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define SQ_HIP(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            throw ::sq::Error(SQBEM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define SQ_REQUIRE(cond, code, msg)                 \
+    do {                                            \
+        if (!(cond)) throw ::sq::Error((code), (msg)); \
+    } while (0)
+
+// Per-process state: the device and the stream every launch goes to.
+struct Context {
+    int device = -1;
+    hipStream_t stream = nullptr;      // library-owned stream
+    hipStream_t user_stream = nullptr; // caller override (sqbem_set_stream)
+    int cu_count = 0;
+    size_t hbm_bytes = 0;
+    char name[256] = {0};
+    hipStream_t s() const { return user_stream ? user_stream : stream; }
+};
+
+Context& ctx();          // initialises device 0 on first use
+void ensure_init();      // throws on failure
+void upload_tables();    // quadrature tables to constant memory (assemble.hip)
+// Zero device bytes with a kernel on the library stream (no memset nodes, so the
+// sequence stays safe inside captured graphs).
+void fill_zero(void* p, size_t bytes);
+
+// Owned device allocation (no copy, movable).
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { alloc(count); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) return;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error(SQBEM_ERR_NOMEM, "device allocation of " + std::to_string(count * sizeof(T)) +
+                                             " bytes failed: " + hipGetErrorString(e));
+        }
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void zero() { if (p) fill_zero(p, n * sizeof(T)); }
+    void upload(const T* h, size_t count) {
+        SQ_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, ctx().s()));
+    }
+    void download(T* h, size_t count) const {
+        SQ_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, ctx().s()));
+        SQ_HIP(hipStreamSynchronize(ctx().s()));
+    }
+    T* get() const { return p; }
+};
+
+inline void sync() { SQ_HIP(hipStreamSynchronize(ctx().s())); }
+
+inline void check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw Error(SQBEM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// Wrap a C ABI body: exceptions become status codes plus sqbem_last_error().
+template <typename F>
+int guarded(F&& f) {
+    try {
+        ensure_init();
+        f();
+        return SQBEM_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return SQBEM_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return SQBEM_ERR_ARG;
+    }
+}
+
+template <typename T, typename F>
+T* guarded_ptr(F&& f) {
+    try {
+        ensure_init();
+        return f();
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return nullptr;
+    }
+}
+
+}  // namespace sq

@@ -1,0 +1,940 @@
+// dense.hip -- dense linear algebra for gfx950: MFMA GEMMs, partial-pivoting LU, triangular
+// solves. Replaces the OpenBLAS calls of src/utils/linalg.hpp:17-77 (dgemm_, dgetrf_,
+// dgetrs_, dgesv_).
+//
+// Design:
+//  * GEMM (C = alpha A B + beta C, column-major): 128x128 workgroup tiles, four waves of
+//    64x64, double-buffered LDS, MFMA 16x16x4 for FP64 and FP32, MFMA 16x16x32 for
+//    FP16/BF16 with FP32 accumulation and per-row/per-column power-of-two scaling.
+//    Workgroup ids are remapped so each XCD (own L2) works on a contiguous, grouped
+//    range of output tiles.
+//  * LU: recursive (Toledo) split of the columns, so almost all flops are large GEMMs.
+//    Leaves of <= 32 columns are factored by one cooperative kernel: the panel rows are
+//    spread over up to one workgroup per CU, held in LDS, and each column's pivot is
+//    chosen with one grid-wide barrier (double-buffered candidate rows). Pivot search,
+//    swap and scaling are done in FP64 whatever the storage type.
+//  * TRSM: recursive split with 64-row leaf kernels (one thread per right-hand side,
+//    the triangle in LDS), GEMM updates in between.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+#include "dense.hpp"
+
+namespace sq {
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+typedef __bf16 b8v __attribute__((ext_vector_type(8)));
+
+// ------------------------------------------------------------------ tile scheduling
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range
+// of logical tiles, then walk tiles in groups of GROUP_M tile-rows so neighbours in time
+// share A row-panels and B column-panels in that XCD's L2.
+__device__ __forceinline__ void tile_of(int bid, int nblocks, int tiles_m, int& tm, int& tn) {
+    constexpr int XCD = 8, GROUP_M = 8;
+    int xcd = bid % XCD, idx = bid / XCD;
+    int per = nblocks / XCD, rem = nblocks % XCD;
+    int logical = xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+    int tiles_n = nblocks / tiles_m;
+    int in_group = GROUP_M * tiles_n;
+    int g = logical / in_group;
+    int first = g * GROUP_M;
+    int gsz = min(tiles_m - first, GROUP_M);
+    int r = logical % in_group;
+    tm = first + r % gsz;
+    tn = r / gsz;
+}
+
+// ------------------------------------------------------------------ FP64 / FP32 GEMM
+template <typename T>
+struct MfmaOp;
+// Accumulator row of value r in lane group lk = lane/16 (column is lane%16), measured on
+// gfx950: FP64 16x16x4 interleaves the groups (row lk + 4r), the FP32/FP16/BF16 forms
+// keep them in blocks (row 4lk + r).
+template <>
+struct MfmaOp<double> {
+    using acc = d4v;
+    static __device__ __forceinline__ acc mma(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
+};
+template <>
+struct MfmaOp<float> {
+    using acc = f4v;
+    static __device__ __forceinline__ acc mma(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
+};
+
+constexpr int GBM = 128, GBN = 128, GBK = 16, GPAD = GBK + 1;
+
+// MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16] and B[l/16][l%16]; the four
+// accumulator values of lane l are D[Op::row(l/16, r)][l%16].
+template <typename T>
+__global__ void __launch_bounds__(256) gemm_nn_kernel(int m, int n, int k, T alpha, const T* __restrict__ A,
+                                                      int lda, const T* __restrict__ B, int ldb, T beta,
+                                                      T* __restrict__ C, int ldc, int tiles_m) {
+    using Op = MfmaOp<T>;
+    using Acc = typename Op::acc;
+    __shared__ T As[2][GBM][GPAD];
+    __shared__ T Bs[2][GBN][GPAD];
+
+    int tm, tn;
+    tile_of(blockIdx.x, gridDim.x, tiles_m, tm, tn);
+    const int m0 = tm * GBM, n0 = tn * GBN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+
+    // global -> register staging: A column kk has 128 contiguous rows; B column j has 16
+    // contiguous k values.
+    const int a_i = tid & 127, a_k0 = tid >> 7;  // rows a_i, k = a_k0 + 2q
+    const int b_k = tid & 15, b_j0 = tid >> 4;   // k = b_k, cols b_j0 + 16q
+    T ra[8], rb[8];
+
+    auto load = [&](int k0) {
+        const int gi = m0 + a_i;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            int gk = k0 + a_k0 + 2 * q;
+            ra[q] = (gi < m && gk < k) ? A[gi + (size_t)gk * lda] : T(0);
+        }
+        const int gk = k0 + b_k;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            int gj = n0 + b_j0 + 16 * q;
+            rb[q] = (gk < k && gj < n) ? B[gk + (size_t)gj * ldb] : T(0);
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) As[buf][a_i][a_k0 + 2 * q] = ra[q];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) Bs[buf][b_j0 + 16 * q][b_k] = rb[q];
+    };
+
+    Acc acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = Acc{0, 0, 0, 0};
+
+    const int nk = (k + GBK - 1) / GBK;
+    load(0);
+    store(0);
+    __syncthreads();
+    const int lr = lane & 15, lk = lane >> 4;
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) load((kt + 1) * GBK);
+#pragma unroll
+        for (int s = 0; s < GBK / 4; ++s) {
+            T af[4], bf[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) af[i] = As[buf][wm + i * 16 + lr][4 * s + lk];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bf[j] = Bs[buf][wn + j * 16 + lr][4 * s + lk];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = Op::mma(af[i], bf[j], acc[i][j]);
+        }
+        if (kt + 1 < nk) store(buf ^ 1);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = n0 + wn + j * 16 + lr;
+            if (col >= n) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wm + i * 16 + Op::row(lk, r);
+                if (row < m) {
+                    T* c = C + row + (size_t)col * ldc;
+                    T v = alpha * acc[i][j][r];
+                    *c = beta == T(0) ? v : v + beta * *c;
+                }
+            }
+        }
+}
+
+// ------------------------------------------------------------------ 16-bit MFMA GEMM
+// FP32 operands in global memory, converted to FP16/BF16 when staged into LDS, MFMA
+// 16x16x32 with FP32 accumulation. Optional per-row scale of A and per-column scale of B
+// (powers of two, so the unscaling in the epilogue is exact) keep FP16 values in range.
+constexpr int HBK = 64, HPAD = HBK + 8;
+
+template <bool BF16>
+struct Half;
+template <>
+struct Half<false> {
+    using t = _Float16;
+    using v8 = h8v;
+    static __device__ __forceinline__ f4v mma(v8 a, v8 b, f4v c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct Half<true> {
+    using t = __bf16;
+    using v8 = b8v;
+    static __device__ __forceinline__ f4v mma(v8 a, v8 b, f4v c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+// MFMA 16x16x32 operand layout: lane l supplies A[l%16][8*(l/16)+0..7] and
+// B[8*(l/16)+0..7][l%16]; accumulator value r of lane l is D[4*(l/16)+r][l%16].
+template <bool BF16>
+__global__ void __launch_bounds__(256) gemm16_nn_kernel(int m, int n, int k, float alpha,
+                                                        const float* __restrict__ A, int lda,
+                                                        const float* __restrict__ B, int ldb, float beta,
+                                                        float* __restrict__ C, int ldc, int tiles_m,
+                                                        const float* __restrict__ rs,
+                                                        const float* __restrict__ cs) {
+    using H = Half<BF16>;
+    using ht = typename H::t;
+    using v8 = typename H::v8;
+    __shared__ __attribute__((aligned(16))) ht As[2][GBM][HPAD];
+    __shared__ __attribute__((aligned(16))) ht Bs[2][GBN][HPAD];
+
+    int tm, tn;
+    tile_of(blockIdx.x, gridDim.x, tiles_m, tm, tn);
+    const int m0 = tm * GBM, n0 = tn * GBN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+
+    // A tile 128 x 64: thread reads rows a_i (contiguous along the column), k = a_k0 + 2q
+    const int a_i = tid & 127, a_k0 = tid >> 7;
+    // B tile 64 x 128: 16 threads per column, 4 consecutive k each
+    const int b_k = (tid & 15) * 4, b_j0 = tid >> 4;
+    float ra[32];
+    float rb[8][4];
+    const float sa = (rs && m0 + a_i < m) ? rs[m0 + a_i] : 1.0f;
+
+    auto load = [&](int k0) {
+        const int gi = m0 + a_i;
+#pragma unroll
+        for (int q = 0; q < 32; ++q) {
+            int gk = k0 + a_k0 + 2 * q;
+            ra[q] = (gi < m && gk < k) ? A[gi + (size_t)gk * lda] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            int gj = n0 + b_j0 + 16 * q;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                int gk = k0 + b_k + e;
+                rb[q][e] = (gk < k && gj < n) ? B[gk + (size_t)gj * ldb] : 0.0f;
+            }
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 32; ++q) As[buf][a_i][a_k0 + 2 * q] = (ht)(ra[q] * sa);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int j = b_j0 + 16 * q;
+            const float sb = (cs && n0 + j < n) ? cs[n0 + j] : 1.0f;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) Bs[buf][j][b_k + e] = (ht)(rb[q][e] * sb);
+        }
+    };
+
+    f4v acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f4v{0, 0, 0, 0};
+
+    const int nk = (k + HBK - 1) / HBK;
+    load(0);
+    store(0);
+    __syncthreads();
+    const int lr = lane & 15, lk = lane >> 4;
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) load((kt + 1) * HBK);
+#pragma unroll
+        for (int s = 0; s < HBK / 32; ++s) {
+            v8 af[4], bf[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) af[i] = *(const v8*)&As[buf][wm + i * 16 + lr][32 * s + 8 * lk];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bf[j] = *(const v8*)&Bs[buf][wn + j * 16 + lr][32 * s + 8 * lk];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = H::mma(af[i], bf[j], acc[i][j]);
+        }
+        if (kt + 1 < nk) store(buf ^ 1);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = n0 + wn + j * 16 + lr;
+            if (col >= n) continue;
+            const float ucol = cs ? 1.0f / cs[col] : 1.0f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wm + i * 16 + 4 * lk + r;
+                if (row < m) {
+                    const float urow = rs ? 1.0f / rs[row] : 1.0f;
+                    float* c = C + row + (size_t)col * ldc;
+                    float v = alpha * (acc[i][j][r] * urow * ucol);
+                    *c = beta == 0.0f ? v : v + beta * *c;
+                }
+            }
+        }
+}
+
+// Power-of-two scale per row of A (m x k) or per column of B (k x n) so the largest
+// magnitude maps to about 2^14 (FP16 max is 65504).
+__global__ void pow2_scale_rows(int m, int k, const float* __restrict__ A, int lda, float* __restrict__ s) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    float mx = 0.0f;
+    for (int c = 0; c < k; ++c) mx = fmaxf(mx, fabsf(A[i + (size_t)c * lda]));
+    s[i] = mx > 0.0f ? exp2f(14.0f - (float)ilogbf(mx)) : 1.0f;
+}
+__global__ void pow2_scale_cols(int k, int n, const float* __restrict__ B, int ldb, float* __restrict__ s) {
+    // one wave per column
+    int j = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    int lane = threadIdx.x & 63;
+    if (j >= n) return;
+    float mx = 0.0f;
+    for (int r = lane; r < k; r += 64) mx = fmaxf(mx, fabsf(B[r + (size_t)j * ldb]));
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+    if (lane == 0) s[j] = mx > 0.0f ? exp2f(14.0f - (float)ilogbf(mx)) : 1.0f;
+}
+
+// ------------------------------------------------------------------ host GEMM wrappers
+template <typename T>
+static void gemm_launch(int m, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb, T beta,
+                        T* C, int ldc) {
+    if (m <= 0 || n <= 0) return;
+    if (k <= 0 || alpha == T(0)) {
+        // C = beta C
+        if (beta == T(1)) return;
+        scale_matrix(m, n, beta, C, ldc);
+        return;
+    }
+    int tiles_m = (int)ceil_div(m, GBM), tiles_n = (int)ceil_div(n, GBN);
+    hipLaunchKernelGGL(gemm_nn_kernel<T>, dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, k, alpha, A,
+                       lda, B, ldb, beta, C, ldc, tiles_m);
+    check_launch("gemm");
+}
+
+void dgemm(int m, int n, int k, double alpha, const double* A, int lda, const double* B, int ldb, double beta,
+           double* C, int ldc) {
+    gemm_launch<double>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+void sgemm(int m, int n, int k, float alpha, const float* A, int lda, const float* B, int ldb, float beta,
+           float* C, int ldc) {
+    gemm_launch<float>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+struct ScaleWork {
+    DBuf<float> buf;
+    float* get(size_t n) {
+        if (buf.n < n) buf.alloc(std::max<size_t>(n, 1 << 16));
+        return buf.p;
+    }
+};
+static ScaleWork g_scale;
+
+void gemm16(int kind, int m, int n, int k, float alpha, const float* A, int lda, const float* B, int ldb,
+            float beta, float* C, int ldc) {
+    if (m <= 0 || n <= 0) return;
+    if (k <= 0 || alpha == 0.0f) {
+        if (beta != 1.0f) scale_matrix(m, n, beta, C, ldc);
+        return;
+    }
+    const bool bf = kind == SQBEM_BF16_IR;
+    float *rs = nullptr, *cs = nullptr;
+    if (!bf) {
+        float* s = g_scale.get((size_t)m + n);
+        rs = s;
+        cs = s + m;
+        hipLaunchKernelGGL(pow2_scale_rows, dim3(ceil_div(m, 256)), dim3(256), 0, ctx().s(), m, k, A, lda, rs);
+        hipLaunchKernelGGL(pow2_scale_cols, dim3(ceil_div(n, 4)), dim3(256), 0, ctx().s(), k, n, B, ldb, cs);
+    }
+    int tiles_m = (int)ceil_div(m, GBM), tiles_n = (int)ceil_div(n, GBN);
+    if (bf)
+        hipLaunchKernelGGL(gemm16_nn_kernel<true>, dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, k,
+                           alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, rs, cs);
+    else
+        hipLaunchKernelGGL(gemm16_nn_kernel<false>, dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, k,
+                           alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, rs, cs);
+    check_launch("gemm16");
+}
+
+// ------------------------------------------------------------------ elementwise helpers
+template <typename T>
+__global__ void scale_matrix_kernel(int m, int n, T beta, T* C, int ldc) {
+    size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    size_t total = (size_t)m * n;
+    for (; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        int i = (int)(idx % m), j = (int)(idx / m);
+        T* c = C + i + (size_t)j * ldc;
+        *c = beta == T(0) ? T(0) : beta * *c;
+    }
+}
+template <typename T>
+void scale_matrix(int m, int n, T beta, T* C, int ldc) {
+    size_t total = (size_t)m * n;
+    unsigned blocks = (unsigned)std::min<size_t>(ceil_div(total, 256), 65535);
+    hipLaunchKernelGGL(scale_matrix_kernel<T>, dim3(blocks), dim3(256), 0, ctx().s(), m, n, beta, C, ldc);
+    check_launch("scale_matrix");
+}
+template void scale_matrix<double>(int, int, double, double*, int);
+template void scale_matrix<float>(int, int, float, float*, int);
+
+template <typename Ti, typename To>
+__global__ void convert_kernel(int m, int n, const Ti* __restrict__ A, int lda, To* __restrict__ B, int ldb) {
+    size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    size_t total = (size_t)m * n;
+    for (; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        int i = (int)(idx % m), j = (int)(idx / m);
+        B[i + (size_t)j * ldb] = (To)A[i + (size_t)j * lda];
+    }
+}
+template <typename Ti, typename To>
+void convert_matrix(int m, int n, const Ti* A, int lda, To* B, int ldb) {
+    if (m <= 0 || n <= 0) return;
+    size_t total = (size_t)m * n;
+    unsigned blocks = (unsigned)std::min<size_t>(ceil_div(total, 256), 1 << 20);
+    hipLaunchKernelGGL((convert_kernel<Ti, To>), dim3(blocks), dim3(256), 0, ctx().s(), m, n, A, lda, B, ldb);
+    check_launch("convert_matrix");
+}
+template void convert_matrix<double, float>(int, int, const double*, int, float*, int);
+template void convert_matrix<float, double>(int, int, const float*, int, double*, int);
+template void convert_matrix<double, double>(int, int, const double*, int, double*, int);
+template void convert_matrix<float, float>(int, int, const float*, int, float*, int);
+
+// ------------------------------------------------------------------ cooperative panel LU
+constexpr int LEAF = 32;       // columns per cooperative panel kernel
+constexpr int PANEL_WG = 256;  // threads per workgroup
+
+struct PanelWork {
+    DBuf<double> cand;     // 2 x G x (LEAF + 2): (|pivot|, row, row values)
+    DBuf<double> rowj;     // 2 x LEAF: the row being replaced
+    DBuf<unsigned> bar;    // 2 ping-pong barrier counters
+    DBuf<int> status;      // [0] first zero pivot (1-based), [1] barrier timeout
+    int gmax = 0;
+    void ensure(int g) {
+        if (gmax >= g && cand.p) return;
+        gmax = std::max(g, 256);
+        cand.alloc((size_t)2 * gmax * (LEAF + 2));
+        rowj.alloc(2 * LEAF);
+        bar.alloc(2);
+        status.alloc(2);
+    }
+};
+static PanelWork g_panel;
+static int g_bar_parity = 0;
+
+// Bounded grid barrier: counter reaches target when every workgroup has arrived. A
+// sticky timeout flag ends all waits if a workgroup never arrives, so the kernel exits.
+__device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned target, int* timeout) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned spins = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+            if (++spins > (1u << 21)) {
+                __hip_atomic_store(timeout, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __threadfence();
+    }
+    __syncthreads();
+}
+
+// Factor the m x w panel (w <= LEAF) at A with partial pivoting (LAPACK dgetf2 order:
+// pivot = first row of largest magnitude, swap, scale by the reciprocal pivot, rank-1
+// update). ipiv[j] receives base + pivot row (0-based, absolute).
+template <typename T>
+__global__ void __launch_bounds__(PANEL_WG) getf2_coop(T* A, int lda, int m, int w, int rpw, int* ipiv,
+                                                       int base, double* cand, double* rowj, unsigned* bar,
+                                                       unsigned* bar_next, int* status) {
+    extern __shared__ double lds[];  // rpw x w, column-major, then scratch
+    const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
+    const int r0 = g * rpw, nr = max(0, min(m, r0 + rpw) - r0);
+    double* prow = lds + (size_t)rpw * w;   // pivot row values [LEAF]
+    double* redv = prow + LEAF;             // reduction scratch [PANEL_WG]
+    int* redi = (int*)(redv + PANEL_WG);    // [PANEL_WG]
+    __shared__ int s_piv;
+    __shared__ int s_best;
+    __shared__ double s_pval;
+
+    for (int e = tid; e < nr * w; e += PANEL_WG) {
+        int lr = e % nr, c = e / nr;
+        lds[c * rpw + lr] = (double)A[(r0 + lr) + (size_t)c * lda];
+    }
+    if (g == 0 && tid == 0) *bar_next = 0u;  // reset the other counter for the next launch
+    __syncthreads();
+
+    const int csz = LEAF + 2;
+    for (int j = 0; j < w; ++j) {
+        double* cj = cand + (size_t)(j & 1) * G * csz;
+        double* rj = rowj + (j & 1) * LEAF;
+        // local candidate: largest |a| in column j among rows >= j (first on ties)
+        double best = -1.0;
+        int brow = 0x7fffffff;
+        for (int lr = tid; lr < nr; lr += PANEL_WG) {
+            int r = r0 + lr;
+            if (r < j) continue;
+            double v = fabs(lds[j * rpw + lr]);
+            if (v > best || (v == best && r < brow)) { best = v; brow = r; }
+        }
+        redv[tid] = best;
+        redi[tid] = brow;
+        __syncthreads();
+        for (int off = PANEL_WG / 2; off > 0; off >>= 1) {
+            if (tid < off) {
+                double v = redv[tid + off];
+                int r = redi[tid + off];
+                if (v > redv[tid] || (v == redv[tid] && r < redi[tid])) { redv[tid] = v; redi[tid] = r; }
+            }
+            __syncthreads();
+        }
+        const double lbest = redv[0];
+        const int lrow = redi[0];
+        __syncthreads();
+        if (tid == 0) {
+            cj[(size_t)g * csz] = lbest;
+            cj[(size_t)g * csz + 1] = (double)lrow;
+        }
+        if (lbest >= 0.0)
+            for (int c = tid; c < w; c += PANEL_WG) cj[(size_t)g * csz + 2 + c] = lds[c * rpw + (lrow - r0)];
+        if (j >= r0 && j < r0 + nr)
+            for (int c = tid; c < w; c += PANEL_WG) rj[c] = lds[c * rpw + (j - r0)];
+
+        grid_barrier(bar, (unsigned)(j + 1) * G, status + 1);
+
+        // global pivot: every workgroup reduces the G candidates identically
+        best = -1.0;
+        brow = 0x7fffffff;
+        int bg = -1;
+        for (int q = tid; q < G; q += PANEL_WG) {
+            double v = cj[(size_t)q * csz];
+            int r = (int)cj[(size_t)q * csz + 1];
+            if (v > best || (v == best && v >= 0.0 && r < brow)) { best = v; brow = r; bg = q; }
+        }
+        redv[tid] = best;
+        redi[tid] = bg;
+        __syncthreads();
+        for (int off = PANEL_WG / 2; off > 0; off >>= 1) {
+            if (tid < off) {
+                double v = redv[tid + off];
+                int q2 = redi[tid + off], q1 = redi[tid];
+                int ra = q1 >= 0 ? (int)cj[(size_t)q1 * csz + 1] : 0x7fffffff;
+                int rb = q2 >= 0 ? (int)cj[(size_t)q2 * csz + 1] : 0x7fffffff;
+                if (v > redv[tid] || (v == redv[tid] && rb < ra)) { redv[tid] = v; redi[tid] = q2; }
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            // no finite candidate anywhere (column all NaN): keep row j, report a zero pivot
+            s_best = redi[0];
+            s_piv = redi[0] >= 0 ? (int)cj[(size_t)redi[0] * csz + 1] : j;
+        }
+        __syncthreads();
+        const int pr = s_piv;
+        const double* pvals = s_best >= 0 ? cj + (size_t)s_best * csz + 2 : rj;
+        for (int c = tid; c < w; c += PANEL_WG) prow[c] = pvals[c];
+        __syncthreads();
+        if (tid == 0) {
+            s_pval = prow[j];
+            if (g == 0) {
+                ipiv[j] = base + pr;
+                if (!(prow[j] != 0.0 && prow[j] == prow[j]) && status[0] == 0) status[0] = base + j + 1;
+            }
+        }
+        // swap rows j and pr inside the owning workgroups
+        if (pr != j) {
+            if (j >= r0 && j < r0 + nr)
+                for (int c = tid; c < w; c += PANEL_WG) lds[c * rpw + (j - r0)] = prow[c];
+            if (pr >= r0 && pr < r0 + nr)
+                for (int c = tid; c < w; c += PANEL_WG) lds[c * rpw + (pr - r0)] = rj[c];
+        }
+        __syncthreads();
+        const double pv = s_pval;
+        const double rinv = pv != 0.0 ? 1.0 / pv : 0.0;
+        for (int lr = tid; lr < nr; lr += PANEL_WG) {
+            if (r0 + lr <= j) continue;
+            double l = lds[j * rpw + lr] * rinv;
+            lds[j * rpw + lr] = l;
+            for (int c = j + 1; c < w; ++c) lds[c * rpw + lr] -= l * prow[c];
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < nr * w; e += PANEL_WG) {
+        int lr = e % nr, c = e / nr;
+        A[(r0 + lr) + (size_t)c * lda] = (T)lds[c * rpw + lr];
+    }
+}
+
+template <typename T>
+static void panel_leaf(T* A, int lda, int m, int w, int* ipiv, int base) {
+    // one workgroup per >= 128 rows, at most one per CU
+    int G = std::max(1, std::min((int)ceil_div(m, 128), std::max(ctx().cu_count, 1)));
+    int rpw = (int)ceil_div(m, G);
+    G = (int)ceil_div(m, rpw);
+    g_panel.ensure(G);
+    size_t shmem = ((size_t)rpw * w + LEAF + PANEL_WG) * sizeof(double) + PANEL_WG * sizeof(int);
+    // default dynamic LDS limit per workgroup (64 KiB): up to about 60k rows per panel
+    SQ_REQUIRE(shmem <= 64 * 1024, SQBEM_ERR_ARG, "panel too tall for LDS (more than ~60k rows)");
+    unsigned* bar = g_panel.bar.p + g_bar_parity;
+    unsigned* bar_next = g_panel.bar.p + (g_bar_parity ^ 1);
+    g_bar_parity ^= 1;
+    double* cand = g_panel.cand.p;
+    double* rowj = g_panel.rowj.p;
+    int* status = g_panel.status.p;
+    void* args[] = {&A, &lda, &m, &w, &rpw, &ipiv, &base, &cand, &rowj, &bar, &bar_next, &status};
+    SQ_HIP(hipLaunchCooperativeKernel((const void*)getf2_coop<T>, dim3(G), dim3(PANEL_WG), args, (unsigned)shmem,
+                                      ctx().s()));
+}
+
+// ------------------------------------------------------------------ row interchanges
+// For every column: rows j and p = ipiv[ibase+j] - vbase swapped, j = j0..j1-1 in order
+// (dir > 0) or in reverse order (dir < 0, the inverse permutation).
+template <typename T>
+__global__ void laswp_kernel(T* A, int lda, int ncols, const int* __restrict__ ipiv, int ibase, int vbase, int j0,
+                             int j1, int dir) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    T* col = A + (size_t)c * lda;
+    if (dir > 0) {
+        for (int j = j0; j < j1; ++j) {
+            int p = ipiv[ibase + j] - vbase;
+            if (p != j) { T t = col[j]; col[j] = col[p]; col[p] = t; }
+        }
+    } else {
+        for (int j = j1 - 1; j >= j0; --j) {
+            int p = ipiv[ibase + j] - vbase;
+            if (p != j) { T t = col[j]; col[j] = col[p]; col[p] = t; }
+        }
+    }
+}
+
+template <typename T>
+static void laswp(T* A, int lda, int ncols, const int* ipiv, int ibase, int vbase, int j0, int j1, int dir = 1) {
+    if (ncols <= 0 || j1 <= j0) return;
+    hipLaunchKernelGGL(laswp_kernel<T>, dim3(ceil_div(ncols, 256)), dim3(256), 0, ctx().s(), A, lda, ncols, ipiv,
+                       ibase, vbase, j0, j1, dir);
+    check_launch("laswp");
+}
+
+// ------------------------------------------------------------------ triangular leaves
+// B (w x nrhs, w <= 64) <- T^-1 B for T the w x w triangle at L. One wave per 64
+// right-hand sides: the triangle and the 64 columns are staged in LDS (coalesced), each
+// lane solves one column. The triangle is padded to 64 with an identity.
+constexpr int TL = 64;
+template <typename T, bool LOWER, bool UNIT, bool TRANS>
+__global__ void __launch_bounds__(64) trsm_leaf(int w, int nrhs, const T* __restrict__ L, int ldl, T* B, int ldb) {
+    __shared__ T Ls[TL][TL + 1];
+    __shared__ T Xs[TL][64];
+    const int tid = threadIdx.x, c0 = blockIdx.x * 64;
+    const int nc = min(64, nrhs - c0);
+    for (int e = tid; e < TL * TL; e += 64) {
+        int i = e % TL, j = e / TL;
+        T v = (i == j) ? T(1) : T(0);
+        if (i < w && j < w) v = TRANS ? L[j + (size_t)i * ldl] : L[i + (size_t)j * ldl];
+        Ls[i][j] = v;
+    }
+    for (int e = tid; e < TL * 64; e += 64) {
+        int i = e % TL, c = e / TL;
+        Xs[i][c] = (i < w && c < nc) ? B[i + (size_t)(c0 + c) * ldb] : T(0);
+    }
+    __syncthreads();
+    if (LOWER) {
+        for (int i = 0; i < TL; ++i) {
+            T s = Xs[i][tid];
+            for (int j = 0; j < i; ++j) s -= Ls[i][j] * Xs[j][tid];
+            Xs[i][tid] = UNIT ? s : s / Ls[i][i];
+        }
+    } else {
+        for (int i = TL - 1; i >= 0; --i) {
+            T s = Xs[i][tid];
+            for (int j = i + 1; j < TL; ++j) s -= Ls[i][j] * Xs[j][tid];
+            Xs[i][tid] = UNIT ? s : s / Ls[i][i];
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < TL * 64; e += 64) {
+        int i = e % TL, c = e / TL;
+        if (i < w && c < nc) B[i + (size_t)(c0 + c) * ldb] = Xs[i][c];
+    }
+}
+
+template <typename T>
+static void gemm_kind(int kind, int m, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb, T beta, T* C,
+                      int ldc);
+template <>
+void gemm_kind<double>(int, int m, int n, int k, double alpha, const double* A, int lda, const double* B, int ldb,
+                       double beta, double* C, int ldc) {
+    dgemm(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+template <>
+void gemm_kind<float>(int kind, int m, int n, int k, float alpha, const float* A, int lda, const float* B, int ldb,
+                      float beta, float* C, int ldc) {
+    if (kind == SQBEM_FP16_IR || kind == SQBEM_BF16_IR)
+        gemm16(kind, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    else
+        sgemm(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+static int split_point(int w, int unit) {
+    int h = (w / 2 + unit - 1) / unit * unit;
+    return std::min(std::max(h, unit), w - 1);
+}
+
+// Recursive triangular solve with the n x n triangle at L (leading dim ldl):
+//   LOWER && UNIT : B <- L^-1 B (unit lower, the L of an LU)
+//   !LOWER        : B <- U^-1 B (upper, non-unit)
+// TRANS uses the transposed triangle (for the A^T solves of the adjoint).
+template <typename T, bool LOWER, bool UNIT, bool TRANS>
+static void trsm_rec(int kind, int n, int nrhs, const T* L, int ldl, T* B, int ldb) {
+    if (n <= 0 || nrhs <= 0) return;
+    if (n <= TL) {
+        hipLaunchKernelGGL((trsm_leaf<T, LOWER, UNIT, TRANS>), dim3(ceil_div(nrhs, 64)), dim3(64), 0, ctx().s(), n,
+                           nrhs, L, ldl, B, ldb);
+        check_launch("trsm_leaf");
+        return;
+    }
+    const int n1 = split_point(n, TL), n2 = n - n1;
+    // effective triangle blocks: L11 = L[0:n1,0:n1], L22 = L[n1:,n1:], off-diagonal X
+    const T* L11 = L;
+    const T* L22 = L + n1 + (size_t)n1 * ldl;
+    T* B1 = B;
+    T* B2 = B + n1;
+    if (LOWER) {
+        // (effective lower) X1 = L11^-1 B1; B2 -= L21 X1; X2 = L22^-1 B2
+        trsm_rec<T, LOWER, UNIT, TRANS>(kind, n1, nrhs, L11, ldl, B1, ldb);
+        if (!TRANS)
+            gemm_kind<T>(kind, n2, nrhs, n1, T(-1), L + n1, ldl, B1, ldb, T(1), B2, ldb);
+        else
+            gemm_tn<T>(kind, n2, nrhs, n1, T(-1), L + (size_t)n1 * ldl, ldl, B1, ldb, T(1), B2, ldb);
+        trsm_rec<T, LOWER, UNIT, TRANS>(kind, n2, nrhs, L22, ldl, B2, ldb);
+    } else {
+        // (effective upper) X2 = U22^-1 B2; B1 -= U12 X2; X1 = U11^-1 B1
+        trsm_rec<T, LOWER, UNIT, TRANS>(kind, n2, nrhs, L22, ldl, B2, ldb);
+        if (!TRANS)
+            gemm_kind<T>(kind, n1, nrhs, n2, T(-1), L + (size_t)n1 * ldl, ldl, B2, ldb, T(1), B1, ldb);
+        else
+            gemm_tn<T>(kind, n1, nrhs, n2, T(-1), L + n1, ldl, B2, ldb, T(1), B1, ldb);
+        trsm_rec<T, LOWER, UNIT, TRANS>(kind, n1, nrhs, L11, ldl, B1, ldb);
+    }
+}
+
+// ------------------------------------------------------------------ transposed-A GEMM
+// C = alpha A^T B + beta C (A is k x m). Used only by the transposed triangular solves
+// (adjoint); FP64/FP32 via a transposing copy into a workspace, then the NN kernel.
+struct TransWork {
+    DBuf<char> buf;
+    void* get(size_t bytes) {
+        if (buf.n < bytes) buf.alloc(std::max<size_t>(bytes, 1 << 20));
+        return buf.p;
+    }
+};
+static TransWork g_trans;
+
+template <typename T>
+__global__ void transpose_kernel(int rows, int cols, const T* __restrict__ A, int lda, T* __restrict__ B, int ldb) {
+    __shared__ T tile[32][33];
+    int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 32 x 8
+    for (int r = ty; r < 32; r += 8) {
+        int i = bx + tx, j = by + r;
+        if (i < rows && j < cols) tile[r][tx] = A[i + (size_t)j * lda];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        int j = by + tx, i = bx + r;
+        if (i < rows && j < cols) B[j + (size_t)i * ldb] = tile[tx][r];
+    }
+}
+
+template <typename T>
+void gemm_tn(int kind, int m, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb, T beta, T* C, int ldc) {
+    if (m <= 0 || n <= 0) return;
+    T* At = (T*)g_trans.get((size_t)m * k * sizeof(T));
+    hipLaunchKernelGGL(transpose_kernel<T>, dim3(ceil_div(k, 32), ceil_div(m, 32)), dim3(256), 0, ctx().s(), k, m, A,
+                       lda, At, m);
+    check_launch("transpose");
+    gemm_kind<T>(kind, m, n, k, alpha, At, m, B, ldb, beta, C, ldc);
+}
+template void gemm_tn<double>(int, int, int, int, double, const double*, int, const double*, int, double, double*,
+                              int);
+template void gemm_tn<float>(int, int, int, int, float, const float*, int, const float*, int, float, float*, int);
+
+// ------------------------------------------------------------------ recursive LU
+template <typename T>
+static void getrf_rec(int kind, T* A, int lda, int m, int w, int* ipiv, int base) {
+    if (w <= LEAF) {
+        panel_leaf<T>(A, lda, m, w, ipiv + base, base);
+        return;
+    }
+    const int w1 = split_point(w, LEAF), w2 = w - w1;
+    getrf_rec<T>(kind, A, lda, m, w1, ipiv, base);
+    T* A12 = A + (size_t)w1 * lda;
+    laswp<T>(A12, lda, w2, ipiv, base, base, 0, w1);
+    trsm_rec<T, true, true, false>(kind, w1, w2, A, lda, A12, lda);
+    gemm_kind<T>(kind, m - w1, w2, w1, T(-1), A + w1, lda, A12, lda, T(1), A12 + w1, lda);
+    getrf_rec<T>(kind, A12 + w1, lda, m - w1, w2, ipiv, base + w1);
+    laswp<T>(A, lda, w1, ipiv, base, base, w1, std::min(w, m));
+}
+
+template <typename T>
+__global__ void ipiv_to_one_based(int n, int* ipiv, int add) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ipiv[i] += add;
+}
+
+template <typename T>
+int getrf(int kind, int n, T* A, int lda, int* ipiv) {
+    if (n <= 0) return 0;
+    SQ_REQUIRE(lda >= n, SQBEM_ERR_ARG, "getrf: lda < n");
+    g_panel.ensure(256);
+    g_panel.bar.zero();
+    g_panel.status.zero();
+    g_bar_parity = 0;
+    getrf_rec<T>(kind, A, lda, n, n, ipiv, 0);
+    hipLaunchKernelGGL(ipiv_to_one_based<T>, dim3(ceil_div(n, 256)), dim3(256), 0, ctx().s(), n, ipiv, 1);
+    check_launch("ipiv");
+    int st[2];
+    g_panel.status.download(st, 2);
+    SQ_REQUIRE(st[1] == 0, SQBEM_ERR_HIP, "getrf: panel barrier timed out");
+    return st[0];
+}
+template int getrf<double>(int, int, double*, int, int*);
+template int getrf<float>(int, int, float*, int, int*);
+
+// B <- A^-1 B (trans = 0) or A^-T B (trans = 1) from getrf factors; ipiv 1-based
+template <typename T>
+void getrs(int kind, int trans, int n, int nrhs, const T* LU, int lda, const int* ipiv, T* B, int ldb) {
+    if (n <= 0 || nrhs <= 0) return;
+    if (!trans) {
+        laswp<T>(B, ldb, nrhs, ipiv, 0, 1, 0, n, +1);
+        trsm_rec<T, true, true, false>(kind, n, nrhs, LU, lda, B, ldb);
+        trsm_rec<T, false, false, false>(kind, n, nrhs, LU, lda, B, ldb);
+    } else {
+        // A^T = U^T L^T P: solve U^T (lower, non-unit), L^T (upper, unit), then P^T
+        trsm_rec<T, true, false, true>(kind, n, nrhs, LU, lda, B, ldb);
+        trsm_rec<T, false, true, true>(kind, n, nrhs, LU, lda, B, ldb);
+        laswp<T>(B, ldb, nrhs, ipiv, 0, 1, 0, n, -1);
+    }
+}
+template void getrs<double>(int, int, int, int, const double*, int, const int*, double*, int);
+template void getrs<float>(int, int, int, int, const float*, int, const int*, float*, int);
+
+}  // namespace sq
+
+// ------------------------------------------------------------------ C ABI
+using namespace sq;
+
+extern "C" {
+
+int sqbem_dgemm(int m, int n, int k, double alpha, const double* A, int lda, const double* B, int ldb, double beta,
+                double* C, int ldc) {
+    return guarded([&] {
+        SQ_REQUIRE(m >= 0 && n >= 0 && k >= 0 && lda >= std::max(1, m) && ldb >= std::max(1, k) &&
+                       ldc >= std::max(1, m),
+                   SQBEM_ERR_ARG, "gemm: dimension mismatch");
+        dgemm(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    });
+}
+
+int sqbem_sgemm(int m, int n, int k, float alpha, const float* A, int lda, const float* B, int ldb, float beta,
+                float* C, int ldc) {
+    return guarded([&] {
+        SQ_REQUIRE(m >= 0 && n >= 0 && k >= 0 && lda >= std::max(1, m) && ldb >= std::max(1, k) &&
+                       ldc >= std::max(1, m),
+                   SQBEM_ERR_ARG, "gemm: dimension mismatch");
+        sgemm(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    });
+}
+
+int sqbem_gemm16(int kind, int m, int n, int k, float alpha, const float* A, int lda, const float* B, int ldb,
+                 float beta, float* C, int ldc) {
+    return guarded([&] {
+        SQ_REQUIRE(kind == SQBEM_FP16_IR || kind == SQBEM_BF16_IR, SQBEM_ERR_ARG, "gemm16: kind must be FP16 or BF16");
+        SQ_REQUIRE(m >= 0 && n >= 0 && k >= 0 && lda >= std::max(1, m) && ldb >= std::max(1, k) &&
+                       ldc >= std::max(1, m),
+                   SQBEM_ERR_ARG, "gemm: dimension mismatch");
+        gemm16(kind, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    });
+}
+
+int sqbem_dgetrf(int n, double* A, int lda, int* ipiv, int* info) {
+    return guarded([&] {
+        SQ_REQUIRE(n >= 0 && lda >= std::max(1, n), SQBEM_ERR_ARG, "dgetrf: bad dimensions");
+        int i = getrf<double>(SQBEM_FP64, n, A, lda, ipiv);
+        if (info) *info = i;
+        SQ_REQUIRE(i == 0, SQBEM_ERR_SINGULAR, "dgetrf failed");
+    });
+}
+
+int sqbem_dgetrs(int n, int nrhs, const double* LU, int lda, const int* ipiv, double* B, int ldb) {
+    return guarded([&] {
+        SQ_REQUIRE(n >= 0 && nrhs >= 0 && lda >= std::max(1, n) && ldb >= std::max(1, n), SQBEM_ERR_ARG,
+                   "dgetrs: bad dimensions");
+        getrs<double>(SQBEM_FP64, 0, n, nrhs, LU, lda, ipiv, B, ldb);
+    });
+}
+
+int sqbem_dgetrs_trans(int n, int nrhs, const double* LU, int lda, const int* ipiv, double* B, int ldb) {
+    return guarded([&] {
+        SQ_REQUIRE(n >= 0 && nrhs >= 0 && lda >= std::max(1, n) && ldb >= std::max(1, n), SQBEM_ERR_ARG,
+                   "dgetrs: bad dimensions");
+        getrs<double>(SQBEM_FP64, 1, n, nrhs, LU, lda, ipiv, B, ldb);
+    });
+}
+
+int sqbem_dgesv(int n, int nrhs, double* A, int lda, int* ipiv, double* B, int ldb, int* info) {
+    return guarded([&] {
+        SQ_REQUIRE(n >= 0 && nrhs >= 0 && lda >= std::max(1, n) && ldb >= std::max(1, n), SQBEM_ERR_ARG,
+                   "dgesv: bad dimensions");
+        int i = getrf<double>(SQBEM_FP64, n, A, lda, ipiv);
+        if (info) *info = i;
+        SQ_REQUIRE(i == 0, SQBEM_ERR_SINGULAR, "dgesv failed");
+        getrs<double>(SQBEM_FP64, 0, n, nrhs, A, lda, ipiv, B, ldb);
+    });
+}
+
+int sqbem_sgetrf(int kind, int n, float* A, int lda, int* ipiv, int* info) {
+    return guarded([&] {
+        SQ_REQUIRE(n >= 0 && lda >= std::max(1, n), SQBEM_ERR_ARG, "sgetrf: bad dimensions");
+        SQ_REQUIRE(kind == SQBEM_FP32_IR || kind == SQBEM_FP16_IR || kind == SQBEM_BF16_IR, SQBEM_ERR_ARG,
+                   "sgetrf: kind must be FP32_IR, FP16_IR or BF16_IR");
+        int i = getrf<float>(kind, n, A, lda, ipiv);
+        if (info) *info = i;
+        SQ_REQUIRE(i == 0, SQBEM_ERR_SINGULAR, "dgetrf failed");
+    });
+}
+
+int sqbem_sgetrs(int n, int nrhs, const float* LU, int lda, const int* ipiv, float* B, int ldb) {
+    return guarded([&] {
+        SQ_REQUIRE(n >= 0 && nrhs >= 0 && lda >= std::max(1, n) && ldb >= std::max(1, n), SQBEM_ERR_ARG,
+                   "sgetrs: bad dimensions");
+        getrs<float>(SQBEM_FP32_IR, 0, n, nrhs, LU, lda, ipiv, B, ldb);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,16 @@
+// lu_handle.hpp -- the opaque sqbem_lu of the C ABI: LU factors resident in HBM, either
+// owned (sqbem_lu_create) or a view onto a tube's factors.
+#pragma once
+#include "common.hpp"
+
+struct sqbem_lu {
+    int n = 0;
+    int kind = SQBEM_FP64;       // trailing-update kind of an FP32 factor
+    const double* LU = nullptr;  // FP64 factors, or
+    const float* LU32 = nullptr; // FP32 factors (refined modes)
+    const int* piv = nullptr;    // 1-based pivots
+    sq::DBuf<double> own;
+    sq::DBuf<int> own_piv;
+    // B (device, n x nrhs) <- A^-1 B
+    void solve_device(int nrhs, double* B, int ldb) const;
+};

@@ -1,0 +1,103 @@
+// runtime.hip -- device selection, library stream, error reporting, small utility kernels.
+#include <cstring>
+#include <mutex>
+
+#include "common.hpp"
+
+namespace sq {
+
+namespace {
+thread_local std::string g_last_error;
+Context g_ctx;
+std::mutex g_init_mutex;
+
+__global__ void fill_zero_kernel(uint4* p16, size_t n16, unsigned char* tail, size_t ntail) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = i; k < n16; k += stride) p16[k] = make_uint4(0, 0, 0, 0);
+    if (i < ntail) tail[i] = 0;
+}
+}  // namespace
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+static void init_device(int device) {
+    int n = 0;
+    SQ_HIP(hipGetDeviceCount(&n));
+    SQ_REQUIRE(n > 0, SQBEM_ERR_HIP, "no HIP device visible");
+    SQ_REQUIRE(device >= 0 && device < n, SQBEM_ERR_ARG, "device index out of range");
+    SQ_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    SQ_HIP(hipGetDeviceProperties(&prop, device));
+    g_ctx.device = device;
+    g_ctx.cu_count = prop.multiProcessorCount;
+    g_ctx.hbm_bytes = prop.totalGlobalMem;
+    std::snprintf(g_ctx.name, sizeof g_ctx.name, "%s (%s)", prop.name, prop.gcnArchName);
+    if (!g_ctx.stream) SQ_HIP(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+    upload_tables();
+}
+
+void ensure_init() {
+    if (g_ctx.device >= 0) {
+        int cur = -1;
+        SQ_HIP(hipGetDevice(&cur));
+        if (cur != g_ctx.device) SQ_HIP(hipSetDevice(g_ctx.device));
+        return;
+    }
+    std::lock_guard<std::mutex> lock(g_init_mutex);
+    if (g_ctx.device < 0) init_device(0);
+}
+
+Context& ctx() { return g_ctx; }
+
+void fill_zero(void* p, size_t bytes) {
+    if (!p || bytes == 0) return;
+    // device allocations are 256-byte aligned; callers pass whole buffers or aligned views
+    size_t n16 = bytes / 16, ntail = bytes % 16;
+    unsigned blocks = (unsigned)std::min<size_t>(std::max<size_t>(ceil_div(n16, 256), 1), 8192);
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(blocks), dim3(256), 0, ctx().s(), (uint4*)p, n16,
+                       (unsigned char*)p + 16 * n16, ntail);
+    check_launch("fill_zero");
+}
+
+}  // namespace sq
+
+using namespace sq;
+
+extern "C" {
+
+const char* sqbem_last_error(void) { return g_last_error.c_str(); }
+
+int sqbem_init(int device) {
+    try {
+        std::lock_guard<std::mutex> lock(g_init_mutex);
+        if (g_ctx.device == device) return SQBEM_OK;
+        SQ_REQUIRE(g_ctx.device < 0, SQBEM_ERR_STATE, "sqbem_init: already bound to another device");
+        init_device(device);
+        return SQBEM_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    }
+}
+
+int sqbem_set_stream(void* s) {
+    return guarded([&] { g_ctx.user_stream = (hipStream_t)s; });
+}
+
+int sqbem_sync(void) {
+    return guarded([&] { sync(); });
+}
+
+int sqbem_device_info(char* name, int len, int* cu_count, double* hbm_gb) {
+    return guarded([&] {
+        if (name && len > 0) {
+            std::strncpy(name, g_ctx.name, (size_t)len - 1);
+            name[len - 1] = 0;
+        }
+        if (cu_count) *cu_count = g_ctx.cu_count;
+        if (hbm_gb) *hbm_gb = g_ctx.hbm_bytes / 1e9;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,883 @@
+// solver.hip -- device-resident solvers behind the C ABI: Tube (T factor), SwimmerSolver
+// (coupling assembly, Schur complement, bordered force/torque-free solve, iterative
+// refinement for the mixed-precision modes), DuctFlow and the field evaluation.
+//
+// Follows src/swimmer.hpp:33-289, src/duct_flow.hpp:22-109 and src/utils/field.hpp:18-34.
+// Everything stays in HBM between calls; only slip fields, rigid-body velocities and
+// tractions cross to the host.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "assemble.hpp"
+#include "common.hpp"
+#include "dense.hpp"
+#include "lu_handle.hpp"
+#include "mesh_handle.hpp"
+
+namespace sq {
+
+static double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static bool mixed(int prec) { return prec == SQBEM_FP32_IR || prec == SQBEM_FP16_IR || prec == SQBEM_BF16_IR; }
+
+// ------------------------------------------------------------------ small kernels
+__global__ void iota_kernel(int n, int* a) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = i;
+}
+
+// AuBs(3i+a, 3i+b) += dlsum_i[3a+b] / (8 pi) - (unbounded ? delta_ab : 0), src/swimmer.hpp:218-220
+__global__ void completion_diag_kernel(int Ns, double* A, long lda, const double* __restrict__ dls, double eight_pi,
+                                       double minus_identity) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Ns) return;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            A[(3 * (size_t)i + a) + (3 * (size_t)i + b) * lda] += dls[9 * (size_t)i + 3 * a + b] / eight_pi -
+                                                                  (a == b ? minus_identity : 0.0);
+}
+
+// C(row, 0..5) = sum_col A(row, col) K(col, 0..5) for the rigid-body map K of
+// src/swimmer.hpp:223-231 (u = U + Omega x r); partial sums over panel chunks.
+constexpr int KP_CHUNKS = 16;
+__global__ void kproj_kernel(int nrows, int Ns, const double* __restrict__ A, long lda, const double* __restrict__ cen,
+                             double xcx, double xcy, double xcz, double* __restrict__ part) {
+    int row = blockIdx.x * blockDim.x + threadIdx.x;
+    int chunk = blockIdx.y;
+    if (row >= nrows) return;
+    int per = (Ns + KP_CHUNKS - 1) / KP_CHUNKS;
+    int pa = chunk * per, pb = min(Ns, pa + per);
+    double c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
+    for (int p = pa; p < pb; ++p) {
+        const double a0 = A[row + (size_t)(3 * p) * lda];
+        const double a1 = A[row + (size_t)(3 * p + 1) * lda];
+        const double a2 = A[row + (size_t)(3 * p + 2) * lda];
+        const double rx = cen[3 * (size_t)p] - xcx, ry = cen[3 * (size_t)p + 1] - xcy, rz = cen[3 * (size_t)p + 2] - xcz;
+        c0 += a0;
+        c1 += a1;
+        c2 += a2;
+        c3 += -a1 * rz + a2 * ry;
+        c4 += a0 * rz - a2 * rx;
+        c5 += -a0 * ry + a1 * rx;
+    }
+    double* o = part + (size_t)chunk * 6 * nrows + row;
+    o[0] = c0; o[(size_t)nrows] = c1; o[2 * (size_t)nrows] = c2;
+    o[3 * (size_t)nrows] = c3; o[4 * (size_t)nrows] = c4; o[5 * (size_t)nrows] = c5;
+}
+__global__ void kproj_reduce(int nrows, const double* __restrict__ part, double* C, long ldc) {
+    int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= nrows) return;
+    for (int c = 0; c < 6; ++c) {
+        double s = 0;
+        for (int k = 0; k < KP_CHUNKS; ++k) s += part[((size_t)k * 6 + c) * nrows + row];
+        C[row + (size_t)c * ldc] = s;
+    }
+}
+
+// max |A(i,j)| over an m x n block (non-negative doubles order like their bit patterns)
+template <typename T>
+__global__ void absmax_kernel_t(int m, int n, const T* __restrict__ A, long lda, unsigned long long* out) {
+    double mx = 0.0;
+    size_t total = (size_t)m * n;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        size_t i = e % m, j = e / m;
+        mx = fmax(mx, fabs((double)A[i + j * lda]));
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(mx));
+}
+
+// bordered matrix of solve_free, src/swimmer.hpp:107-125: copy [Gs | Gc] into the first
+// ns rows, zero the last 6 rows, then the scaled force/torque rows
+template <typename T>
+__global__ void border_copy_kernel(int ns, const T* __restrict__ G, T* M) {
+    const int n = ns + 6;
+    size_t total = (size_t)n * n;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        size_t i = e % n, j = e / n;
+        M[e] = i < (size_t)ns ? G[i + j * ns] : T(0);
+    }
+}
+template <typename T>
+__global__ void border_rows_kernel(int Ns, T* M, const double* __restrict__ cen, const double* __restrict__ area,
+                                   double sc, double xcx, double xcy, double xcz) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= Ns) return;
+    const size_t ns = 3 * (size_t)Ns, n = ns + 6;
+    const double rx = cen[3 * (size_t)p] - xcx, ry = cen[3 * (size_t)p + 1] - xcy, rz = cen[3 * (size_t)p + 2] - xcz;
+    const double A = area[p] * sc;
+    auto at = [&](size_t r, size_t c) -> T& { return M[r + c * n]; };
+    for (int a = 0; a < 3; ++a) at(ns + a, 3 * p + a) = (T)A;
+    at(ns + 3, 3 * p + 1) = (T)(-A * rz); at(ns + 3, 3 * p + 2) = (T)(A * ry);
+    at(ns + 4, 3 * p + 0) = (T)(A * rz);  at(ns + 4, 3 * p + 2) = (T)(-A * rx);
+    at(ns + 5, 3 * p + 0) = (T)(-A * ry); at(ns + 5, 3 * p + 1) = (T)(A * rx);
+}
+
+// res_c(a, k) = -sc sum_p F rows . f_s (constraint residual of the bordered system)
+__global__ void constraint_apply_kernel(int Ns, int nrhs, const double* __restrict__ fs, long ldf,
+                                        const double* __restrict__ cen, const double* __restrict__ area, double sc,
+                                        double xcx, double xcy, double xcz, double* out /* 6 x nrhs */) {
+    // one workgroup per right-hand side, 256 threads
+    const int k = blockIdx.x;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int p = threadIdx.x; p < Ns; p += blockDim.x) {
+        const double rx = cen[3 * (size_t)p] - xcx, ry = cen[3 * (size_t)p + 1] - xcy, rz = cen[3 * (size_t)p + 2] - xcz;
+        const double A = area[p] * sc;
+        const double f0 = fs[3 * (size_t)p + k * ldf], f1 = fs[3 * (size_t)p + 1 + k * ldf], f2 = fs[3 * (size_t)p + 2 + k * ldf];
+        acc[0] += A * f0; acc[1] += A * f1; acc[2] += A * f2;
+        acc[3] += -A * rz * f1 + A * ry * f2;
+        acc[4] += A * rz * f0 - A * rx * f2;
+        acc[5] += -A * ry * f0 + A * rx * f1;
+    }
+    __shared__ double red[6][256];
+    for (int c = 0; c < 6; ++c) red[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int c = 0; c < 6; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[threadIdx.x + 6 * k] = red[threadIdx.x][0];
+}
+
+// y = alpha A x + beta y for a few right-hand sides (nrhs <= 8), split over column chunks
+constexpr int GV_CHUNKS = 32;
+template <typename T>
+__global__ void gemv_part_kernel(int m, int n, int nrhs, const T* __restrict__ A, long lda, const T* __restrict__ X,
+                                 long ldx, double* __restrict__ part) {
+    int row = blockIdx.x * blockDim.x + threadIdx.x;
+    int chunk = blockIdx.y;
+    if (row >= m) return;
+    int per = (n + GV_CHUNKS - 1) / GV_CHUNKS;
+    int ja = chunk * per, jb = min(n, ja + per);
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int j = ja; j < jb; ++j) {
+        const double a = (double)A[row + (size_t)j * lda];
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+            if (r < nrhs) acc[r] += a * (double)X[j + (size_t)r * ldx];
+    }
+    for (int r = 0; r < nrhs; ++r) part[((size_t)chunk * nrhs + r) * m + row] = acc[r];
+}
+template <typename T>
+__global__ void gemv_reduce_kernel(int m, int nrhs, const double* __restrict__ part, double alpha, double beta, T* Y,
+                                   long ldy) {
+    int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= m) return;
+    for (int r = 0; r < nrhs; ++r) {
+        double s = 0;
+        for (int k = 0; k < GV_CHUNKS; ++k) s += part[((size_t)k * nrhs + r) * m + row];
+        T* y = Y + row + (size_t)r * ldy;
+        *y = (T)(beta == 0.0 ? alpha * s : alpha * s + beta * (double)*y);
+    }
+}
+
+static DBuf<double> g_gv;
+template <typename T>
+static void gemv(int m, int n, int nrhs, double alpha, const T* A, long lda, const T* X, long ldx, double beta, T* Y,
+                 long ldy) {
+    if (m <= 0 || nrhs <= 0) return;
+    SQ_REQUIRE(nrhs <= 8, SQBEM_ERR_ARG, "gemv: at most 8 right-hand sides");
+    size_t need = (size_t)GV_CHUNKS * nrhs * m;
+    if (g_gv.n < need) g_gv.alloc(std::max<size_t>(need, 1 << 20));
+    if (n <= 0) {
+        scale_matrix<T>(m, nrhs, (T)beta, Y, (int)ldy);
+        return;
+    }
+    hipLaunchKernelGGL(gemv_part_kernel<T>, dim3(ceil_div(m, 256), GV_CHUNKS), dim3(256), 0, ctx().s(), m, n, nrhs, A,
+                       lda, X, ldx, g_gv.p);
+    hipLaunchKernelGGL(gemv_reduce_kernel<T>, dim3(ceil_div(m, 256)), dim3(256), 0, ctx().s(), m, nrhs, g_gv.p, alpha,
+                       beta, Y, ldy);
+    check_launch("gemv");
+}
+
+// Z = X + alpha Y elementwise over n values
+__global__ void axpy_kernel(size_t n, double alpha, const double* __restrict__ y, double* x) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        x[e] += alpha * y[e];
+}
+static void axpy(size_t n, double alpha, const double* y, double* x) {
+    if (!n) return;
+    unsigned blocks = (unsigned)std::min<size_t>(ceil_div(n, 256), 4096);
+    hipLaunchKernelGGL(axpy_kernel, dim3(blocks), dim3(256), 0, ctx().s(), n, alpha, y, x);
+    check_launch("axpy");
+}
+
+template <typename T>
+static double absmax(int m, int n, const T* A, long lda) {
+    DBuf<unsigned long long> r(1);
+    r.zero();
+    size_t total = (size_t)m * n;
+    unsigned blocks = (unsigned)std::min<size_t>(std::max<size_t>(ceil_div(total, 256), 1), 4096);
+    hipLaunchKernelGGL(absmax_kernel_t<T>, dim3(blocks), dim3(256), 0, ctx().s(), m, n, A, lda, r.p);
+    check_launch("absmax");
+    unsigned long long bits = 0;
+    r.download(&bits, 1);
+    double v;
+    std::memcpy(&v, &bits, sizeof v);
+    return v;
+}
+
+static void device_iota(DBuf<int>& b, int n) {
+    b.alloc(n);
+    hipLaunchKernelGGL(iota_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx().s(), n, b.p);
+    check_launch("iota");
+}
+
+// Rigid-body projection C = A K (columns of C at ldc).
+static void kproj(int nrows, int Ns, const double* A, long lda, const double* cen_xyz, const double* xc, double* C,
+                  long ldc) {
+    if (nrows <= 0) return;
+    DBuf<double> part((size_t)KP_CHUNKS * 6 * nrows);
+    hipLaunchKernelGGL(kproj_kernel, dim3(ceil_div(nrows, 256), KP_CHUNKS), dim3(256), 0, ctx().s(), nrows, Ns, A, lda,
+                       cen_xyz, xc[0], xc[1], xc[2], part.p);
+    hipLaunchKernelGGL(kproj_reduce, dim3(ceil_div(nrows, 256)), dim3(256), 0, ctx().s(), nrows, part.p, C, ldc);
+    check_launch("kproj");
+    sync();  // part is released on return
+}
+
+}  // namespace sq
+
+using namespace sq;
+
+// ------------------------------------------------------------------ tube
+struct sqbem_tube {
+    bem::Mesh mesh;
+    DMesh dm;
+    double mu = 1;
+    int prec = SQBEM_FP64;
+    int Nt = 0, nt = 0;
+    DBuf<double> T64;  // FP64 mode: LU factors; mixed: the FP64 operator
+    DBuf<float> T32;   // mixed: FP32 LU factors
+    DBuf<int> ipiv;
+    sqbem_lu view;  // non-owning handle onto the factors (Tube::lu)
+    double t_assembly = 0, t_factor = 0;
+};
+
+static void tube_build(sqbem_tube* t) {
+    double t0 = now_s();
+    t->Nt = t->mesh.P();
+    t->nt = 3 * t->Nt;
+    const int Nt = t->Nt, nt = t->nt;
+    t->dm.upload(t->mesh);
+    DBuf<int> self;
+    device_iota(self, Nt);
+    t->T64.alloc((size_t)nt * nt);
+    // Wielandt deflation of the f = c m null mode, fused into the assembly
+    // (Tube::deflate, src/swimmer.hpp:64-74): += sc n_i[a] n_p[b] A_p, sc = 1/(8 pi mu sqrt(At))
+    double At = 0;
+    for (int p = 0; p < Nt; ++p) At += t->mesh.area[p];
+    const double sc = 1.0 / (bem::EIGHT_PI * t->mu * std::sqrt(At));
+    assemble(t->dm, 0, Nt, t->dm.cen_xyz.p, Nt, self.p, bem::EIGHT_PI * t->mu, t->T64.p, nt, 1.0, nullptr, 0, nullptr,
+             sc, t->dm.nrm_xyz.p);
+    sync();
+    double t1 = now_s();
+    t->ipiv.alloc(nt);
+    int info;
+    if (mixed(t->prec)) {
+        t->T32.alloc((size_t)nt * nt);
+        convert_matrix<double, float>(nt, nt, t->T64.p, nt, t->T32.p, nt);
+        info = getrf<float>(t->prec, nt, t->T32.p, nt, t->ipiv.p);
+    } else {
+        info = getrf<double>(SQBEM_FP64, nt, t->T64.p, nt, t->ipiv.p);
+    }
+    SQ_REQUIRE(info == 0, SQBEM_ERR_SINGULAR, "dgetrf failed");
+    t->view.n = nt;
+    t->view.kind = t->prec;
+    t->view.LU = mixed(t->prec) ? nullptr : t->T64.p;
+    t->view.LU32 = mixed(t->prec) ? t->T32.p : nullptr;
+    t->view.piv = t->ipiv.p;
+    t->t_assembly = t1 - t0;
+    t->t_factor = now_s() - t1;
+}
+
+// ------------------------------------------------------------------ swimmer
+struct sqbem_swimmer {
+    const sqbem_tube* tube = nullptr;
+    bem::Mesh S;
+    DMesh dm;
+    double xc[3] = {0, 0, 0};
+    DBuf<double> xc_d;
+    double mu = 1;
+    int prec = SQBEM_FP64;
+    int Ns = 0, Nt = 0, ns = 0, nt = 0;
+    // FP64 operator pieces (src/swimmer.hpp:290-292 naming)
+    DBuf<double> AuBt, AuBs, D, Y, G;  // Y = T^-1 [B | Ct] (FP64 mode); G = [Gs | Gc]
+    DBuf<double> B, Ct, E, Cs;         // mixed: kept in FP64 for the refinement residual
+    DBuf<float> Y32, G32, D32, M32;    // mixed: low-precision Schur pieces and bordered factors
+    DBuf<int> Mpiv;
+    DBuf<float> Gs32;                  // mixed, towed problem: factors of Gs alone
+    DBuf<int> Gspiv;
+    double sc32 = 0;
+    double t_assembly = 0, t_schur = 0;
+    int ir_iters = 0;
+    double ir_residual = 0;
+};
+
+static void swimmer_build(sqbem_swimmer* s) {
+    double t0 = now_s();
+    const sqbem_tube* tb = s->tube;
+    s->Ns = s->S.P();
+    s->Nt = tb ? tb->Nt : 0;
+    s->ns = 3 * s->Ns;
+    s->nt = 3 * s->Nt;
+    const int Ns = s->Ns, Nt = s->Nt, ns = s->ns, nt = s->nt;
+    const double mu = s->mu, e8 = bem::EIGHT_PI;
+    const bool mx = mixed(s->prec);
+    s->dm.upload(s->S);
+    s->xc_d.alloc(3);
+    s->xc_d.upload(s->xc, 3);
+    DBuf<int> self;
+    device_iota(self, Ns);
+    DBuf<double> dlsum((size_t)9 * Ns);
+    dlsum.zero();
+    // tube rows <- sphere sources: B = SL/(8 pi mu), AuBt = -DL/(8 pi)   (src/swimmer.hpp:188-198)
+    if (tb) {
+        s->AuBt.alloc((size_t)nt * ns);
+        double* Bdst;
+        if (mx) {
+            s->B.alloc((size_t)nt * ns);
+            Bdst = s->B.p;
+        } else {
+            s->Y.alloc((size_t)nt * (ns + 6));
+            Bdst = s->Y.p;
+        }
+        assemble(s->dm, 0, Ns, tb->dm.cen_xyz.p, Nt, nullptr, e8 * mu, Bdst, nt, e8, s->AuBt.p, nt, nullptr, 0.0,
+                 nullptr);
+        // sphere rows <- tube sources: D = SL/(8 pi mu), DL summed (src/swimmer.hpp:201-207)
+        s->D.alloc((size_t)ns * nt);
+        assemble(tb->dm, 0, Nt, s->dm.cen_xyz.p, Ns, nullptr, e8 * mu, s->D.p, ns, 1.0, nullptr, 0, dlsum.p, 0.0,
+                 nullptr);
+    }
+    // sphere rows <- sphere sources: E (deflated, src/swimmer.hpp:222), AuBs, DL sums
+    double Asum = 0;
+    for (int p = 0; p < Ns; ++p) Asum += s->S.area[p];
+    const double defl = 1.0 / (e8 * mu * std::sqrt(Asum));
+    s->AuBs.alloc((size_t)ns * ns);
+    double* Edst;
+    if (mx) {
+        s->E.alloc((size_t)ns * ns);
+        Edst = s->E.p;
+    } else {
+        s->G.alloc((size_t)ns * (ns + 6));
+        Edst = s->G.p;
+    }
+    assemble(s->dm, 0, Ns, s->dm.cen_xyz.p, Ns, self.p, e8 * mu, Edst, ns, e8, s->AuBs.p, ns, dlsum.p, defl,
+             s->dm.nrm_xyz.p);
+    // completed double layer (src/swimmer.hpp:218-220)
+    hipLaunchKernelGGL(completion_diag_kernel, dim3(ceil_div(Ns, 256)), dim3(256), 0, ctx().s(), Ns, s->AuBs.p,
+                       (long)ns, dlsum.p, e8, tb ? 0.0 : 1.0);
+    check_launch("completion_diag");
+    // rigid-body columns Ct = AuBt K, Cs = AuBs K (src/swimmer.hpp:223-234)
+    if (mx) {
+        if (tb) {
+            s->Ct.alloc((size_t)nt * 6);
+            kproj(nt, Ns, s->AuBt.p, nt, s->dm.cen_xyz.p, s->xc, s->Ct.p, nt);
+        }
+        s->Cs.alloc((size_t)ns * 6);
+        kproj(ns, Ns, s->AuBs.p, ns, s->dm.cen_xyz.p, s->xc, s->Cs.p, ns);
+    } else {
+        if (tb) kproj(nt, Ns, s->AuBt.p, nt, s->dm.cen_xyz.p, s->xc, s->Y.p + (size_t)nt * ns, nt);
+        kproj(ns, Ns, s->AuBs.p, ns, s->dm.cen_xyz.p, s->xc, s->G.p + (size_t)ns * ns, ns);
+    }
+    sync();
+    double t1 = now_s();
+    s->t_assembly = t1 - t0;
+    // Schur complement (src/swimmer.hpp:235-249): Y = T^-1 [B | Ct]; [Gs | Gc] -= D Y
+    if (!mx) {
+        if (tb) {
+            getrs<double>(SQBEM_FP64, 0, nt, ns + 6, tb->T64.p, nt, tb->ipiv.p, s->Y.p, nt);
+            dgemm(ns, ns + 6, nt, -1.0, s->D.p, ns, s->Y.p, nt, 1.0, s->G.p, ns);
+        }
+    } else {
+        const int kind = s->prec;
+        s->G32.alloc((size_t)ns * (ns + 6));
+        convert_matrix<double, float>(ns, ns, s->E.p, ns, s->G32.p, ns);
+        convert_matrix<double, float>(ns, 6, s->Cs.p, ns, s->G32.p + (size_t)ns * ns, ns);
+        if (tb) {
+            s->Y32.alloc((size_t)nt * (ns + 6));
+            convert_matrix<double, float>(nt, ns, s->B.p, nt, s->Y32.p, nt);
+            convert_matrix<double, float>(nt, 6, s->Ct.p, nt, s->Y32.p + (size_t)nt * ns, nt);
+            getrs<float>(kind, 0, nt, ns + 6, tb->T32.p, nt, tb->ipiv.p, s->Y32.p, nt);
+            s->D32.alloc((size_t)ns * nt);
+            convert_matrix<double, float>(ns, nt, s->D.p, ns, s->D32.p, ns);
+            if (kind == SQBEM_FP32_IR)
+                sgemm(ns, ns + 6, nt, -1.0f, s->D32.p, ns, s->Y32.p, nt, 1.0f, s->G32.p, ns);
+            else
+                gemm16(kind, ns, ns + 6, nt, -1.0f, s->D32.p, ns, s->Y32.p, nt, 1.0f, s->G32.p, ns);
+        }
+        // bordered low-precision factors, reused by every refinement step
+        const int n = ns + 6;
+        double amax = absmax<float>(ns, ns, s->G32.p, ns);
+        double Amax = 0;
+        for (int p = 0; p < Ns; ++p) Amax = std::max(Amax, 5.0 * s->S.area[p]);
+        s->sc32 = amax / Amax;
+        s->M32.alloc((size_t)n * n);
+        hipLaunchKernelGGL(border_copy_kernel<float>, dim3(std::min<unsigned>(ceil_div((size_t)n * n, 256), 65535)),
+                           dim3(256), 0, ctx().s(), ns, s->G32.p, s->M32.p);
+        hipLaunchKernelGGL(border_rows_kernel<float>, dim3(ceil_div(Ns, 256)), dim3(256), 0, ctx().s(), Ns, s->M32.p,
+                           s->dm.cen_xyz.p, s->dm.area.p, s->sc32, s->xc[0], s->xc[1], s->xc[2]);
+        check_launch("border");
+        s->Mpiv.alloc(n);
+        int info = getrf<float>(kind, n, s->M32.p, n, s->Mpiv.p);
+        SQ_REQUIRE(info == 0, SQBEM_ERR_SINGULAR, "dgesv failed");
+    }
+    sync();
+    s->t_schur = now_s() - t1;
+}
+
+// Reduced right-hand sides for nslip slip fields u (device, ns x k):
+//   rs = -AuBs u - D T^-1 (-AuBt u),   rt = T^-1 (-AuBt u)      (src/swimmer.hpp:256-269)
+static void reduced_rhs(sqbem_swimmer* s, int k, const double* u, double* rs, long ldrs, double* rt) {
+    const int ns = s->ns, nt = s->nt;
+    gemv<double>(ns, ns, k, -1.0, s->AuBs.p, ns, u, ns, 0.0, rs, ldrs);
+    if (s->tube) {
+        gemv<double>(nt, ns, k, -1.0, s->AuBt.p, nt, u, ns, 0.0, rt, nt);
+        getrs<double>(SQBEM_FP64, 0, nt, k, s->tube->T64.p, nt, s->tube->ipiv.p, rt, nt);
+        gemv<double>(ns, nt, k, -1.0, s->D.p, ns, rt, nt, 1.0, rs, ldrs);
+    }
+}
+
+// FP64 path of solve_free / solve_prescribed (src/swimmer.hpp:107-153)
+static void solve_fp64(sqbem_swimmer* s, int k, const double* u_host, bool towed, const double* w_host, double* X_host,
+                       double* ft_host) {
+    const int Ns = s->Ns, ns = s->ns, nt = s->nt, n = ns + 6;
+    DBuf<double> X((size_t)n * k), rt((size_t)std::max(nt, 1) * k);
+    if (!towed) {
+        DBuf<double> u((size_t)ns * k);
+        u.upload(u_host, (size_t)ns * k);
+        X.zero();
+        reduced_rhs(s, k, u.p, X.p, n, rt.p);
+        // bordered matrix with the constraint rows scaled by max|Gs| / max(5 A)
+        double amax = absmax<double>(ns, ns, s->G.p, ns);
+        double Amax = 0;
+        for (int p = 0; p < Ns; ++p) Amax = std::max(Amax, 5.0 * s->S.area[p]);
+        const double sc = amax / Amax;
+        DBuf<double> M((size_t)n * n);
+        hipLaunchKernelGGL(border_copy_kernel<double>, dim3(std::min<unsigned>(ceil_div((size_t)n * n, 256), 65535)),
+                           dim3(256), 0, ctx().s(), ns, s->G.p, M.p);
+        hipLaunchKernelGGL(border_rows_kernel<double>, dim3(ceil_div(Ns, 256)), dim3(256), 0, ctx().s(), Ns, M.p,
+                           s->dm.cen_xyz.p, s->dm.area.p, sc, s->xc[0], s->xc[1], s->xc[2]);
+        check_launch("border");
+        DBuf<int> piv(n);
+        int info = getrf<double>(SQBEM_FP64, n, M.p, n, piv.p);
+        SQ_REQUIRE(info == 0, SQBEM_ERR_SINGULAR, "dgesv failed");
+        getrs<double>(SQBEM_FP64, 0, n, k, M.p, n, piv.p, X.p, n);
+    } else {
+        // towed sphere: Gs f = -Gc w, solved on a copy of Gs (src/swimmer.hpp:142-153)
+        DBuf<double> w(6), Gs((size_t)ns * ns);
+        w.upload(w_host, 6);
+        convert_matrix<double, double>(ns, ns, s->G.p, ns, Gs.p, ns);
+        gemv<double>(ns, 6, 1, -1.0, s->G.p + (size_t)ns * ns, ns, w.p, 6, 0.0, X.p, n);
+        SQ_HIP(hipMemcpyAsync(X.p + ns, w.p, 6 * sizeof(double), hipMemcpyDeviceToDevice, ctx().s()));
+        DBuf<int> piv(ns);
+        int info = getrf<double>(SQBEM_FP64, ns, Gs.p, ns, piv.p);
+        SQ_REQUIRE(info == 0, SQBEM_ERR_SINGULAR, "dgesv failed");
+        getrs<double>(SQBEM_FP64, 0, ns, 1, Gs.p, ns, piv.p, X.p, n);
+        if (nt) rt.zero();
+    }
+    // f_t = T^-1 r_t - Y x (unpack, src/swimmer.hpp:274-288)
+    if (s->tube) gemv<double>(nt, n, k, -1.0, s->Y.p, nt, X.p, n, 1.0, rt.p, nt);
+    X.download(X_host, (size_t)n * k);
+    if (s->tube && ft_host) rt.download(ft_host, (size_t)nt * k);
+}
+
+// ---- mixed precision: iterative refinement on the full bordered system
+//   [ T  B  Ct ] [f_t]   [r_t]
+//   [ D  E  Cs ] [f_s] = [r_s]         r_t = -AuBt u, r_s = -AuBs u
+//   [ 0 scF 0  ] [ w ]   [ 0 ]
+// Residuals in FP64 from the FP64 operator; corrections from the FP32 factors of T and
+// of the bordered Schur matrix (or of Gs alone for the towed problem, whose w is given and
+// whose unknowns are f_t, f_s only). The Schur elimination is exact algebra, so the
+// refined solution is the reference's solution to the refinement tolerance.
+//
+// x, b, r: device, N = nt + ns + 6 rows per right-hand side, blocks [t; s; c].
+static void full_residual(sqbem_swimmer* s, int k, const double* x, const double* b, double* r, bool towed) {
+    const int ns = s->ns, nt = s->nt, N = nt + ns + 6;
+    convert_matrix<double, double>(N, k, b, N, r, N);
+    DBuf<double> c(6);
+    for (int j = 0; j < k; ++j) {
+        const double* xt = x + (size_t)j * N;
+        const double* xs = xt + nt;
+        const double* xw = xs + ns;
+        double* rt = r + (size_t)j * N;
+        double* rs = rt + nt;
+        double* rc = rs + ns;
+        if (s->tube) {
+            gemv<double>(nt, nt, 1, -1.0, s->tube->T64.p, nt, xt, N, 1.0, rt, N);
+            gemv<double>(nt, ns, 1, -1.0, s->B.p, nt, xs, N, 1.0, rt, N);
+            if (!towed) gemv<double>(nt, 6, 1, -1.0, s->Ct.p, nt, xw, N, 1.0, rt, N);
+            gemv<double>(ns, nt, 1, -1.0, s->D.p, ns, xt, N, 1.0, rs, N);
+        }
+        gemv<double>(ns, ns, 1, -1.0, s->E.p, ns, xs, N, 1.0, rs, N);
+        if (!towed) {
+            gemv<double>(ns, 6, 1, -1.0, s->Cs.p, ns, xw, N, 1.0, rs, N);
+            hipLaunchKernelGGL(constraint_apply_kernel, dim3(1), dim3(256), 0, ctx().s(), s->Ns, 1, xs, (long)N,
+                               s->dm.cen_xyz.p, s->dm.area.p, s->sc32, s->xc[0], s->xc[1], s->xc[2], c.p);
+            check_launch("constraint_apply");
+            axpy(6, -1.0, c.p, rc);
+        }
+    }
+}
+
+// d = approx A^-1 r from the low-precision factors
+static void approx_solve(sqbem_swimmer* s, int k, const double* r, double* d, bool towed) {
+    const int ns = s->ns, nt = s->nt, N = nt + ns + 6, n = ns + 6;
+    const int kind = s->prec;
+    const int nv = towed ? ns : n;  // unknowns of the reduced system
+    DBuf<float> z32((size_t)std::max(nt, 1) * k), v32((size_t)n * k);
+    // z = T^-1 r_t ; v = [r_s - D z ; r_c]
+    if (nt) {
+        convert_matrix<double, float>(nt, k, r, N, z32.p, nt);
+        getrs<float>(kind, 0, nt, k, s->tube->T32.p, nt, s->tube->ipiv.p, z32.p, nt);
+    }
+    convert_matrix<double, float>(nv, k, r + nt, N, v32.p, n);
+    if (nt) gemv<float>(ns, nt, k, -1.0, s->D32.p, ns, z32.p, nt, 1.0, v32.p, n);
+    if (towed)
+        getrs<float>(kind, 0, ns, k, s->Gs32.p, ns, s->Gspiv.p, v32.p, n);
+    else
+        getrs<float>(kind, 0, n, k, s->M32.p, n, s->Mpiv.p, v32.p, n);
+    // f_t = z - Y [f_s; w]   (towed: w is not an unknown, only the f_s columns of Y)
+    if (nt) gemv<float>(nt, nv, k, -1.0, s->Y32.p, nt, v32.p, n, 1.0, z32.p, nt);
+    fill_zero(d, sizeof(double) * (size_t)N * k);
+    if (nt) convert_matrix<float, double>(nt, k, z32.p, nt, d, N);
+    convert_matrix<float, double>(nv, k, v32.p, n, d + nt, N);
+}
+
+static void col_norms(int N, int k, const double* a, std::vector<double>& out) {
+    std::vector<double> h((size_t)N * k);
+    SQ_HIP(hipMemcpyAsync(h.data(), a, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx().s()));
+    sync();
+    out.assign(k, 0.0);
+    for (int j = 0; j < k; ++j) {
+        double s2 = 0;
+        for (int i = 0; i < N; ++i) s2 += h[(size_t)j * N + i] * h[(size_t)j * N + i];
+        out[j] = std::sqrt(s2);
+    }
+}
+
+static void solve_mixed(sqbem_swimmer* s, int k, const double* u_host, bool towed, const double* w_host,
+                        double* X_host, double* ft_host) {
+    const int ns = s->ns, nt = s->nt, N = nt + ns + 6, n = ns + 6;
+    DBuf<double> b((size_t)N * k), x((size_t)N * k), r((size_t)N * k), d((size_t)N * k);
+    b.zero();
+    x.zero();
+    if (!towed) {
+        DBuf<double> u((size_t)ns * k);
+        u.upload(u_host, (size_t)ns * k);
+        for (int j = 0; j < k; ++j) {
+            if (nt) gemv<double>(nt, ns, 1, -1.0, s->AuBt.p, nt, u.p + (size_t)j * ns, ns, 0.0, b.p + (size_t)j * N, N);
+            gemv<double>(ns, ns, 1, -1.0, s->AuBs.p, ns, u.p + (size_t)j * ns, ns, 0.0, b.p + (size_t)j * N + nt, N);
+        }
+    } else {
+        SQ_REQUIRE(k == 1, SQBEM_ERR_ARG, "towed solve takes one right-hand side");
+        // Gs factor for the towed corrections (built on first use)
+        if (!s->Gs32.p) {
+            s->Gs32.alloc((size_t)ns * ns);
+            convert_matrix<float, float>(ns, ns, s->G32.p, ns, s->Gs32.p, ns);
+            s->Gspiv.alloc(ns);
+            int info = getrf<float>(s->prec, ns, s->Gs32.p, ns, s->Gspiv.p);
+            SQ_REQUIRE(info == 0, SQBEM_ERR_SINGULAR, "dgesv failed");
+        }
+        DBuf<double> w(6);
+        w.upload(w_host, 6);
+        if (nt) gemv<double>(nt, 6, 1, -1.0, s->Ct.p, nt, w.p, 6, 0.0, b.p, N);
+        gemv<double>(ns, 6, 1, -1.0, s->Cs.p, ns, w.p, 6, 0.0, b.p + nt, N);
+    }
+    std::vector<double> bn, rn;
+    col_norms(N, k, b.p, bn);
+    convert_matrix<double, double>(N, k, b.p, N, r.p, N);
+    const double tol = 1e-12;
+    int it = 0;
+    double worst = 1;
+    for (; it < 40; ++it) {
+        approx_solve(s, k, r.p, d.p, towed);
+        axpy((size_t)N * k, 1.0, d.p, x.p);
+        full_residual(s, k, x.p, b.p, r.p, towed);
+        col_norms(N, k, r.p, rn);
+        worst = 0;
+        for (int j = 0; j < k; ++j) worst = std::max(worst, bn[j] > 0 ? rn[j] / bn[j] : rn[j]);
+        if (worst <= tol) {
+            ++it;
+            break;
+        }
+    }
+    s->ir_iters = it;
+    s->ir_residual = worst;
+    SQ_REQUIRE(worst <= 1e-9, SQBEM_ERR_CONVERGE,
+               "iterative refinement stalled at relative residual " + std::to_string(worst));
+    std::vector<double> hx((size_t)N * k);
+    SQ_HIP(hipMemcpyAsync(hx.data(), x.p, hx.size() * sizeof(double), hipMemcpyDeviceToHost, ctx().s()));
+    sync();
+    for (int j = 0; j < k; ++j) {
+        std::memcpy(X_host + (size_t)j * n, hx.data() + (size_t)j * N + nt, sizeof(double) * n);
+        if (towed)
+            for (int c = 0; c < 6; ++c) X_host[(size_t)j * n + ns + c] = w_host[c];
+        if (ft_host && nt) std::memcpy(ft_host + (size_t)j * nt, hx.data() + (size_t)j * N, sizeof(double) * nt);
+    }
+}
+
+// ------------------------------------------------------------------ duct
+struct sqbem_duct {
+    bem::Mesh mesh;
+    DMesh dm;
+    double mu = 1;
+    int P = 0, n = 0;
+    DBuf<double> LU, Fk;
+    DBuf<int> ipiv;
+    std::vector<int> cap;
+};
+
+// column selection of src/duct_flow.hpp:97-99: column 3p+b from A_f when u is known
+__global__ void duct_select_kernel(int n, const int* __restrict__ tags, const double* __restrict__ Af,
+                                   const double* __restrict__ Au, double* M) {
+    size_t total = (size_t)n * n;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        size_t j = e / n;
+        int p = (int)(j / 3), b = (int)(j % 3);
+        bool known = !(tags[p] != bem::WALL && b == 2);
+        M[e] = known ? Af[e] : Au[e];
+    }
+}
+__global__ void duct_diag_kernel(int P, const int* __restrict__ tags, const double* __restrict__ dls, double* M,
+                                 double eight_pi) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const size_t n = 3 * (size_t)P;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            bool known = !(tags[i] != bem::WALL && b == 2);
+            if (!known) M[(3 * (size_t)i + a) + (3 * (size_t)i + b) * n] += dls[9 * (size_t)i + 3 * a + b] / eight_pi;
+        }
+}
+__global__ void gather_cols_kernel(int n, int ncols, const int* __restrict__ cols, const double* __restrict__ A,
+                                   double* out) {
+    size_t total = (size_t)n * ncols;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        size_t i = e % n, c = e / n;
+        out[e] = A[i + (size_t)cols[c] * n];
+    }
+}
+
+static void duct_build(sqbem_duct* d) {
+    d->P = d->mesh.P();
+    d->n = 3 * d->P;
+    const int P = d->P, n = d->n;
+    d->dm.upload(d->mesh);
+    DBuf<int> self;
+    device_iota(self, P);
+    DBuf<double> Af((size_t)n * n), Au((size_t)n * n), dls((size_t)9 * P);
+    dls.zero();
+    assemble(d->dm, 0, P, d->dm.cen_xyz.p, P, self.p, bem::EIGHT_PI * d->mu, Af.p, n, bem::EIGHT_PI, Au.p, n, dls.p, 0.0,
+             nullptr);
+    DBuf<int> tags(P);
+    tags.upload(d->mesh.tags.data(), P);
+    d->LU.alloc((size_t)n * n);
+    unsigned blocks = std::min<unsigned>(ceil_div((size_t)n * n, 256), 65535);
+    hipLaunchKernelGGL(duct_select_kernel, dim3(blocks), dim3(256), 0, ctx().s(), n, tags.p, Af.p, Au.p, d->LU.p);
+    hipLaunchKernelGGL(duct_diag_kernel, dim3(ceil_div(P, 256)), dim3(256), 0, ctx().s(), P, tags.p, dls.p, d->LU.p,
+                       bem::EIGHT_PI);
+    check_launch("duct");
+    d->cap.clear();
+    for (int p = 0; p < P; ++p)
+        if (d->mesh.tags[p] != bem::WALL) d->cap.push_back(p);
+    const int nc = (int)d->cap.size();
+    std::vector<int> cols(nc);
+    for (int k = 0; k < nc; ++k) cols[k] = 3 * d->cap[k] + 2;
+    DBuf<int> cols_d(std::max(nc, 1));
+    if (nc) cols_d.upload(cols.data(), nc);
+    d->Fk.alloc((size_t)n * std::max(nc, 1));
+    if (nc) {
+        hipLaunchKernelGGL(gather_cols_kernel, dim3(std::min<unsigned>(ceil_div((size_t)n * nc, 256), 65535)), dim3(256),
+                           0, ctx().s(), n, nc, cols_d.p, Af.p, d->Fk.p);
+        check_launch("gather_cols");
+    }
+    d->ipiv.alloc(n);
+    int info = getrf<double>(SQBEM_FP64, n, d->LU.p, n, d->ipiv.p);
+    SQ_REQUIRE(info == 0, SQBEM_ERR_SINGULAR, "dgetrf failed");
+    sync();
+}
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+sqbem_tube* sqbem_tube_create(const sqbem_mesh* mesh, double mu, int precision) {
+    return guarded_ptr<sqbem_tube>([&] {
+        SQ_REQUIRE(mesh && mesh->m.P() > 0, SQBEM_ERR_ARG, "sqbem_tube_create: empty mesh");
+        SQ_REQUIRE(precision >= SQBEM_FP64 && precision <= SQBEM_BF16_IR, SQBEM_ERR_ARG, "unknown precision");
+        auto* t = new sqbem_tube;
+        try {
+            t->mesh = mesh->m;
+            t->mu = mu;
+            t->prec = precision;
+            tube_build(t);
+        } catch (...) {
+            delete t;
+            throw;
+        }
+        return t;
+    });
+}
+
+int sqbem_tube_info(const sqbem_tube* t, int* Nt, double* ta, double* tf) {
+    return guarded([&] {
+        SQ_REQUIRE(t, SQBEM_ERR_ARG, "null tube");
+        if (Nt) *Nt = t->Nt;
+        if (ta) *ta = t->t_assembly;
+        if (tf) *tf = t->t_factor;
+    });
+}
+
+const sqbem_lu* sqbem_tube_lu(const sqbem_tube* t) { return t ? &t->view : nullptr; }
+
+void sqbem_tube_free(sqbem_tube* t) { delete t; }
+
+sqbem_swimmer* sqbem_swimmer_create(const sqbem_tube* tube, const sqbem_mesh* sphere, const double* center, double mu,
+                                    int precision) {
+    return guarded_ptr<sqbem_swimmer>([&] {
+        SQ_REQUIRE(sphere && sphere->m.P() > 0 && center, SQBEM_ERR_ARG, "sqbem_swimmer_create: bad arguments");
+        SQ_REQUIRE(precision >= SQBEM_FP64 && precision <= SQBEM_BF16_IR, SQBEM_ERR_ARG, "unknown precision");
+        SQ_REQUIRE(!tube || mixed(tube->prec) == mixed(precision), SQBEM_ERR_ARG,
+                   "swimmer precision must match the tube's (FP64 vs refined)");
+        SQ_REQUIRE(!tube || tube->prec == precision || !mixed(precision), SQBEM_ERR_ARG,
+                   "swimmer precision must match the tube's");
+        auto* s = new sqbem_swimmer;
+        try {
+            s->tube = tube;
+            s->S = sphere->m;
+            for (int c = 0; c < 3; ++c) s->xc[c] = center[c];
+            s->mu = mu;
+            s->prec = precision;
+            swimmer_build(s);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        return s;
+    });
+}
+
+static void unpack_out(sqbem_swimmer* s, int k, const std::vector<double>& X, const std::vector<double>& ft, double* U,
+                       double* Om, double* fs, double* ftube) {
+    const int ns = s->ns, n = ns + 6, nt = s->nt;
+    for (int j = 0; j < k; ++j) {
+        const double* x = X.data() + (size_t)j * n;
+        if (U)
+            for (int c = 0; c < 3; ++c) U[3 * j + c] = x[ns + c];
+        if (Om)
+            for (int c = 0; c < 3; ++c) Om[3 * j + c] = x[ns + 3 + c];
+        if (fs) std::memcpy(fs + (size_t)j * ns, x, sizeof(double) * ns);
+        if (ftube && nt) std::memcpy(ftube + (size_t)j * nt, ft.data() + (size_t)j * nt, sizeof(double) * nt);
+    }
+}
+
+int sqbem_swimmer_solve_free(sqbem_swimmer* s, int nslip, const double* slips, double* U, double* Om, double* fs,
+                             double* ft) {
+    return guarded([&] {
+        SQ_REQUIRE(s && slips && nslip > 0 && nslip <= 8, SQBEM_ERR_ARG, "solve_free: 1..8 slip fields required");
+        const int n = s->ns + 6;
+        std::vector<double> X((size_t)n * nslip), FT((size_t)std::max(s->nt, 1) * nslip);
+        if (mixed(s->prec))
+            solve_mixed(s, nslip, slips, false, nullptr, X.data(), FT.data());
+        else
+            solve_fp64(s, nslip, slips, false, nullptr, X.data(), FT.data());
+        unpack_out(s, nslip, X, FT, U, Om, fs, ft);
+    });
+}
+
+int sqbem_swimmer_solve_prescribed(sqbem_swimmer* s, const double* U, const double* Om, double* fs, double* ft) {
+    return guarded([&] {
+        SQ_REQUIRE(s && U && Om, SQBEM_ERR_ARG, "solve_prescribed: null argument");
+        const int n = s->ns + 6;
+        double w[6] = {U[0], U[1], U[2], Om[0], Om[1], Om[2]};
+        std::vector<double> X((size_t)n), FT((size_t)std::max(s->nt, 1));
+        if (mixed(s->prec))
+            solve_mixed(s, 1, nullptr, true, w, X.data(), FT.data());
+        else
+            solve_fp64(s, 1, nullptr, true, w, X.data(), FT.data());
+        unpack_out(s, 1, X, FT, nullptr, nullptr, fs, ft);
+    });
+}
+
+int sqbem_swimmer_times(const sqbem_swimmer* s, double* ta, double* ts, int* it, double* res) {
+    return guarded([&] {
+        SQ_REQUIRE(s, SQBEM_ERR_ARG, "null swimmer");
+        if (ta) *ta = s->t_assembly;
+        if (ts) *ts = s->t_schur;
+        if (it) *it = s->ir_iters;
+        if (res) *res = s->ir_residual;
+    });
+}
+
+void sqbem_swimmer_free(sqbem_swimmer* s) { delete s; }
+
+int sqbem_velocity_at(const sqbem_mesh* m, const double* u, const double* f, double mu, const double* pts, int npts,
+                      double* out) {
+    return guarded([&] {
+        SQ_REQUIRE(m && u && f && (npts == 0 || (pts && out)), SQBEM_ERR_ARG, "velocity_at: null argument");
+        if (npts == 0) return;
+        DMesh d(m->m);
+        const int P = m->m.P();
+        DBuf<double> du((size_t)3 * P), df((size_t)3 * P), dp((size_t)3 * npts), dout((size_t)3 * npts);
+        du.upload(u, (size_t)3 * P);
+        df.upload(f, (size_t)3 * P);
+        dp.upload(pts, (size_t)3 * npts);
+        field(d, dp.p, npts, du.p, df.p, mu, dout.p);
+        dout.download(out, (size_t)3 * npts);
+    });
+}
+
+sqbem_duct* sqbem_duct_create(const sqbem_mesh* mesh, double mu) {
+    return guarded_ptr<sqbem_duct>([&] {
+        SQ_REQUIRE(mesh && mesh->m.P() > 0, SQBEM_ERR_ARG, "sqbem_duct_create: empty mesh");
+        auto* d = new sqbem_duct;
+        try {
+            d->mesh = mesh->m;
+            d->mu = mu;
+            duct_build(d);
+        } catch (...) {
+            delete d;
+            throw;
+        }
+        return d;
+    });
+}
+
+int sqbem_duct_solve(const sqbem_duct* d, double p_in, double p_out, double* u, double* f, double* Q) {
+    return guarded([&] {
+        SQ_REQUIRE(d, SQBEM_ERR_ARG, "null duct");
+        const int P = d->P, n = d->n, nc = (int)d->cap.size();
+        std::vector<double> fk(nc);
+        for (int k = 0; k < nc; ++k) fk[k] = d->mesh.tags[d->cap[k]] == bem::INLET ? p_in : -p_out;
+        DBuf<double> fkd(std::max(nc, 1)), rhs(n);
+        if (nc) fkd.upload(fk.data(), nc);
+        gemv<double>(n, nc, 1, -1.0, d->Fk.p, n, fkd.p, std::max(nc, 1), 0.0, rhs.p, n);
+        getrs<double>(SQBEM_FP64, 0, n, 1, d->LU.p, n, d->ipiv.p, rhs.p, n);
+        std::vector<double> x(n);
+        rhs.download(x.data(), n);
+        std::vector<double> uu((size_t)3 * P, 0.0), ff((size_t)3 * P, 0.0);
+        for (int p = 0; p < P; ++p)
+            for (int b = 0; b < 3; ++b) {
+                bool known = !(d->mesh.tags[p] != bem::WALL && b == 2);
+                (known ? ff : uu)[3 * (size_t)p + b] = x[3 * (size_t)p + b];
+            }
+        double q = 0;
+        for (int k = 0; k < nc; ++k) {
+            int p = d->cap[k];
+            ff[3 * (size_t)p + 2] = fk[k];
+            if (d->mesh.tags[p] == bem::OUTLET) q += uu[3 * (size_t)p + 2] * d->mesh.area[p];
+        }
+        if (u) std::memcpy(u, uu.data(), sizeof(double) * 3 * P);
+        if (f) std::memcpy(f, ff.data(), sizeof(double) * 3 * P);
+        if (Q) *Q = q;
+    });
+}
+
+void sqbem_duct_free(sqbem_duct* d) { delete d; }
+
+}  // extern "C"

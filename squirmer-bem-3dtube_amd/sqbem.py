@@ -1,0 +1,520 @@
+"""Python binding of libsqbem (the MI355X squirmer-in-tube BEM library) over its C ABI.
+
+Mirrors the reference's solver interface (src/swimmer.hpp, src/duct_flow.hpp,
+src/utils/field.hpp, src/utils/mesh.hpp): same class and method names, argument meaning
+and error texts. Every numerical step runs in libsqbem on the GPU; importing this module
+fails loudly when the library is missing or the device is unavailable, there is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsqbem.so")
+
+FP64, FP32_IR, FP16_IR, BF16_IR = 0, 1, 2, 3
+PRECISION_NAMES = {FP64: "fp64", FP32_IR: "fp32+ir", FP16_IR: "fp16+ir", BF16_IR: "bf16+ir"}
+PI = 3.14159265358979323846
+EIGHT_PI = 8.0 * PI
+WALL, INLET, OUTLET, SPHERE = 0, 1, 2, 10
+
+_vp, _i, _d, _l = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
+_f = ctypes.c_float
+_SIGS = {
+    "sqbem_last_error": (ctypes.c_char_p, []),
+    "sqbem_init": (_i, [_i]),
+    "sqbem_set_stream": (_i, [_vp]),
+    "sqbem_sync": (_i, []),
+    "sqbem_device_info": (_i, [ctypes.c_char_p, _i, _vp, _vp]),
+    "sqbem_mesh_swimmer_tube": (_vp, [_d, _d, _d, _i, _i]),
+    "sqbem_mesh_tube": (_vp, [_d, _vp, _i, _i, _i]),
+    "sqbem_mesh_duct": (_vp, [_d, _d, _i, _i, _i]),
+    "sqbem_mesh_icosphere": (_vp, [_d, _i, _d, _d, _d]),
+    "sqbem_mesh_from_arrays": (_vp, [_vp, _i, _vp, _vp, _i]),
+    "sqbem_mesh_merge": (_vp, [_vp, _vp]),
+    "sqbem_mesh_translate": (_i, [_vp, _d, _d, _d]),
+    "sqbem_mesh_sizes": (_i, [_vp, _vp, _vp]),
+    "sqbem_mesh_get": (_i, [_vp] * 9),
+    "sqbem_mesh_free": (None, [_vp]),
+    "sqbem_dgemm": (_i, [_i, _i, _i, _d, _vp, _i, _vp, _i, _d, _vp, _i]),
+    "sqbem_sgemm": (_i, [_i, _i, _i, _f, _vp, _i, _vp, _i, _f, _vp, _i]),
+    "sqbem_gemm16": (_i, [_i, _i, _i, _i, _f, _vp, _i, _vp, _i, _f, _vp, _i]),
+    "sqbem_dgetrf": (_i, [_i, _vp, _i, _vp, _vp]),
+    "sqbem_dgetrs": (_i, [_i, _i, _vp, _i, _vp, _vp, _i]),
+    "sqbem_dgetrs_trans": (_i, [_i, _i, _vp, _i, _vp, _vp, _i]),
+    "sqbem_dgesv": (_i, [_i, _i, _vp, _i, _vp, _vp, _i, _vp]),
+    "sqbem_sgetrf": (_i, [_i, _i, _vp, _i, _vp, _vp]),
+    "sqbem_sgetrs": (_i, [_i, _i, _vp, _i, _vp, _vp, _i]),
+    "sqbem_host_dgemm": (_i, [_i, _i, _i, _d, _vp, _i, _vp, _i, _d, _vp, _i]),
+    "sqbem_host_dgesv": (_i, [_i, _i, _vp, _i, _vp, _i]),
+    "sqbem_lu_create": (_vp, [_i, _vp, _i, _vp]),
+    "sqbem_lu_solve": (_i, [_vp, _i, _vp, _i]),
+    "sqbem_lu_get": (_i, [_vp, _vp, _vp]),
+    "sqbem_lu_size": (_i, [_vp]),
+    "sqbem_lu_free": (None, [_vp]),
+    "sqbem_dmesh_upload": (_vp, [_vp]),
+    "sqbem_dmesh_free": (None, [_vp]),
+    "sqbem_assemble": (_i, [_vp, _i, _i, _vp, _i, _vp, _d, _vp, _l, _d, _vp, _l, _vp]),
+    "sqbem_panel_integrals": (_i, [_vp, _i, _vp, _i, _vp, _vp]),
+    "sqbem_tube_create": (_vp, [_vp, _d, _i]),
+    "sqbem_tube_info": (_i, [_vp, _vp, _vp, _vp]),
+    "sqbem_tube_lu": (_vp, [_vp]),
+    "sqbem_tube_free": (None, [_vp]),
+    "sqbem_swimmer_create": (_vp, [_vp, _vp, _vp, _d, _i]),
+    "sqbem_swimmer_solve_free": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
+    "sqbem_swimmer_solve_prescribed": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "sqbem_swimmer_times": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "sqbem_swimmer_free": (None, [_vp]),
+    "sqbem_velocity_at": (_i, [_vp, _vp, _vp, _d, _vp, _i, _vp]),
+    "sqbem_duct_create": (_vp, [_vp, _d]),
+    "sqbem_duct_solve": (_i, [_vp, _d, _d, _vp, _vp, _vp]),
+    "sqbem_duct_free": (None, [_vp]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libsqbem.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libsqbem.so not found at {path}: run __graft_entry__.build() first")
+    L = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(L, name, None)
+        if fn is None:
+            continue  # symbols absent from an older build are reported by the export test
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def lib():
+    return load()
+
+
+def last_error() -> str:
+    return lib().sqbem_last_error().decode()
+
+
+def check(rc: int):
+    if rc != 0:
+        raise RuntimeError(last_error())
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data_as(ctypes.c_void_p)
+    return ctypes.c_void_p(a.data_ptr())  # torch tensor on the device
+
+
+def _obj(h):
+    if not h:
+        raise RuntimeError(last_error())
+    return h
+
+
+def init(device: int = 0):
+    check(lib().sqbem_init(device))
+
+
+def sync():
+    check(lib().sqbem_sync())
+
+
+def device_info():
+    name = ctypes.create_string_buffer(256)
+    cu = ctypes.c_int()
+    gb = ctypes.c_double()
+    check(lib().sqbem_device_info(name, 256, ctypes.byref(cu), ctypes.byref(gb)))
+    return name.value.decode(), cu.value, gb.value
+
+
+# ------------------------------------------------------------------ meshes
+class Mesh:
+    """bem::Mesh (src/utils/mesh.hpp:50-91) owned by libsqbem; geometry read back as numpy."""
+
+    def __init__(self, handle):
+        self.h_ = _obj(handle)
+        self._load()
+
+    def _load(self):
+        nn, npn = ctypes.c_int(), ctypes.c_int()
+        check(lib().sqbem_mesh_sizes(self.h_, ctypes.byref(nn), ctypes.byref(npn)))
+        nn, npn = nn.value, npn.value
+        self.nodes = np.zeros((nn, 3))
+        self.tris = np.zeros((npn, 3), dtype=np.int32)
+        self.tags = np.zeros(npn, dtype=np.int32)
+        self.V = np.zeros((npn, 3, 3))
+        self.normal = np.zeros((npn, 3))
+        self.centroid = np.zeros((npn, 3))
+        self.area = np.zeros(npn)
+        self.h = np.zeros(npn)
+        check(lib().sqbem_mesh_get(self.h_, _ptr(self.nodes), _ptr(self.tris), _ptr(self.tags), _ptr(self.V),
+                                   _ptr(self.normal), _ptr(self.centroid), _ptr(self.area), _ptr(self.h)))
+
+    def __del__(self):
+        try:
+            if getattr(self, "h_", None) and _lib is not None:
+                _lib.sqbem_mesh_free(self.h_)
+        except Exception:
+            pass
+
+    def P(self):
+        return len(self.tris)
+
+    def total_area(self):
+        s = 0.0
+        for A in self.area:
+            s += A
+        return s
+
+    def translate(self, d):
+        check(lib().sqbem_mesh_translate(self.h_, float(d[0]), float(d[1]), float(d[2])))
+        self._load()
+
+    def copy(self):
+        return Mesh(lib().sqbem_mesh_from_arrays(_ptr(np.ascontiguousarray(self.nodes)), len(self.nodes),
+                                                 _ptr(np.ascontiguousarray(self.tris)),
+                                                 _ptr(np.ascontiguousarray(self.tags)), self.P()))
+
+
+def swimmer_tube_mesh(R, L, a, n_theta, n_r=4):
+    return Mesh(lib().sqbem_mesh_swimmer_tube(R, L, a, n_theta, n_r))
+
+
+def tube_mesh(R, z, n_theta, n_r):
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    return Mesh(lib().sqbem_mesh_tube(R, _ptr(z), len(z), n_theta, n_r))
+
+
+def duct_mesh(R, L, n_theta, n_z, n_r):
+    return Mesh(lib().sqbem_mesh_duct(R, L, n_theta, n_z, n_r))
+
+
+def icosphere(a, level, center=(0.0, 0.0, 0.0)):
+    return Mesh(lib().sqbem_mesh_icosphere(a, level, float(center[0]), float(center[1]), float(center[2])))
+
+
+def merge(a, b):
+    return Mesh(lib().sqbem_mesh_merge(a.h_, b.h_))
+
+
+# ------------------------------------------------------------------ panel integrals
+def panel_integrals(M, p, x0, self_=False):
+    """panel_integrals (src/utils/kernels.hpp:67-82) for one pair, on the device."""
+    SL, DL = np.zeros(9), np.zeros(9)
+    x = np.ascontiguousarray(x0, dtype=np.float64)
+    check(lib().sqbem_panel_integrals(M.h_, p, _ptr(x), int(self_), _ptr(SL), _ptr(DL)))
+    return SL.reshape(3, 3), DL.reshape(3, 3)
+
+
+class DeviceMesh:
+    def __init__(self, M):
+        self.h_ = _obj(lib().sqbem_dmesh_upload(M.h_))
+        self.np = M.P()
+
+    def __del__(self):
+        try:
+            if getattr(self, "h_", None) and _lib is not None:
+                _lib.sqbem_dmesh_free(self.h_)
+        except Exception:
+            pass
+
+
+def assemble(dmesh, x0, self_src=None, p0=0, np_=None, sl_div=1.0, SL=None, dl_div=1.0, DL=None, dlsum=None):
+    """Device block assembly (sqbem_assemble) on torch device tensors."""
+    np_ = dmesh.np - p0 if np_ is None else np_
+    nt = x0.numel() // 3
+    ld_sl = SL.stride(1) if SL is not None else 0
+    ld_dl = DL.stride(1) if DL is not None else 0
+    check(lib().sqbem_assemble(dmesh.h_, p0, np_, _ptr(x0), nt, _ptr(self_src), sl_div, _ptr(SL), ld_sl, dl_div,
+                               _ptr(DL), ld_dl, _ptr(dlsum)))
+
+
+# ------------------------------------------------------------------ dense algebra (device)
+def _ld(t):
+    # column-major view: torch tensor of shape (m, n) with stride (1, ld)
+    assert t.stride(0) == 1, "expected a column-major (Fortran-ordered) device tensor"
+    return t.stride(1) if t.dim() == 2 and t.shape[1] > 1 else max(1, t.shape[0])
+
+
+def dgemm(alpha, A, B, beta, C):
+    m, k = A.shape
+    n = B.shape[1]
+    check(lib().sqbem_dgemm(m, n, k, alpha, _ptr(A), _ld(A), _ptr(B), _ld(B), beta, _ptr(C), _ld(C)))
+
+
+def sgemm(alpha, A, B, beta, C):
+    m, k = A.shape
+    n = B.shape[1]
+    check(lib().sqbem_sgemm(m, n, k, alpha, _ptr(A), _ld(A), _ptr(B), _ld(B), beta, _ptr(C), _ld(C)))
+
+
+def gemm16(kind, alpha, A, B, beta, C):
+    m, k = A.shape
+    n = B.shape[1]
+    check(lib().sqbem_gemm16(kind, m, n, k, alpha, _ptr(A), _ld(A), _ptr(B), _ld(B), beta, _ptr(C), _ld(C)))
+
+
+def dgetrf(A, ipiv):
+    info = ctypes.c_int(0)
+    rc = lib().sqbem_dgetrf(A.shape[0], _ptr(A), _ld(A), _ptr(ipiv), ctypes.byref(info))
+    if rc != 0 and info.value == 0:
+        check(rc)
+    return info.value
+
+
+def dgetrs(LU, ipiv, B, trans=False):
+    fn = lib().sqbem_dgetrs_trans if trans else lib().sqbem_dgetrs
+    check(fn(LU.shape[0], B.shape[1], _ptr(LU), _ld(LU), _ptr(ipiv), _ptr(B), _ld(B)))
+
+
+def sgetrf(kind, A, ipiv):
+    info = ctypes.c_int(0)
+    rc = lib().sqbem_sgetrf(kind, A.shape[0], _ptr(A), _ld(A), _ptr(ipiv), ctypes.byref(info))
+    if rc != 0 and info.value == 0:
+        check(rc)
+    return info.value
+
+
+def sgetrs(LU, ipiv, B):
+    check(lib().sqbem_sgetrs(LU.shape[0], B.shape[1], _ptr(LU), _ld(LU), _ptr(ipiv), _ptr(B), _ld(B)))
+
+
+# host-matrix API (src/utils/linalg.hpp:42-77)
+def gemm(alpha, A, B, beta, C):
+    """C = alpha A B + beta C on numpy Fortran arrays (in place on C)."""
+    A = np.asfortranarray(A, dtype=np.float64)
+    B = np.asfortranarray(B, dtype=np.float64)
+    assert C.flags.f_contiguous and C.dtype == np.float64
+    if A.shape[1] != B.shape[0] or C.shape[0] != A.shape[0] or C.shape[1] != B.shape[1]:
+        raise RuntimeError("gemm: dimension mismatch")
+    m, k = A.shape
+    n = B.shape[1]
+    check(lib().sqbem_host_dgemm(m, n, k, alpha, _ptr(A), max(1, m), _ptr(B), max(1, k), beta, _ptr(C), max(1, m)))
+
+
+def solve_dense(A, B):
+    """B <- A^-1 B (A unchanged), in place on the Fortran array B."""
+    A = np.asfortranarray(A, dtype=np.float64)
+    assert B.flags.f_contiguous and B.dtype == np.float64
+    n = A.shape[0]
+    check(lib().sqbem_host_dgesv(n, B.shape[1], _ptr(A), max(1, n), _ptr(B), max(1, n)))
+
+
+class LUFactor:
+    """LUFactor (src/utils/linalg.hpp:62-78) with the factors resident in HBM."""
+
+    def __init__(self, handle=None, owner=None):
+        self.h_ = handle
+        self._owner = owner  # keeps a tube alive for a borrowed view
+        self._owned = handle is not None and owner is None
+
+    def factor(self, A):
+        A = np.asfortranarray(A, dtype=np.float64)
+        info = ctypes.c_int(0)
+        self.h_ = _obj(lib().sqbem_lu_create(A.shape[0], _ptr(A), max(1, A.shape[0]), ctypes.byref(info)))
+        self._owned = True
+
+    def solve(self, B):
+        """B <- A^-1 B in place (B: numpy Fortran array, n x nrhs)."""
+        assert B.flags.f_contiguous and B.dtype == np.float64
+        if B.shape[1] == 0:
+            return B
+        check(lib().sqbem_lu_solve(self.h_, B.shape[1], _ptr(B), B.shape[0]))
+        return B
+
+    def get(self):
+        n = lib().sqbem_lu_size(self.h_)
+        LU = np.zeros((n, n), order="F")
+        piv = np.zeros(n, dtype=np.int32)
+        check(lib().sqbem_lu_get(self.h_, _ptr(LU), _ptr(piv)))
+        return LU, piv
+
+    def __del__(self):
+        try:
+            if self._owned and self.h_ and _lib is not None:
+                _lib.sqbem_lu_free(self.h_)
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ solvers
+class Tube:
+    """Tube (src/swimmer.hpp:33-75): tube-tube block assembled, deflated and factorised once
+    on the device."""
+
+    def __init__(self, R, L, a, n_theta, n_r=4, mu=1.0, precision=FP64, mesh=None):
+        self.R, self.L, self.mu, self.precision = R, L, mu, precision
+        self.mesh = mesh if mesh is not None else swimmer_tube_mesh(R, L, a, n_theta, n_r)
+        self.h_ = _obj(lib().sqbem_tube_create(self.mesh.h_, mu, precision))
+        Nt, ta, tf = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        check(lib().sqbem_tube_info(self.h_, ctypes.byref(Nt), ctypes.byref(ta), ctypes.byref(tf)))
+        self.Nt, self.t_assembly, self.t_factor = Nt.value, ta.value, tf.value
+        self.lu = LUFactor(lib().sqbem_tube_lu(self.h_), owner=self)
+
+    def __del__(self):
+        try:
+            if getattr(self, "h_", None) and _lib is not None:
+                _lib.sqbem_tube_free(self.h_)
+        except Exception:
+            pass
+
+
+@dataclass
+class SwimmerSolution:
+    """src/swimmer.hpp:78-81."""
+    U: np.ndarray
+    Om: np.ndarray
+    f_sphere: np.ndarray
+    f_tube: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+
+
+def squirmer_slip(x, center, e, B1, B2):
+    """squirmer_slip (src/utils/kernels.hpp:107-111): (B1 + B2 cos th)(cos th rhat - e)."""
+    x = np.asarray(x, dtype=np.float64)
+    r = x - np.asarray(center, dtype=np.float64)
+    rh = (1.0 / np.sqrt(np.sum(r * r, axis=-1, keepdims=True))) * r
+    e = np.asarray(e, dtype=np.float64)
+    ct = np.sum(rh * e, axis=-1, keepdims=True)
+    return (B1 + B2 * ct) * (ct * rh - e)
+
+
+class SwimmerSolver:
+    """SwimmerSolver (src/swimmer.hpp:83-289) on the device: couplings, Schur complement
+    against the tube factors, force- and torque-free bordered solve."""
+
+    def __init__(self, tube, sphere, center, mu=1.0, precision=None):
+        self.tube = tube
+        self.S = sphere
+        self.xc = np.asarray(center, dtype=np.float64).copy()
+        self.mu = mu
+        self.precision = precision if precision is not None else (tube.precision if tube is not None else FP64)
+        self.Ns = sphere.P()
+        self.Nt = tube.Nt if tube is not None else 0
+        c = np.ascontiguousarray(self.xc)
+        self.h_ = _obj(lib().sqbem_swimmer_create(tube.h_ if tube is not None else None, sphere.h_, _ptr(c), mu,
+                                                  self.precision))
+        self._times()
+
+    def _times(self):
+        ta, ts, it, res = ctypes.c_double(), ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
+        check(lib().sqbem_swimmer_times(self.h_, ctypes.byref(ta), ctypes.byref(ts), ctypes.byref(it),
+                                        ctypes.byref(res)))
+        self.t_assembly, self.t_schur, self.ir_iters, self.ir_residual = ta.value, ts.value, it.value, res.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "h_", None) and _lib is not None:
+                _lib.sqbem_swimmer_free(self.h_)
+        except Exception:
+            pass
+
+    def slip(self, e, B1, B2):
+        return squirmer_slip(self.S.centroid, self.xc, e, B1, B2)
+
+    def solve_free(self, slips):
+        k = len(slips)
+        u = np.ascontiguousarray(np.stack([np.asarray(s, dtype=np.float64).reshape(-1) for s in slips]))
+        U = np.zeros((k, 3))
+        Om = np.zeros((k, 3))
+        fs = np.zeros((k, self.Ns, 3))
+        ft = np.zeros((k, max(self.Nt, 0), 3))
+        check(lib().sqbem_swimmer_solve_free(self.h_, k, _ptr(u), _ptr(U), _ptr(Om), _ptr(fs),
+                                             _ptr(ft) if self.Nt else None))
+        self._times()
+        return [SwimmerSolution(U[j].copy(), Om[j].copy(), fs[j].copy(), ft[j].copy()) for j in range(k)]
+
+    def solve_prescribed(self, U, Om):
+        Uv = np.ascontiguousarray(U, dtype=np.float64)
+        Ov = np.ascontiguousarray(Om, dtype=np.float64)
+        fs = np.zeros((self.Ns, 3))
+        ft = np.zeros((max(self.Nt, 0), 3))
+        check(lib().sqbem_swimmer_solve_prescribed(self.h_, _ptr(Uv), _ptr(Ov), _ptr(fs), _ptr(ft) if self.Nt else None))
+        self._times()
+        return SwimmerSolution(Uv.copy(), Ov.copy(), fs, ft)
+
+    def force(self, s):
+        """Force on the sphere, F = -sum A f (src/swimmer.hpp:156-160)."""
+        F = np.zeros(3)
+        for p in range(self.Ns):
+            F = F - self.S.area[p] * s.f_sphere[p]
+        return F
+
+    def velocity(self, s, slip_field, pts):
+        """src/swimmer.hpp:163-173."""
+        allm = merge(self.tube.mesh, self.S) if self.tube is not None else self.S
+        P = allm.P()
+        u = np.zeros((P, 3))
+        f = np.zeros((P, 3))
+        Nt = self.Nt
+        if Nt:
+            f[:Nt] = s.f_tube
+        r = self.S.centroid - self.xc
+        u[Nt:] = s.U + np.cross(s.Om, r) + np.asarray(slip_field)
+        f[Nt:] = s.f_sphere
+        return velocity_at(allm, u, f, self.mu, pts)
+
+
+def velocity_at(M, u, f, mu, pts):
+    """velocity_at (src/utils/field.hpp:18-34) on the device."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros_like(pts)
+    check(lib().sqbem_velocity_at(M.h_, _ptr(u), _ptr(f), mu, _ptr(pts), len(pts), _ptr(out)))
+    return out
+
+
+class DuctFlow:
+    """DuctFlow (src/duct_flow.hpp:22-109) on the device."""
+
+    @dataclass
+    class Solution:
+        u: np.ndarray
+        f: np.ndarray
+        Q: float = 0.0
+
+    def __init__(self, R, L, n_theta, n_z, n_r, mu=1.0):
+        self.R, self.L, self.mu = R, L, mu
+        self.mesh = duct_mesh(R, L, n_theta, n_z, n_r)
+        self.h_ = _obj(lib().sqbem_duct_create(self.mesh.h_, mu))
+
+    def __del__(self):
+        try:
+            if getattr(self, "h_", None) and _lib is not None:
+                _lib.sqbem_duct_free(self.h_)
+        except Exception:
+            pass
+
+    def u_known(self, p, b):
+        return not (self.mesh.tags[p] != WALL and b == 2)
+
+    def solve(self, p_in, p_out):
+        P = self.mesh.P()
+        u = np.zeros((P, 3))
+        f = np.zeros((P, 3))
+        Q = ctypes.c_double()
+        check(lib().sqbem_duct_solve(self.h_, p_in, p_out, _ptr(u), _ptr(f), ctypes.byref(Q)))
+        return DuctFlow.Solution(u, f, Q.value)
+
+    def velocity(self, s, pts):
+        return velocity_at(self.mesh, s.u, s.f, self.mu, pts)
+
+    def exact_uz(self, r, dP):
+        return dP / self.L * (self.R * self.R - r * r) / (4 * self.mu)
+
+    def exact_Q(self, dP):
+        return PI * self.R ** 4 * dP / self.L / (8 * self.mu)

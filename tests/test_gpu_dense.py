@@ -1,0 +1,179 @@
+"""Dense kernels of libsqbem on the GPU against float64 references (numpy/scipy LAPACK).
+
+GEMM: FP64 MFMA to rounding (k * eps), FP32 MFMA to FP32 accuracy, FP16/BF16 MFMA with
+FP32 accumulation to the input rounding of the 16-bit formats. LU: partial-pivoting
+factors and solves against scipy's LAPACK dgetrf/dgetrs (pivot rows compared exactly on
+random matrices, backward error bounded by c n eps).
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def dev(a, dtype=None):
+    a = np.asarray(a)
+    dt = {np.float64: torch.float64, np.float32: torch.float32}[(dtype or a.dtype).type]
+    t = torch.empty_strided(a.shape, (1, a.shape[0]) if a.ndim == 2 else (1,), dtype=dt, device="cuda")
+    t.copy_(torch.from_numpy(np.asarray(a, dtype=dtype or a.dtype)))
+    return t
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+SHAPES = [(1, 1, 1), (37, 53, 19), (128, 128, 16), (129, 130, 17), (300, 257, 129), (1000, 2, 700), (64, 960, 3)]
+
+
+@pytest.mark.parametrize("m,n,k", SHAPES)
+def test_dgemm(sq, m, n, k):
+    rng = np.random.default_rng(m * 1000 + n + k)
+    A, B, C = rng.standard_normal((m, k)), rng.standard_normal((k, n)), rng.standard_normal((m, n))
+    alpha, beta = -1.25, 0.5
+    dA, dB, dC = dev(A), dev(B), dev(C)
+    torch.cuda.synchronize()
+    sq.dgemm(alpha, dA, dB, beta, dC)
+    sq.sync()
+    ref = alpha * (A @ B) + beta * C
+    scale = np.abs(A) @ np.abs(B)
+    err = np.abs(host(dC) - ref) / (np.abs(alpha) * scale + np.abs(beta * C) + 1e-300)
+    assert err.max() < 4 * k * 2.2e-16 + 1e-15, err.max()
+
+
+def test_dgemm_submatrix_and_beta_zero(sq):
+    rng = np.random.default_rng(1)
+    big = rng.standard_normal((200, 200))
+    dbig = dev(big)
+    # C = A B on sub-blocks of one matrix with lda = 200; C starts as NaN (beta = 0 must not read it)
+    A = dbig[10:60, 0:30]
+    B = dbig[0:30, 100:140]
+    C = torch.full((50, 40), float("nan"), dtype=torch.float64, device="cuda").t().contiguous().t()
+    torch.cuda.synchronize()
+    sq.dgemm(1.0, A, B, 0.0, C)
+    sq.sync()
+    ref = big[10:60, 0:30] @ big[0:30, 100:140]
+    assert np.allclose(host(C), ref, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("m,n,k", SHAPES)
+def test_sgemm(sq, m, n, k):
+    rng = np.random.default_rng(7 + m + n + k)
+    A, B, C = (rng.standard_normal(s).astype(np.float32) for s in [(m, k), (k, n), (m, n)])
+    dA, dB, dC = dev(A), dev(B), dev(C)
+    torch.cuda.synchronize()
+    sq.sgemm(2.0, dA, dB, -1.0, dC)
+    sq.sync()
+    ref = 2.0 * (A.astype(np.float64) @ B.astype(np.float64)) - C.astype(np.float64)
+    scale = 2.0 * (np.abs(A).astype(np.float64) @ np.abs(B)) + np.abs(C)
+    err = np.abs(host(dC) - ref) / scale
+    assert err.max() < 4 * k * 6e-8 + 1e-6, err.max()
+
+
+@pytest.mark.parametrize("kind,tol", [(2, 2.5e-3), (3, 2e-2)])
+@pytest.mark.parametrize("m,n,k", [(128, 128, 64), (300, 257, 129), (1000, 7, 700)])
+def test_gemm16(sq, kind, tol, m, n, k):
+    """16-bit MFMA products, FP32 accumulation: error within the 16-bit input rounding."""
+    rng = np.random.default_rng(kind * 31 + m)
+    # rows/columns of very different magnitude exercise the FP16 power-of-two scaling
+    A = (rng.standard_normal((m, k)) * np.logspace(-6, 3, m)[:, None]).astype(np.float32)
+    B = (rng.standard_normal((k, n)) * np.logspace(2, -5, n)[None, :]).astype(np.float32)
+    C = np.zeros((m, n), dtype=np.float32)
+    dA, dB, dC = dev(A), dev(B), dev(C)
+    torch.cuda.synchronize()
+    sq.gemm16(kind, 1.0, dA, dB, 0.0, dC)
+    sq.sync()
+    ref = A.astype(np.float64) @ B.astype(np.float64)
+    scale = np.abs(A).astype(np.float64) @ np.abs(B).astype(np.float64)
+    err = np.abs(host(dC) - ref) / scale
+    assert err.max() < tol, err.max()
+
+
+@pytest.mark.parametrize("n", [1, 5, 31, 32, 33, 64, 65, 100, 257, 1000])
+def test_dgetrf_matches_lapack(sq, n):
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    dA = dev(A)
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    info = sq.dgetrf(dA, ipiv)
+    assert info == 0
+    lu, piv = sla.lu_factor(A)
+    got_piv = host(ipiv) - 1
+    assert np.array_equal(got_piv, piv), "pivot rows differ from LAPACK"
+    assert np.allclose(host(dA), lu, rtol=1e-10, atol=1e-11 * np.abs(lu).max())
+
+
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (40, 3), (257, 966), (1500, 2)])
+@pytest.mark.parametrize("trans", [False, True])
+def test_dgetrs_backward_error(sq, n, nrhs, trans):
+    rng = np.random.default_rng(n + nrhs)
+    A = rng.standard_normal((n, n)) + n ** 0.5 * np.eye(n)
+    X = rng.standard_normal((n, nrhs))
+    Bm = (A.T if trans else A) @ X
+    dA, dB = dev(A), dev(Bm)
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    assert sq.dgetrf(dA, ipiv) == 0
+    sq.dgetrs(dA, ipiv, dB, trans=trans)
+    sq.sync()
+    Xg = host(dB)
+    Aop = A.T if trans else A
+    r = np.linalg.norm(Aop @ Xg - Bm, axis=0) / (np.linalg.norm(Aop, 2) * np.linalg.norm(Xg, axis=0))
+    assert r.max() < 30 * n * 2.2e-16, r.max()
+
+
+def test_dgetrf_singular_reports_info(sq):
+    n = 70
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((n, n))
+    A[:, 40] = 0.0  # exact zero column -> zero pivot at column 41 (1-based)
+    dA = dev(A)
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    info = sq.dgetrf(dA, ipiv)
+    assert info == 41
+    assert "dgetrf failed" in sq.last_error()
+
+
+@pytest.mark.parametrize("kind", [1, 2, 3])
+def test_sgetrf_solves(sq, kind):
+    n = 700
+    rng = np.random.default_rng(kind)
+    A = rng.standard_normal((n, n)) + 3 * n ** 0.5 * np.eye(n)
+    X = rng.standard_normal((n, 2))
+    Bm = (A @ X).astype(np.float32)
+    dA, dB = dev(A.astype(np.float32)), dev(Bm)
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    assert sq.sgetrf(kind, dA, ipiv) == 0
+    sq.sgetrs(dA, ipiv, dB)
+    sq.sync()
+    err = np.linalg.norm(host(dB) - X) / np.linalg.norm(X)
+    tol = {1: 1e-5, 2: 2e-2, 3: 5e-2}[kind]
+    assert err < tol, err
+
+
+def test_host_linalg_api(sq):
+    """gemm / solve_dense / LUFactor on host matrices (src/utils/linalg.hpp:42-78)."""
+    rng = np.random.default_rng(11)
+    A = np.asfortranarray(rng.standard_normal((60, 60)) + 8 * np.eye(60))
+    B = np.asfortranarray(rng.standard_normal((60, 4)))
+    C = np.asfortranarray(rng.standard_normal((60, 4)))
+    C0 = C.copy()
+    sq.gemm(2.0, A, B, -1.0, C)
+    assert np.allclose(C, 2 * A @ B - C0, rtol=1e-13, atol=1e-12)
+    X = B.copy(order="F")
+    sq.solve_dense(A, X)
+    assert np.allclose(A @ X, B, rtol=1e-12, atol=1e-12)
+    lu = sq.LUFactor()
+    lu.factor(A)
+    Y = B.copy(order="F")
+    lu.solve(Y)
+    assert np.allclose(Y, X, rtol=1e-13, atol=1e-13)
+    with pytest.raises(RuntimeError, match="gemm: dimension mismatch"):
+        sq.gemm(1.0, A, B, 0.0, np.zeros((5, 5), order="F"))
