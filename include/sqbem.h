@@ -40,6 +40,20 @@ int sqbem_init(int device);
 int sqbem_set_stream(void* hip_stream); /* NULL: the library's own stream */
 int sqbem_sync(void);
 int sqbem_device_info(char* name, int name_len, int* cu_count, double* hbm_gb);
+/* Kernel timing (no reference counterpart; measurement only). sqbem_profile(1) resets
+ * and starts recording HIP events around every launch of the classes below on the
+ * library stream; sqbem_profile_get returns launches, algorithmic flops and bytes, and
+ * summed kernel time (ms) of one class. */
+#define SQBEM_PROF_DGEMM 0
+#define SQBEM_PROF_SGEMM 1
+#define SQBEM_PROF_GEMM16 2
+#define SQBEM_PROF_PANEL 3
+#define SQBEM_PROF_TRSM 4
+#define SQBEM_PROF_ASSEMBLE 5
+#define SQBEM_PROF_GEMV 6
+#define SQBEM_PROF_LASWP 7
+int sqbem_profile(int enable);
+int sqbem_profile_get(int kernel_class, long* launches, double* flops, double* bytes, double* ms);
 
 /* ------------------------------------------------------------------ meshes (host)
  * Generators with the reference's node/panel order, src/utils/mesh.hpp:45-247. */

@@ -383,6 +383,9 @@ void assemble(const DMesh& src, int p0, int np, const double* x0, int nt, const 
     const int ntiles = (int)ceil_div(np, TS);
     if (dlsum) a.dls_part = part_buffer((size_t)ntiles * 9 * nt);
     dim3 grid(ceil_div(nt, TT), ntiles);
+    // algorithmic bytes: the stored 3x3 blocks; flops: 72 per far-field point (7 per pair)
+    ProfScope prof(PROF_ASSEMBLE, 72.0 * 7.0 * nt * np,
+                   8.0 * 9.0 * nt * np * ((SL ? 1 : 0) + (DL ? 1 : 0)));
     hipLaunchKernelGGL(assemble_kernel<MODE_BLOCKS>, grid, dim3(TT), 0, ctx().s(), a);
     check_launch("assemble");
     if (dlsum) {

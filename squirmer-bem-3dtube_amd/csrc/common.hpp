@@ -93,6 +93,19 @@ struct DBuf {
 
 inline void sync() { SQ_HIP(hipStreamSynchronize(ctx().s())); }
 
+// Kernel timing for roofline reports: when enabled, a ProfScope records HIP events around
+// one launch on the library stream together with its algorithmic flops and bytes.
+enum ProfClass { PROF_DGEMM = 0, PROF_SGEMM, PROF_GEMM16, PROF_PANEL, PROF_TRSM, PROF_ASSEMBLE, PROF_GEMV, PROF_LASWP,
+                 PROF_NCLASS };
+bool prof_enabled();
+struct ProfScope {
+    int cls;
+    double flops, bytes;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ProfScope(int c, double f, double b);
+    ~ProfScope();
+};
+
 inline void check_launch(const char* what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw Error(SQBEM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));

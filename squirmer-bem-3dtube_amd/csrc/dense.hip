@@ -330,6 +330,8 @@ static void gemm_launch(int m, int n, int k, T alpha, const T* A, int lda, const
         return;
     }
     int tiles_m = (int)ceil_div(m, GBM), tiles_n = (int)ceil_div(n, GBN);
+    ProfScope prof(sizeof(T) == 8 ? PROF_DGEMM : PROF_SGEMM, 2.0 * m * n * k,
+                   (double)sizeof(T) * ((double)m * k + (double)k * n + (beta != T(0) ? 2.0 : 1.0) * m * n));
     hipLaunchKernelGGL(gemm_nn_kernel<T>, dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, k, alpha, A,
                        lda, B, ldb, beta, C, ldc, tiles_m);
     check_launch("gemm");
@@ -370,6 +372,8 @@ void gemm16(int kind, int m, int n, int k, float alpha, const float* A, int lda,
         hipLaunchKernelGGL(pow2_scale_cols, dim3(ceil_div(n, 4)), dim3(256), 0, ctx().s(), k, n, B, ldb, cs);
     }
     int tiles_m = (int)ceil_div(m, GBM), tiles_n = (int)ceil_div(n, GBN);
+    ProfScope prof(PROF_GEMM16, 2.0 * m * n * k,
+                   4.0 * ((double)m * k + (double)k * n + (beta != 0.0f ? 2.0 : 1.0) * m * n));
     if (bf)
         hipLaunchKernelGGL(gemm16_nn_kernel<true>, dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, k,
                            alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, rs, cs);
@@ -422,27 +426,193 @@ template void convert_matrix<float, double>(int, int, const float*, int, double*
 template void convert_matrix<double, double>(int, int, const double*, int, double*, int);
 template void convert_matrix<float, float>(int, int, const float*, int, float*, int);
 
-// ------------------------------------------------------------------ cooperative panel LU
-constexpr int LEAF = 32;       // columns per cooperative panel kernel
+// ------------------------------------------------------------------ panel LU
+constexpr int LEAF = 32;       // columns per panel kernel
 constexpr int PANEL_WG = 256;  // threads per workgroup
+constexpr int IB = LEAF;       // diagonal blocks inverted for the triangular solves
+constexpr int IBP = IB + 1;
 
+// Workspace of one factorisation: candidates of the cooperative pivot search, a saved
+// copy of the current panel, one verification flag per panel, and the inverses of the
+// IB x IB diagonal blocks of L (unit lower) used by the triangular solves inside getrf.
 struct PanelWork {
-    DBuf<double> cand;     // 2 x G x (LEAF + 2): (|pivot|, row, row values)
-    DBuf<double> rowj;     // 2 x LEAF: the row being replaced
-    DBuf<unsigned> bar;    // 2 ping-pong barrier counters
-    DBuf<int> status;      // [0] first zero pivot (1-based), [1] barrier timeout
+    DBuf<double> cand;  // 2 x G x (LEAF + 2): (|pivot|, row, row values)
+    DBuf<double> rowj;  // 2 x LEAF: the row being replaced
+    DBuf<unsigned> bar; // 2 ping-pong barrier counters
+    DBuf<int> status;   // [0] first zero pivot (1-based), [1] barrier timeout
+    DBuf<int> flags;    // per panel: 1 when row interchanges were needed
+    DBuf<char> backup;  // m x LEAF copy of the panel in its storage type
+    DBuf<char> inv;     // Linv blocks then Uinv blocks, IB x IB each, storage type
     int gmax = 0;
-    void ensure(int g) {
-        if (gmax >= g && cand.p) return;
-        gmax = std::max(g, 256);
-        cand.alloc((size_t)2 * gmax * (LEAF + 2));
-        rowj.alloc(2 * LEAF);
-        bar.alloc(2);
-        status.alloc(2);
+    void ensure(int g, int n, size_t elem) {
+        if (gmax < g || !cand.p) {
+            gmax = std::max(g, 256);
+            cand.alloc((size_t)2 * gmax * (LEAF + 2));
+            rowj.alloc(2 * LEAF);
+        }
+        if (!bar.p) bar.alloc(2);
+        if (!status.p) status.alloc(2);
+        const size_t nb = (size_t)(n + IB - 1) / IB;
+        if (flags.n < nb + 1) flags.alloc(nb + 1);
+        if (backup.n < (size_t)n * LEAF * elem) backup.alloc((size_t)n * LEAF * elem);
+        if (inv.n < 2 * nb * IB * IB * elem) inv.alloc(2 * nb * IB * IB * elem);
     }
 };
 static PanelWork g_panel;
 static int g_bar_parity = 0;
+
+// Inverses of one IB x IB diagonal block B (LDS, row-major; strict lower part = L with a unit
+// diagonal, upper part = U), padded to IB with the identity beyond w. Threads 0..IB-1 solve
+// L x = e_j, threads IB..2IB-1 solve U x = e_j; columns go to Xl / Xu (LDS) and then to
+// Linv / Uinv (global, column-major IB x IB).
+template <typename T>
+__device__ void invert_block(const double (*B)[IBP], int w, double (*Xl)[IBP], double (*Xu)[IBP], T* Linv, T* Uinv) {
+    const int t = threadIdx.x;
+    if (t < IB) {
+        const int j = t;
+        for (int i = 0; i < IB; ++i) {
+            double x;
+            if (i < j) x = 0.0;
+            else if (i == j) x = 1.0;
+            else {
+                double s = 0.0;
+                if (i < w)
+                    for (int k = j; k < i; ++k) s += B[i][k] * Xl[k][j];
+                x = -s;
+            }
+            Xl[i][j] = x;
+        }
+    } else if (t < 2 * IB) {
+        const int j = t - IB;
+        for (int i = IB - 1; i >= 0; --i) {
+            double x = 0.0;
+            if (i <= j) {
+                double s = (i == j) ? 1.0 : 0.0;
+                if (i < w)
+                    for (int k = i + 1; k <= j && k < w; ++k) s -= B[i][k] * Xu[k][j];
+                x = s / (i < w ? B[i][i] : 1.0);
+            }
+            Xu[i][j] = x;
+        }
+    }
+    __syncthreads();
+    for (int e = t; e < IB * IB; e += blockDim.x) {
+        const int i = e % IB, j = e / IB;
+        if (Linv) Linv[i + j * IB] = (T)Xl[i][j];
+        if (Uinv) Uinv[i + j * IB] = (T)Xu[i][j];
+    }
+}
+
+// Inverses of all diagonal blocks of an LU factor (one workgroup per block)
+template <typename T>
+__global__ void __launch_bounds__(64) invert_diag_kernel(int n, const T* __restrict__ LU, int lda, T* Linv, T* Uinv) {
+    __shared__ double Bk[IB][IBP], Xl[IB][IBP], Xu[IB][IBP];
+    const int b = blockIdx.x, r0 = b * IB, w = min(IB, n - r0);
+    for (int e = threadIdx.x; e < IB * IB; e += blockDim.x) {
+        const int i = e % IB, j = e / IB;
+        Bk[i][j] = (i < w && j < w) ? (double)LU[(r0 + i) + (size_t)(r0 + j) * lda] : 0.0;
+    }
+    __syncthreads();
+    invert_block<T>(Bk, w, Xl, Xu, Linv ? Linv + (size_t)b * IB * IB : nullptr,
+                    Uinv ? Uinv + (size_t)b * IB * IB : nullptr);
+}
+
+// Speculative panel factorisation without row interchanges. Every workgroup factors the
+// w x w top block redundantly (no pivoting), then its own rows below it as
+// L_r = A_r U11^-1. Partial pivoting keeps the diagonal pivot exactly when every
+// multiplier satisfies |l| <= 1 (LAPACK takes the first of equal magnitudes), which is
+// checked here; otherwise *flag is raised and the cooperative pivoting kernel redoes the
+// panel from the saved copy. Workgroup 0 also writes ipiv (identity) and L11^-1.
+constexpr int SPEC_ROWS = 64;  // rows per workgroup of the speculative pass
+template <typename T>
+__global__ void __launch_bounds__(PANEL_WG) getf2_spec(T* A, int lda, int m, int w, int* ipiv, int base,
+                                                       T* backup, int* flag, T* Linv) {
+    __shared__ double TB[IB][IBP];
+    __shared__ double Ui[IB][IBP];
+    __shared__ double Xl[IB][IBP];
+    __shared__ double Ch[IB][SPEC_ROWS];  // this workgroup's rows, column-major
+    __shared__ int bad;
+    const int g = blockIdx.x, tid = threadIdx.x;
+    const int r0 = g * SPEC_ROWS, nr = max(0, min(m, r0 + SPEC_ROWS) - r0);
+    if (tid == 0) bad = 0;
+    for (int e = tid; e < IB * IB; e += PANEL_WG) {
+        const int i = e % IB, j = e / IB;
+        TB[i][j] = (i < w && j < w) ? (double)A[i + (size_t)j * lda] : (i == j ? 1.0 : 0.0);
+    }
+    for (int e = tid; e < nr * w; e += PANEL_WG) {
+        const int lr = e % nr, c = e / nr;
+        const T v = A[(r0 + lr) + (size_t)c * lda];
+        backup[(r0 + lr) + (size_t)c * m] = v;
+        Ch[c][lr] = (double)v;
+    }
+    __syncthreads();
+    for (int j = 0; j < w; ++j) {
+        const double p = TB[j][j];
+        if (tid == 0 && !(p != 0.0 && p == p)) bad = 1;
+        const double rinv = 1.0 / p;
+        for (int i = j + 1 + tid; i < w; i += PANEL_WG) {
+            const double l = TB[i][j] * rinv;
+            TB[i][j] = l;
+            if (!(fabs(l) <= 1.0)) bad = 1;
+        }
+        __syncthreads();
+        const int nn = w - j - 1;
+        for (int e = tid; e < nn * nn; e += PANEL_WG) {
+            const int i = j + 1 + e % nn, c = j + 1 + e / nn;
+            TB[i][c] -= TB[i][j] * TB[j][c];
+        }
+        __syncthreads();
+    }
+    // U11^-1 (all workgroups) and L11^-1 (workgroup 0) by column solves
+    if (tid < IB) {
+        const int j = tid;
+        for (int i = IB - 1; i >= 0; --i) {
+            double x = 0.0;
+            if (i <= j) {
+                double s = (i == j) ? 1.0 : 0.0;
+                for (int k = i + 1; k <= j; ++k) s -= TB[i][k] * Ui[k][j];
+                x = s / TB[i][i];
+            }
+            Ui[i][j] = x;
+        }
+    } else if (g == 0 && tid >= 64 && tid < 64 + IB) {
+        const int j = tid - 64;
+        for (int i = 0; i < IB; ++i) {
+            double x;
+            if (i < j) x = 0.0;
+            else if (i == j) x = 1.0;
+            else {
+                double s = 0.0;
+                for (int k = j; k < i; ++k) s += TB[i][k] * Xl[k][j];
+                x = -s;
+            }
+            Xl[i][j] = x;
+        }
+    }
+    __syncthreads();
+    // L(r, c) = sum_{k <= c} A(r, k) U11^-1(k, c) for the rows below the top block
+    for (int e = tid; e < nr * w; e += PANEL_WG) {
+        const int lr = e % nr, c = e / nr, r = r0 + lr;
+        double v;
+        if (r < w) {
+            v = TB[r][c];
+        } else {
+            v = 0.0;
+            for (int k = 0; k <= c; ++k) v += Ch[k][lr] * Ui[k][c];
+            if (!(fabs(v) <= 1.0)) bad = 1;
+        }
+        A[r + (size_t)c * lda] = (T)v;
+    }
+    if (g == 0) {
+        for (int j = tid; j < w; j += PANEL_WG) ipiv[j] = base + j;
+        for (int e = tid; e < IB * IB; e += PANEL_WG) {
+            const int i = e % IB, j = e / IB;
+            Linv[i + j * IB] = (T)Xl[i][j];
+        }
+    }
+    __syncthreads();
+    if (tid == 0 && bad) atomicOr(flag, 1);
+}
 
 // Bounded grid barrier: counter reaches target when every workgroup has arrived. A
 // sticky timeout flag ends all waits if a workgroup never arrives, so the kernel exits.
@@ -465,17 +635,24 @@ __device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned target, int
     __syncthreads();
 }
 
-// Factor the m x w panel (w <= LEAF) at A with partial pivoting (LAPACK dgetf2 order:
-// pivot = first row of largest magnitude, swap, scale by the reciprocal pivot, rank-1
-// update). ipiv[j] receives base + pivot row (0-based, absolute).
+// Partial-pivoting panel factorisation (LAPACK dgetf2 order: pivot = first row of largest
+// magnitude, swap, scale by the reciprocal pivot, rank-1 update), run only when the
+// speculative pass flagged the panel: it restarts from the saved copy. The rows are
+// spread over the workgroups and held in LDS; each column's pivot is chosen with one grid
+// barrier (double-buffered candidate rows). ipiv[j] = base + pivot row (0-based).
+// Workgroup 0 finishes with the inverse of the new L11.
 template <typename T>
-__global__ void __launch_bounds__(PANEL_WG) getf2_coop(T* A, int lda, int m, int w, int rpw, int* ipiv,
-                                                       int base, double* cand, double* rowj, unsigned* bar,
-                                                       unsigned* bar_next, int* status) {
-    extern __shared__ double lds[];  // rpw x w, column-major, then scratch
+__global__ void __launch_bounds__(PANEL_WG) getf2_coop(T* A, int lda, int m, int w, int rpw, int* ipiv, int base,
+                                                       double* cand, double* rowj, unsigned* bar, unsigned* bar_next,
+                                                       int* status, const int* flag, const T* backup, T* Linv) {
+    extern __shared__ double lds[];  // max(rpw x w, 2 IB IBP), then scratch
+    __shared__ double TBs[IB][IBP];
     const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
+    if (g == 0 && tid == 0) *bar_next = 0u;  // reset the other counter for the next launch
+    if (*flag == 0) return;                  // speculative factors verified: nothing to do
     const int r0 = g * rpw, nr = max(0, min(m, r0 + rpw) - r0);
-    double* prow = lds + (size_t)rpw * w;   // pivot row values [LEAF]
+    const int area = max(rpw * w, 2 * IB * IBP);
+    double* prow = lds + area;              // pivot row values [LEAF]
     double* redv = prow + LEAF;             // reduction scratch [PANEL_WG]
     int* redi = (int*)(redv + PANEL_WG);    // [PANEL_WG]
     __shared__ int s_piv;
@@ -484,9 +661,8 @@ __global__ void __launch_bounds__(PANEL_WG) getf2_coop(T* A, int lda, int m, int
 
     for (int e = tid; e < nr * w; e += PANEL_WG) {
         int lr = e % nr, c = e / nr;
-        lds[c * rpw + lr] = (double)A[(r0 + lr) + (size_t)c * lda];
+        lds[c * rpw + lr] = (double)backup[(r0 + lr) + (size_t)c * m];
     }
-    if (g == 0 && tid == 0) *bar_next = 0u;  // reset the other counter for the next launch
     __syncthreads();
 
     const int csz = LEAF + 2;
@@ -584,29 +760,55 @@ __global__ void __launch_bounds__(PANEL_WG) getf2_coop(T* A, int lda, int m, int
         }
         __syncthreads();
     }
+    if (g == 0)
+        for (int e = tid; e < IB * IB; e += PANEL_WG) {
+            const int i = e % IB, j = e / IB;
+            TBs[i][j] = (i < w && j < w) ? lds[j * rpw + i] : 0.0;
+        }
     for (int e = tid; e < nr * w; e += PANEL_WG) {
         int lr = e % nr, c = e / nr;
         A[(r0 + lr) + (size_t)c * lda] = (T)lds[c * rpw + lr];
+    }
+    if (g == 0) {
+        __syncthreads();
+        double(*Xl)[IBP] = (double(*)[IBP])lds;
+        double(*Xu)[IBP] = (double(*)[IBP])(lds + IB * IBP);
+        invert_block<T>(TBs, w, Xl, Xu, Linv, nullptr);
     }
 }
 
 template <typename T>
 static void panel_leaf(T* A, int lda, int m, int w, int* ipiv, int base) {
-    // one workgroup per >= 128 rows, at most one per CU
+    // one workgroup per >= 128 rows, at most one per CU (the fallback is cooperative)
     int G = std::max(1, std::min((int)ceil_div(m, 128), std::max(ctx().cu_count, 1)));
     int rpw = (int)ceil_div(m, G);
     G = (int)ceil_div(m, rpw);
-    g_panel.ensure(G);
-    size_t shmem = ((size_t)rpw * w + LEAF + PANEL_WG) * sizeof(double) + PANEL_WG * sizeof(int);
-    // default dynamic LDS limit per workgroup (64 KiB): up to about 60k rows per panel
-    SQ_REQUIRE(shmem <= 64 * 1024, SQBEM_ERR_ARG, "panel too tall for LDS (more than ~60k rows)");
+    const size_t area = std::max<size_t>((size_t)rpw * w, 2 * IB * IBP);
+    size_t shmem = (area + LEAF + PANEL_WG) * sizeof(double) + PANEL_WG * sizeof(int);
+    // default dynamic LDS limit per workgroup (64 KiB) minus the static part: ~60k rows
+    SQ_REQUIRE(shmem + IB * IBP * sizeof(double) <= 64 * 1024, SQBEM_ERR_ARG,
+               "panel too tall for LDS (more than ~60k rows)");
+    const int blk = base / IB;
+    T* backup = (T*)g_panel.backup.p;
+    int* flag = g_panel.flags.p + blk;
+    T* Linv = (T*)g_panel.inv.p + (size_t)blk * IB * IB;
+    {
+        ProfScope prof(PROF_PANEL, (double)m * w * w, 3.0 * sizeof(T) * m * w);
+        hipLaunchKernelGGL(getf2_spec<T>, dim3(ceil_div(m, SPEC_ROWS)), dim3(PANEL_WG), 0, ctx().s(), A, lda, m, w,
+                           ipiv, base, backup, flag, Linv);
+        check_launch("getf2_spec");
+    }
     unsigned* bar = g_panel.bar.p + g_bar_parity;
     unsigned* bar_next = g_panel.bar.p + (g_bar_parity ^ 1);
     g_bar_parity ^= 1;
     double* cand = g_panel.cand.p;
     double* rowj = g_panel.rowj.p;
     int* status = g_panel.status.p;
-    void* args[] = {&A, &lda, &m, &w, &rpw, &ipiv, &base, &cand, &rowj, &bar, &bar_next, &status};
+    const int* cflag = flag;
+    const T* cbackup = backup;
+    void* args[] = {&A, &lda, &m, &w, &rpw, &ipiv, &base, &cand, &rowj, &bar, &bar_next, &status, &cflag, &cbackup,
+                    &Linv};
+    ProfScope prof(PROF_PANEL, 0.0, 0.0);
     SQ_HIP(hipLaunchCooperativeKernel((const void*)getf2_coop<T>, dim3(G), dim3(PANEL_WG), args, (unsigned)shmem,
                                       ctx().s()));
 }
@@ -636,6 +838,7 @@ __global__ void laswp_kernel(T* A, int lda, int ncols, const int* __restrict__ i
 template <typename T>
 static void laswp(T* A, int lda, int ncols, const int* ipiv, int ibase, int vbase, int j0, int j1, int dir = 1) {
     if (ncols <= 0 || j1 <= j0) return;
+    ProfScope prof(PROF_LASWP, 0.0, 0.0);
     hipLaunchKernelGGL(laswp_kernel<T>, dim3(ceil_div(ncols, 256)), dim3(256), 0, ctx().s(), A, lda, ncols, ipiv,
                        ibase, vbase, j0, j1, dir);
     check_launch("laswp");
@@ -713,6 +916,7 @@ template <typename T, bool LOWER, bool UNIT, bool TRANS>
 static void trsm_rec(int kind, int n, int nrhs, const T* L, int ldl, T* B, int ldb) {
     if (n <= 0 || nrhs <= 0) return;
     if (n <= TL) {
+        ProfScope prof(PROF_TRSM, (double)n * n * nrhs, sizeof(T) * (2.0 * n * nrhs + (double)n * n));
         hipLaunchKernelGGL((trsm_leaf<T, LOWER, UNIT, TRANS>), dim3(ceil_div(nrhs, 64)), dim3(64), 0, ctx().s(), n,
                            nrhs, L, ldl, B, ldb);
         check_launch("trsm_leaf");
@@ -784,6 +988,45 @@ template void gemm_tn<double>(int, int, int, int, double, const double*, int, co
                               int);
 template void gemm_tn<float>(int, int, int, int, float, const float*, int, const float*, int, float, float*, int);
 
+// ------------------------------------------------------------------ inverse-based triangular solves
+// B <- T^-1 B for the n x n triangle T at L whose IB x IB diagonal blocks have their
+// inverses at inv (consecutive blocks, column-major IB x IB). Recursive halving at
+// multiples of IB; a leaf is one in-place GEMM with the block inverse (m <= IB < 128: one
+// row tile, every workgroup reads its columns of B completely before writing them).
+template <typename T>
+static void gemm_exact(int m, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb, T beta, T* C, int ldc);
+template <>
+void gemm_exact<double>(int m, int n, int k, double alpha, const double* A, int lda, const double* B, int ldb,
+                        double beta, double* C, int ldc) {
+    dgemm(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+template <>
+void gemm_exact<float>(int m, int n, int k, float alpha, const float* A, int lda, const float* B, int ldb, float beta,
+                       float* C, int ldc) {
+    sgemm(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+template <typename T, bool LOWER>
+static void trsm_inv(int kind, int n, int nrhs, const T* L, int ldl, const T* inv, T* B, int ldb) {
+    if (n <= 0 || nrhs <= 0) return;
+    if (n <= IB) {
+        ProfScope prof(PROF_TRSM, 2.0 * n * n * nrhs, sizeof(T) * (2.0 * n * nrhs + (double)n * n));
+        gemm_exact<T>(n, nrhs, n, T(1), inv, IB, B, ldb, T(0), B, ldb);
+        return;
+    }
+    const int n1 = split_point(n, IB), n2 = n - n1;
+    const T* inv2 = inv + (size_t)(n1 / IB) * IB * IB;
+    if (LOWER) {
+        trsm_inv<T, LOWER>(kind, n1, nrhs, L, ldl, inv, B, ldb);
+        gemm_kind<T>(kind, n2, nrhs, n1, T(-1), L + n1, ldl, B, ldb, T(1), B + n1, ldb);
+        trsm_inv<T, LOWER>(kind, n2, nrhs, L + n1 + (size_t)n1 * ldl, ldl, inv2, B + n1, ldb);
+    } else {
+        trsm_inv<T, LOWER>(kind, n2, nrhs, L + n1 + (size_t)n1 * ldl, ldl, inv2, B + n1, ldb);
+        gemm_kind<T>(kind, n1, nrhs, n2, T(-1), L + (size_t)n1 * ldl, ldl, B + n1, ldb, T(1), B, ldb);
+        trsm_inv<T, LOWER>(kind, n1, nrhs, L, ldl, inv, B, ldb);
+    }
+}
+
 // ------------------------------------------------------------------ recursive LU
 template <typename T>
 static void getrf_rec(int kind, T* A, int lda, int m, int w, int* ipiv, int base) {
@@ -795,7 +1038,8 @@ static void getrf_rec(int kind, T* A, int lda, int m, int w, int* ipiv, int base
     getrf_rec<T>(kind, A, lda, m, w1, ipiv, base);
     T* A12 = A + (size_t)w1 * lda;
     laswp<T>(A12, lda, w2, ipiv, base, base, 0, w1);
-    trsm_rec<T, true, true, false>(kind, w1, w2, A, lda, A12, lda);
+    const T* Linv = (const T*)g_panel.inv.p + (size_t)(base / IB) * IB * IB;
+    trsm_inv<T, true>(kind, w1, w2, A, lda, Linv, A12, lda);
     gemm_kind<T>(kind, m - w1, w2, w1, T(-1), A + w1, lda, A12, lda, T(1), A12 + w1, lda);
     getrf_rec<T>(kind, A12 + w1, lda, m - w1, w2, ipiv, base + w1);
     laswp<T>(A, lda, w1, ipiv, base, base, w1, std::min(w, m));
@@ -811,9 +1055,10 @@ template <typename T>
 int getrf(int kind, int n, T* A, int lda, int* ipiv) {
     if (n <= 0) return 0;
     SQ_REQUIRE(lda >= n, SQBEM_ERR_ARG, "getrf: lda < n");
-    g_panel.ensure(256);
+    g_panel.ensure(256, n, sizeof(T));
     g_panel.bar.zero();
     g_panel.status.zero();
+    g_panel.flags.zero();
     g_bar_parity = 0;
     getrf_rec<T>(kind, A, lda, n, n, ipiv, 0);
     hipLaunchKernelGGL(ipiv_to_one_based<T>, dim3(ceil_div(n, 256)), dim3(256), 0, ctx().s(), n, ipiv, 1);
@@ -826,14 +1071,34 @@ int getrf(int kind, int n, T* A, int lda, int* ipiv) {
 template int getrf<double>(int, int, double*, int, int*);
 template int getrf<float>(int, int, float*, int, int*);
 
+// number of panels that needed row interchanges in the last getrf (0 = diagonal pivots)
+int last_getrf_pivoted_panels(int n) {
+    const int nb = (n + LEAF - 1) / LEAF;
+    std::vector<int> f(nb);
+    SQ_HIP(hipMemcpyAsync(f.data(), g_panel.flags.p, nb * sizeof(int), hipMemcpyDeviceToHost, ctx().s()));
+    sync();
+    int c = 0;
+    for (int v : f) c += v != 0;
+    return c;
+}
+
+static DBuf<char> g_getrs_inv;
+
 // B <- A^-1 B (trans = 0) or A^-T B (trans = 1) from getrf factors; ipiv 1-based
 template <typename T>
 void getrs(int kind, int trans, int n, int nrhs, const T* LU, int lda, const int* ipiv, T* B, int ldb) {
     if (n <= 0 || nrhs <= 0) return;
     if (!trans) {
+        const int nb = (n + IB - 1) / IB;
+        const size_t need = 2 * (size_t)nb * IB * IB * sizeof(T);
+        if (g_getrs_inv.n < need) g_getrs_inv.alloc(need);
+        T* Linv = (T*)g_getrs_inv.p;
+        T* Uinv = Linv + (size_t)nb * IB * IB;
+        hipLaunchKernelGGL(invert_diag_kernel<T>, dim3(nb), dim3(64), 0, ctx().s(), n, LU, lda, Linv, Uinv);
+        check_launch("invert_diag");
         laswp<T>(B, ldb, nrhs, ipiv, 0, 1, 0, n, +1);
-        trsm_rec<T, true, true, false>(kind, n, nrhs, LU, lda, B, ldb);
-        trsm_rec<T, false, false, false>(kind, n, nrhs, LU, lda, B, ldb);
+        trsm_inv<T, true>(kind, n, nrhs, LU, lda, Linv, B, ldb);
+        trsm_inv<T, false>(kind, n, nrhs, LU, lda, Uinv, B, ldb);
     } else {
         // A^T = U^T L^T P: solve U^T (lower, non-unit), L^T (upper, unit), then P^T
         trsm_rec<T, true, false, true>(kind, n, nrhs, LU, lda, B, ldb);

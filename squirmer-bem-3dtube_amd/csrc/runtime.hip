@@ -1,6 +1,7 @@
 // runtime.hip -- device selection, library stream, error reporting, small utility kernels.
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "common.hpp"
 
@@ -60,11 +61,94 @@ void fill_zero(void* p, size_t bytes) {
     check_launch("fill_zero");
 }
 
+// ---- kernel timing
+namespace {
+struct ProfRec {
+    int cls;
+    double flops, bytes;
+    hipEvent_t e0, e1;
+};
+struct ProfTotals {
+    long launches = 0;
+    double flops = 0, bytes = 0, ms = 0;
+};
+bool g_prof = false;
+std::vector<ProfRec> g_pending;
+std::vector<hipEvent_t> g_event_pool;
+ProfTotals g_tot[PROF_NCLASS];
+
+hipEvent_t take_event() {
+    if (!g_event_pool.empty()) {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    SQ_HIP(hipEventCreate(&e));
+    return e;
+}
+void resolve_pending() {
+    for (auto& r : g_pending) {
+        float ms = 0;
+        SQ_HIP(hipEventSynchronize(r.e1));
+        SQ_HIP(hipEventElapsedTime(&ms, r.e0, r.e1));
+        auto& t = g_tot[r.cls];
+        t.launches += 1;
+        t.flops += r.flops;
+        t.bytes += r.bytes;
+        t.ms += ms;
+        g_event_pool.push_back(r.e0);
+        g_event_pool.push_back(r.e1);
+    }
+    g_pending.clear();
+}
+}  // namespace
+
+bool prof_enabled() { return g_prof; }
+
+ProfScope::ProfScope(int c, double f, double b) : cls(c), flops(f), bytes(b) {
+    if (!g_prof) return;
+    e0 = take_event();
+    e1 = take_event();
+    SQ_HIP(hipEventRecord(e0, ctx().s()));
+}
+
+ProfScope::~ProfScope() {
+    if (!e0) return;
+    if (hipEventRecord(e1, ctx().s()) == hipSuccess) g_pending.push_back({cls, flops, bytes, e0, e1});
+    if (g_pending.size() > 4096) {
+        try {
+            resolve_pending();
+        } catch (...) {
+        }
+    }
+}
+
 }  // namespace sq
 
 using namespace sq;
 
 extern "C" {
+
+int sqbem_profile(int enable) {
+    return guarded([&] {
+        resolve_pending();
+        for (auto& t : g_tot) t = ProfTotals{};
+        g_prof = enable != 0;
+    });
+}
+
+int sqbem_profile_get(int cls, long* launches, double* flops, double* bytes, double* ms) {
+    return guarded([&] {
+        SQ_REQUIRE(cls >= 0 && cls < PROF_NCLASS, SQBEM_ERR_ARG, "unknown kernel class");
+        resolve_pending();
+        const auto& t = g_tot[cls];
+        if (launches) *launches = t.launches;
+        if (flops) *flops = t.flops;
+        if (bytes) *bytes = t.bytes;
+        if (ms) *ms = t.ms;
+    });
+}
 
 const char* sqbem_last_error(void) { return g_last_error.c_str(); }
 

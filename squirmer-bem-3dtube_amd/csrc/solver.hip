@@ -189,6 +189,7 @@ static void gemv(int m, int n, int nrhs, double alpha, const T* A, long lda, con
         scale_matrix<T>(m, nrhs, (T)beta, Y, (int)ldy);
         return;
     }
+    ProfScope prof(PROF_GEMV, 2.0 * m * n * nrhs, sizeof(T) * ((double)m * n + (double)n * nrhs + 2.0 * m * nrhs));
     hipLaunchKernelGGL(gemv_part_kernel<T>, dim3(ceil_div(m, 256), GV_CHUNKS), dim3(256), 0, ctx().s(), m, n, nrhs, A,
                        lda, X, ldx, g_gv.p);
     hipLaunchKernelGGL(gemv_reduce_kernel<T>, dim3(ceil_div(m, 256)), dim3(256), 0, ctx().s(), m, nrhs, g_gv.p, alpha,

@@ -32,6 +32,8 @@ _SIGS = {
     "sqbem_set_stream": (_i, [_vp]),
     "sqbem_sync": (_i, []),
     "sqbem_device_info": (_i, [ctypes.c_char_p, _i, _vp, _vp]),
+    "sqbem_profile": (_i, [_i]),
+    "sqbem_profile_get": (_i, [_i, _vp, _vp, _vp, _vp]),
     "sqbem_mesh_swimmer_tube": (_vp, [_d, _d, _d, _i, _i]),
     "sqbem_mesh_tube": (_vp, [_d, _vp, _i, _i, _i]),
     "sqbem_mesh_duct": (_vp, [_d, _d, _i, _i, _i]),
@@ -139,6 +141,21 @@ def device_info():
     gb = ctypes.c_double()
     check(lib().sqbem_device_info(name, 256, ctypes.byref(cu), ctypes.byref(gb)))
     return name.value.decode(), cu.value, gb.value
+
+
+PROF_CLASSES = {"dgemm": 0, "sgemm": 1, "gemm16": 2, "panel": 3, "trsm": 4, "assemble": 5, "gemv": 6, "laswp": 7}
+
+
+def profile(enable=True):
+    """Reset and enable (or disable) per-kernel HIP-event timing inside libsqbem."""
+    check(lib().sqbem_profile(int(enable)))
+
+
+def profile_get(name):
+    n, fl, by, ms = ctypes.c_long(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    check(lib().sqbem_profile_get(PROF_CLASSES[name], ctypes.byref(n), ctypes.byref(fl), ctypes.byref(by),
+                                  ctypes.byref(ms)))
+    return {"launches": n.value, "flops": fl.value, "bytes": by.value, "ms": ms.value}
 
 
 # ------------------------------------------------------------------ meshes

@@ -21,6 +21,15 @@ def sq():
     """The libsqbem Python binding, bound to device 0."""
     import __graft_entry__ as ge
 
+    # torch ships its own HIP runtime; it has to initialise the device before libsqbem's
+    # runtime does, so that both see it (the tests pass torch device tensors to the C ABI)
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except ImportError:
+        pass
     pkg = ge.load_package()
     pkg.init(0)
     return pkg
