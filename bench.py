@@ -142,7 +142,7 @@ def main():
     for _ in range(args.warmup):
         cold_solve(pkg, W, prec, e)
     pkg.sync()
-    pkg.profile(True)
+    pkg.profile(True, classes=["dgemm", "sgemm", "gemm16"])
     t0 = time.perf_counter()
     runs = []
     for _ in range(args.steps):

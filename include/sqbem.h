@@ -40,10 +40,10 @@ int sqbem_init(int device);
 int sqbem_set_stream(void* hip_stream); /* NULL: the library's own stream */
 int sqbem_sync(void);
 int sqbem_device_info(char* name, int name_len, int* cu_count, double* hbm_gb);
-/* Kernel timing (no reference counterpart; measurement only). sqbem_profile(1) resets
- * and starts recording HIP events around every launch of the classes below on the
- * library stream; sqbem_profile_get returns launches, algorithmic flops and bytes, and
- * summed kernel time (ms) of one class. */
+/* Kernel timing (no reference counterpart; measurement only). sqbem_profile(mask) resets
+ * the totals and records HIP events around every launch, on the library stream, of the
+ * classes in mask (bit 1 << class; -1 = all classes; 0 = off); sqbem_profile_get returns
+ * launches, algorithmic flops and bytes, and summed kernel time (ms) of one class. */
 #define SQBEM_PROF_DGEMM 0
 #define SQBEM_PROF_SGEMM 1
 #define SQBEM_PROF_GEMM16 2
@@ -52,7 +52,7 @@ int sqbem_device_info(char* name, int name_len, int* cu_count, double* hbm_gb);
 #define SQBEM_PROF_ASSEMBLE 5
 #define SQBEM_PROF_GEMV 6
 #define SQBEM_PROF_LASWP 7
-int sqbem_profile(int enable);
+int sqbem_profile(int mask);
 int sqbem_profile_get(int kernel_class, long* launches, double* flops, double* bytes, double* ms);
 
 /* ------------------------------------------------------------------ meshes (host)
@@ -188,6 +188,15 @@ sqbem_duct* sqbem_duct_create(const sqbem_mesh* mesh, double mu);
 int sqbem_duct_solve(const sqbem_duct* d, double p_in, double p_out, double* u, double* f,
                      double* Q);
 void sqbem_duct_free(sqbem_duct* d);
+
+/* run_trajectory (src/trajectory.hpp:23-151): RK4 start-up + Adams-Bashforth 4 with the
+ * tube factors resident in HBM; writes <out_prefix>.csv (and <out_prefix>.ckpt while
+ * paused), status receives "finished", "wall contact" or "paused". n_r is the rings per
+ * cap (the reference's Tube default is 4); precision as for sqbem_tube_create. */
+int sqbem_trajectory_run(double alpha, double a_over_R, double beta0, double pitch0_deg,
+                         double yaw0_deg, double dt, double t_max, double L_over_R, int n_theta,
+                         int sphere_level, double beta_stop, double budget_s, int n_r, int precision,
+                         const char* out_prefix, int verbose, char* status, int status_len);
 
 #ifdef __cplusplus
 }

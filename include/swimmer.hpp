@@ -1,0 +1,188 @@
+// swimmer.hpp -- a spherical squirmer in a closed rigid tube (or in unbounded fluid),
+// solved on the GPU by libsqbem.
+//
+// Boundary conditions: tube wall and caps u = 0; sphere u = U + Omega x (x - xc) + u_s(x);
+// unknowns: tractions everywhere, U, Omega; closure: force-free and torque-free.
+// Linear system (t = tube, s = sphere), T the fixed tube-tube block:
+//   [ T  B  Ct ] [f_t ]   [r_t]
+//   [ D  E  Cs ] [f_s ] = [r_s]
+//   [ 0  F  0  ] [UOm ]   [ 0 ]
+// T is assembled, deflated and LU-factorised once in HBM (Tube); the swimmer is eliminated
+// through the Schur complement G = [E | Cs] - D T^-1 [B | Ct] of size 3 Ns + 6, formed
+// with MFMA GEMMs against the resident tube factors.
+//
+// API of src/swimmer.hpp:33-289 (Tube, SwimmerSolution, SwimmerSolver). Extension: an
+// optional `precision` (SQBEM_FP32_IR, SQBEM_FP16_IR, SQBEM_BF16_IR) factors in low
+// precision and recovers FP64 accuracy by iterative refinement against the FP64 operator.
+#pragma once
+#include <memory>
+#include <stdexcept>
+#include <vector>
+
+#include "sqbem.h"
+#include "utils/field.hpp"
+#include "utils/io.hpp"
+#include "utils/kernels.hpp"
+#include "utils/linalg.hpp"
+#include "utils/mesh.hpp"
+
+namespace bem {
+
+// ---------------------------------------------------------------- tube
+struct Tube {
+    Mesh mesh;
+    int Nt = 0;
+    double R = 1, L = 4, mu = 1;
+    LUFactor lu;  // view onto the factors held in HBM
+    int precision = SQBEM_FP64;
+    double t_assembly = 0, t_factor = 0;
+
+    Tube() = default;
+    Tube(double R_, double L_, double a, int n_theta, int n_r = 4, double mu_ = 1, int precision_ = SQBEM_FP64)
+        : R(R_), L(L_), mu(mu_), precision(precision_) {
+        std::shared_ptr<sqbem_mesh> m(sqbem_mesh_swimmer_tube(R, L, a, n_theta, n_r), sqbem_mesh_free);
+        if (!m) throw std::runtime_error(sqbem_last_error());
+        mesh = detail::from_sqbem(m.get());
+        Nt = mesh.P();
+        sqbem_tube* t = sqbem_tube_create(m.get(), mu, precision);
+        if (!t) throw std::runtime_error(sqbem_last_error());
+        handle_.reset(t, sqbem_tube_free);
+        sqbem_check(sqbem_tube_info(t, nullptr, &t_assembly, &t_factor));
+        lu.attach(sqbem_tube_lu(t), 3 * Nt);
+    }
+
+    // rank-one deflation of the null mode f = c m on panels [p0, p0+np) of M, on a host Mat
+    // (the device solver applies the same term inside its assembly)
+    static void deflate(const Mesh& M, int p0, int np, Mat& A, int off, double mu) {
+        double At = 0;
+        for (int p = p0; p < p0 + np; ++p) At += M.area[p];
+        const double sc = 1.0 / (EIGHT_PI * mu * std::sqrt(At));
+        for (int i = 0; i < np; ++i)
+            for (int p = 0; p < np; ++p)
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b)
+                        A(off + 3 * i + a, off + 3 * p + b) +=
+                            sc * M.normal[p0 + i][a] * M.normal[p0 + p][b] * M.area[p0 + p];
+    }
+
+    const sqbem_tube* handle() const { return handle_.get(); }
+
+private:
+    std::shared_ptr<sqbem_tube> handle_;
+};
+
+// ---------------------------------------------------------------- swimmer
+struct SwimmerSolution {
+    V3 U, Om;
+    std::vector<V3> f_sphere, f_tube;  // tractions (f = sigma.m, m out of the fluid)
+};
+
+class SwimmerSolver {
+public:
+    const Tube* tube;  // nullptr -> unbounded fluid
+    Mesh S;            // sphere mesh
+    V3 xc;
+    double mu;
+    int Ns, Nt;
+    double t_assembly = 0, t_schur = 0;
+    int ir_iters = 0;  // refinement steps of the last solve (refined precisions)
+
+    SwimmerSolver(const Tube* t, const Mesh& sphere, V3 center, double mu_ = 1.0, int precision = -1)
+        : tube(t), S(sphere), xc(center), mu(mu_) {
+        Ns = S.P();
+        Nt = tube ? tube->Nt : 0;
+        const int prec = precision >= 0 ? precision : (tube ? tube->precision : SQBEM_FP64);
+        auto m = detail::to_sqbem(S);
+        const double c[3] = {xc.x, xc.y, xc.z};
+        sqbem_swimmer* s = sqbem_swimmer_create(tube ? tube->handle() : nullptr, m.get(), c, mu, prec);
+        if (!s) throw std::runtime_error(sqbem_last_error());
+        h_.reset(s, sqbem_swimmer_free);
+        times();
+    }
+
+    // squirming slip on every sphere panel for orientation e and modes B1, B2
+    std::vector<V3> slip(V3 e, double B1, double B2) const {
+        std::vector<V3> u(Ns);
+        for (int p = 0; p < Ns; ++p) u[p] = squirmer_slip(S.centroid[p], xc, e, B1, B2);
+        return u;
+    }
+
+    // force- and torque-free swimming, one solution per slip field
+    std::vector<SwimmerSolution> solve_free(const std::vector<std::vector<V3>>& slips) {
+        const int k = (int)slips.size();
+        std::vector<SwimmerSolution> out(k);
+        for (int j0 = 0; j0 < k; j0 += 8) {  // the device solve takes up to 8 fields at once
+            const int kk = std::min(8, k - j0);
+            std::vector<double> u((size_t)kk * 3 * Ns), U(3 * kk), Om(3 * kk), fs((size_t)kk * 3 * Ns),
+                ft((size_t)kk * 3 * std::max(Nt, 1));
+            for (int j = 0; j < kk; ++j)
+                for (int p = 0; p < Ns; ++p)
+                    for (int c = 0; c < 3; ++c) u[(size_t)j * 3 * Ns + 3 * p + c] = slips[j0 + j][p][c];
+            sqbem_check(sqbem_swimmer_solve_free(h_.get(), kk, u.data(), U.data(), Om.data(), fs.data(),
+                                                 Nt ? ft.data() : nullptr));
+            for (int j = 0; j < kk; ++j) out[j0 + j] = unpack(U.data() + 3 * j, Om.data() + 3 * j, fs, ft, j);
+        }
+        times();
+        return out;
+    }
+
+    // rigid sphere with prescribed (U, Omega) and no slip ("towed" problem)
+    SwimmerSolution solve_prescribed(V3 U, V3 Om) {
+        std::vector<double> fs(3 * (size_t)Ns), ft(3 * (size_t)std::max(Nt, 1));
+        const double u[3] = {U.x, U.y, U.z}, w[3] = {Om.x, Om.y, Om.z};
+        sqbem_check(sqbem_swimmer_solve_prescribed(h_.get(), u, w, fs.data(), Nt ? ft.data() : nullptr));
+        times();
+        return unpack(u, w, fs, ft, 0);
+    }
+
+    // hydrodynamic force ON the sphere: F = -sum f dA (m points into it)
+    V3 force(const SwimmerSolution& s) const {
+        V3 F{};
+        for (int p = 0; p < Ns; ++p) F = F - S.area[p] * s.f_sphere[p];
+        return F;
+    }
+
+    // velocity at points in the fluid (lab frame)
+    std::vector<V3> velocity(const SwimmerSolution& s, const std::vector<V3>& slip_field,
+                             const std::vector<V3>& pts) const {
+        Mesh all = tube ? merge(tube->mesh, S) : S;
+        std::vector<V3> u(all.P()), f(all.P());
+        for (int p = 0; p < Nt; ++p) f[p] = s.f_tube[p];
+        for (int p = 0; p < Ns; ++p) {
+            u[Nt + p] = s.U + cross(s.Om, S.centroid[p] - xc) + slip_field[p];
+            f[Nt + p] = s.f_sphere[p];
+        }
+        return velocity_at(all, u, f, mu, pts);
+    }
+
+    const sqbem_swimmer* handle() const { return h_.get(); }
+
+private:
+    std::shared_ptr<sqbem_swimmer> h_;
+
+    void times() {
+        double ir_res = 0;
+        sqbem_check(sqbem_swimmer_times(h_.get(), &t_assembly, &t_schur, &ir_iters, &ir_res));
+    }
+    SwimmerSolution unpack(const double* U, const double* Om, const std::vector<double>& fs,
+                           const std::vector<double>& ft, int j) const {
+        SwimmerSolution s;
+        s.U = V3{U[0], U[1], U[2]};
+        s.Om = V3{Om[0], Om[1], Om[2]};
+        s.f_sphere.resize(Ns);
+        for (int p = 0; p < Ns; ++p) {
+            const double* q = fs.data() + (size_t)j * 3 * Ns + 3 * p;
+            s.f_sphere[p] = V3{q[0], q[1], q[2]};
+        }
+        if (Nt) {
+            s.f_tube.resize(Nt);
+            for (int p = 0; p < Nt; ++p) {
+                const double* q = ft.data() + (size_t)j * 3 * Nt + 3 * p;
+                s.f_tube[p] = V3{q[0], q[1], q[2]};
+            }
+        }
+        return s;
+    }
+};
+
+}  // namespace bem

@@ -1027,6 +1027,94 @@ static void trsm_inv(int kind, int n, int nrhs, const T* L, int ldl, const T* in
     }
 }
 
+// ------------------------------------------------------------------ skinny triangular solves
+// B (n x nrhs, nrhs <= 8) <- T^-1 B, one launch per SV-row diagonal block: every
+// workgroup solves the block's SV x SV triangle redundantly in its first wave (lane i owns
+// row i; the finished x_j is broadcast with a shuffle, so no barriers), workgroup 0 stores
+// it, and each workgroup subtracts its own rows' share L(rows, block) x_block. Reads the
+// factor once per solve: the memory-bound form of getrs for a few right-hand sides.
+constexpr int SV = 64;
+template <typename T, bool LOWER, bool UNIT>
+__global__ void __launch_bounds__(256) trsv_block(int n, int nrhs, const T* __restrict__ L, int ldl, T* B, int ldb,
+                                                  int j0) {
+    __shared__ double Ld[SV][SV + 1];
+    __shared__ double X[8][SV];
+    const int jb = min(SV, n - j0);
+    const int tid = threadIdx.x;
+    for (int e = tid; e < SV * SV; e += blockDim.x) {
+        const int i = e % SV, j = e / SV;
+        Ld[i][j] = (i < jb && j < jb) ? (double)L[(j0 + i) + (size_t)(j0 + j) * ldl] : (i == j ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    if (tid < 64) {
+        const int i = tid;
+        double x[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) x[r] = (r < nrhs && i < jb) ? (double)B[(j0 + i) + (size_t)r * ldb] : 0.0;
+        if (LOWER) {
+            for (int j = 0; j < SV; ++j) {
+                if (!UNIT && i == j) {
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) x[r] /= Ld[j][j];
+                }
+                const double lij = Ld[i][j];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    const double xj = __shfl(x[r], j);
+                    if (i > j) x[r] -= lij * xj;
+                }
+            }
+        } else {
+            for (int j = SV - 1; j >= 0; --j) {
+                if (!UNIT && i == j) {
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) x[r] /= Ld[j][j];
+                }
+                const double lij = Ld[i][j];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    const double xj = __shfl(x[r], j);
+                    if (i < j) x[r] -= lij * xj;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r) X[r][i] = x[r];
+        if (blockIdx.x == 0 && i < jb)
+            for (int r = 0; r < nrhs; ++r) B[(j0 + i) + (size_t)r * ldb] = (T)x[r];
+    }
+    __syncthreads();
+    // rows outside the block: below it (forward) or above it (backward)
+    const int rs = LOWER ? j0 + jb : 0, re = LOWER ? n : j0;
+    const int row = rs + blockIdx.x * blockDim.x + tid;
+    if (row >= re) return;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < jb; ++k) {
+        const double l = (double)L[row + (size_t)(j0 + k) * ldl];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) acc[r] += l * X[r][k];
+    }
+    for (int r = 0; r < nrhs; ++r) {
+        T* b = B + row + (size_t)r * ldb;
+        *b = (T)((double)*b - acc[r]);
+    }
+}
+
+template <typename T, bool LOWER, bool UNIT>
+static void trsv_skinny(int n, int nrhs, const T* L, int ldl, T* B, int ldb) {
+    const int nb = (n + SV - 1) / SV;
+    ProfScope prof(PROF_TRSM, (double)n * n * nrhs, sizeof(T) * 0.5 * n * n);
+    for (int q = 0; q < nb; ++q) {
+        const int blk = LOWER ? q : nb - 1 - q;
+        const int j0 = blk * SV, jb = std::min(SV, n - j0);
+        const int rows = LOWER ? n - j0 - jb : j0;
+        const unsigned grid = std::max(1u, ceil_div(rows, 256));
+        hipLaunchKernelGGL((trsv_block<T, LOWER, UNIT>), dim3(grid), dim3(256), 0, ctx().s(), n, nrhs, L, ldl, B, ldb,
+                           j0);
+    }
+    check_launch("trsv_block");
+}
+
 // ------------------------------------------------------------------ recursive LU
 template <typename T>
 static void getrf_rec(int kind, T* A, int lda, int m, int w, int* ipiv, int base) {
@@ -1089,6 +1177,12 @@ template <typename T>
 void getrs(int kind, int trans, int n, int nrhs, const T* LU, int lda, const int* ipiv, T* B, int ldb) {
     if (n <= 0 || nrhs <= 0) return;
     if (!trans) {
+        laswp<T>(B, ldb, nrhs, ipiv, 0, 1, 0, n, +1);
+        if (nrhs <= 8) {
+            trsv_skinny<T, true, true>(n, nrhs, LU, lda, B, ldb);
+            trsv_skinny<T, false, false>(n, nrhs, LU, lda, B, ldb);
+            return;
+        }
         const int nb = (n + IB - 1) / IB;
         const size_t need = 2 * (size_t)nb * IB * IB * sizeof(T);
         if (g_getrs_inv.n < need) g_getrs_inv.alloc(need);
@@ -1096,7 +1190,6 @@ void getrs(int kind, int trans, int n, int nrhs, const T* LU, int lda, const int
         T* Uinv = Linv + (size_t)nb * IB * IB;
         hipLaunchKernelGGL(invert_diag_kernel<T>, dim3(nb), dim3(64), 0, ctx().s(), n, LU, lda, Linv, Uinv);
         check_launch("invert_diag");
-        laswp<T>(B, ldb, nrhs, ipiv, 0, 1, 0, n, +1);
         trsm_inv<T, true>(kind, n, nrhs, LU, lda, Linv, B, ldb);
         trsm_inv<T, false>(kind, n, nrhs, LU, lda, Uinv, B, ldb);
     } else {

@@ -73,6 +73,7 @@ struct ProfTotals {
     double flops = 0, bytes = 0, ms = 0;
 };
 bool g_prof = false;
+unsigned g_prof_mask = 0;
 std::vector<ProfRec> g_pending;
 std::vector<hipEvent_t> g_event_pool;
 ProfTotals g_tot[PROF_NCLASS];
@@ -107,7 +108,7 @@ void resolve_pending() {
 bool prof_enabled() { return g_prof; }
 
 ProfScope::ProfScope(int c, double f, double b) : cls(c), flops(f), bytes(b) {
-    if (!g_prof) return;
+    if (!g_prof || !(g_prof_mask & (1u << c))) return;
     e0 = take_event();
     e1 = take_event();
     SQ_HIP(hipEventRecord(e0, ctx().s()));
@@ -130,11 +131,12 @@ using namespace sq;
 
 extern "C" {
 
-int sqbem_profile(int enable) {
+int sqbem_profile(int mask) {
     return guarded([&] {
         resolve_pending();
         for (auto& t : g_tot) t = ProfTotals{};
-        g_prof = enable != 0;
+        g_prof = mask != 0;
+        g_prof_mask = mask == -1 ? 0xffffffffu : (unsigned)mask;
     });
 }
 

@@ -501,28 +501,28 @@ static void solve_fp64(sqbem_swimmer* s, int k, const double* u_host, bool towed
 static void full_residual(sqbem_swimmer* s, int k, const double* x, const double* b, double* r, bool towed) {
     const int ns = s->ns, nt = s->nt, N = nt + ns + 6;
     convert_matrix<double, double>(N, k, b, N, r, N);
-    DBuf<double> c(6);
-    for (int j = 0; j < k; ++j) {
-        const double* xt = x + (size_t)j * N;
-        const double* xs = xt + nt;
-        const double* xw = xs + ns;
-        double* rt = r + (size_t)j * N;
-        double* rs = rt + nt;
-        double* rc = rs + ns;
-        if (s->tube) {
-            gemv<double>(nt, nt, 1, -1.0, s->tube->T64.p, nt, xt, N, 1.0, rt, N);
-            gemv<double>(nt, ns, 1, -1.0, s->B.p, nt, xs, N, 1.0, rt, N);
-            if (!towed) gemv<double>(nt, 6, 1, -1.0, s->Ct.p, nt, xw, N, 1.0, rt, N);
-            gemv<double>(ns, nt, 1, -1.0, s->D.p, ns, xt, N, 1.0, rs, N);
-        }
-        gemv<double>(ns, ns, 1, -1.0, s->E.p, ns, xs, N, 1.0, rs, N);
-        if (!towed) {
-            gemv<double>(ns, 6, 1, -1.0, s->Cs.p, ns, xw, N, 1.0, rs, N);
-            hipLaunchKernelGGL(constraint_apply_kernel, dim3(1), dim3(256), 0, ctx().s(), s->Ns, 1, xs, (long)N,
-                               s->dm.cen_xyz.p, s->dm.area.p, s->sc32, s->xc[0], s->xc[1], s->xc[2], c.p);
-            check_launch("constraint_apply");
-            axpy(6, -1.0, c.p, rc);
-        }
+    const double* xt = x;
+    const double* xs = x + nt;
+    const double* xw = xs + ns;
+    double* rt = r;
+    double* rs = rt + nt;
+    double* rc = rs + ns;
+    // all right-hand sides per pass: the FP64 operator is read once per refinement step
+    if (s->tube) {
+        gemv<double>(nt, nt, k, -1.0, s->tube->T64.p, nt, xt, N, 1.0, rt, N);
+        gemv<double>(nt, ns, k, -1.0, s->B.p, nt, xs, N, 1.0, rt, N);
+        if (!towed) gemv<double>(nt, 6, k, -1.0, s->Ct.p, nt, xw, N, 1.0, rt, N);
+        gemv<double>(ns, nt, k, -1.0, s->D.p, ns, xt, N, 1.0, rs, N);
+    }
+    gemv<double>(ns, ns, k, -1.0, s->E.p, ns, xs, N, 1.0, rs, N);
+    if (!towed) {
+        gemv<double>(ns, 6, k, -1.0, s->Cs.p, ns, xw, N, 1.0, rs, N);
+        DBuf<double> c((size_t)6 * k);
+        hipLaunchKernelGGL(constraint_apply_kernel, dim3(k), dim3(256), 0, ctx().s(), s->Ns, k, xs, (long)N,
+                           s->dm.cen_xyz.p, s->dm.area.p, s->sc32, s->xc[0], s->xc[1], s->xc[2], c.p);
+        check_launch("constraint_apply");
+        for (int j = 0; j < k; ++j) axpy(6, -1.0, c.p + 6 * j, rc + (size_t)j * N);
+        sync();  // c is released on return
     }
 }
 

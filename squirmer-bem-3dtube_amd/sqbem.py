@@ -146,9 +146,18 @@ def device_info():
 PROF_CLASSES = {"dgemm": 0, "sgemm": 1, "gemm16": 2, "panel": 3, "trsm": 4, "assemble": 5, "gemv": 6, "laswp": 7}
 
 
-def profile(enable=True):
-    """Reset and enable (or disable) per-kernel HIP-event timing inside libsqbem."""
-    check(lib().sqbem_profile(int(enable)))
+def profile(enable=True, classes=None):
+    """Reset and enable (or disable) per-kernel HIP-event timing inside libsqbem, for all
+    kernel classes or for the named ones."""
+    if not enable:
+        mask = 0
+    elif classes is None:
+        mask = -1
+    else:
+        mask = 0
+        for c in classes:
+            mask |= 1 << PROF_CLASSES[c]
+    check(lib().sqbem_profile(mask))
 
 
 def profile_get(name):
