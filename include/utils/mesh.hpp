@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <array>
 #include <functional>
+#include <cmath>
 #include <map>
 #include <utility>
 #include <vector>
@@ -24,6 +25,13 @@
 namespace bem {
 
 enum Tag { WALL = 0, INLET = 1, OUTLET = 2, SPHERE = 10 };
+
+namespace detail {
+// cos and sin of one angle through glibc's sincos, which the reference build also calls for
+// each cos/sin pair of the polygon vertices (its compiler fuses the two calls); separate
+// calls can differ in the last bit
+inline void cos_sin(double x, double& c, double& s) { ::sincos(x, &s, &c); }
+}  // namespace detail
 
 struct Mesh {
     std::vector<V3> nodes;
@@ -121,7 +129,11 @@ inline Mesh tube_cap(double z0, int tag, double sg, int nth, int nr, double Rp, 
     m.nodes.push_back(V3{0, 0, z0});
     for (int k = 1; k <= nr; ++k) {
         const double rk = Rp * k / nr;
-        for (int i = 0; i < nth; ++i) m.nodes.push_back(V3{rk * std::cos(i * al), rk * std::sin(i * al), z0});
+        for (int i = 0; i < nth; ++i) {
+            double c, s;
+            cos_sin(i * al, c, s);
+            m.nodes.push_back(V3{rk * c, rk * s, z0});
+        }
     }
     auto node = [nth](int ring, int i) { return 1 + ring * nth + i % nth; };
     for (int i = 0; i < nth; ++i) m.tris.push_back({0, node(0, i), node(0, i + 1)});
@@ -145,9 +157,11 @@ inline Mesh tube_mesh(double R, const std::vector<double>& z, int nth, int nr) {
     const double Rp = R * std::sqrt(al / std::sin(al));
     const int nz = (int)z.size() - 1;
     Mesh wall;
-    for (int i = 0; i < nth; ++i)
-        for (int k = 0; k <= nz; ++k)
-            wall.nodes.push_back(V3{Rp * std::cos(i * al), Rp * std::sin(i * al), z[k]});
+    for (int i = 0; i < nth; ++i) {
+        double c, s;
+        detail::cos_sin(i * al, c, s);
+        for (int k = 0; k <= nz; ++k) wall.nodes.push_back(V3{Rp * c, Rp * s, z[k]});
+    }
     auto node = [nth, nz](int i, int k) { return (i % nth) * (nz + 1) + k; };
     for (int i = 0; i < nth; ++i)
         for (int k = 0; k < nz; ++k) {

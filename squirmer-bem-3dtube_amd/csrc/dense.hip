@@ -299,13 +299,21 @@ __global__ void __launch_bounds__(256) gemm16_nn_kernel(int m, int n, int k, flo
 }
 
 // Power-of-two scale per row of A (m x k) or per column of B (k x n) so the largest
-// magnitude maps to about 2^14 (FP16 max is 65504).
-__global__ void pow2_scale_rows(int m, int k, const float* __restrict__ A, int lda, float* __restrict__ s) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+// magnitude maps to about 2^14 (FP16 max is 65504). Row maxima: 64 x 64 tiles, one atomic
+// max per row and tile on the float bit pattern (non-negative floats order like integers).
+__global__ void row_absmax_kernel(int m, int k, const float* __restrict__ A, int lda, unsigned* __restrict__ mx) {
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int c0 = blockIdx.y * 64 + (threadIdx.x >> 6) * 16;
     if (i >= m) return;
-    float mx = 0.0f;
-    for (int c = 0; c < k; ++c) mx = fmaxf(mx, fabsf(A[i + (size_t)c * lda]));
-    s[i] = mx > 0.0f ? exp2f(14.0f - (float)ilogbf(mx)) : 1.0f;
+    float v = 0.0f;
+    for (int c = c0; c < min(k, c0 + 16); ++c) v = fmaxf(v, fabsf(A[i + (size_t)c * lda]));
+    atomicMax(mx + i, __float_as_uint(v));
+}
+__global__ void bits_to_pow2_scale(int m, const unsigned* __restrict__ mx, float* __restrict__ s) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const float v = __uint_as_float(mx[i]);
+    s[i] = v > 0.0f ? exp2f(14.0f - (float)ilogbf(v)) : 1.0f;
 }
 __global__ void pow2_scale_cols(int k, int n, const float* __restrict__ B, int ldb, float* __restrict__ s) {
     // one wave per column
@@ -316,6 +324,133 @@ __global__ void pow2_scale_cols(int k, int n, const float* __restrict__ B, int l
     for (int r = lane; r < k; r += 64) mx = fmaxf(mx, fabsf(B[r + (size_t)j * ldb]));
     for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
     if (lane == 0) s[j] = mx > 0.0f ? exp2f(14.0f - (float)ilogbf(mx)) : 1.0f;
+}
+
+// ------------------------------------------------------------------ 16-bit GEMM on converted operands
+// For large products the FP32 operands are converted once into padded 16-bit copies:
+// A as Ah[i * kp + k] (row i contiguous in k, rows padded to a multiple of 128) and B as
+// Bh[j * kp + k] (columns padded to a multiple of 256), k padded to a multiple of 64 with
+// zeros, FP16 values scaled per row / column by powers of two. The GEMM then streams
+// 16-byte chunks straight into LDS with no conversion and no bounds checks in its main
+// loop: workgroup tile 128 x 256, four waves of 64 x 128 (4 x 8 MFMA 16x16x32 tiles).
+constexpr int QBM = 128, QBN = 256, QBK = 64, QPAD = QBK + 8;
+
+template <bool BF16>
+__global__ void __launch_bounds__(256) gemm16q_kernel(int m, int n, int kp, float alpha,
+                                                      const typename Half<BF16>::t* __restrict__ Ah,
+                                                      const typename Half<BF16>::t* __restrict__ Bh, float beta,
+                                                      float* __restrict__ C, int ldc, int tiles_m,
+                                                      const float* __restrict__ rs, const float* __restrict__ cs) {
+    using H = Half<BF16>;
+    using ht = typename H::t;
+    using v8 = typename H::v8;
+    __shared__ __attribute__((aligned(16))) ht As[QBM][QPAD];
+    __shared__ __attribute__((aligned(16))) ht Bs[QBN][QPAD];
+
+    int tm, tn;
+    tile_of(blockIdx.x, gridDim.x, tiles_m, tm, tn);
+    const int m0 = tm * QBM, n0 = tn * QBN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 128;
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    u4 ra[4], rb[8];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = tid + 256 * q, row = c >> 3, kc = c & 7;
+            ra[q] = *(const u4*)(Ah + (size_t)(m0 + row) * kp + k0 + 8 * kc);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int c = tid + 256 * q, col = c >> 3, kc = c & 7;
+            rb[q] = *(const u4*)(Bh + (size_t)(n0 + col) * kp + k0 + 8 * kc);
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = tid + 256 * q, row = c >> 3, kc = c & 7;
+            *(u4*)&As[row][8 * kc] = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int c = tid + 256 * q, col = c >> 3, kc = c & 7;
+            *(u4*)&Bs[col][8 * kc] = rb[q];
+        }
+    };
+    f4v acc[4][8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = f4v{0, 0, 0, 0};
+    const int nk = kp / QBK;
+    const int lr = lane & 15, lk = lane >> 4;
+    load(0);
+    for (int kt = 0; kt < nk; ++kt) {
+        __syncthreads();  // the previous tile's reads are done
+        store();
+        __syncthreads();
+        if (kt + 1 < nk) load((kt + 1) * QBK);
+#pragma unroll
+        for (int s2 = 0; s2 < QBK / 32; ++s2) {
+            v8 af[4], bf[8];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) af[i] = *(const v8*)&As[wm + i * 16 + lr][32 * s2 + 8 * lk];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) bf[j] = *(const v8*)&Bs[wn + j * 16 + lr][32 * s2 + 8 * lk];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) acc[i][j] = H::mma(af[i], bf[j], acc[i][j]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int col = n0 + wn + j * 16 + lr;
+            if (col >= n) continue;
+            const float ucol = cs ? 1.0f / cs[col] : 1.0f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wm + i * 16 + 4 * lk + r;
+                if (row < m) {
+                    const float urow = rs ? 1.0f / rs[row] : 1.0f;
+                    float* c = C + row + (size_t)col * ldc;
+                    const float v = alpha * (acc[i][j][r] * urow * ucol);
+                    *c = beta == 0.0f ? v : v + beta * *c;
+                }
+            }
+        }
+}
+
+// Ah[i * kp + k] = A(i, k) * rs[i] (rows >= m and k >= kdim zero), via 64 x 64 LDS tiles
+template <bool BF16>
+__global__ void convert_rows16_kernel(int m, int k, const float* __restrict__ A, int lda, const float* __restrict__ rs,
+                                      typename Half<BF16>::t* __restrict__ Ah, int kp) {
+    __shared__ float tile[64][65];
+    const int i0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 256 threads: 64 x 4
+    for (int c = ty; c < 64; c += 4) {
+        const int i = i0 + tx, kk = k0 + c;
+        tile[c][tx] = (i < m && kk < k) ? A[i + (size_t)kk * lda] * (rs ? rs[i] : 1.0f) : 0.0f;
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        const int i = i0 + r, kk = k0 + tx;
+        Ah[(size_t)i * kp + kk] = (typename Half<BF16>::t)tile[tx][r];
+    }
+}
+// Bh[j * kp + k] = B(k, j) * cs[j] (columns >= n and k >= kdim zero)
+template <bool BF16>
+__global__ void convert_cols16_kernel(int k, int n, const float* __restrict__ B, int ldb, const float* __restrict__ cs,
+                                      typename Half<BF16>::t* __restrict__ Bh, int kp, int np) {
+    const size_t total = (size_t)np * kp;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const int kk = (int)(e % kp), j = (int)(e / kp);
+        const float v = (j < n && kk < k) ? B[kk + (size_t)j * ldb] * (cs ? cs[j] : 1.0f) : 0.0f;
+        Bh[e] = (typename Half<BF16>::t)v;
+    }
 }
 
 // ------------------------------------------------------------------ host GEMM wrappers
@@ -355,6 +490,31 @@ struct ScaleWork {
 };
 static ScaleWork g_scale;
 
+static DBuf<char> g_q16;
+
+template <bool BF16>
+static void gemm16q(int m, int n, int k, float alpha, const float* A, int lda, const float* B, int ldb, float beta,
+                    float* C, int ldc, float* rs, float* cs) {
+    using ht = typename Half<BF16>::t;
+    const int mp = (m + QBM - 1) / QBM * QBM, np = (n + QBN - 1) / QBN * QBN, kp = (k + QBK - 1) / QBK * QBK;
+    const size_t need = ((size_t)mp + np) * kp * sizeof(ht);
+    if (g_q16.n < need) g_q16.alloc(need);
+    ht* Ah = (ht*)g_q16.p;
+    ht* Bh = Ah + (size_t)mp * kp;
+    hipLaunchKernelGGL(convert_rows16_kernel<BF16>, dim3(mp / 64, kp / 64), dim3(256), 0, ctx().s(), m, k, A, lda, rs,
+                       Ah, kp);
+    const size_t tb = (size_t)np * kp;
+    hipLaunchKernelGGL(convert_cols16_kernel<BF16>, dim3((unsigned)std::min<size_t>(ceil_div(tb, 256), 1 << 16)),
+                       dim3(256), 0, ctx().s(), k, n, B, ldb, cs, Bh, kp, np);
+    check_launch("convert16");
+    const int tiles_m = mp / QBM, tiles_n = np / QBN;
+    ProfScope prof(PROF_GEMM16, 2.0 * m * n * k,
+                   2.0 * ((double)m * k + (double)k * n) + 4.0 * (beta != 0.0f ? 2.0 : 1.0) * m * n);
+    hipLaunchKernelGGL(gemm16q_kernel<BF16>, dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, kp, alpha, Ah, Bh,
+                       beta, C, ldc, tiles_m, rs, cs);
+    check_launch("gemm16q");
+}
+
 void gemm16(int kind, int m, int n, int k, float alpha, const float* A, int lda, const float* B, int ldb,
             float beta, float* C, int ldc) {
     if (m <= 0 || n <= 0) return;
@@ -363,12 +523,35 @@ void gemm16(int kind, int m, int n, int k, float alpha, const float* A, int lda,
         return;
     }
     const bool bf = kind == SQBEM_BF16_IR;
+    if (m >= 256 && n >= 256 && k >= 128) {
+        float *rs = nullptr, *cs = nullptr;
+        if (!bf) {
+            float* sp = g_scale.get((size_t)2 * m + n);
+            rs = sp;
+            cs = sp + m;
+            unsigned* mx = (unsigned*)g_scale.get((size_t)2 * m + n) + m + n;
+            fill_zero(mx, sizeof(unsigned) * m);
+            hipLaunchKernelGGL(row_absmax_kernel, dim3(ceil_div(m, 64), ceil_div(k, 64)), dim3(256), 0, ctx().s(), m,
+                               k, A, lda, mx);
+            hipLaunchKernelGGL(bits_to_pow2_scale, dim3(ceil_div(m, 256)), dim3(256), 0, ctx().s(), m, mx, rs);
+            hipLaunchKernelGGL(pow2_scale_cols, dim3(ceil_div(n, 4)), dim3(256), 0, ctx().s(), k, n, B, ldb, cs);
+        }
+        if (bf)
+            gemm16q<true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs);
+        else
+            gemm16q<false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs);
+        return;
+    }
     float *rs = nullptr, *cs = nullptr;
     if (!bf) {
-        float* s = g_scale.get((size_t)m + n);
-        rs = s;
-        cs = s + m;
-        hipLaunchKernelGGL(pow2_scale_rows, dim3(ceil_div(m, 256)), dim3(256), 0, ctx().s(), m, k, A, lda, rs);
+        float* sp = g_scale.get((size_t)2 * m + n);
+        rs = sp;
+        cs = sp + m;
+        unsigned* mx = (unsigned*)g_scale.get((size_t)2 * m + n) + m + n;
+        fill_zero(mx, sizeof(unsigned) * m);
+        hipLaunchKernelGGL(row_absmax_kernel, dim3(ceil_div(m, 64), ceil_div(k, 64)), dim3(256), 0, ctx().s(), m, k, A,
+                           lda, mx);
+        hipLaunchKernelGGL(bits_to_pow2_scale, dim3(ceil_div(m, 256)), dim3(256), 0, ctx().s(), m, mx, rs);
         hipLaunchKernelGGL(pow2_scale_cols, dim3(ceil_div(n, 4)), dim3(256), 0, ctx().s(), k, n, B, ldb, cs);
     }
     int tiles_m = (int)ceil_div(m, GBM), tiles_n = (int)ceil_div(n, GBN);
