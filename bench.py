@@ -25,7 +25,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAKS = {"dgemm": 78.6e12, "sgemm": 157.3e12, "gemm16": 2.5e15}
+PEAKS = {"dgemm": 78.6e12, "sgemm": 157.3e12, "gemm16": 2.5e15, "gemm16q": 2.5e15}
 MODES = {"fp64": 0, "fp32": 1, "fp16": 2, "bf16": 3}
 
 
@@ -137,12 +137,12 @@ def main():
     W = workload(args)
     e = np.array([0.0, 0.0, 1.0])
     prec = MODES[args.mode]
-    kname = {0: "dgemm", 1: "sgemm", 2: "gemm16", 3: "gemm16"}[prec]
+    kname = {0: "dgemm", 1: "sgemm", 2: "gemm16q", 3: "gemm16q"}[prec]
 
     for _ in range(args.warmup):
         cold_solve(pkg, W, prec, e)
     pkg.sync()
-    pkg.profile(True, classes=["dgemm", "sgemm", "gemm16"])
+    pkg.profile(True, classes=["dgemm", "sgemm", "gemm16", "gemm16q"])
     t0 = time.perf_counter()
     runs = []
     for _ in range(args.steps):

@@ -52,6 +52,7 @@ int sqbem_device_info(char* name, int name_len, int* cu_count, double* hbm_gb);
 #define SQBEM_PROF_ASSEMBLE 5
 #define SQBEM_PROF_GEMV 6
 #define SQBEM_PROF_LASWP 7
+#define SQBEM_PROF_GEMM16Q 8 /* 16-bit GEMM on pre-converted, padded operands (large shapes) */
 int sqbem_profile(int mask);
 int sqbem_profile_get(int kernel_class, long* launches, double* flops, double* bytes, double* ms);
 

@@ -96,6 +96,7 @@ inline void sync() { SQ_HIP(hipStreamSynchronize(ctx().s())); }
 // Kernel timing for roofline reports: when enabled, a ProfScope records HIP events around
 // one launch on the library stream together with its algorithmic flops and bytes.
 enum ProfClass { PROF_DGEMM = 0, PROF_SGEMM, PROF_GEMM16, PROF_PANEL, PROF_TRSM, PROF_ASSEMBLE, PROF_GEMV, PROF_LASWP,
+                 PROF_GEMM16Q,
                  PROF_NCLASS };
 bool prof_enabled();
 struct ProfScope {

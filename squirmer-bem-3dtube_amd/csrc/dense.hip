@@ -508,7 +508,7 @@ static void gemm16q(int m, int n, int k, float alpha, const float* A, int lda, c
                        dim3(256), 0, ctx().s(), k, n, B, ldb, cs, Bh, kp, np);
     check_launch("convert16");
     const int tiles_m = mp / QBM, tiles_n = np / QBN;
-    ProfScope prof(PROF_GEMM16, 2.0 * m * n * k,
+    ProfScope prof(PROF_GEMM16Q, 2.0 * m * n * k,
                    2.0 * ((double)m * k + (double)k * n) + 4.0 * (beta != 0.0f ? 2.0 : 1.0) * m * n);
     hipLaunchKernelGGL(gemm16q_kernel<BF16>, dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, kp, alpha, Ah, Bh,
                        beta, C, ldc, tiles_m, rs, cs);
@@ -643,6 +643,7 @@ struct PanelWork {
 };
 static PanelWork g_panel;
 static int g_bar_parity = 0;
+static bool g_coop_ok = true;  // cleared once the runtime refuses a cooperative launch
 
 // Inverses of one IB x IB diagonal block B (LDS, row-major; strict lower part = L with a unit
 // diagonal, upper part = U), padded to IB with the identity beyond w. Threads 0..IB-1 solve
@@ -992,8 +993,21 @@ static void panel_leaf(T* A, int lda, int m, int w, int* ipiv, int base) {
     void* args[] = {&A, &lda, &m, &w, &rpw, &ipiv, &base, &cand, &rowj, &bar, &bar_next, &status, &cflag, &cbackup,
                     &Linv};
     ProfScope prof(PROF_PANEL, 0.0, 0.0);
-    SQ_HIP(hipLaunchCooperativeKernel((const void*)getf2_coop<T>, dim3(G), dim3(PANEL_WG), args, (unsigned)shmem,
-                                      ctx().s()));
+    // The cooperative launch guarantees that all G <= CU-count workgroups are resident. The
+    // runtime refuses it when another process on the device holds the cooperative (global
+    // wave sync) queue; then an ordinary launch of the same grid is used: one workgroup
+    // per CU of an otherwise idle device is resident as well, and if one ever is not, the
+    // bounded barrier times out and getrf reports the error instead of waiting.
+    if (g_coop_ok) {
+        const hipError_t e = hipLaunchCooperativeKernel((const void*)getf2_coop<T>, dim3(G), dim3(PANEL_WG), args,
+                                                        (unsigned)shmem, ctx().s());
+        if (e == hipSuccess) return;
+        (void)hipGetLastError();  // launch refused (nothing was enqueued): clear it
+        g_coop_ok = false;
+    }
+    hipLaunchKernelGGL(getf2_coop<T>, dim3(G), dim3(PANEL_WG), shmem, ctx().s(), A, lda, m, w, rpw, ipiv, base, cand,
+                       rowj, bar, bar_next, status, cflag, cbackup, Linv);
+    check_launch("getf2_coop");
 }
 
 // ------------------------------------------------------------------ row interchanges

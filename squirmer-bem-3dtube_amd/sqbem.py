@@ -77,6 +77,8 @@ _SIGS = {
     "sqbem_duct_create": (_vp, [_vp, _d]),
     "sqbem_duct_solve": (_i, [_vp, _d, _d, _vp, _vp, _vp]),
     "sqbem_duct_free": (None, [_vp]),
+    "sqbem_trajectory_run": (_i, [_d, _d, _d, _d, _d, _d, _d, _d, _i, _i, _d, _d, _i, _i,
+                                  ctypes.c_char_p, _i, ctypes.c_char_p, _i]),
 }
 
 _lib = None
@@ -143,7 +145,7 @@ def device_info():
     return name.value.decode(), cu.value, gb.value
 
 
-PROF_CLASSES = {"dgemm": 0, "sgemm": 1, "gemm16": 2, "panel": 3, "trsm": 4, "assemble": 5, "gemv": 6, "laswp": 7}
+PROF_CLASSES = {"dgemm": 0, "sgemm": 1, "gemm16": 2, "panel": 3, "trsm": 4, "assemble": 5, "gemv": 6, "laswp": 7, "gemm16q": 8}
 
 
 def profile(enable=True, classes=None):
@@ -544,3 +546,33 @@ class DuctFlow:
 
     def exact_Q(self, dP):
         return PI * self.R ** 4 * dP / self.L / (8 * self.mu)
+
+
+@dataclass
+class TrajectoryParams:
+    """TrajectoryParams (src/trajectory.hpp:23-38); n_r and precision are extensions."""
+    alpha: float = 0.0
+    a_over_R: float = 0.3
+    beta0: float = 0.5
+    pitch0_deg: float = 0.0
+    yaw0_deg: float = 0.0
+    dt: float = 0.5
+    t_max: float = 100.0
+    L_over_R: float = 4.0
+    n_theta: int = 30
+    sphere_level: int = 2
+    beta_stop: float = 0.95
+    budget_s: float = 0.0
+    n_r: int = 4
+    precision: int = FP64
+
+
+def run_trajectory(P: TrajectoryParams, out_prefix: str, verbose: bool = False) -> str:
+    """run_trajectory (src/trajectory.hpp:40-151) on the device: writes <out_prefix>.csv
+    (and <out_prefix>.ckpt while paused); returns "finished", "wall contact" or "paused"."""
+    status = ctypes.create_string_buffer(64)
+    check(lib().sqbem_trajectory_run(P.alpha, P.a_over_R, P.beta0, P.pitch0_deg, P.yaw0_deg, P.dt,
+                                     P.t_max, P.L_over_R, P.n_theta, P.sphere_level, P.beta_stop,
+                                     P.budget_s, P.n_r, P.precision, out_prefix.encode(),
+                                     1 if verbose else 0, status, len(status)))
+    return status.value.decode()

@@ -37,7 +37,9 @@ CSV_HEAD = [
     "outputs/reference/trajectories/puller_a3.csv",
     "outputs/reference/trajectories/puller_a5.csv",
     "outputs/reference/traj3d/neutral_yaw15.csv",
+    "outputs/reference/traj3d/puller_a3_yaw15.csv",
 ]
+INPUTS = "inputs"  # the reference's key=value input files, copied verbatim
 HEAD_ROWS = 12
 
 
@@ -58,6 +60,14 @@ def copy_csvs():
                 data += 1
         with open(os.path.join(out, os.path.basename(rel)), "w") as f:
             f.writelines(keep)
+
+
+def copy_inputs():
+    out = os.path.join(HERE, "inputs")
+    os.makedirs(out, exist_ok=True)
+    for fn in sorted(os.listdir(os.path.join(REF, INPUTS))):
+        if fn.endswith(".in"):
+            shutil.copyfile(os.path.join(REF, INPUTS, fn), os.path.join(out, fn))
 
 
 def ref_lib():
@@ -164,5 +174,6 @@ if __name__ == "__main__":
     if not os.path.exists(REFLIB):
         sys.exit("build the reference build first: make -C oracle")
     copy_csvs()
+    copy_inputs()
     ref_vectors()
     print("golden fixtures written to", HERE)
