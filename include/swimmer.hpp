@@ -85,7 +85,8 @@ public:
     double mu;
     int Ns, Nt;
     double t_assembly = 0, t_schur = 0;
-    int ir_iters = 0;  // refinement steps of the last solve (refined precisions)
+    int ir_iters = 0;          // refinement steps of the last solve (refined precisions)
+    double ir_residual = 0.0;  // its final relative residual
 
     SwimmerSolver(const Tube* t, const Mesh& sphere, V3 center, double mu_ = 1.0, int precision = -1)
         : tube(t), S(sphere), xc(center), mu(mu_) {
@@ -160,10 +161,7 @@ public:
 private:
     std::shared_ptr<sqbem_swimmer> h_;
 
-    void times() {
-        double ir_res = 0;
-        sqbem_check(sqbem_swimmer_times(h_.get(), &t_assembly, &t_schur, &ir_iters, &ir_res));
-    }
+    void times() { sqbem_check(sqbem_swimmer_times(h_.get(), &t_assembly, &t_schur, &ir_iters, &ir_residual)); }
     SwimmerSolution unpack(const double* U, const double* Om, const std::vector<double>& fs,
                            const std::vector<double>& ft, int j) const {
         SwimmerSolution s;
