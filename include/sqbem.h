@@ -53,6 +53,7 @@ int sqbem_device_info(char* name, int name_len, int* cu_count, double* hbm_gb);
 #define SQBEM_PROF_GEMV 6
 #define SQBEM_PROF_LASWP 7
 #define SQBEM_PROF_GEMM16Q 8 /* 16-bit GEMM on pre-converted, padded operands (large shapes) */
+#define SQBEM_PROF_TRSV 9    /* triangular solves with at most 8 right-hand sides (one launch each) */
 int sqbem_profile(int mask);
 int sqbem_profile_get(int kernel_class, long* launches, double* flops, double* bytes, double* ms);
 

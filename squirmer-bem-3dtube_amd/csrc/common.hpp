@@ -91,12 +91,21 @@ struct DBuf {
     T* get() const { return p; }
 };
 
-inline void sync() { SQ_HIP(hipStreamSynchronize(ctx().s())); }
+// Sticky device error word (zero-initialised, device memory): kernels with bounded waits
+// set it when a wait times out. sync() checks it whenever such a kernel was launched since
+// the last check, so a timed-out wait surfaces as an error at the next synchronisation.
+int* device_error_word();
+void mark_device_error_check();
+void check_device_error();
+inline void sync() {
+    SQ_HIP(hipStreamSynchronize(ctx().s()));
+    check_device_error();
+}
 
 // Kernel timing for roofline reports: when enabled, a ProfScope records HIP events around
 // one launch on the library stream together with its algorithmic flops and bytes.
 enum ProfClass { PROF_DGEMM = 0, PROF_SGEMM, PROF_GEMM16, PROF_PANEL, PROF_TRSM, PROF_ASSEMBLE, PROF_GEMV, PROF_LASWP,
-                 PROF_GEMM16Q,
+                 PROF_GEMM16Q, PROF_TRSV,
                  PROF_NCLASS };
 bool prof_enabled();
 struct ProfScope {

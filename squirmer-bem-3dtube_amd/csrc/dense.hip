@@ -1225,91 +1225,161 @@ static void trsm_inv(int kind, int n, int nrhs, const T* L, int ldl, const T* in
 }
 
 // ------------------------------------------------------------------ skinny triangular solves
-// B (n x nrhs, nrhs <= 8) <- T^-1 B, one launch per SV-row diagonal block: every
-// workgroup solves the block's SV x SV triangle redundantly in its first wave (lane i owns
-// row i; the finished x_j is broadcast with a shuffle, so no barriers), workgroup 0 stores
-// it, and each workgroup subtracts its own rows' share L(rows, block) x_block. Reads the
-// factor once per solve: the memory-bound form of getrs for a few right-hand sides.
-constexpr int SV = 64;
-template <typename T, bool LOWER, bool UNIT>
-__global__ void __launch_bounds__(256) trsv_block(int n, int nrhs, const T* __restrict__ L, int ldl, T* B, int ldb,
-                                                  int j0) {
-    __shared__ double Ld[SV][SV + 1];
-    __shared__ double X[8][SV];
-    const int jb = min(SV, n - j0);
-    const int tid = threadIdx.x;
-    for (int e = tid; e < SV * SV; e += blockDim.x) {
-        const int i = e % SV, j = e / SV;
-        Ld[i][j] = (i < jb && j < jb) ? (double)L[(j0 + i) + (size_t)(j0 + j) * ldl] : (i == j ? 1.0 : 0.0);
+// B (n x nrhs, nrhs <= 8) <- T^-1 B in one launch, reading the triangle once. Workgroup k,
+// in the order workgroups start (an atomic ticket), owns the k-th TV-row block in solve
+// order. It streams its block row of the triangle tile by tile (the loads do not depend
+// on the solution, so they are issued before any wait), applies each solved block x_j as
+// soon as the completion counter shows it, then solves its own diagonal block (wave 0:
+// lane i owns row i, x_j broadcast by shuffle) and publishes x_k. Blocks complete in
+// ticket order, so one monotonic counter serves as every block's flag, and a workgroup
+// only waits on tickets taken before its own: the chain cannot deadlock whatever the
+// residency. Both counters grow by nb per launch and are never reset; a wait is bounded
+// and a timeout is reported through the device error word. Sums run in FP64.
+constexpr int TV = 64;
+struct TrsvState {
+    unsigned long long ticket, done;
+};
+static DBuf<TrsvState> g_trsv;
+static unsigned long long g_trsv_base = 0;
+
+template <typename T, bool LOWER, bool UNIT, int NR>
+__global__ void __launch_bounds__(256) trsv_chain(int n, int nrhs, const T* __restrict__ L, int ldl, T* B, int ldb,
+                                                  TrsvState* st, unsigned long long base, int* err) {
+    __shared__ double Ld[TV][TV + 1];
+    __shared__ double X[NR][TV];
+    __shared__ double part[4][NR][TV];
+    __shared__ int s_k;
+    __shared__ unsigned long long s_seen;
+    const int tid = threadIdx.x, lane = tid & 63, cg = tid >> 6;
+    const int nb = (n + TV - 1) / TV;
+    if (tid == 0)
+        s_k = (int)(__hip_atomic_fetch_add(&st->ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - base);
+    __syncthreads();
+    const int k = s_k;
+    const int blk = LOWER ? k : nb - 1 - k;
+    const int r0 = blk * TV, rb = min(TV, n - r0);
+    for (int e = tid; e < TV * TV; e += 256) {
+        const int i = e % TV, j = e / TV;
+        Ld[i][j] = (i < rb && j < rb) ? (double)L[(r0 + i) + (size_t)(r0 + j) * ldl] : (i == j ? 1.0 : 0.0);
     }
+    double acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = 0.0;
+    const bool rowok = lane < rb;
+    const T* Lrow = L + (r0 + (rowok ? lane : 0));
+    unsigned long long seen = base;
+    for (int q = 0; q < k; ++q) {
+        const int jb = LOWER ? q : nb - 1 - q;
+        const int c0 = jb * TV + cg * 16, cend = min(n, jb * TV + TV);
+        T l[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) l[c] = (rowok && c0 + c < cend) ? Lrow[(size_t)(c0 + c) * ldl] : T(0);
+        if (seen < base + q + 1) {
+            if (tid == 0) {
+                unsigned long long d;
+                unsigned spins = 0;
+                while ((d = __hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < base + q + 1) {
+                    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                    if (++spins > (1u << 22)) {
+                        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                s_seen = d;
+            }
+            __syncthreads();
+            seen = s_seen;
+        }
+        for (int e = tid; e < NR * TV; e += 256) {
+            const int r = e / TV, i = e % TV, row = jb * TV + i;
+            X[r][i] = (r < nrhs && row < n) ? (double)B[row + (size_t)r * ldb] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            const double lv = (double)l[c];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[r] += lv * X[r][cg * 16 + c];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) part[cg][r][lane] = acc[r];
     __syncthreads();
     if (tid < 64) {
-        const int i = tid;
-        double x[8];
+        const int i = lane;
+        double x[NR];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) x[r] = (r < nrhs && i < jb) ? (double)B[(j0 + i) + (size_t)r * ldb] : 0.0;
+        for (int r = 0; r < NR; ++r)
+            x[r] = (r < nrhs && rowok)
+                       ? (double)B[(r0 + i) + (size_t)r * ldb] - ((part[0][r][i] + part[1][r][i]) + (part[2][r][i] + part[3][r][i]))
+                       : 0.0;
         if (LOWER) {
-            for (int j = 0; j < SV; ++j) {
+            for (int j = 0; j < TV; ++j) {
                 if (!UNIT && i == j) {
 #pragma unroll
-                    for (int r = 0; r < 8; ++r) x[r] /= Ld[j][j];
+                    for (int r = 0; r < NR; ++r) x[r] /= Ld[j][j];
                 }
                 const double lij = Ld[i][j];
 #pragma unroll
-                for (int r = 0; r < 8; ++r) {
+                for (int r = 0; r < NR; ++r) {
                     const double xj = __shfl(x[r], j);
                     if (i > j) x[r] -= lij * xj;
                 }
             }
         } else {
-            for (int j = SV - 1; j >= 0; --j) {
+            for (int j = TV - 1; j >= 0; --j) {
                 if (!UNIT && i == j) {
 #pragma unroll
-                    for (int r = 0; r < 8; ++r) x[r] /= Ld[j][j];
+                    for (int r = 0; r < NR; ++r) x[r] /= Ld[j][j];
                 }
                 const double lij = Ld[i][j];
 #pragma unroll
-                for (int r = 0; r < 8; ++r) {
+                for (int r = 0; r < NR; ++r) {
                     const double xj = __shfl(x[r], j);
                     if (i < j) x[r] -= lij * xj;
                 }
             }
         }
-#pragma unroll
-        for (int r = 0; r < 8; ++r) X[r][i] = x[r];
-        if (blockIdx.x == 0 && i < jb)
-            for (int r = 0; r < nrhs; ++r) B[(j0 + i) + (size_t)r * ldb] = (T)x[r];
+        if (rowok)
+            for (int r = 0; r < nrhs; ++r) B[(r0 + i) + (size_t)r * ldb] = (T)x[r];
+        __threadfence();
     }
     __syncthreads();
-    // rows outside the block: below it (forward) or above it (backward)
-    const int rs = LOWER ? j0 + jb : 0, re = LOWER ? n : j0;
-    const int row = rs + blockIdx.x * blockDim.x + tid;
-    if (row >= re) return;
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int k = 0; k < jb; ++k) {
-        const double l = (double)L[row + (size_t)(j0 + k) * ldl];
-#pragma unroll
-        for (int r = 0; r < 8; ++r) acc[r] += l * X[r][k];
-    }
-    for (int r = 0; r < nrhs; ++r) {
-        T* b = B + row + (size_t)r * ldb;
-        *b = (T)((double)*b - acc[r]);
-    }
+    if (tid == 0)
+        __hip_atomic_fetch_max(&st->done, base + (unsigned long long)k + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <typename T, bool LOWER, bool UNIT>
 static void trsv_skinny(int n, int nrhs, const T* L, int ldl, T* B, int ldb) {
-    const int nb = (n + SV - 1) / SV;
-    ProfScope prof(PROF_TRSM, (double)n * n * nrhs, sizeof(T) * 0.5 * n * n);
-    for (int q = 0; q < nb; ++q) {
-        const int blk = LOWER ? q : nb - 1 - q;
-        const int j0 = blk * SV, jb = std::min(SV, n - j0);
-        const int rows = LOWER ? n - j0 - jb : j0;
-        const unsigned grid = std::max(1u, ceil_div(rows, 256));
-        hipLaunchKernelGGL((trsv_block<T, LOWER, UNIT>), dim3(grid), dim3(256), 0, ctx().s(), n, nrhs, L, ldl, B, ldb,
-                           j0);
+    SQ_REQUIRE(nrhs >= 1 && nrhs <= 8, SQBEM_ERR_ARG, "trsv: 1..8 right-hand sides");
+    const int nb = (n + TV - 1) / TV;
+    if (!g_trsv.p) {
+        g_trsv.alloc(1);
+        g_trsv.zero();
+        g_trsv_base = 0;
     }
-    check_launch("trsv_block");
+    int* err = device_error_word();
+    const unsigned long long base = g_trsv_base;
+    g_trsv_base += (unsigned long long)nb;
+    mark_device_error_check();
+    ProfScope prof(PROF_TRSV, (double)n * n * nrhs, sizeof(T) * 0.5 * n * n);
+    const dim3 grid(nb), blk(256);
+    if (nrhs == 1)
+        hipLaunchKernelGGL((trsv_chain<T, LOWER, UNIT, 1>), grid, blk, 0, ctx().s(), n, nrhs, L, ldl, B, ldb, g_trsv.p,
+                           base, err);
+    else if (nrhs == 2)
+        hipLaunchKernelGGL((trsv_chain<T, LOWER, UNIT, 2>), grid, blk, 0, ctx().s(), n, nrhs, L, ldl, B, ldb, g_trsv.p,
+                           base, err);
+    else if (nrhs <= 4)
+        hipLaunchKernelGGL((trsv_chain<T, LOWER, UNIT, 4>), grid, blk, 0, ctx().s(), n, nrhs, L, ldl, B, ldb, g_trsv.p,
+                           base, err);
+    else
+        hipLaunchKernelGGL((trsv_chain<T, LOWER, UNIT, 8>), grid, blk, 0, ctx().s(), n, nrhs, L, ldl, B, ldb, g_trsv.p,
+                           base, err);
+    check_launch("trsv_chain");
 }
 
 // ------------------------------------------------------------------ recursive LU

@@ -51,6 +51,31 @@ void ensure_init() {
 
 Context& ctx() { return g_ctx; }
 
+static int* g_dev_err = nullptr;
+static bool g_dev_err_check = false;
+
+int* device_error_word() {
+    if (!g_dev_err) {
+        SQ_HIP(hipMalloc((void**)&g_dev_err, 256));
+        fill_zero(g_dev_err, 256);
+    }
+    return g_dev_err;
+}
+
+void mark_device_error_check() { g_dev_err_check = true; }
+
+void check_device_error() {
+    if (!g_dev_err_check || !g_dev_err) return;
+    g_dev_err_check = false;
+    int v = 0;
+    SQ_HIP(hipMemcpy(&v, g_dev_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (v) {
+        fill_zero(g_dev_err, 256);
+        SQ_HIP(hipStreamSynchronize(ctx().s()));
+        throw Error(SQBEM_ERR_HIP, "triangular solve: a dependency wait timed out");
+    }
+}
+
 void fill_zero(void* p, size_t bytes) {
     if (!p || bytes == 0) return;
     // device allocations are 256-byte aligned; callers pass whole buffers or aligned views

@@ -145,7 +145,7 @@ def device_info():
     return name.value.decode(), cu.value, gb.value
 
 
-PROF_CLASSES = {"dgemm": 0, "sgemm": 1, "gemm16": 2, "panel": 3, "trsm": 4, "assemble": 5, "gemv": 6, "laswp": 7, "gemm16q": 8}
+PROF_CLASSES = {"dgemm": 0, "sgemm": 1, "gemm16": 2, "panel": 3, "trsm": 4, "assemble": 5, "gemv": 6, "laswp": 7, "gemm16q": 8, "trsv": 9}
 
 
 def profile(enable=True, classes=None):

@@ -127,6 +127,32 @@ def test_dgetrs_backward_error(sq, n, nrhs, trans):
     assert r.max() < 30 * n * 2.2e-16, r.max()
 
 
+@pytest.mark.parametrize("n", [63, 64, 65, 1000, 4099])
+@pytest.mark.parametrize("nrhs", [1, 2, 3, 5, 8])
+def test_getrs_few_rhs_chained_solve(sq, n, nrhs):
+    """The one-launch triangular solves (<= 8 right-hand sides): FP64 and FP32 factors,
+    a leading dimension larger than n, two solves in a row (the kernel's counters carry
+    over between launches), against the backward-error bound."""
+    rng = np.random.default_rng(7 * n + nrhs)
+    A = rng.standard_normal((n, n)) + n ** 0.5 * np.eye(n)
+    X = rng.standard_normal((n, nrhs))
+    Bm = A @ X
+    for dt, tol in [(np.float64, 30 * n * 2.2e-16), (np.float32, 30 * n * 1.2e-7)]:
+        dA = dev(A.astype(dt))
+        ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        assert (sq.dgetrf(dA, ipiv) if dt == np.float64 else sq.sgetrf(1, dA, ipiv)) == 0
+        for rep in range(2):
+            full = dev(np.vstack([Bm, np.full((5, nrhs), 7.0)]).astype(dt))  # ld = n + 5
+            dB = full[:n]
+            (sq.dgetrs if dt == np.float64 else sq.sgetrs)(dA, ipiv, dB)
+            sq.sync()
+            Xg = host(dB).astype(np.float64)
+            assert np.all(host(full)[n:] == 7.0), "rows beyond n were written"
+            r = np.linalg.norm(A @ Xg - Bm, axis=0) / (np.linalg.norm(A, 2) * np.linalg.norm(Xg, axis=0))
+            assert r.max() < tol, (dt, rep, r.max())
+
+
 def test_dgetrf_singular_reports_info(sq):
     n = 70
     rng = np.random.default_rng(3)
