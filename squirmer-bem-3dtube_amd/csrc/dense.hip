@@ -1203,25 +1203,234 @@ void gemm_exact<float>(int m, int n, int k, float alpha, const float* A, int lda
     sgemm(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
-template <typename T, bool LOWER>
+// Triangular solve through inverted BS x BS diagonal blocks (consecutive in inv, leading
+// dimension BS): recursive split, each leaf one in-place GEMM X = inv_kk B_k in the exact
+// (storage) precision, the off-diagonal updates in the precision of kind.
+template <typename T, bool LOWER, int BS = IB>
 static void trsm_inv(int kind, int n, int nrhs, const T* L, int ldl, const T* inv, T* B, int ldb) {
     if (n <= 0 || nrhs <= 0) return;
-    if (n <= IB) {
+    if (n <= BS) {
         ProfScope prof(PROF_TRSM, 2.0 * n * n * nrhs, sizeof(T) * (2.0 * n * nrhs + (double)n * n));
-        gemm_exact<T>(n, nrhs, n, T(1), inv, IB, B, ldb, T(0), B, ldb);
+        gemm_exact<T>(n, nrhs, n, T(1), inv, BS, B, ldb, T(0), B, ldb);
         return;
     }
-    const int n1 = split_point(n, IB), n2 = n - n1;
-    const T* inv2 = inv + (size_t)(n1 / IB) * IB * IB;
+    const int n1 = split_point(n, BS), n2 = n - n1;
+    const T* inv2 = inv + (size_t)(n1 / BS) * BS * BS;
     if (LOWER) {
-        trsm_inv<T, LOWER>(kind, n1, nrhs, L, ldl, inv, B, ldb);
+        trsm_inv<T, LOWER, BS>(kind, n1, nrhs, L, ldl, inv, B, ldb);
         gemm_kind<T>(kind, n2, nrhs, n1, T(-1), L + n1, ldl, B, ldb, T(1), B + n1, ldb);
-        trsm_inv<T, LOWER>(kind, n2, nrhs, L + n1 + (size_t)n1 * ldl, ldl, inv2, B + n1, ldb);
+        trsm_inv<T, LOWER, BS>(kind, n2, nrhs, L + n1 + (size_t)n1 * ldl, ldl, inv2, B + n1, ldb);
     } else {
-        trsm_inv<T, LOWER>(kind, n2, nrhs, L + n1 + (size_t)n1 * ldl, ldl, inv2, B + n1, ldb);
+        trsm_inv<T, LOWER, BS>(kind, n2, nrhs, L + n1 + (size_t)n1 * ldl, ldl, inv2, B + n1, ldb);
         gemm_kind<T>(kind, n1, nrhs, n2, T(-1), L + (size_t)n1 * ldl, ldl, B + n1, ldb, T(1), B, ldb);
-        trsm_inv<T, LOWER>(kind, n1, nrhs, L, ldl, inv, B, ldb);
+        trsm_inv<T, LOWER, BS>(kind, n1, nrhs, L, ldl, inv, B, ldb);
     }
+}
+
+// ------------------------------------------------------------------ no-pivot blocked LU
+// Matrices of this code's operators never need row interchanges (the multipliers of
+// partial pivoting stay below ~0.7), so getrf first factors without pivoting in
+// 128-column blocks and verifies |l| <= 1 for every multiplier, which is exactly the
+// condition under which partial pivoting keeps every diagonal pivot (LAPACK's first-
+// largest tie rule included). If any multiplier fails (or a pivot is zero or not finite),
+// the matrix is restored from its copy and factored by the pivoting path.
+//
+// lu128_factor_kernel factors one <= 128 x 128 diagonal block held in registers (1024
+// threads: thread t owns column t % 128, rows 16 (t / 128) .. +15), two barriers per
+// column, FP64 arithmetic; lu128_inverse_kernel then forms the inverses of its unit-lower
+// L and upper U (one workgroup each, one barrier per step) for the triangular solves that
+// follow. The padded part of a smaller block is the identity.
+constexpr int NB = 128;
+// the in-place GEMMs (L21 = A21 U11^-1, X = inv B) rely on one tile spanning the block
+static_assert(GBM >= NB && GBN >= NB, "GEMM tile narrower than the inverted blocks");
+
+// One sweep of the triangular inversion: lower forms L^-1 (unit lower, forward), otherwise
+// U^-1 (backward, diagonal dU). Column j of the factor is published by its owners, row j
+// of the inverse by its owners; one barrier per step with double-buffered broadcasts.
+// Inner loops are branch-free (selects), so the LDS reads of a step issue together;
+// register arrays are indexed with compile-time indices only, so they stay in VGPRs. The
+// direction is a run-time argument so that one register allocation serves both sweeps.
+template <typename T>
+__device__ __forceinline__ void lu128_inverse(bool lower, const double (&a)[16], int c, int g, double (*rowb)[NB],
+                                              double (*colb)[NB], const double* dU, T* Xout) {
+    const int i0 = g * 16;
+    double x[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) x[e] = (i0 + e == c) ? 1.0 : 0.0;
+    for (int s = 0; s < NB; ++s) {
+        const int j = lower ? s : NB - 1 - s, p = s & 1;
+        if (g == (j >> 4)) {
+            double v = 0.0;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) v = (e == (j & 15)) ? x[e] : v;
+            if (!lower) {
+                v = v / dU[j];
+#pragma unroll
+                for (int e = 0; e < 16; ++e) x[e] = (e == (j & 15)) ? v : x[e];
+            }
+            rowb[p][c] = v;
+        }
+        if (c == j) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) colb[p][i0 + e] = a[e];
+        }
+        __syncthreads();
+        // rows i with i > j (lower) or i < j (upper): i in [lo, hi)
+        const int lo = lower ? j + 1 : 0, hi = lower ? NB : j;
+        if (i0 + 15 >= lo && i0 < hi) {
+            const double r = rowb[p][c];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int i = i0 + e;
+                const double v = x[e] - colb[p][i] * r;
+                x[e] = (i >= lo && i < hi) ? v : x[e];
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) Xout[(i0 + e) + (size_t)c * NB] = (T)x[e];
+}
+
+// Factors the <= 128 x 128 diagonal block at A (order w) without pivoting, writes L\U back
+// and sets *flag when a pivot is zero or not finite or a multiplier exceeds 1 in modulus.
+template <typename T>
+__global__ void __launch_bounds__(1024) lu128_factor_kernel(T* A, int lda, int w, int* flag) {
+    __shared__ double rowb[2][NB], colb[2][NB];
+    __shared__ int bad;
+    const int t = threadIdx.x, c = t & (NB - 1), g = t >> 7, i0 = g * 16;
+    double a[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int i = i0 + e;
+        a[e] = (i < w && c < w) ? (double)A[i + (size_t)c * lda] : (i == c ? 1.0 : 0.0);
+    }
+    if (t == 0) bad = 0;
+    __syncthreads();
+    int mybad = 0;
+    for (int j = 0; j < NB; ++j) {
+        const int p = j & 1;
+        if (g == (j >> 4) && c >= j) {
+            double v = 0.0;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) v = (e == (j & 15)) ? a[e] : v;
+            rowb[p][c] = v;
+        }
+        __syncthreads();
+        if (c == j) {
+            const double piv = rowb[p][j];
+            if (g == 0 && !(piv != 0.0 && fabs(piv) <= 1.79e308)) mybad = 1;
+            const double r = 1.0 / piv;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const bool below = i0 + e > j;
+                const double l = a[e] * r;
+                colb[p][i0 + e] = l;  // rows <= j are never read back
+                if (below && !(fabs(l) <= 1.0)) mybad = 1;
+                a[e] = below ? l : a[e];
+            }
+        }
+        __syncthreads();
+        if (c > j && i0 + 15 > j) {
+            const double u = rowb[p][c];
+            double cl[16];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) cl[e] = colb[p][i0 + e];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const double v = a[e] - cl[e] * u;
+                a[e] = (i0 + e > j) ? v : a[e];
+            }
+        }
+    }
+    if (mybad) bad = 1;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int i = i0 + e;
+        if (i < w && c < w) A[i + (size_t)c * lda] = (T)a[e];
+    }
+    __syncthreads();
+    if (t == 0 && bad) atomicOr(flag, 1);
+}
+
+// Inverses of the factors of the diagonal blocks of an LU (order n): grid 2 x nblocks,
+// workgroup (b, 0) writes L_bb^-1 to Linv + b NB^2, workgroup (b, 1) U_bb^-1 to Uinv + b NB^2
+// (NB x NB, identity-padded). The two sweeps run on different CUs.
+template <typename T>
+__global__ void __launch_bounds__(1024) lu128_inverse_kernel(const T* A, int lda, int n, T* Linv, T* Uinv) {
+    __shared__ double rowb[2][NB], colb[2][NB];
+    __shared__ double dU[NB];
+    const int b = blockIdx.x >> 1, upper = blockIdx.x & 1;
+    A += (size_t)b * NB * ((size_t)lda + 1);
+    const int w = min(NB, n - b * NB);
+    const int t = threadIdx.x, c = t & (NB - 1), g = t >> 7, i0 = g * 16;
+    double a[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int i = i0 + e;
+        a[e] = (i < w && c < w) ? (double)A[i + (size_t)c * lda] : (i == c ? 1.0 : 0.0);
+        if (i == c) dU[c] = a[e];
+    }
+    __syncthreads();
+    lu128_inverse<T>(!upper, a, c, g, rowb, colb, dU, (upper ? Uinv : Linv) + (size_t)b * NB * NB);
+}
+
+// flag |= (some |x| > 1 or not finite) over an m x n block
+template <typename T>
+__global__ void __launch_bounds__(256) check_unit_bound_kernel(int m, int n, const T* __restrict__ X, int ldx, int* flag) {
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    const size_t total = (size_t)m * n;
+    int mine = 0;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(idx % m), j = (int)(idx / m);
+        if (!(fabs((double)X[i + (size_t)j * ldx]) <= 1.0)) mine = 1;
+    }
+    if (mine) bad = 1;
+    __syncthreads();
+    if (threadIdx.x == 0 && bad) atomicOr(flag, 1);
+}
+
+struct NoPivotWork {
+    DBuf<char> inv;     // Linv blocks, then Uinv blocks (NB x NB each, storage type)
+    DBuf<char> backup;  // copy of the matrix for the pivoting fallback
+    DBuf<int> flag;
+    void ensure(int n, size_t elem, bool need_backup) {
+        const size_t nb = (size_t)(n + NB - 1) / NB;
+        if (inv.n < 2 * nb * NB * NB * elem) inv.alloc(2 * nb * NB * NB * elem);
+        if (need_backup && backup.n < (size_t)n * n * elem) backup.alloc((size_t)n * n * elem);
+        if (!flag.p) flag.alloc(1);
+    }
+};
+static NoPivotWork g_np;
+
+template <typename T>
+static void getrf_np(int kind, T* A, int lda, int m, int w, int base, T* Linv, T* Uinv, int* flag) {
+    if (w <= NB) {
+        const size_t off = (size_t)(base / NB) * NB * NB;
+        {
+            ProfScope prof(PROF_PANEL, (2.0 / 3.0) * w * w * w, 2.0 * sizeof(T) * w * w);
+            hipLaunchKernelGGL(lu128_factor_kernel<T>, dim3(1), dim3(1024), 0, ctx().s(), A, lda, w, flag);
+            hipLaunchKernelGGL(lu128_inverse_kernel<T>, dim3(2), dim3(1024), 0, ctx().s(), A, lda, w, Linv + off,
+                               Uinv + off);
+            check_launch("lu128");
+        }
+        if (m > w) {
+            // L21 = A21 U11^-1, in place: one column tile, each row tile read before written
+            gemm_exact<T>(m - w, w, w, T(1), A + w, lda, Uinv + off, NB, T(0), A + w, lda);
+            const size_t tot = (size_t)(m - w) * w;
+            hipLaunchKernelGGL(check_unit_bound_kernel<T>, dim3((unsigned)std::min<size_t>(ceil_div(tot, 256 * 8), 1024)),
+                               dim3(256), 0, ctx().s(), m - w, w, A + w, lda, flag);
+            check_launch("check_unit_bound");
+        }
+        return;
+    }
+    const int w1 = split_point(w, NB), w2 = w - w1;
+    getrf_np<T>(kind, A, lda, m, w1, base, Linv, Uinv, flag);
+    T* A12 = A + (size_t)w1 * lda;
+    trsm_inv<T, true, NB>(kind, w1, w2, A, lda, Linv + (size_t)(base / NB) * NB * NB, A12, lda);
+    gemm_kind<T>(kind, m - w1, w2, w1, T(-1), A + w1, lda, A12, lda, T(1), A12 + w1, lda);
+    getrf_np<T>(kind, A12 + w1, lda, m - w1, w2, base + w1, Linv, Uinv, flag);
 }
 
 // ------------------------------------------------------------------ skinny triangular solves
@@ -1406,10 +1615,41 @@ __global__ void ipiv_to_one_based(int n, int* ipiv, int add) {
     if (i < n) ipiv[i] += add;
 }
 
+__global__ void iota_pivots_kernel(int n, int* ipiv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ipiv[i] = i + 1;
+}
+
+// orders at least this large try the no-pivot factorisation first
+constexpr int NP_MIN = 512;
+static int g_last_getrf_path = 0;  // 0 pivoting, 1 no-pivot verified, 2 no-pivot rejected
+
 template <typename T>
 int getrf(int kind, int n, T* A, int lda, int* ipiv) {
     if (n <= 0) return 0;
     SQ_REQUIRE(lda >= n, SQBEM_ERR_ARG, "getrf: lda < n");
+    g_last_getrf_path = 0;
+    if (n >= NP_MIN) {
+        g_np.ensure(n, sizeof(T), true);
+        T* backup = (T*)g_np.backup.p;
+        SQ_HIP(hipMemcpy2DAsync(backup, (size_t)n * sizeof(T), A, (size_t)lda * sizeof(T), (size_t)n * sizeof(T), n,
+                                hipMemcpyDeviceToDevice, ctx().s()));
+        g_np.flag.zero();
+        const size_t nb = (size_t)(n + NB - 1) / NB;
+        T* Linv = (T*)g_np.inv.p;
+        getrf_np<T>(kind, A, lda, n, n, 0, Linv, Linv + nb * NB * NB, g_np.flag.p);
+        int bad = 0;
+        g_np.flag.download(&bad, 1);
+        if (!bad) {
+            hipLaunchKernelGGL(iota_pivots_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx().s(), n, ipiv);
+            check_launch("iota_pivots");
+            g_last_getrf_path = 1;
+            return 0;
+        }
+        SQ_HIP(hipMemcpy2DAsync(A, (size_t)lda * sizeof(T), backup, (size_t)n * sizeof(T), (size_t)n * sizeof(T), n,
+                                hipMemcpyDeviceToDevice, ctx().s()));
+        g_last_getrf_path = 2;
+    }
     g_panel.ensure(256, n, sizeof(T));
     g_panel.bar.zero();
     g_panel.status.zero();
@@ -1450,15 +1690,17 @@ void getrs(int kind, int trans, int n, int nrhs, const T* LU, int lda, const int
             trsv_skinny<T, false, false>(n, nrhs, LU, lda, B, ldb);
             return;
         }
-        const int nb = (n + IB - 1) / IB;
-        const size_t need = 2 * (size_t)nb * IB * IB * sizeof(T);
+        // inverses of the 128 x 128 diagonal blocks of L and U (one workgroup per block),
+        // then recursive solves whose leaves are single in-place GEMMs
+        const int nb = (n + NB - 1) / NB;
+        const size_t need = 2 * (size_t)nb * NB * NB * sizeof(T);
         if (g_getrs_inv.n < need) g_getrs_inv.alloc(need);
         T* Linv = (T*)g_getrs_inv.p;
-        T* Uinv = Linv + (size_t)nb * IB * IB;
-        hipLaunchKernelGGL(invert_diag_kernel<T>, dim3(nb), dim3(64), 0, ctx().s(), n, LU, lda, Linv, Uinv);
-        check_launch("invert_diag");
-        trsm_inv<T, true>(kind, n, nrhs, LU, lda, Linv, B, ldb);
-        trsm_inv<T, false>(kind, n, nrhs, LU, lda, Uinv, B, ldb);
+        T* Uinv = Linv + (size_t)nb * NB * NB;
+        hipLaunchKernelGGL(lu128_inverse_kernel<T>, dim3(2 * nb), dim3(1024), 0, ctx().s(), LU, lda, n, Linv, Uinv);
+        check_launch("lu128 inverses");
+        trsm_inv<T, true, NB>(kind, n, nrhs, LU, lda, Linv, B, ldb);
+        trsm_inv<T, false, NB>(kind, n, nrhs, LU, lda, Uinv, B, ldb);
     } else {
         // A^T = U^T L^T P: solve U^T (lower, non-unit), L^T (upper, unit), then P^T
         trsm_rec<T, true, false, true>(kind, n, nrhs, LU, lda, B, ldb);

@@ -153,6 +153,37 @@ def test_getrs_few_rhs_chained_solve(sq, n, nrhs):
             assert r.max() < tol, (dt, rep, r.max())
 
 
+@pytest.mark.parametrize("n", [512, 700, 1500])
+@pytest.mark.parametrize("swap", [None, (600, 650)])
+def test_getrf_no_pivot_path_and_fallback(sq, n, swap):
+    """Column-dominant matrices need no row interchanges: the 128-block no-pivot path must
+    give LAPACK's factors and identity pivots. With two rows exchanged, partial pivoting
+    must move a row there: the no-pivot attempt is rejected and the pivoting path used."""
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + 2.0 * n ** 0.5 * np.eye(n)
+    A += np.diag(np.abs(A).sum(axis=0))  # strictly column dominant: GEPP keeps the diagonal
+    if swap is not None and swap[1] < n:
+        A[[swap[0], swap[1]]] = A[[swap[1], swap[0]]]
+    lu, piv = sla.lu_factor(A)
+    for dt in (np.float64, np.float32):
+        dA = dev(A.astype(dt))
+        ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        info = sq.dgetrf(dA, ipiv) if dt == np.float64 else sq.sgetrf(1, dA, ipiv)
+        assert info == 0
+        assert np.array_equal(host(ipiv) - 1, piv), "pivot rows differ from LAPACK"
+        tol = 1e-11 if dt == np.float64 else 2e-5
+        assert np.allclose(host(dA).astype(np.float64), lu, rtol=tol, atol=tol * np.abs(lu).max())
+        # and a solve with many right-hand sides through the 128-block inverses
+        X = rng.standard_normal((n, 20))
+        dB = dev((A @ X).astype(dt))
+        (sq.dgetrs if dt == np.float64 else sq.sgetrs)(dA, ipiv, dB)
+        sq.sync()
+        Xg = host(dB).astype(np.float64)
+        r = np.linalg.norm(A @ Xg - A @ X, axis=0) / (np.linalg.norm(A, 2) * np.linalg.norm(Xg, axis=0))
+        assert r.max() < 30 * n * (2.2e-16 if dt == np.float64 else 1.2e-7), (dt, r.max())
+
+
 def test_dgetrf_singular_reports_info(sq):
     n = 70
     rng = np.random.default_rng(3)
