@@ -1032,10 +1032,73 @@ __global__ void laswp_kernel(T* A, int lda, int ncols, const int* __restrict__ i
     }
 }
 
+// Long pivot ranges: one workgroup first compacts the interchanges that move a row into an
+// ordered list (ballot + prefix count per chunk), then the swap kernel walks only that list,
+// so identity pivots (the no-pivot factorisation) cost one short pass over ipiv.
+__global__ void __launch_bounds__(1024) piv_compact_kernel(const int* __restrict__ ipiv, int ibase, int vbase, int j0,
+                                                           int j1, int* list, int* count) {
+    __shared__ int wsum[16];
+    __shared__ int s_total;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int total = 0;
+    for (int b = j0; b < j1; b += 1024) {
+        const int j = b + tid;
+        const int p = j < j1 ? ipiv[ibase + j] - vbase : j;
+        const bool mv = j < j1 && p != j;
+        const unsigned long long m = __ballot(mv);
+        const int before = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wv] = __popcll(m);
+        __syncthreads();
+        int off = total;
+        for (int q = 0; q < wv; ++q) off += wsum[q];
+        if (mv) {
+            list[2 * (off + before)] = j;
+            list[2 * (off + before) + 1] = p;
+        }
+        if (tid == 0) {
+            int t = 0;
+            for (int q = 0; q < 16; ++q) t += wsum[q];
+            s_total = total + t;
+        }
+        __syncthreads();
+        total = s_total;
+        __syncthreads();
+    }
+    if (tid == 0) *count = total;
+}
+
+template <typename T>
+__global__ void laswp_list_kernel(T* A, int lda, int ncols, const int* __restrict__ list, const int* __restrict__ count,
+                                  int dir) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nsw = *count;
+    if (c >= ncols || nsw == 0) return;
+    T* col = A + (size_t)c * lda;
+    for (int q = 0; q < nsw; ++q) {
+        const int qq = dir > 0 ? q : nsw - 1 - q;
+        const int j = list[2 * qq], p = list[2 * qq + 1];
+        const T t = col[j];
+        col[j] = col[p];
+        col[p] = t;
+    }
+}
+
+static DBuf<int> g_swaps;  // [0] count, then (row, pivot row) pairs
+
 template <typename T>
 static void laswp(T* A, int lda, int ncols, const int* ipiv, int ibase, int vbase, int j0, int j1, int dir = 1) {
     if (ncols <= 0 || j1 <= j0) return;
     ProfScope prof(PROF_LASWP, 0.0, 0.0);
+    if (j1 - j0 > 256) {
+        const size_t need = 1 + 2 * (size_t)(j1 - j0);
+        if (g_swaps.n < need) g_swaps.alloc(need);
+        hipLaunchKernelGGL(piv_compact_kernel, dim3(1), dim3(1024), 0, ctx().s(), ipiv, ibase, vbase, j0, j1,
+                           g_swaps.p + 1, g_swaps.p);
+        hipLaunchKernelGGL(laswp_list_kernel<T>, dim3(ceil_div(ncols, 256)), dim3(256), 0, ctx().s(), A, lda, ncols,
+                           g_swaps.p + 1, g_swaps.p, dir);
+        check_launch("laswp");
+        return;
+    }
     hipLaunchKernelGGL(laswp_kernel<T>, dim3(ceil_div(ncols, 256)), dim3(256), 0, ctx().s(), A, lda, ncols, ipiv,
                        ibase, vbase, j0, j1, dir);
     check_launch("laswp");
@@ -1244,6 +1307,17 @@ constexpr int NB = 128;
 // the in-place GEMMs (L21 = A21 U11^-1, X = inv B) rely on one tile spanning the block
 static_assert(GBM >= NB && GBN >= NB, "GEMM tile narrower than the inverted blocks");
 
+// Wave-uniform broadcast of 16 doubles held in LDS at v[0..15]: lane l loads v[l & 15]
+// once, then each value is read out of its lane into scalar registers (v_readlane), so a
+// wave moves 16 doubles through LDS instead of 16 x 64.
+__device__ __forceinline__ void bcast16(const double* v, double (&out)[16]) {
+    const double mine = v[__lane_id() & 15];
+    const int lo = __double2loint(mine), hi = __double2hiint(mine);
+#pragma unroll
+    for (int e = 0; e < 16; ++e)
+        out[e] = __hiloint2double(__builtin_amdgcn_readlane(hi, e), __builtin_amdgcn_readlane(lo, e));
+}
+
 // One sweep of the triangular inversion: lower forms L^-1 (unit lower, forward), otherwise
 // U^-1 (backward, diagonal dU). Column j of the factor is published by its owners, row j
 // of the inverse by its owners; one barrier per step with double-buffered broadcasts.
@@ -1279,10 +1353,12 @@ __device__ __forceinline__ void lu128_inverse(bool lower, const double (&a)[16],
         const int lo = lower ? j + 1 : 0, hi = lower ? NB : j;
         if (i0 + 15 >= lo && i0 < hi) {
             const double r = rowb[p][c];
+            double cb[16];
+            bcast16(&colb[p][i0], cb);
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
                 const int i = i0 + e;
-                const double v = x[e] - colb[p][i] * r;
+                const double v = x[e] - cb[e] * r;
                 x[e] = (i >= lo && i < hi) ? v : x[e];
             }
         }
@@ -1330,15 +1406,17 @@ __global__ void __launch_bounds__(1024) lu128_factor_kernel(T* A, int lda, int w
             }
         }
         __syncthreads();
-        if (c > j && i0 + 15 > j) {
-            const double u = rowb[p][c];
+        // wave-uniform test (c | 63 is the wave's last column): the broadcast needs all lanes
+        if (i0 + 15 > j && (c | 63) > j) {
             double cl[16];
+            bcast16(&colb[p][i0], cl);
+            if (c > j) {
+                const double u = rowb[p][c];
 #pragma unroll
-            for (int e = 0; e < 16; ++e) cl[e] = colb[p][i0 + e];
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const double v = a[e] - cl[e] * u;
-                a[e] = (i0 + e > j) ? v : a[e];
+                for (int e = 0; e < 16; ++e) {
+                    const double v = a[e] - cl[e] * u;
+                    a[e] = (i0 + e > j) ? v : a[e];
+                }
             }
         }
     }
@@ -1454,9 +1532,14 @@ static unsigned long long g_trsv_base = 0;
 template <typename T, bool LOWER, bool UNIT, int NR>
 __global__ void __launch_bounds__(256) trsv_chain(int n, int nrhs, const T* __restrict__ L, int ldl, T* B, int ldb,
                                                   TrsvState* st, unsigned long long base, int* err) {
-    __shared__ double Ld[TV][TV + 1];
-    __shared__ double X[NR][TV];
-    __shared__ double part[4][NR][TV];
+    // tiles per group: their triangle values are loaded together, and all the ready ones
+    // among them are applied with one solution load and two barriers
+    constexpr int GQ = sizeof(T) == 4 ? 8 : 4;
+    constexpr int XB = GQ * NR * TV * (int)sizeof(T), PB = 4 * NR * TV * (int)sizeof(double);
+    __shared__ T Ld[TV][TV + 1];
+    __shared__ __align__(16) unsigned char smem[XB > PB ? XB : PB];
+    T(*Xs)[NR][TV] = reinterpret_cast<T(*)[NR][TV]>(smem);            // [GQ][NR][TV]
+    double(*part)[NR][TV] = reinterpret_cast<double(*)[NR][TV]>(smem);  // [4][NR][TV], after the loop
     __shared__ int s_k;
     __shared__ unsigned long long s_seen;
     const int tid = threadIdx.x, lane = tid & 63, cg = tid >> 6;
@@ -1469,7 +1552,7 @@ __global__ void __launch_bounds__(256) trsv_chain(int n, int nrhs, const T* __re
     const int r0 = blk * TV, rb = min(TV, n - r0);
     for (int e = tid; e < TV * TV; e += 256) {
         const int i = e % TV, j = e / TV;
-        Ld[i][j] = (i < rb && j < rb) ? (double)L[(r0 + i) + (size_t)(r0 + j) * ldl] : (i == j ? 1.0 : 0.0);
+        Ld[i][j] = (i < rb && j < rb) ? L[(r0 + i) + (size_t)(r0 + j) * ldl] : (i == j ? T(1) : T(0));
     }
     double acc[NR];
 #pragma unroll
@@ -1477,42 +1560,61 @@ __global__ void __launch_bounds__(256) trsv_chain(int n, int nrhs, const T* __re
     const bool rowok = lane < rb;
     const T* Lrow = L + (r0 + (rowok ? lane : 0));
     unsigned long long seen = base;
-    for (int q = 0; q < k; ++q) {
-        const int jb = LOWER ? q : nb - 1 - q;
-        const int c0 = jb * TV + cg * 16, cend = min(n, jb * TV + TV);
-        T l[16];
+    for (int q0 = 0; q0 < k; q0 += GQ) {
+        const int nq = min(GQ, k - q0);
+        T l[GQ][16];
 #pragma unroll
-        for (int c = 0; c < 16; ++c) l[c] = (rowok && c0 + c < cend) ? Lrow[(size_t)(c0 + c) * ldl] : T(0);
-        if (seen < base + q + 1) {
-            if (tid == 0) {
-                unsigned long long d;
-                unsigned spins = 0;
-                while ((d = __hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < base + q + 1) {
-                    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-                    if (++spins > (1u << 22)) {
-                        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        break;
+        for (int g = 0; g < GQ; ++g) {
+            const int jb = LOWER ? q0 + g : nb - 1 - (q0 + g);
+            const int c0 = jb * TV + cg * 16, cend = min(n, jb * TV + TV);
+#pragma unroll
+            for (int c = 0; c < 16; ++c) l[g][c] = (g < nq && rowok && c0 + c < cend) ? Lrow[(size_t)(c0 + c) * ldl] : T(0);
+        }
+        int qd = q0;  // next tile of the group to apply
+        while (qd < q0 + nq) {
+            if (seen < base + qd + 1) {
+                if (tid == 0) {
+                    unsigned long long d;
+                    unsigned spins = 0;
+                    while ((d = __hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) <
+                           base + qd + 1) {
+                        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                        if (++spins > (1u << 22)) {
+                            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
                     }
-                    __builtin_amdgcn_s_sleep(1);
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    // after a timeout the loop still ends: treat the rest as ready (the
+                    // result is wrong and the error word reports it)
+                    s_seen = d >= base + qd + 1 ? d : base + (unsigned long long)k;
                 }
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                s_seen = d;
+                __syncthreads();
+                seen = s_seen;
+            }
+            const int upto = min(q0 + nq, (int)(seen - base));  // tiles [qd, upto) are ready
+            for (int e = tid; e < (upto - qd) * NR * TV; e += 256) {
+                const int g = e / (NR * TV), r = (e / TV) % NR, i = e % TV;
+                const int jb = LOWER ? qd + g : nb - 1 - (qd + g);
+                const int row = jb * TV + i;
+                Xs[qd - q0 + g][r][i] = (r < nrhs && row < n) ? B[row + (size_t)r * ldb] : T(0);
             }
             __syncthreads();
-            seen = s_seen;
-        }
-        for (int e = tid; e < NR * TV; e += 256) {
-            const int r = e / TV, i = e % TV, row = jb * TV + i;
-            X[r][i] = (r < nrhs && row < n) ? (double)B[row + (size_t)r * ldb] : 0.0;
-        }
-        __syncthreads();
 #pragma unroll
-        for (int c = 0; c < 16; ++c) {
-            const double lv = (double)l[c];
+            for (int g = 0; g < GQ; ++g) {
+                if (g >= qd - q0 && g < upto - q0) {
 #pragma unroll
-            for (int r = 0; r < NR; ++r) acc[r] += lv * X[r][cg * 16 + c];
+                    for (int c = 0; c < 16; ++c) {
+                        const double lv = (double)l[g][c];
+#pragma unroll
+                        for (int r = 0; r < NR; ++r) acc[r] += lv * (double)Xs[g][r][cg * 16 + c];
+                    }
+                }
+            }
+            __syncthreads();
+            qd = upto;
         }
-        __syncthreads();
     }
 #pragma unroll
     for (int r = 0; r < NR; ++r) part[cg][r][lane] = acc[r];
@@ -1529,9 +1631,9 @@ __global__ void __launch_bounds__(256) trsv_chain(int n, int nrhs, const T* __re
             for (int j = 0; j < TV; ++j) {
                 if (!UNIT && i == j) {
 #pragma unroll
-                    for (int r = 0; r < NR; ++r) x[r] /= Ld[j][j];
+                    for (int r = 0; r < NR; ++r) x[r] /= (double)Ld[j][j];
                 }
-                const double lij = Ld[i][j];
+                const double lij = (double)Ld[i][j];
 #pragma unroll
                 for (int r = 0; r < NR; ++r) {
                     const double xj = __shfl(x[r], j);
@@ -1542,9 +1644,9 @@ __global__ void __launch_bounds__(256) trsv_chain(int n, int nrhs, const T* __re
             for (int j = TV - 1; j >= 0; --j) {
                 if (!UNIT && i == j) {
 #pragma unroll
-                    for (int r = 0; r < NR; ++r) x[r] /= Ld[j][j];
+                    for (int r = 0; r < NR; ++r) x[r] /= (double)Ld[j][j];
                 }
-                const double lij = Ld[i][j];
+                const double lij = (double)Ld[i][j];
 #pragma unroll
                 for (int r = 0; r < NR; ++r) {
                     const double xj = __shfl(x[r], j);
