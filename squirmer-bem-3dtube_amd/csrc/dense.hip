@@ -16,6 +16,7 @@
 //  * TRSM: recursive split with 64-row leaf kernels (one thread per right-hand side,
 //    the triangle in LDS), GEMM updates in between.
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <vector>
 
@@ -173,12 +174,16 @@ constexpr int HBK = 64, HPAD = HBK + 8;
 
 template <bool BF16>
 struct Half;
+typedef float f16x __attribute__((ext_vector_type(16)));
 template <>
 struct Half<false> {
     using t = _Float16;
     using v8 = h8v;
     static __device__ __forceinline__ f4v mma(v8 a, v8 b, f4v c) {
         return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ f16x mma32(v8 a, v8 b, f16x c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
     }
 };
 template <>
@@ -187,6 +192,9 @@ struct Half<true> {
     using v8 = b8v;
     static __device__ __forceinline__ f4v mma(v8 a, v8 b, f4v c) {
         return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ f16x mma32(v8 a, v8 b, f16x c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
     }
 };
 
@@ -328,14 +336,22 @@ __global__ void pow2_scale_cols(int k, int n, const float* __restrict__ B, int l
 
 // ------------------------------------------------------------------ 16-bit GEMM on converted operands
 // For large products the FP32 operands are converted once into padded 16-bit copies:
-// A as Ah[i * kp + k] (row i contiguous in k, rows padded to a multiple of 128) and B as
-// Bh[j * kp + k] (columns padded to a multiple of 256), k padded to a multiple of 64 with
-// zeros, FP16 values scaled per row / column by powers of two. The GEMM then streams
-// 16-byte chunks straight into LDS with no conversion and no bounds checks in its main
-// loop: workgroup tile 128 x 256, four waves of 64 x 128 (4 x 8 MFMA 16x16x32 tiles).
-constexpr int QBM = 128, QBN = 256, QBK = 64, QPAD = QBK + 8;
+// A as Ah[i * kp + k] (row i contiguous in k) and B as Bh[j * kp + k] (column j contiguous
+// in k), rows and columns padded to multiples of 256 and k to a multiple of 64 with zeros,
+// FP16 values scaled per row / column by powers of two. The GEMM streams 16-byte chunks
+// into LDS with no conversion and no bounds checks in its main loop.
+//   workgroup tile 256 x 256, four waves of 128 x 128, MFMA 32x32x16 (4 x 4 tiles a wave),
+//   k tile 64, two LDS stages (one barrier per k tile: the next tile's chunks are written
+//   into the other stage while this one is read), 144 KiB of LDS, one workgroup per CU.
+// MFMA 32x32x16 layout (measured on gfx950): lane l supplies A[l%32][8(l/32)+0..7] and
+// B[8(l/32)+0..7][l%32]; accumulator value r of lane l is D[8(r/4) + 4(l/32) + r%4][l%32].
+constexpr int QBK = 64, QPAD = QBK + 8;
+template <int TM, int TN>
+constexpr size_t q_lds_bytes() { return 2 * (size_t)(TM + TN) * QPAD * 2; }
 
-template <bool BF16>
+// Workgroup tile TM x TN (256 x 256 for large products, 128 x 128 for mid-sized ones), four
+// waves of TM/2 x TN/2, each a grid of 32 x 32 MFMA tiles.
+template <bool BF16, int TM, int TN>
 __global__ void __launch_bounds__(256) gemm16q_kernel(int m, int n, int kp, float alpha,
                                                       const typename Half<BF16>::t* __restrict__ Ah,
                                                       const typename Half<BF16>::t* __restrict__ Bh, float beta,
@@ -344,76 +360,90 @@ __global__ void __launch_bounds__(256) gemm16q_kernel(int m, int n, int kp, floa
     using H = Half<BF16>;
     using ht = typename H::t;
     using v8 = typename H::v8;
-    __shared__ __attribute__((aligned(16))) ht As[QBM][QPAD];
-    __shared__ __attribute__((aligned(16))) ht Bs[QBN][QPAD];
+    constexpr int STAGE = (TM + TN) * QPAD;          // 16-bit elements per LDS stage
+    constexpr int IM = TM / 64, JN = TN / 64;        // 32 x 32 tiles per wave
+    constexpr int CA = TM * (QBK / 8) / 256, CB = TN * (QBK / 8) / 256;  // 16-byte chunks a thread
+    extern __shared__ __attribute__((aligned(16))) unsigned char q_smem[];
+    ht* lds = reinterpret_cast<ht*>(q_smem);
 
     int tm, tn;
     tile_of(blockIdx.x, gridDim.x, tiles_m, tm, tn);
-    const int m0 = tm * QBM, n0 = tn * QBN;
+    const int m0 = tm * TM, n0 = tn * TN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 128;
+    const int wm = (wave & 1) * (TM / 2), wn = (wave >> 1) * (TN / 2);
     typedef unsigned int u4 __attribute__((ext_vector_type(4)));
-    u4 ra[4], rb[8];
+    // chunk c = tid + 256 q of a panel: row c >> 3, 8-element k chunk c & 7
+    u4 ra[CA], rb[CB];
     auto load = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < CA; ++q) {
             const int c = tid + 256 * q, row = c >> 3, kc = c & 7;
             ra[q] = *(const u4*)(Ah + (size_t)(m0 + row) * kp + k0 + 8 * kc);
         }
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int c = tid + 256 * q, col = c >> 3, kc = c & 7;
-            rb[q] = *(const u4*)(Bh + (size_t)(n0 + col) * kp + k0 + 8 * kc);
-        }
-    };
-    auto store = [&]() {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < CB; ++q) {
             const int c = tid + 256 * q, row = c >> 3, kc = c & 7;
-            *(u4*)&As[row][8 * kc] = ra[q];
-        }
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int c = tid + 256 * q, col = c >> 3, kc = c & 7;
-            *(u4*)&Bs[col][8 * kc] = rb[q];
+            rb[q] = *(const u4*)(Bh + (size_t)(n0 + row) * kp + k0 + 8 * kc);
         }
     };
-    f4v acc[4][8];
+    auto store = [&](int stage) {
+        ht* As = lds + stage * STAGE;
+        ht* Bs = As + TM * QPAD;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 8; ++j) acc[i][j] = f4v{0, 0, 0, 0};
-    const int nk = kp / QBK;
-    const int lr = lane & 15, lk = lane >> 4;
-    load(0);
-    for (int kt = 0; kt < nk; ++kt) {
-        __syncthreads();  // the previous tile's reads are done
-        store();
-        __syncthreads();
-        if (kt + 1 < nk) load((kt + 1) * QBK);
-#pragma unroll
-        for (int s2 = 0; s2 < QBK / 32; ++s2) {
-            v8 af[4], bf[8];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) af[i] = *(const v8*)&As[wm + i * 16 + lr][32 * s2 + 8 * lk];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) bf[j] = *(const v8*)&Bs[wn + j * 16 + lr][32 * s2 + 8 * lk];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 8; ++j) acc[i][j] = H::mma(af[i], bf[j], acc[i][j]);
+        for (int q = 0; q < CA; ++q) {
+            const int c = tid + 256 * q, row = c >> 3, kc = c & 7;
+            *(u4*)&As[row * QPAD + 8 * kc] = ra[q];
         }
+#pragma unroll
+        for (int q = 0; q < CB; ++q) {
+            const int c = tid + 256 * q, row = c >> 3, kc = c & 7;
+            *(u4*)&Bs[row * QPAD + 8 * kc] = rb[q];
+        }
+    };
+    f16x acc[IM][JN];
+#pragma unroll
+    for (int i = 0; i < IM; ++i)
+#pragma unroll
+        for (int j = 0; j < JN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+    const int nk = kp / QBK;
+    const int lr = lane & 31, lk = lane >> 5;
+    load(0);
+    store(0);
+    if (nk > 1) load(QBK);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const ht* As = lds + (kt & 1) * STAGE;
+        const ht* Bs = As + TM * QPAD;
+#pragma unroll
+        for (int s2 = 0; s2 < QBK / 16; ++s2) {
+            v8 af[IM], bf[JN];
+#pragma unroll
+            for (int i = 0; i < IM; ++i) af[i] = *(const v8*)&As[(wm + i * 32 + lr) * QPAD + 16 * s2 + 8 * lk];
+#pragma unroll
+            for (int j = 0; j < JN; ++j) bf[j] = *(const v8*)&Bs[(wn + j * 32 + lr) * QPAD + 16 * s2 + 8 * lk];
+#pragma unroll
+            for (int i = 0; i < IM; ++i)
+#pragma unroll
+                for (int j = 0; j < JN; ++j) acc[i][j] = H::mma32(af[i], bf[j], acc[i][j]);
+        }
+        if (kt + 1 < nk) {
+            store((kt + 1) & 1);  // the other stage: last read before the previous barrier
+            if (kt + 2 < nk) load((kt + 2) * QBK);
+        }
+        __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < IM; ++i)
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int col = n0 + wn + j * 16 + lr;
+        for (int j = 0; j < JN; ++j) {
+            const int col = n0 + wn + j * 32 + lr;
             if (col >= n) continue;
             const float ucol = cs ? 1.0f / cs[col] : 1.0f;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wm + i * 16 + 4 * lk + r;
+            for (int r = 0; r < 16; ++r) {
+                const int row = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * lk + (r & 3);
                 if (row < m) {
                     const float urow = rs ? 1.0f / rs[row] : 1.0f;
                     float* c = C + row + (size_t)col * ldc;
@@ -492,11 +522,11 @@ static ScaleWork g_scale;
 
 static DBuf<char> g_q16;
 
-template <bool BF16>
+template <bool BF16, int TM, int TN>
 static void gemm16q(int m, int n, int k, float alpha, const float* A, int lda, const float* B, int ldb, float beta,
                     float* C, int ldc, float* rs, float* cs) {
     using ht = typename Half<BF16>::t;
-    const int mp = (m + QBM - 1) / QBM * QBM, np = (n + QBN - 1) / QBN * QBN, kp = (k + QBK - 1) / QBK * QBK;
+    const int mp = (m + TM - 1) / TM * TM, np = (n + TN - 1) / TN * TN, kp = (k + QBK - 1) / QBK * QBK;
     const size_t need = ((size_t)mp + np) * kp * sizeof(ht);
     if (g_q16.n < need) g_q16.alloc(need);
     ht* Ah = (ht*)g_q16.p;
@@ -507,11 +537,18 @@ static void gemm16q(int m, int n, int k, float alpha, const float* A, int lda, c
     hipLaunchKernelGGL(convert_cols16_kernel<BF16>, dim3((unsigned)std::min<size_t>(ceil_div(tb, 256), 1 << 16)),
                        dim3(256), 0, ctx().s(), k, n, B, ldb, cs, Bh, kp, np);
     check_launch("convert16");
-    const int tiles_m = mp / QBM, tiles_n = np / QBN;
+    const int tiles_m = mp / TM, tiles_n = np / TN;
+    constexpr size_t lds = q_lds_bytes<TM, TN>();
+    static bool lds_set = false;  // per instantiation: allow the dynamic LDS beyond 64 KiB once
+    if (!lds_set) {
+        SQ_HIP(hipFuncSetAttribute((const void*)gemm16q_kernel<BF16, TM, TN>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        lds_set = true;
+    }
     ProfScope prof(PROF_GEMM16Q, 2.0 * m * n * k,
                    2.0 * ((double)m * k + (double)k * n) + 4.0 * (beta != 0.0f ? 2.0 : 1.0) * m * n);
-    hipLaunchKernelGGL(gemm16q_kernel<BF16>, dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, kp, alpha, Ah, Bh,
-                       beta, C, ldc, tiles_m, rs, cs);
+    hipLaunchKernelGGL((gemm16q_kernel<BF16, TM, TN>), dim3(tiles_m * tiles_n), dim3(256), lds, ctx().s(), m, n, kp,
+                       alpha, Ah, Bh, beta, C, ldc, tiles_m, rs, cs);
     check_launch("gemm16q");
 }
 
@@ -536,10 +573,15 @@ void gemm16(int kind, int m, int n, int k, float alpha, const float* A, int lda,
             hipLaunchKernelGGL(bits_to_pow2_scale, dim3(ceil_div(m, 256)), dim3(256), 0, ctx().s(), m, mx, rs);
             hipLaunchKernelGGL(pow2_scale_cols, dim3(ceil_div(n, 4)), dim3(256), 0, ctx().s(), k, n, B, ldb, cs);
         }
-        if (bf)
-            gemm16q<true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs);
-        else
-            gemm16q<false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs);
+        // 256 x 256 tiles once they fill the device, 128 x 128 tiles below that
+        const bool big = ceil_div(m, 256) * ceil_div(n, 256) >= 192;
+        if (bf) {
+            if (big) gemm16q<true, 256, 256>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs);
+            else gemm16q<true, 128, 128>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs);
+        } else {
+            if (big) gemm16q<false, 256, 256>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs);
+            else gemm16q<false, 128, 128>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs);
+        }
         return;
     }
     float *rs = nullptr, *cs = nullptr;
@@ -1298,158 +1340,205 @@ static void trsm_inv(int kind, int n, int nrhs, const T* L, int ldl, const T* in
 // largest tie rule included). If any multiplier fails (or a pivot is zero or not finite),
 // the matrix is restored from its copy and factored by the pivoting path.
 //
-// lu128_factor_kernel factors one <= 128 x 128 diagonal block held in registers (1024
-// threads: thread t owns column t % 128, rows 16 (t / 128) .. +15), two barriers per
-// column, FP64 arithmetic; lu128_inverse_kernel then forms the inverses of its unit-lower
-// L and upper U (one workgroup each, one barrier per step) for the triangular solves that
-// follow. The padded part of a smaller block is the identity.
+// lu128_kernel factors one <= 128 x 128 diagonal block without pivoting and forms the
+// inverses of its unit-lower L and upper U, in one workgroup of 1024 threads, FP64
+// arithmetic. Wave w owns columns 8w .. 8w+7, lane l rows l and l + 64, so every column
+// lives in one wave: at step j the wave owning column j forms the multipliers (the pivot
+// is read out of its own lane, v_readlane) and publishes them through LDS; after the one
+// barrier of the step every wave updates its columns with the multipliers and its own
+// row-j values (again read out of a lane). The factor is then kept in LDS (128 KiB) and
+// each wave inverts L and U for its own eight columns with no further barriers: the
+// solution row it needs is in its own registers and the factor column comes from LDS.
+// The padded part of a smaller block is the identity. Register arrays are indexed with
+// compile-time indices only (selects), so they stay in VGPRs.
 constexpr int NB = 128;
 // the in-place GEMMs (L21 = A21 U11^-1, X = inv B) rely on one tile spanning the block
 static_assert(GBM >= NB && GBN >= NB, "GEMM tile narrower than the inverted blocks");
+constexpr int LU_LD = NB + 1;                                   // LDS row stride (doubles)
+constexpr size_t LU_LDS = (size_t)NB * LU_LD * sizeof(double);  // 132 KiB
 
-// Wave-uniform broadcast of 16 doubles held in LDS at v[0..15]: lane l loads v[l & 15]
-// once, then each value is read out of its lane into scalar registers (v_readlane), so a
-// wave moves 16 doubles through LDS instead of 16 x 64.
-__device__ __forceinline__ void bcast16(const double* v, double (&out)[16]) {
-    const double mine = v[__lane_id() & 15];
-    const int lo = __double2loint(mine), hi = __double2hiint(mine);
-#pragma unroll
-    for (int e = 0; e < 16; ++e)
-        out[e] = __hiloint2double(__builtin_amdgcn_readlane(hi, e), __builtin_amdgcn_readlane(lo, e));
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
 }
 
-// One sweep of the triangular inversion: lower forms L^-1 (unit lower, forward), otherwise
-// U^-1 (backward, diagonal dU). Column j of the factor is published by its owners, row j
-// of the inverse by its owners; one barrier per step with double-buffered broadcasts.
-// Inner loops are branch-free (selects), so the LDS reads of a step issue together;
-// register arrays are indexed with compile-time indices only, so they stay in VGPRs. The
-// direction is a run-time argument so that one register allocation serves both sweeps.
+// factor = 1: factor the block at A (grid of one), check it, write L\U back, and the
+// inverses. factor = 0: the blocks are already factored; workgroup b inverts diagonal
+// block b (A advanced by b NB (lda + 1), inverses by b NB^2).
+// Every loop over the 128 steps is split at j = 64 so that the half of the rows holding
+// row j (h = j / 64) is a compile-time constant: a run-time choice between a[c][0] and
+// a[c][1] would be folded into an indexed access and move the arrays to scratch memory.
 template <typename T>
-__device__ __forceinline__ void lu128_inverse(bool lower, const double (&a)[16], int c, int g, double (*rowb)[NB],
-                                              double (*colb)[NB], const double* dU, T* Xout) {
-    const int i0 = g * 16;
-    double x[16];
-#pragma unroll
-    for (int e = 0; e < 16; ++e) x[e] = (i0 + e == c) ? 1.0 : 0.0;
-    for (int s = 0; s < NB; ++s) {
-        const int j = lower ? s : NB - 1 - s, p = s & 1;
-        if (g == (j >> 4)) {
-            double v = 0.0;
-#pragma unroll
-            for (int e = 0; e < 16; ++e) v = (e == (j & 15)) ? x[e] : v;
-            if (!lower) {
-                v = v / dU[j];
-#pragma unroll
-                for (int e = 0; e < 16; ++e) x[e] = (e == (j & 15)) ? v : x[e];
-            }
-            rowb[p][c] = v;
-        }
-        if (c == j) {
-#pragma unroll
-            for (int e = 0; e < 16; ++e) colb[p][i0 + e] = a[e];
-        }
-        __syncthreads();
-        // rows i with i > j (lower) or i < j (upper): i in [lo, hi)
-        const int lo = lower ? j + 1 : 0, hi = lower ? NB : j;
-        if (i0 + 15 >= lo && i0 < hi) {
-            const double r = rowb[p][c];
-            double cb[16];
-            bcast16(&colb[p][i0], cb);
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const int i = i0 + e;
-                const double v = x[e] - cb[e] * r;
-                x[e] = (i >= lo && i < hi) ? v : x[e];
-            }
-        }
-    }
-#pragma unroll
-    for (int e = 0; e < 16; ++e) Xout[(i0 + e) + (size_t)c * NB] = (T)x[e];
-}
-
-// Factors the <= 128 x 128 diagonal block at A (order w) without pivoting, writes L\U back
-// and sets *flag when a pivot is zero or not finite or a multiplier exceeds 1 in modulus.
-template <typename T>
-__global__ void __launch_bounds__(1024) lu128_factor_kernel(T* A, int lda, int w, int* flag) {
-    __shared__ double rowb[2][NB], colb[2][NB];
+__global__ void __launch_bounds__(1024) lu128_kernel(T* A, int lda, int n, T* Linv, T* Uinv, int* flag, int factor) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char lu_smem[];
+    double* M = reinterpret_cast<double*>(lu_smem);  // NB x LU_LD, row-major: M[i * LU_LD + j]
+    __shared__ double colb[2][NB];
     __shared__ int bad;
-    const int t = threadIdx.x, c = t & (NB - 1), g = t >> 7, i0 = g * 16;
-    double a[16];
-#pragma unroll
-    for (int e = 0; e < 16; ++e) {
-        const int i = i0 + e;
-        a[e] = (i < w && c < w) ? (double)A[i + (size_t)c * lda] : (i == c ? 1.0 : 0.0);
-    }
+    const int b = blockIdx.x;
+    A += (size_t)b * NB * ((size_t)lda + 1);
+    Linv += (size_t)b * NB * NB;
+    Uinv += (size_t)b * NB * NB;
+    const int w = min(NB, n - b * NB);
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6, c0 = 8 * wv;
+    using H0 = std::integral_constant<int, 0>;
+    using H1 = std::integral_constant<int, 1>;
     if (t == 0) bad = 0;
-    __syncthreads();
-    int mybad = 0;
-    for (int j = 0; j < NB; ++j) {
-        const int p = j & 1;
-        if (g == (j >> 4) && c >= j) {
-            double v = 0.0;
+    if (factor) {
+        double a[8][2];
 #pragma unroll
-            for (int e = 0; e < 16; ++e) v = (e == (j & 15)) ? a[e] : v;
-            rowb[p][c] = v;
-        }
-        __syncthreads();
-        if (c == j) {
-            const double piv = rowb[p][j];
-            if (g == 0 && !(piv != 0.0 && fabs(piv) <= 1.79e308)) mybad = 1;
-            const double r = 1.0 / piv;
+        for (int c = 0; c < 8; ++c)
 #pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const bool below = i0 + e > j;
-                const double l = a[e] * r;
-                colb[p][i0 + e] = l;  // rows <= j are never read back
-                if (below && !(fabs(l) <= 1.0)) mybad = 1;
-                a[e] = below ? l : a[e];
+            for (int h = 0; h < 2; ++h) {
+                const int i = lane + 64 * h, k = c0 + c;
+                a[c][h] = (i < w && k < w) ? (double)A[i + (size_t)k * lda] : (i == k ? 1.0 : 0.0);
             }
-        }
         __syncthreads();
-        // wave-uniform test (c | 63 is the wave's last column): the broadcast needs all lanes
-        if (i0 + 15 > j && (c | 63) > j) {
-            double cl[16];
-            bcast16(&colb[p][i0], cl);
-            if (c > j) {
-                const double u = rowb[p][c];
+        int mybad = 0;
+        auto fstep = [&](auto hjc, int j) {
+            constexpr int HJ = decltype(hjc)::value;
+            const int p = j & 1, lj = j & 63;
+            if (wv == (j >> 3)) {
+                // this wave owns column j: multipliers l_i = a_ij / a_jj for i > j
+                double col[2];
 #pragma unroll
-                for (int e = 0; e < 16; ++e) {
-                    const double v = a[e] - cl[e] * u;
-                    a[e] = (i0 + e > j) ? v : a[e];
+                for (int h = 0; h < 2; ++h) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) v = (c == (j & 7)) ? a[c][h] : v;
+                    col[h] = v;
+                }
+                const double piv = readlane_d(col[HJ], lj);
+                if (lane == 0 && !(piv != 0.0 && fabs(piv) <= 1.79e308)) mybad = 1;
+                const double r = 1.0 / piv;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int i = lane + 64 * h;
+                    const bool below = i > j;
+                    const double l = col[h] * r;
+                    colb[p][i] = l;  // rows <= j are never read back
+                    if (below && !(fabs(l) <= 1.0)) mybad = 1;
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) a[c][h] = (below && c == (j & 7)) ? l : a[c][h];
                 }
             }
+            __syncthreads();
+            if (c0 + 7 > j) {  // wave-uniform: the wave has columns right of j
+                double u[8];
+#pragma unroll
+                for (int c = 0; c < 8; ++c) u[c] = readlane_d(a[c][HJ], lj);
+#pragma unroll
+                for (int h = HJ; h < 2; ++h) {  // rows below j lie in halves h >= HJ
+                    const int i = lane + 64 * h;
+                    const double lv = colb[p][i];
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) {
+                        const double v = a[c][h] - lv * u[c];
+                        a[c][h] = (i > j && c0 + c > j) ? v : a[c][h];
+                    }
+                }
+            }
+        };
+        for (int j = 0; j < 64; ++j) fstep(H0{}, j);
+        for (int j = 64; j < NB; ++j) fstep(H1{}, j);
+        if (mybad) bad = 1;
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int i = lane + 64 * h, k = c0 + c;
+                M[i * LU_LD + k] = a[c][h];
+                if (i < w && k < w) A[i + (size_t)k * lda] = (T)a[c][h];
+            }
+    } else {
+        for (int e = t; e < NB * NB; e += 1024) {
+            const int i = e % NB, k = e / NB;
+            M[i * LU_LD + k] = (i < w && k < w) ? (double)A[i + (size_t)k * lda] : (i == k ? 1.0 : 0.0);
         }
     }
-    if (mybad) bad = 1;
-#pragma unroll
-    for (int e = 0; e < 16; ++e) {
-        const int i = i0 + e;
-        if (i < w && c < w) A[i + (size_t)c * lda] = (T)a[e];
-    }
     __syncthreads();
-    if (t == 0 && bad) atomicOr(flag, 1);
+    // inverses, eight columns per wave, no barriers: X = L^-1 (forward) and X = U^-1 (backward)
+    double x[8][2];
+    auto init_identity = [&]() {
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) x[c][h] = (lane + 64 * h == c0 + c) ? 1.0 : 0.0;
+    };
+    auto store_x = [&](T* X) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) X[(lane + 64 * h) + (size_t)(c0 + c) * NB] = (T)x[c][h];
+    };
+    // L^-1 is lower: row j of it is zero right of column j, so only waves with a column <= j
+    // take part; rows below j lie in halves h >= HJ
+    auto lstep = [&](auto hjc, int j) {
+        constexpr int HJ = decltype(hjc)::value;
+        if (c0 > j) return;
+        const int lj = j & 63;
+        double xj[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) xj[c] = readlane_d(x[c][HJ], lj);
+#pragma unroll
+        for (int h = HJ; h < 2; ++h) {
+            const int i = lane + 64 * h;
+            const double fv = M[i * LU_LD + j];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const double v = x[c][h] - fv * xj[c];
+                x[c][h] = i > j ? v : x[c][h];
+            }
+        }
+    };
+    // U^-1 is upper: row j is final once divided by u_jj; only waves with a column >= j take
+    // part; rows above j lie in halves h <= HJ
+    auto ustep = [&](auto hjc, int j) {
+        constexpr int HJ = decltype(hjc)::value;
+        const int lj = j & 63;
+        const double rd = 1.0 / M[j * LU_LD + j];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) x[c][HJ] = lane == lj ? x[c][HJ] * rd : x[c][HJ];
+        if (c0 + 7 < j) return;
+        double xj[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) xj[c] = readlane_d(x[c][HJ], lj);
+#pragma unroll
+        for (int h = 0; h <= HJ; ++h) {
+            const int i = lane + 64 * h;
+            const double fv = M[i * LU_LD + j];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const double v = x[c][h] - fv * xj[c];
+                x[c][h] = i < j ? v : x[c][h];
+            }
+        }
+    };
+    init_identity();
+    for (int j = 0; j < 64; ++j) lstep(H0{}, j);
+    for (int j = 64; j < NB; ++j) lstep(H1{}, j);
+    store_x(Linv);
+    init_identity();
+    for (int j = NB - 1; j >= 64; --j) ustep(H1{}, j);
+    for (int j = 63; j >= 0; --j) ustep(H0{}, j);
+    store_x(Uinv);
+    if (factor) {
+        __syncthreads();
+        if (t == 0 && bad) atomicOr(flag, 1);
+    }
 }
 
-// Inverses of the factors of the diagonal blocks of an LU (order n): grid 2 x nblocks,
-// workgroup (b, 0) writes L_bb^-1 to Linv + b NB^2, workgroup (b, 1) U_bb^-1 to Uinv + b NB^2
-// (NB x NB, identity-padded). The two sweeps run on different CUs.
 template <typename T>
-__global__ void __launch_bounds__(1024) lu128_inverse_kernel(const T* A, int lda, int n, T* Linv, T* Uinv) {
-    __shared__ double rowb[2][NB], colb[2][NB];
-    __shared__ double dU[NB];
-    const int b = blockIdx.x >> 1, upper = blockIdx.x & 1;
-    A += (size_t)b * NB * ((size_t)lda + 1);
-    const int w = min(NB, n - b * NB);
-    const int t = threadIdx.x, c = t & (NB - 1), g = t >> 7, i0 = g * 16;
-    double a[16];
-#pragma unroll
-    for (int e = 0; e < 16; ++e) {
-        const int i = i0 + e;
-        a[e] = (i < w && c < w) ? (double)A[i + (size_t)c * lda] : (i == c ? 1.0 : 0.0);
-        if (i == c) dU[c] = a[e];
+static void lu128_launch(int grid, T* A, int lda, int n, T* Linv, T* Uinv, int* flag, int factor) {
+    static bool lds_set = false;  // per instantiation: allow the 132 KiB of dynamic LDS once
+    if (!lds_set) {
+        SQ_HIP(hipFuncSetAttribute((const void*)lu128_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)LU_LDS));
+        lds_set = true;
     }
-    __syncthreads();
-    lu128_inverse<T>(!upper, a, c, g, rowb, colb, dU, (upper ? Uinv : Linv) + (size_t)b * NB * NB);
+    hipLaunchKernelGGL(lu128_kernel<T>, dim3(grid), dim3(1024), LU_LDS, ctx().s(), A, lda, n, Linv, Uinv, flag,
+                       factor);
+    check_launch("lu128");
 }
 
 // flag |= (some |x| > 1 or not finite) over an m x n block
@@ -1488,10 +1577,7 @@ static void getrf_np(int kind, T* A, int lda, int m, int w, int base, T* Linv, T
         const size_t off = (size_t)(base / NB) * NB * NB;
         {
             ProfScope prof(PROF_PANEL, (2.0 / 3.0) * w * w * w, 2.0 * sizeof(T) * w * w);
-            hipLaunchKernelGGL(lu128_factor_kernel<T>, dim3(1), dim3(1024), 0, ctx().s(), A, lda, w, flag);
-            hipLaunchKernelGGL(lu128_inverse_kernel<T>, dim3(2), dim3(1024), 0, ctx().s(), A, lda, w, Linv + off,
-                               Uinv + off);
-            check_launch("lu128");
+            lu128_launch<T>(1, A, lda, w, Linv + off, Uinv + off, flag, 1);
         }
         if (m > w) {
             // L21 = A21 U11^-1, in place: one column tile, each row tile read before written
@@ -1799,7 +1885,7 @@ void getrs(int kind, int trans, int n, int nrhs, const T* LU, int lda, const int
         if (g_getrs_inv.n < need) g_getrs_inv.alloc(need);
         T* Linv = (T*)g_getrs_inv.p;
         T* Uinv = Linv + (size_t)nb * NB * NB;
-        hipLaunchKernelGGL(lu128_inverse_kernel<T>, dim3(2 * nb), dim3(1024), 0, ctx().s(), LU, lda, n, Linv, Uinv);
+        lu128_launch<T>(nb, const_cast<T*>(LU), lda, n, Linv, Uinv, nullptr, 0);
         check_launch("lu128 inverses");
         trsm_inv<T, true, NB>(kind, n, nrhs, LU, lda, Linv, B, ldb);
         trsm_inv<T, false, NB>(kind, n, nrhs, LU, lda, Uinv, B, ldb);
