@@ -114,7 +114,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=list(MODES), default="fp32")
+    ap.add_argument("--mode", choices=list(MODES), default="bf16")
     ap.add_argument("--n_theta", type=int, default=64)
     ap.add_argument("--L_over_R", type=float, default=4.0)
     ap.add_argument("--n_r", type=int, default=16)

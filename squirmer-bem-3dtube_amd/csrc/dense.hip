@@ -72,7 +72,8 @@ struct MfmaOp<float> {
     static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
 };
 
-constexpr int GBM = 128, GBN = 128, GBK = 16, GPAD = GBK + 1;
+constexpr int GBM = 128, GBN = 128;
+constexpr int GBK = 16, GPAD = GBK + 1;
 
 // MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16] and B[l/16][l%16]; the four
 // accumulator values of lane l are D[Op::row(l/16, r)][l%16].
@@ -286,24 +287,36 @@ __global__ void __launch_bounds__(256) gemm16_nn_kernel(int m, int n, int k, flo
         __syncthreads();
     }
 
+    // epilogue: scales and C values of a row of tiles are read together from clamped
+    // addresses, then the results stored under the bounds test
+    float ucol[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) ucol[j] = cs ? 1.0f / cs[min(n0 + wn + j * 16 + lr, n - 1)] : 1.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float urow[4], cv[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) urow[r] = rs ? 1.0f / rs[min(m0 + wm + i * 16 + 4 * lk + r, m - 1)] : 1.0f;
+        if (beta != 0.0f) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    cv[j][r] = C[min(m0 + wm + i * 16 + 4 * lk + r, m - 1) + (size_t)min(n0 + wn + j * 16 + lr, n - 1) * ldc];
+        }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int col = n0 + wn + j * 16 + lr;
-            if (col >= n) continue;
-            const float ucol = cs ? 1.0f / cs[col] : 1.0f;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = m0 + wm + i * 16 + 4 * lk + r;
-                if (row < m) {
-                    const float urow = rs ? 1.0f / rs[row] : 1.0f;
-                    float* c = C + row + (size_t)col * ldc;
-                    float v = alpha * (acc[i][j][r] * urow * ucol);
-                    *c = beta == 0.0f ? v : v + beta * *c;
+                if (row < m && col < n) {
+                    const float v = alpha * (acc[i][j][r] * urow[r] * ucol[j]);
+                    C[row + (size_t)col * ldc] = beta == 0.0f ? v : v + beta * cv[j][r];
                 }
             }
         }
+    }
 }
 
 // Power-of-two scale per row of A (m x k) or per column of B (k x n) so the largest
@@ -346,8 +359,6 @@ __global__ void pow2_scale_cols(int k, int n, const float* __restrict__ B, int l
 // MFMA 32x32x16 layout (measured on gfx950): lane l supplies A[l%32][8(l/32)+0..7] and
 // B[8(l/32)+0..7][l%32]; accumulator value r of lane l is D[8(r/4) + 4(l/32) + r%4][l%32].
 constexpr int QBK = 64, QPAD = QBK + 8;
-template <int TM, int TN>
-constexpr size_t q_lds_bytes() { return 2 * (size_t)(TM + TN) * QPAD * 2; }
 
 // Workgroup tile TM x TN (256 x 256 for large products, 128 x 128 for mid-sized ones), four
 // waves of TM/2 x TN/2, each a grid of 32 x 32 MFMA tiles.
@@ -360,11 +371,12 @@ __global__ void __launch_bounds__(256) gemm16q_kernel(int m, int n, int kp, floa
     using H = Half<BF16>;
     using ht = typename H::t;
     using v8 = typename H::v8;
-    constexpr int STAGE = (TM + TN) * QPAD;          // 16-bit elements per LDS stage
     constexpr int IM = TM / 64, JN = TN / 64;        // 32 x 32 tiles per wave
     constexpr int CA = TM * (QBK / 8) / 256, CB = TN * (QBK / 8) / 256;  // 16-byte chunks a thread
-    extern __shared__ __attribute__((aligned(16))) unsigned char q_smem[];
-    ht* lds = reinterpret_cast<ht*>(q_smem);
+    // two stages of each operand tile; separate static arrays tell the compiler that A and B
+    // tiles never alias (up to 144 KiB of the CU's 160 KiB of LDS)
+    __shared__ __attribute__((aligned(16))) ht As_[2][TM][QPAD];
+    __shared__ __attribute__((aligned(16))) ht Bs_[2][TN][QPAD];
 
     int tm, tn;
     tile_of(blockIdx.x, gridDim.x, tiles_m, tm, tn);
@@ -387,17 +399,15 @@ __global__ void __launch_bounds__(256) gemm16q_kernel(int m, int n, int kp, floa
         }
     };
     auto store = [&](int stage) {
-        ht* As = lds + stage * STAGE;
-        ht* Bs = As + TM * QPAD;
 #pragma unroll
         for (int q = 0; q < CA; ++q) {
             const int c = tid + 256 * q, row = c >> 3, kc = c & 7;
-            *(u4*)&As[row * QPAD + 8 * kc] = ra[q];
+            *(u4*)&As_[stage][row][8 * kc] = ra[q];
         }
 #pragma unroll
         for (int q = 0; q < CB; ++q) {
             const int c = tid + 256 * q, row = c >> 3, kc = c & 7;
-            *(u4*)&Bs[row * QPAD + 8 * kc] = rb[q];
+            *(u4*)&Bs_[stage][row][8 * kc] = rb[q];
         }
     };
     f16x acc[IM][JN];
@@ -414,15 +424,14 @@ __global__ void __launch_bounds__(256) gemm16q_kernel(int m, int n, int kp, floa
     if (nk > 1) load(QBK);
     __syncthreads();
     for (int kt = 0; kt < nk; ++kt) {
-        const ht* As = lds + (kt & 1) * STAGE;
-        const ht* Bs = As + TM * QPAD;
+        const int st = kt & 1;
 #pragma unroll
         for (int s2 = 0; s2 < QBK / 16; ++s2) {
             v8 af[IM], bf[JN];
 #pragma unroll
-            for (int i = 0; i < IM; ++i) af[i] = *(const v8*)&As[(wm + i * 32 + lr) * QPAD + 16 * s2 + 8 * lk];
+            for (int i = 0; i < IM; ++i) af[i] = *(const v8*)&As_[st][wm + i * 32 + lr][16 * s2 + 8 * lk];
 #pragma unroll
-            for (int j = 0; j < JN; ++j) bf[j] = *(const v8*)&Bs[(wn + j * 32 + lr) * QPAD + 16 * s2 + 8 * lk];
+            for (int j = 0; j < JN; ++j) bf[j] = *(const v8*)&Bs_[st][wn + j * 32 + lr][16 * s2 + 8 * lk];
 #pragma unroll
             for (int i = 0; i < IM; ++i)
 #pragma unroll
@@ -434,24 +443,36 @@ __global__ void __launch_bounds__(256) gemm16q_kernel(int m, int n, int kp, floa
         }
         __syncthreads();
     }
+    // epilogue, one 32-row band of tiles at a time: scales and C values are read together
+    // from clamped addresses, then the results stored under the bounds test
+    float ucol[JN];
 #pragma unroll
-    for (int i = 0; i < IM; ++i)
+    for (int j = 0; j < JN; ++j) ucol[j] = cs ? 1.0f / cs[min(n0 + wn + j * 32 + lr, n - 1)] : 1.0f;
+#pragma unroll
+    for (int i = 0; i < IM; ++i) {
+        float urow[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            urow[r] = rs ? 1.0f / rs[min(m0 + wm + i * 32 + 8 * (r >> 2) + 4 * lk + (r & 3), m - 1)] : 1.0f;
 #pragma unroll
         for (int j = 0; j < JN; ++j) {
-            const int col = n0 + wn + j * 32 + lr;
-            if (col >= n) continue;
-            const float ucol = cs ? 1.0f / cs[col] : 1.0f;
+            const int col = n0 + wn + j * 32 + lr, colc = min(col, n - 1);
+            float cv[16];
+            if (beta != 0.0f) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    cv[r] = C[min(m0 + wm + i * 32 + 8 * (r >> 2) + 4 * lk + (r & 3), m - 1) + (size_t)colc * ldc];
+            }
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int row = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * lk + (r & 3);
-                if (row < m) {
-                    const float urow = rs ? 1.0f / rs[row] : 1.0f;
-                    float* c = C + row + (size_t)col * ldc;
-                    const float v = alpha * (acc[i][j][r] * urow * ucol);
-                    *c = beta == 0.0f ? v : v + beta * *c;
+                if (row < m && col < n) {
+                    const float v = alpha * (acc[i][j][r] * urow[r] * ucol[j]);
+                    C[row + (size_t)col * ldc] = beta == 0.0f ? v : v + beta * cv[r];
                 }
             }
         }
+    }
 }
 
 // Ah[i * kp + k] = A(i, k) * rs[i] (rows >= m and k >= kdim zero), via 64 x 64 LDS tiles
@@ -497,8 +518,8 @@ static void gemm_launch(int m, int n, int k, T alpha, const T* A, int lda, const
     int tiles_m = (int)ceil_div(m, GBM), tiles_n = (int)ceil_div(n, GBN);
     ProfScope prof(sizeof(T) == 8 ? PROF_DGEMM : PROF_SGEMM, 2.0 * m * n * k,
                    (double)sizeof(T) * ((double)m * k + (double)k * n + (beta != T(0) ? 2.0 : 1.0) * m * n));
-    hipLaunchKernelGGL(gemm_nn_kernel<T>, dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, k, alpha, A,
-                       lda, B, ldb, beta, C, ldc, tiles_m);
+    hipLaunchKernelGGL(gemm_nn_kernel<T>, dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, k, alpha, A, lda,
+                       B, ldb, beta, C, ldc, tiles_m);
     check_launch("gemm");
 }
 
@@ -538,16 +559,9 @@ static void gemm16q(int m, int n, int k, float alpha, const float* A, int lda, c
                        dim3(256), 0, ctx().s(), k, n, B, ldb, cs, Bh, kp, np);
     check_launch("convert16");
     const int tiles_m = mp / TM, tiles_n = np / TN;
-    constexpr size_t lds = q_lds_bytes<TM, TN>();
-    static bool lds_set = false;  // per instantiation: allow the dynamic LDS beyond 64 KiB once
-    if (!lds_set) {
-        SQ_HIP(hipFuncSetAttribute((const void*)gemm16q_kernel<BF16, TM, TN>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        lds_set = true;
-    }
     ProfScope prof(PROF_GEMM16Q, 2.0 * m * n * k,
                    2.0 * ((double)m * k + (double)k * n) + 4.0 * (beta != 0.0f ? 2.0 : 1.0) * m * n);
-    hipLaunchKernelGGL((gemm16q_kernel<BF16, TM, TN>), dim3(tiles_m * tiles_n), dim3(256), lds, ctx().s(), m, n, kp,
+    hipLaunchKernelGGL((gemm16q_kernel<BF16, TM, TN>), dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, kp,
                        alpha, Ah, Bh, beta, C, ldc, tiles_m, rs, cs);
     check_launch("gemm16q");
 }
@@ -1599,64 +1613,73 @@ static void getrf_np(int kind, T* A, int lda, int m, int w, int base, T* Linv, T
 
 // ------------------------------------------------------------------ skinny triangular solves
 // B (n x nrhs, nrhs <= 8) <- T^-1 B in one launch, reading the triangle once. Workgroup k,
-// in the order workgroups start (an atomic ticket), owns the k-th TV-row block in solve
+// in the order workgroups start (an atomic ticket), owns the k-th 128-row block in solve
 // order. It streams its block row of the triangle tile by tile (the loads do not depend
 // on the solution, so they are issued before any wait), applies each solved block x_j as
-// soon as the completion counter shows it, then solves its own diagonal block (wave 0:
-// lane i owns row i, x_j broadcast by shuffle) and publishes x_k. Blocks complete in
-// ticket order, so one monotonic counter serves as every block's flag, and a workgroup
-// only waits on tickets taken before its own: the chain cannot deadlock whatever the
-// residency. Both counters grow by nb per launch and are never reset; a wait is bounded
-// and a timeout is reported through the device error word. Sums run in FP64.
-constexpr int TV = 64;
+// soon as the completion counter shows it, then forms its own x_k and publishes it.
+// Blocks complete in ticket order, so one monotonic counter serves as every block's flag,
+// and a workgroup only waits on tickets taken before its own: the chain cannot deadlock
+// whatever the residency. Both counters grow by nb per launch and are never reset; a wait
+// is bounded and a timeout is reported through the device error word. Sums run in FP64.
 struct TrsvState {
     unsigned long long ticket, done;
 };
 static DBuf<TrsvState> g_trsv;
 static unsigned long long g_trsv_base = 0;
 
-template <typename T, bool LOWER, bool UNIT, int NR>
-__global__ void __launch_bounds__(256) trsv_chain(int n, int nrhs, const T* __restrict__ L, int ldl, T* B, int ldb,
-                                                  TrsvState* st, unsigned long long base, int* err) {
-    // tiles per group: their triangle values are loaded together, and all the ready ones
-    // among them are applied with one solution load and two barriers
-    constexpr int GQ = sizeof(T) == 4 ? 8 : 4;
-    constexpr int XB = GQ * NR * TV * (int)sizeof(T), PB = 4 * NR * TV * (int)sizeof(double);
-    __shared__ T Ld[TV][TV + 1];
-    __shared__ __align__(16) unsigned char smem[XB > PB ? XB : PB];
-    T(*Xs)[NR][TV] = reinterpret_cast<T(*)[NR][TV]>(smem);            // [GQ][NR][TV]
-    double(*part)[NR][TV] = reinterpret_cast<double(*)[NR][TV]>(smem);  // [4][NR][TV], after the loop
+// B (n x nrhs, nrhs <= 8) <- T^-1 B with the inverted 128 x 128 diagonal blocks of T at
+// hand (Dinv: nb blocks, NB x NB each, consecutive): the same ticket-ordered chain as
+// trsv_chain with 128-row blocks, where a block's own solve is the product of its inverse
+// (staged in LDS while the workgroup waits) with the accumulated right-hand side, spread
+// over all 256 threads. Thread t owns row t % 128 and column half t / 128 of every tile.
+template <typename T, bool LOWER, int NR>
+__global__ void __launch_bounds__(256) trsv_inv_chain(int n, int nrhs, const T* __restrict__ Lt,
+                                                      const T* __restrict__ Dinv, T* B, int ldb, TrsvState* st,
+                                                      unsigned long long base, int* err) {
+    constexpr int GQ = sizeof(T) == 4 ? 2 : 1;
+    constexpr int DLD = NB + 1;
+    extern __shared__ __attribute__((aligned(16))) unsigned char tv_smem[];
+    T* Dv = reinterpret_cast<T*>(tv_smem);                                    // [NB][DLD]
+    // the tile buffer (main loop) and the reduction buffers (after it) share one region
+    T(*Xs)[NR][NB] = reinterpret_cast<T(*)[NR][NB]>(tv_smem + sizeof(T) * NB * DLD);  // [GQ][NR][NB]
+    double(*part)[NR][NB] = reinterpret_cast<double(*)[NR][NB]>(tv_smem + sizeof(T) * NB * DLD);  // [2][NR][NB]
+    double(*rv)[NB] = reinterpret_cast<double(*)[NB]>(tv_smem + sizeof(T) * NB * DLD +
+                                                      sizeof(double) * 2 * NR * NB);  // [NR][NB]
     __shared__ int s_k;
     __shared__ unsigned long long s_seen;
-    const int tid = threadIdx.x, lane = tid & 63, cg = tid >> 6;
-    const int nb = (n + TV - 1) / TV;
+    const int tid = threadIdx.x, row = tid & (NB - 1), hf = tid >> 7;
+    const int nb = (n + NB - 1) / NB;
     if (tid == 0)
         s_k = (int)(__hip_atomic_fetch_add(&st->ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - base);
     __syncthreads();
     const int k = s_k;
     const int blk = LOWER ? k : nb - 1 - k;
-    const int r0 = blk * TV, rb = min(TV, n - r0);
-    for (int e = tid; e < TV * TV; e += 256) {
-        const int i = e % TV, j = e / TV;
-        Ld[i][j] = (i < rb && j < rb) ? L[(r0 + i) + (size_t)(r0 + j) * ldl] : (i == j ? T(1) : T(0));
+    const int r0 = blk * NB, rb = min(NB, n - r0);
+    {
+        const T* D = Dinv + (size_t)blk * NB * NB;
+        for (int e = tid; e < NB * NB; e += 256) Dv[(e % NB) * DLD + e / NB] = D[e];
     }
     double acc[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = 0.0;
-    const bool rowok = lane < rb;
-    const T* Lrow = L + (r0 + (rowok ? lane : 0));
+    const bool rowok = row < rb;
+    // tiles of this block row, contiguous in the tiled copy (zero beyond n)
+    const T* Lrow = Lt + (size_t)blk * nb * NB * NB + (size_t)(hf * 64) * NB + row;
     unsigned long long seen = base;
     for (int q0 = 0; q0 < k; q0 += GQ) {
         const int nq = min(GQ, k - q0);
-        T l[GQ][16];
+        T l[GQ][64];
+        // unconditional loads (a tile past the group's end re-reads its last one, unused), so
+        // all of them are in flight together: a guarded load would be waited for one by one
 #pragma unroll
         for (int g = 0; g < GQ; ++g) {
-            const int jb = LOWER ? q0 + g : nb - 1 - (q0 + g);
-            const int c0 = jb * TV + cg * 16, cend = min(n, jb * TV + TV);
+            const int qq = q0 + min(g, nq - 1);
+            const int jb = LOWER ? qq : nb - 1 - qq;
+            const T* tile = Lrow + (size_t)jb * NB * NB;
 #pragma unroll
-            for (int c = 0; c < 16; ++c) l[g][c] = (g < nq && rowok && c0 + c < cend) ? Lrow[(size_t)(c0 + c) * ldl] : T(0);
+            for (int c = 0; c < 64; ++c) l[g][c] = tile[(size_t)c * NB];
         }
-        int qd = q0;  // next tile of the group to apply
+        int qd = q0;
         while (qd < q0 + nq) {
             if (seen < base + qd + 1) {
                 if (tid == 0) {
@@ -1672,29 +1695,27 @@ __global__ void __launch_bounds__(256) trsv_chain(int n, int nrhs, const T* __re
                         __builtin_amdgcn_s_sleep(1);
                     }
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                    // after a timeout the loop still ends: treat the rest as ready (the
-                    // result is wrong and the error word reports it)
                     s_seen = d >= base + qd + 1 ? d : base + (unsigned long long)k;
                 }
                 __syncthreads();
                 seen = s_seen;
             }
-            const int upto = min(q0 + nq, (int)(seen - base));  // tiles [qd, upto) are ready
-            for (int e = tid; e < (upto - qd) * NR * TV; e += 256) {
-                const int g = e / (NR * TV), r = (e / TV) % NR, i = e % TV;
+            const int upto = min(q0 + nq, (int)(seen - base));
+            for (int e = tid; e < (upto - qd) * NR * NB; e += 256) {
+                const int g = e / (NR * NB), r = (e / NB) % NR, i = e % NB;
                 const int jb = LOWER ? qd + g : nb - 1 - (qd + g);
-                const int row = jb * TV + i;
-                Xs[qd - q0 + g][r][i] = (r < nrhs && row < n) ? B[row + (size_t)r * ldb] : T(0);
+                const int rr = jb * NB + i;
+                Xs[qd - q0 + g][r][i] = (r < nrhs && rr < n) ? B[rr + (size_t)r * ldb] : T(0);
             }
             __syncthreads();
 #pragma unroll
             for (int g = 0; g < GQ; ++g) {
                 if (g >= qd - q0 && g < upto - q0) {
 #pragma unroll
-                    for (int c = 0; c < 16; ++c) {
+                    for (int c = 0; c < 64; ++c) {
                         const double lv = (double)l[g][c];
 #pragma unroll
-                        for (int r = 0; r < NR; ++r) acc[r] += lv * (double)Xs[g][r][cg * 16 + c];
+                        for (int r = 0; r < NR; ++r) acc[r] += lv * (double)Xs[g][r][hf * 64 + c];
                     }
                 }
             }
@@ -1703,45 +1724,30 @@ __global__ void __launch_bounds__(256) trsv_chain(int n, int nrhs, const T* __re
         }
     }
 #pragma unroll
-    for (int r = 0; r < NR; ++r) part[cg][r][lane] = acc[r];
+    for (int r = 0; r < NR; ++r) part[hf][r][row] = acc[r];
     __syncthreads();
-    if (tid < 64) {
-        const int i = lane;
-        double x[NR];
+    if (hf == 0) {
 #pragma unroll
         for (int r = 0; r < NR; ++r)
-            x[r] = (r < nrhs && rowok)
-                       ? (double)B[(r0 + i) + (size_t)r * ldb] - ((part[0][r][i] + part[1][r][i]) + (part[2][r][i] + part[3][r][i]))
-                       : 0.0;
-        if (LOWER) {
-            for (int j = 0; j < TV; ++j) {
-                if (!UNIT && i == j) {
+            rv[r][row] = (r < nrhs && rowok) ? (double)B[(r0 + row) + (size_t)r * ldb] - (part[0][r][row] + part[1][r][row])
+                                             : 0.0;
+    }
+    __syncthreads();
+    // x = Dinv r: thread (row, hf) sums over the columns of its half
+    double xs[NR];
 #pragma unroll
-                    for (int r = 0; r < NR; ++r) x[r] /= (double)Ld[j][j];
-                }
-                const double lij = (double)Ld[i][j];
+    for (int r = 0; r < NR; ++r) xs[r] = 0.0;
+    for (int c = 0; c < 64; ++c) {
+        const int cc = hf * 64 + c;
+        const double dv = (double)Dv[row * DLD + cc];
 #pragma unroll
-                for (int r = 0; r < NR; ++r) {
-                    const double xj = __shfl(x[r], j);
-                    if (i > j) x[r] -= lij * xj;
-                }
-            }
-        } else {
-            for (int j = TV - 1; j >= 0; --j) {
-                if (!UNIT && i == j) {
+        for (int r = 0; r < NR; ++r) xs[r] += dv * rv[r][cc];
+    }
 #pragma unroll
-                    for (int r = 0; r < NR; ++r) x[r] /= (double)Ld[j][j];
-                }
-                const double lij = (double)Ld[i][j];
-#pragma unroll
-                for (int r = 0; r < NR; ++r) {
-                    const double xj = __shfl(x[r], j);
-                    if (i < j) x[r] -= lij * xj;
-                }
-            }
-        }
-        if (rowok)
-            for (int r = 0; r < nrhs; ++r) B[(r0 + i) + (size_t)r * ldb] = (T)x[r];
+    for (int r = 0; r < NR; ++r) part[hf][r][row] = xs[r];
+    __syncthreads();
+    if (hf == 0 && rowok) {
+        for (int r = 0; r < nrhs; ++r) B[(r0 + row) + (size_t)r * ldb] = (T)(part[0][r][row] + part[1][r][row]);
         __threadfence();
     }
     __syncthreads();
@@ -1749,10 +1755,30 @@ __global__ void __launch_bounds__(256) trsv_chain(int n, int nrhs, const T* __re
         __hip_atomic_fetch_max(&st->done, base + (unsigned long long)k + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <typename T, bool LOWER, bool UNIT>
-static void trsv_skinny(int n, int nrhs, const T* L, int ldl, T* B, int ldb) {
+template <typename T, bool LOWER, int NR>
+static void trsv_inv_launch(int n, int nrhs, const T* Lt, const T* Dinv, T* B, int ldb, unsigned long long base,
+                            int* err) {
+    constexpr int GQ = sizeof(T) == 4 ? 2 : 1;
+    constexpr size_t xs = sizeof(T) * GQ * NR * NB, red = sizeof(double) * 3 * NR * NB;
+    constexpr size_t lds = sizeof(T) * NB * (NB + 1) + (xs > red ? xs : red);
+    static_assert(lds + 64 <= 160 * 1024, "trsv_inv_chain exceeds the LDS of a CU");
+    static bool lds_set = false;
+    if (!lds_set) {
+        SQ_HIP(hipFuncSetAttribute((const void*)trsv_inv_chain<T, LOWER, NR>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        lds_set = true;
+    }
+    const int nb = (n + NB - 1) / NB;
+    hipLaunchKernelGGL((trsv_inv_chain<T, LOWER, NR>), dim3(nb), dim3(256), lds, ctx().s(), n, nrhs, Lt, Dinv, B, ldb,
+                       g_trsv.p, base, err);
+}
+
+// one launch of the chained solve through inverted diagonal blocks; Lt is the tiled copy
+// of the factors (tile_copy), Dinv the inverted diagonal blocks of the triangle used
+template <typename T, bool LOWER>
+static void trsv_inv(int n, int nrhs, const T* Lt, const T* Dinv, T* B, int ldb) {
     SQ_REQUIRE(nrhs >= 1 && nrhs <= 8, SQBEM_ERR_ARG, "trsv: 1..8 right-hand sides");
-    const int nb = (n + TV - 1) / TV;
+    const int nb = (n + NB - 1) / NB;
     if (!g_trsv.p) {
         g_trsv.alloc(1);
         g_trsv.zero();
@@ -1763,20 +1789,34 @@ static void trsv_skinny(int n, int nrhs, const T* L, int ldl, T* B, int ldb) {
     g_trsv_base += (unsigned long long)nb;
     mark_device_error_check();
     ProfScope prof(PROF_TRSV, (double)n * n * nrhs, sizeof(T) * 0.5 * n * n);
-    const dim3 grid(nb), blk(256);
-    if (nrhs == 1)
-        hipLaunchKernelGGL((trsv_chain<T, LOWER, UNIT, 1>), grid, blk, 0, ctx().s(), n, nrhs, L, ldl, B, ldb, g_trsv.p,
-                           base, err);
-    else if (nrhs == 2)
-        hipLaunchKernelGGL((trsv_chain<T, LOWER, UNIT, 2>), grid, blk, 0, ctx().s(), n, nrhs, L, ldl, B, ldb, g_trsv.p,
-                           base, err);
-    else if (nrhs <= 4)
-        hipLaunchKernelGGL((trsv_chain<T, LOWER, UNIT, 4>), grid, blk, 0, ctx().s(), n, nrhs, L, ldl, B, ldb, g_trsv.p,
-                           base, err);
-    else
-        hipLaunchKernelGGL((trsv_chain<T, LOWER, UNIT, 8>), grid, blk, 0, ctx().s(), n, nrhs, L, ldl, B, ldb, g_trsv.p,
-                           base, err);
-    check_launch("trsv_chain");
+    if (nrhs == 1) trsv_inv_launch<T, LOWER, 1>(n, nrhs, Lt, Dinv, B, ldb, base, err);
+    else if (nrhs == 2) trsv_inv_launch<T, LOWER, 2>(n, nrhs, Lt, Dinv, B, ldb, base, err);
+    else if (nrhs <= 4) trsv_inv_launch<T, LOWER, 4>(n, nrhs, Lt, Dinv, B, ldb, base, err);
+    else trsv_inv_launch<T, LOWER, 8>(n, nrhs, Lt, Dinv, B, ldb, base, err);
+    check_launch("trsv_inv_chain");
+}
+
+// Copy of an n x n matrix as 128 x 128 tiles, tile (I, J) at (I nb + J) 128^2, column-major
+// inside (zero beyond n): a block row of tiles is then one contiguous range, which the
+// chained solves stream far faster than 512-byte pieces of columns 4n bytes apart.
+template <typename T>
+__global__ void __launch_bounds__(256) tile_copy_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ At) {
+    const int nb = (n + NB - 1) / NB;
+    const int I = blockIdx.x % nb, J = blockIdx.x / nb;
+    T* dst = At + ((size_t)I * nb + J) * NB * NB;
+    const int r = threadIdx.x & 127, c1 = threadIdx.x >> 7;
+    const int gi = I * NB + r;
+    for (int c = c1; c < NB; c += 2) {
+        const int gj = J * NB + c;
+        dst[(size_t)c * NB + r] = (gi < n && gj < n) ? A[gi + (size_t)gj * lda] : T(0);
+    }
+}
+
+template <typename T>
+static void tile_copy(int n, const T* A, int lda, T* At) {
+    const int nb = (n + NB - 1) / NB;
+    hipLaunchKernelGGL(tile_copy_kernel<T>, dim3((unsigned)nb * nb), dim3(256), 0, ctx().s(), n, A, lda, At);
+    check_launch("tile_copy");
 }
 
 // ------------------------------------------------------------------ recursive LU
@@ -1812,25 +1852,29 @@ __global__ void iota_pivots_kernel(int n, int* ipiv) {
 constexpr int NP_MIN = 512;
 static int g_last_getrf_path = 0;  // 0 pivoting, 1 no-pivot verified, 2 no-pivot rejected
 
+// aux (optional, getrf_aux_elems(n) elements): receives the solve aid for getrs: the
+// inverted 128 x 128 diagonal blocks of L then of U, then the tiled copy of the factors.
 template <typename T>
-int getrf(int kind, int n, T* A, int lda, int* ipiv) {
+int getrf(int kind, int n, T* A, int lda, int* ipiv, T* aux) {
+    T* inv = aux;
     if (n <= 0) return 0;
     SQ_REQUIRE(lda >= n, SQBEM_ERR_ARG, "getrf: lda < n");
     g_last_getrf_path = 0;
+    const size_t nb = (size_t)(n + NB - 1) / NB;
     if (n >= NP_MIN) {
         g_np.ensure(n, sizeof(T), true);
         T* backup = (T*)g_np.backup.p;
         SQ_HIP(hipMemcpy2DAsync(backup, (size_t)n * sizeof(T), A, (size_t)lda * sizeof(T), (size_t)n * sizeof(T), n,
                                 hipMemcpyDeviceToDevice, ctx().s()));
         g_np.flag.zero();
-        const size_t nb = (size_t)(n + NB - 1) / NB;
-        T* Linv = (T*)g_np.inv.p;
+        T* Linv = inv ? inv : (T*)g_np.inv.p;
         getrf_np<T>(kind, A, lda, n, n, 0, Linv, Linv + nb * NB * NB, g_np.flag.p);
         int bad = 0;
         g_np.flag.download(&bad, 1);
         if (!bad) {
             hipLaunchKernelGGL(iota_pivots_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx().s(), n, ipiv);
             check_launch("iota_pivots");
+            if (aux) tile_copy<T>(n, A, lda, aux + 2 * nb * NB * NB);
             g_last_getrf_path = 1;
             return 0;
         }
@@ -1849,10 +1893,14 @@ int getrf(int kind, int n, T* A, int lda, int* ipiv) {
     int st[2];
     g_panel.status.download(st, 2);
     SQ_REQUIRE(st[1] == 0, SQBEM_ERR_HIP, "getrf: panel barrier timed out");
+    if (aux && st[0] == 0) {
+        lu128_launch<T>((int)nb, A, lda, n, inv, inv + nb * NB * NB, nullptr, 0);
+        tile_copy<T>(n, A, lda, aux + 2 * nb * NB * NB);
+    }
     return st[0];
 }
-template int getrf<double>(int, int, double*, int, int*);
-template int getrf<float>(int, int, float*, int, int*);
+template int getrf<double>(int, int, double*, int, int*, double*);
+template int getrf<float>(int, int, float*, int, int*, float*);
 
 // number of panels that needed row interchanges in the last getrf (0 = diagonal pivots)
 int last_getrf_pivoted_panels(int n) {
@@ -1867,26 +1915,33 @@ int last_getrf_pivoted_panels(int n) {
 
 static DBuf<char> g_getrs_inv;
 
-// B <- A^-1 B (trans = 0) or A^-T B (trans = 1) from getrf factors; ipiv 1-based
+// B <- A^-1 B (trans = 0) or A^-T B (trans = 1) from getrf factors; ipiv 1-based. aux, when
+// given, is getrf's solve aid for these factors (inverted 128 x 128 diagonal blocks of L and
+// U, then the tiled copy of the factors); otherwise what a solve needs is formed here.
 template <typename T>
-void getrs(int kind, int trans, int n, int nrhs, const T* LU, int lda, const int* ipiv, T* B, int ldb) {
+void getrs(int kind, int trans, int n, int nrhs, const T* LU, int lda, const int* ipiv, T* B, int ldb,
+           const T* aux) {
     if (n <= 0 || nrhs <= 0) return;
     if (!trans) {
         laswp<T>(B, ldb, nrhs, ipiv, 0, 1, 0, n, +1);
+        const int nb = (n + NB - 1) / NB;
+        const size_t ninv = 2 * (size_t)nb * NB * NB;
+        if (!aux) {
+            const size_t need = (ninv + (nrhs <= 8 ? (size_t)nb * nb * NB * NB : 0)) * sizeof(T);
+            if (g_getrs_inv.n < need) g_getrs_inv.alloc(need);
+            T* w = (T*)g_getrs_inv.p;
+            lu128_launch<T>(nb, const_cast<T*>(LU), lda, n, w, w + (size_t)nb * NB * NB, nullptr, 0);
+            if (nrhs <= 8) tile_copy<T>(n, LU, lda, w + ninv);
+            aux = w;
+        }
+        const T* Linv = aux;
+        const T* Uinv = aux + (size_t)nb * NB * NB;
         if (nrhs <= 8) {
-            trsv_skinny<T, true, true>(n, nrhs, LU, lda, B, ldb);
-            trsv_skinny<T, false, false>(n, nrhs, LU, lda, B, ldb);
+            trsv_inv<T, true>(n, nrhs, aux + ninv, Linv, B, ldb);
+            trsv_inv<T, false>(n, nrhs, aux + ninv, Uinv, B, ldb);
             return;
         }
-        // inverses of the 128 x 128 diagonal blocks of L and U (one workgroup per block),
-        // then recursive solves whose leaves are single in-place GEMMs
-        const int nb = (n + NB - 1) / NB;
-        const size_t need = 2 * (size_t)nb * NB * NB * sizeof(T);
-        if (g_getrs_inv.n < need) g_getrs_inv.alloc(need);
-        T* Linv = (T*)g_getrs_inv.p;
-        T* Uinv = Linv + (size_t)nb * NB * NB;
-        lu128_launch<T>(nb, const_cast<T*>(LU), lda, n, Linv, Uinv, nullptr, 0);
-        check_launch("lu128 inverses");
+        // recursive solves whose leaves are single in-place GEMMs with the inverses
         trsm_inv<T, true, NB>(kind, n, nrhs, LU, lda, Linv, B, ldb);
         trsm_inv<T, false, NB>(kind, n, nrhs, LU, lda, Uinv, B, ldb);
     } else {
@@ -1896,8 +1951,8 @@ void getrs(int kind, int trans, int n, int nrhs, const T* LU, int lda, const int
         laswp<T>(B, ldb, nrhs, ipiv, 0, 1, 0, n, -1);
     }
 }
-template void getrs<double>(int, int, int, int, const double*, int, const int*, double*, int);
-template void getrs<float>(int, int, int, int, const float*, int, const int*, float*, int);
+template void getrs<double>(int, int, int, int, const double*, int, const int*, double*, int, const double*);
+template void getrs<float>(int, int, int, int, const float*, int, const int*, float*, int, const float*);
 
 }  // namespace sq
 

@@ -30,11 +30,11 @@ void sqbem_lu::solve_device(int nrhs, double* B, int ldb) const {
     if (LU32) {
         DBuf<float> b32((size_t)n * nrhs);
         convert_matrix<double, float>(n, nrhs, B, ldb, b32.p, n);
-        getrs<float>(kind, 0, n, nrhs, LU32, n, piv, b32.p, n);
+        getrs<float>(kind, 0, n, nrhs, LU32, n, piv, b32.p, n, aux32);
         convert_matrix<float, double>(n, nrhs, b32.p, n, B, ldb);
         sync();
     } else {
-        getrs<double>(SQBEM_FP64, 0, n, nrhs, LU, n, piv, B, ldb);
+        getrs<double>(SQBEM_FP64, 0, n, nrhs, LU, n, piv, B, ldb, aux);
     }
 }
 

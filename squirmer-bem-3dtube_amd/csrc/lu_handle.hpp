@@ -9,6 +9,8 @@ struct sqbem_lu {
     const double* LU = nullptr;  // FP64 factors, or
     const float* LU32 = nullptr; // FP32 factors (refined modes)
     const int* piv = nullptr;    // 1-based pivots
+    const double* aux = nullptr;   // getrf's solve aid for LU (or null)
+    const float* aux32 = nullptr;  // same for LU32
     sq::DBuf<double> own;
     sq::DBuf<int> own_piv;
     // B (device, n x nrhs) <- A^-1 B

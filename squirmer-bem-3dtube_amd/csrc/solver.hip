@@ -256,6 +256,8 @@ struct sqbem_tube {
     DBuf<double> T64;  // FP64 mode: LU factors; mixed: the FP64 operator
     DBuf<float> T32;   // mixed: FP32 LU factors
     DBuf<int> ipiv;
+    DBuf<double> aux64;  // getrf's solve aid for the factors (inverted blocks, tiled copy)
+    DBuf<float> aux32;
     sqbem_lu view;  // non-owning handle onto the factors (Tube::lu)
     double t_assembly = 0, t_factor = 0;
 };
@@ -283,9 +285,11 @@ static void tube_build(sqbem_tube* t) {
     if (mixed(t->prec)) {
         t->T32.alloc((size_t)nt * nt);
         convert_matrix<double, float>(nt, nt, t->T64.p, nt, t->T32.p, nt);
-        info = getrf<float>(t->prec, nt, t->T32.p, nt, t->ipiv.p);
+        t->aux32.alloc(getrf_aux_elems(nt));
+        info = getrf<float>(t->prec, nt, t->T32.p, nt, t->ipiv.p, t->aux32.p);
     } else {
-        info = getrf<double>(SQBEM_FP64, nt, t->T64.p, nt, t->ipiv.p);
+        t->aux64.alloc(getrf_aux_elems(nt));
+        info = getrf<double>(SQBEM_FP64, nt, t->T64.p, nt, t->ipiv.p, t->aux64.p);
     }
     SQ_REQUIRE(info == 0, SQBEM_ERR_SINGULAR, "dgetrf failed");
     t->view.n = nt;
@@ -293,6 +297,8 @@ static void tube_build(sqbem_tube* t) {
     t->view.LU = mixed(t->prec) ? nullptr : t->T64.p;
     t->view.LU32 = mixed(t->prec) ? t->T32.p : nullptr;
     t->view.piv = t->ipiv.p;
+    t->view.aux = mixed(t->prec) ? nullptr : t->aux64.p;
+    t->view.aux32 = mixed(t->prec) ? t->aux32.p : nullptr;
     t->t_assembly = t1 - t0;
     t->t_factor = now_s() - t1;
 }
@@ -312,8 +318,10 @@ struct sqbem_swimmer {
     DBuf<double> B, Ct, E, Cs;         // mixed: kept in FP64 for the refinement residual
     DBuf<float> Y32, G32, D32, M32;    // mixed: low-precision Schur pieces and bordered factors
     DBuf<int> Mpiv;
+    DBuf<float> Maux;                  // getrf's solve aid for the M32 factors
     DBuf<float> Gs32;                  // mixed, towed problem: factors of Gs alone
     DBuf<int> Gspiv;
+    DBuf<float> Gsaux;
     double sc32 = 0;
     double t_assembly = 0, t_schur = 0;
     int ir_iters = 0;
@@ -392,7 +400,7 @@ static void swimmer_build(sqbem_swimmer* s) {
     // Schur complement (src/swimmer.hpp:235-249): Y = T^-1 [B | Ct]; [Gs | Gc] -= D Y
     if (!mx) {
         if (tb) {
-            getrs<double>(SQBEM_FP64, 0, nt, ns + 6, tb->T64.p, nt, tb->ipiv.p, s->Y.p, nt);
+            getrs<double>(SQBEM_FP64, 0, nt, ns + 6, tb->T64.p, nt, tb->ipiv.p, s->Y.p, nt, tb->aux64.p);
             dgemm(ns, ns + 6, nt, -1.0, s->D.p, ns, s->Y.p, nt, 1.0, s->G.p, ns);
         }
     } else {
@@ -404,7 +412,7 @@ static void swimmer_build(sqbem_swimmer* s) {
             s->Y32.alloc((size_t)nt * (ns + 6));
             convert_matrix<double, float>(nt, ns, s->B.p, nt, s->Y32.p, nt);
             convert_matrix<double, float>(nt, 6, s->Ct.p, nt, s->Y32.p + (size_t)nt * ns, nt);
-            getrs<float>(kind, 0, nt, ns + 6, tb->T32.p, nt, tb->ipiv.p, s->Y32.p, nt);
+            getrs<float>(kind, 0, nt, ns + 6, tb->T32.p, nt, tb->ipiv.p, s->Y32.p, nt, tb->aux32.p);
             s->D32.alloc((size_t)ns * nt);
             convert_matrix<double, float>(ns, nt, s->D.p, ns, s->D32.p, ns);
             if (kind == SQBEM_FP32_IR)
@@ -425,7 +433,8 @@ static void swimmer_build(sqbem_swimmer* s) {
                            s->dm.cen_xyz.p, s->dm.area.p, s->sc32, s->xc[0], s->xc[1], s->xc[2]);
         check_launch("border");
         s->Mpiv.alloc(n);
-        int info = getrf<float>(kind, n, s->M32.p, n, s->Mpiv.p);
+        s->Maux.alloc(getrf_aux_elems(n));
+        int info = getrf<float>(kind, n, s->M32.p, n, s->Mpiv.p, s->Maux.p);
         SQ_REQUIRE(info == 0, SQBEM_ERR_SINGULAR, "dgesv failed");
     }
     sync();
@@ -439,7 +448,7 @@ static void reduced_rhs(sqbem_swimmer* s, int k, const double* u, double* rs, lo
     gemv<double>(ns, ns, k, -1.0, s->AuBs.p, ns, u, ns, 0.0, rs, ldrs);
     if (s->tube) {
         gemv<double>(nt, ns, k, -1.0, s->AuBt.p, nt, u, ns, 0.0, rt, nt);
-        getrs<double>(SQBEM_FP64, 0, nt, k, s->tube->T64.p, nt, s->tube->ipiv.p, rt, nt);
+        getrs<double>(SQBEM_FP64, 0, nt, k, s->tube->T64.p, nt, s->tube->ipiv.p, rt, nt, s->tube->aux64.p);
         gemv<double>(ns, nt, k, -1.0, s->D.p, ns, rt, nt, 1.0, rs, ldrs);
     }
 }
@@ -535,14 +544,14 @@ static void approx_solve(sqbem_swimmer* s, int k, const double* r, double* d, bo
     // z = T^-1 r_t ; v = [r_s - D z ; r_c]
     if (nt) {
         convert_matrix<double, float>(nt, k, r, N, z32.p, nt);
-        getrs<float>(kind, 0, nt, k, s->tube->T32.p, nt, s->tube->ipiv.p, z32.p, nt);
+        getrs<float>(kind, 0, nt, k, s->tube->T32.p, nt, s->tube->ipiv.p, z32.p, nt, s->tube->aux32.p);
     }
     convert_matrix<double, float>(nv, k, r + nt, N, v32.p, n);
     if (nt) gemv<float>(ns, nt, k, -1.0, s->D32.p, ns, z32.p, nt, 1.0, v32.p, n);
     if (towed)
-        getrs<float>(kind, 0, ns, k, s->Gs32.p, ns, s->Gspiv.p, v32.p, n);
+        getrs<float>(kind, 0, ns, k, s->Gs32.p, ns, s->Gspiv.p, v32.p, n, s->Gsaux.p);
     else
-        getrs<float>(kind, 0, n, k, s->M32.p, n, s->Mpiv.p, v32.p, n);
+        getrs<float>(kind, 0, n, k, s->M32.p, n, s->Mpiv.p, v32.p, n, s->Maux.p);
     // f_t = z - Y [f_s; w]   (towed: w is not an unknown, only the f_s columns of Y)
     if (nt) gemv<float>(nt, nv, k, -1.0, s->Y32.p, nt, v32.p, n, 1.0, z32.p, nt);
     fill_zero(d, sizeof(double) * (size_t)N * k);
@@ -582,7 +591,8 @@ static void solve_mixed(sqbem_swimmer* s, int k, const double* u_host, bool towe
             s->Gs32.alloc((size_t)ns * ns);
             convert_matrix<float, float>(ns, ns, s->G32.p, ns, s->Gs32.p, ns);
             s->Gspiv.alloc(ns);
-            int info = getrf<float>(s->prec, ns, s->Gs32.p, ns, s->Gspiv.p);
+            s->Gsaux.alloc(getrf_aux_elems(ns));
+            int info = getrf<float>(s->prec, ns, s->Gs32.p, ns, s->Gspiv.p, s->Gsaux.p);
             SQ_REQUIRE(info == 0, SQBEM_ERR_SINGULAR, "dgesv failed");
         }
         DBuf<double> w(6);
