@@ -1355,21 +1355,24 @@ static void trsm_inv(int kind, int n, int nrhs, const T* L, int ldl, const T* in
 // the matrix is restored from its copy and factored by the pivoting path.
 //
 // lu128_kernel factors one <= 128 x 128 diagonal block without pivoting and forms the
-// inverses of its unit-lower L and upper U, in one workgroup of 1024 threads, FP64
-// arithmetic. Wave w owns columns 8w .. 8w+7, lane l rows l and l + 64, so every column
-// lives in one wave: at step j the wave owning column j forms the multipliers (the pivot
-// is read out of its own lane, v_readlane) and publishes them through LDS; after the one
-// barrier of the step every wave updates its columns with the multipliers and its own
-// row-j values (again read out of a lane). The factor is then kept in LDS (128 KiB) and
-// each wave inverts L and U for its own eight columns with no further barriers: the
-// solution row it needs is in its own registers and the factor column comes from LDS.
-// The padded part of a smaller block is the identity. Register arrays are indexed with
-// compile-time indices only (selects), so they stay in VGPRs.
+// inverses of its unit-lower L and upper U, in one workgroup of four waves, in FP64. The
+// block lives in LDS (row-major, 132 KiB) for the whole kernel. The factor runs in four
+// 32-column panels. A panel is factored without barriers: every wave holds the panel's top
+// 32 rows (lanes 0-31, the same rows in every wave) and 32 of the rows below it (lanes
+// 32-63), one row per lane in registers, so the pivot row is always in its own wave and is
+// read out of lane j (v_readlane). Then U12 = L11^-1 A12 (one column per thread) and the
+// trailing update A22 -= L21 U12 (register tiles). The inverses follow in LAPACK trtri
+// order in 32 x 32 blocks: the eight diagonal blocks by substitution (wave b, lanes 0-31
+// L's block b, lanes 32-63 U's block b, U taken in reversed order so that both are lower
+// triangular), then each block column from two triangular products. Every loop whose
+// bounds depend on a panel or block index is unrolled with compile-time indices, so the
+// register arrays stay in VGPRs and the loads of a step are independent of its sums.
+// The padded part of a smaller block is the identity.
 constexpr int NB = 128;
 // the in-place GEMMs (L21 = A21 U11^-1, X = inv B) rely on one tile spanning the block
 static_assert(GBM >= NB && GBN >= NB, "GEMM tile narrower than the inverted blocks");
-constexpr int LU_LD = NB + 1;                                   // LDS row stride (doubles)
-constexpr size_t LU_LDS = (size_t)NB * LU_LD * sizeof(double);  // 132 KiB
+constexpr int LU_LD = NB + 1;  // LDS row stride (doubles)
+constexpr int LU_THREADS = 256;
 
 __device__ __forceinline__ double readlane_d(double v, int lane) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
@@ -1377,180 +1380,236 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
     return __hiloint2double(hi, lo);
 }
 
+// Panel P (columns 32P .. 32P+31, rows 32P ..) of the factor. Waves 0 .. NW-1 take part.
+template <int P>
+__device__ __forceinline__ void lu128_panel(double* M, int t, int& mybad) {
+    constexpr int p0 = 32 * P, NREST = 96 - 32 * P, NW = NREST >= 32 ? NREST / 32 : 1;
+    const int lane = t & 63, wv = t >> 6;
+    if (wv >= NW) return;
+    const bool top = lane < 32;
+    const bool valid = top || NREST > 0;
+    const int row = valid ? (top ? p0 + lane : p0 + 32 + 32 * wv + (lane - 32)) : p0;
+    double* Mr = M + row * LU_LD + p0;
+    double a[32];
+#pragma unroll
+    for (int c = 0; c < 32; ++c) a[c] = Mr[c];
+#pragma unroll
+    for (int j = 0; j < 32; ++j) {
+        const double piv = readlane_d(a[j], j);  // lane j holds row p0 + j in every wave
+        if (lane == 0 && !(piv != 0.0 && fabs(piv) <= 1.79e308)) mybad = 1;
+        const bool below = valid && (!top || lane > j);
+        const double lm = below ? a[j] * (1.0 / piv) : 0.0;
+        if (below && !(fabs(lm) <= 1.0)) mybad = 1;
+        a[j] = below ? lm : a[j];
+#pragma unroll
+        for (int c = j + 1; c < 32; ++c) a[c] = fma(-lm, readlane_d(a[c], j), a[c]);  // lm = 0 above
+    }
+    if (valid && (wv == 0 || !top)) {
+#pragma unroll
+        for (int c = 0; c < 32; ++c) Mr[c] = a[c];
+    }
+}
+
+// U12 = L11^-1 A12 for panel P: rows 32P .. 32P+31, one column right of the panel per thread.
+template <int P>
+__device__ __forceinline__ void lu128_u12(double* M, int t) {
+    constexpr int p0 = 32 * P, NC = 96 - 32 * P;
+    if (t >= NC) return;
+    const int col = p0 + 32 + t;
+    double x[32];
+#pragma unroll
+    for (int i = 0; i < 32; ++i) x[i] = M[(p0 + i) * LU_LD + col];
+#pragma unroll
+    for (int i = 1; i < 32; ++i)
+#pragma unroll
+        for (int q = 0; q < i; ++q) x[i] = fma(-M[(p0 + i) * LU_LD + p0 + q], x[q], x[i]);
+#pragma unroll
+    for (int i = 1; i < 32; ++i) M[(p0 + i) * LU_LD + col] = x[i];
+}
+
+// A22 -= L21 U12 after panel P: rows and columns 32P+32 .., inner dimension 32. Thread
+// (ty, tx) of a 16 x 16 grid owns rows ty + 16 ii and columns tx + 16 jj.
+template <int P>
+__device__ __forceinline__ void lu128_update(double* M, int t) {
+    constexpr int p0 = 32 * P, r0 = p0 + 32, R = 96 - 32 * P, MT = R / 16;
+    const int ty = t >> 4, tx = t & 15;
+    double acc[MT][MT];
+#pragma unroll
+    for (int ii = 0; ii < MT; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < MT; ++jj) acc[ii][jj] = M[(r0 + ty + 16 * ii) * LU_LD + r0 + tx + 16 * jj];
+#pragma unroll 4
+    for (int k = 0; k < 32; ++k) {
+        double a[MT], b[MT];
+#pragma unroll
+        for (int ii = 0; ii < MT; ++ii) a[ii] = M[(r0 + ty + 16 * ii) * LU_LD + p0 + k];
+#pragma unroll
+        for (int jj = 0; jj < MT; ++jj) b[jj] = M[(p0 + k) * LU_LD + r0 + tx + 16 * jj];
+#pragma unroll
+        for (int ii = 0; ii < MT; ++ii)
+#pragma unroll
+            for (int jj = 0; jj < MT; ++jj) acc[ii][jj] = fma(-a[ii], b[jj], acc[ii][jj]);
+    }
+#pragma unroll
+    for (int ii = 0; ii < MT; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < MT; ++jj) M[(r0 + ty + 16 * ii) * LU_LD + r0 + tx + 16 * jj] = acc[ii][jj];
+}
+
+// Block column J of the inverse of L (S = 2 - J) or of U (S = J - 1), diagonal blocks
+// already inverted; R = 32 (S + 1) rows. Thread (ty, tx) of a 32 x 8 grid owns rows
+// ty + 32 ii and columns tx + 8 jj.
+//   L: Tmp = X(r0.., r0..) L(r0.., cJ), X unit lower;  X(r0.., cJ) = -Tmp X(J, J), r0 = cJ + 32
+//   U: Tmp = X(0..R, 0..R) U(0..R, cJ), X upper;       X(0..R, cJ)  = -Tmp X(J, J), R = cJ
+template <bool LOWER, int J>
+__device__ __forceinline__ void lu128_inv_column(double* M, double* Tmp, int t) {
+    constexpr int cj = 32 * J, R = LOWER ? 96 - cj : cj, RB = R / 32, r0 = LOWER ? cj + 32 : 0;
+    const int ty = t >> 3, tx = t & 7;
+    double acc[RB][4];
+#pragma unroll
+    for (int ii = 0; ii < RB; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) acc[ii][jj] = 0.0;
+#pragma unroll
+    for (int kk = 0; kk < RB; ++kk) {
+#pragma unroll 8
+        for (int qq = 0; qq < 32; ++qq) {
+            const int q = 32 * kk + qq;  // index within the R x R triangle
+            double b[4];
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) b[jj] = M[(r0 + q) * LU_LD + cj + tx + 8 * jj];
+#pragma unroll
+            for (int ii = 0; ii < RB; ++ii) {
+                // X(i, q), i = ty + 32 ii: only chunks kk <= ii (L) or kk >= ii (U) are nonzero;
+                // in the diagonal chunk the triangle is applied by a select
+                if (LOWER ? kk > ii : kk < ii) continue;
+                const double m = M[(r0 + ty + 32 * ii) * LU_LD + r0 + q];
+                double a = m;
+                if (kk == ii) a = LOWER ? (qq < ty ? m : (qq == ty ? 1.0 : 0.0)) : (qq >= ty ? m : 0.0);
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) acc[ii][jj] = fma(a, b[jj], acc[ii][jj]);
+            }
+        }
+    }
+#pragma unroll
+    for (int ii = 0; ii < RB; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) Tmp[(ty + 32 * ii) * 33 + tx + 8 * jj] = acc[ii][jj];
+    __syncthreads();
+#pragma unroll
+    for (int ii = 0; ii < RB; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) acc[ii][jj] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 32; ++q) {
+        double tq[RB];
+#pragma unroll
+        for (int ii = 0; ii < RB; ++ii) tq[ii] = Tmp[(ty + 32 * ii) * 33 + q];
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            // X(J, J)(q, c), c = tx + 8 jj: unit lower needs q >= c, upper q <= c
+            if (LOWER ? q < 8 * jj : q > 8 * jj + 7) continue;
+            const int c = tx + 8 * jj;
+            const double m = M[(cj + q) * LU_LD + cj + c];
+            const double x = LOWER ? (q > c ? m : (q == c ? 1.0 : 0.0)) : (q <= c ? m : 0.0);
+#pragma unroll
+            for (int ii = 0; ii < RB; ++ii) acc[ii][jj] = fma(tq[ii], x, acc[ii][jj]);
+        }
+    }
+#pragma unroll
+    for (int ii = 0; ii < RB; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) M[(r0 + ty + 32 * ii) * LU_LD + cj + tx + 8 * jj] = -acc[ii][jj];
+    __syncthreads();
+}
+
 // factor = 1: factor the block at A (grid of one), check it, write L\U back, and the
 // inverses. factor = 0: the blocks are already factored; workgroup b inverts diagonal
 // block b (A advanced by b NB (lda + 1), inverses by b NB^2).
-// Every loop over the 128 steps is split at j = 64 so that the half of the rows holding
-// row j (h = j / 64) is a compile-time constant: a run-time choice between a[c][0] and
-// a[c][1] would be folded into an indexed access and move the arrays to scratch memory.
 template <typename T>
-__global__ void __launch_bounds__(1024) lu128_kernel(T* A, int lda, int n, T* Linv, T* Uinv, int* flag, int factor) {
-    extern __shared__ __attribute__((aligned(16))) unsigned char lu_smem[];
-    double* M = reinterpret_cast<double*>(lu_smem);  // NB x LU_LD, row-major: M[i * LU_LD + j]
-    __shared__ double colb[2][NB];
+__global__ void __launch_bounds__(LU_THREADS) lu128_kernel(T* A, int lda, int n, T* Linv, T* Uinv, int* flag,
+                                                           int factor) {
+    __shared__ double M[NB * LU_LD];  // the block, row-major: M[i * LU_LD + j] (132 KiB)
+    __shared__ double Tmp[96 * 33];   // products of the inversion (25 KiB)
     __shared__ int bad;
     const int b = blockIdx.x;
     A += (size_t)b * NB * ((size_t)lda + 1);
     Linv += (size_t)b * NB * NB;
     Uinv += (size_t)b * NB * NB;
     const int w = min(NB, n - b * NB);
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6, c0 = 8 * wv;
-    using H0 = std::integral_constant<int, 0>;
-    using H1 = std::integral_constant<int, 1>;
+    const int t = threadIdx.x;
     if (t == 0) bad = 0;
-    if (factor) {
-        double a[8][2];
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int i = lane + 64 * h, k = c0 + c;
-                a[c][h] = (i < w && k < w) ? (double)A[i + (size_t)k * lda] : (i == k ? 1.0 : 0.0);
-            }
-        __syncthreads();
-        int mybad = 0;
-        auto fstep = [&](auto hjc, int j) {
-            constexpr int HJ = decltype(hjc)::value;
-            const int p = j & 1, lj = j & 63;
-            if (wv == (j >> 3)) {
-                // this wave owns column j: multipliers l_i = a_ij / a_jj for i > j
-                double col[2];
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    double v = 0.0;
-#pragma unroll
-                    for (int c = 0; c < 8; ++c) v = (c == (j & 7)) ? a[c][h] : v;
-                    col[h] = v;
-                }
-                const double piv = readlane_d(col[HJ], lj);
-                if (lane == 0 && !(piv != 0.0 && fabs(piv) <= 1.79e308)) mybad = 1;
-                const double r = 1.0 / piv;
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int i = lane + 64 * h;
-                    const bool below = i > j;
-                    const double l = col[h] * r;
-                    colb[p][i] = l;  // rows <= j are never read back
-                    if (below && !(fabs(l) <= 1.0)) mybad = 1;
-#pragma unroll
-                    for (int c = 0; c < 8; ++c) a[c][h] = (below && c == (j & 7)) ? l : a[c][h];
-                }
-            }
-            __syncthreads();
-            if (c0 + 7 > j) {  // wave-uniform: the wave has columns right of j
-                double u[8];
-#pragma unroll
-                for (int c = 0; c < 8; ++c) u[c] = readlane_d(a[c][HJ], lj);
-#pragma unroll
-                for (int h = HJ; h < 2; ++h) {  // rows below j lie in halves h >= HJ
-                    const int i = lane + 64 * h;
-                    const double lv = colb[p][i];
-#pragma unroll
-                    for (int c = 0; c < 8; ++c) {
-                        const double v = a[c][h] - lv * u[c];
-                        a[c][h] = (i > j && c0 + c > j) ? v : a[c][h];
-                    }
-                }
-            }
-        };
-        for (int j = 0; j < 64; ++j) fstep(H0{}, j);
-        for (int j = 64; j < NB; ++j) fstep(H1{}, j);
-        if (mybad) bad = 1;
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int i = lane + 64 * h, k = c0 + c;
-                M[i * LU_LD + k] = a[c][h];
-                if (i < w && k < w) A[i + (size_t)k * lda] = (T)a[c][h];
-            }
-    } else {
-        for (int e = t; e < NB * NB; e += 1024) {
-            const int i = e % NB, k = e / NB;
-            M[i * LU_LD + k] = (i < w && k < w) ? (double)A[i + (size_t)k * lda] : (i == k ? 1.0 : 0.0);
-        }
+    for (int e = t; e < NB * NB; e += LU_THREADS) {
+        const int i = e & (NB - 1), k = e >> 7;
+        M[i * LU_LD + k] = (i < w && k < w) ? (double)A[i + (size_t)k * lda] : (i == k ? 1.0 : 0.0);
     }
     __syncthreads();
-    // inverses, eight columns per wave, no barriers: X = L^-1 (forward) and X = U^-1 (backward)
-    double x[8][2];
-    auto init_identity = [&]() {
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-#pragma unroll
-            for (int h = 0; h < 2; ++h) x[c][h] = (lane + 64 * h == c0 + c) ? 1.0 : 0.0;
-    };
-    auto store_x = [&](T* X) {
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-#pragma unroll
-            for (int h = 0; h < 2; ++h) X[(lane + 64 * h) + (size_t)(c0 + c) * NB] = (T)x[c][h];
-    };
-    // L^-1 is lower: row j of it is zero right of column j, so only waves with a column <= j
-    // take part; rows below j lie in halves h >= HJ
-    auto lstep = [&](auto hjc, int j) {
-        constexpr int HJ = decltype(hjc)::value;
-        if (c0 > j) return;
-        const int lj = j & 63;
-        double xj[8];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) xj[c] = readlane_d(x[c][HJ], lj);
-#pragma unroll
-        for (int h = HJ; h < 2; ++h) {
-            const int i = lane + 64 * h;
-            const double fv = M[i * LU_LD + j];
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const double v = x[c][h] - fv * xj[c];
-                x[c][h] = i > j ? v : x[c][h];
-            }
-        }
-    };
-    // U^-1 is upper: row j is final once divided by u_jj; only waves with a column >= j take
-    // part; rows above j lie in halves h <= HJ
-    auto ustep = [&](auto hjc, int j) {
-        constexpr int HJ = decltype(hjc)::value;
-        const int lj = j & 63;
-        const double rd = 1.0 / M[j * LU_LD + j];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) x[c][HJ] = lane == lj ? x[c][HJ] * rd : x[c][HJ];
-        if (c0 + 7 < j) return;
-        double xj[8];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) xj[c] = readlane_d(x[c][HJ], lj);
-#pragma unroll
-        for (int h = 0; h <= HJ; ++h) {
-            const int i = lane + 64 * h;
-            const double fv = M[i * LU_LD + j];
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const double v = x[c][h] - fv * xj[c];
-                x[c][h] = i < j ? v : x[c][h];
-            }
-        }
-    };
-    init_identity();
-    for (int j = 0; j < 64; ++j) lstep(H0{}, j);
-    for (int j = 64; j < NB; ++j) lstep(H1{}, j);
-    store_x(Linv);
-    init_identity();
-    for (int j = NB - 1; j >= 64; --j) ustep(H1{}, j);
-    for (int j = 63; j >= 0; --j) ustep(H0{}, j);
-    store_x(Uinv);
     if (factor) {
+        int mybad = 0;
+        lu128_panel<0>(M, t, mybad);
         __syncthreads();
-        if (t == 0 && bad) atomicOr(flag, 1);
+        lu128_u12<0>(M, t);
+        __syncthreads();
+        lu128_update<0>(M, t);
+        __syncthreads();
+        lu128_panel<1>(M, t, mybad);
+        __syncthreads();
+        lu128_u12<1>(M, t);
+        __syncthreads();
+        lu128_update<1>(M, t);
+        __syncthreads();
+        lu128_panel<2>(M, t, mybad);
+        __syncthreads();
+        lu128_u12<2>(M, t);
+        __syncthreads();
+        lu128_update<2>(M, t);
+        __syncthreads();
+        lu128_panel<3>(M, t, mybad);
+        if (mybad) bad = 1;
+        __syncthreads();
+        for (int e = t; e < NB * NB; e += LU_THREADS) {
+            const int i = e & (NB - 1), k = e >> 7;
+            if (i < w && k < w) A[i + (size_t)k * lda] = (T)M[i * LU_LD + k];
+        }
     }
+    {
+        // diagonal blocks in place: lane c of the lower half forms column c of inv(L_bb),
+        // lane c of the upper half column c of inv(Ur) with Ur(i, q) = U_bb(31-i, 31-q)
+        const int lane = t & 63, o = 32 * (t >> 6), c = lane & 31;
+        const bool up = lane >= 32;
+        const int base = up ? (o + 31) * LU_LD + o + 31 : o * LU_LD + o;
+        const int s = up ? -1 : 1;  // element (i, q) of the lower-triangular operand: base + s (i LU_LD + q)
+        double x[32];
+#pragma unroll
+        for (int i = 0; i < 32; ++i) {
+            double v = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+            for (int q = 0; q < i; ++q) v = fma(-M[base + s * (i * LU_LD + q)], x[q], v);
+            x[i] = up ? v / M[base + s * (i * LU_LD + i)] : v;
+        }
+#pragma unroll
+        for (int i = 0; i < 32; ++i)
+            if (up ? i >= c : i > c) M[base + s * (i * LU_LD + c)] = x[i];
+    }
+    __syncthreads();
+    lu128_inv_column<true, 2>(M, Tmp, t);
+    lu128_inv_column<true, 1>(M, Tmp, t);
+    lu128_inv_column<true, 0>(M, Tmp, t);
+    lu128_inv_column<false, 1>(M, Tmp, t);
+    lu128_inv_column<false, 2>(M, Tmp, t);
+    lu128_inv_column<false, 3>(M, Tmp, t);
+    for (int e = t; e < NB * NB; e += LU_THREADS) {
+        const int i = e & (NB - 1), c = e >> 7;  // column-major outputs, consecutive rows a wave
+        const double v = M[i * LU_LD + c];
+        Linv[i + (size_t)c * NB] = (T)(i > c ? v : (i == c ? 1.0 : 0.0));
+        Uinv[i + (size_t)c * NB] = (T)(i <= c ? v : 0.0);
+    }
+    if (factor && t == 0 && bad) atomicOr(flag, 1);
 }
 
 template <typename T>
 static void lu128_launch(int grid, T* A, int lda, int n, T* Linv, T* Uinv, int* flag, int factor) {
-    static bool lds_set = false;  // per instantiation: allow the 132 KiB of dynamic LDS once
-    if (!lds_set) {
-        SQ_HIP(hipFuncSetAttribute((const void*)lu128_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)LU_LDS));
-        lds_set = true;
-    }
-    hipLaunchKernelGGL(lu128_kernel<T>, dim3(grid), dim3(1024), LU_LDS, ctx().s(), A, lda, n, Linv, Uinv, flag,
+    hipLaunchKernelGGL(lu128_kernel<T>, dim3(grid), dim3(LU_THREADS), 0, ctx().s(), A, lda, n, Linv, Uinv, flag,
                        factor);
     check_launch("lu128");
 }
