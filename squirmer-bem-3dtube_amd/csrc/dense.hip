@@ -1539,9 +1539,21 @@ __global__ void __launch_bounds__(LU_THREADS) lu128_kernel(T* A, int lda, int n,
     const int w = min(NB, n - b * NB);
     const int t = threadIdx.x;
     if (t == 0) bad = 0;
-    for (int e = t; e < NB * NB; e += LU_THREADS) {
-        const int i = e & (NB - 1), k = e >> 7;
-        M[i * LU_LD + k] = (i < w && k < w) ? (double)A[i + (size_t)k * lda] : (i == k ? 1.0 : 0.0);
+    // Thread t moves row t % 128 of columns t / 128 + 2 m. Loads are issued 16 at a time
+    // from clamped (always valid) addresses and masked afterwards: a guarded load would be
+    // waited for before the next one is issued.
+    const int ri = t & (NB - 1), rk = t >> 7;
+    const T* Ai = A + min(ri, w - 1);
+#pragma unroll 1
+    for (int m0 = 0; m0 < NB / 2; m0 += 16) {
+        T v[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) v[m] = Ai[(size_t)min(rk + 2 * (m0 + m), w - 1) * lda];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int k = rk + 2 * (m0 + m);
+            M[ri * LU_LD + k] = (ri < w && k < w) ? (double)v[m] : (ri == k ? 1.0 : 0.0);
+        }
     }
     __syncthreads();
     if (factor) {
@@ -1567,9 +1579,16 @@ __global__ void __launch_bounds__(LU_THREADS) lu128_kernel(T* A, int lda, int n,
         lu128_panel<3>(M, t, mybad);
         if (mybad) bad = 1;
         __syncthreads();
-        for (int e = t; e < NB * NB; e += LU_THREADS) {
-            const int i = e & (NB - 1), k = e >> 7;
-            if (i < w && k < w) A[i + (size_t)k * lda] = (T)M[i * LU_LD + k];
+#pragma unroll 1
+        for (int m0 = 0; m0 < NB / 2; m0 += 16) {
+            double v[16];
+#pragma unroll
+            for (int m = 0; m < 16; ++m) v[m] = M[ri * LU_LD + rk + 2 * (m0 + m)];
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int k = rk + 2 * (m0 + m);
+                if (ri < w && k < w) A[ri + (size_t)k * lda] = (T)v[m];
+            }
         }
     }
     {
@@ -1598,11 +1617,17 @@ __global__ void __launch_bounds__(LU_THREADS) lu128_kernel(T* A, int lda, int n,
     lu128_inv_column<false, 1>(M, Tmp, t);
     lu128_inv_column<false, 2>(M, Tmp, t);
     lu128_inv_column<false, 3>(M, Tmp, t);
-    for (int e = t; e < NB * NB; e += LU_THREADS) {
-        const int i = e & (NB - 1), c = e >> 7;  // column-major outputs, consecutive rows a wave
-        const double v = M[i * LU_LD + c];
-        Linv[i + (size_t)c * NB] = (T)(i > c ? v : (i == c ? 1.0 : 0.0));
-        Uinv[i + (size_t)c * NB] = (T)(i <= c ? v : 0.0);
+#pragma unroll 1
+    for (int m0 = 0; m0 < NB / 2; m0 += 16) {  // column-major outputs, consecutive rows a wave
+        double v[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) v[m] = M[ri * LU_LD + rk + 2 * (m0 + m)];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int c = rk + 2 * (m0 + m);
+            Linv[ri + (size_t)c * NB] = (T)(ri > c ? v[m] : (ri == c ? 1.0 : 0.0));
+            Uinv[ri + (size_t)c * NB] = (T)(ri <= c ? v[m] : 0.0);
+        }
     }
     if (factor && t == 0 && bad) atomicOr(flag, 1);
 }
