@@ -178,6 +178,17 @@ int sqbem_swimmer_solve_prescribed(sqbem_swimmer* s, const double* U, const doub
 int sqbem_swimmer_times(const sqbem_swimmer* s, double* t_assembly, double* t_schur,
                         int* ir_iters, double* ir_residual);
 void sqbem_swimmer_free(sqbem_swimmer* s);
+/* Adjoint gradients (no counterpart in the reference, which has no derivatives): for the
+ * free-swimming solution with slip squirmer_slip(c_p, x_c, e, B1, B2) (src/utils/kernels.hpp:
+ * 107-111) and nfun (1..8) functionals J_j = g[6j..6j+5] . (U, Omega), returns U, Om (host,
+ * 3 each), dxc[3*nfun] = dJ/dx_c (sphere centre; the tube stays), de[3*nfun] = dJ/de,
+ * dB[2*nfun] = dJ/d(B1, B2), du[nfun*Ns*3] = dJ/d(slip field). Transposed solves reuse the
+ * tube's LU and the bordered matrix's LU; the centre derivative contracts the translation
+ * derivative of the tube-sphere coupling kernels with the adjoint state. FP64 swimmers
+ * only; any output may be NULL. */
+int sqbem_swimmer_gradient(sqbem_swimmer* s, const double* e, double B1, double B2, int nfun,
+                           const double* g, double* U, double* Om, double* dxc, double* de,
+                           double* dB, double* du);
 
 /* velocity_at (src/utils/field.hpp:18-34) on the device: u, f host [3*np]; pts host
  * [3*npts]; out host [3*npts] */
@@ -199,6 +210,21 @@ int sqbem_trajectory_run(double alpha, double a_over_R, double beta0, double pit
                          double yaw0_deg, double dt, double t_max, double L_over_R, int n_theta,
                          int sphere_level, double beta_stop, double budget_s, int n_r, int precision,
                          const char* out_prefix, int verbose, char* status, int status_len);
+
+/* Trajectory sensitivities (no counterpart in the reference): integrates n_steps steps of
+ * run_trajectory's scheme (src/trajectory.hpp:106-142; RK4 start-up, Adams-Bashforth 4,
+ * renormalised e) from beta0 and the initial tilt pitch0_deg (yaw0_deg = 0 as given), then
+ * differentiates beta_final = |(x, y)| / (R - a) and tilt_final_deg (angle of e from the
+ * tube axis, degrees) with respect to beta0 and pitch0_deg by a discrete adjoint of the
+ * integrator: checkpoints every ckpt_every steps, each segment recomputed forward with its
+ * solvers kept in device memory, then swept backward (one adjoint solve per evaluation).
+ * out[0..1] = beta_final, tilt_final_deg; out[2..5] = d beta_f/d beta0, d beta_f/d pitch0,
+ * d tilt_f/d beta0, d tilt_f/d pitch0; out[6] = forward evaluations including recomputation.
+ * adjoint = 0 runs the forward integration only (out[2..5] = 0). FP64 only. */
+int sqbem_trajectory_gradient(double alpha, double a_over_R, double beta0, double pitch0_deg,
+                              double yaw0_deg, double dt, int n_steps, int ckpt_every,
+                              double L_over_R, int n_theta, int sphere_level, int n_r, int adjoint,
+                              double* out);
 
 #ifdef __cplusplus
 }

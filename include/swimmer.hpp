@@ -15,6 +15,7 @@
 // optional `precision` (SQBEM_FP32_IR, SQBEM_FP16_IR, SQBEM_BF16_IR) factors in low
 // precision and recovers FP64 accuracy by iterative refinement against the FP64 operator.
 #pragma once
+#include <array>
 #include <memory>
 #include <stdexcept>
 #include <vector>
@@ -154,6 +155,39 @@ public:
             f[Nt + p] = s.f_sphere[p];
         }
         return velocity_at(all, u, f, mu, pts);
+    }
+
+    // Adjoint gradients (an addition; the reference has no derivatives). For each weight
+    // vector g_j on (U, Omega): J_j = g_j . (U, Omega) of the free-swimming solution with
+    // slip(e, B1, B2), and dJ_j/d(x_c, e, B1, B2). FP64 swimmers.
+    struct Gradient {
+        V3 dxc, de;
+        double dB1 = 0, dB2 = 0;
+    };
+    struct GradientResult {
+        V3 U, Om;
+        std::vector<Gradient> d;
+    };
+    GradientResult gradient(V3 e, double B1, double B2, const std::vector<std::array<double, 6>>& g) {
+        const int k = (int)g.size();
+        std::vector<double> gw(6 * (size_t)k), dxc(3 * (size_t)k), de(3 * (size_t)k), dB(2 * (size_t)k);
+        for (int j = 0; j < k; ++j)
+            for (int c = 0; c < 6; ++c) gw[6 * (size_t)j + c] = g[j][c];
+        const double ev[3] = {e.x, e.y, e.z};
+        double U[3], Om[3];
+        sqbem_check(sqbem_swimmer_gradient(h_.get(), ev, B1, B2, k, gw.data(), U, Om, dxc.data(), de.data(),
+                                           dB.data(), nullptr));
+        GradientResult r;
+        r.U = V3{U[0], U[1], U[2]};
+        r.Om = V3{Om[0], Om[1], Om[2]};
+        r.d.resize(k);
+        for (int j = 0; j < k; ++j) {
+            r.d[j].dxc = V3{dxc[3 * j], dxc[3 * j + 1], dxc[3 * j + 2]};
+            r.d[j].de = V3{de[3 * j], de[3 * j + 1], de[3 * j + 2]};
+            r.d[j].dB1 = dB[2 * j];
+            r.d[j].dB2 = dB[2 * j + 1];
+        }
+        return r;
     }
 
     const sqbem_swimmer* handle() const { return h_.get(); }

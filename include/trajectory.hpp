@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <memory>
 #include <string>
 
 #include "swimmer.hpp"
@@ -146,6 +147,210 @@ inline std::string run_trajectory(const TrajectoryParams& P, const std::string& 
                     beta_of(y), y[2], nevals, clock.seconds(), nevals ? clock.seconds() / nevals : 0.0);
     if (status != "paused") std::remove(ck.c_str());
     return status;
+}
+
+// ---------------------------------------------------------------- trajectory sensitivities
+// An addition with no counterpart in the reference. n_steps steps of the scheme above from
+// (beta0, pitch0_deg, yaw0_deg), then the derivatives of beta_final = |(x, y)| / (R - a)
+// and tilt_final_deg (angle of e from the tube axis) with respect to beta0 and pitch0_deg by
+// the discrete adjoint of the integrator: the forward pass keeps the state every ckpt_every
+// steps; backward, each segment is recomputed from its checkpoint with every evaluation's
+// solver kept in device memory, and the adjoint state is swept back through AB4 / RK4 /
+// renormalisation, one adjoint solve (SwimmerSolver::gradient) per evaluation.
+struct TrajectoryGradient {
+    double beta_final = 0, tilt_final_deg = 0;
+    double dbeta_dbeta0 = 0, dbeta_dpitch0 = 0, dtilt_dbeta0 = 0, dtilt_dpitch0 = 0;
+    int evaluations = 0;
+};
+
+inline TrajectoryGradient trajectory_gradient(const TrajectoryParams& P, int n_steps, int ckpt_every = 50,
+                                              bool adjoint = true) {
+    using State = std::array<double, 6>;
+    using Adj = std::array<State, 2>;  // adjoint states of (beta_final, tilt_final_deg)
+    if (P.precision != SQBEM_FP64) throw std::runtime_error("trajectory_gradient: FP64 only");
+    if (n_steps < 1 || ckpt_every < 1) throw std::runtime_error("trajectory_gradient: bad step counts");
+    const double a = 1, R = a / P.a_over_R;
+    Tube tube(R, P.L_over_R * R, a, P.n_theta, P.n_r, 1.0, SQBEM_FP64);
+    const Mesh sph0 = icosphere(a, P.sphere_level);
+    int nevals = 0;
+
+    struct Eval {
+        State s{}, d{};
+        V3 U, Om;
+        std::unique_ptr<SwimmerSolver> sw;  // kept for the adjoint sweep
+    };
+    // the evaluation of run_trajectory (same arithmetic)
+    auto evaluate = [&](const State& y, bool keep) {
+        Eval r;
+        r.s = y;
+        const V3 c{y[0], y[1], 0}, e = unit(V3{y[3], y[4], y[5]});
+        Mesh S = sph0;
+        S.translate(c);
+        auto sw = std::make_unique<SwimmerSolver>(&tube, S, c);
+        auto k = sw->solve_free({sw->slip(e, 1, 0), sw->slip(e, 0, 1)});
+        r.U = k[0].U + P.alpha * k[1].U;
+        r.Om = k[0].Om + P.alpha * k[1].Om;
+        const V3 de = cross(r.Om, e);
+        r.d = {r.U.x, r.U.y, r.U.z, de.x, de.y, de.z};
+        if (keep) r.sw = std::move(sw);
+        ++nevals;
+        return r;
+    };
+    // abar . df/ds at an evaluation, f(s) = (U, Omega x e), e = s_e / |s_e|, centre (s0, s1, 0):
+    // abar_U . U + abar_e . (Omega x e) = abar_U . U + (e x abar_e) . Omega, plus the explicit
+    // e-dependence (abar_e x Omega) . de
+    auto vjp = [&](Eval& ev, const Adj& ab) {
+        const V3 se{ev.s[3], ev.s[4], ev.s[5]};
+        const double ne = norm(se);
+        const V3 e = unit(se);
+        std::vector<std::array<double, 6>> g(2);
+        for (int j = 0; j < 2; ++j) {
+            const V3 ae{ab[j][3], ab[j][4], ab[j][5]}, w = cross(e, ae);
+            g[j] = {ab[j][0], ab[j][1], ab[j][2], w.x, w.y, w.z};
+        }
+        const auto gr = ev.sw->gradient(e, 1.0, P.alpha, g);
+        ev.sw.reset();
+        Adj out{};
+        for (int j = 0; j < 2; ++j) {
+            const V3 ae{ab[j][3], ab[j][4], ab[j][5]};
+            const V3 ge = gr.d[j].de + cross(ae, ev.Om);
+            const V3 pe = (1.0 / ne) * (ge - dot(e, ge) * e);
+            out[j] = {gr.d[j].dxc.x, gr.d[j].dxc.y, 0.0, pe.x, pe.y, pe.z};
+        }
+        return out;
+    };
+    struct StepRec {
+        Eval f, k2, k3, k4;
+        State z{};  // state before the renormalisation of e
+        bool rk = false;
+    };
+    // one step of run_trajectory's loop (same arithmetic); rec keeps the evaluations
+    auto step = [&](State& y, std::vector<State>& hist, StepRec* rec) {
+        const bool keep = rec != nullptr;
+        Eval f = evaluate(y, keep);
+        State z{};
+        const bool rk = hist.size() < 3;
+        if (rk) {
+            auto axpy = [](const State& s, double h, const State& d) {
+                State o;
+                for (int i = 0; i < 6; ++i) o[i] = s[i] + h * d[i];
+                return o;
+            };
+            Eval k2 = evaluate(axpy(y, P.dt / 2, f.d), keep);
+            Eval k3 = evaluate(axpy(y, P.dt / 2, k2.d), keep);
+            Eval k4 = evaluate(axpy(y, P.dt, k3.d), keep);
+            for (int i = 0; i < 6; ++i) z[i] = y[i] + P.dt / 6 * (f.d[i] + 2 * k2.d[i] + 2 * k3.d[i] + k4.d[i]);
+            if (rec) {
+                rec->k2 = std::move(k2);
+                rec->k3 = std::move(k3);
+                rec->k4 = std::move(k4);
+            }
+        } else {
+            const State &f1 = hist[2], &f2 = hist[1], &f3 = hist[0];
+            for (int i = 0; i < 6; ++i) z[i] = y[i] + P.dt / 24 * (55 * f.d[i] - 59 * f1[i] + 37 * f2[i] - 9 * f3[i]);
+        }
+        y = z;
+        const double n = std::sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
+        for (int i = 3; i < 6; ++i) y[i] /= n;
+        hist.push_back(f.d);
+        if (hist.size() > 3) hist.erase(hist.begin());
+        if (rec) {
+            rec->z = z;
+            rec->rk = rk;
+            rec->f = std::move(f);
+        }
+    };
+    auto beta_of = [&](const State& y) { return std::hypot(y[0], y[1]) / (R - a); };
+
+    // ---- forward, keeping checkpoints
+    const double b = P.beta0 * (R - a), xi = P.pitch0_deg * PI / 180, chi = P.yaw0_deg * PI / 180;
+    State y = {-b, 0, 0, std::cos(chi) * std::sin(xi), std::sin(chi), std::cos(chi) * std::cos(xi)};
+    struct Ckpt {
+        State y;
+        std::vector<State> hist;
+    };
+    std::vector<Ckpt> ck;
+    std::vector<State> hist;
+    for (int n = 0; n < n_steps; ++n) {
+        if (n % ckpt_every == 0) ck.push_back({y, hist});
+        if (beta_of(y) > P.beta_stop) throw std::runtime_error("trajectory_gradient: wall contact before the last step");
+        step(y, hist, nullptr);
+    }
+    TrajectoryGradient G;
+    const double hyp = std::hypot(y[0], y[1]), rho = std::hypot(y[3], y[4]), ez = y[5];
+    G.beta_final = hyp / (R - a);
+    G.tilt_final_deg = std::atan2(rho, ez) * 180 / PI;
+    if (!adjoint) {
+        G.evaluations = nevals;
+        return G;
+    }
+    if (hyp == 0 || rho == 0) throw std::runtime_error("trajectory_gradient: final state on the axis");
+    // ---- adjoint: d/dy_N of beta_final and tilt_final_deg
+    Adj yb{};
+    yb[0] = {y[0] / (hyp * (R - a)), y[1] / (hyp * (R - a)), 0, 0, 0, 0};
+    const double dd = 180 / PI / (rho * rho + ez * ez);
+    yb[1] = {0, 0, 0, dd * ez * y[3] / rho, dd * ez * y[4] / rho, -dd * rho};
+    std::vector<Adj> Fb(n_steps, Adj{});  // adjoint of f(y_n), the derivative that enters steps n..n+3
+    auto add = [](Adj& dst, double c, const Adj& src) {
+        for (int j = 0; j < 2; ++j)
+            for (int i = 0; i < 6; ++i) dst[j][i] += c * src[j][i];
+    };
+    for (int c = (int)ck.size() - 1; c >= 0; --c) {
+        const int n0 = c * ckpt_every, n1 = std::min(n_steps, n0 + ckpt_every);
+        State ys = ck[c].y;
+        std::vector<State> hs = ck[c].hist;
+        std::vector<StepRec> rec(n1 - n0);
+        for (int n = n0; n < n1; ++n) step(ys, hs, &rec[n - n0]);
+        for (int n = n1 - 1; n >= n0; --n) {
+            StepRec& r = rec[n - n0];
+            // through the renormalisation y_e = z_e / |z_e|
+            const V3 ze{r.z[3], r.z[4], r.z[5]};
+            const double nz = norm(ze);
+            const V3 u = unit(ze);
+            Adj zb = yb;
+            for (int j = 0; j < 2; ++j) {
+                const V3 ye{yb[j][3], yb[j][4], yb[j][5]};
+                const V3 q = (1.0 / nz) * (ye - dot(u, ye) * u);
+                zb[j][3] = q.x;
+                zb[j][4] = q.y;
+                zb[j][5] = q.z;
+            }
+            Adj yn = zb;  // z = y_n + ...
+            if (!r.rk) {
+                const double h = P.dt / 24;
+                add(Fb[n], 55 * h, zb);
+                add(Fb[n - 1], -59 * h, zb);
+                add(Fb[n - 2], 37 * h, zb);
+                add(Fb[n - 3], -9 * h, zb);
+                add(yn, 1.0, vjp(r.f, Fb[n]));
+            } else {
+                Adj k1b = Fb[n], k2b{}, k3b{}, k4b{};
+                add(k1b, P.dt / 6, zb);
+                add(k2b, P.dt / 3, zb);
+                add(k3b, P.dt / 3, zb);
+                add(k4b, P.dt / 6, zb);
+                const Adj s4 = vjp(r.k4, k4b);  // k4 = f(y + dt k3)
+                add(yn, 1.0, s4);
+                add(k3b, P.dt, s4);
+                const Adj s3 = vjp(r.k3, k3b);  // k3 = f(y + dt/2 k2)
+                add(yn, 1.0, s3);
+                add(k2b, P.dt / 2, s3);
+                const Adj s2 = vjp(r.k2, k2b);  // k2 = f(y + dt/2 k1)
+                add(yn, 1.0, s2);
+                add(k1b, P.dt / 2, s2);
+                add(yn, 1.0, vjp(r.f, k1b));
+            }
+            yb = yn;
+        }
+    }
+    // y_0 = (-beta0 (R - a), 0, 0, cos(chi) sin(xi), sin(chi), cos(chi) cos(xi))
+    const double dxi = PI / 180;
+    G.dbeta_dbeta0 = -(R - a) * yb[0][0];
+    G.dtilt_dbeta0 = -(R - a) * yb[1][0];
+    G.dbeta_dpitch0 = dxi * std::cos(chi) * (std::cos(xi) * yb[0][3] - std::sin(xi) * yb[0][5]);
+    G.dtilt_dpitch0 = dxi * std::cos(chi) * (std::cos(xi) * yb[1][3] - std::sin(xi) * yb[1][5]);
+    G.evaluations = nevals;
+    return G;
 }
 
 }  // namespace bem

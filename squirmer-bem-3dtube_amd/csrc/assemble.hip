@@ -354,6 +354,138 @@ __global__ void reduce_tiles_kernel(int nt, int ntiles, int width, const double*
     }
 }
 
+// ------------------------------------------------------------------ centre derivative
+// Contraction of the translation derivative of the coupling blocks with an adjoint vector:
+//   g_k(i) = sigma sum_p sum_q w_q [ a_sl lam_i . (d_k K_SL(r_q)) F_p + a_dl lam_i . (d_k K_DL(r_q)) V ]
+// with r = y_q - x0 (source point minus target), K_SL(r) = I/|r| + r r^T/|r|^3 and
+// K_DL(r) = -6 (r.m) r r^T/|r|^5 (the integrands of accum_kernels, src/utils/kernels.hpp:29-40).
+// sigma = +1 when the source panels move with the sphere centre, -1 when the targets do.
+// V is the source panel's vector (src_v) or the target's own (tgt_v). Quadrature rules and
+// levels are those of the assembly, so this is the exact derivative of the assembled blocks
+// (away from a change of rule level). Self pairs do not occur (tube-sphere couplings only).
+struct AdjArgs {
+    int nt;
+    const double* x0;
+    const double* lam;
+    const double* tgt_v;
+    int p0, np, src_np;
+    const double *V, *nrm, *cen, *area, *h, *q7;
+    const double* src_f;
+    const double* src_v;
+    double a_sl, a_dl, sigma;
+    double* part;
+};
+
+// one quadrature point of the derivative contraction (r = y - x0, w = weight x area)
+__device__ __forceinline__ void dpoint(double* g, double rx, double ry, double rz, double w, const double* L,
+                                       const double* F, const double* Vv, double mx, double my, double mz,
+                                       double a_sl, double a_dl) {
+    const double r2 = rx * rx + ry * ry + rz * rz;
+    const double ir = 1.0 / sqrt(r2);
+    const double ir3 = ir / r2, ir5 = ir3 / r2, ir7 = ir5 / r2;
+    const double rL = rx * L[0] + ry * L[1] + rz * L[2];
+    const double rF = rx * F[0] + ry * F[1] + rz * F[2];
+    const double LF = L[0] * F[0] + L[1] * F[1] + L[2] * F[2];
+    const double rV = rx * Vv[0] + ry * Vv[1] + rz * Vv[2];
+    const double rm = rx * mx + ry * my + rz * mz;
+    const double r[3] = {rx, ry, rz}, m[3] = {mx, my, mz};
+    // lam . d_k K_SL F = -(L.F) r_k/|r|^3 + (L_k (r.F) + F_k (r.L))/|r|^3 - 3 (r.L)(r.F) r_k/|r|^5
+    // lam . d_k K_DL V = -6 (m_k (r.L)(r.V) + (r.m)(L_k (r.V) + V_k (r.L)))/|r|^5
+    //                    + 30 (r.m)(r.L)(r.V) r_k/|r|^7
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double tsl = (-LF * r[k] + L[k] * rF + F[k] * rL) * ir3 - 3.0 * rL * rF * r[k] * ir5;
+        const double tdl = -6.0 * (m[k] * rL * rV + rm * (L[k] * rV + Vv[k] * rL)) * ir5 + 30.0 * rm * rL * rV * r[k] * ir7;
+        g[k] += w * (a_sl * tsl + a_dl * tdl);
+    }
+}
+
+__global__ void __launch_bounds__(TT) adjoint_xc_kernel(AdjArgs a) {
+    __shared__ SrcTile t;
+    __shared__ double tf[TS][3], tv[TS][3];
+    const int tile = blockIdx.y;
+    const int pt0 = a.p0 + tile * TS;
+    const int ns = min(TS, a.p0 + a.np - pt0);
+    const int i = blockIdx.x * TT + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool valid = i < a.nt;
+    AsmArgs la{};
+    la.src_np = a.src_np;
+    la.V = a.V; la.nrm = a.nrm; la.cen = a.cen; la.area = a.area; la.h = a.h; la.q7 = a.q7;
+    load_tile(t, la, pt0, ns);
+    for (int e = threadIdx.x; e < ns * 3; e += blockDim.x) {
+        const int s = e / 3, k = e % 3;
+        tf[s][k] = a.src_f ? a.src_f[3 * (size_t)(pt0 + s) + k] : 0.0;
+        tv[s][k] = a.src_v ? a.src_v[3 * (size_t)(pt0 + s) + k] : 0.0;
+    }
+    __syncthreads();
+    double x0 = 0, y0 = 0, z0 = 0, L[3] = {0, 0, 0}, Vt[3] = {0, 0, 0};
+    if (valid) {
+        x0 = a.x0[3 * (size_t)i];
+        y0 = a.x0[3 * (size_t)i + 1];
+        z0 = a.x0[3 * (size_t)i + 2];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            L[k] = a.lam[3 * (size_t)i + k];
+            Vt[k] = a.tgt_v ? a.tgt_v[3 * (size_t)i + k] : 0.0;
+        }
+    }
+    double g[3] = {0, 0, 0};
+    unsigned near_mask = 0u;
+    if (valid) {
+        for (int s = 0; s < ns; ++s) {
+            const int lev = rule_level(t.cen[s][0], t.cen[s][1], t.cen[s][2], x0, y0, z0, t.h[s]);
+            if (lev != 0) {
+                near_mask |= 1u << s;
+                continue;
+            }
+            const double* Vv = a.tgt_v ? Vt : tv[s];
+#pragma unroll
+            for (int k = 0; k < 7; ++k)
+                dpoint(g, t.q7[s][4 * k] - x0, t.q7[s][4 * k + 1] - y0, t.q7[s][4 * k + 2] - z0, t.q7[s][4 * k + 3], L,
+                       tf[s], Vv, t.nrm[s][0], t.nrm[s][1], t.nrm[s][2], a.a_sl, a.a_dl);
+        }
+    }
+    // near pairs: the wave integrates one queued pair at a time
+    for (;;) {
+        const unsigned long long pending = __ballot(near_mask != 0u);
+        if (pending == 0ull) break;
+        const int owner = __ffsll((long long)pending) - 1;
+        const int s = __shfl(__ffs((int)near_mask) - 1, owner);
+        if (lane == owner) near_mask &= ~(1u << s);
+        const double ox = __shfl(x0, owner), oy = __shfl(y0, owner), oz = __shfl(z0, owner);
+        double oL[3], oV[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            oL[k] = __shfl(L[k], owner);
+            oV[k] = a.tgt_v ? __shfl(Vt[k], owner) : tv[s][k];
+        }
+        const int lev = rule_level(t.cen[s][0], t.cen[s][1], t.cen[s][2], ox, oy, oz, t.h[s]);
+        const int off = c_rule_off[lev], n = c_rule_n[lev];
+        const double A = t.area[s];
+        double q[3] = {0, 0, 0};
+        for (int k = lane; k < n; k += 64) {
+            const double* r = c_rule + 4 * (off + k);
+            const double px = r[0] * t.V[s][0] + r[1] * t.V[s][3] + r[2] * t.V[s][6];
+            const double py = r[0] * t.V[s][1] + r[1] * t.V[s][4] + r[2] * t.V[s][7];
+            const double pz = r[0] * t.V[s][2] + r[1] * t.V[s][5] + r[2] * t.V[s][8];
+            dpoint(q, px - ox, py - oy, pz - oz, r[3] * A, oL, tf[s], oV, t.nrm[s][0], t.nrm[s][1], t.nrm[s][2], a.a_sl,
+                   a.a_dl);
+        }
+        q[0] = wave_sum(q[0]);
+        q[1] = wave_sum(q[1]);
+        q[2] = wave_sum(q[2]);
+        if (lane == owner) {
+            g[0] += q[0];
+            g[1] += q[1];
+            g[2] += q[2];
+        }
+    }
+    if (!valid) return;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) a.part[((size_t)tile * 3 + k) * a.nt + i] = a.sigma * g[k];
+}
+
 // ------------------------------------------------------------------ host side
 static DBuf<double> g_part;
 
@@ -415,6 +547,33 @@ void field(const DMesh& src, const double* pts, int npts, const double* u, const
     check_launch("field");
     hipLaunchKernelGGL(reduce_tiles_kernel, dim3(ceil_div(npts, 256)), dim3(256), 0, ctx().s(), npts, ntiles, 3,
                        a.vel_part, out, 0);
+    check_launch("reduce_tiles");
+}
+
+void adjoint_xc(const DMesh& src, const double* x0, int nt, const double* lam, const double* tgt_v,
+                const double* src_f, const double* src_v, double a_sl, double a_dl, double sigma, double* out) {
+    if (nt <= 0 || src.np <= 0) return;
+    AdjArgs a{};
+    a.nt = nt;
+    a.x0 = x0;
+    a.lam = lam;
+    a.tgt_v = tgt_v;
+    a.p0 = 0;
+    a.np = src.np;
+    a.src_np = src.np;
+    a.V = src.V.p; a.nrm = src.nrm.p; a.cen = src.cen.p; a.area = src.area.p; a.h = src.h.p; a.q7 = src.q7.p;
+    a.src_f = src_f;
+    a.src_v = src_v;
+    a.a_sl = a_sl;
+    a.a_dl = a_dl;
+    a.sigma = sigma;
+    const int ntiles = (int)ceil_div(src.np, TS);
+    a.part = part_buffer((size_t)ntiles * 3 * nt);
+    dim3 grid(ceil_div(nt, TT), ntiles);
+    hipLaunchKernelGGL(adjoint_xc_kernel, grid, dim3(TT), 0, ctx().s(), a);
+    check_launch("adjoint_xc");
+    hipLaunchKernelGGL(reduce_tiles_kernel, dim3(ceil_div(nt, 256)), dim3(256), 0, ctx().s(), nt, ntiles, 3, a.part,
+                       out, 1);
     check_launch("reduce_tiles");
 }
 

@@ -55,4 +55,12 @@ void assemble(const DMesh& src, int p0, int np, const double* x0, int nt, const 
 // tractions f (device, 3 per panel): out (device, 3 npts)
 void field(const DMesh& src, const double* pts, int npts, const double* u, const double* f, double mu, double* out);
 
+// out[3i + k] += sigma sum_p sum_q w_q [a_sl lam_i . d_k K_SL F_p + a_dl lam_i . d_k K_DL V]
+// over the source panels of src for the targets x0 (3 per target): the derivative of the
+// assembled coupling blocks under a translation of the sources (sigma = +1) or of the
+// targets (sigma = -1), contracted with lam (3 per target) and with the source tractions
+// src_f and velocities (src_v per source, or tgt_v per target). Device pointers.
+void adjoint_xc(const DMesh& src, const double* x0, int nt, const double* lam, const double* tgt_v,
+                const double* src_f, const double* src_v, double a_sl, double a_dl, double sigma, double* out);
+
 }  // namespace sq

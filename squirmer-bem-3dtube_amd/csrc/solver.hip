@@ -11,6 +11,7 @@
 #include <cstring>
 #include <vector>
 
+#include "../../include/utils/kernels.hpp"
 #include "assemble.hpp"
 #include "common.hpp"
 #include "dense.hpp"
@@ -453,30 +454,38 @@ static void reduced_rhs(sqbem_swimmer* s, int k, const double* u, double* rs, lo
     }
 }
 
+// LU of the bordered matrix of solve_free (src/swimmer.hpp:107-125): [Gs | Gc] over the
+// force/torque rows scaled by max|Gs| / max(5 A), FP64
+static void bordered_factor_fp64(sqbem_swimmer* s, DBuf<double>& M, DBuf<int>& piv) {
+    const int Ns = s->Ns, ns = s->ns, n = ns + 6;
+    double amax = absmax<double>(ns, ns, s->G.p, ns);
+    double Amax = 0;
+    for (int p = 0; p < Ns; ++p) Amax = std::max(Amax, 5.0 * s->S.area[p]);
+    const double sc = amax / Amax;
+    M.alloc((size_t)n * n);
+    hipLaunchKernelGGL(border_copy_kernel<double>, dim3(std::min<unsigned>(ceil_div((size_t)n * n, 256), 65535)),
+                       dim3(256), 0, ctx().s(), ns, s->G.p, M.p);
+    hipLaunchKernelGGL(border_rows_kernel<double>, dim3(ceil_div(Ns, 256)), dim3(256), 0, ctx().s(), Ns, M.p,
+                       s->dm.cen_xyz.p, s->dm.area.p, sc, s->xc[0], s->xc[1], s->xc[2]);
+    check_launch("border");
+    piv.alloc(n);
+    int info = getrf<double>(SQBEM_FP64, n, M.p, n, piv.p);
+    SQ_REQUIRE(info == 0, SQBEM_ERR_SINGULAR, "dgesv failed");
+}
+
 // FP64 path of solve_free / solve_prescribed (src/swimmer.hpp:107-153)
 static void solve_fp64(sqbem_swimmer* s, int k, const double* u_host, bool towed, const double* w_host, double* X_host,
                        double* ft_host) {
-    const int Ns = s->Ns, ns = s->ns, nt = s->nt, n = ns + 6;
+    const int ns = s->ns, nt = s->nt, n = ns + 6;
     DBuf<double> X((size_t)n * k), rt((size_t)std::max(nt, 1) * k);
     if (!towed) {
         DBuf<double> u((size_t)ns * k);
         u.upload(u_host, (size_t)ns * k);
         X.zero();
         reduced_rhs(s, k, u.p, X.p, n, rt.p);
-        // bordered matrix with the constraint rows scaled by max|Gs| / max(5 A)
-        double amax = absmax<double>(ns, ns, s->G.p, ns);
-        double Amax = 0;
-        for (int p = 0; p < Ns; ++p) Amax = std::max(Amax, 5.0 * s->S.area[p]);
-        const double sc = amax / Amax;
-        DBuf<double> M((size_t)n * n);
-        hipLaunchKernelGGL(border_copy_kernel<double>, dim3(std::min<unsigned>(ceil_div((size_t)n * n, 256), 65535)),
-                           dim3(256), 0, ctx().s(), ns, s->G.p, M.p);
-        hipLaunchKernelGGL(border_rows_kernel<double>, dim3(ceil_div(Ns, 256)), dim3(256), 0, ctx().s(), Ns, M.p,
-                           s->dm.cen_xyz.p, s->dm.area.p, sc, s->xc[0], s->xc[1], s->xc[2]);
-        check_launch("border");
-        DBuf<int> piv(n);
-        int info = getrf<double>(SQBEM_FP64, n, M.p, n, piv.p);
-        SQ_REQUIRE(info == 0, SQBEM_ERR_SINGULAR, "dgesv failed");
+        DBuf<double> M;
+        DBuf<int> piv;
+        bordered_factor_fp64(s, M, piv);
         getrs<double>(SQBEM_FP64, 0, n, k, M.p, n, piv.p, X.p, n);
     } else {
         // towed sphere: Gs f = -Gc w, solved on a copy of Gs (src/swimmer.hpp:142-153)
@@ -630,6 +639,117 @@ static void solve_mixed(sqbem_swimmer* s, int k, const double* u_host, bool towe
         if (towed)
             for (int c = 0; c < 6; ++c) X_host[(size_t)j * n + ns + c] = w_host[c];
         if (ft_host && nt) std::memcpy(ft_host + (size_t)j * nt, hx.data() + (size_t)j * N, sizeof(double) * nt);
+    }
+}
+
+// ------------------------------------------------------------------ adjoint
+// y(j, r) = alpha sum_i A(i, j) x(i, r) + beta y(j, r): one wave per column of A
+__global__ void gemv_t_kernel(int m, int n, int k, double alpha, const double* __restrict__ A, long lda,
+                              const double* __restrict__ X, long ldx, double beta, double* Y, long ldy) {
+    const int col = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
+    if (col >= n) return;  // whole waves
+    const double* a = A + (size_t)col * lda;
+    for (int r = 0; r < k; ++r) {
+        const double* x = X + (size_t)r * ldx;
+        double acc = 0.0;
+        for (int i = lane; i < m; i += 64) acc += a[i] * x[i];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+        if (lane == 0) {
+            double* y = Y + col + (size_t)r * ldy;
+            *y = beta == 0.0 ? alpha * acc : alpha * acc + beta * *y;
+        }
+    }
+}
+static void gemv_t(int m, int n, int k, double alpha, const double* A, long lda, const double* X, long ldx,
+                   double beta, double* Y, long ldy) {
+    if (n <= 0 || k <= 0) return;
+    ProfScope prof(PROF_GEMV, 2.0 * m * n * k, 8.0 * ((double)m * n + (double)m * k + 2.0 * n * k));
+    hipLaunchKernelGGL(gemv_t_kernel, dim3(ceil_div((size_t)n * 64, 256)), dim3(256), 0, ctx().s(), m, n, k, alpha, A,
+                       lda, X, ldx, beta, Y, ldy);
+    check_launch("gemv_t");
+}
+
+// Gradients of nfun functionals J = g_j . (U, Omega) of the free-swimming solution for one
+// slip field u (FP64 swimmer). Full system (unknowns f_t, f_s, w = (U, Omega); rows tube,
+// sphere, constraints):
+//   [ T  B   Ct ] [f_t]   [-AuBt u]
+//   [ D  E   Cs ] [f_s] = [-AuBs u]        Ct = AuBt K, Cs = AuBs K
+//   [ 0  scF 0  ] [ w ]   [   0   ]
+// Adjoint A^T lam = dJ/dz by the same Schur elimination, transposed:
+//   M^T [lam_s; lam_c] = [0; g]   (M the bordered matrix of solve_free, its LU reused)
+//   lam_t = -T^-T D^T lam_s       (the tube LU reused)
+// Then dJ/du = -(AuBt^T lam_t + AuBs^T lam_s) (u enters the right-hand side only), and for
+// the sphere centre x_c, which moves the sphere panels but not the tube,
+//   dJ/dx_c = -lam^T (dA/dx_c z - db/dx_c)
+//           = -[ lam_t . (dB f_s + dAuBt v) + lam_s . (dD f_t + d(AuBs diagonal) v) ],
+// v = K w + u the sphere's surface velocity: the sphere-sphere blocks, K and the
+// constraint rows are invariant under a translation of the sphere.
+static void adjoint_fp64(sqbem_swimmer* s, const double* u_host, int nfun, const double* g_host, double* X_host,
+                         double* ft_host, double* dxc_host, double* du_host) {
+    const int Ns = s->Ns, Nt = s->Nt, ns = s->ns, nt = s->nt, n = ns + 6;
+    const double e8 = bem::EIGHT_PI, mu = s->mu;
+    DBuf<double> u(ns), X(n), rt((size_t)std::max(nt, 1));
+    u.upload(u_host, ns);
+    X.zero();
+    reduced_rhs(s, 1, u.p, X.p, n, rt.p);
+    DBuf<double> M;
+    DBuf<int> piv;
+    bordered_factor_fp64(s, M, piv);
+    getrs<double>(SQBEM_FP64, 0, n, 1, M.p, n, piv.p, X.p, n);
+    if (s->tube) gemv<double>(nt, n, 1, -1.0, s->Y.p, nt, X.p, n, 1.0, rt.p, nt);  // f_t
+    // adjoint of the reduced system
+    std::vector<double> hl((size_t)n * nfun, 0.0);
+    for (int j = 0; j < nfun; ++j)
+        for (int c = 0; c < 6; ++c) hl[(size_t)j * n + ns + c] = g_host[6 * j + c];
+    DBuf<double> lam((size_t)n * nfun), lt((size_t)std::max(nt, 1) * nfun), psi((size_t)ns * nfun);
+    lam.upload(hl.data(), hl.size());
+    getrs<double>(SQBEM_FP64, 1, n, nfun, M.p, n, piv.p, lam.p, n);
+    if (s->tube) {
+        gemv_t(ns, nt, nfun, -1.0, s->D.p, ns, lam.p, n, 0.0, lt.p, nt);
+        getrs<double>(SQBEM_FP64, 1, nt, nfun, s->tube->T64.p, nt, s->tube->ipiv.p, lt.p, nt);
+    }
+    // dJ/du = -(AuBs^T lam_s + AuBt^T lam_t)
+    gemv_t(ns, ns, nfun, -1.0, s->AuBs.p, ns, lam.p, n, 0.0, psi.p, ns);
+    if (s->tube) gemv_t(nt, ns, nfun, -1.0, s->AuBt.p, nt, lt.p, nt, 1.0, psi.p, ns);
+    X.download(X_host, n);
+    if (ft_host && nt) rt.download(ft_host, nt);
+    psi.download(du_host, (size_t)ns * nfun);
+    for (int j = 0; j < 3 * nfun; ++j) dxc_host[j] = 0.0;
+    if (!s->tube) {
+        sync();
+        return;  // no confinement: translation invariant
+    }
+    // sphere surface velocity v = U + Omega x (c - x_c) + u
+    std::vector<double> hv((size_t)ns);
+    const double* w = X_host + ns;
+    for (int p = 0; p < Ns; ++p) {
+        const bem::V3 r = s->S.centroid[p] - bem::V3{s->xc[0], s->xc[1], s->xc[2]};
+        const double om[3] = {w[3], w[4], w[5]};
+        const double cr[3] = {om[1] * r.z - om[2] * r.y, om[2] * r.x - om[0] * r.z, om[0] * r.y - om[1] * r.x};
+        for (int c = 0; c < 3; ++c) hv[3 * (size_t)p + c] = w[c] + cr[c] + u_host[3 * (size_t)p + c];
+    }
+    DBuf<double> v(ns), gt((size_t)3 * Nt), gs((size_t)3 * Ns);
+    v.upload(hv.data(), ns);
+    std::vector<double> ht((size_t)3 * Nt), hs((size_t)3 * Ns);
+    for (int j = 0; j < nfun; ++j) {
+        gt.zero();
+        gs.zero();
+        // tube targets, sphere sources moving with x_c: B = SL/(8 pi mu) on f_s, AuBt = -DL/(8 pi) on v
+        adjoint_xc(s->dm, s->tube->dm.cen_xyz.p, Nt, lt.p + (size_t)j * nt, nullptr, X.p, v.p, 1.0 / (e8 * mu),
+                   -1.0 / e8, +1.0, gt.p);
+        // sphere targets moving with x_c, tube sources: D = SL/(8 pi mu) on f_t, and the tube
+        // part of the completed double layer, +DL/(8 pi) on the target's own v
+        adjoint_xc(s->tube->dm, s->dm.cen_xyz.p, Ns, lam.p + (size_t)j * n, v.p, rt.p, nullptr, 1.0 / (e8 * mu),
+                   1.0 / e8, -1.0, gs.p);
+        gt.download(ht.data(), ht.size());
+        gs.download(hs.data(), hs.size());
+        for (int k = 0; k < 3; ++k) {
+            double acc = 0.0;
+            for (int i = 0; i < Nt; ++i) acc += ht[3 * (size_t)i + k];
+            for (int i = 0; i < Ns; ++i) acc += hs[3 * (size_t)i + k];
+            dxc_host[3 * j + k] = -acc;
+        }
     }
 }
 
@@ -812,6 +932,55 @@ int sqbem_swimmer_solve_prescribed(sqbem_swimmer* s, const double* U, const doub
         else
             solve_fp64(s, 1, nullptr, true, w, X.data(), FT.data());
         unpack_out(s, 1, X, FT, nullptr, nullptr, fs, ft);
+    });
+}
+
+int sqbem_swimmer_gradient(sqbem_swimmer* s, const double* e, double B1, double B2, int nfun, const double* g,
+                           double* U, double* Om, double* dxc, double* de, double* dB, double* du) {
+    return guarded([&] {
+        SQ_REQUIRE(s && e && g && nfun > 0 && nfun <= 8, SQBEM_ERR_ARG, "gradient: 1..8 functionals required");
+        SQ_REQUIRE(!mixed(s->prec), SQBEM_ERR_ARG, "gradient: the adjoint runs on an FP64 swimmer");
+        const int Ns = s->Ns, ns = s->ns, n = ns + 6;
+        const bem::V3 ev{e[0], e[1], e[2]}, xc{s->xc[0], s->xc[1], s->xc[2]};
+        std::vector<double> u(ns), X(n), dxc_h((size_t)3 * nfun), du_h((size_t)ns * nfun);
+        for (int p = 0; p < Ns; ++p) {
+            const bem::V3 v = bem::squirmer_slip(s->S.centroid[p], xc, ev, B1, B2);
+            u[3 * (size_t)p] = v.x;
+            u[3 * (size_t)p + 1] = v.y;
+            u[3 * (size_t)p + 2] = v.z;
+        }
+        adjoint_fp64(s, u.data(), nfun, g, X.data(), nullptr, dxc_h.data(), du_h.data());
+        if (U)
+            for (int c = 0; c < 3; ++c) U[c] = X[ns + c];
+        if (Om)
+            for (int c = 0; c < 3; ++c) Om[c] = X[ns + 3 + c];
+        if (dxc) std::memcpy(dxc, dxc_h.data(), sizeof(double) * 3 * nfun);
+        if (du) std::memcpy(du, du_h.data(), sizeof(double) * ns * nfun);
+        // chain rule through u_p = (B1 + B2 c)(c rh - e), c = rh . e (src/utils/kernels.hpp:107-111)
+        for (int j = 0; j < nfun; ++j) {
+            double ge[3] = {0, 0, 0}, g1 = 0, g2 = 0;
+            const double* gu = du_h.data() + (size_t)j * ns;
+            for (int p = 0; p < Ns; ++p) {
+                const bem::V3 rh = bem::unit(s->S.centroid[p] - xc);
+                const double c = bem::dot(rh, ev);
+                const double r[3] = {rh.x, rh.y, rh.z}, ee[3] = {ev.x, ev.y, ev.z};
+                const double* gp = gu + 3 * (size_t)p;
+                double gt = 0, gr = 0;  // g . (c rh - e), g . rh
+                for (int a = 0; a < 3; ++a) {
+                    gt += gp[a] * (c * r[a] - ee[a]);
+                    gr += gp[a] * r[a];
+                }
+                g1 += gt;
+                g2 += c * gt;
+                for (int k = 0; k < 3; ++k) ge[k] += B2 * r[k] * gt + (B1 + B2 * c) * (r[k] * gr - gp[k]);
+            }
+            if (de)
+                for (int k = 0; k < 3; ++k) de[3 * j + k] = ge[k];
+            if (dB) {
+                dB[2 * j] = g1;
+                dB[2 * j + 1] = g2;
+            }
+        }
     });
 }
 

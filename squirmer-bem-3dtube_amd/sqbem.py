@@ -73,6 +73,8 @@ _SIGS = {
     "sqbem_swimmer_solve_prescribed": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "sqbem_swimmer_times": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "sqbem_swimmer_free": (None, [_vp]),
+    "sqbem_swimmer_gradient": (_i, [_vp, _vp, _d, _d, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sqbem_trajectory_gradient": (_i, [_d, _d, _d, _d, _d, _d, _i, _i, _d, _i, _i, _i, _i, _vp]),
     "sqbem_velocity_at": (_i, [_vp, _vp, _vp, _d, _vp, _i, _vp]),
     "sqbem_duct_create": (_vp, [_vp, _d]),
     "sqbem_duct_solve": (_i, [_vp, _d, _d, _vp, _vp, _vp]),
@@ -474,6 +476,21 @@ class SwimmerSolver:
         self._times()
         return SwimmerSolution(Uv.copy(), Ov.copy(), fs, ft)
 
+    def gradient(self, e, B1, B2, g):
+        """Adjoint gradients (an addition; the reference has no derivatives) of the
+        functionals J_j = g[j] . (U, Omega) of the free-swimming solution with slip
+        (e, B1, B2): returns U, Om and per functional dJ/dx_c, dJ/de, dJ/d(B1, B2) and
+        dJ/d(slip field). FP64 swimmers."""
+        gw = np.ascontiguousarray(np.atleast_2d(np.asarray(g, dtype=np.float64)))
+        k = gw.shape[0]
+        ev = np.ascontiguousarray(e, dtype=np.float64)
+        U, Om = np.zeros(3), np.zeros(3)
+        dxc, de, dB = np.zeros((k, 3)), np.zeros((k, 3)), np.zeros((k, 2))
+        du = np.zeros((k, self.Ns, 3))
+        check(lib().sqbem_swimmer_gradient(self.h_, _ptr(ev), float(B1), float(B2), k, _ptr(gw), _ptr(U), _ptr(Om),
+                                           _ptr(dxc), _ptr(de), _ptr(dB), _ptr(du)))
+        return {"U": U, "Om": Om, "dxc": dxc, "de": de, "dB": dB, "du": du}
+
     def force(self, s):
         """Force on the sphere, F = -sum A f (src/swimmer.hpp:156-160)."""
         F = np.zeros(3)
@@ -576,3 +593,16 @@ def run_trajectory(P: TrajectoryParams, out_prefix: str, verbose: bool = False) 
                                      P.budget_s, P.n_r, P.precision, out_prefix.encode(),
                                      1 if verbose else 0, status, len(status)))
     return status.value.decode()
+
+
+def trajectory_gradient(P: TrajectoryParams, n_steps: int, ckpt_every: int = 50, adjoint: bool = True) -> dict:
+    """Trajectory sensitivities (an addition): n_steps steps of run_trajectory's scheme, then
+    d(beta_final, tilt_final_deg)/d(beta0, pitch0_deg) by the checkpointed discrete adjoint
+    of the integrator (FP64). adjoint=False integrates only."""
+    out = np.zeros(7)
+    check(lib().sqbem_trajectory_gradient(P.alpha, P.a_over_R, P.beta0, P.pitch0_deg, P.yaw0_deg, P.dt, int(n_steps),
+                                          int(ckpt_every), P.L_over_R, P.n_theta, P.sphere_level, P.n_r,
+                                          1 if adjoint else 0, _ptr(out)))
+    return {"beta_final": out[0], "tilt_final_deg": out[1],
+            "dbeta_dbeta0": out[2], "dbeta_dpitch0": out[3], "dtilt_dbeta0": out[4], "dtilt_dpitch0": out[5],
+            "evaluations": int(out[6])}
