@@ -220,11 +220,12 @@ int sqbem_trajectory_run(double alpha, double a_over_R, double beta0, double pit
  * solvers kept in device memory, then swept backward (one adjoint solve per evaluation).
  * out[0..1] = beta_final, tilt_final_deg; out[2..5] = d beta_f/d beta0, d beta_f/d pitch0,
  * d tilt_f/d beta0, d tilt_f/d pitch0; out[6] = forward evaluations including recomputation.
- * adjoint = 0 runs the forward integration only (out[2..5] = 0). FP64 only. */
+ * adjoint = 0 runs the forward integration only (out[2..5] = 0); verbose != 0 prints
+ * progress to stderr. FP64 only. */
 int sqbem_trajectory_gradient(double alpha, double a_over_R, double beta0, double pitch0_deg,
                               double yaw0_deg, double dt, int n_steps, int ckpt_every,
                               double L_over_R, int n_theta, int sphere_level, int n_r, int adjoint,
-                              double* out);
+                              int verbose, double* out);
 
 #ifdef __cplusplus
 }

@@ -164,7 +164,8 @@ struct TrajectoryGradient {
 };
 
 inline TrajectoryGradient trajectory_gradient(const TrajectoryParams& P, int n_steps, int ckpt_every = 50,
-                                              bool adjoint = true) {
+                                              bool adjoint = true, bool verbose = false) {
+    Timer clock;
     using State = std::array<double, 6>;
     using Adj = std::array<State, 2>;  // adjoint states of (beta_final, tilt_final_deg)
     if (P.precision != SQBEM_FP64) throw std::runtime_error("trajectory_gradient: FP64 only");
@@ -272,7 +273,10 @@ inline TrajectoryGradient trajectory_gradient(const TrajectoryParams& P, int n_s
     std::vector<Ckpt> ck;
     std::vector<State> hist;
     for (int n = 0; n < n_steps; ++n) {
-        if (n % ckpt_every == 0) ck.push_back({y, hist});
+        if (n % ckpt_every == 0) {
+            ck.push_back({y, hist});
+            if (verbose && n > 0) std::fprintf(stderr, "forward: step %d of %d, %.1f s\n", n, n_steps, clock.seconds());
+        }
         if (beta_of(y) > P.beta_stop) throw std::runtime_error("trajectory_gradient: wall contact before the last step");
         step(y, hist, nullptr);
     }
@@ -342,6 +346,9 @@ inline TrajectoryGradient trajectory_gradient(const TrajectoryParams& P, int n_s
             }
             yb = yn;
         }
+        if (verbose)
+            std::fprintf(stderr, "adjoint: segment %d of %zu done, %.1f s\n", (int)ck.size() - c, ck.size(),
+                         clock.seconds());
     }
     // y_0 = (-beta0 (R - a), 0, 0, cos(chi) sin(xi), sin(chi), cos(chi) cos(xi))
     const double dxi = PI / 180;

@@ -44,7 +44,8 @@ extern "C" int sqbem_trajectory_run(double alpha, double a_over_R, double beta0,
 
 extern "C" int sqbem_trajectory_gradient(double alpha, double a_over_R, double beta0, double pitch0_deg,
                                          double yaw0_deg, double dt, int n_steps, int ckpt_every, double L_over_R,
-                                         int n_theta, int sphere_level, int n_r, int adjoint, double* out) {
+                                         int n_theta, int sphere_level, int n_r, int adjoint, int verbose,
+                                         double* out) {
     try {
         if (!out) throw std::runtime_error("sqbem_trajectory_gradient: null output");
         bem::TrajectoryParams P;
@@ -59,7 +60,7 @@ extern "C" int sqbem_trajectory_gradient(double alpha, double a_over_R, double b
         P.sphere_level = sphere_level;
         P.n_r = n_r;
         P.precision = SQBEM_FP64;
-        const bem::TrajectoryGradient G = bem::trajectory_gradient(P, n_steps, ckpt_every, adjoint != 0);
+        const bem::TrajectoryGradient G = bem::trajectory_gradient(P, n_steps, ckpt_every, adjoint != 0, verbose != 0);
         out[0] = G.beta_final;
         out[1] = G.tilt_final_deg;
         out[2] = G.dbeta_dbeta0;

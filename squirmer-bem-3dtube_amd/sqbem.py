@@ -74,7 +74,7 @@ _SIGS = {
     "sqbem_swimmer_times": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "sqbem_swimmer_free": (None, [_vp]),
     "sqbem_swimmer_gradient": (_i, [_vp, _vp, _d, _d, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
-    "sqbem_trajectory_gradient": (_i, [_d, _d, _d, _d, _d, _d, _i, _i, _d, _i, _i, _i, _i, _vp]),
+    "sqbem_trajectory_gradient": (_i, [_d, _d, _d, _d, _d, _d, _i, _i, _d, _i, _i, _i, _i, _i, _vp]),
     "sqbem_velocity_at": (_i, [_vp, _vp, _vp, _d, _vp, _i, _vp]),
     "sqbem_duct_create": (_vp, [_vp, _d]),
     "sqbem_duct_solve": (_i, [_vp, _d, _d, _vp, _vp, _vp]),
@@ -595,14 +595,15 @@ def run_trajectory(P: TrajectoryParams, out_prefix: str, verbose: bool = False) 
     return status.value.decode()
 
 
-def trajectory_gradient(P: TrajectoryParams, n_steps: int, ckpt_every: int = 50, adjoint: bool = True) -> dict:
+def trajectory_gradient(P: TrajectoryParams, n_steps: int, ckpt_every: int = 50, adjoint: bool = True,
+                        verbose: bool = False) -> dict:
     """Trajectory sensitivities (an addition): n_steps steps of run_trajectory's scheme, then
     d(beta_final, tilt_final_deg)/d(beta0, pitch0_deg) by the checkpointed discrete adjoint
     of the integrator (FP64). adjoint=False integrates only."""
     out = np.zeros(7)
     check(lib().sqbem_trajectory_gradient(P.alpha, P.a_over_R, P.beta0, P.pitch0_deg, P.yaw0_deg, P.dt, int(n_steps),
                                           int(ckpt_every), P.L_over_R, P.n_theta, P.sphere_level, P.n_r,
-                                          1 if adjoint else 0, _ptr(out)))
+                                          1 if adjoint else 0, 1 if verbose else 0, _ptr(out)))
     return {"beta_final": out[0], "tilt_final_deg": out[1],
             "dbeta_dbeta0": out[2], "dbeta_dpitch0": out[3], "dtilt_dbeta0": out[4], "dtilt_dpitch0": out[5],
             "evaluations": int(out[6])}
