@@ -190,6 +190,26 @@ int sqbem_swimmer_gradient(sqbem_swimmer* s, const double* e, double B1, double 
                            const double* g, double* U, double* Om, double* dxc, double* de,
                            double* dB, double* du);
 
+/* Batched uneven solves (no single counterpart in the reference, whose kinematics sweeps
+ * build one SwimmerSolver per configuration and reuse the Tube, src/main.cpp:82-100):
+ * n independent free-swimming problems, instance i on tube mesh tubes[i] with sphere mesh
+ * spheres[i] (already placed at its centre) centred at centers[3i..3i+2], squirming along
+ * e[3i..3i+2] with modes B[2i], B[2i+1] (slip squirmer_slip at the sphere centroids,
+ * src/utils/kernels.hpp:107-111; solve_free, src/swimmer.hpp:107-139). Tube meshes with
+ * identical contents share one assembly and factorisation. Instances on one tube are
+ * processed in chunks of at most mem_budget_gb of device memory (<= 0: 64 GB): one
+ * persistent assembly launch for all their coupling blocks, one triangular solve through
+ * the shared tube factors for all their Schur right-hand sides, and (refined modes) a
+ * refinement of all their full systems together. Outputs (host): U[3n], Om[3n]; iters[n]
+ * and resid[n] (refinement steps and final relative residual; 0 in FP64 mode; may be
+ * NULL); stats[8] (may be NULL): seconds in tube assembly + factorisation, coupling
+ * assembly, Schur + bordered factors, solves; unique tubes, chunks, most refinement
+ * steps, worst final residual. Every instance needs a tube (tubes[i] != NULL). */
+int sqbem_batch_solve(int n, const sqbem_mesh* const* tubes, const sqbem_mesh* const* spheres,
+                      const double* centers, const double* e, const double* B, double mu, int precision,
+                      double mem_budget_gb, double* U, double* Om, int* iters, double* resid,
+                      double* stats);
+
 /* velocity_at (src/utils/field.hpp:18-34) on the device: u, f host [3*np]; pts host
  * [3*npts]; out host [3*npts] */
 int sqbem_velocity_at(const sqbem_mesh* m, const double* u, const double* f, double mu,

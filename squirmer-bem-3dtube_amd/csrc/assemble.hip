@@ -230,13 +230,13 @@ __device__ __forceinline__ void consume(const AsmArgs& a, const Acc& q, int i, i
     }
 }
 
+// One workgroup's share of an assembly: targets [bx TT, bx TT + TT) against source tile
+// `tile` (TS panels from p0 + tile TS), staged in the LDS tile t.
 template <int MODE>
-__global__ void __launch_bounds__(TT) assemble_kernel(AsmArgs a) {
-    __shared__ SrcTile t;
-    const int tile = blockIdx.y;
+__device__ __forceinline__ void assemble_tile(const AsmArgs& a, SrcTile& t, int bx, int tile) {
     const int pt0 = a.p0 + tile * TS;
     const int ns = min(TS, a.p0 + a.np - pt0);
-    const int i = blockIdx.x * TT + threadIdx.x;
+    const int i = bx * TT + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool valid = i < a.nt;
     load_tile(t, a, pt0, ns);
@@ -338,6 +338,42 @@ __global__ void __launch_bounds__(TT) assemble_kernel(AsmArgs a) {
     if (MODE == MODE_FIELD) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) a.vel_part[((size_t)tile * 3 + k) * a.nt + i] = vel[k];
+    }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(TT) assemble_kernel(AsmArgs a) {
+    __shared__ SrcTile t;
+    assemble_tile<MODE>(a, t, blockIdx.x, blockIdx.y);
+}
+
+// Persistent assembly of many jobs (target set x source panel range, each with its own
+// outputs) in one launch. Work item w in [0, total) is tile w - start[j] of the job j with
+// start[j] <= w < start[j + 1], taken target-tile-fastest; workgroups draw items from an
+// atomic queue (counter zeroed before the launch), so the costly near-field tiles of one
+// job do not hold up the others. Every workgroup leaves once the queue is drained.
+__global__ void __launch_bounds__(TT) assemble_batch_kernel(const AsmArgs* __restrict__ jobs,
+                                                            const long* __restrict__ start, int njobs,
+                                                            unsigned long long* queue) {
+    __shared__ SrcTile t;
+    __shared__ long s_w;
+    const long total = start[njobs];
+    for (;;) {
+        __syncthreads();  // the previous item's readers of t are done
+        if (threadIdx.x == 0) s_w = (long)atomicAdd(queue, 1ull);
+        __syncthreads();
+        const long w = s_w;
+        if (w >= total) break;
+        int lo = 0, hi = njobs - 1;  // last job with start <= w
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (start[mid] <= w) lo = mid;
+            else hi = mid - 1;
+        }
+        const AsmArgs& a = jobs[lo];
+        const long local = w - start[lo];
+        const int gx = (a.nt + TT - 1) / TT;
+        assemble_tile<MODE_BLOCKS>(a, t, (int)(local % gx), (int)(local / gx));
     }
 }
 
@@ -525,6 +561,86 @@ void assemble(const DMesh& src, int p0, int np, const double* x0, int nt, const 
                            a.dls_part, dlsum, 1);
         check_launch("reduce_tiles");
     }
+}
+
+int asm_tiles(int np) { return (int)ceil_div(np, TS); }
+
+// out[9i+k] = (0 + sum_t part1) + sum_t part2: the order of two accumulating reduce_tiles
+__global__ void reduce_dls_batch_kernel(const DlsReduce* __restrict__ jobs) {
+    const DlsReduce& r = jobs[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= r.nt) return;
+    for (int k = 0; k < 9; ++k) {
+        double s = 0.0;
+        for (int t = 0; t < r.ntiles1; ++t) s += r.part1[((size_t)t * 9 + k) * r.nt + i];
+        double o = 0.0 + s;
+        if (r.part2) {
+            double s2 = 0.0;
+            for (int t = 0; t < r.ntiles2; ++t) s2 += r.part2[((size_t)t * 9 + k) * r.nt + i];
+            o = o + s2;
+        }
+        r.out[9 * (size_t)i + k] = o;
+    }
+}
+
+static DBuf<AsmArgs> g_jobs;
+static DBuf<long> g_job_start;
+static DBuf<unsigned long long> g_queue;
+static DBuf<DlsReduce> g_dls_jobs;
+
+void assemble_batch(const std::vector<AsmJob>& jobs) {
+    std::vector<AsmArgs> args;
+    std::vector<long> start(1, 0);
+    double flops = 0, bytes = 0;
+    for (const AsmJob& j : jobs) {
+        if (j.nt <= 0 || j.np <= 0) continue;
+        SQ_REQUIRE(j.p0 >= 0 && j.p0 + j.np <= j.src->np, SQBEM_ERR_ARG, "assemble: source panel range out of bounds");
+        SQ_REQUIRE(!j.SL || j.ld_sl >= 3L * j.nt, SQBEM_ERR_ARG, "assemble: SL leading dimension too small");
+        SQ_REQUIRE(!j.DL || j.ld_dl >= 3L * j.nt, SQBEM_ERR_ARG, "assemble: DL leading dimension too small");
+        AsmArgs a{};
+        a.nt = j.nt;
+        a.x0 = j.x0;
+        a.self_src = j.self_src;
+        a.p0 = j.p0;
+        a.np = j.np;
+        a.src_np = j.src->np;
+        a.V = j.src->V.p; a.nrm = j.src->nrm.p; a.cen = j.src->cen.p; a.area = j.src->area.p; a.h = j.src->h.p;
+        a.q7 = j.src->q7.p;
+        a.sl_div = j.sl_div; a.dl_div = j.dl_div;
+        a.SL = j.SL; a.ld_sl = j.ld_sl; a.DL = j.DL; a.ld_dl = j.ld_dl;
+        a.dls_part = j.dls_part;
+        a.defl_sc = j.defl_sc; a.defl_tn = j.defl_tn;
+        args.push_back(a);
+        start.push_back(start.back() + (long)ceil_div(j.nt, TT) * asm_tiles(j.np));
+        flops += 72.0 * 7.0 * j.nt * j.np;
+        bytes += 8.0 * 9.0 * j.nt * j.np * ((j.SL ? 1 : 0) + (j.DL ? 1 : 0));
+    }
+    if (args.empty()) return;
+    if (g_jobs.n < args.size()) g_jobs.alloc(std::max<size_t>(args.size(), 256));
+    if (g_job_start.n < start.size()) g_job_start.alloc(std::max<size_t>(start.size(), 257));
+    if (!g_queue.p) g_queue.alloc(1);
+    g_jobs.upload(args.data(), args.size());
+    g_job_start.upload(start.data(), start.size());
+    g_queue.zero();
+    const long total = start.back();
+    const unsigned grid = (unsigned)std::min<long>(total, 8L * std::max(ctx().cu_count, 1));
+    ProfScope prof(PROF_ASSEMBLE, flops, bytes);
+    hipLaunchKernelGGL(assemble_batch_kernel, dim3(grid), dim3(TT), 0, ctx().s(), g_jobs.p, g_job_start.p,
+                       (int)args.size(), g_queue.p);
+    check_launch("assemble_batch");
+    sync();  // the job tables are host staging for the next call
+}
+
+void reduce_dls_batch(const std::vector<DlsReduce>& jobs) {
+    if (jobs.empty()) return;
+    int mx = 1;
+    for (const DlsReduce& r : jobs) mx = std::max(mx, r.nt);
+    if (g_dls_jobs.n < jobs.size()) g_dls_jobs.alloc(std::max<size_t>(jobs.size(), 256));
+    g_dls_jobs.upload(jobs.data(), jobs.size());
+    hipLaunchKernelGGL(reduce_dls_batch_kernel, dim3(ceil_div(mx, 256), (unsigned)jobs.size()), dim3(256), 0, ctx().s(),
+                       g_dls_jobs.p);
+    check_launch("reduce_dls_batch");
+    sync();
 }
 
 void field(const DMesh& src, const double* pts, int npts, const double* u, const double* f, double mu, double* out) {

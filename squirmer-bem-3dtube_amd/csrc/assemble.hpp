@@ -1,6 +1,8 @@
 // assemble.hpp -- internal interface of the panel-integral kernels (assemble.hip).
 #pragma once
 #include "../../include/utils/mesh.hpp"
+#include <vector>
+
 #include "common.hpp"
 
 namespace sq {
@@ -50,6 +52,40 @@ struct AsmArgs {
 void assemble(const DMesh& src, int p0, int np, const double* x0, int nt, const int* self_src, double sl_div,
               double* SL, long ld_sl, double dl_div, double* DL, long ld_dl, double* dlsum, double defl_sc,
               const double* defl_tn);
+
+// One job of a batched assembly: the arguments of assemble(), except that the stresslet
+// row sums stay per source tile in dls_part (asm_tiles(np) x 9 x nt doubles, or null) for
+// reduce_dls_batch.
+struct AsmJob {
+    const DMesh* src;
+    int p0, np;
+    const double* x0;
+    int nt;
+    const int* self_src;
+    double sl_div;
+    double* SL;
+    long ld_sl;
+    double dl_div;
+    double* DL;
+    long ld_dl;
+    double* dls_part;
+    double defl_sc;
+    const double* defl_tn;
+};
+int asm_tiles(int np);
+// all jobs in one persistent launch (work queue over every job's workgroup tiles)
+void assemble_batch(const std::vector<AsmJob>& jobs);
+// out[9i + k] = (sum over tiles of part1) + (sum over tiles of part2), tiles in order (the
+// row sums of two consecutive assemble() calls into one zeroed dlsum); part2 may be null
+struct DlsReduce {
+    int nt;
+    int ntiles1;
+    const double* part1;
+    int ntiles2;
+    const double* part2;
+    double* out;
+};
+void reduce_dls_batch(const std::vector<DlsReduce>& jobs);
 
 // velocity at npts points (device, xyz interleaved) from panel velocities u and
 // tractions f (device, 3 per panel): out (device, 3 npts)

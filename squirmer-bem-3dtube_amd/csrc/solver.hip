@@ -17,14 +17,14 @@
 #include "dense.hpp"
 #include "lu_handle.hpp"
 #include "mesh_handle.hpp"
+#include "solver.hpp"
 
 namespace sq {
 
-static double now_s() {
+double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-static bool mixed(int prec) { return prec == SQBEM_FP32_IR || prec == SQBEM_FP16_IR || prec == SQBEM_BF16_IR; }
 
 // ------------------------------------------------------------------ small kernels
 __global__ void iota_kernel(int n, int* a) {
@@ -122,7 +122,7 @@ __global__ void border_rows_kernel(int Ns, T* M, const double* __restrict__ cen,
 // res_c(a, k) = -sc sum_p F rows . f_s (constraint residual of the bordered system)
 __global__ void constraint_apply_kernel(int Ns, int nrhs, const double* __restrict__ fs, long ldf,
                                         const double* __restrict__ cen, const double* __restrict__ area, double sc,
-                                        double xcx, double xcy, double xcz, double* out /* 6 x nrhs */) {
+                                        double xcx, double xcy, double xcz, double sign, double* out, long ldo) {
     // one workgroup per right-hand side, 256 threads
     const int k = blockIdx.x;
     double acc[6] = {0, 0, 0, 0, 0, 0};
@@ -143,7 +143,7 @@ __global__ void constraint_apply_kernel(int Ns, int nrhs, const double* __restri
             for (int c = 0; c < 6; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x < 6) out[threadIdx.x + 6 * k] = red[threadIdx.x][0];
+    if (threadIdx.x < 6) out[threadIdx.x + ldo * k] = sign * red[threadIdx.x][0];
 }
 
 // y = alpha A x + beta y for a few right-hand sides (nrhs <= 8), split over column chunks
@@ -180,7 +180,7 @@ __global__ void gemv_reduce_kernel(int m, int nrhs, const double* __restrict__ p
 
 static DBuf<double> g_gv;
 template <typename T>
-static void gemv(int m, int n, int nrhs, double alpha, const T* A, long lda, const T* X, long ldx, double beta, T* Y,
+void gemv(int m, int n, int nrhs, double alpha, const T* A, long lda, const T* X, long ldx, double beta, T* Y,
                  long ldy) {
     if (m <= 0 || nrhs <= 0) return;
     SQ_REQUIRE(nrhs <= 8, SQBEM_ERR_ARG, "gemv: at most 8 right-hand sides");
@@ -243,27 +243,64 @@ static void kproj(int nrows, int Ns, const double* A, long lda, const double* ce
     sync();  // part is released on return
 }
 
+template void gemv<double>(int, int, int, double, const double*, long, const double*, long, double, double*, long);
+template void gemv<float>(int, int, int, double, const float*, long, const float*, long, double, float*, long);
+
+void completion_diag(int Ns, double* A, long lda, const double* dls, double minus_identity) {
+    if (Ns <= 0) return;
+    hipLaunchKernelGGL(completion_diag_kernel, dim3(ceil_div(Ns, 256)), dim3(256), 0, ctx().s(), Ns, A, lda, dls,
+                       bem::EIGHT_PI, minus_identity);
+    check_launch("completion_diag");
+}
+
+size_t kproj_part_elems(int nrows) { return (size_t)KP_CHUNKS * 6 * std::max(nrows, 1); }
+
+void kproj_async(int nrows, int Ns, const double* A, long lda, const double* cen_xyz, const double* xc, double* C,
+                 long ldc, double* part) {
+    if (nrows <= 0) return;
+    hipLaunchKernelGGL(kproj_kernel, dim3(ceil_div(nrows, 256), KP_CHUNKS), dim3(256), 0, ctx().s(), nrows, Ns, A, lda,
+                       cen_xyz, xc[0], xc[1], xc[2], part);
+    hipLaunchKernelGGL(kproj_reduce, dim3(ceil_div(nrows, 256)), dim3(256), 0, ctx().s(), nrows, part, C, ldc);
+    check_launch("kproj");
+}
+
+template <typename T>
+void absmax_async(int m, int n, const T* A, long lda, unsigned long long* out) {
+    const size_t total = (size_t)m * n;
+    if (!total) return;
+    unsigned blocks = (unsigned)std::min<size_t>(ceil_div(total, 256), 4096);
+    hipLaunchKernelGGL(absmax_kernel_t<T>, dim3(blocks), dim3(256), 0, ctx().s(), m, n, A, lda, out);
+    check_launch("absmax");
+}
+template void absmax_async<double>(int, int, const double*, long, unsigned long long*);
+template void absmax_async<float>(int, int, const float*, long, unsigned long long*);
+
+template <typename T>
+void border_build(int Ns, const T* G, T* M, const double* cen_xyz, const double* area, double sc, const double* xc) {
+    const int ns = 3 * Ns, n = ns + 6;
+    hipLaunchKernelGGL(border_copy_kernel<T>, dim3(std::min<unsigned>(ceil_div((size_t)n * n, 256), 65535)), dim3(256),
+                       0, ctx().s(), ns, G, M);
+    hipLaunchKernelGGL(border_rows_kernel<T>, dim3(ceil_div(Ns, 256)), dim3(256), 0, ctx().s(), Ns, M, cen_xyz, area, sc,
+                       xc[0], xc[1], xc[2]);
+    check_launch("border");
+}
+template void border_build<double>(int, const double*, double*, const double*, const double*, double, const double*);
+template void border_build<float>(int, const float*, float*, const double*, const double*, double, const double*);
+
+void constraint_apply(int Ns, int nrhs, const double* fs, long ldf, const double* cen_xyz, const double* area,
+                      double sc, const double* xc, double* out, long ldo) {
+    if (nrhs <= 0) return;
+    hipLaunchKernelGGL(constraint_apply_kernel, dim3(nrhs), dim3(256), 0, ctx().s(), Ns, nrhs, fs, ldf, cen_xyz, area,
+                       sc, xc[0], xc[1], xc[2], -1.0, out, ldo);
+    check_launch("constraint_apply");
+}
+
 }  // namespace sq
 
 using namespace sq;
 
 // ------------------------------------------------------------------ tube
-struct sqbem_tube {
-    bem::Mesh mesh;
-    DMesh dm;
-    double mu = 1;
-    int prec = SQBEM_FP64;
-    int Nt = 0, nt = 0;
-    DBuf<double> T64;  // FP64 mode: LU factors; mixed: the FP64 operator
-    DBuf<float> T32;   // mixed: FP32 LU factors
-    DBuf<int> ipiv;
-    DBuf<double> aux64;  // getrf's solve aid for the factors (inverted blocks, tiled copy)
-    DBuf<float> aux32;
-    sqbem_lu view;  // non-owning handle onto the factors (Tube::lu)
-    double t_assembly = 0, t_factor = 0;
-};
-
-static void tube_build(sqbem_tube* t) {
+void sq::tube_build(sqbem_tube* t) {
     double t0 = now_s();
     t->Nt = t->mesh.P();
     t->nt = 3 * t->Nt;
@@ -537,7 +574,7 @@ static void full_residual(sqbem_swimmer* s, int k, const double* x, const double
         gemv<double>(ns, 6, k, -1.0, s->Cs.p, ns, xw, N, 1.0, rs, N);
         DBuf<double> c((size_t)6 * k);
         hipLaunchKernelGGL(constraint_apply_kernel, dim3(k), dim3(256), 0, ctx().s(), s->Ns, k, xs, (long)N,
-                           s->dm.cen_xyz.p, s->dm.area.p, s->sc32, s->xc[0], s->xc[1], s->xc[2], c.p);
+                           s->dm.cen_xyz.p, s->dm.area.p, s->sc32, s->xc[0], s->xc[1], s->xc[2], 1.0, c.p, 6L);
         check_launch("constraint_apply");
         for (int j = 0; j < k; ++j) axpy(6, -1.0, c.p + 6 * j, rc + (size_t)j * N);
         sync();  // c is released on return

@@ -1,0 +1,61 @@
+// solver.hpp -- internals shared by the single-instance solvers (solver.hip) and the
+// batched solver (batch.hip): the tube object and the small device helpers of the Schur
+// complement and of the bordered force/torque-free system.
+#pragma once
+#include "../../include/utils/mesh.hpp"
+#include "assemble.hpp"
+#include "common.hpp"
+#include "lu_handle.hpp"
+
+// Tube::Tube + Tube::factor (src/swimmer.hpp:40-60): FP64 operator and its factors in HBM.
+struct sqbem_tube {
+    bem::Mesh mesh;
+    sq::DMesh dm;
+    double mu = 1;
+    int prec = SQBEM_FP64;
+    int Nt = 0, nt = 0;
+    sq::DBuf<double> T64;  // FP64 mode: LU factors; mixed: the FP64 operator
+    sq::DBuf<float> T32;   // mixed: FP32 LU factors
+    sq::DBuf<int> ipiv;
+    sq::DBuf<double> aux64;  // getrf's solve aid for the factors (inverted blocks, tiled copy)
+    sq::DBuf<float> aux32;
+    sqbem_lu view;  // non-owning handle onto the factors (Tube::lu)
+    double t_assembly = 0, t_factor = 0;
+};
+
+namespace sq {
+
+double now_s();
+inline bool mixed(int prec) { return prec == SQBEM_FP32_IR || prec == SQBEM_FP16_IR || prec == SQBEM_BF16_IR; }
+
+// assembles, deflates and factors t->mesh (t->mu, t->prec set by the caller)
+void tube_build(sqbem_tube* t);
+
+// Y = alpha A X + beta Y for at most 8 right-hand sides (FP64 sums)
+template <typename T>
+void gemv(int m, int n, int nrhs, double alpha, const T* A, long lda, const T* X, long ldx, double beta, T* Y,
+          long ldy);
+
+// AuBs(3i+a, 3i+b) += dls_i[3a+b] / (8 pi) - minus_identity delta_ab (completed double layer)
+void completion_diag(int Ns, double* A, long lda, const double* dls, double minus_identity);
+
+// C = A K for the rigid-body map K about xc (host), columns of C at ldc; part: workspace of
+// kproj_part_elems(nrows) doubles. No synchronisation.
+size_t kproj_part_elems(int nrows);
+void kproj_async(int nrows, int Ns, const double* A, long lda, const double* cen_xyz, const double* xc, double* C,
+                 long ldc, double* part);
+
+// *out = max(*out, max |A(i,j)|) as the bit pattern of a non-negative double (no download)
+template <typename T>
+void absmax_async(int m, int n, const T* A, long lda, unsigned long long* out);
+
+// bordered matrix M ((ns + 6)^2) of solve_free from G = [Gs | Gc] (ns x (ns + 6)) and the
+// force/torque rows scaled by sc (src/swimmer.hpp:107-125)
+template <typename T>
+void border_build(int Ns, const T* G, T* M, const double* cen_xyz, const double* area, double sc, const double* xc);
+
+// out(0..5, k) = -sc F f_s(:, k): the constraint rows applied to sphere tractions
+void constraint_apply(int Ns, int nrhs, const double* fs, long ldf, const double* cen_xyz, const double* area,
+                      double sc, const double* xc, double* out, long ldo);
+
+}  // namespace sq

@@ -75,6 +75,7 @@ _SIGS = {
     "sqbem_swimmer_free": (None, [_vp]),
     "sqbem_swimmer_gradient": (_i, [_vp, _vp, _d, _d, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sqbem_trajectory_gradient": (_i, [_d, _d, _d, _d, _d, _d, _i, _i, _d, _i, _i, _i, _i, _i, _vp]),
+    "sqbem_batch_solve": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _d, _i, _d, _vp, _vp, _vp, _vp, _vp]),
     "sqbem_velocity_at": (_i, [_vp, _vp, _vp, _d, _vp, _i, _vp]),
     "sqbem_duct_create": (_vp, [_vp, _d]),
     "sqbem_duct_solve": (_i, [_vp, _d, _d, _vp, _vp, _vp]),
@@ -511,6 +512,89 @@ class SwimmerSolver:
         u[Nt:] = s.U + np.cross(s.Om, r) + np.asarray(slip_field)
         f[Nt:] = s.f_sphere
         return velocity_at(allm, u, f, self.mu, pts)
+
+
+@dataclass
+class BatchInstance:
+    """One configuration of a batched solve: tube and sphere meshes (the sphere placed at
+    its centre), centre, squirming direction e and modes B1, B2."""
+    tube: Mesh
+    sphere: Mesh
+    center: np.ndarray
+    e: np.ndarray
+    B1: float = 1.0
+    B2: float = 0.0
+
+
+@dataclass
+class BatchResult:
+    U: np.ndarray       # (n, 3)
+    Om: np.ndarray      # (n, 3)
+    iters: np.ndarray   # (n,) refinement steps (0 in FP64 mode)
+    resid: np.ndarray   # (n,) final relative residual of the full system
+    stats: dict
+
+
+def batch_solve(instances, mu=1.0, precision=FP64, mem_budget_gb=0.0) -> BatchResult:
+    """Free-swimming U, Omega of many independent configurations in one call (an addition:
+    the reference loops over SwimmerSolver, src/main.cpp:82-100). Each result equals
+    SwimmerSolver(Tube(mesh=inst.tube), inst.sphere, inst.center).solve_free([slip(e, B1, B2)])
+    to the solver's accuracy; identical tube meshes share one factorisation."""
+    n = len(instances)
+    tubes = (ctypes.c_void_p * max(n, 1))(*[q.tube.h_ if q.tube is not None else None for q in instances])
+    spheres = (ctypes.c_void_p * max(n, 1))(*[q.sphere.h_ for q in instances])
+    c = np.ascontiguousarray(np.array([np.asarray(q.center, dtype=np.float64) for q in instances]).reshape(-1, 3))
+    e = np.ascontiguousarray(np.array([np.asarray(q.e, dtype=np.float64) for q in instances]).reshape(-1, 3))
+    B = np.ascontiguousarray(np.array([[q.B1, q.B2] for q in instances], dtype=np.float64).reshape(-1, 2))
+    U = np.zeros((n, 3))
+    Om = np.zeros((n, 3))
+    it = np.zeros(n, dtype=np.int32)
+    res = np.zeros(n)
+    st = np.zeros(8)
+    check(lib().sqbem_batch_solve(n, ctypes.cast(tubes, ctypes.c_void_p), ctypes.cast(spheres, ctypes.c_void_p),
+                                  _ptr(c), _ptr(e), _ptr(B), mu, precision, float(mem_budget_gb), _ptr(U), _ptr(Om),
+                                  _ptr(it), _ptr(res), _ptr(st)))
+    stats = {"tube_s": st[0], "assembly_s": st[1], "schur_s": st[2], "solve_s": st[3], "unique_tubes": int(st[4]),
+             "chunks": int(st[5]), "max_iters": int(st[6]), "worst_residual": st[7]}
+    return BatchResult(U, Om, it, res, stats)
+
+
+def draw_batch_params(n=256, seed=0):
+    """Random uneven configurations: a/R ~ U[0.1, 0.7], radial offset b/R ~ U[0, 0.8 (1 - a/R)],
+    B2/B1 ~ U[-5, 5] (B1 = 1), e uniform on the unit sphere, sphere level from {2, 3}, tube
+    n_theta from {16, 24, 32} with n_z = 2 n_theta, L/R = 4, n_r = n_theta / 4; numpy's
+    default_rng(seed), drawn in that order."""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0.1, 0.7, n)
+    b = rng.uniform(0.0, 1.0, n) * 0.8 * (1.0 - a)
+    alpha = rng.uniform(-5.0, 5.0, n)
+    e = rng.standard_normal((n, 3))
+    e /= np.linalg.norm(e, axis=1, keepdims=True)
+    level = rng.choice([2, 3], n)
+    nth = rng.choice([16, 24, 32], n)
+    return [dict(a=float(a[i]), b=float(b[i]), alpha=float(alpha[i]), e=e[i].copy(), level=int(level[i]),
+                 n_theta=int(nth[i]), n_z=2 * int(nth[i]), L=4.0, n_r=int(nth[i]) // 4) for i in range(n)]
+
+
+def batch_instances(params, R=1.0, graded=False):
+    """BatchInstance list for draw_batch_params output. graded=False: uniform axial nodes
+    (n_z = 2 n_theta over [-L/2, L/2]), so the tube depends on n_theta only and the batch
+    holds three distinct tubes; graded=True: the reference's swimmer_tube_mesh, graded
+    around each sphere (one tube per configuration, n_z then follows from the grading)."""
+    tubes = {}
+    out = []
+    for q in params:
+        if graded:
+            tm = swimmer_tube_mesh(R, q["L"] * R, q["a"] * R, q["n_theta"], q["n_r"])
+        else:
+            key = (q["n_theta"], q["n_z"], q["L"], q["n_r"])
+            if key not in tubes:
+                z = np.array([-q["L"] * R / 2 + q["L"] * R * k / q["n_z"] for k in range(q["n_z"] + 1)])
+                tubes[key] = tube_mesh(R, z, q["n_theta"], q["n_r"])
+            tm = tubes[key]
+        c = np.array([q["b"] * R, 0.0, 0.0])
+        out.append(BatchInstance(tm, icosphere(q["a"] * R, q["level"], c), c, np.asarray(q["e"]), 1.0, q["alpha"]))
+    return out
 
 
 def velocity_at(M, u, f, mu, pts):
