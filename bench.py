@@ -206,6 +206,47 @@ def field_bench(pkg):
     return out
 
 
+def batch_bench(pkg, n=256, seed=0, modes=(("fp32+ir", 1), ("fp16+ir", 2)), check=4):
+    """Batched uneven solves: n configurations drawn from seed (a/R ~ U[0.1, 0.7], b/R ~
+    U[0, 0.8 (1 - a/R)], B2/B1 ~ U[-5, 5], random e, sphere level {2, 3}, tube n_theta
+    {16, 24, 32} with uniform n_z = 2 n_theta, L/R = 4, n_r = n_theta/4), solved in one
+    sqbem_batch_solve call per mode (meshes built on the host beforehand). Checked: the
+    smallest, the largest and a few random instances against single-instance FP64 solves."""
+    params = pkg.draw_batch_params(n, seed=seed)
+    inst = pkg.batch_instances(params)
+    dof = np.array([3 * (q.tube.P() + q.sphere.P()) + 6 for q in inst])
+    out = {"workload": f"{n} configurations from seed {seed}: a/R U[0.1,0.7], b/R U[0,0.8(1-a/R)], B2/B1 U[-5,5], "
+                       "random e, level {2,3}, n_theta {16,24,32}, n_z=2 n_theta uniform, L/R=4, n_r=n_theta/4",
+           "instances": n, "dof_min": int(dof.min()), "dof_max": int(dof.max()), "dof_total": int(dof.sum())}
+    res = {}
+    for name, prec in modes:
+        pkg.sync()
+        t0 = time.perf_counter()
+        r = pkg.batch_solve(inst, precision=prec)
+        t = time.perf_counter() - t0
+        res[name] = r
+        out[name] = {"s": t, "instances_per_s": n / t, "dof_per_s": float(dof.sum()) / t, **r.stats,
+                     "iters_max": int(r.iters.max()), "resid_max": float(r.resid.max())}
+    rng = np.random.default_rng(1)
+    pick = [int(np.argmin(dof)), int(np.argmax(dof))] + [int(i) for i in rng.choice(n, size=max(check - 2, 0),
+                                                                                    replace=False)]
+    errs = {name: [] for name in res}
+    for i in pick:
+        q, pr = inst[i], params[i]
+        t = pkg.Tube(1.0, pr["L"], pr["a"], pr["n_theta"], pr["n_r"], mesh=q.tube)
+        s = pkg.SwimmerSolver(t, q.sphere, q.center)
+        sol = s.solve_free([s.slip(q.e, 1.0, q.B2)])[0]
+        ref = np.r_[sol.U, sol.Om]
+        for name, r in res.items():
+            errs[name].append(float(np.linalg.norm(np.r_[r.U[i], r.Om[i]] - ref) / np.linalg.norm(ref)))
+        del s, t
+        gc.collect()
+    out["checked_instances"] = pick
+    out["rel_err_vs_single_fp64"] = {name: max(v) for name, v in errs.items()}
+    out["U_Omega_checksum"] = {name: float(np.sum(np.abs(r.U)) + np.sum(np.abs(r.Om))) for name, r in res.items()}
+    return out
+
+
 def cpu_baseline(W, pkg_mesh):
     """Oracle (FP64, scipy/OpenBLAS) timed on a bounded sample of the same solve: the panel
     integrals of a subset of rows of every block on the real 46k meshes (single thread, as
@@ -262,7 +303,8 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=4,
+                    help="untimed solves (the tile autotuner times each candidate once per shape class in these)")
     ap.add_argument("--mode", choices=list(MODES), default="bf16")
     ap.add_argument("--n_theta", type=int, default=64)
     ap.add_argument("--L_over_R", type=float, default=4.0)
@@ -275,6 +317,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-grad", action="store_true", help="skip the adjoint-gradient timing")
     ap.add_argument("--no-field", action="store_true", help="skip the 57k-DOF solve + field grid")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-configuration batched solve")
+    ap.add_argument("--calib-copy-gib", type=float, default=0.0, metavar="GIB",
+                    help="also run one plain device-to-device copy of GIB GiB (reference kernel for HBM counters)")
     ap.add_argument("--traj-grad", type=int, default=0, metavar="STEPS",
                     help="also run the trajectory adjoint over STEPS steps (2000 for the full case; minutes)")
     args = ap.parse_args()
@@ -285,6 +330,10 @@ def main():
 
     import __graft_entry__ as ge
 
+    if args.calib_copy_gib > 0:
+        import torch  # its HIP runtime has to see the device before libsqbem's does
+
+        torch.cuda.init()
     pkg = ge.load_package()
     pkg.init(0)
     W = workload(args)
@@ -309,7 +358,7 @@ def main():
     value = args.steps / elapsed
     dof = 3 * (phases["Nt"] + phases["Ns"]) + 6
 
-    detail = {"dof": dof, "phases_last_step": phases, "precision": args.mode}
+    detail = {"dof": dof, "phases_last_step": phases, "precision": args.mode, "tile_autotuner": pkg.tune_info()}
     if not args.no_fp64 and prec != 0:
         U64, Om64, fs64, ph64 = cold_solve(pkg, W, 0, e)
         detail["fp64_solve_s"] = ph64["total_s"]
@@ -329,6 +378,20 @@ def main():
                 "launches": kp["launches"], "avg_launch_ms": kp["ms"] / max(kp["launches"], 1),
                 "flops_per_launch": kp["flops"] / max(kp["launches"], 1)}
 
+    if args.calib_copy_gib > 0:
+        # a plain copy of known size: what FETCH_SIZE / WRITE_SIZE read for it calibrates
+        # the same counters on the solver's kernels
+        import torch
+
+        nel = int(args.calib_copy_gib * 2 ** 30) // 8
+        src_t = torch.ones(nel, dtype=torch.float64, device="cuda")
+        dst_t = torch.empty_like(src_t)
+        torch.cuda.synchronize()
+        dst_t.copy_(src_t)
+        torch.cuda.synchronize()
+        detail["calib_copy_bytes"] = nel * 8
+        del src_t, dst_t
+
     if not args.no_grad:
         try:
             detail["gradient"] = grad_bench(pkg)
@@ -337,6 +400,12 @@ def main():
 
     if args.traj_grad > 0:
         detail["trajectory_gradient"] = traj_grad_bench(pkg, args.traj_grad)
+
+    if not args.no_batch:
+        try:
+            detail["batch"] = batch_bench(pkg)
+        except Exception as ex:
+            detail["batch"] = {"error": str(ex)}
 
     if not args.no_field:
         try:

@@ -109,6 +109,16 @@ int sqbem_dgesv(int n, int nrhs, double* A, int lda, int* ipiv, double* B, int l
 int sqbem_sgetrf(int kind, int n, float* A, int lda, int* ipiv, int* info);
 int sqbem_sgetrs(int n, int nrhs, const float* LU, int lda, const int* ipiv, float* B, int ldb);
 
+/* Tile autotuner of the 16-bit GEMMs (no reference counterpart): mode 1 (the default)
+ * learns the fastest of four workgroup tiles (128/256 rows x 128/256 columns) per shape
+ * class (operand type, binary exponents of m, n, k) by timing each candidate once on live
+ * calls; mode 0 uses the fixed rule (256 x 256 tiles when they fill the device, else
+ * 128 x 128). Either call clears what was learned. sqbem_tune_info: number of shape
+ * classes seen, how many are decided, and counts[4] of decided classes per tile
+ * (128x128, 128x256, 256x128, 256x256); any pointer may be NULL. */
+int sqbem_tune(int mode);
+int sqbem_tune_info(int* classes, int* decided, int* counts);
+
 /* Host-array forms of the same calls, for the C++ linalg.hpp API (Mat lives in host
  * memory): the library stages the operands through HBM. */
 /* gemm(alpha, A, B, beta, C), src/utils/linalg.hpp:42-48 */
@@ -202,9 +212,10 @@ int sqbem_swimmer_gradient(sqbem_swimmer* s, const double* e, double B1, double 
  * the shared tube factors for all their Schur right-hand sides, and (refined modes) a
  * refinement of all their full systems together. Outputs (host): U[3n], Om[3n]; iters[n]
  * and resid[n] (refinement steps and final relative residual; 0 in FP64 mode; may be
- * NULL); stats[8] (may be NULL): seconds in tube assembly + factorisation, coupling
- * assembly, Schur + bordered factors, solves; unique tubes, chunks, most refinement
- * steps, worst final residual. Every instance needs a tube (tubes[i] != NULL). */
+ * NULL); stats[10] (may be NULL): seconds in tube assembly + factorisation (+ inverse),
+ * coupling assembly, Schur + bordered factors, solves; unique tubes, chunks, most
+ * refinement steps, worst final residual, bordered matrices that needed pivoting, bytes
+ * of the chunk workspace. Every instance needs a tube (tubes[i] != NULL). */
 int sqbem_batch_solve(int n, const sqbem_mesh* const* tubes, const sqbem_mesh* const* spheres,
                       const double* centers, const double* e, const double* B, double mu, int precision,
                       double mem_budget_gb, double* U, double* Om, int* iters, double* resid,

@@ -7,17 +7,22 @@
 //  * tube meshes with identical contents share one assembly and factorisation (a tube
 //    whose mesh does not depend on the swimmer, e.g. uniform axial nodes, is factored once
 //    for every configuration in it);
-//  * instances on the same tube are processed in chunks sized to a memory budget; every
-//    coupling block of a chunk (tube rows <- sphere, sphere rows <- tube, sphere <- sphere)
-//    is integrated by one persistent work-queue launch (assemble_batch);
+//  * instances on the same tube are processed in chunks sized to a memory budget, in one
+//    workspace allocated once per call; every coupling block of a chunk (tube rows <-
+//    sphere, sphere rows <- tube, sphere <- sphere) is integrated by one persistent
+//    work-queue launch (assemble_batch);
 //  * the Schur right-hand sides [B_i | Ct_i] of all instances of a chunk sit side by side
-//    in one nt x sum(ns_i + 6) matrix, so T^-1 [B | Ct] is one triangular solve with many
-//    columns through the shared factors; D_i Y_i is one GEMM per instance;
-//  * the bordered force/torque-free matrices are factored per instance;
-//  * FP64 mode solves the reduced systems directly (src/swimmer.hpp:107-139, 256-288); the
-//    refined modes iterate on every instance's full bordered system together, with the
-//    shared-tube products T X of all instances as one GEMM, and one host round trip per
-//    refinement step for all convergence checks.
+//    in one nt x sum(ns_i + 6) matrix. FP64 mode solves them through the shared LU factors
+//    in one triangular solve; the refined modes form the tube's inverse once (FP32, from
+//    its factors) and take T^-1 [B | Ct] as one large GEMM in the mode's MFMA precision;
+//  * the bordered force/torque-free matrices of all instances are factored together by the
+//    batched no-pivot LU (one launch per step for the whole chunk), each verified against
+//    the partial-pivoting condition and refactored with pivoting if it fails;
+//  * FP64 mode then solves the reduced systems directly (src/swimmer.hpp:107-139,
+//    256-288); the refined modes iterate on every instance's full bordered system together:
+//    the shared-tube products are single GEMMs, the per-instance products one batched
+//    matrix-vector launch, the bordered solves one batched chained solve, and one host
+//    round trip per refinement step serves all convergence checks.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -66,16 +71,99 @@ bool same_mesh(const bem::Mesh& a, const bem::Mesh& b) {
            std::memcmp(a.tris.data(), b.tris.data(), a.tris.size() * sizeof(a.tris[0])) == 0 && a.tags == b.tags;
 }
 
-// device bytes one instance needs inside a chunk on a tube of nt unknowns
-double inst_bytes(int nt, int ns, int prec) {
-    const double m = ns + 6.0, nb = (m + 127) / 128;
-    const double aux = (2 * nb + nb * nb) * 128.0 * 128.0;
-    double b = 8.0 * nt * (m + 2.0 * ns) + 8.0 * (ns * m + (double)ns * ns + m * m + aux) + 8.0 * 9 * ns / 3 * 2.0 +
-               8.0 * 4 * (nt + m);
-    if (mixed(prec)) b += 4.0 * nt * (m + ns) + 4.0 * (ns * m + m * m + aux) + 4.0 * 2 * (nt + m);
-    // stresslet row-sum partials of the two jobs with sphere targets
-    b += 8.0 * 9.0 * (ns / 3.0) * (asm_tiles(nt / 3) + asm_tiles(ns / 3));
-    return b;
+// ------------------------------------------------------------------ workspace
+// Bump allocation over one device buffer. A sizing pass (base == nullptr) only counts.
+struct Arena {
+    unsigned char* base = nullptr;
+    size_t cap = 0, off = 0;
+    template <typename T>
+    T* take(size_t count) {
+        off = (off + 255) / 256 * 256;
+        T* p = base ? (T*)(base + off) : nullptr;
+        off += count * sizeof(T);
+        SQ_REQUIRE(!base || off <= cap, SQBEM_ERR_STATE, "batch workspace overflow");
+        return p;
+    }
+};
+
+// ------------------------------------------------------------------ batched small kernels
+// y_j += alpha_j (A1_j x1_j + A2_j x2_j) for many independent jobs (one right-hand side
+// each, FP64 sums): column chunks of every job in parallel, then a reduction per job.
+struct GvJob {
+    int m, n1, n2;
+    const void* A1;
+    long lda1;
+    const void* x1;
+    const void* A2;
+    long lda2;
+    const void* x2;
+    void* y;
+    double alpha;
+    long poff;  // partial sums at part + poff, GV_CH x m
+};
+constexpr int GV_CH = 16;
+
+template <typename T>
+__global__ void gemv_batch_part(const GvJob* __restrict__ jobs, double* __restrict__ part) {
+    const GvJob& g = jobs[blockIdx.z];
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= g.m) return;
+    const int n = g.n1 + g.n2, per = (n + GV_CH - 1) / GV_CH;
+    const int ja = blockIdx.y * per, jb = min(n, ja + per);
+    const T* A1 = (const T*)g.A1;
+    const T* x1 = (const T*)g.x1;
+    const T* A2 = (const T*)g.A2;
+    const T* x2 = (const T*)g.x2;
+    double acc = 0.0;
+    for (int j = ja; j < jb; ++j) {
+        if (j < g.n1) acc += (double)A1[row + (size_t)j * g.lda1] * (double)x1[j];
+        else acc += (double)A2[row + (size_t)(j - g.n1) * g.lda2] * (double)x2[j - g.n1];
+    }
+    part[g.poff + (size_t)blockIdx.y * g.m + row] = acc;
+}
+
+template <typename T>
+__global__ void gemv_batch_reduce(const GvJob* __restrict__ jobs, const double* __restrict__ part) {
+    const GvJob& g = jobs[blockIdx.y];
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= g.m) return;
+    double s = 0.0;
+    for (int c = 0; c < GV_CH; ++c) s += part[g.poff + (size_t)c * g.m + row];
+    T* y = (T*)g.y;
+    y[row] = (T)((double)y[row] + g.alpha * s);
+}
+
+// r_c = -sc F f_s for many instances (the constraint rows of the bordered system)
+struct CJob {
+    int Ns;
+    const double* fs;
+    const double* cen;
+    const double* area;
+    double sc, xc[3];
+    double* out;
+};
+__global__ void constraint_batch_kernel(const CJob* __restrict__ jobs) {
+    const CJob& j = jobs[blockIdx.x];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int p = threadIdx.x; p < j.Ns; p += blockDim.x) {
+        const double rx = j.cen[3 * (size_t)p] - j.xc[0], ry = j.cen[3 * (size_t)p + 1] - j.xc[1],
+                     rz = j.cen[3 * (size_t)p + 2] - j.xc[2];
+        const double A = j.area[p] * j.sc;
+        const double f0 = j.fs[3 * (size_t)p], f1 = j.fs[3 * (size_t)p + 1], f2 = j.fs[3 * (size_t)p + 2];
+        acc[0] += A * f0; acc[1] += A * f1; acc[2] += A * f2;
+        acc[3] += -A * rz * f1 + A * ry * f2;
+        acc[4] += A * rz * f0 - A * rx * f2;
+        acc[5] += -A * ry * f0 + A * rx * f1;
+    }
+    __shared__ double red[6][256];
+    for (int c = 0; c < 6; ++c) red[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int c = 0; c < 6; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) j.out[threadIdx.x] = -red[threadIdx.x][0];
 }
 
 // ||r_i|| over the tube block Rt(:, i) and the instance's block of Rsc
@@ -104,6 +192,12 @@ __global__ void batch_iota_kernel(int n, int* a) {
     if (i < n) a[i] = i;
 }
 
+__global__ void identity_kernel(int n, float* A) {
+    const size_t total = (size_t)n * n;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x)
+        A[e] = (e % n) == (e / n) ? 1.0f : 0.0f;
+}
+
 __global__ void batch_axpy_kernel(size_t n, const double* __restrict__ y, double* x) {
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
         x[e] += y[e];
@@ -115,48 +209,205 @@ void add_into(size_t n, const double* y, double* x) {
     check_launch("batch_axpy");
 }
 
+// job tables of the batched small kernels: staged through one device buffer per call site
+struct JobTable {
+    DBuf<unsigned char> d;
+    template <typename J>
+    const J* put(const std::vector<J>& v) {
+        const size_t bytes = v.size() * sizeof(J);
+        if (d.n < bytes) d.alloc(std::max<size_t>(bytes, 1 << 16));
+        SQ_HIP(hipMemcpyAsync(d.p, v.data(), bytes, hipMemcpyHostToDevice, ctx().s()));
+        return (const J*)d.p;
+    }
+};
+
+template <typename T>
+void gemv_batch(JobTable& tab, std::vector<GvJob> jobs, double* part) {
+    if (jobs.empty()) return;
+    long off = 0;
+    int mmax = 1;
+    double bytes = 0, flops = 0;
+    for (GvJob& g : jobs) {
+        g.poff = off;
+        off += (long)GV_CH * g.m;
+        mmax = std::max(mmax, g.m);
+        bytes += sizeof(T) * (double)g.m * (g.n1 + g.n2);
+        flops += 2.0 * g.m * (double)(g.n1 + g.n2);
+    }
+    const GvJob* d = tab.put(jobs);
+    ProfScope prof(PROF_GEMV, flops, bytes);
+    hipLaunchKernelGGL(gemv_batch_part<T>, dim3(ceil_div(mmax, 256), GV_CH, (unsigned)jobs.size()), dim3(256), 0,
+                       ctx().s(), d, part);
+    hipLaunchKernelGGL(gemv_batch_reduce<T>, dim3(ceil_div(mmax, 256), (unsigned)jobs.size()), dim3(256), 0, ctx().s(),
+                       d, part);
+    check_launch("gemv_batch");
+    sync();  // the host job vector is staging
+}
+
+size_t gemv_part_elems(int nt, const std::vector<int>& ns, const std::vector<int>& m) {
+    size_t s = 0;
+    for (size_t i = 0; i < ns.size(); ++i) s += (size_t)GV_CH * (nt + ns[i] + m[i]);
+    return std::max<size_t>(s, 1);
+}
+
 struct Stats {
     double t_tubes = 0, t_assembly = 0, t_schur = 0, t_solve = 0;
-    int unique_tubes = 0, chunks = 0, max_iters = 0;
+    int unique_tubes = 0, chunks = 0, max_iters = 0, pivoted = 0;
     double worst = 0;
 };
 
-// One chunk: instances I (all on tube tb), built, reduced and solved together.
-void run_chunk(const sqbem_tube* tb, std::vector<Inst*>& I, int prec, double mu, double* U, double* Om, int* iters,
+// A tube shared by instances: its factors, and for the refined modes its FP32 inverse.
+struct SharedTube {
+    std::unique_ptr<sqbem_tube> t;
+    DBuf<float> inv;  // T^-1 (nt x nt), refined modes
+};
+
+void build_shared_tube(SharedTube& S, const bem::Mesh& mesh, double mu, int prec) {
+    S.t.reset(new sqbem_tube);
+    S.t->mesh = mesh;
+    S.t->mu = mu;
+    S.t->prec = prec;
+    tube_build(S.t.get());
+    if (mixed(prec)) {
+        const int nt = S.t->nt;
+        S.inv.alloc((size_t)nt * nt);
+        hipLaunchKernelGGL(identity_kernel, dim3(8192), dim3(256), 0, ctx().s(), nt, S.inv.p);
+        check_launch("identity");
+        // FP32 updates whatever the mode: the inverse carries the factors' accuracy
+        getrs<float>(SQBEM_FP32_IR, 0, nt, nt, S.t->T32.p, nt, S.t->ipiv.p, S.inv.p, nt, S.t->aux32.p);
+    }
+    sync();
+}
+
+// Device memory of one chunk (per-instance offsets into concatenated buffers, then the
+// buffers themselves, taken from the arena in a fixed order; with an empty arena this only
+// measures the chunk).
+struct Work {
+    int k = 0, maxNs = 1;
+    std::vector<int> coff, soff, Noff, nsv, mv;
+    std::vector<size_t> oD, oG, oA, oP2, oP3, oM, oX;
+    size_t C = 0, S = 0;
+    double *BC, *AuBt, *D, *GE, *AuBs, *part, *dls, *kp, *u, *Bt, *Bsc, *gvp, *nrm;
+    int *self, *flags, *dcoff, *dmlen, *piv;
+    unsigned long long* amax;
+    double *M64 = nullptr, *X64 = nullptr, *Z64 = nullptr, *Xd = nullptr;
+    float *Y32 = nullptr, *B32 = nullptr, *D32 = nullptr, *G32 = nullptr, *M32 = nullptr, *X32 = nullptr;
+    double *Xt = nullptr, *Xsc = nullptr, *Rt = nullptr, *Rsc = nullptr, *Dt = nullptr, *Dsc = nullptr;
+    float *Z32 = nullptr, *Zo32 = nullptr, *V32 = nullptr;
+};
+
+void layout(Arena& ar, int Nt, const std::vector<Inst*>& I, bool mx, Work& w) {
+    const int k = (int)I.size(), nt = 3 * Nt;
+    w.k = k;
+    w.coff.assign(k + 1, 0);
+    w.soff.assign(k + 1, 0);
+    w.Noff.assign(k + 1, 0);
+    w.nsv.resize(k);
+    w.mv.resize(k);
+    w.maxNs = 1;
+    for (int i = 0; i < k; ++i) {
+        w.coff[i + 1] = w.coff[i] + I[i]->m;
+        w.soff[i + 1] = w.soff[i] + I[i]->ns;
+        w.Noff[i + 1] = w.Noff[i] + I[i]->Ns;
+        w.maxNs = std::max(w.maxNs, I[i]->Ns);
+        w.nsv[i] = I[i]->ns;
+        w.mv[i] = I[i]->m;
+    }
+    w.C = w.coff[k];
+    w.S = w.soff[k];
+    for (auto* v : {&w.oD, &w.oG, &w.oA, &w.oP2, &w.oP3, &w.oM, &w.oX}) v->resize(k);
+    size_t nD = 0, nG = 0, nA = 0, nP = 0, nM = 0, nX = 0;
+    for (int i = 0; i < k; ++i) {
+        const size_t ns = I[i]->ns, m = I[i]->m, Ns = I[i]->Ns;
+        w.oD[i] = nD; nD += ns * nt;
+        w.oG[i] = nG; nG += ns * m;
+        w.oA[i] = nA; nA += ns * ns;
+        w.oP2[i] = nP; nP += (size_t)asm_tiles(Nt) * 9 * Ns;
+        w.oP3[i] = nP; nP += (size_t)asm_tiles((int)Ns) * 9 * Ns;
+        w.oM[i] = nM; nM += m * m;
+        w.oX[i] = nX; nX += getrf_aux_elems((int)m);
+    }
+    const size_t C = w.C, S = w.S;
+    ar.off = 0;
+    w.BC = ar.take<double>((size_t)nt * C);    // [B_i | Ct_i], FP64 (Y in FP64 mode)
+    w.AuBt = ar.take<double>((size_t)nt * S);  // -DL/(8 pi), tube rows <- sphere
+    w.D = ar.take<double>(nD);                 // SL/(8 pi mu), sphere rows <- tube
+    w.GE = ar.take<double>(nG);                // [E_i | Cs_i] (the Schur G in FP64 mode)
+    w.AuBs = ar.take<double>(nA);
+    w.part = ar.take<double>(nP);
+    w.dls = ar.take<double>(9 * (size_t)w.Noff[k]);
+    w.kp = ar.take<double>(kproj_part_elems(std::max(nt, 3 * w.maxNs)));
+    w.u = ar.take<double>(S);
+    w.Bt = ar.take<double>((size_t)nt * k);
+    w.Bsc = ar.take<double>(C);
+    w.gvp = ar.take<double>(gemv_part_elems(nt, w.nsv, w.mv));
+    w.nrm = ar.take<double>(k);
+    w.self = ar.take<int>(w.maxNs);
+    w.flags = ar.take<int>(k);
+    w.dcoff = ar.take<int>(k);
+    w.dmlen = ar.take<int>(k);
+    w.piv = ar.take<int>(C);
+    w.amax = ar.take<unsigned long long>(k);
+    if (!mx) {
+        w.M64 = ar.take<double>(nM);
+        w.X64 = ar.take<double>(nX);
+        w.Z64 = ar.take<double>((size_t)nt * k);
+        w.Xd = ar.take<double>(C);
+    } else {
+        w.Y32 = ar.take<float>((size_t)nt * C);
+        w.B32 = ar.take<float>((size_t)nt * C);
+        w.D32 = ar.take<float>(nD);
+        w.G32 = ar.take<float>(nG);
+        w.M32 = ar.take<float>(nM);
+        w.X32 = ar.take<float>(nX);
+        w.Xt = ar.take<double>((size_t)nt * k);
+        w.Xsc = ar.take<double>(C);
+        w.Rt = ar.take<double>((size_t)nt * k);
+        w.Rsc = ar.take<double>(C);
+        w.Dt = ar.take<double>((size_t)nt * k);
+        w.Dsc = ar.take<double>(C);
+        w.Z32 = ar.take<float>((size_t)nt * k);
+        w.Zo32 = ar.take<float>((size_t)nt * k);
+        w.V32 = ar.take<float>(C);
+    }
+}
+
+size_t chunk_bytes(int Nt, const std::vector<Inst*>& I, bool mx) {
+    Arena a;
+    Work w;
+    layout(a, Nt, I, mx, w);
+    return a.off;
+}
+
+// One chunk: instances I (all on tube S), built, reduced and solved together.
+void run_chunk(SharedTube& ST, std::vector<Inst*>& I, int prec, double mu, Arena& ar, double* U, double* Om, int* iters,
                double* resid, Stats& st) {
+    const sqbem_tube* tb = ST.t.get();
     const int k = (int)I.size();
     const int Nt = tb->Nt, nt = tb->nt;
     const double e8 = bem::EIGHT_PI;
     const bool mx = mixed(prec);
-    std::vector<int> coff(k + 1, 0), soff(k + 1, 0), Noff(k + 1, 0);
-    int maxNs = 1;
-    for (int i = 0; i < k; ++i) {
-        coff[i + 1] = coff[i] + I[i]->m;
-        soff[i + 1] = soff[i] + I[i]->ns;
-        Noff[i + 1] = Noff[i] + I[i]->Ns;
-        maxNs = std::max(maxNs, I[i]->Ns);
-    }
-    const size_t C = coff[k], S = soff[k];
-    // per-instance offsets into the concatenated buffers
-    std::vector<size_t> oD(k), oG(k), oA(k), oP2(k), oP3(k);
-    size_t nD = 0, nG = 0, nA = 0, nP = 0;
-    for (int i = 0; i < k; ++i) {
-        const size_t ns = I[i]->ns, m = I[i]->m, Ns = I[i]->Ns;
-        oD[i] = nD; nD += ns * nt;
-        oG[i] = nG; nG += ns * m;
-        oA[i] = nA; nA += ns * ns;
-        oP2[i] = nP; nP += (size_t)asm_tiles(Nt) * 9 * Ns;
-        oP3[i] = nP; nP += (size_t)asm_tiles((int)Ns) * 9 * Ns;
-    }
+    Work w;
+    layout(ar, Nt, I, mx, w);
+    const std::vector<int>&coff = w.coff, &soff = w.soff, &Noff = w.Noff, &mv = w.mv;
+    const std::vector<size_t>&oD = w.oD, &oG = w.oG, &oA = w.oA, &oP2 = w.oP2, &oP3 = w.oP3, &oM = w.oM, &oX = w.oX;
+    const size_t C = w.C, S = w.S;
+    const int maxNs = w.maxNs;
+    double *BC = w.BC, *AuBt = w.AuBt, *D = w.D, *GE = w.GE, *AuBs = w.AuBs, *part = w.part, *dls = w.dls, *kp = w.kp,
+           *u = w.u, *Bt = w.Bt, *Bsc = w.Bsc, *gvp = w.gvp, *nrm = w.nrm;
+    int *self = w.self, *flags = w.flags, *dcoff = w.dcoff, *dmlen = w.dmlen, *piv = w.piv;
+    unsigned long long* amax = w.amax;
+    double *M64 = w.M64, *X64 = w.X64, *Z64 = w.Z64, *Xd = w.Xd;
+    float *Y32 = w.Y32, *B32 = w.B32, *D32 = w.D32, *G32 = w.G32, *M32 = w.M32, *X32 = w.X32;
+    double *Xt = w.Xt, *Xsc = w.Xsc, *Rt = w.Rt, *Rsc = w.Rsc, *Dt = w.Dt, *Dsc = w.Dsc;
+    float *Z32 = w.Z32, *Zo32 = w.Zo32, *V32 = w.V32;
 
     // ---- coupling assembly (src/swimmer.hpp:188-234), one persistent launch
     double t0 = now_s();
     std::vector<DMesh> dm(k);
     for (int i = 0; i < k; ++i) dm[i].upload(I[i]->S);
-    DBuf<int> self(maxNs);
-    hipLaunchKernelGGL(batch_iota_kernel, dim3(ceil_div(maxNs, 256)), dim3(256), 0, ctx().s(), maxNs, self.p);
+    hipLaunchKernelGGL(batch_iota_kernel, dim3(ceil_div(maxNs, 256)), dim3(256), 0, ctx().s(), maxNs, self);
     check_launch("iota");
-    DBuf<double> BC((size_t)nt * C), AuBt((size_t)nt * S), D(nD), GE(nG), AuBs(nA), part(nP), dls(9 * (size_t)Noff[k]);
     std::vector<AsmJob> jobs;
     std::vector<DlsReduce> red;
     for (int i = 0; i < k; ++i) {
@@ -165,34 +416,29 @@ void run_chunk(const sqbem_tube* tb, std::vector<Inst*>& I, int prec, double mu,
         for (int p = 0; p < q.Ns; ++p) Asum += q.S.area[p];
         const double defl = 1.0 / (e8 * mu * std::sqrt(Asum));
         // tube rows <- sphere sources: B = SL/(8 pi mu), AuBt = -DL/(8 pi)
-        jobs.push_back({&dm[i], 0, q.Ns, tb->dm.cen_xyz.p, Nt, nullptr, e8 * mu, BC.p + (size_t)coff[i] * nt, nt, e8,
-                        AuBt.p + (size_t)soff[i] * nt, nt, nullptr, 0.0, nullptr});
+        jobs.push_back({&dm[i], 0, q.Ns, tb->dm.cen_xyz.p, Nt, nullptr, e8 * mu, BC + (size_t)coff[i] * nt, nt, e8,
+                        AuBt + (size_t)soff[i] * nt, nt, nullptr, 0.0, nullptr});
         // sphere rows <- tube sources: D = SL/(8 pi mu), stresslet row sums
-        jobs.push_back({&tb->dm, 0, Nt, dm[i].cen_xyz.p, q.Ns, nullptr, e8 * mu, D.p + oD[i], q.ns, 1.0, nullptr, 0,
-                        part.p + oP2[i], 0.0, nullptr});
+        jobs.push_back({&tb->dm, 0, Nt, dm[i].cen_xyz.p, q.Ns, nullptr, e8 * mu, D + oD[i], q.ns, 1.0, nullptr, 0,
+                        part + oP2[i], 0.0, nullptr});
         // sphere rows <- sphere sources: E (deflated), AuBs, stresslet row sums
-        jobs.push_back({&dm[i], 0, q.Ns, dm[i].cen_xyz.p, q.Ns, self.p, e8 * mu, GE.p + oG[i], q.ns, e8,
-                        AuBs.p + oA[i], q.ns, part.p + oP3[i], defl, dm[i].nrm_xyz.p});
-        red.push_back({q.Ns, asm_tiles(Nt), part.p + oP2[i], asm_tiles(q.Ns), part.p + oP3[i], dls.p + 9 * (size_t)Noff[i]});
+        jobs.push_back({&dm[i], 0, q.Ns, dm[i].cen_xyz.p, q.Ns, self, e8 * mu, GE + oG[i], q.ns, e8, AuBs + oA[i], q.ns,
+                        part + oP3[i], defl, dm[i].nrm_xyz.p});
+        red.push_back({q.Ns, asm_tiles(Nt), part + oP2[i], asm_tiles(q.Ns), part + oP3[i], dls + 9 * (size_t)Noff[i]});
     }
     assemble_batch(jobs);
     reduce_dls_batch(red);
-    {
-        DBuf<double> kp(kproj_part_elems(std::max(nt, 3 * maxNs)));
-        for (int i = 0; i < k; ++i) {
-            const Inst& q = *I[i];
-            completion_diag(q.Ns, AuBs.p + oA[i], q.ns, dls.p + 9 * (size_t)Noff[i], 0.0);
-            // Ct = AuBt K beside B_i, Cs = AuBs K beside E_i (src/swimmer.hpp:223-234)
-            kproj_async(nt, q.Ns, AuBt.p + (size_t)soff[i] * nt, nt, dm[i].cen_xyz.p, q.xc,
-                        BC.p + ((size_t)coff[i] + q.ns) * nt, nt, kp.p);
-            kproj_async(q.ns, q.Ns, AuBs.p + oA[i], q.ns, dm[i].cen_xyz.p, q.xc, GE.p + oG[i] + (size_t)q.ns * q.ns, q.ns,
-                        kp.p);
-        }
-        sync();
+    for (int i = 0; i < k; ++i) {
+        const Inst& q = *I[i];
+        completion_diag(q.Ns, AuBs + oA[i], q.ns, dls + 9 * (size_t)Noff[i], 0.0);
+        // Ct = AuBt K beside B_i, Cs = AuBs K beside E_i (src/swimmer.hpp:223-234)
+        kproj_async(nt, q.Ns, AuBt + (size_t)soff[i] * nt, nt, dm[i].cen_xyz.p, q.xc, BC + ((size_t)coff[i] + q.ns) * nt,
+                    nt, kp);
+        kproj_async(q.ns, q.Ns, AuBs + oA[i], q.ns, dm[i].cen_xyz.p, q.xc, GE + oG[i] + (size_t)q.ns * q.ns, q.ns, kp);
     }
-    // right-hand sides of the full bordered system for the slip of each instance:
+    // right-hand sides of the full bordered system for each instance's slip:
     // b_t = -AuBt u, b_s = -AuBs u, b_c = 0 (src/swimmer.hpp:256-269)
-    DBuf<double> u(S), Bt((size_t)nt * k), Bsc(C);
+    JobTable jt;
     {
         std::vector<double> hu(S);
         for (int i = 0; i < k; ++i) {
@@ -205,105 +451,122 @@ void run_chunk(const sqbem_tube* tb, std::vector<Inst*>& I, int prec, double mu,
                 hu[soff[i] + 3 * (size_t)p + 2] = v.z;
             }
         }
-        u.upload(hu.data(), S);
-        Bsc.zero();
+        SQ_HIP(hipMemcpyAsync(u, hu.data(), S * sizeof(double), hipMemcpyHostToDevice, ctx().s()));
+        fill_zero(Bt, sizeof(double) * (size_t)nt * k);
+        fill_zero(Bsc, sizeof(double) * C);
+        std::vector<GvJob> g;
         for (int i = 0; i < k; ++i) {
             const Inst& q = *I[i];
-            gemv<double>(nt, q.ns, 1, -1.0, AuBt.p + (size_t)soff[i] * nt, nt, u.p + soff[i], q.ns, 0.0,
-                         Bt.p + (size_t)i * nt, nt);
-            gemv<double>(q.ns, q.ns, 1, -1.0, AuBs.p + oA[i], q.ns, u.p + soff[i], q.ns, 0.0, Bsc.p + coff[i], q.m);
+            g.push_back({nt, q.ns, 0, AuBt + (size_t)soff[i] * nt, nt, u + soff[i], nullptr, 0, nullptr,
+                         Bt + (size_t)i * nt, -1.0, 0});
+            g.push_back({q.ns, q.ns, 0, AuBs + oA[i], q.ns, u + soff[i], nullptr, 0, nullptr, Bsc + coff[i], -1.0, 0});
         }
+        gemv_batch<double>(jt, g, gvp);  // synchronises (the slip upload is complete)
     }
-    AuBt.release();
-    AuBs.release();
-    part.release();
-    sync();
     double t1 = now_s();
     st.t_assembly += t1 - t0;
 
     // ---- Schur complement (src/swimmer.hpp:235-249) and bordered factors
-    std::vector<double> Amax(k, 0.0);
+    std::vector<double> Amax(k, 0.0), sc(k);
     for (int i = 0; i < k; ++i)
         for (int p = 0; p < I[i]->Ns; ++p) Amax[i] = std::max(Amax[i], 5.0 * I[i]->S.area[p]);
-    DBuf<unsigned long long> amax(k);
-    amax.zero();
     std::vector<unsigned long long> abits(k);
-    std::vector<double> sc(k);
-    std::vector<DBuf<int>> Mpiv(k);
-    // FP64 mode
-    std::vector<DBuf<double>> M64(k);
-    // refined modes
-    DBuf<float> Y32, D32, G32;
-    std::vector<DBuf<float>> M32(k), Maux(k);
+    fill_zero(amax, sizeof(unsigned long long) * k);
     if (!mx) {
-        getrs<double>(SQBEM_FP64, 0, nt, (int)C, tb->T64.p, nt, tb->ipiv.p, BC.p, nt, tb->aux64.p);  // Y in place
+        getrs<double>(SQBEM_FP64, 0, nt, (int)C, tb->T64.p, nt, tb->ipiv.p, BC, nt, tb->aux64.p);  // Y in place
         for (int i = 0; i < k; ++i) {
             const Inst& q = *I[i];
-            dgemm(q.ns, q.m, nt, -1.0, D.p + oD[i], q.ns, BC.p + (size_t)coff[i] * nt, nt, 1.0, GE.p + oG[i], q.ns);
-            absmax_async<double>(q.ns, q.ns, GE.p + oG[i], q.ns, amax.p + i);
-        }
-        amax.download(abits.data(), k);
-        for (int i = 0; i < k; ++i) {
-            const Inst& q = *I[i];
-            double a;
-            std::memcpy(&a, &abits[i], sizeof a);
-            sc[i] = a / Amax[i];
-            M64[i].alloc((size_t)q.m * q.m);
-            border_build<double>(q.Ns, GE.p + oG[i], M64[i].p, dm[i].cen_xyz.p, dm[i].area.p, sc[i], q.xc);
-            Mpiv[i].alloc(q.m);
-            const int info = getrf<double>(SQBEM_FP64, q.m, M64[i].p, q.m, Mpiv[i].p);
-            SQ_REQUIRE(info == 0, SQBEM_ERR_SINGULAR, "dgesv failed");
+            dgemm(q.ns, q.m, nt, -1.0, D + oD[i], q.ns, BC + (size_t)coff[i] * nt, nt, 1.0, GE + oG[i], q.ns);
+            absmax_async<double>(q.ns, q.ns, GE + oG[i], q.ns, amax + i);
         }
     } else {
-        Y32.alloc((size_t)nt * C);
-        convert_matrix<double, float>(nt, (int)C, BC.p, nt, Y32.p, nt);
-        getrs<float>(prec, 0, nt, (int)C, tb->T32.p, nt, tb->ipiv.p, Y32.p, nt, tb->aux32.p);
-        D32.alloc(nD);
-        G32.alloc(nG);
+        // Y = T^-1 [B | Ct] for every instance: one GEMM through the shared inverse
+        convert_matrix<double, float>(nt, (int)C, BC, nt, B32, nt);
+        if (prec == SQBEM_FP32_IR)
+            sgemm(nt, (int)C, nt, 1.0f, ST.inv.p, nt, B32, nt, 0.0f, Y32, nt);
+        else
+            gemm16(prec, nt, (int)C, nt, 1.0f, ST.inv.p, nt, B32, nt, 0.0f, Y32, nt);
         for (int i = 0; i < k; ++i) {
             const Inst& q = *I[i];
-            convert_matrix<double, float>(q.ns, nt, D.p + oD[i], q.ns, D32.p + oD[i], q.ns);
-            convert_matrix<double, float>(q.ns, q.m, GE.p + oG[i], q.ns, G32.p + oG[i], q.ns);
-            const float* Yi = Y32.p + (size_t)coff[i] * nt;
+            convert_matrix<double, float>(q.ns, nt, D + oD[i], q.ns, D32 + oD[i], q.ns);
+            convert_matrix<double, float>(q.ns, q.m, GE + oG[i], q.ns, G32 + oG[i], q.ns);
+            const float* Yi = Y32 + (size_t)coff[i] * nt;
             if (prec == SQBEM_FP32_IR)
-                sgemm(q.ns, q.m, nt, -1.0f, D32.p + oD[i], q.ns, Yi, nt, 1.0f, G32.p + oG[i], q.ns);
+                sgemm(q.ns, q.m, nt, -1.0f, D32 + oD[i], q.ns, Yi, nt, 1.0f, G32 + oG[i], q.ns);
             else
-                gemm16(prec, q.ns, q.m, nt, -1.0f, D32.p + oD[i], q.ns, Yi, nt, 1.0f, G32.p + oG[i], q.ns);
-            absmax_async<float>(q.ns, q.ns, G32.p + oG[i], q.ns, amax.p + i);
+                gemm16(prec, q.ns, q.m, nt, -1.0f, D32 + oD[i], q.ns, Yi, nt, 1.0f, G32 + oG[i], q.ns);
+            absmax_async<float>(q.ns, q.ns, G32 + oG[i], q.ns, amax + i);
         }
-        amax.download(abits.data(), k);
-        for (int i = 0; i < k; ++i) {
-            const Inst& q = *I[i];
-            double a;
-            std::memcpy(&a, &abits[i], sizeof a);
-            sc[i] = a / Amax[i];
-            M32[i].alloc((size_t)q.m * q.m);
-            border_build<float>(q.Ns, G32.p + oG[i], M32[i].p, dm[i].cen_xyz.p, dm[i].area.p, sc[i], q.xc);
-            Mpiv[i].alloc(q.m);
-            Maux[i].alloc(getrf_aux_elems(q.m));
-            const int info = getrf<float>(prec, q.m, M32[i].p, q.m, Mpiv[i].p, Maux[i].p);
-            SQ_REQUIRE(info == 0, SQBEM_ERR_SINGULAR, "dgesv failed");
-        }
-        G32.release();
+    }
+    SQ_HIP(hipMemcpyAsync(abits.data(), amax, sizeof(unsigned long long) * k, hipMemcpyDeviceToHost, ctx().s()));
+    sync();
+    for (int i = 0; i < k; ++i) {
+        double a;
+        std::memcpy(&a, &abits[i], sizeof a);
+        sc[i] = a / Amax[i];
+    }
+    // bordered matrices, factored together without pivoting; any that partial pivoting
+    // would interchange are refactored alone with pivoting
+    std::vector<char> pivoted(k, 0);
+    auto border = [&](int i) {
+        const Inst& q = *I[i];
+        if (mx) border_build<float>(q.Ns, G32 + oG[i], M32 + oM[i], dm[i].cen_xyz.p, dm[i].area.p, sc[i], q.xc);
+        else border_build<double>(q.Ns, GE + oG[i], M64 + oM[i], dm[i].cen_xyz.p, dm[i].area.p, sc[i], q.xc);
+    };
+    for (int i = 0; i < k; ++i) border(i);
+    std::vector<int> hflag(k, 0);
+    if (mx) {
+        std::vector<BatchMat<float>> bm;
+        for (int i = 0; i < k; ++i) bm.push_back({M32 + oM[i], I[i]->m, I[i]->m, X32 + oX[i], flags + i});
+        getrf_np_batch<float>(bm);
+    } else {
+        std::vector<BatchMat<double>> bm;
+        for (int i = 0; i < k; ++i) bm.push_back({M64 + oM[i], I[i]->m, I[i]->m, X64 + oX[i], flags + i});
+        getrf_np_batch<double>(bm);
+    }
+    SQ_HIP(hipMemcpyAsync(hflag.data(), flags, sizeof(int) * k, hipMemcpyDeviceToHost, ctx().s()));
+    sync();
+    for (int i = 0; i < k; ++i) {
+        if (!hflag[i]) continue;
+        pivoted[i] = 1;
+        ++st.pivoted;
+        border(i);
+        const int info = mx ? getrf<float>(prec, I[i]->m, M32 + oM[i], I[i]->m, piv + coff[i], X32 + oX[i])
+                            : getrf<double>(SQBEM_FP64, I[i]->m, M64 + oM[i], I[i]->m, piv + coff[i], X64 + oX[i]);
+        SQ_REQUIRE(info == 0, SQBEM_ERR_SINGULAR, "dgesv failed");
     }
     sync();
     double t2 = now_s();
     st.t_schur += t2 - t1;
 
+    // bordered solves of all instances: B_i <- M_i^-1 B_i
+    auto bordered_solve = [&](auto* Mbase, auto* Xbase, auto* V) {
+        using T = std::remove_pointer_t<decltype(V)>;
+        std::vector<BatchSolve<T>> bs;
+        for (int i = 0; i < k; ++i)
+            if (!pivoted[i]) bs.push_back({I[i]->m, Xbase + oX[i], V + coff[i]});
+        getrs_np_batch<T>(bs);
+        for (int i = 0; i < k; ++i)
+            if (pivoted[i])
+                getrs<T>(mx ? prec : SQBEM_FP64, 0, I[i]->m, 1, Mbase + oM[i], I[i]->m, piv + coff[i], V + coff[i],
+                         I[i]->m, Xbase + oX[i]);
+    };
+
     // ---- solves
     std::vector<double> X(C);
     if (!mx) {
         // z = T^-1 b_t for all instances at once; x_i = M_i^-1 [b_s - D_i z_i; 0]
-        DBuf<double> Z((size_t)nt * k), Xd(C);
-        convert_matrix<double, double>(nt, k, Bt.p, nt, Z.p, nt);
-        getrs<double>(SQBEM_FP64, 0, nt, k, tb->T64.p, nt, tb->ipiv.p, Z.p, nt, tb->aux64.p);
-        convert_matrix<double, double>((int)C, 1, Bsc.p, (int)C, Xd.p, (int)C);
-        for (int i = 0; i < k; ++i) {
-            const Inst& q = *I[i];
-            gemv<double>(q.ns, nt, 1, -1.0, D.p + oD[i], q.ns, Z.p + (size_t)i * nt, nt, 1.0, Xd.p + coff[i], q.m);
-            getrs<double>(SQBEM_FP64, 0, q.m, 1, M64[i].p, q.m, Mpiv[i].p, Xd.p + coff[i], q.m);
-        }
-        Xd.download(X.data(), C);
+        convert_matrix<double, double>(nt, k, Bt, nt, Z64, nt);
+        getrs<double>(SQBEM_FP64, 0, nt, k, tb->T64.p, nt, tb->ipiv.p, Z64, nt, tb->aux64.p);
+        convert_matrix<double, double>((int)C, 1, Bsc, (int)C, Xd, (int)C);
+        std::vector<GvJob> g;
+        for (int i = 0; i < k; ++i)
+            g.push_back({I[i]->ns, nt, 0, D + oD[i], I[i]->ns, Z64 + (size_t)i * nt, nullptr, 0, nullptr, Xd + coff[i],
+                         -1.0, 0});
+        gemv_batch<double>(jt, g, gvp);
+        bordered_solve(M64, X64, Xd);
+        SQ_HIP(hipMemcpyAsync(X.data(), Xd, C * sizeof(double), hipMemcpyDeviceToHost, ctx().s()));
+        sync();
         for (int i = 0; i < k; ++i) {
             iters[I[i]->idx] = 0;
             resid[I[i]->idx] = 0.0;
@@ -311,63 +574,69 @@ void run_chunk(const sqbem_tube* tb, std::vector<Inst*>& I, int prec, double mu,
     } else {
         // iterative refinement of every instance's full bordered system
         //   [T B Ct; D E Cs; 0 scF 0] [f_t; f_s; w] = [b_t; b_s; 0]
-        // residuals from the FP64 operator, corrections from the low-precision factors
-        DBuf<double> Xt((size_t)nt * k), Xsc(C), Rt((size_t)nt * k), Rsc(C), Dt((size_t)nt * k), Dsc(C), nrm(k);
-        DBuf<float> Z32((size_t)nt * k), V32(C);
-        DBuf<int> dcoff(k), dmlen(k);
+        // residuals from the FP64 operator, corrections from the low-precision Schur pieces
         {
-            std::vector<int> ml(k);
-            for (int i = 0; i < k; ++i) ml[i] = I[i]->m;
-            dcoff.upload(coff.data(), k);
-            dmlen.upload(ml.data(), k);
+            std::vector<int> hc(coff.begin(), coff.begin() + k);
+            SQ_HIP(hipMemcpyAsync(dcoff, hc.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx().s()));
+            SQ_HIP(hipMemcpyAsync(dmlen, mv.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx().s()));
         }
-        Xt.zero();
-        Xsc.zero();
-        convert_matrix<double, double>(nt, k, Bt.p, nt, Rt.p, nt);
-        convert_matrix<double, double>((int)C, 1, Bsc.p, (int)C, Rsc.p, (int)C);
+        fill_zero(Xt, sizeof(double) * (size_t)nt * k);
+        fill_zero(Xsc, sizeof(double) * C);
+        convert_matrix<double, double>(nt, k, Bt, nt, Rt, nt);
+        convert_matrix<double, double>((int)C, 1, Bsc, (int)C, Rsc, (int)C);
         std::vector<double> bn(k), rn(k), rel(k, 1.0);
         std::vector<int> itc(k, 0);
-        hipLaunchKernelGGL(batch_norms_kernel, dim3(k), dim3(256), 0, ctx().s(), nt, Bt.p, Bsc.p, dcoff.p, dmlen.p,
-                           nrm.p);
+        hipLaunchKernelGGL(batch_norms_kernel, dim3(k), dim3(256), 0, ctx().s(), nt, Bt, Bsc, dcoff, dmlen, nrm);
         check_launch("batch_norms");
-        nrm.download(bn.data(), k);
+        SQ_HIP(hipMemcpyAsync(bn.data(), nrm, sizeof(double) * k, hipMemcpyDeviceToHost, ctx().s()));
+        sync();
+        // job lists of the per-instance products (fixed for the whole refinement)
+        std::vector<GvJob> gv_s, gv_t, gv_r;
+        std::vector<CJob> cj;
+        for (int i = 0; i < k; ++i) {
+            const Inst& q = *I[i];
+            float* zi = Zo32 + (size_t)i * nt;
+            float* vi = V32 + coff[i];
+            gv_s.push_back({q.ns, nt, 0, D32 + oD[i], q.ns, zi, nullptr, 0, nullptr, vi, -1.0, 0});
+            gv_t.push_back({nt, q.m, 0, Y32 + (size_t)coff[i] * nt, nt, vi, nullptr, 0, nullptr, zi, -1.0, 0});
+            const double* xs = Xsc + coff[i];
+            gv_r.push_back({nt, q.m, 0, BC + (size_t)coff[i] * nt, nt, xs, nullptr, 0, nullptr, Rt + (size_t)i * nt,
+                            -1.0, 0});
+            gv_r.push_back({q.ns, nt, q.m, D + oD[i], q.ns, Xt + (size_t)i * nt, GE + oG[i], q.ns, xs, Rsc + coff[i],
+                            -1.0, 0});
+            cj.push_back({q.Ns, xs, dm[i].cen_xyz.p, dm[i].area.p, sc[i], {q.xc[0], q.xc[1], q.xc[2]},
+                          Rsc + coff[i] + q.ns});
+        }
+        JobTable jc;
         const double tol = 1e-12;
         int it = 0;
         double worst = 1.0;
         for (; it < 40; ++it) {
-            // corrections: Z = T^-1 r_t; v_i = M_i^-1 [r_s - D_i z_i; r_c]; z_i -= Y_i v_i
-            convert_matrix<double, float>(nt, k, Rt.p, nt, Z32.p, nt);
-            getrs<float>(prec, 0, nt, k, tb->T32.p, nt, tb->ipiv.p, Z32.p, nt, tb->aux32.p);
-            convert_matrix<double, float>((int)C, 1, Rsc.p, (int)C, V32.p, (int)C);
-            for (int i = 0; i < k; ++i) {
-                const Inst& q = *I[i];
-                float* zi = Z32.p + (size_t)i * nt;
-                float* vi = V32.p + coff[i];
-                gemv<float>(q.ns, nt, 1, -1.0, D32.p + oD[i], q.ns, zi, nt, 1.0, vi, q.m);
-                getrs<float>(prec, 0, q.m, 1, M32[i].p, q.m, Mpiv[i].p, vi, q.m, Maux[i].p);
-                gemv<float>(nt, q.m, 1, -1.0, Y32.p + (size_t)coff[i] * nt, nt, vi, q.m, 1.0, zi, nt);
+            // corrections: z = T^-1 r_t; v_i = M_i^-1 [r_s - D_i z_i; r_c]; z_i -= Y_i v_i
+            convert_matrix<double, float>(nt, k, Rt, nt, Z32, nt);
+            sgemm(nt, k, nt, 1.0f, ST.inv.p, nt, Z32, nt, 0.0f, Zo32, nt);
+            convert_matrix<double, float>((int)C, 1, Rsc, (int)C, V32, (int)C);
+            gemv_batch<float>(jt, gv_s, gvp);
+            bordered_solve(M32, X32, V32);
+            gemv_batch<float>(jt, gv_t, gvp);
+            convert_matrix<float, double>(nt, k, Zo32, nt, Dt, nt);
+            convert_matrix<float, double>((int)C, 1, V32, (int)C, Dsc, (int)C);
+            add_into((size_t)nt * k, Dt, Xt);
+            add_into(C, Dsc, Xsc);
+            // residual R = B - A X; the shared T X of all instances as one product
+            convert_matrix<double, double>(nt, k, Bt, nt, Rt, nt);
+            dgemm(nt, k, nt, -1.0, tb->T64.p, nt, Xt, nt, 1.0, Rt, nt);
+            convert_matrix<double, double>((int)C, 1, Bsc, (int)C, Rsc, (int)C);
+            gemv_batch<double>(jt, gv_r, gvp);
+            {
+                const CJob* d = jc.put(cj);
+                hipLaunchKernelGGL(constraint_batch_kernel, dim3(k), dim3(256), 0, ctx().s(), d);
+                check_launch("constraint_batch");
             }
-            convert_matrix<float, double>(nt, k, Z32.p, nt, Dt.p, nt);
-            convert_matrix<float, double>((int)C, 1, V32.p, (int)C, Dsc.p, (int)C);
-            add_into((size_t)nt * k, Dt.p, Xt.p);
-            add_into(C, Dsc.p, Xsc.p);
-            // residual: R = B - A X, the shared T X of all instances as one product
-            convert_matrix<double, double>(nt, k, Bt.p, nt, Rt.p, nt);
-            dgemm(nt, k, nt, -1.0, tb->T64.p, nt, Xt.p, nt, 1.0, Rt.p, nt);
-            convert_matrix<double, double>((int)C, 1, Bsc.p, (int)C, Rsc.p, (int)C);
-            for (int i = 0; i < k; ++i) {
-                const Inst& q = *I[i];
-                const double* xs = Xsc.p + coff[i];
-                double* rs = Rsc.p + coff[i];
-                gemv<double>(nt, q.m, 1, -1.0, BC.p + (size_t)coff[i] * nt, nt, xs, q.m, 1.0, Rt.p + (size_t)i * nt, nt);
-                gemv<double>(q.ns, nt, 1, -1.0, D.p + oD[i], q.ns, Xt.p + (size_t)i * nt, nt, 1.0, rs, q.m);
-                gemv<double>(q.ns, q.m, 1, -1.0, GE.p + oG[i], q.ns, xs, q.m, 1.0, rs, q.m);
-                constraint_apply(q.Ns, 1, xs, q.m, dm[i].cen_xyz.p, dm[i].area.p, sc[i], q.xc, rs + q.ns, q.m);
-            }
-            hipLaunchKernelGGL(batch_norms_kernel, dim3(k), dim3(256), 0, ctx().s(), nt, Rt.p, Rsc.p, dcoff.p, dmlen.p,
-                               nrm.p);
+            hipLaunchKernelGGL(batch_norms_kernel, dim3(k), dim3(256), 0, ctx().s(), nt, Rt, Rsc, dcoff, dmlen, nrm);
             check_launch("batch_norms");
-            nrm.download(rn.data(), k);
+            SQ_HIP(hipMemcpyAsync(rn.data(), nrm, sizeof(double) * k, hipMemcpyDeviceToHost, ctx().s()));
+            sync();
             worst = 0.0;
             for (int i = 0; i < k; ++i) {
                 rel[i] = bn[i] > 0 ? rn[i] / bn[i] : rn[i];
@@ -381,7 +650,8 @@ void run_chunk(const sqbem_tube* tb, std::vector<Inst*>& I, int prec, double mu,
         }
         SQ_REQUIRE(worst <= 1e-9, SQBEM_ERR_CONVERGE,
                    "iterative refinement stalled at relative residual " + std::to_string(worst));
-        Xsc.download(X.data(), C);
+        SQ_HIP(hipMemcpyAsync(X.data(), Xsc, C * sizeof(double), hipMemcpyDeviceToHost, ctx().s()));
+        sync();
         for (int i = 0; i < k; ++i) {
             iters[I[i]->idx] = itc[i];
             resid[I[i]->idx] = rel[i];
@@ -454,34 +724,47 @@ int sqbem_batch_solve(int n, const sqbem_mesh* const* tubes, const sqbem_mesh* c
         }
         st.unique_tubes = (int)utube.size();
         const double budget = (mem_budget_gb > 0 ? mem_budget_gb : 64.0) * 1e9;
-        // per tube: build, run its chunks (largest instances first), release
+        const bool mx = mixed(precision);
+        // chunks: per tube, largest instances first, greedily up to the budget (at least one)
+        std::vector<std::vector<std::vector<Inst*>>> chunks(utube.size());
+        size_t need = 0;
         for (int u = 0; u < (int)utube.size(); ++u) {
             std::vector<Inst*> mine;
             for (Inst& q : inst)
                 if (q.tube == u) mine.push_back(&q);
             std::stable_sort(mine.begin(), mine.end(), [](const Inst* a, const Inst* b) { return a->Ns > b->Ns; });
-            double t0 = now_s();
-            std::unique_ptr<sqbem_tube> tb(new sqbem_tube);
-            tb->mesh = *utube[u];
-            tb->mu = mu;
-            tb->prec = precision;
-            tube_build(tb.get());
-            sync();
-            st.t_tubes += now_s() - t0;
-            size_t a = 0;
-            while (a < mine.size()) {
-                size_t b = a;
-                double bytes = 0;
-                while (b < mine.size()) {
-                    const double need = inst_bytes(tb->nt, mine[b]->ns, precision);
-                    if (b > a && bytes + need > budget) break;
-                    bytes += need;
-                    ++b;
+            const int Nt = utube[u]->P();
+            std::vector<Inst*> cur;
+            for (Inst* q : mine) {
+                cur.push_back(q);
+                if (cur.size() > 1 && (double)chunk_bytes(Nt, cur, mx) > budget) {
+                    cur.pop_back();
+                    need = std::max(need, chunk_bytes(Nt, cur, mx));
+                    chunks[u].push_back(cur);
+                    cur.assign(1, q);
                 }
-                std::vector<Inst*> chunk(mine.begin() + a, mine.begin() + b);
-                run_chunk(tb.get(), chunk, precision, mu, U, Om, iters, resid, st);
+            }
+            if (!cur.empty()) {
+                need = std::max(need, chunk_bytes(Nt, cur, mx));
+                chunks[u].push_back(cur);
+            }
+        }
+        // one workspace for every chunk of the call
+        DBuf<unsigned char> ws;
+        double t0 = now_s();
+        ws.alloc(need);
+        Arena ar;
+        ar.base = ws.p;
+        ar.cap = need;
+        st.t_assembly += now_s() - t0;
+        for (int u = 0; u < (int)utube.size(); ++u) {
+            double t1 = now_s();
+            SharedTube S;
+            build_shared_tube(S, *utube[u], mu, precision);
+            st.t_tubes += now_s() - t1;
+            for (auto& ch : chunks[u]) {
+                run_chunk(S, ch, precision, mu, ar, U, Om, iters, resid, st);
                 ++st.chunks;
-                a = b;
             }
         }
         if (stats) {
@@ -493,6 +776,8 @@ int sqbem_batch_solve(int n, const sqbem_mesh* const* tubes, const sqbem_mesh* c
             stats[5] = st.chunks;
             stats[6] = st.max_iters;
             stats[7] = st.worst;
+            stats[8] = st.pivoted;
+            stats[9] = (double)need;
         }
     });
 }

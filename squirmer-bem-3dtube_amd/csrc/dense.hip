@@ -18,6 +18,8 @@
 #include <algorithm>
 #include <type_traits>
 #include <cmath>
+#include <cstring>
+#include <map>
 #include <vector>
 
 #include "common.hpp"
@@ -78,16 +80,14 @@ constexpr int GBK = 16, GPAD = GBK + 1;
 // MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16] and B[l/16][l%16]; the four
 // accumulator values of lane l are D[Op::row(l/16, r)][l%16].
 template <typename T>
-__global__ void __launch_bounds__(256) gemm_nn_kernel(int m, int n, int k, T alpha, const T* __restrict__ A,
-                                                      int lda, const T* __restrict__ B, int ldb, T beta,
-                                                      T* __restrict__ C, int ldc, int tiles_m) {
+__device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const T* __restrict__ A, int lda,
+                                             const T* __restrict__ B, int ldb, T beta, T* __restrict__ C, int ldc,
+                                             int tm, int tn) {
     using Op = MfmaOp<T>;
     using Acc = typename Op::acc;
     __shared__ T As[2][GBM][GPAD];
     __shared__ T Bs[2][GBN][GPAD];
 
-    int tm, tn;
-    tile_of(blockIdx.x, gridDim.x, tiles_m, tm, tn);
     const int m0 = tm * GBM, n0 = tn * GBN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
@@ -165,6 +165,26 @@ __global__ void __launch_bounds__(256) gemm_nn_kernel(int m, int n, int k, T alp
                 }
             }
         }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) gemm_nn_kernel(int m, int n, int k, T alpha, const T* __restrict__ A,
+                                                      int lda, const T* __restrict__ B, int ldb, T beta,
+                                                      T* __restrict__ C, int ldc, int tiles_m) {
+    int tm, tn;
+    tile_of(blockIdx.x, gridDim.x, tiles_m, tm, tn);
+    gemm_nn_tile<T>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn);
+}
+
+// Many independent GEMMs C_j = alpha_j A_j B_j + beta_j C_j in one launch: workgroup
+// (x, y) computes tile x (row tiles fastest) of job y and returns when job y has fewer.
+template <typename T>
+__global__ void __launch_bounds__(256) gemm_nn_batch_kernel(const GemmJob<T>* __restrict__ jobs) {
+    const GemmJob<T>& g = jobs[blockIdx.y];
+    const int tiles_m = (g.m + GBM - 1) / GBM, tiles_n = (g.n + GBN - 1) / GBN;
+    if ((int)blockIdx.x >= tiles_m * tiles_n) return;
+    gemm_nn_tile<T>(g.m, g.n, g.k, g.alpha, g.A, g.lda, g.B, g.ldb, g.beta, g.C, g.ldc, blockIdx.x % tiles_m,
+                    blockIdx.x / tiles_m);
 }
 
 // ------------------------------------------------------------------ 16-bit MFMA GEMM
@@ -543,6 +563,108 @@ static ScaleWork g_scale;
 
 static DBuf<char> g_q16;
 
+// ------------------------------------------------------------------ tile autotuner
+// The 16-bit GEMM on converted operands comes in four workgroup tiles (128 or 256 rows by
+// 128 or 256 columns). The fastest depends on the shape: how many workgroups the product
+// gives the 256 CUs, how long its k loop is, how much of C is padding. The choice is
+// learned per shape class (operand type and the binary exponents of m, n and k) from live
+// calls: the first calls of a class each run another candidate between two events, and
+// once every candidate's time is in, the class uses the fastest. Every candidate computes
+// the same product, so exploring costs time, never correctness. sqbem_tune(0) restores
+// the fixed rule.
+namespace tune {
+constexpr int NC = 4;  // 0: 128 x 128, 1: 128 x 256, 2: 256 x 128, 3: 256 x 256
+struct Cls {
+    int tried = 0, best = -1;
+    float ms[NC] = {0, 0, 0, 0};
+    bool done[NC] = {false, false, false, false};
+    hipEvent_t e0[NC] = {nullptr, nullptr, nullptr, nullptr}, e1[NC] = {nullptr, nullptr, nullptr, nullptr};
+};
+static std::map<unsigned, Cls> g_table;
+static int g_mode = 1;
+
+static int lg(int v) {
+    int l = 0;
+    while ((1 << (l + 1)) <= v && l < 30) ++l;
+    return l;
+}
+
+static void poll(Cls& c) {
+    bool all = c.tried == NC;
+    for (int i = 0; i < c.tried; ++i) {
+        if (c.done[i]) continue;
+        if (hipEventQuery(c.e1[i]) == hipSuccess) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, c.e0[i], c.e1[i]) == hipSuccess) c.ms[i] = ms;
+            (void)hipEventDestroy(c.e0[i]);
+            (void)hipEventDestroy(c.e1[i]);
+            c.e0[i] = c.e1[i] = nullptr;
+            c.done[i] = true;
+        } else {
+            all = false;
+        }
+    }
+    if (all && c.best < 0) {
+        int b = 0;
+        for (int i = 1; i < NC; ++i)
+            if (c.ms[i] < c.ms[b]) b = i;
+        c.best = b;
+    }
+}
+
+// Candidate for one call, with its timing events around the call when it explores.
+struct Probe {
+    int cand;
+    Cls* cls = nullptr;
+    int slot = -1;
+    Probe(int kind, int m, int n, int k, int rule) : cand(rule) {
+        if (!g_mode) return;
+        Cls& c = g_table[(unsigned)kind << 24 | (unsigned)lg(m) << 16 | (unsigned)lg(n) << 8 | (unsigned)lg(k)];
+        poll(c);
+        if (c.best >= 0) {
+            cand = c.best;
+        } else if (c.tried < NC) {
+            slot = c.tried++;
+            cand = slot;
+            cls = &c;
+            SQ_HIP(hipEventCreate(&c.e0[slot]));
+            SQ_HIP(hipEventCreate(&c.e1[slot]));
+            SQ_HIP(hipEventRecord(c.e0[slot], ctx().s()));
+        }
+    }
+    ~Probe() {
+        if (cls) (void)hipEventRecord(cls->e1[slot], ctx().s());
+    }
+};
+
+static void reset(int mode) {
+    SQ_HIP(hipStreamSynchronize(ctx().s()));
+    for (auto& kv : g_table)
+        for (int i = 0; i < NC; ++i) {
+            if (kv.second.e0[i]) (void)hipEventDestroy(kv.second.e0[i]);
+            if (kv.second.e1[i]) (void)hipEventDestroy(kv.second.e1[i]);
+        }
+    g_table.clear();
+    g_mode = mode;
+}
+
+static void info(int* classes, int* decided, int* counts) {
+    SQ_HIP(hipStreamSynchronize(ctx().s()));
+    int d = 0;
+    if (counts)
+        for (int i = 0; i < NC; ++i) counts[i] = 0;
+    for (auto& kv : g_table) {
+        poll(kv.second);
+        if (kv.second.best >= 0) {
+            ++d;
+            if (counts) ++counts[kv.second.best];
+        }
+    }
+    if (classes) *classes = (int)g_table.size();
+    if (decided) *decided = d;
+}
+}  // namespace tune
+
 template <bool BF16, int TM, int TN>
 static void gemm16q(int m, int n, int k, float alpha, const float* A, int lda, const float* B, int ldb, float beta,
                     float* C, int ldc, float* rs, float* cs) {
@@ -587,15 +709,21 @@ void gemm16(int kind, int m, int n, int k, float alpha, const float* A, int lda,
             hipLaunchKernelGGL(bits_to_pow2_scale, dim3(ceil_div(m, 256)), dim3(256), 0, ctx().s(), m, mx, rs);
             hipLaunchKernelGGL(pow2_scale_cols, dim3(ceil_div(n, 4)), dim3(256), 0, ctx().s(), k, n, B, ldb, cs);
         }
-        // 256 x 256 tiles once they fill the device, 128 x 128 tiles below that
-        const bool big = ceil_div(m, 256) * ceil_div(n, 256) >= 192;
-        if (bf) {
-            if (big) gemm16q<true, 256, 256>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs);
-            else gemm16q<true, 128, 128>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs);
-        } else {
-            if (big) gemm16q<false, 256, 256>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs);
-            else gemm16q<false, 128, 128>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs);
-        }
+        // fixed rule: 256 x 256 tiles once they fill the device, 128 x 128 below that;
+        // the autotuner replaces it per shape class once it has timed every tile
+        const int rule = ceil_div(m, 256) * ceil_div(n, 256) >= 192 ? 3 : 0;
+        tune::Probe probe(bf ? 1 : 0, m, n, k, rule);
+        auto run = [&](auto tag_bf) {
+            constexpr bool BFv = decltype(tag_bf)::value;
+            switch (probe.cand) {
+                case 0: gemm16q<BFv, 128, 128>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs); break;
+                case 1: gemm16q<BFv, 128, 256>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs); break;
+                case 2: gemm16q<BFv, 256, 128>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs); break;
+                default: gemm16q<BFv, 256, 256>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs); break;
+            }
+        };
+        if (bf) run(std::true_type{});
+        else run(std::false_type{});
         return;
     }
     float *rs = nullptr, *cs = nullptr;
@@ -1527,12 +1655,10 @@ __device__ __forceinline__ void lu128_inv_column(double* M, double* Tmp, int t) 
 // inverses. factor = 0: the blocks are already factored; workgroup b inverts diagonal
 // block b (A advanced by b NB (lda + 1), inverses by b NB^2).
 template <typename T>
-__global__ void __launch_bounds__(LU_THREADS) lu128_kernel(T* A, int lda, int n, T* Linv, T* Uinv, int* flag,
-                                                           int factor) {
+__device__ __forceinline__ void lu128_block(T* A, int lda, int n, T* Linv, T* Uinv, int* flag, int factor, int b) {
     __shared__ double M[NB * LU_LD];  // the block, row-major: M[i * LU_LD + j] (132 KiB)
     __shared__ double Tmp[96 * 33];   // products of the inversion (25 KiB)
     __shared__ int bad;
-    const int b = blockIdx.x;
     A += (size_t)b * NB * ((size_t)lda + 1);
     Linv += (size_t)b * NB * NB;
     Uinv += (size_t)b * NB * NB;
@@ -1633,6 +1759,27 @@ __global__ void __launch_bounds__(LU_THREADS) lu128_kernel(T* A, int lda, int n,
 }
 
 template <typename T>
+__global__ void __launch_bounds__(LU_THREADS) lu128_kernel(T* A, int lda, int n, T* Linv, T* Uinv, int* flag,
+                                                           int factor) {
+    lu128_block<T>(A, lda, n, Linv, Uinv, flag, factor, blockIdx.x);
+}
+
+// one <= 128 x 128 diagonal block per workgroup, each of its own matrix (factor and invert)
+template <typename T>
+struct Lu128Job {
+    T* A;
+    int lda, n;
+    T* Linv;
+    T* Uinv;
+    int* flag;
+};
+template <typename T>
+__global__ void __launch_bounds__(LU_THREADS) lu128_batch_kernel(const Lu128Job<T>* __restrict__ jobs) {
+    const Lu128Job<T>& j = jobs[blockIdx.x];
+    lu128_block<T>(j.A, j.lda, j.n, j.Linv, j.Uinv, j.flag, 1, 0);
+}
+
+template <typename T>
 static void lu128_launch(int grid, T* A, int lda, int n, T* Linv, T* Uinv, int* flag, int factor) {
     hipLaunchKernelGGL(lu128_kernel<T>, dim3(grid), dim3(LU_THREADS), 0, ctx().s(), A, lda, n, Linv, Uinv, flag,
                        factor);
@@ -1717,12 +1864,11 @@ static unsigned long long g_trsv_base = 0;
 // (staged in LDS while the workgroup waits) with the accumulated right-hand side, spread
 // over all 256 threads. Thread t owns row t % 128 and column half t / 128 of every tile.
 template <typename T, bool LOWER, int NR>
-__global__ void __launch_bounds__(256) trsv_inv_chain(int n, int nrhs, const T* __restrict__ Lt,
-                                                      const T* __restrict__ Dinv, T* B, int ldb, TrsvState* st,
-                                                      unsigned long long base, int* err) {
+__device__ __forceinline__ void trsv_inv_body(int n, int nrhs, const T* __restrict__ Lt, const T* __restrict__ Dinv,
+                                              T* B, int ldb, TrsvState* st, unsigned long long base, int* err,
+                                              unsigned char* tv_smem) {
     constexpr int GQ = sizeof(T) == 4 ? 2 : 1;
     constexpr int DLD = NB + 1;
-    extern __shared__ __attribute__((aligned(16))) unsigned char tv_smem[];
     T* Dv = reinterpret_cast<T*>(tv_smem);                                    // [NB][DLD]
     // the tile buffer (main loop) and the reduction buffers (after it) share one region
     T(*Xs)[NR][NB] = reinterpret_cast<T(*)[NR][NB]>(tv_smem + sizeof(T) * NB * DLD);  // [GQ][NR][NB]
@@ -1737,6 +1883,7 @@ __global__ void __launch_bounds__(256) trsv_inv_chain(int n, int nrhs, const T* 
         s_k = (int)(__hip_atomic_fetch_add(&st->ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - base);
     __syncthreads();
     const int k = s_k;
+    if (k >= nb) return;  // surplus workgroup of a batched launch (whole workgroup)
     const int blk = LOWER ? k : nb - 1 - k;
     const int r0 = blk * NB, rb = min(NB, n - r0);
     {
@@ -1837,6 +1984,32 @@ __global__ void __launch_bounds__(256) trsv_inv_chain(int n, int nrhs, const T* 
     __syncthreads();
     if (tid == 0)
         __hip_atomic_fetch_max(&st->done, base + (unsigned long long)k + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T, bool LOWER, int NR>
+__global__ void __launch_bounds__(256) trsv_inv_chain(int n, int nrhs, const T* __restrict__ Lt,
+                                                      const T* __restrict__ Dinv, T* B, int ldb, TrsvState* st,
+                                                      unsigned long long base, int* err) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char tv_smem[];
+    trsv_inv_body<T, LOWER, NR>(n, nrhs, Lt, Dinv, B, ldb, st, base, err, tv_smem);
+}
+
+// Independent chained solves, one right-hand side each, in one launch: row y of the grid
+// serves job y with its own (zeroed) ticket and completion counters; workgroups beyond a
+// job's block count leave at once.
+template <typename T>
+struct TrsvJob {
+    int n;
+    const T* Lt;
+    const T* Dinv;
+    T* B;
+};
+template <typename T, bool LOWER>
+__global__ void __launch_bounds__(256) trsv_inv_batch_kernel(const TrsvJob<T>* __restrict__ jobs, TrsvState* st,
+                                                             int* err) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char tv_smem[];
+    const TrsvJob<T>& j = jobs[blockIdx.y];
+    trsv_inv_body<T, LOWER, 1>(j.n, 1, j.Lt, j.Dinv, j.B, j.n, st + blockIdx.y, 0ull, err, tv_smem);
 }
 
 template <typename T, bool LOWER, int NR>
@@ -2038,6 +2211,253 @@ void getrs(int kind, int trans, int n, int nrhs, const T* LU, int lda, const int
 template void getrs<double>(int, int, int, int, const double*, int, const int*, double*, int, const double*);
 template void getrs<float>(int, int, int, int, const float*, int, const int*, float*, int, const float*);
 
+// ------------------------------------------------------------------ batched factorisations
+// A batch is planned on the host first: the recursion over the largest matrix is walked
+// once, every step's per-matrix work (clipped to that matrix's order; matrices already
+// finished drop out) becomes one job table, and all tables go to the device in a single
+// copy before the launches are issued in order. A step is then one launch for the whole
+// batch, so a batch of k matrices costs the launches of one factorisation, and the small
+// diagonal-block kernels of different matrices run side by side.
+template <typename T>
+__global__ void __launch_bounds__(256) check_unit_bound_batch_kernel(const GemmJob<T>* __restrict__ jobs) {
+    const GemmJob<T>& g = jobs[blockIdx.y];
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    const size_t total = (size_t)g.m * g.n;
+    int mine = 0;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(idx % g.m), j = (int)(idx / g.m);
+        if (!(fabs((double)g.C[i + (size_t)j * g.ldc]) <= 1.0)) mine = 1;
+    }
+    if (mine) bad = 1;
+    __syncthreads();
+    if (threadIdx.x == 0 && bad) atomicOr((int*)g.A, 1);  // A carries the flag of the job
+}
+
+namespace {
+enum PlanKind { PK_LU128, PK_GEMM, PK_CHECK };
+struct PlanStep {
+    int kind;
+    size_t off;
+    int count, gridx;
+};
+struct Plan {
+    std::vector<unsigned char> bytes;
+    std::vector<PlanStep> steps;
+    template <typename J>
+    size_t add(const std::vector<J>& v) {
+        size_t off = (bytes.size() + 63) / 64 * 64;
+        bytes.resize(off + v.size() * sizeof(J));
+        std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(J));
+        return off;
+    }
+};
+DBuf<unsigned char> g_plan_dev;
+
+template <typename T>
+struct NpPlanner {
+    Plan& P;
+    const std::vector<BatchMat<T>>& M;
+    struct TJ {
+        const T* L;
+        int ldl;
+        const T* inv;
+        T* B;
+        int ldb, nrhs;
+    };
+    static size_t nbk(int n) { return (size_t)(n + NB - 1) / NB; }
+    T* Linv(int i, int base) const { return M[i].aux + (size_t)(base / NB) * NB * NB; }
+    T* Uinv(int i, int base) const { return M[i].aux + (nbk(M[i].n) + (size_t)(base / NB)) * NB * NB; }
+    void gemm(const std::vector<GemmJob<T>>& g, int kind = PK_GEMM) {
+        if (g.empty()) return;
+        int tiles = 1;
+        for (const auto& j : g) tiles = std::max(tiles, (int)(ceil_div(j.m, GBM) * ceil_div(j.n, GBN)));
+        if (kind == PK_CHECK) tiles = 64;
+        P.steps.push_back({kind, P.add(g), (int)g.size(), tiles});
+    }
+    void trsm(int n, const std::vector<TJ>& J) {
+        if (J.empty()) return;
+        std::vector<GemmJob<T>> g;
+        if (n <= NB) {
+            // X = inv B in place: one row tile, each workgroup reads its columns before writing
+            for (const TJ& j : J) g.push_back({n, j.nrhs, n, T(1), T(0), j.inv, NB, j.B, j.ldb, j.B, j.ldb});
+            gemm(g);
+            return;
+        }
+        const int n1 = split_point(n, NB), n2 = n - n1;
+        trsm(n1, J);
+        for (const TJ& j : J) g.push_back({n2, j.nrhs, n1, T(-1), T(1), j.L + n1, j.ldl, j.B, j.ldb, j.B + n1, j.ldb});
+        gemm(g);
+        std::vector<TJ> J2;
+        for (const TJ& j : J)
+            J2.push_back({j.L + n1 + (size_t)n1 * j.ldl, j.ldl, j.inv + (size_t)(n1 / NB) * NB * NB, j.B + n1, j.ldb,
+                          j.nrhs});
+        trsm(n2, J2);
+    }
+    void rec(int base, int w) {
+        const int nm = (int)M.size();
+        if (w <= NB) {
+            std::vector<Lu128Job<T>> lu;
+            std::vector<GemmJob<T>> g, chk;
+            for (int i = 0; i < nm; ++i) {
+                const int wi = std::min(w, M[i].n - base);
+                if (wi <= 0) continue;
+                T* Ad = M[i].A + base + (size_t)base * M[i].lda;
+                lu.push_back({Ad, M[i].lda, wi, Linv(i, base), Uinv(i, base), M[i].flag});
+                const int mi = M[i].n - base - wi;
+                if (mi > 0) {
+                    // L21 = A21 U11^-1 in place (one column tile)
+                    g.push_back({mi, wi, wi, T(1), T(0), Ad + wi, M[i].lda, Uinv(i, base), NB, Ad + wi, M[i].lda});
+                    chk.push_back({mi, wi, 0, T(0), T(0), (const T*)M[i].flag, 0, nullptr, 0, Ad + wi, M[i].lda});
+                }
+            }
+            if (!lu.empty()) P.steps.push_back({PK_LU128, P.add(lu), (int)lu.size(), (int)lu.size()});
+            gemm(g);
+            gemm(chk, PK_CHECK);
+            return;
+        }
+        const int w1 = split_point(w, NB), w2 = w - w1;
+        rec(base, w1);
+        std::vector<TJ> J;
+        std::vector<GemmJob<T>> g;
+        for (int i = 0; i < nm; ++i) {
+            const int w2i = std::min(w2, M[i].n - base - w1);
+            if (w2i <= 0) continue;
+            const int ld = M[i].lda;
+            T* Ad = M[i].A + base + (size_t)base * ld;
+            T* A12 = Ad + (size_t)w1 * ld;
+            J.push_back({Ad, ld, Linv(i, base), A12, ld, w2i});
+            g.push_back({M[i].n - base - w1, w2i, w1, T(-1), T(1), Ad + w1, ld, A12, ld, A12 + w1, ld});
+        }
+        trsm(w1, J);
+        gemm(g);
+        rec(base + w1, w2);
+    }
+};
+
+template <typename T>
+void run_plan(const Plan& P) {
+    if (P.steps.empty()) return;
+    if (g_plan_dev.n < P.bytes.size()) g_plan_dev.alloc(std::max<size_t>(P.bytes.size(), 1 << 20));
+    SQ_HIP(hipMemcpyAsync(g_plan_dev.p, P.bytes.data(), P.bytes.size(), hipMemcpyHostToDevice, ctx().s()));
+    for (const PlanStep& s : P.steps) {
+        const void* tab = g_plan_dev.p + s.off;
+        if (s.kind == PK_LU128) {
+            hipLaunchKernelGGL(lu128_batch_kernel<T>, dim3(s.count), dim3(LU_THREADS), 0, ctx().s(),
+                               (const Lu128Job<T>*)tab);
+        } else if (s.kind == PK_GEMM) {
+            hipLaunchKernelGGL(gemm_nn_batch_kernel<T>, dim3(s.gridx, s.count), dim3(256), 0, ctx().s(),
+                               (const GemmJob<T>*)tab);
+        } else {
+            hipLaunchKernelGGL(check_unit_bound_batch_kernel<T>, dim3(s.gridx, s.count), dim3(256), 0, ctx().s(),
+                               (const GemmJob<T>*)tab);
+        }
+        check_launch("batched factorisation step");
+    }
+    sync();  // the host plan is staging for the next batch
+}
+}  // namespace
+
+template <typename T>
+void getrf_np_batch(const std::vector<BatchMat<T>>& mats) {
+    int nmax = 0;
+    for (const auto& m : mats) {
+        SQ_REQUIRE(m.n >= 0 && m.lda >= std::max(1, m.n) && m.A && m.aux && m.flag, SQBEM_ERR_ARG,
+                   "getrf_np_batch: bad matrix");
+        nmax = std::max(nmax, m.n);
+    }
+    if (!nmax) return;
+    for (const auto& m : mats) fill_zero(m.flag, sizeof(int));
+    Plan P;
+    NpPlanner<T> pl{P, mats};
+    pl.rec(0, nmax);
+    double flops = 0;
+    for (const auto& m : mats) flops += (2.0 / 3.0) * m.n * (double)m.n * m.n;
+    {
+        ProfScope prof(PROF_PANEL, flops, 0.0);
+        run_plan<T>(P);
+    }
+    for (const auto& m : mats)
+        if (m.n > 0) tile_copy<T>(m.n, m.A, m.lda, m.aux + 2 * NpPlanner<T>::nbk(m.n) * NB * NB);
+}
+template void getrf_np_batch<float>(const std::vector<BatchMat<float>>&);
+template void getrf_np_batch<double>(const std::vector<BatchMat<double>>&);
+
+static DBuf<TrsvState> g_trsv_batch;
+static DBuf<unsigned char> g_trsv_jobs;
+
+template <typename T, bool LOWER>
+static void trsv_batch_launch(const std::vector<TrsvJob<T>>& J, const void* dev_jobs, int nbmax) {
+    constexpr int GQ = sizeof(T) == 4 ? 2 : 1;
+    constexpr size_t xs = sizeof(T) * GQ * NB, red = sizeof(double) * 3 * NB;
+    constexpr size_t lds = sizeof(T) * NB * (NB + 1) + (xs > red ? xs : red);
+    static bool lds_set = false;
+    if (!lds_set) {
+        SQ_HIP(hipFuncSetAttribute((const void*)trsv_inv_batch_kernel<T, LOWER>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        lds_set = true;
+    }
+    fill_zero(g_trsv_batch.p, sizeof(TrsvState) * J.size());
+    hipLaunchKernelGGL((trsv_inv_batch_kernel<T, LOWER>), dim3(nbmax, (unsigned)J.size()), dim3(256), lds, ctx().s(),
+                       (const TrsvJob<T>*)dev_jobs, g_trsv_batch.p, device_error_word());
+    check_launch("trsv_inv_batch");
+}
+
+template <typename T>
+void getrs_np_batch(const std::vector<BatchSolve<T>>& s) {
+    if (s.empty()) return;
+    std::vector<TrsvJob<T>> L, U;
+    int nbmax = 0;
+    double flops = 0, bytes = 0;
+    for (const auto& q : s) {
+        const size_t nb = (size_t)(q.n + NB - 1) / NB;
+        nbmax = std::max(nbmax, (int)nb);
+        L.push_back({q.n, q.aux + 2 * nb * NB * NB, q.aux, q.B});
+        U.push_back({q.n, q.aux + 2 * nb * NB * NB, q.aux + nb * NB * NB, q.B});
+        flops += 2.0 * q.n * (double)q.n;
+        bytes += sizeof(T) * (double)q.n * q.n;
+    }
+    if (!nbmax) return;
+    if (g_trsv_batch.n < s.size()) g_trsv_batch.alloc(std::max<size_t>(s.size(), 256));
+    const size_t half = (L.size() * sizeof(TrsvJob<T>) + 63) / 64 * 64;
+    if (g_trsv_jobs.n < 2 * half) g_trsv_jobs.alloc(std::max<size_t>(2 * half, 1 << 16));
+    std::vector<unsigned char> h(2 * half);
+    std::memcpy(h.data(), L.data(), L.size() * sizeof(TrsvJob<T>));
+    std::memcpy(h.data() + half, U.data(), U.size() * sizeof(TrsvJob<T>));
+    SQ_HIP(hipMemcpyAsync(g_trsv_jobs.p, h.data(), h.size(), hipMemcpyHostToDevice, ctx().s()));
+    mark_device_error_check();
+    ProfScope prof(PROF_TRSV, flops, bytes);
+    trsv_batch_launch<T, true>(L, g_trsv_jobs.p, nbmax);
+    trsv_batch_launch<T, false>(U, g_trsv_jobs.p + half, nbmax);
+    sync();  // h is staging; also surfaces a timed-out wait
+}
+template void getrs_np_batch<float>(const std::vector<BatchSolve<float>>&);
+template void getrs_np_batch<double>(const std::vector<BatchSolve<double>>&);
+
+template <typename T>
+void gemm_batch(const std::vector<GemmJob<T>>& jobs) {
+    std::vector<GemmJob<T>> g;
+    for (const auto& j : jobs)
+        if (j.m > 0 && j.n > 0) {
+            SQ_REQUIRE(j.k > 0, SQBEM_ERR_ARG, "gemm_batch: k must be positive");
+            g.push_back(j);
+        }
+    if (g.empty()) return;
+    Plan P;
+    int tiles = 1;
+    double flops = 0;
+    for (const auto& j : g) {
+        tiles = std::max(tiles, (int)(ceil_div(j.m, GBM) * ceil_div(j.n, GBN)));
+        flops += 2.0 * j.m * (double)j.n * j.k;
+    }
+    P.steps.push_back({PK_GEMM, P.add(g), (int)g.size(), tiles});
+    ProfScope prof(sizeof(T) == 8 ? PROF_DGEMM : PROF_SGEMM, flops, 0.0);
+    run_plan<T>(P);
+}
+template void gemm_batch<float>(const std::vector<GemmJob<float>>&);
+template void gemm_batch<double>(const std::vector<GemmJob<double>>&);
+
 }  // namespace sq
 
 // ------------------------------------------------------------------ C ABI
@@ -2121,6 +2541,17 @@ int sqbem_sgetrf(int kind, int n, float* A, int lda, int* ipiv, int* info) {
         if (info) *info = i;
         SQ_REQUIRE(i == 0, SQBEM_ERR_SINGULAR, "dgetrf failed");
     });
+}
+
+int sqbem_tune(int mode) {
+    return guarded([&] {
+        SQ_REQUIRE(mode == 0 || mode == 1, SQBEM_ERR_ARG, "sqbem_tune: mode must be 0 (fixed rule) or 1 (online)");
+        tune::reset(mode);
+    });
+}
+
+int sqbem_tune_info(int* classes, int* decided, int* counts) {
+    return guarded([&] { tune::info(classes, decided, counts); });
 }
 
 int sqbem_sgetrs(int n, int nrhs, const float* LU, int lda, const int* ipiv, float* B, int ldb) {

@@ -1,6 +1,7 @@
 // dense.hpp -- internal interface of the dense linear algebra (dense.hip).
 #pragma once
 #include <cstddef>
+#include <vector>
 
 namespace sq {
 
@@ -34,5 +35,43 @@ inline size_t getrf_aux_elems(int n) {
 template <typename T>
 void getrs(int kind, int trans, int n, int nrhs, const T* LU, int lda, const int* ipiv, T* B, int ldb,
            const T* aux = nullptr);
+
+// ---- batched forms (one launch per step for all matrices of a batch)
+// C_j = alpha_j A_j B_j + beta_j C_j (column-major, storage-precision MFMA)
+template <typename T>
+struct GemmJob {
+    int m, n, k;
+    T alpha, beta;
+    const T* A;
+    int lda;
+    const T* B;
+    int ldb;
+    T* C;
+    int ldc;
+};
+template <typename T>
+void gemm_batch(const std::vector<GemmJob<T>>& jobs);
+// LU without row interchanges of independent square matrices of any orders, in storage
+// precision; aux (getrf_aux_elems(n)) receives the solve aid as getrf's; *flag is set
+// when a multiplier exceeds 1 in magnitude or is not finite, i.e. when partial pivoting
+// would have interchanged rows (the factors are then not the pivoted ones: refactor).
+template <typename T>
+struct BatchMat {
+    T* A;
+    int lda, n;
+    T* aux;
+    int* flag;
+};
+template <typename T>
+void getrf_np_batch(const std::vector<BatchMat<T>>& mats);
+// B_j <- A_j^-1 B_j, one right-hand side each, from getrf_np_batch factors (their aux)
+template <typename T>
+struct BatchSolve {
+    int n;
+    const T* aux;
+    T* B;
+};
+template <typename T>
+void getrs_np_batch(const std::vector<BatchSolve<T>>& s);
 
 }  // namespace sq

@@ -34,6 +34,8 @@ _SIGS = {
     "sqbem_device_info": (_i, [ctypes.c_char_p, _i, _vp, _vp]),
     "sqbem_profile": (_i, [_i]),
     "sqbem_profile_get": (_i, [_i, _vp, _vp, _vp, _vp]),
+    "sqbem_tune": (_i, [_i]),
+    "sqbem_tune_info": (_i, [_vp, _vp, _vp]),
     "sqbem_mesh_swimmer_tube": (_vp, [_d, _d, _d, _i, _i]),
     "sqbem_mesh_tube": (_vp, [_d, _vp, _i, _i, _i]),
     "sqbem_mesh_duct": (_vp, [_d, _d, _i, _i, _i]),
@@ -173,6 +175,22 @@ def profile_get(name):
 
 
 # ------------------------------------------------------------------ meshes
+def tune(mode: int = 1):
+    """Tile autotuner of the 16-bit GEMMs: 1 learns per shape class from live calls, 0 uses
+    the fixed rule; either clears what was learned."""
+    check(lib().sqbem_tune(int(mode)))
+
+
+TUNE_TILES = ["128x128", "128x256", "256x128", "256x256"]
+
+
+def tune_info() -> dict:
+    cl, dec = ctypes.c_int(), ctypes.c_int()
+    counts = np.zeros(4, dtype=np.int32)
+    check(lib().sqbem_tune_info(ctypes.byref(cl), ctypes.byref(dec), _ptr(counts)))
+    return {"classes": cl.value, "decided": dec.value, "tiles": dict(zip(TUNE_TILES, counts.tolist()))}
+
+
 class Mesh:
     """bem::Mesh (src/utils/mesh.hpp:50-91) owned by libsqbem; geometry read back as numpy."""
 
@@ -550,12 +568,13 @@ def batch_solve(instances, mu=1.0, precision=FP64, mem_budget_gb=0.0) -> BatchRe
     Om = np.zeros((n, 3))
     it = np.zeros(n, dtype=np.int32)
     res = np.zeros(n)
-    st = np.zeros(8)
+    st = np.zeros(10)
     check(lib().sqbem_batch_solve(n, ctypes.cast(tubes, ctypes.c_void_p), ctypes.cast(spheres, ctypes.c_void_p),
                                   _ptr(c), _ptr(e), _ptr(B), mu, precision, float(mem_budget_gb), _ptr(U), _ptr(Om),
                                   _ptr(it), _ptr(res), _ptr(st)))
     stats = {"tube_s": st[0], "assembly_s": st[1], "schur_s": st[2], "solve_s": st[3], "unique_tubes": int(st[4]),
-             "chunks": int(st[5]), "max_iters": int(st[6]), "worst_residual": st[7]}
+             "chunks": int(st[5]), "max_iters": int(st[6]), "worst_residual": st[7], "pivoted": int(st[8]),
+             "workspace_gb": st[9] / 1e9}
     return BatchResult(U, Om, it, res, stats)
 
 

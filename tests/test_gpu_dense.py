@@ -93,6 +93,51 @@ def test_gemm16(sq, kind, tol, m, n, k):
     assert err.max() < tol, err.max()
 
 
+@pytest.mark.parametrize("kind,tol", [(2, 2.5e-3), (3, 2e-2)])
+def test_gemm16_autotuner_every_tile(sq, kind, tol):
+    """The tile autotuner runs each of its four workgroup tiles once on a fresh shape class
+    (odd sizes, beta != 0) before settling: every call's product must be right, and the
+    class must end decided."""
+    m, n, k = 700, 555, 300
+    rng = np.random.default_rng(7 + kind)
+    A = (rng.standard_normal((m, k)) * np.logspace(-3, 2, m)[:, None]).astype(np.float32)
+    B = rng.standard_normal((k, n)).astype(np.float32)
+    C0 = rng.standard_normal((m, n)).astype(np.float32)
+    ref = 0.5 * (A.astype(np.float64) @ B.astype(np.float64)) - 2.0 * C0
+    scale = 0.5 * (np.abs(A).astype(np.float64) @ np.abs(B).astype(np.float64)) + 2.0 * np.abs(C0)
+    sq.tune(1)
+    try:
+        for call in range(6):
+            dA, dB, dC = dev(A), dev(B), dev(C0)
+            torch.cuda.synchronize()
+            sq.gemm16(kind, 0.5, dA, dB, -2.0, dC)
+            sq.sync()
+            err = np.abs(host(dC) - ref) / scale
+            assert err.max() < tol, (call, err.max())
+        info = sq.tune_info()
+        assert info["classes"] >= 1 and info["decided"] >= 1, info
+    finally:
+        sq.tune(1)
+
+
+def test_gemm16_fixed_rule_mode(sq):
+    m, n, k = 512, 512, 256
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((m, k)).astype(np.float32)
+    B = rng.standard_normal((k, n)).astype(np.float32)
+    sq.tune(0)
+    try:
+        dA, dB, dC = dev(A), dev(B), dev(np.zeros((m, n), dtype=np.float32))
+        torch.cuda.synchronize()
+        sq.gemm16(2, 1.0, dA, dB, 0.0, dC)
+        sq.sync()
+        ref = A.astype(np.float64) @ B.astype(np.float64)
+        assert np.abs(host(dC) - ref).max() < 2.5e-3 * np.abs(ref).max()
+        assert sq.tune_info()["classes"] == 0
+    finally:
+        sq.tune(1)
+
+
 @pytest.mark.parametrize("n", [1, 5, 31, 32, 33, 64, 65, 100, 257, 1000])
 def test_dgetrf_matches_lapack(sq, n):
     rng = np.random.default_rng(n)
