@@ -318,6 +318,10 @@ def main():
     ap.add_argument("--no-grad", action="store_true", help="skip the adjoint-gradient timing")
     ap.add_argument("--no-field", action="store_true", help="skip the 57k-DOF solve + field grid")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-configuration batched solve")
+    ap.add_argument("--tune-table", default="shipped",
+                    help="GEMM tile table to start from: 'shipped' (the package's tune_gfx950.txt), a path, "
+                         "or 'none' (learn every class online in the warm-up)")
+    ap.add_argument("--tune-save", default="", metavar="PATH", help="write the tile table after the run")
     ap.add_argument("--calib-copy-gib", type=float, default=0.0, metavar="GIB",
                     help="also run one plain device-to-device copy of GIB GiB (reference kernel for HBM counters)")
     ap.add_argument("--traj-grad", type=int, default=0, metavar="STEPS",
@@ -336,6 +340,11 @@ def main():
         torch.cuda.init()
     pkg = ge.load_package()
     pkg.init(0)
+    tune_loaded = 0
+    if args.tune_table != "none":
+        path = pkg.TUNE_TABLE if args.tune_table == "shipped" else args.tune_table
+        if os.path.exists(path):
+            tune_loaded = pkg.tune_load(path)
     W = workload(args)
     e = np.array([0.0, 0.0, 1.0])
     prec = MODES[args.mode]
@@ -358,7 +367,8 @@ def main():
     value = args.steps / elapsed
     dof = 3 * (phases["Nt"] + phases["Ns"]) + 6
 
-    detail = {"dof": dof, "phases_last_step": phases, "precision": args.mode, "tile_autotuner": pkg.tune_info()}
+    detail = {"dof": dof, "phases_last_step": phases, "precision": args.mode,
+              "tile_autotuner": dict(pkg.tune_info(), loaded_from_table=tune_loaded)}
     if not args.no_fp64 and prec != 0:
         U64, Om64, fs64, ph64 = cold_solve(pkg, W, 0, e)
         detail["fp64_solve_s"] = ph64["total_s"]
@@ -441,6 +451,8 @@ def main():
         "cpu_baseline": cpu,
         "detail": detail,
     }
+    if args.tune_save:
+        pkg.tune_save(args.tune_save)
     print(json.dumps(line), flush=True)
 
 

@@ -40,6 +40,10 @@ int sqbem_init(int device);
 int sqbem_set_stream(void* hip_stream); /* NULL: the library's own stream */
 int sqbem_sync(void);
 int sqbem_device_info(char* name, int name_len, int* cu_count, double* hbm_gb);
+/* Device memory the library releases is kept for reuse by later requests of the same size
+ * (repeated solves then allocate nothing); sqbem_trim returns all of it to the driver and
+ * reports how much (GB). An allocation that fails empties the cache and retries first. */
+int sqbem_trim(double* released_gb);
 /* Kernel timing (no reference counterpart; measurement only). sqbem_profile(mask) resets
  * the totals and records HIP events around every launch, on the library stream, of the
  * classes in mask (bit 1 << class; -1 = all classes; 0 = off); sqbem_profile_get returns
@@ -118,6 +122,11 @@ int sqbem_sgetrs(int n, int nrhs, const float* LU, int lda, const int* ipiv, flo
  * (128x128, 128x256, 256x128, 256x256); any pointer may be NULL. */
 int sqbem_tune(int mode);
 int sqbem_tune_info(int* classes, int* decided, int* counts);
+/* Decided classes to a text file, and back: a loaded class uses its saved tile from the
+ * first call (the package ships tune_gfx950.txt, learned on an MI355X); *classes (may be
+ * NULL) receives how many classes were read. */
+int sqbem_tune_save(const char* path);
+int sqbem_tune_load(const char* path, int* classes);
 
 /* Host-array forms of the same calls, for the C++ linalg.hpp API (Mat lives in host
  * memory): the library stages the operands through HBM. */

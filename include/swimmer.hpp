@@ -217,4 +217,64 @@ private:
     }
 };
 
+// ---------------------------------------------------------------- batched solves
+// Free-swimming U, Omega of many independent configurations in one call (an addition: the
+// reference loops over SwimmerSolver one configuration at a time, src/main.cpp:82-100).
+// Configuration i: tube mesh, sphere mesh placed at its centre, squirming direction e and
+// modes B1, B2 (slip squirmer_slip at the sphere centroids). Identical tube meshes share
+// one factorisation; the device pipeline is described at sqbem_batch_solve.
+struct BatchConfig {
+    const Mesh* tube = nullptr;
+    const Mesh* sphere = nullptr;
+    V3 center, e{0, 0, 1};
+    double B1 = 1, B2 = 0;
+};
+struct BatchResult {
+    std::vector<V3> U, Om;
+    std::vector<int> iters;       // refinement steps per configuration (0 in FP64 mode)
+    std::vector<double> resid;    // final relative residual of the full system
+    std::array<double, 10> stats; // see sqbem_batch_solve
+};
+inline BatchResult batch_solve(const std::vector<BatchConfig>& cfg, double mu = 1, int precision = SQBEM_FP64,
+                               double mem_budget_gb = 0) {
+    const int n = (int)cfg.size();
+    // one handle per distinct mesh object (identical contents are also shared by the library)
+    std::vector<std::shared_ptr<sqbem_mesh>> keep;
+    std::vector<const Mesh*> seen;
+    std::vector<sqbem_mesh*> handle_of;
+    auto handle = [&](const Mesh* M) -> const sqbem_mesh* {
+        if (!M) throw std::runtime_error("batch_solve: every configuration needs a tube and a sphere");
+        for (size_t j = 0; j < seen.size(); ++j)
+            if (seen[j] == M) return handle_of[j];
+        keep.push_back(detail::to_sqbem(*M));
+        seen.push_back(M);
+        handle_of.push_back(keep.back().get());
+        return keep.back().get();
+    };
+    std::vector<const sqbem_mesh*> tubes(n), spheres(n);
+    std::vector<double> c(3 * (size_t)n), e(3 * (size_t)n), B(2 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        tubes[i] = handle(cfg[i].tube);
+        spheres[i] = handle(cfg[i].sphere);
+        c[3 * i] = cfg[i].center.x; c[3 * i + 1] = cfg[i].center.y; c[3 * i + 2] = cfg[i].center.z;
+        e[3 * i] = cfg[i].e.x; e[3 * i + 1] = cfg[i].e.y; e[3 * i + 2] = cfg[i].e.z;
+        B[2 * i] = cfg[i].B1;
+        B[2 * i + 1] = cfg[i].B2;
+    }
+    std::vector<double> U(3 * (size_t)n), Om(3 * (size_t)n);
+    BatchResult r;
+    r.iters.resize(n);
+    r.resid.resize(n);
+    sqbem_check(sqbem_batch_solve(n, tubes.data(), spheres.data(), c.data(), e.data(), B.data(), mu, precision,
+                                  mem_budget_gb, U.data(), Om.data(), r.iters.data(), r.resid.data(),
+                                  r.stats.data()));
+    r.U.resize(n);
+    r.Om.resize(n);
+    for (int i = 0; i < n; ++i) {
+        r.U[i] = V3{U[3 * i], U[3 * i + 1], U[3 * i + 2]};
+        r.Om[i] = V3{Om[3 * i], Om[3 * i + 1], Om[3 * i + 2]};
+    }
+    return r;
+}
+
 }  // namespace bem

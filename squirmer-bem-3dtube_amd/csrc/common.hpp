@@ -49,6 +49,16 @@ void upload_tables();    // quadrature tables to constant memory (assemble.hip)
 // sequence stays safe inside captured graphs).
 void fill_zero(void* p, size_t bytes);
 
+// Device memory through a cache: a released block is kept (by its rounded size) and handed
+// to the next request of that size instead of going back to the driver, so repeated solves
+// of one size allocate nothing after the first. All library work is ordered on one stream,
+// which makes a block's reuse safe without synchronisation. A failed allocation empties
+// the cache and retries once.
+void* dev_alloc(size_t bytes);  // nullptr when the device is out of memory
+void dev_free(void* p);
+void dev_cache_release();       // return every cached block to the driver
+size_t dev_cache_bytes();
+
 // Owned device allocation (no copy, movable).
 template <typename T>
 struct DBuf {
@@ -67,16 +77,12 @@ struct DBuf {
     void alloc(size_t count) {
         release();
         if (count == 0) return;
-        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
-        if (e != hipSuccess) {
-            p = nullptr;
-            throw Error(SQBEM_ERR_NOMEM, "device allocation of " + std::to_string(count * sizeof(T)) +
-                                             " bytes failed: " + hipGetErrorString(e));
-        }
+        p = (T*)dev_alloc(count * sizeof(T));
+        if (!p) throw Error(SQBEM_ERR_NOMEM, "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
         n = count;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) dev_free(p);
         p = nullptr;
         n = 0;
     }

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <type_traits>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -646,6 +647,51 @@ static void reset(int mode) {
         }
     g_table.clear();
     g_mode = mode;
+}
+
+// decided classes as text, one per line: key, best tile, the four times (ms)
+static void save(const char* path) {
+    SQ_HIP(hipStreamSynchronize(ctx().s()));
+    FILE* f = std::fopen(path, "w");
+    SQ_REQUIRE(f, SQBEM_ERR_ARG, std::string("cannot open tune table for writing: ") + path);
+    std::fprintf(f, "# sqbem gemm16q tile table: key(kind<<24|lg m<<16|lg n<<8|lg k) best ms0 ms1 ms2 ms3\n");
+    for (auto& kv : g_table) {
+        poll(kv.second);
+        if (kv.second.best < 0) continue;
+        const Cls& c = kv.second;
+        std::fprintf(f, "%u %d %.6g %.6g %.6g %.6g\n", kv.first, c.best, c.ms[0], c.ms[1], c.ms[2], c.ms[3]);
+    }
+    std::fclose(f);
+}
+
+// adds the classes of a saved table as decided (online learning continues for the others)
+static int load(const char* path) {
+    FILE* f = std::fopen(path, "r");
+    SQ_REQUIRE(f, SQBEM_ERR_ARG, std::string("cannot open tune table: ") + path);
+    char line[256];
+    int n = 0;
+    while (std::fgets(line, sizeof line, f)) {
+        if (line[0] == '#') continue;
+        unsigned key;
+        int best;
+        float ms[NC];
+        if (std::sscanf(line, "%u %d %g %g %g %g", &key, &best, &ms[0], &ms[1], &ms[2], &ms[3]) != 6 || best < 0 ||
+            best >= NC)
+            continue;
+        Cls& c = g_table[key];
+        for (int i = 0; i < NC; ++i) {
+            if (c.e0[i]) (void)hipEventDestroy(c.e0[i]);
+            if (c.e1[i]) (void)hipEventDestroy(c.e1[i]);
+            c.e0[i] = c.e1[i] = nullptr;
+            c.ms[i] = ms[i];
+            c.done[i] = true;
+        }
+        c.tried = NC;
+        c.best = best;
+        ++n;
+    }
+    std::fclose(f);
+    return n;
 }
 
 static void info(int* classes, int* decided, int* counts) {
@@ -2552,6 +2598,21 @@ int sqbem_tune(int mode) {
 
 int sqbem_tune_info(int* classes, int* decided, int* counts) {
     return guarded([&] { tune::info(classes, decided, counts); });
+}
+
+int sqbem_tune_save(const char* path) {
+    return guarded([&] {
+        SQ_REQUIRE(path, SQBEM_ERR_ARG, "sqbem_tune_save: null path");
+        tune::save(path);
+    });
+}
+
+int sqbem_tune_load(const char* path, int* classes) {
+    return guarded([&] {
+        SQ_REQUIRE(path, SQBEM_ERR_ARG, "sqbem_tune_load: null path");
+        const int n = tune::load(path);
+        if (classes) *classes = n;
+    });
 }
 
 int sqbem_sgetrs(int n, int nrhs, const float* LU, int lda, const int* ipiv, float* B, int ldb) {

@@ -1,6 +1,8 @@
 // runtime.hip -- device selection, library stream, error reporting, small utility kernels.
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "common.hpp"
@@ -75,6 +77,76 @@ void check_device_error() {
         throw Error(SQBEM_ERR_HIP, "triangular solve: a dependency wait timed out");
     }
 }
+
+// ------------------------------------------------------------------ device memory cache
+namespace {
+struct MemCache {
+    std::multimap<size_t, void*> free_blocks;  // rounded size -> block
+    std::unordered_map<void*, size_t> live;    // block -> rounded size
+    size_t cached = 0;
+};
+// never destroyed: buffers with static storage release into it during process exit
+MemCache& mem() {
+    static MemCache* m = new MemCache;
+    return *m;
+}
+size_t round_bytes(size_t b) {
+    constexpr size_t big = size_t(2) << 20;
+    if (b >= big) return (b + big - 1) / big * big;
+    size_t r = 256;
+    while (r < b) r <<= 1;
+    return r;
+}
+}  // namespace
+
+void* dev_alloc(size_t bytes) {
+    MemCache& M = mem();
+    const size_t r = round_bytes(bytes);
+    auto it = M.free_blocks.find(r);
+    if (it != M.free_blocks.end()) {
+        void* p = it->second;
+        M.free_blocks.erase(it);
+        M.cached -= r;
+        M.live[p] = r;
+        return p;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, r) != hipSuccess) {
+        (void)hipGetLastError();
+        dev_cache_release();
+        p = nullptr;
+        if (hipMalloc(&p, r) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    M.live[p] = r;
+    return p;
+}
+
+void dev_free(void* p) {
+    if (!p) return;
+    MemCache& M = mem();
+    auto it = M.live.find(p);
+    if (it == M.live.end()) {
+        (void)hipFree(p);
+        return;
+    }
+    M.free_blocks.emplace(it->second, p);
+    M.cached += it->second;
+    M.live.erase(it);
+}
+
+void dev_cache_release() {
+    MemCache& M = mem();
+    if (M.free_blocks.empty()) return;
+    (void)hipDeviceSynchronize();
+    for (auto& kv : M.free_blocks) (void)hipFree(kv.second);
+    M.free_blocks.clear();
+    M.cached = 0;
+}
+
+size_t dev_cache_bytes() { return mem().cached; }
 
 void fill_zero(void* p, size_t bytes) {
     if (!p || bytes == 0) return;
@@ -193,7 +265,20 @@ int sqbem_init(int device) {
 }
 
 int sqbem_set_stream(void* s) {
-    return guarded([&] { g_ctx.user_stream = (hipStream_t)s; });
+    return guarded([&] {
+        // cached device blocks are reused without synchronisation on the assumption that all
+        // work runs on one stream: finish the old stream's work before switching
+        SQ_HIP(hipStreamSynchronize(g_ctx.s()));
+        g_ctx.user_stream = (hipStream_t)s;
+    });
+}
+
+int sqbem_trim(double* released_gb) {
+    return guarded([&] {
+        const size_t b = dev_cache_bytes();
+        dev_cache_release();
+        if (released_gb) *released_gb = b / 1e9;
+    });
 }
 
 int sqbem_sync(void) {

@@ -35,7 +35,10 @@ _SIGS = {
     "sqbem_profile": (_i, [_i]),
     "sqbem_profile_get": (_i, [_i, _vp, _vp, _vp, _vp]),
     "sqbem_tune": (_i, [_i]),
+    "sqbem_trim": (_i, [_vp]),
     "sqbem_tune_info": (_i, [_vp, _vp, _vp]),
+    "sqbem_tune_save": (_i, [ctypes.c_char_p]),
+    "sqbem_tune_load": (_i, [ctypes.c_char_p, _vp]),
     "sqbem_mesh_swimmer_tube": (_vp, [_d, _d, _d, _i, _i]),
     "sqbem_mesh_tube": (_vp, [_d, _vp, _i, _i, _i]),
     "sqbem_mesh_duct": (_vp, [_d, _d, _i, _i, _i]),
@@ -175,6 +178,13 @@ def profile_get(name):
 
 
 # ------------------------------------------------------------------ meshes
+def trim() -> float:
+    """Return the device memory the library keeps for reuse to the driver; GB released."""
+    gb = ctypes.c_double()
+    check(lib().sqbem_trim(ctypes.byref(gb)))
+    return gb.value
+
+
 def tune(mode: int = 1):
     """Tile autotuner of the 16-bit GEMMs: 1 learns per shape class from live calls, 0 uses
     the fixed rule; either clears what was learned."""
@@ -182,6 +192,18 @@ def tune(mode: int = 1):
 
 
 TUNE_TILES = ["128x128", "128x256", "256x128", "256x256"]
+TUNE_TABLE = os.path.join(HERE, "tune_gfx950.txt")  # learned on an MI355X by bench.py --tune-save
+
+
+def tune_save(path: str):
+    check(lib().sqbem_tune_save(path.encode()))
+
+
+def tune_load(path: str = TUNE_TABLE) -> int:
+    """Adds the decided classes of a saved tile table; returns how many were read."""
+    n = ctypes.c_int()
+    check(lib().sqbem_tune_load(path.encode(), ctypes.byref(n)))
+    return n.value
 
 
 def tune_info() -> dict:

@@ -120,6 +120,38 @@ def test_gemm16_autotuner_every_tile(sq, kind, tol):
         sq.tune(1)
 
 
+def test_gemm16_tune_table_save_and_load(sq, tmp_path):
+    """A decided class survives save -> reset -> load and is used from the first call."""
+    m, n, k = 640, 600, 320
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((m, k)).astype(np.float32)
+    B = rng.standard_normal((k, n)).astype(np.float32)
+    ref = A.astype(np.float64) @ B.astype(np.float64)
+    sq.tune(1)
+    try:
+        for _ in range(5):
+            dA, dB, dC = dev(A), dev(B), dev(np.zeros((m, n), dtype=np.float32))
+            torch.cuda.synchronize()
+            sq.gemm16(3, 1.0, dA, dB, 0.0, dC)
+            sq.sync()
+        before = sq.tune_info()
+        path = str(tmp_path / "tune.txt")
+        sq.tune_save(path)
+        sq.tune(1)
+        assert sq.tune_info()["classes"] == 0
+        assert sq.tune_load(path) == before["decided"] >= 1
+        after = sq.tune_info()
+        assert after["decided"] == before["decided"] and after["tiles"] == before["tiles"]
+        dA, dB, dC = dev(A), dev(B), dev(np.zeros((m, n), dtype=np.float32))
+        torch.cuda.synchronize()
+        sq.gemm16(3, 1.0, dA, dB, 0.0, dC)
+        sq.sync()
+        assert np.abs(host(dC) - ref).max() < 2e-2 * np.abs(ref).max()
+        assert sq.tune_info()["classes"] == after["classes"]
+    finally:
+        sq.tune(1)
+
+
 def test_gemm16_fixed_rule_mode(sq):
     m, n, k = 512, 512, 256
     rng = np.random.default_rng(3)

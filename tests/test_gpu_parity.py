@@ -150,3 +150,21 @@ def test_duct_flow_parity(sq, O):
     assert abs(sg.Q - Qo) < 1e-10 * abs(Qo)
     assert rel(sg.f, fo) < 1e-10
     assert rel(sg.u, uo) < 1e-10
+
+
+def test_device_memory_cache_reuse_and_trim(sq):
+    """A second identical solve reuses the cached device blocks and gives the same answer
+    bit for bit; sqbem_trim hands the cache back to the driver."""
+    def solve():
+        t = sq.Tube(1.0, 4.0, 0.3, 12, 3)
+        s = sq.SwimmerSolver(t, sq.icosphere(0.3, 1, (0.1, 0.0, 0.0)), (0.1, 0.0, 0.0))
+        r = s.solve_free([s.slip(E_Z, 1.0, 0.5)])[0]
+        del s, t
+        return np.r_[r.U, r.Om]
+
+    a = solve()
+    b = solve()
+    assert np.array_equal(a, b)
+    assert sq.trim() > 0.0
+    assert sq.trim() == 0.0
+    assert np.array_equal(solve(), a)
