@@ -30,7 +30,14 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAKS = {"dgemm": 78.6e12, "sgemm": 157.3e12, "gemm16": 2.5e15, "gemm16q": 2.5e15}
+PEAKS = {"dgemm": 78.6e12, "sgemm": 157.3e12, "gemm16": 2.5e15, "gemm16q": 2.5e15, "gemm16q_128x128": 2.5e15,
+         "gemm16q_128x256": 2.5e15, "gemm16q_256x128": 2.5e15, "gemm16q_256x256": 2.5e15}
+# the kernel each timing class measures (as rocprofv3 names it)
+KERNEL_NAMES = {"dgemm": "gemm_nn_kernel<double>", "sgemm": "gemm_nn_kernel<float>", "gemm16": "gemm16_nn_kernel",
+                "gemm16q": "gemm16q_kernel (all tiles)", "gemm16q_128x128": "gemm16q_kernel<{t}, 128, 128>",
+                "gemm16q_128x256": "gemm16q_kernel<{t}, 128, 256>", "gemm16q_256x128": "gemm16q_kernel<{t}, 256, 128>",
+                "gemm16q_256x256": "gemm16q_kernel<{t}, 256, 256>"}
+TIMED = ["dgemm", "sgemm", "gemm16", "gemm16q", "gemm16q_128x128", "gemm16q_128x256", "gemm16q_256x128", "gemm16q_256x256"]
 MODES = {"fp64": 0, "fp32": 1, "fp16": 2, "bf16": 3}
 
 
@@ -206,6 +213,21 @@ def field_bench(pkg):
     return out
 
 
+def hbm_traffic(kernel):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter summary of this
+    workload (FETCH_SIZE and WRITE_SIZE per dispatch, calibrated on a plain copy of known
+    size), or None when the summary has no entry for it."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_46k_bf16.json")
+    try:
+        tab = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    e = tab.get("kernels", {}).get(kernel)
+    if not e:
+        return None
+    return {"bytes_per_launch": e["hbm_bytes_per_launch"], "source": "profiles/hbm_traffic_46k_bf16.json"}
+
+
 def batch_bench(pkg, n=256, seed=0, modes=(("fp32+ir", 1), ("fp16+ir", 2)), check=4):
     """Batched uneven solves: n configurations drawn from seed (a/R ~ U[0.1, 0.7], b/R ~
     U[0, 0.8 (1 - a/R)], B2/B1 ~ U[-5, 5], random e, sphere level {2, 3}, tube n_theta
@@ -318,6 +340,8 @@ def main():
     ap.add_argument("--no-grad", action="store_true", help="skip the adjoint-gradient timing")
     ap.add_argument("--no-field", action="store_true", help="skip the 57k-DOF solve + field grid")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-configuration batched solve")
+    ap.add_argument("--batch-only", default="", metavar="MODES",
+                    help="run only the 256-configuration batch in these modes (e.g. fp16,fp32) and print its line")
     ap.add_argument("--tune-table", default="shipped",
                     help="GEMM tile table to start from: 'shipped' (the package's tune_gfx950.txt), a path, "
                          "or 'none' (learn every class online in the warm-up)")
@@ -345,15 +369,24 @@ def main():
         path = pkg.TUNE_TABLE if args.tune_table == "shipped" else args.tune_table
         if os.path.exists(path):
             tune_loaded = pkg.tune_load(path)
+    if args.batch_only:
+        names = {"fp64": ("fp64", 0), "fp32": ("fp32+ir", 1), "fp16": ("fp16+ir", 2), "bf16": ("bf16+ir", 3)}
+        modes = tuple(names[m] for m in args.batch_only.split(","))
+        b = batch_bench(pkg, modes=modes)
+        first = b[modes[0][0]]
+        print(json.dumps({"metric": "configurations/s, 256 batched uneven squirmer-in-tube solves", "value":
+                          first["instances_per_s"], "unit": "configurations/s", "n_gpus": 1, "higher_is_better": True,
+                          "dtype": modes[0][0], "data": "synthetic (seed 0)", "config": {"workload": b["workload"]},
+                          "detail": b}), flush=True)
+        return
     W = workload(args)
     e = np.array([0.0, 0.0, 1.0])
     prec = MODES[args.mode]
-    kname = {0: "dgemm", 1: "sgemm", 2: "gemm16q", 3: "gemm16q"}[prec]
 
     for _ in range(args.warmup):
         cold_solve(pkg, W, prec, e)
     pkg.sync()
-    pkg.profile(True, classes=["dgemm", "sgemm", "gemm16", "gemm16q"])
+    pkg.profile(True, classes=TIMED)
     t0 = time.perf_counter()
     runs = []
     for _ in range(args.steps):
@@ -381,12 +414,21 @@ def main():
                              "tflops": (v["flops"] / (v["ms"] * 1e-3) / 1e12) if v["ms"] > 0 else 0.0}
                          for k, v in prof.items() if v["launches"]}
 
+    # dominant kernel: the single kernel (one template instance, as rocprofv3 lists it) with
+    # the most time in the timed steps
+    kname = max((k for k in TIMED if k != "gemm16q"), key=lambda k: prof[k]["ms"])
     kp = prof[kname]
     achieved = kp["flops"] / (kp["ms"] * 1e-3) if kp["ms"] > 0 else 0.0
-    roofline = {"bound": "mfma", "kernel": kname, "achieved": achieved / 1e12, "peak": PEAKS[kname] / 1e12,
-                "unit": "TFLOP/s", "frac": achieved / PEAKS[kname], "traffic": None,
-                "launches": kp["launches"], "avg_launch_ms": kp["ms"] / max(kp["launches"], 1),
-                "flops_per_launch": kp["flops"] / max(kp["launches"], 1)}
+    nl = max(kp["launches"], 1)
+    tname = "__bf16" if prec == 3 else "_Float16"
+    roofline = {"bound": "mfma", "kernel": KERNEL_NAMES[kname].format(t=tname), "achieved": achieved / 1e12,
+                "peak": PEAKS[kname] / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAKS[kname], "traffic": None,
+                "launches": kp["launches"], "avg_launch_ms": kp["ms"] / nl, "flops_per_launch": kp["flops"] / nl,
+                "algorithmic_bytes_per_launch": kp["bytes"] / nl}
+    traffic = hbm_traffic(roofline["kernel"])
+    if traffic:
+        roofline["traffic"] = traffic["bytes_per_launch"]
+        roofline["traffic_source"] = traffic["source"]
 
     if args.calib_copy_gib > 0:
         # a plain copy of known size: what FETCH_SIZE / WRITE_SIZE read for it calibrates

@@ -58,6 +58,11 @@ int sqbem_trim(double* released_gb);
 #define SQBEM_PROF_LASWP 7
 #define SQBEM_PROF_GEMM16Q 8 /* 16-bit GEMM on pre-converted, padded operands (large shapes) */
 #define SQBEM_PROF_TRSV 9    /* triangular solves with at most 8 right-hand sides (one launch each) */
+/* gemm16q launches by workgroup tile (each also counted in SQBEM_PROF_GEMM16Q) */
+#define SQBEM_PROF_G16Q_128x128 10
+#define SQBEM_PROF_G16Q_128x256 11
+#define SQBEM_PROF_G16Q_256x128 12
+#define SQBEM_PROF_G16Q_256x256 13
 int sqbem_profile(int mask);
 int sqbem_profile_get(int kernel_class, long* launches, double* flops, double* bytes, double* ms);
 

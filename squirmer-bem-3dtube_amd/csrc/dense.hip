@@ -727,8 +727,11 @@ static void gemm16q(int m, int n, int k, float alpha, const float* A, int lda, c
                        dim3(256), 0, ctx().s(), k, n, B, ldb, cs, Bh, kp, np);
     check_launch("convert16");
     const int tiles_m = mp / TM, tiles_n = np / TN;
-    ProfScope prof(PROF_GEMM16Q, 2.0 * m * n * k,
-                   2.0 * ((double)m * k + (double)k * n) + 4.0 * (beta != 0.0f ? 2.0 : 1.0) * m * n);
+    const double fl = 2.0 * m * n * k, by = 2.0 * ((double)m * k + (double)k * n) + 4.0 * (beta != 0.0f ? 2.0 : 1.0) * m * n;
+    ProfScope prof(PROF_GEMM16Q, fl, by);
+    ProfScope prof_tile(TM == 128 ? (TN == 128 ? PROF_G16Q_128x128 : PROF_G16Q_128x256)
+                                  : (TN == 128 ? PROF_G16Q_256x128 : PROF_G16Q_256x256),
+                        fl, by);
     hipLaunchKernelGGL((gemm16q_kernel<BF16, TM, TN>), dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, kp,
                        alpha, Ah, Bh, beta, C, ldc, tiles_m, rs, cs);
     check_launch("gemm16q");

@@ -153,7 +153,9 @@ def device_info():
     return name.value.decode(), cu.value, gb.value
 
 
-PROF_CLASSES = {"dgemm": 0, "sgemm": 1, "gemm16": 2, "panel": 3, "trsm": 4, "assemble": 5, "gemv": 6, "laswp": 7, "gemm16q": 8, "trsv": 9}
+PROF_CLASSES = {"dgemm": 0, "sgemm": 1, "gemm16": 2, "panel": 3, "trsm": 4, "assemble": 5, "gemv": 6, "laswp": 7,
+                "gemm16q": 8, "trsv": 9, "gemm16q_128x128": 10, "gemm16q_128x256": 11, "gemm16q_256x128": 12,
+                "gemm16q_256x256": 13}
 
 
 def profile(enable=True, classes=None):
