@@ -76,7 +76,13 @@ struct MfmaOp<float> {
 };
 
 constexpr int GBM = 128, GBN = 128;
-constexpr int GBK = 16, GPAD = GBK + 1;
+// k tile of the FP32/FP64 GEMM: 32 (FP32) or 16 (FP64) values, so that a short product
+// (k = 128: the leaves of the factorisation and of the triangular solves) waits on few
+// global-load round trips; both keep the two LDS stages within 68 KiB
+template <typename T>
+struct GemmK {
+    static constexpr int KB = sizeof(T) == 4 ? 32 : 16, PAD = KB + 1;
+};
 
 // MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16] and B[l/16][l%16]; the four
 // accumulator values of lane l are D[Op::row(l/16, r)][l%16].
@@ -86,38 +92,39 @@ __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const
                                              int tm, int tn) {
     using Op = MfmaOp<T>;
     using Acc = typename Op::acc;
-    __shared__ T As[2][GBM][GPAD];
-    __shared__ T Bs[2][GBN][GPAD];
+    constexpr int KB = GemmK<T>::KB, PAD = GemmK<T>::PAD, NQ = KB / 2, JS = 256 / KB;
+    __shared__ T As[2][GBM][PAD];
+    __shared__ T Bs[2][GBN][PAD];
 
     const int m0 = tm * GBM, n0 = tn * GBN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
 
-    // global -> register staging: A column kk has 128 contiguous rows; B column j has 16
+    // global -> register staging: A column kk has 128 contiguous rows; B column j has KB
     // contiguous k values.
-    const int a_i = tid & 127, a_k0 = tid >> 7;  // rows a_i, k = a_k0 + 2q
-    const int b_k = tid & 15, b_j0 = tid >> 4;   // k = b_k, cols b_j0 + 16q
-    T ra[8], rb[8];
+    const int a_i = tid & 127, a_k0 = tid >> 7;     // rows a_i, k = a_k0 + 2q
+    const int b_k = tid % KB, b_j0 = tid / KB;      // k = b_k, cols b_j0 + JS q
+    T ra[NQ], rb[NQ];
 
     auto load = [&](int k0) {
         const int gi = m0 + a_i;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < NQ; ++q) {
             int gk = k0 + a_k0 + 2 * q;
             ra[q] = (gi < m && gk < k) ? A[gi + (size_t)gk * lda] : T(0);
         }
         const int gk = k0 + b_k;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            int gj = n0 + b_j0 + 16 * q;
+        for (int q = 0; q < NQ; ++q) {
+            int gj = n0 + b_j0 + JS * q;
             rb[q] = (gk < k && gj < n) ? B[gk + (size_t)gj * ldb] : T(0);
         }
     };
     auto store = [&](int buf) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) As[buf][a_i][a_k0 + 2 * q] = ra[q];
+        for (int q = 0; q < NQ; ++q) As[buf][a_i][a_k0 + 2 * q] = ra[q];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) Bs[buf][b_j0 + 16 * q][b_k] = rb[q];
+        for (int q = 0; q < NQ; ++q) Bs[buf][b_j0 + JS * q][b_k] = rb[q];
     };
 
     Acc acc[4][4];
@@ -126,16 +133,16 @@ __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = Acc{0, 0, 0, 0};
 
-    const int nk = (k + GBK - 1) / GBK;
+    const int nk = (k + KB - 1) / KB;
     load(0);
     store(0);
     __syncthreads();
     const int lr = lane & 15, lk = lane >> 4;
     for (int kt = 0; kt < nk; ++kt) {
         const int buf = kt & 1;
-        if (kt + 1 < nk) load((kt + 1) * GBK);
+        if (kt + 1 < nk) load((kt + 1) * KB);
 #pragma unroll
-        for (int s = 0; s < GBK / 4; ++s) {
+        for (int s = 0; s < KB / 4; ++s) {
             T af[4], bf[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) af[i] = As[buf][wm + i * 16 + lr][4 * s + lk];

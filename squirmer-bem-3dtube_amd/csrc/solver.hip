@@ -83,11 +83,11 @@ __global__ void kproj_reduce(int nrows, const double* __restrict__ part, double*
 // max |A(i,j)| over an m x n block (non-negative doubles order like their bit patterns)
 template <typename T>
 __global__ void absmax_kernel_t(int m, int n, const T* __restrict__ A, long lda, unsigned long long* out) {
+    // workgroups stride over columns, threads over the rows of a column (no divisions)
     double mx = 0.0;
-    size_t total = (size_t)m * n;
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        size_t i = e % m, j = e / m;
-        mx = fmax(mx, fabs((double)A[i + j * lda]));
+    for (int j = blockIdx.x; j < n; j += gridDim.x) {
+        const T* col = A + (size_t)j * lda;
+        for (int i = threadIdx.x; i < m; i += blockDim.x) mx = fmax(mx, fabs((double)col[i]));
     }
     for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
     if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(mx));
@@ -214,8 +214,7 @@ template <typename T>
 static double absmax(int m, int n, const T* A, long lda) {
     DBuf<unsigned long long> r(1);
     r.zero();
-    size_t total = (size_t)m * n;
-    unsigned blocks = (unsigned)std::min<size_t>(std::max<size_t>(ceil_div(total, 256), 1), 4096);
+    unsigned blocks = (unsigned)std::max(1, std::min(n, 4096));
     hipLaunchKernelGGL(absmax_kernel_t<T>, dim3(blocks), dim3(256), 0, ctx().s(), m, n, A, lda, r.p);
     check_launch("absmax");
     unsigned long long bits = 0;
@@ -266,9 +265,8 @@ void kproj_async(int nrows, int Ns, const double* A, long lda, const double* cen
 
 template <typename T>
 void absmax_async(int m, int n, const T* A, long lda, unsigned long long* out) {
-    const size_t total = (size_t)m * n;
-    if (!total) return;
-    unsigned blocks = (unsigned)std::min<size_t>(ceil_div(total, 256), 4096);
+    if (m <= 0 || n <= 0) return;
+    unsigned blocks = (unsigned)std::min(n, 4096);
     hipLaunchKernelGGL(absmax_kernel_t<T>, dim3(blocks), dim3(256), 0, ctx().s(), m, n, A, lda, out);
     check_launch("absmax");
 }
