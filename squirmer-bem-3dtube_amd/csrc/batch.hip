@@ -209,39 +209,58 @@ void add_into(size_t n, const double* y, double* x) {
     check_launch("batch_axpy");
 }
 
-// job tables of the batched small kernels: staged through one device buffer per call site
-struct JobTable {
+// Job table of a batched small kernel, uploaded once (the launches that use it then copy
+// nothing, so they can be replayed from a captured graph while the table lives).
+template <typename J>
+struct Table {
     DBuf<unsigned char> d;
-    template <typename J>
-    const J* put(const std::vector<J>& v) {
-        const size_t bytes = v.size() * sizeof(J);
-        if (d.n < bytes) d.alloc(std::max<size_t>(bytes, 1 << 16));
-        SQ_HIP(hipMemcpyAsync(d.p, v.data(), bytes, hipMemcpyHostToDevice, ctx().s()));
-        return (const J*)d.p;
+    int n = 0;
+    void put(const std::vector<J>& v) {
+        n = (int)v.size();
+        if (!n) return;
+        d.alloc(v.size() * sizeof(J));
+        SQ_HIP(hipMemcpyAsync(d.p, v.data(), v.size() * sizeof(J), hipMemcpyHostToDevice, ctx().s()));
+        sync();  // v is staging
     }
+    const J* get() const { return (const J*)d.p; }
 };
 
-template <typename T>
-void gemv_batch(JobTable& tab, std::vector<GvJob> jobs, double* part) {
-    if (jobs.empty()) return;
-    long off = 0;
+// y_j += alpha_j (A1_j x1_j + A2_j x2_j) for a prepared set of jobs
+struct GvPlan {
+    Table<GvJob> tab;
     int mmax = 1;
-    double bytes = 0, flops = 0;
+    double flops = 0, bytes = 0;
+};
+template <typename T>
+void gemv_prepare(GvPlan& p, std::vector<GvJob> jobs) {
+    long off = 0;
+    p.mmax = 1;
+    p.flops = p.bytes = 0;
     for (GvJob& g : jobs) {
         g.poff = off;
         off += (long)GV_CH * g.m;
-        mmax = std::max(mmax, g.m);
-        bytes += sizeof(T) * (double)g.m * (g.n1 + g.n2);
-        flops += 2.0 * g.m * (double)(g.n1 + g.n2);
+        p.mmax = std::max(p.mmax, g.m);
+        p.bytes += sizeof(T) * (double)g.m * (g.n1 + g.n2);
+        p.flops += 2.0 * g.m * (double)(g.n1 + g.n2);
     }
-    const GvJob* d = tab.put(jobs);
-    ProfScope prof(PROF_GEMV, flops, bytes);
-    hipLaunchKernelGGL(gemv_batch_part<T>, dim3(ceil_div(mmax, 256), GV_CH, (unsigned)jobs.size()), dim3(256), 0,
-                       ctx().s(), d, part);
-    hipLaunchKernelGGL(gemv_batch_reduce<T>, dim3(ceil_div(mmax, 256), (unsigned)jobs.size()), dim3(256), 0, ctx().s(),
-                       d, part);
+    p.tab.put(jobs);
+}
+template <typename T>
+void gemv_launch(const GvPlan& p, double* part) {
+    if (!p.tab.n) return;
+    ProfScope prof(PROF_GEMV, p.flops, p.bytes);
+    hipLaunchKernelGGL(gemv_batch_part<T>, dim3(ceil_div(p.mmax, 256), GV_CH, (unsigned)p.tab.n), dim3(256), 0,
+                       ctx().s(), p.tab.get(), part);
+    hipLaunchKernelGGL(gemv_batch_reduce<T>, dim3(ceil_div(p.mmax, 256), (unsigned)p.tab.n), dim3(256), 0, ctx().s(),
+                       p.tab.get(), part);
     check_launch("gemv_batch");
-    sync();  // the host job vector is staging
+}
+template <typename T>
+void gemv_batch(std::vector<GvJob> jobs, double* part) {
+    GvPlan p;
+    gemv_prepare<T>(p, std::move(jobs));
+    gemv_launch<T>(p, part);
+    sync();
 }
 
 size_t gemv_part_elems(int nt, const std::vector<int>& ns, const std::vector<int>& m) {
@@ -252,9 +271,11 @@ size_t gemv_part_elems(int nt, const std::vector<int>& ns, const std::vector<int
 
 struct Stats {
     double t_tubes = 0, t_assembly = 0, t_schur = 0, t_solve = 0;
-    int unique_tubes = 0, chunks = 0, max_iters = 0, pivoted = 0;
+    int unique_tubes = 0, chunks = 0, max_iters = 0, pivoted = 0, graph_replays = 0;
     double worst = 0;
 };
+
+bool g_use_graph = true;  // capture the refinement step once per chunk and replay it
 
 // A tube shared by instances: its factors, and for the refined modes its FP32 inverse.
 struct SharedTube {
@@ -288,7 +309,7 @@ struct Work {
     std::vector<size_t> oD, oG, oA, oP2, oP3, oM, oX;
     size_t C = 0, S = 0;
     double *BC, *AuBt, *D, *GE, *AuBs, *part, *dls, *kp, *u, *Bt, *Bsc, *gvp, *nrm;
-    int *self, *flags, *dcoff, *dmlen, *piv;
+    int *self, *flags, *dcoff, *dmlen, *piv, *perm;
     unsigned long long* amax;
     double *M64 = nullptr, *X64 = nullptr, *Z64 = nullptr, *Xd = nullptr;
     float *Y32 = nullptr, *B32 = nullptr, *D32 = nullptr, *G32 = nullptr, *M32 = nullptr, *X32 = nullptr;
@@ -347,6 +368,7 @@ void layout(Arena& ar, int Nt, const std::vector<Inst*>& I, bool mx, Work& w) {
     w.dcoff = ar.take<int>(k);
     w.dmlen = ar.take<int>(k);
     w.piv = ar.take<int>(C);
+    w.perm = ar.take<int>(C);
     w.amax = ar.take<unsigned long long>(k);
     if (!mx) {
         w.M64 = ar.take<double>(nM);
@@ -395,7 +417,7 @@ void run_chunk(SharedTube& ST, std::vector<Inst*>& I, int prec, double mu, Arena
     const int maxNs = w.maxNs;
     double *BC = w.BC, *AuBt = w.AuBt, *D = w.D, *GE = w.GE, *AuBs = w.AuBs, *part = w.part, *dls = w.dls, *kp = w.kp,
            *u = w.u, *Bt = w.Bt, *Bsc = w.Bsc, *gvp = w.gvp, *nrm = w.nrm;
-    int *self = w.self, *flags = w.flags, *dcoff = w.dcoff, *dmlen = w.dmlen, *piv = w.piv;
+    int *self = w.self, *flags = w.flags, *dcoff = w.dcoff, *dmlen = w.dmlen, *piv = w.piv, *perm = w.perm;
     unsigned long long* amax = w.amax;
     double *M64 = w.M64, *X64 = w.X64, *Z64 = w.Z64, *Xd = w.Xd;
     float *Y32 = w.Y32, *B32 = w.B32, *D32 = w.D32, *G32 = w.G32, *M32 = w.M32, *X32 = w.X32;
@@ -438,7 +460,6 @@ void run_chunk(SharedTube& ST, std::vector<Inst*>& I, int prec, double mu, Arena
     }
     // right-hand sides of the full bordered system for each instance's slip:
     // b_t = -AuBt u, b_s = -AuBs u, b_c = 0 (src/swimmer.hpp:256-269)
-    JobTable jt;
     {
         std::vector<double> hu(S);
         for (int i = 0; i < k; ++i) {
@@ -461,7 +482,7 @@ void run_chunk(SharedTube& ST, std::vector<Inst*>& I, int prec, double mu, Arena
                          Bt + (size_t)i * nt, -1.0, 0});
             g.push_back({q.ns, q.ns, 0, AuBs + oA[i], q.ns, u + soff[i], nullptr, 0, nullptr, Bsc + coff[i], -1.0, 0});
         }
-        gemv_batch<double>(jt, g, gvp);  // synchronises (the slip upload is complete)
+        gemv_batch<double>(g, gvp);  // synchronises (the slip upload is complete)
     }
     double t1 = now_s();
     st.t_assembly += t1 - t0;
@@ -534,22 +555,28 @@ void run_chunk(SharedTube& ST, std::vector<Inst*>& I, int prec, double mu, Arena
         const int info = mx ? getrf<float>(prec, I[i]->m, M32 + oM[i], I[i]->m, piv + coff[i], X32 + oX[i])
                             : getrf<double>(SQBEM_FP64, I[i]->m, M64 + oM[i], I[i]->m, piv + coff[i], X64 + oX[i]);
         SQ_REQUIRE(info == 0, SQBEM_ERR_SINGULAR, "dgesv failed");
+        // LAPACK's sequential interchanges as one gather: (P v)[r] = v[perm[r]]
+        const int m = I[i]->m;
+        std::vector<int> ip(m), pr(m);
+        SQ_HIP(hipMemcpyAsync(ip.data(), piv + coff[i], sizeof(int) * m, hipMemcpyDeviceToHost, ctx().s()));
+        sync();
+        for (int r = 0; r < m; ++r) pr[r] = r;
+        for (int r = 0; r < m; ++r) std::swap(pr[r], pr[ip[r] - 1]);
+        SQ_HIP(hipMemcpyAsync(perm + coff[i], pr.data(), sizeof(int) * m, hipMemcpyHostToDevice, ctx().s()));
+        sync();
     }
     sync();
     double t2 = now_s();
     st.t_schur += t2 - t1;
 
-    // bordered solves of all instances: B_i <- M_i^-1 B_i
-    auto bordered_solve = [&](auto* Mbase, auto* Xbase, auto* V) {
+    // bordered solves of all instances, B_i <- M_i^-1 B_i: one batched chained solve (the
+    // pivoted ones gather their rows through the permutation first)
+    auto bordered_plan = [&](auto* Xbase, auto* V, auto& plan) {
         using T = std::remove_pointer_t<decltype(V)>;
         std::vector<BatchSolve<T>> bs;
         for (int i = 0; i < k; ++i)
-            if (!pivoted[i]) bs.push_back({I[i]->m, Xbase + oX[i], V + coff[i]});
-        getrs_np_batch<T>(bs);
-        for (int i = 0; i < k; ++i)
-            if (pivoted[i])
-                getrs<T>(mx ? prec : SQBEM_FP64, 0, I[i]->m, 1, Mbase + oM[i], I[i]->m, piv + coff[i], V + coff[i],
-                         I[i]->m, Xbase + oX[i]);
+            bs.push_back({I[i]->m, Xbase + oX[i], V + coff[i], pivoted[i] ? perm + coff[i] : nullptr});
+        getrs_np_batch_prepare<T>(bs, plan);
     };
 
     // ---- solves
@@ -563,8 +590,10 @@ void run_chunk(SharedTube& ST, std::vector<Inst*>& I, int prec, double mu, Arena
         for (int i = 0; i < k; ++i)
             g.push_back({I[i]->ns, nt, 0, D + oD[i], I[i]->ns, Z64 + (size_t)i * nt, nullptr, 0, nullptr, Xd + coff[i],
                          -1.0, 0});
-        gemv_batch<double>(jt, g, gvp);
-        bordered_solve(M64, X64, Xd);
+        gemv_batch<double>(g, gvp);
+        BatchSolvePlan<double> bp;
+        bordered_plan(X64, Xd, bp);
+        getrs_np_batch_launch<double>(bp);
         SQ_HIP(hipMemcpyAsync(X.data(), Xd, C * sizeof(double), hipMemcpyDeviceToHost, ctx().s()));
         sync();
         for (int i = 0; i < k; ++i) {
@@ -579,6 +608,7 @@ void run_chunk(SharedTube& ST, std::vector<Inst*>& I, int prec, double mu, Arena
             std::vector<int> hc(coff.begin(), coff.begin() + k);
             SQ_HIP(hipMemcpyAsync(dcoff, hc.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx().s()));
             SQ_HIP(hipMemcpyAsync(dmlen, mv.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx().s()));
+            sync();  // hc is staging
         }
         fill_zero(Xt, sizeof(double) * (size_t)nt * k);
         fill_zero(Xsc, sizeof(double) * C);
@@ -607,18 +637,23 @@ void run_chunk(SharedTube& ST, std::vector<Inst*>& I, int prec, double mu, Arena
             cj.push_back({q.Ns, xs, dm[i].cen_xyz.p, dm[i].area.p, sc[i], {q.xc[0], q.xc[1], q.xc[2]},
                           Rsc + coff[i] + q.ns});
         }
-        JobTable jc;
-        const double tol = 1e-12;
-        int it = 0;
-        double worst = 1.0;
-        for (; it < 40; ++it) {
+        GvPlan ps, pt, pr;
+        gemv_prepare<float>(ps, gv_s);
+        gemv_prepare<float>(pt, gv_t);
+        gemv_prepare<double>(pr, gv_r);
+        Table<CJob> ct;
+        ct.put(cj);
+        BatchSolvePlan<float> bp;
+        bordered_plan(X32, V32, bp);
+        // one refinement step: launches only, all tables prepared above
+        auto step = [&]() {
             // corrections: z = T^-1 r_t; v_i = M_i^-1 [r_s - D_i z_i; r_c]; z_i -= Y_i v_i
             convert_matrix<double, float>(nt, k, Rt, nt, Z32, nt);
             sgemm(nt, k, nt, 1.0f, ST.inv.p, nt, Z32, nt, 0.0f, Zo32, nt);
             convert_matrix<double, float>((int)C, 1, Rsc, (int)C, V32, (int)C);
-            gemv_batch<float>(jt, gv_s, gvp);
-            bordered_solve(M32, X32, V32);
-            gemv_batch<float>(jt, gv_t, gvp);
+            gemv_launch<float>(ps, gvp);
+            getrs_np_batch_launch<float>(bp);
+            gemv_launch<float>(pt, gvp);
             convert_matrix<float, double>(nt, k, Zo32, nt, Dt, nt);
             convert_matrix<float, double>((int)C, 1, V32, (int)C, Dsc, (int)C);
             add_into((size_t)nt * k, Dt, Xt);
@@ -627,27 +662,60 @@ void run_chunk(SharedTube& ST, std::vector<Inst*>& I, int prec, double mu, Arena
             convert_matrix<double, double>(nt, k, Bt, nt, Rt, nt);
             dgemm(nt, k, nt, -1.0, tb->T64.p, nt, Xt, nt, 1.0, Rt, nt);
             convert_matrix<double, double>((int)C, 1, Bsc, (int)C, Rsc, (int)C);
-            gemv_batch<double>(jt, gv_r, gvp);
-            {
-                const CJob* d = jc.put(cj);
-                hipLaunchKernelGGL(constraint_batch_kernel, dim3(k), dim3(256), 0, ctx().s(), d);
-                check_launch("constraint_batch");
-            }
+            gemv_launch<double>(pr, gvp);
+            hipLaunchKernelGGL(constraint_batch_kernel, dim3(k), dim3(256), 0, ctx().s(), ct.get());
+            check_launch("constraint_batch");
             hipLaunchKernelGGL(batch_norms_kernel, dim3(k), dim3(256), 0, ctx().s(), nt, Rt, Rsc, dcoff, dmlen, nrm);
             check_launch("batch_norms");
-            SQ_HIP(hipMemcpyAsync(rn.data(), nrm, sizeof(double) * k, hipMemcpyDeviceToHost, ctx().s()));
-            sync();
-            worst = 0.0;
-            for (int i = 0; i < k; ++i) {
-                rel[i] = bn[i] > 0 ? rn[i] / bn[i] : rn[i];
-                if (rel[i] > tol || itc[i] == 0) itc[i] = it + 1;
-                worst = std::max(worst, rel[i]);
+        };
+        // the step is a fixed launch sequence: capture it once and replay it (not while the
+        // per-launch timing is on, whose events would be captured too)
+        hipGraphExec_t exec = nullptr;
+        if (g_use_graph && !prof_enabled()) {
+            hipGraph_t graph = nullptr;
+            SQ_HIP(hipStreamBeginCapture(ctx().s(), hipStreamCaptureModeThreadLocal));
+            try {
+                step();
+            } catch (...) {
+                (void)hipStreamEndCapture(ctx().s(), &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
             }
-            if (worst <= tol) {
-                ++it;
-                break;
-            }
+            SQ_HIP(hipStreamEndCapture(ctx().s(), &graph));
+            const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            SQ_REQUIRE(ie == hipSuccess, SQBEM_ERR_HIP, std::string("refinement graph: ") + hipGetErrorString(ie));
         }
+        const double tol = 1e-12;
+        int it = 0;
+        double worst = 1.0;
+        try {
+            for (; it < 40; ++it) {
+                if (exec) {
+                    SQ_HIP(hipGraphLaunch(exec, ctx().s()));
+                    ++st.graph_replays;
+                } else {
+                    step();
+                }
+                SQ_HIP(hipMemcpyAsync(rn.data(), nrm, sizeof(double) * k, hipMemcpyDeviceToHost, ctx().s()));
+                mark_device_error_check();  // the chained solves report a timed-out wait here
+                sync();
+                worst = 0.0;
+                for (int i = 0; i < k; ++i) {
+                    rel[i] = bn[i] > 0 ? rn[i] / bn[i] : rn[i];
+                    if (rel[i] > tol || itc[i] == 0) itc[i] = it + 1;
+                    worst = std::max(worst, rel[i]);
+                }
+                if (worst <= tol) {
+                    ++it;
+                    break;
+                }
+            }
+        } catch (...) {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            throw;
+        }
+        if (exec) (void)hipGraphExecDestroy(exec);
         SQ_REQUIRE(worst <= 1e-9, SQBEM_ERR_CONVERGE,
                    "iterative refinement stalled at relative residual " + std::to_string(worst));
         SQ_HIP(hipMemcpyAsync(X.data(), Xsc, C * sizeof(double), hipMemcpyDeviceToHost, ctx().s()));
@@ -675,6 +743,10 @@ void run_chunk(SharedTube& ST, std::vector<Inst*>& I, int prec, double mu, Arena
 using namespace sq;
 
 extern "C" {
+
+int sqbem_batch_options(int use_graph) {
+    return guarded([&] { g_use_graph = use_graph != 0; });
+}
 
 int sqbem_batch_solve(int n, const sqbem_mesh* const* tubes, const sqbem_mesh* const* spheres, const double* centers,
                       const double* e, const double* B, double mu, int precision, double mem_budget_gb, double* U,
@@ -778,6 +850,7 @@ int sqbem_batch_solve(int n, const sqbem_mesh* const* tubes, const sqbem_mesh* c
             stats[7] = st.worst;
             stats[8] = st.pivoted;
             stats[9] = (double)need;
+            stats[10] = st.graph_replays;
         }
     });
 }

@@ -2440,56 +2440,123 @@ void getrf_np_batch(const std::vector<BatchMat<T>>& mats) {
 template void getrf_np_batch<float>(const std::vector<BatchMat<float>>&);
 template void getrf_np_batch<double>(const std::vector<BatchMat<double>>&);
 
-static DBuf<TrsvState> g_trsv_batch;
-static DBuf<unsigned char> g_trsv_jobs;
+// rows of a right-hand side gathered through a permutation: tmp = v[perm], then v = tmp
+template <typename T>
+struct PermJob {
+    int n;
+    const int* perm;
+    T* v;
+    T* tmp;
+};
+template <typename T>
+__global__ void perm_gather_kernel(const PermJob<T>* __restrict__ jobs) {
+    const PermJob<T>& j = jobs[blockIdx.y];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < j.n; i += gridDim.x * blockDim.x) j.tmp[i] = j.v[j.perm[i]];
+}
+template <typename T>
+__global__ void perm_store_kernel(const PermJob<T>* __restrict__ jobs) {
+    const PermJob<T>& j = jobs[blockIdx.y];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < j.n; i += gridDim.x * blockDim.x) j.v[i] = j.tmp[i];
+}
 
 template <typename T, bool LOWER>
-static void trsv_batch_launch(const std::vector<TrsvJob<T>>& J, const void* dev_jobs, int nbmax) {
+static constexpr size_t trsv_batch_lds() {
     constexpr int GQ = sizeof(T) == 4 ? 2 : 1;
     constexpr size_t xs = sizeof(T) * GQ * NB, red = sizeof(double) * 3 * NB;
-    constexpr size_t lds = sizeof(T) * NB * (NB + 1) + (xs > red ? xs : red);
+    return sizeof(T) * NB * (NB + 1) + (xs > red ? xs : red);
+}
+// the kernel's LDS limit, set once, outside any graph capture (from the prepare step)
+template <typename T, bool LOWER>
+static void trsv_batch_setup() {
     static bool lds_set = false;
     if (!lds_set) {
         SQ_HIP(hipFuncSetAttribute((const void*)trsv_inv_batch_kernel<T, LOWER>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)trsv_batch_lds<T, LOWER>()));
         lds_set = true;
     }
-    fill_zero(g_trsv_batch.p, sizeof(TrsvState) * J.size());
-    hipLaunchKernelGGL((trsv_inv_batch_kernel<T, LOWER>), dim3(nbmax, (unsigned)J.size()), dim3(256), lds, ctx().s(),
-                       (const TrsvJob<T>*)dev_jobs, g_trsv_batch.p, device_error_word());
+}
+template <typename T, bool LOWER>
+static void trsv_batch_launch(int njobs, const void* dev_jobs, int nbmax, TrsvState* state) {
+    constexpr size_t lds = trsv_batch_lds<T, LOWER>();
+    trsv_batch_setup<T, LOWER>();
+    fill_zero(state, sizeof(TrsvState) * njobs);  // a kernel, not a memset: safe in captured graphs
+    hipLaunchKernelGGL((trsv_inv_batch_kernel<T, LOWER>), dim3(nbmax, (unsigned)njobs), dim3(256), lds, ctx().s(),
+                       (const TrsvJob<T>*)dev_jobs, state, device_error_word());
     check_launch("trsv_inv_batch");
 }
 
 template <typename T>
-void getrs_np_batch(const std::vector<BatchSolve<T>>& s) {
-    if (s.empty()) return;
+void getrs_np_batch_prepare(const std::vector<BatchSolve<T>>& s, BatchSolvePlan<T>& p) {
     std::vector<TrsvJob<T>> L, U;
-    int nbmax = 0;
-    double flops = 0, bytes = 0;
+    std::vector<PermJob<T>> P;
+    p.njobs = (int)s.size();
+    p.nbmax = 0;
+    p.flops = p.bytes = 0;
+    size_t tmp_need = 0;
     for (const auto& q : s) {
         const size_t nb = (size_t)(q.n + NB - 1) / NB;
-        nbmax = std::max(nbmax, (int)nb);
+        p.nbmax = std::max(p.nbmax, (int)nb);
         L.push_back({q.n, q.aux + 2 * nb * NB * NB, q.aux, q.B});
         U.push_back({q.n, q.aux + 2 * nb * NB * NB, q.aux + nb * NB * NB, q.B});
-        flops += 2.0 * q.n * (double)q.n;
-        bytes += sizeof(T) * (double)q.n * q.n;
+        p.flops += 2.0 * q.n * (double)q.n;
+        p.bytes += sizeof(T) * (double)q.n * q.n;
+        if (q.perm) tmp_need += q.n;
     }
-    if (!nbmax) return;
-    if (g_trsv_batch.n < s.size()) g_trsv_batch.alloc(std::max<size_t>(s.size(), 256));
-    const size_t half = (L.size() * sizeof(TrsvJob<T>) + 63) / 64 * 64;
-    if (g_trsv_jobs.n < 2 * half) g_trsv_jobs.alloc(std::max<size_t>(2 * half, 1 << 16));
-    std::vector<unsigned char> h(2 * half);
+    if (s.empty() || !p.nbmax) return;
+    if (tmp_need) p.tmp.alloc(tmp_need);
+    size_t toff = 0;
+    for (const auto& q : s)
+        if (q.perm) {
+            P.push_back({q.n, q.perm, q.B, p.tmp.p + toff});
+            toff += q.n;
+        }
+    p.nperm = (int)P.size();
+    p.permmax = 1;
+    for (const auto& j : P) p.permmax = std::max(p.permmax, j.n);
+    p.half = (L.size() * sizeof(TrsvJob<T>) + 63) / 64 * 64;
+    const size_t pbytes = P.size() * sizeof(PermJob<T>);
+    std::vector<unsigned char> h(2 * p.half + pbytes);
     std::memcpy(h.data(), L.data(), L.size() * sizeof(TrsvJob<T>));
-    std::memcpy(h.data() + half, U.data(), U.size() * sizeof(TrsvJob<T>));
-    SQ_HIP(hipMemcpyAsync(g_trsv_jobs.p, h.data(), h.size(), hipMemcpyHostToDevice, ctx().s()));
+    std::memcpy(h.data() + p.half, U.data(), U.size() * sizeof(TrsvJob<T>));
+    if (pbytes) std::memcpy(h.data() + 2 * p.half, P.data(), pbytes);
+    p.jobs.alloc(h.size());
+    p.state.alloc(std::max<size_t>(s.size(), 1));
+    SQ_HIP(hipMemcpyAsync(p.jobs.p, h.data(), h.size(), hipMemcpyHostToDevice, ctx().s()));
+    (void)device_error_word();  // allocated here, outside any capture
+    trsv_batch_setup<T, true>();
+    trsv_batch_setup<T, false>();
+    sync();                     // h is staging
+}
+
+template <typename T>
+void getrs_np_batch_launch(const BatchSolvePlan<T>& p) {
+    if (!p.njobs || !p.nbmax) return;
+    if (p.nperm) {
+        const void* pj = p.jobs.p + 2 * p.half;
+        const unsigned gx = ceil_div(p.permmax, 256);
+        hipLaunchKernelGGL(perm_gather_kernel<T>, dim3(gx, p.nperm), dim3(256), 0, ctx().s(), (const PermJob<T>*)pj);
+        hipLaunchKernelGGL(perm_store_kernel<T>, dim3(gx, p.nperm), dim3(256), 0, ctx().s(), (const PermJob<T>*)pj);
+        check_launch("perm");
+    }
     mark_device_error_check();
-    ProfScope prof(PROF_TRSV, flops, bytes);
-    trsv_batch_launch<T, true>(L, g_trsv_jobs.p, nbmax);
-    trsv_batch_launch<T, false>(U, g_trsv_jobs.p + half, nbmax);
-    sync();  // h is staging; also surfaces a timed-out wait
+    ProfScope prof(PROF_TRSV, p.flops, p.bytes);
+    trsv_batch_launch<T, true>(p.njobs, p.jobs.p, p.nbmax, p.state.p);
+    trsv_batch_launch<T, false>(p.njobs, p.jobs.p + p.half, p.nbmax, p.state.p);
+}
+
+template <typename T>
+void getrs_np_batch(const std::vector<BatchSolve<T>>& s) {
+    BatchSolvePlan<T> p;
+    getrs_np_batch_prepare<T>(s, p);
+    getrs_np_batch_launch<T>(p);
+    sync();  // surfaces a timed-out wait; p's buffers go back to the cache afterwards
 }
 template void getrs_np_batch<float>(const std::vector<BatchSolve<float>>&);
 template void getrs_np_batch<double>(const std::vector<BatchSolve<double>>&);
+template void getrs_np_batch_prepare<float>(const std::vector<BatchSolve<float>>&, BatchSolvePlan<float>&);
+template void getrs_np_batch_prepare<double>(const std::vector<BatchSolve<double>>&, BatchSolvePlan<double>&);
+template void getrs_np_batch_launch<float>(const BatchSolvePlan<float>&);
+template void getrs_np_batch_launch<double>(const BatchSolvePlan<double>&);
 
 template <typename T>
 void gemm_batch(const std::vector<GemmJob<T>>& jobs) {

@@ -3,6 +3,8 @@
 #include <cstddef>
 #include <vector>
 
+#include "common.hpp"
+
 namespace sq {
 
 // C = alpha A B + beta C on the library stream (column-major, device pointers)
@@ -64,14 +66,34 @@ struct BatchMat {
 };
 template <typename T>
 void getrf_np_batch(const std::vector<BatchMat<T>>& mats);
-// B_j <- A_j^-1 B_j, one right-hand side each, from getrf_np_batch factors (their aux)
+// B_j <- A_j^-1 B_j, one right-hand side each, from factors with getrf's solve aid (aux):
+// rows first gathered through perm (device, n entries: B <- B[perm], the row interchanges
+// of a pivoted factorisation as one permutation) unless perm is null
 template <typename T>
 struct BatchSolve {
     int n;
     const T* aux;
     T* B;
+    const int* perm = nullptr;
 };
 template <typename T>
 void getrs_np_batch(const std::vector<BatchSolve<T>>& s);
+// the same split in two: prepare uploads the job tables (and synchronises); launch only
+// issues kernels (no allocation, copy or synchronisation), so it may be captured in a
+// graph and replayed while the plan lives
+struct TrsvState;
+template <typename T>
+struct BatchSolvePlan {
+    int njobs = 0, nbmax = 0, nperm = 0, permmax = 1;
+    size_t half = 0;
+    double flops = 0, bytes = 0;
+    DBuf<unsigned char> jobs;
+    DBuf<TrsvState> state;
+    DBuf<T> tmp;
+};
+template <typename T>
+void getrs_np_batch_prepare(const std::vector<BatchSolve<T>>& s, BatchSolvePlan<T>& p);
+template <typename T>
+void getrs_np_batch_launch(const BatchSolvePlan<T>& p);
 
 }  // namespace sq

@@ -81,6 +81,7 @@ _SIGS = {
     "sqbem_swimmer_gradient": (_i, [_vp, _vp, _d, _d, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sqbem_trajectory_gradient": (_i, [_d, _d, _d, _d, _d, _d, _i, _i, _d, _i, _i, _i, _i, _i, _vp]),
     "sqbem_batch_solve": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _d, _i, _d, _vp, _vp, _vp, _vp, _vp]),
+    "sqbem_batch_options": (_i, [_i]),
     "sqbem_velocity_at": (_i, [_vp, _vp, _vp, _d, _vp, _i, _vp]),
     "sqbem_duct_create": (_vp, [_vp, _d]),
     "sqbem_duct_solve": (_i, [_vp, _d, _d, _vp, _vp, _vp]),
@@ -592,14 +593,19 @@ def batch_solve(instances, mu=1.0, precision=FP64, mem_budget_gb=0.0) -> BatchRe
     Om = np.zeros((n, 3))
     it = np.zeros(n, dtype=np.int32)
     res = np.zeros(n)
-    st = np.zeros(10)
+    st = np.zeros(11)
     check(lib().sqbem_batch_solve(n, ctypes.cast(tubes, ctypes.c_void_p), ctypes.cast(spheres, ctypes.c_void_p),
                                   _ptr(c), _ptr(e), _ptr(B), mu, precision, float(mem_budget_gb), _ptr(U), _ptr(Om),
                                   _ptr(it), _ptr(res), _ptr(st)))
     stats = {"tube_s": st[0], "assembly_s": st[1], "schur_s": st[2], "solve_s": st[3], "unique_tubes": int(st[4]),
              "chunks": int(st[5]), "max_iters": int(st[6]), "worst_residual": st[7], "pivoted": int(st[8]),
-             "workspace_gb": st[9] / 1e9}
+             "workspace_gb": st[9] / 1e9, "graph_replays": int(st[10])}
     return BatchResult(U, Om, it, res, stats)
+
+
+def batch_options(use_graph=True):
+    """use_graph: capture each chunk's refinement step as a HIP graph and replay it."""
+    check(lib().sqbem_batch_options(1 if use_graph else 0))
 
 
 def draw_batch_params(n=256, seed=0):
