@@ -30,14 +30,16 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAKS = {"dgemm": 78.6e12, "sgemm": 157.3e12, "gemm16": 2.5e15, "gemm16q": 2.5e15, "gemm16q_128x128": 2.5e15,
-         "gemm16q_128x256": 2.5e15, "gemm16q_256x128": 2.5e15, "gemm16q_256x256": 2.5e15}
-# the kernel each timing class measures (as rocprofv3 names it)
-KERNEL_NAMES = {"dgemm": "gemm_nn_kernel<double>", "sgemm": "gemm_nn_kernel<float>", "gemm16": "gemm16_nn_kernel",
-                "gemm16q": "gemm16q_kernel (all tiles)", "gemm16q_128x128": "gemm16q_kernel<{t}, 128, 128>",
-                "gemm16q_128x256": "gemm16q_kernel<{t}, 128, 256>", "gemm16q_256x128": "gemm16q_kernel<{t}, 256, 128>",
-                "gemm16q_256x256": "gemm16q_kernel<{t}, 256, 256>"}
-TIMED = ["dgemm", "sgemm", "gemm16", "gemm16q", "gemm16q_128x128", "gemm16q_128x256", "gemm16q_256x128", "gemm16q_256x256"]
+PEAKS = {"dgemm": 78.6e12, "sgemm": 157.3e12, "gemm16": 2.5e15, "gemm16q": 2.5e15}
+# the kernel each timing class measures (as rocprofv3 names it; {t} = true for BF16, false for FP16)
+KERNEL_NAMES = {"dgemm": "gemm_nn_kernel<double>", "sgemm": "gemm_nn_kernel<float>", "gemm16": "gemm16_nn_kernel<{t}>",
+                "gemm16q": "gemm16q_kernel (all candidates)"}
+for _c, (_tm, _tn, _pf) in enumerate([(128, 128, 1), (128, 256, 1), (256, 128, 1), (256, 256, 1), (128, 128, 2),
+                                       (128, 256, 2), (256, 128, 2)]):
+    _k = "gemm16q_" + ["128x128", "128x256", "256x128", "256x256", "128x128_pf2", "128x256_pf2", "256x128_pf2"][_c]
+    PEAKS[_k] = 2.5e15
+    KERNEL_NAMES[_k] = "gemm16q_kernel<{t}, %d, %d, %d>" % (_tm, _tn, _pf)
+TIMED = list(KERNEL_NAMES)
 MODES = {"fp64": 0, "fp32": 1, "fp16": 2, "bf16": 3}
 
 
@@ -428,7 +430,7 @@ def main():
     kp = prof[kname]
     achieved = kp["flops"] / (kp["ms"] * 1e-3) if kp["ms"] > 0 else 0.0
     nl = max(kp["launches"], 1)
-    tname = "__bf16" if prec == 3 else "_Float16"
+    tname = "true" if prec == 3 else "false"
     roofline = {"bound": "mfma", "kernel": KERNEL_NAMES[kname].format(t=tname), "achieved": achieved / 1e12,
                 "peak": PEAKS[kname] / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAKS[kname], "traffic": None,
                 "launches": kp["launches"], "avg_launch_ms": kp["ms"] / nl, "flops_per_launch": kp["flops"] / nl,

@@ -58,11 +58,10 @@ int sqbem_trim(double* released_gb);
 #define SQBEM_PROF_LASWP 7
 #define SQBEM_PROF_GEMM16Q 8 /* 16-bit GEMM on pre-converted, padded operands (large shapes) */
 #define SQBEM_PROF_TRSV 9    /* triangular solves with at most 8 right-hand sides (one launch each) */
-/* gemm16q launches by workgroup tile (each also counted in SQBEM_PROF_GEMM16Q) */
-#define SQBEM_PROF_G16Q_128x128 10
-#define SQBEM_PROF_G16Q_128x256 11
-#define SQBEM_PROF_G16Q_256x128 12
-#define SQBEM_PROF_G16Q_256x256 13
+/* gemm16q launches by autotuner candidate c = 0..6 (each also counted in SQBEM_PROF_GEMM16Q):
+ * workgroup tiles 128x128, 128x256, 256x128, 256x256 with loads staged one k tile ahead
+ * (c = 0..3), and 128x128, 128x256, 256x128 staged two k tiles ahead (c = 4..6) */
+#define SQBEM_PROF_G16Q(c) (10 + (c))
 int sqbem_profile(int mask);
 int sqbem_profile_get(int kernel_class, long* launches, double* flops, double* bytes, double* ms);
 
@@ -119,12 +118,12 @@ int sqbem_sgetrf(int kind, int n, float* A, int lda, int* ipiv, int* info);
 int sqbem_sgetrs(int n, int nrhs, const float* LU, int lda, const int* ipiv, float* B, int ldb);
 
 /* Tile autotuner of the 16-bit GEMMs (no reference counterpart): mode 1 (the default)
- * learns the fastest of four workgroup tiles (128/256 rows x 128/256 columns) per shape
- * class (operand type, binary exponents of m, n, k) by timing each candidate once on live
- * calls; mode 0 uses the fixed rule (256 x 256 tiles when they fill the device, else
- * 128 x 128). Either call clears what was learned. sqbem_tune_info: number of shape
- * classes seen, how many are decided, and counts[4] of decided classes per tile
- * (128x128, 128x256, 256x128, 256x256); any pointer may be NULL. */
+ * learns the fastest of seven candidates (the SQBEM_PROF_G16Q list: workgroup tile and
+ * load staging depth) per shape class (operand type, binary exponents of m, n, k) by
+ * timing each candidate once on live calls; mode 0 uses the fixed rule (256 x 256 tiles
+ * when they fill the device, else 128 x 128). Either call clears what was learned.
+ * sqbem_tune_info: number of shape classes seen, how many are decided, and counts[7] of
+ * decided classes per candidate; any pointer may be NULL. */
 int sqbem_tune(int mode);
 int sqbem_tune_info(int* classes, int* decided, int* counts);
 /* Decided classes to a text file, and back: a loaded class uses its saved tile from the

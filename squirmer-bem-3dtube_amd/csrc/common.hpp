@@ -112,8 +112,8 @@ inline void sync() {
 // one launch on the library stream together with its algorithmic flops and bytes.
 enum ProfClass { PROF_DGEMM = 0, PROF_SGEMM, PROF_GEMM16, PROF_PANEL, PROF_TRSM, PROF_ASSEMBLE, PROF_GEMV, PROF_LASWP,
                  PROF_GEMM16Q, PROF_TRSV,
-                 // gemm16q by workgroup tile (each launch also counts in PROF_GEMM16Q)
-                 PROF_G16Q_128x128, PROF_G16Q_128x256, PROF_G16Q_256x128, PROF_G16Q_256x256,
+                 // gemm16q by autotuner candidate 0..6 (each launch also counts in PROF_GEMM16Q)
+                 PROF_G16Q_C0, PROF_G16Q_C1, PROF_G16Q_C2, PROF_G16Q_C3, PROF_G16Q_C4, PROF_G16Q_C5, PROF_G16Q_C6,
                  PROF_NCLASS };
 bool prof_enabled();
 struct ProfScope {

@@ -389,8 +389,11 @@ __global__ void pow2_scale_cols(int k, int n, const float* __restrict__ B, int l
 constexpr int QBK = 64, QPAD = QBK + 8;
 
 // Workgroup tile TM x TN (256 x 256 for large products, 128 x 128 for mid-sized ones), four
-// waves of TM/2 x TN/2, each a grid of 32 x 32 MFMA tiles.
-template <bool BF16, int TM, int TN>
+// waves of TM/2 x TN/2, each a grid of 32 x 32 MFMA tiles. PF register sets stage the global
+// loads: with PF = 1 a k tile is loaded one tile before it is stored into LDS, with PF = 2
+// two tiles before, so a load has two k tiles of MFMA work to arrive in (short-k products
+// wait on the load latency otherwise).
+template <bool BF16, int TM, int TN, int PF>
 __global__ void __launch_bounds__(256) gemm16q_kernel(int m, int n, int kp, float alpha,
                                                       const typename Half<BF16>::t* __restrict__ Ah,
                                                       const typename Half<BF16>::t* __restrict__ Bh, float beta,
@@ -413,29 +416,30 @@ __global__ void __launch_bounds__(256) gemm16q_kernel(int m, int n, int kp, floa
     const int wm = (wave & 1) * (TM / 2), wn = (wave >> 1) * (TN / 2);
     typedef unsigned int u4 __attribute__((ext_vector_type(4)));
     // chunk c = tid + 256 q of a panel: row c >> 3, 8-element k chunk c & 7
-    u4 ra[CA], rb[CB];
-    auto load = [&](int k0) {
+    u4 ra[CA], rb[CB];                              // register set 0
+    u4 ra1[PF == 2 ? CA : 1], rb1[PF == 2 ? CB : 1];  // register set 1 (PF = 2)
+    auto load_into = [&](int k0, u4* A_, u4* B_) {
 #pragma unroll
         for (int q = 0; q < CA; ++q) {
             const int c = tid + 256 * q, row = c >> 3, kc = c & 7;
-            ra[q] = *(const u4*)(Ah + (size_t)(m0 + row) * kp + k0 + 8 * kc);
+            A_[q] = *(const u4*)(Ah + (size_t)(m0 + row) * kp + k0 + 8 * kc);
         }
 #pragma unroll
         for (int q = 0; q < CB; ++q) {
             const int c = tid + 256 * q, row = c >> 3, kc = c & 7;
-            rb[q] = *(const u4*)(Bh + (size_t)(n0 + row) * kp + k0 + 8 * kc);
+            B_[q] = *(const u4*)(Bh + (size_t)(n0 + row) * kp + k0 + 8 * kc);
         }
     };
-    auto store = [&](int stage) {
+    auto store_from = [&](int stage, const u4* A_, const u4* B_) {
 #pragma unroll
         for (int q = 0; q < CA; ++q) {
             const int c = tid + 256 * q, row = c >> 3, kc = c & 7;
-            *(u4*)&As_[stage][row][8 * kc] = ra[q];
+            *(u4*)&As_[stage][row][8 * kc] = A_[q];
         }
 #pragma unroll
         for (int q = 0; q < CB; ++q) {
             const int c = tid + 256 * q, row = c >> 3, kc = c & 7;
-            *(u4*)&Bs_[stage][row][8 * kc] = rb[q];
+            *(u4*)&Bs_[stage][row][8 * kc] = B_[q];
         }
     };
     f16x acc[IM][JN];
@@ -447,12 +451,7 @@ __global__ void __launch_bounds__(256) gemm16q_kernel(int m, int n, int kp, floa
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
     const int nk = kp / QBK;
     const int lr = lane & 31, lk = lane >> 5;
-    load(0);
-    store(0);
-    if (nk > 1) load(QBK);
-    __syncthreads();
-    for (int kt = 0; kt < nk; ++kt) {
-        const int st = kt & 1;
+    auto compute = [&](int st) {
 #pragma unroll
         for (int s2 = 0; s2 < QBK / 16; ++s2) {
             v8 af[IM], bf[JN];
@@ -465,11 +464,42 @@ __global__ void __launch_bounds__(256) gemm16q_kernel(int m, int n, int kp, floa
 #pragma unroll
                 for (int j = 0; j < JN; ++j) acc[i][j] = H::mma32(af[i], bf[j], acc[i][j]);
         }
-        if (kt + 1 < nk) {
-            store((kt + 1) & 1);  // the other stage: last read before the previous barrier
-            if (kt + 2 < nk) load((kt + 2) * QBK);
-        }
+    };
+    if constexpr (PF == 1) {
+        load_into(0, ra, rb);
+        store_from(0, ra, rb);
+        if (nk > 1) load_into(QBK, ra, rb);
         __syncthreads();
+        for (int kt = 0; kt < nk; ++kt) {
+            compute(kt & 1);
+            if (kt + 1 < nk) {
+                store_from((kt + 1) & 1, ra, rb);  // the other stage: last read before the previous barrier
+                if (kt + 2 < nk) load_into((kt + 2) * QBK, ra, rb);
+            }
+            __syncthreads();
+        }
+    } else {
+        // tile t waits in register set t & 1 and is computed from LDS stage t & 1
+        load_into(0, ra, rb);
+        store_from(0, ra, rb);
+        if (nk > 1) load_into(QBK, ra1, rb1);
+        if (nk > 2) load_into(2 * QBK, ra, rb);
+        __syncthreads();
+        for (int kt = 0; kt < nk; kt += 2) {
+            compute(0);
+            if (kt + 1 < nk) {
+                store_from(1, ra1, rb1);
+                if (kt + 3 < nk) load_into((kt + 3) * QBK, ra1, rb1);
+            }
+            __syncthreads();
+            if (kt + 1 >= nk) break;
+            compute(1);
+            if (kt + 2 < nk) {
+                store_from(0, ra, rb);
+                if (kt + 4 < nk) load_into((kt + 4) * QBK, ra, rb);
+            }
+            __syncthreads();
+        }
     }
     // epilogue, one 32-row band of tiles at a time: scales and C values are read together
     // from clamped addresses, then the results stored under the bounds test
@@ -581,12 +611,14 @@ static DBuf<char> g_q16;
 // the same product, so exploring costs time, never correctness. sqbem_tune(0) restores
 // the fixed rule.
 namespace tune {
-constexpr int NC = 4;  // 0: 128 x 128, 1: 128 x 256, 2: 256 x 128, 3: 256 x 256
+// 0: 128 x 128, 1: 128 x 256, 2: 256 x 128, 3: 256 x 256 (loads staged one k tile ahead);
+// 4: 128 x 128, 5: 128 x 256, 6: 256 x 128 (two k tiles ahead)
+constexpr int NC = 7;
 struct Cls {
     int tried = 0, best = -1;
-    float ms[NC] = {0, 0, 0, 0};
-    bool done[NC] = {false, false, false, false};
-    hipEvent_t e0[NC] = {nullptr, nullptr, nullptr, nullptr}, e1[NC] = {nullptr, nullptr, nullptr, nullptr};
+    float ms[NC] = {};
+    bool done[NC] = {};
+    hipEvent_t e0[NC] = {}, e1[NC] = {};
 };
 static std::map<unsigned, Cls> g_table;
 static int g_mode = 1;
@@ -661,12 +693,14 @@ static void save(const char* path) {
     SQ_HIP(hipStreamSynchronize(ctx().s()));
     FILE* f = std::fopen(path, "w");
     SQ_REQUIRE(f, SQBEM_ERR_ARG, std::string("cannot open tune table for writing: ") + path);
-    std::fprintf(f, "# sqbem gemm16q tile table: key(kind<<24|lg m<<16|lg n<<8|lg k) best ms0 ms1 ms2 ms3\n");
+    std::fprintf(f, "# sqbem gemm16q tile table: key(kind<<24|lg m<<16|lg n<<8|lg k) best ms[0..%d]\n", NC - 1);
     for (auto& kv : g_table) {
         poll(kv.second);
         if (kv.second.best < 0) continue;
         const Cls& c = kv.second;
-        std::fprintf(f, "%u %d %.6g %.6g %.6g %.6g\n", kv.first, c.best, c.ms[0], c.ms[1], c.ms[2], c.ms[3]);
+        std::fprintf(f, "%u %d", kv.first, c.best);
+        for (int i = 0; i < NC; ++i) std::fprintf(f, " %.6g", c.ms[i]);
+        std::fprintf(f, "\n");
     }
     std::fclose(f);
 }
@@ -680,11 +714,11 @@ static int load(const char* path) {
     while (std::fgets(line, sizeof line, f)) {
         if (line[0] == '#') continue;
         unsigned key;
-        int best;
-        float ms[NC];
-        if (std::sscanf(line, "%u %d %g %g %g %g", &key, &best, &ms[0], &ms[1], &ms[2], &ms[3]) != 6 || best < 0 ||
-            best >= NC)
-            continue;
+        int best, pos = 0, used = 0;
+        float ms[NC] = {};
+        if (std::sscanf(line, "%u %d%n", &key, &best, &pos) != 2 || best < 0 || best >= NC) continue;
+        // older tables hold fewer candidates: the missing times stay 0 (unused, best is given)
+        for (int i = 0; i < NC && std::sscanf(line + pos, " %g%n", &ms[i], &used) == 1; ++i) pos += used;
         Cls& c = g_table[key];
         for (int i = 0; i < NC; ++i) {
             if (c.e0[i]) (void)hipEventDestroy(c.e0[i]);
@@ -718,7 +752,7 @@ static void info(int* classes, int* decided, int* counts) {
 }
 }  // namespace tune
 
-template <bool BF16, int TM, int TN>
+template <bool BF16, int TM, int TN, int PF>
 static void gemm16q(int m, int n, int k, float alpha, const float* A, int lda, const float* B, int ldb, float beta,
                     float* C, int ldc, float* rs, float* cs) {
     using ht = typename Half<BF16>::t;
@@ -736,10 +770,10 @@ static void gemm16q(int m, int n, int k, float alpha, const float* A, int lda, c
     const int tiles_m = mp / TM, tiles_n = np / TN;
     const double fl = 2.0 * m * n * k, by = 2.0 * ((double)m * k + (double)k * n) + 4.0 * (beta != 0.0f ? 2.0 : 1.0) * m * n;
     ProfScope prof(PROF_GEMM16Q, fl, by);
-    ProfScope prof_tile(TM == 128 ? (TN == 128 ? PROF_G16Q_128x128 : PROF_G16Q_128x256)
-                                  : (TN == 128 ? PROF_G16Q_256x128 : PROF_G16Q_256x256),
-                        fl, by);
-    hipLaunchKernelGGL((gemm16q_kernel<BF16, TM, TN>), dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, kp,
+    constexpr int cand = PF == 2 ? (TM == 128 ? (TN == 128 ? 4 : 5) : 6)
+                                 : (TM == 128 ? (TN == 128 ? 0 : 1) : (TN == 128 ? 2 : 3));
+    ProfScope prof_tile(PROF_G16Q_C0 + cand, fl, by);
+    hipLaunchKernelGGL((gemm16q_kernel<BF16, TM, TN, PF>), dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, kp,
                        alpha, Ah, Bh, beta, C, ldc, tiles_m, rs, cs);
     check_launch("gemm16q");
 }
@@ -772,10 +806,13 @@ void gemm16(int kind, int m, int n, int k, float alpha, const float* A, int lda,
         auto run = [&](auto tag_bf) {
             constexpr bool BFv = decltype(tag_bf)::value;
             switch (probe.cand) {
-                case 0: gemm16q<BFv, 128, 128>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs); break;
-                case 1: gemm16q<BFv, 128, 256>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs); break;
-                case 2: gemm16q<BFv, 256, 128>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs); break;
-                default: gemm16q<BFv, 256, 256>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs); break;
+                case 0: gemm16q<BFv, 128, 128, 1>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs); break;
+                case 1: gemm16q<BFv, 128, 256, 1>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs); break;
+                case 2: gemm16q<BFv, 256, 128, 1>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs); break;
+                case 4: gemm16q<BFv, 128, 128, 2>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs); break;
+                case 5: gemm16q<BFv, 128, 256, 2>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs); break;
+                case 6: gemm16q<BFv, 256, 128, 2>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs); break;
+                default: gemm16q<BFv, 256, 256, 1>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs); break;
             }
         };
         if (bf) run(std::true_type{});

@@ -154,9 +154,10 @@ def device_info():
     return name.value.decode(), cu.value, gb.value
 
 
+TUNE_TILES = ["128x128", "128x256", "256x128", "256x256", "128x128/pf2", "128x256/pf2", "256x128/pf2"]
 PROF_CLASSES = {"dgemm": 0, "sgemm": 1, "gemm16": 2, "panel": 3, "trsm": 4, "assemble": 5, "gemv": 6, "laswp": 7,
-                "gemm16q": 8, "trsv": 9, "gemm16q_128x128": 10, "gemm16q_128x256": 11, "gemm16q_256x128": 12,
-                "gemm16q_256x256": 13}
+                "gemm16q": 8, "trsv": 9}
+PROF_CLASSES.update({"gemm16q_" + t.replace("/", "_"): 10 + c for c, t in enumerate(TUNE_TILES)})
 
 
 def profile(enable=True, classes=None):
@@ -194,7 +195,6 @@ def tune(mode: int = 1):
     check(lib().sqbem_tune(int(mode)))
 
 
-TUNE_TILES = ["128x128", "128x256", "256x128", "256x256"]
 TUNE_TABLE = os.path.join(HERE, "tune_gfx950.txt")  # learned on an MI355X by bench.py --tune-save
 
 
@@ -211,7 +211,7 @@ def tune_load(path: str = TUNE_TABLE) -> int:
 
 def tune_info() -> dict:
     cl, dec = ctypes.c_int(), ctypes.c_int()
-    counts = np.zeros(4, dtype=np.int32)
+    counts = np.zeros(len(TUNE_TILES), dtype=np.int32)
     check(lib().sqbem_tune_info(ctypes.byref(cl), ctypes.byref(dec), _ptr(counts)))
     return {"classes": cl.value, "decided": dec.value, "tiles": dict(zip(TUNE_TILES, counts.tolist()))}
 

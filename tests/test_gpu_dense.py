@@ -95,7 +95,7 @@ def test_gemm16(sq, kind, tol, m, n, k):
 
 @pytest.mark.parametrize("kind,tol", [(2, 2.5e-3), (3, 2e-2)])
 def test_gemm16_autotuner_every_tile(sq, kind, tol):
-    """The tile autotuner runs each of its four workgroup tiles once on a fresh shape class
+    """The tile autotuner runs each of its seven candidates once on a fresh shape class
     (odd sizes, beta != 0) before settling: every call's product must be right, and the
     class must end decided."""
     m, n, k = 700, 555, 300
@@ -107,7 +107,7 @@ def test_gemm16_autotuner_every_tile(sq, kind, tol):
     scale = 0.5 * (np.abs(A).astype(np.float64) @ np.abs(B).astype(np.float64)) + 2.0 * np.abs(C0)
     sq.tune(1)
     try:
-        for call in range(6):
+        for call in range(9):
             dA, dB, dC = dev(A), dev(B), dev(C0)
             torch.cuda.synchronize()
             sq.gemm16(kind, 0.5, dA, dB, -2.0, dC)
@@ -129,7 +129,7 @@ def test_gemm16_tune_table_save_and_load(sq, tmp_path):
     ref = A.astype(np.float64) @ B.astype(np.float64)
     sq.tune(1)
     try:
-        for _ in range(5):
+        for _ in range(8):
             dA, dB, dC = dev(A), dev(B), dev(np.zeros((m, n), dtype=np.float32))
             torch.cuda.synchronize()
             sq.gemm16(3, 1.0, dA, dB, 0.0, dC)
