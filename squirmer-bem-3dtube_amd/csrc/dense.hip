@@ -86,25 +86,28 @@ struct GemmK {
 
 // MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16] and B[l/16][l%16]; the four
 // accumulator values of lane l are D[Op::row(l/16, r)][l%16].
-template <typename T>
+// Workgroup tile 128 x TN (TN = 128, or 64 for products whose 128 x 128 tiles would leave
+// CUs idle): four waves of 64 x TN/2.
+template <typename T, int TN = GBN>
 __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const T* __restrict__ A, int lda,
                                              const T* __restrict__ B, int ldb, T beta, T* __restrict__ C, int ldc,
                                              int tm, int tn) {
     using Op = MfmaOp<T>;
     using Acc = typename Op::acc;
-    constexpr int KB = GemmK<T>::KB, PAD = GemmK<T>::PAD, NQ = KB / 2, JS = 256 / KB;
+    constexpr int KB = GemmK<T>::KB, PAD = GemmK<T>::PAD, NQ = KB / 2, JS = 256 / KB, NQB = TN * KB / 256;
+    constexpr int JT = TN / 32;  // 16-column MFMA tiles per wave
     __shared__ T As[2][GBM][PAD];
-    __shared__ T Bs[2][GBN][PAD];
+    __shared__ T Bs[2][TN][PAD];
 
-    const int m0 = tm * GBM, n0 = tn * GBN;
+    const int m0 = tm * GBM, n0 = tn * TN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * (TN / 2);
 
     // global -> register staging: A column kk has 128 contiguous rows; B column j has KB
     // contiguous k values.
     const int a_i = tid & 127, a_k0 = tid >> 7;     // rows a_i, k = a_k0 + 2q
     const int b_k = tid % KB, b_j0 = tid / KB;      // k = b_k, cols b_j0 + JS q
-    T ra[NQ], rb[NQ];
+    T ra[NQ], rb[NQB];
 
     auto load = [&](int k0) {
         const int gi = m0 + a_i;
@@ -115,7 +118,7 @@ __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const
         }
         const int gk = k0 + b_k;
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
+        for (int q = 0; q < NQB; ++q) {
             int gj = n0 + b_j0 + JS * q;
             rb[q] = (gk < k && gj < n) ? B[gk + (size_t)gj * ldb] : T(0);
         }
@@ -124,14 +127,14 @@ __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const
 #pragma unroll
         for (int q = 0; q < NQ; ++q) As[buf][a_i][a_k0 + 2 * q] = ra[q];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) Bs[buf][b_j0 + JS * q][b_k] = rb[q];
+        for (int q = 0; q < NQB; ++q) Bs[buf][b_j0 + JS * q][b_k] = rb[q];
     };
 
-    Acc acc[4][4];
+    Acc acc[4][JT];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = Acc{0, 0, 0, 0};
+        for (int j = 0; j < JT; ++j) acc[i][j] = Acc{0, 0, 0, 0};
 
     const int nk = (k + KB - 1) / KB;
     load(0);
@@ -143,15 +146,15 @@ __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const
         if (kt + 1 < nk) load((kt + 1) * KB);
 #pragma unroll
         for (int s = 0; s < KB / 4; ++s) {
-            T af[4], bf[4];
+            T af[4], bf[JT];
 #pragma unroll
             for (int i = 0; i < 4; ++i) af[i] = As[buf][wm + i * 16 + lr][4 * s + lk];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) bf[j] = Bs[buf][wn + j * 16 + lr][4 * s + lk];
+            for (int j = 0; j < JT; ++j) bf[j] = Bs[buf][wn + j * 16 + lr][4 * s + lk];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = Op::mma(af[i], bf[j], acc[i][j]);
+                for (int j = 0; j < JT; ++j) acc[i][j] = Op::mma(af[i], bf[j], acc[i][j]);
         }
         if (kt + 1 < nk) store(buf ^ 1);
         __syncthreads();
@@ -160,7 +163,7 @@ __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < JT; ++j) {
             const int col = n0 + wn + j * 16 + lr;
             if (col >= n) continue;
 #pragma unroll
@@ -175,13 +178,13 @@ __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const
         }
 }
 
-template <typename T>
+template <typename T, int TN = GBN>
 __global__ void __launch_bounds__(256) gemm_nn_kernel(int m, int n, int k, T alpha, const T* __restrict__ A,
                                                       int lda, const T* __restrict__ B, int ldb, T beta,
                                                       T* __restrict__ C, int ldc, int tiles_m) {
     int tm, tn;
     tile_of(blockIdx.x, gridDim.x, tiles_m, tm, tn);
-    gemm_nn_tile<T>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn);
+    gemm_nn_tile<T, TN>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn);
 }
 
 // Many independent GEMMs C_j = alpha_j A_j B_j + beta_j C_j in one launch: workgroup
@@ -573,11 +576,27 @@ static void gemm_launch(int m, int n, int k, T alpha, const T* A, int lda, const
         scale_matrix(m, n, beta, C, ldc);
         return;
     }
-    int tiles_m = (int)ceil_div(m, GBM), tiles_n = (int)ceil_div(n, GBN);
+    const int tiles_m = (int)ceil_div(m, GBM);
     ProfScope prof(sizeof(T) == 8 ? PROF_DGEMM : PROF_SGEMM, 2.0 * m * n * k,
                    (double)sizeof(T) * ((double)m * k + (double)k * n + (beta != T(0) ? 2.0 : 1.0) * m * n));
-    hipLaunchKernelGGL(gemm_nn_kernel<T>, dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, k, alpha, A, lda,
-                       B, ldb, beta, C, ldc, tiles_m);
+    // 128 x 64 tiles when 128 x 128 ones would leave CUs idle (a one-row-tile product with
+    // many columns, e.g. the inverse-block leaves of the triangular solves). Not when A and C
+    // overlap: an in-place product (L21 = A21 U11^-1) relies on one workgroup holding all
+    // columns of its rows, which it reads before it writes them.
+    const char* a0 = (const char*)A;
+    const char* a1 = (const char*)(A + (size_t)(k - 1) * lda + m);
+    const char* c0 = (const char*)C;
+    const char* c1 = (const char*)(C + (size_t)(n - 1) * ldc + m);
+    const bool overlap = a0 < c1 && c0 < a1;
+    if (!overlap && (int)(tiles_m * ceil_div(n, GBN)) < std::max(ctx().cu_count, 1)) {
+        const int tiles_n = (int)ceil_div(n, 64);
+        hipLaunchKernelGGL((gemm_nn_kernel<T, 64>), dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, k, alpha,
+                           A, lda, B, ldb, beta, C, ldc, tiles_m);
+    } else {
+        const int tiles_n = (int)ceil_div(n, GBN);
+        hipLaunchKernelGGL((gemm_nn_kernel<T, GBN>), dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, k, alpha,
+                           A, lda, B, ldb, beta, C, ldc, tiles_m);
+    }
     check_launch("gemm");
 }
 
