@@ -78,10 +78,12 @@ struct MfmaOp<float> {
 constexpr int GBM = 128, GBN = 128;
 // k tile of the FP32/FP64 GEMM: 32 (FP32) or 16 (FP64) values, so that a short product
 // (k = 128: the leaves of the factorisation and of the triangular solves) waits on few
-// global-load round trips; both keep the two LDS stages within 68 KiB
+// global-load round trips; both keep the two LDS stages within 70 KiB. Row padding: FP32
+// rows of 34 words put the 16 rows x 2 k of a half-wave's operand read in 32 distinct
+// banks (33-word rows pair them up: measured 0.4 conflicts per LDS cycle)
 template <typename T>
 struct GemmK {
-    static constexpr int KB = sizeof(T) == 4 ? 32 : 16, PAD = KB + 1;
+    static constexpr int KB = sizeof(T) == 4 ? 32 : 16, PAD = sizeof(T) == 4 ? KB + 2 : KB + 1;
 };
 
 // MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16] and B[l/16][l%16]; the four
