@@ -253,21 +253,24 @@ __global__ void __launch_bounds__(256) gemm16_nn_kernel(int m, int n, int k, flo
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
 
-    // A tile 128 x 64: thread reads rows a_i (contiguous along the column), k = a_k0 + 2q
-    const int a_i = tid & 127, a_k0 = tid >> 7;
-    // B tile 64 x 128: 16 threads per column, 4 consecutive k each
+    // A tile 128 x 64: thread holds row a_i (consecutive threads, consecutive rows of a
+    // column) at 8 consecutive k of chunks a_c0 + 2q, stored as one 16-byte LDS write each
+    const int a_i = tid & 127, a_c0 = tid >> 7;
+    // B tile 64 x 128: 16 threads per column, 4 consecutive k each, one 8-byte write each
     const int b_k = (tid & 15) * 4, b_j0 = tid >> 4;
-    float ra[32];
+    float ra[4][8];
     float rb[8][4];
     const float sa = (rs && m0 + a_i < m) ? rs[m0 + a_i] : 1.0f;
 
     auto load = [&](int k0) {
         const int gi = m0 + a_i;
 #pragma unroll
-        for (int q = 0; q < 32; ++q) {
-            int gk = k0 + a_k0 + 2 * q;
-            ra[q] = (gi < m && gk < k) ? A[gi + (size_t)gk * lda] : 0.0f;
-        }
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int gk = k0 + 8 * (a_c0 + 2 * q) + e;
+                ra[q][e] = (gi < m && gk < k) ? A[gi + (size_t)gk * lda] : 0.0f;
+            }
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             int gj = n0 + b_j0 + 16 * q;
@@ -278,15 +281,23 @@ __global__ void __launch_bounds__(256) gemm16_nn_kernel(int m, int n, int k, flo
             }
         }
     };
+    typedef ht h4v __attribute__((ext_vector_type(4)));
     auto store = [&](int buf) {
 #pragma unroll
-        for (int q = 0; q < 32; ++q) As[buf][a_i][a_k0 + 2 * q] = (ht)(ra[q] * sa);
+        for (int q = 0; q < 4; ++q) {
+            v8 v;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = (ht)(ra[q][e] * sa);
+            *(v8*)&As[buf][a_i][8 * (a_c0 + 2 * q)] = v;
+        }
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int j = b_j0 + 16 * q;
             const float sb = (cs && n0 + j < n) ? cs[n0 + j] : 1.0f;
+            h4v v;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) Bs[buf][j][b_k + e] = (ht)(rb[q][e] * sb);
+            for (int e = 0; e < 4; ++e) v[e] = (ht)(rb[q][e] * sb);
+            *(h4v*)&Bs[buf][j][b_k] = v;
         }
     };
 
