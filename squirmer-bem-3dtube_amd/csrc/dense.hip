@@ -19,6 +19,7 @@
 #include <type_traits>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -549,33 +550,74 @@ __global__ void __launch_bounds__(256) gemm16q_kernel(int m, int n, int kp, floa
     }
 }
 
-// Ah[i * kp + k] = A(i, k) * rs[i] (rows >= m and k >= kdim zero), via 64 x 64 LDS tiles
+// Operand conversion of gemm16q_kernel, both operands in one launch (zero beyond m, n, k):
+//   workgroups [0, nrb):  Ah[i * kp + k] = A(i, k) rs[i], one 64 x 64 tile each, transposed
+//                         through LDS; a thread issues its 16 loads together and writes 16
+//                         consecutive k of one row as two 16-byte chunks;
+//   workgroups [nrb, ..): Bh[j * kp + k] = B(k, j) cs[j], 8 consecutive k of one column a
+//                         thread, one 16-byte store (float4 loads where B is 16-byte aligned).
+// kp is a multiple of 64 and (np kp / 8) < 2^32 (checked by the host).
 template <bool BF16>
-__global__ void convert_rows16_kernel(int m, int k, const float* __restrict__ A, int lda, const float* __restrict__ rs,
-                                      typename Half<BF16>::t* __restrict__ Ah, int kp) {
-    __shared__ float tile[64][65];
-    const int i0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 256 threads: 64 x 4
-    for (int c = ty; c < 64; c += 4) {
-        const int i = i0 + tx, kk = k0 + c;
-        tile[c][tx] = (i < m && kk < k) ? A[i + (size_t)kk * lda] * (rs ? rs[i] : 1.0f) : 0.0f;
+__global__ void __launch_bounds__(256) convert16_kernel(int m, int n, int k, const float* __restrict__ A, int lda,
+                                                        const float* __restrict__ rs, const float* __restrict__ B,
+                                                        int ldb, const float* __restrict__ cs,
+                                                        typename Half<BF16>::t* __restrict__ Ah,
+                                                        typename Half<BF16>::t* __restrict__ Bh, int kp, int np,
+                                                        int nrb, int b_vec) {
+    using H = Half<BF16>;
+    using ht = typename H::t;
+    using v8 = typename H::v8;
+    __shared__ float tile[64][65];  // tile[k][i]
+    const int t = threadIdx.x;
+    if ((int)blockIdx.x < nrb) {
+        const int kb = kp / 64;
+        const int i0 = (blockIdx.x / kb) * 64, k0 = (blockIdx.x % kb) * 64;
+        const int tx = t & 63, ty = t >> 6;
+        const int i = i0 + tx;
+        const float sc = rs ? rs[min(i, m - 1)] : 1.0f;
+        const float* Ai = A + min(i, m - 1);
+        float v[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) v[c] = Ai[(size_t)min(k0 + ty + 4 * c, k - 1) * lda];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) tile[ty + 4 * c][tx] = (i < m && k0 + ty + 4 * c < k) ? v[c] * sc : 0.0f;
+        __syncthreads();
+        const int r = t >> 2, q = 16 * (t & 3);
+        v8 o0, o1;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            o0[e] = (ht)tile[q + e][r];
+            o1[e] = (ht)tile[q + 8 + e][r];
+        }
+        v8* dst = (v8*)(Ah + (size_t)(i0 + r) * kp + k0 + q);
+        dst[0] = o0;
+        dst[1] = o1;
+        return;
     }
-    __syncthreads();
-    for (int r = ty; r < 64; r += 4) {
-        const int i = i0 + r, kk = k0 + tx;
-        Ah[(size_t)i * kp + kk] = (typename Half<BF16>::t)tile[tx][r];
+    const unsigned q = (unsigned)kp / 8u;
+    const unsigned c = (blockIdx.x - (unsigned)nrb) * 256u + (unsigned)t;
+    if (c >= q * (unsigned)np) return;
+    const int j = (int)(c / q), k0 = 8 * (int)(c - (unsigned)j * q);
+    const float sc = j < n ? (cs ? cs[j] : 1.0f) : 0.0f;
+    const float* Bj = B + (size_t)min(j, n - 1) * ldb;
+    float v[8];
+    if (b_vec && j < n && k0 + 8 <= k) {
+        const f4v x0 = *(const f4v*)(Bj + k0), x1 = *(const f4v*)(Bj + k0 + 4);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            v[e] = x0[e] * sc;
+            v[4 + e] = x1[e] * sc;
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = Bj[min(k0 + e, k - 1)];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = (j < n && k0 + e < k) ? v[e] * sc : 0.0f;
     }
-}
-// Bh[j * kp + k] = B(k, j) * cs[j] (columns >= n and k >= kdim zero)
-template <bool BF16>
-__global__ void convert_cols16_kernel(int k, int n, const float* __restrict__ B, int ldb, const float* __restrict__ cs,
-                                      typename Half<BF16>::t* __restrict__ Bh, int kp, int np) {
-    const size_t total = (size_t)np * kp;
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        const int kk = (int)(e % kp), j = (int)(e / kp);
-        const float v = (j < n && kk < k) ? B[kk + (size_t)j * ldb] * (cs ? cs[j] : 1.0f) : 0.0f;
-        Bh[e] = (typename Half<BF16>::t)v;
-    }
+    v8 o;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) o[e] = (ht)v[e];
+    *(v8*)(Bh + (size_t)j * kp + k0) = o;
 }
 
 // ------------------------------------------------------------------ host GEMM wrappers
@@ -793,11 +835,12 @@ static void gemm16q(int m, int n, int k, float alpha, const float* A, int lda, c
     if (g_q16.n < need) g_q16.alloc(need);
     ht* Ah = (ht*)g_q16.p;
     ht* Bh = Ah + (size_t)mp * kp;
-    hipLaunchKernelGGL(convert_rows16_kernel<BF16>, dim3(mp / 64, kp / 64), dim3(256), 0, ctx().s(), m, k, A, lda, rs,
-                       Ah, kp);
-    const size_t tb = (size_t)np * kp;
-    hipLaunchKernelGGL(convert_cols16_kernel<BF16>, dim3((unsigned)std::min<size_t>(ceil_div(tb, 256), 1 << 16)),
-                       dim3(256), 0, ctx().s(), k, n, B, ldb, cs, Bh, kp, np);
+    const size_t nrb = (size_t)(mp / 64) * (kp / 64), chunks = (size_t)np * kp / 8;
+    if (chunks >= ((size_t)1 << 32) || nrb + ceil_div(chunks, 256) >= ((size_t)1 << 31))
+        throw std::runtime_error("gemm16: operand too large for the 16-bit conversion");
+    const int b_vec = ((uintptr_t)B % 16 == 0 && ldb % 4 == 0) ? 1 : 0;
+    hipLaunchKernelGGL(convert16_kernel<BF16>, dim3((unsigned)(nrb + ceil_div(chunks, 256))), dim3(256), 0,
+                       ctx().s(), m, n, k, A, lda, rs, B, ldb, cs, Ah, Bh, kp, np, (int)nrb, b_vec);
     check_launch("convert16");
     const int tiles_m = mp / TM, tiles_n = np / TN;
     const double fl = 2.0 * m * n * k, by = 2.0 * ((double)m * k + (double)k * n) + 4.0 * (beta != 0.0f ? 2.0 : 1.0) * m * n;
