@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <vector>
 
@@ -1651,54 +1652,61 @@ static void trsm_inv(int kind, int n, int nrhs, const T* L, int ldl, const T* in
 // the matrix is restored from its copy and factored by the pivoting path.
 //
 // lu128_kernel factors one <= 128 x 128 diagonal block without pivoting and forms the
-// inverses of its unit-lower L and upper U, in one workgroup of four waves, in FP64. The
-// block lives in LDS (row-major, 132 KiB) for the whole kernel. The factor runs in four
+// inverses of its unit-lower L and upper U, in one workgroup of four waves, in the block's
+// own precision (FP64 blocks in FP64; FP32 blocks, whose trailing updates are FP32 or 16-bit
+// products, in FP32 like LAPACK's sgetrf). The block lives in LDS (row-major, 132 KiB in
+// FP64) for the whole kernel. The factor runs in four
 // 32-column panels. A panel is factored without barriers: every wave holds the panel's top
 // 32 rows (lanes 0-31, the same rows in every wave) and 32 of the rows below it (lanes
 // 32-63), one row per lane in registers, so the pivot row is always in its own wave and is
 // read out of lane j (v_readlane). Then U12 = L11^-1 A12 (one column per thread) and the
 // trailing update A22 -= L21 U12 (register tiles). The inverses follow in LAPACK trtri
-// order in 32 x 32 blocks: the eight diagonal blocks by substitution (wave b, lanes 0-31
-// L's block b, lanes 32-63 U's block b, U taken in reversed order so that both are lower
-// triangular), then each block column from two triangular products. Every loop whose
+// order in 32 x 32 blocks: the eight diagonal blocks by substitution (FP32: waves 0-1 the
+// four L blocks, waves 2-3 the four U blocks; FP64: wave b, lanes 0-31 L's block b, lanes
+// 32-63 U's block b, U taken in reversed order so that both are lower triangular; the same
+// operations in the same order either way), then each block column from two triangular
+// products. Every loop whose
 // bounds depend on a panel or block index is unrolled with compile-time indices, so the
 // register arrays stay in VGPRs and the loads of a step are independent of its sums.
 // The padded part of a smaller block is the identity.
 constexpr int NB = 128;
 // the in-place GEMMs (L21 = A21 U11^-1, X = inv B) rely on one tile spanning the block
 static_assert(GBM >= NB && GBN >= NB, "GEMM tile narrower than the inverted blocks");
-constexpr int LU_LD = NB + 1;  // LDS row stride (doubles)
+constexpr int LU_LD = NB + 1;  // LDS row stride (elements)
 constexpr int LU_THREADS = 256;
 
-__device__ __forceinline__ double readlane_d(double v, int lane) {
+__device__ __forceinline__ double readlane_c(double v, int lane) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
     return __hiloint2double(hi, lo);
 }
+__device__ __forceinline__ float readlane_c(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
 
 // Panel P (columns 32P .. 32P+31, rows 32P ..) of the factor. Waves 0 .. NW-1 take part.
-template <int P>
-__device__ __forceinline__ void lu128_panel(double* M, int t, int& mybad) {
+template <int P, typename C>
+__device__ __forceinline__ void lu128_panel(C* M, int t, int& mybad) {
     constexpr int p0 = 32 * P, NREST = 96 - 32 * P, NW = NREST >= 32 ? NREST / 32 : 1;
     const int lane = t & 63, wv = t >> 6;
     if (wv >= NW) return;
     const bool top = lane < 32;
     const bool valid = top || NREST > 0;
     const int row = valid ? (top ? p0 + lane : p0 + 32 + 32 * wv + (lane - 32)) : p0;
-    double* Mr = M + row * LU_LD + p0;
-    double a[32];
+    C* Mr = M + row * LU_LD + p0;
+    C a[32];
 #pragma unroll
     for (int c = 0; c < 32; ++c) a[c] = Mr[c];
 #pragma unroll
     for (int j = 0; j < 32; ++j) {
-        const double piv = readlane_d(a[j], j);  // lane j holds row p0 + j in every wave
-        if (lane == 0 && !(piv != 0.0 && fabs(piv) <= 1.79e308)) mybad = 1;
+        const C piv = readlane_c(a[j], j);  // lane j holds row p0 + j in every wave
+        if (lane == 0 && !(piv != C(0) && fabs(piv) <= std::numeric_limits<C>::max())) mybad = 1;
         const bool below = valid && (!top || lane > j);
-        const double lm = below ? a[j] * (1.0 / piv) : 0.0;
-        if (below && !(fabs(lm) <= 1.0)) mybad = 1;
+        const C lm = below ? a[j] * (C(1) / piv) : C(0);
+        if (below && !(fabs(lm) <= C(1))) mybad = 1;
         a[j] = below ? lm : a[j];
 #pragma unroll
-        for (int c = j + 1; c < 32; ++c) a[c] = fma(-lm, readlane_d(a[c], j), a[c]);  // lm = 0 above
+        for (int c = j + 1; c < 32; ++c) a[c] = fma(-lm, readlane_c(a[c], j), a[c]);  // lm = 0 above
     }
     if (valid && (wv == 0 || !top)) {
 #pragma unroll
@@ -1707,36 +1715,40 @@ __device__ __forceinline__ void lu128_panel(double* M, int t, int& mybad) {
 }
 
 // U12 = L11^-1 A12 for panel P: rows 32P .. 32P+31, one column right of the panel per thread.
-template <int P>
-__device__ __forceinline__ void lu128_u12(double* M, int t) {
+// Column-oriented: step q reads column q of L11 at once (independent loads) and updates every
+// row below it, so row i still takes its updates in the order q = 0 .. i-1.
+template <int P, typename C>
+__device__ __forceinline__ void lu128_u12(C* M, int t) {
     constexpr int p0 = 32 * P, NC = 96 - 32 * P;
     if (t >= NC) return;
     const int col = p0 + 32 + t;
-    double x[32];
+    C x[32];
 #pragma unroll
     for (int i = 0; i < 32; ++i) x[i] = M[(p0 + i) * LU_LD + col];
 #pragma unroll
-    for (int i = 1; i < 32; ++i)
+    for (int q = 0; q < 31; ++q) {
 #pragma unroll
-        for (int q = 0; q < i; ++q) x[i] = fma(-M[(p0 + i) * LU_LD + p0 + q], x[q], x[i]);
+        for (int i = 0; i < 32; ++i)
+            if (i > q) x[i] = fma(-M[(p0 + i) * LU_LD + p0 + q], x[q], x[i]);
+    }
 #pragma unroll
     for (int i = 1; i < 32; ++i) M[(p0 + i) * LU_LD + col] = x[i];
 }
 
 // A22 -= L21 U12 after panel P: rows and columns 32P+32 .., inner dimension 32. Thread
 // (ty, tx) of a 16 x 16 grid owns rows ty + 16 ii and columns tx + 16 jj.
-template <int P>
-__device__ __forceinline__ void lu128_update(double* M, int t) {
+template <int P, typename C>
+__device__ __forceinline__ void lu128_update(C* M, int t) {
     constexpr int p0 = 32 * P, r0 = p0 + 32, R = 96 - 32 * P, MT = R / 16;
     const int ty = t >> 4, tx = t & 15;
-    double acc[MT][MT];
+    C acc[MT][MT];
 #pragma unroll
     for (int ii = 0; ii < MT; ++ii)
 #pragma unroll
         for (int jj = 0; jj < MT; ++jj) acc[ii][jj] = M[(r0 + ty + 16 * ii) * LU_LD + r0 + tx + 16 * jj];
 #pragma unroll 4
     for (int k = 0; k < 32; ++k) {
-        double a[MT], b[MT];
+        C a[MT], b[MT];
 #pragma unroll
         for (int ii = 0; ii < MT; ++ii) a[ii] = M[(r0 + ty + 16 * ii) * LU_LD + p0 + k];
 #pragma unroll
@@ -1757,21 +1769,21 @@ __device__ __forceinline__ void lu128_update(double* M, int t) {
 // ty + 32 ii and columns tx + 8 jj.
 //   L: Tmp = X(r0.., r0..) L(r0.., cJ), X unit lower;  X(r0.., cJ) = -Tmp X(J, J), r0 = cJ + 32
 //   U: Tmp = X(0..R, 0..R) U(0..R, cJ), X upper;       X(0..R, cJ)  = -Tmp X(J, J), R = cJ
-template <bool LOWER, int J>
-__device__ __forceinline__ void lu128_inv_column(double* M, double* Tmp, int t) {
+template <bool LOWER, int J, typename C>
+__device__ __forceinline__ void lu128_inv_column(C* M, C* Tmp, int t) {
     constexpr int cj = 32 * J, R = LOWER ? 96 - cj : cj, RB = R / 32, r0 = LOWER ? cj + 32 : 0;
     const int ty = t >> 3, tx = t & 7;
-    double acc[RB][4];
+    C acc[RB][4];
 #pragma unroll
     for (int ii = 0; ii < RB; ++ii)
 #pragma unroll
-        for (int jj = 0; jj < 4; ++jj) acc[ii][jj] = 0.0;
+        for (int jj = 0; jj < 4; ++jj) acc[ii][jj] = C(0);
 #pragma unroll
     for (int kk = 0; kk < RB; ++kk) {
 #pragma unroll 8
         for (int qq = 0; qq < 32; ++qq) {
             const int q = 32 * kk + qq;  // index within the R x R triangle
-            double b[4];
+            C b[4];
 #pragma unroll
             for (int jj = 0; jj < 4; ++jj) b[jj] = M[(r0 + q) * LU_LD + cj + tx + 8 * jj];
 #pragma unroll
@@ -1779,9 +1791,9 @@ __device__ __forceinline__ void lu128_inv_column(double* M, double* Tmp, int t) 
                 // X(i, q), i = ty + 32 ii: only chunks kk <= ii (L) or kk >= ii (U) are nonzero;
                 // in the diagonal chunk the triangle is applied by a select
                 if (LOWER ? kk > ii : kk < ii) continue;
-                const double m = M[(r0 + ty + 32 * ii) * LU_LD + r0 + q];
-                double a = m;
-                if (kk == ii) a = LOWER ? (qq < ty ? m : (qq == ty ? 1.0 : 0.0)) : (qq >= ty ? m : 0.0);
+                const C m = M[(r0 + ty + 32 * ii) * LU_LD + r0 + q];
+                C a = m;
+                if (kk == ii) a = LOWER ? (qq < ty ? m : (qq == ty ? C(1) : C(0))) : (qq >= ty ? m : C(0));
 #pragma unroll
                 for (int jj = 0; jj < 4; ++jj) acc[ii][jj] = fma(a, b[jj], acc[ii][jj]);
             }
@@ -1795,10 +1807,10 @@ __device__ __forceinline__ void lu128_inv_column(double* M, double* Tmp, int t) 
 #pragma unroll
     for (int ii = 0; ii < RB; ++ii)
 #pragma unroll
-        for (int jj = 0; jj < 4; ++jj) acc[ii][jj] = 0.0;
+        for (int jj = 0; jj < 4; ++jj) acc[ii][jj] = C(0);
 #pragma unroll
     for (int q = 0; q < 32; ++q) {
-        double tq[RB];
+        C tq[RB];
 #pragma unroll
         for (int ii = 0; ii < RB; ++ii) tq[ii] = Tmp[(ty + 32 * ii) * 33 + q];
 #pragma unroll
@@ -1806,8 +1818,8 @@ __device__ __forceinline__ void lu128_inv_column(double* M, double* Tmp, int t) 
             // X(J, J)(q, c), c = tx + 8 jj: unit lower needs q >= c, upper q <= c
             if (LOWER ? q < 8 * jj : q > 8 * jj + 7) continue;
             const int c = tx + 8 * jj;
-            const double m = M[(cj + q) * LU_LD + cj + c];
-            const double x = LOWER ? (q > c ? m : (q == c ? 1.0 : 0.0)) : (q <= c ? m : 0.0);
+            const C m = M[(cj + q) * LU_LD + cj + c];
+            const C x = LOWER ? (q > c ? m : (q == c ? C(1) : C(0))) : (q <= c ? m : C(0));
 #pragma unroll
             for (int ii = 0; ii < RB; ++ii) acc[ii][jj] = fma(tq[ii], x, acc[ii][jj]);
         }
@@ -1824,8 +1836,8 @@ __device__ __forceinline__ void lu128_inv_column(double* M, double* Tmp, int t) 
 // block b (A advanced by b NB (lda + 1), inverses by b NB^2).
 template <typename T>
 __device__ __forceinline__ void lu128_block(T* A, int lda, int n, T* Linv, T* Uinv, int* flag, int factor, int b) {
-    __shared__ double M[NB * LU_LD];  // the block, row-major: M[i * LU_LD + j] (132 KiB)
-    __shared__ double Tmp[96 * 33];   // products of the inversion (25 KiB)
+    __shared__ T M[NB * LU_LD];  // the block, row-major: M[i * LU_LD + j] (132 KiB in FP64)
+    __shared__ T Tmp[96 * 33];   // products of the inversion (25 KiB in FP64)
     __shared__ int bad;
     A += (size_t)b * NB * ((size_t)lda + 1);
     Linv += (size_t)b * NB * NB;
@@ -1846,56 +1858,94 @@ __device__ __forceinline__ void lu128_block(T* A, int lda, int n, T* Linv, T* Ui
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
             const int k = rk + 2 * (m0 + m);
-            M[ri * LU_LD + k] = (ri < w && k < w) ? (double)v[m] : (ri == k ? 1.0 : 0.0);
+            M[ri * LU_LD + k] = (ri < w && k < w) ? v[m] : (ri == k ? T(1) : T(0));
         }
     }
     __syncthreads();
     if (factor) {
         int mybad = 0;
-        lu128_panel<0>(M, t, mybad);
+        lu128_panel<0, T>(M, t, mybad);
         __syncthreads();
-        lu128_u12<0>(M, t);
+        lu128_u12<0, T>(M, t);
         __syncthreads();
-        lu128_update<0>(M, t);
+        lu128_update<0, T>(M, t);
         __syncthreads();
-        lu128_panel<1>(M, t, mybad);
+        lu128_panel<1, T>(M, t, mybad);
         __syncthreads();
-        lu128_u12<1>(M, t);
+        lu128_u12<1, T>(M, t);
         __syncthreads();
-        lu128_update<1>(M, t);
+        lu128_update<1, T>(M, t);
         __syncthreads();
-        lu128_panel<2>(M, t, mybad);
+        lu128_panel<2, T>(M, t, mybad);
         __syncthreads();
-        lu128_u12<2>(M, t);
+        lu128_u12<2, T>(M, t);
         __syncthreads();
-        lu128_update<2>(M, t);
+        lu128_update<2, T>(M, t);
         __syncthreads();
-        lu128_panel<3>(M, t, mybad);
+        lu128_panel<3, T>(M, t, mybad);
         if (mybad) bad = 1;
         __syncthreads();
 #pragma unroll 1
         for (int m0 = 0; m0 < NB / 2; m0 += 16) {
-            double v[16];
+            T v[16];
 #pragma unroll
             for (int m = 0; m < 16; ++m) v[m] = M[ri * LU_LD + rk + 2 * (m0 + m)];
 #pragma unroll
             for (int m = 0; m < 16; ++m) {
                 const int k = rk + 2 * (m0 + m);
-                if (ri < w && k < w) A[ri + (size_t)k * lda] = (T)v[m];
+                if (ri < w && k < w) A[ri + (size_t)k * lda] = v[m];
             }
         }
+        __syncthreads();  // every wave has read L\U before the diagonal blocks are inverted in place
     }
-    {
-        // diagonal blocks in place: lane c of the lower half forms column c of inv(L_bb),
-        // lane c of the upper half column c of inv(Ur) with Ur(i, q) = U_bb(31-i, 31-q)
+    if constexpr (sizeof(T) == 4) {
+        // the four 32 x 32 diagonal blocks in place: waves 0-1 invert L_bb, waves 2-3 U_bb;
+        // lane c of half h of wave w forms column c of block b = 2 (w & 1) + h by substitution
+        // (row i of inv(L) from rows 0 .. i-1, row i of inv(U) from rows 31 .. i+1 and a
+        // division by U(i, i)). Every operand sits at a compile-time offset from the block.
+        const int lane = t & 63, wv = t >> 6, c = lane & 31;
+        T* Bb = M + 32 * (2 * (wv & 1) + (lane >> 5)) * (LU_LD + 1);
+        T x[32];
+        if (wv < 2) {
+#pragma unroll
+            for (int i = 0; i < 32; ++i) {
+                T v = (i == c) ? T(1) : T(0);
+#pragma unroll
+                for (int q = 0; q < 32; ++q)
+                    if (q < i) v = fma(-Bb[i * LU_LD + q], x[q], v);
+                x[i] = v;
+            }
+#pragma unroll
+            for (int i = 1; i < 32; ++i)
+                if (i > c) Bb[i * LU_LD + c] = x[i];
+        } else {
+#pragma unroll
+            for (int ii = 0; ii < 32; ++ii) {
+                const int i = 31 - ii;
+                T v = (i == c) ? T(1) : T(0);
+#pragma unroll
+                for (int qq = 0; qq < 32; ++qq) {
+                    const int q = 31 - qq;
+                    if (q > i) v = fma(-Bb[i * LU_LD + q], x[q], v);
+                }
+                x[i] = v / Bb[i * LU_LD + i];
+            }
+#pragma unroll
+            for (int i = 0; i < 32; ++i)
+                if (i <= c) Bb[i * LU_LD + c] = x[i];
+        }
+    } else {
+        // FP64 (the split above spills there): lane c of the lower half of wave b forms
+        // column c of inv(L_bb), lane c of the upper half column c of inv(Ur) with
+        // Ur(i, q) = U_bb(31-i, 31-q); the same operations in the same order as above
         const int lane = t & 63, o = 32 * (t >> 6), c = lane & 31;
         const bool up = lane >= 32;
         const int base = up ? (o + 31) * LU_LD + o + 31 : o * LU_LD + o;
         const int s = up ? -1 : 1;  // element (i, q) of the lower-triangular operand: base + s (i LU_LD + q)
-        double x[32];
+        T x[32];
 #pragma unroll
         for (int i = 0; i < 32; ++i) {
-            double v = (i == c) ? 1.0 : 0.0;
+            T v = (i == c) ? T(1) : T(0);
 #pragma unroll
             for (int q = 0; q < i; ++q) v = fma(-M[base + s * (i * LU_LD + q)], x[q], v);
             x[i] = up ? v / M[base + s * (i * LU_LD + i)] : v;
@@ -1905,22 +1955,22 @@ __device__ __forceinline__ void lu128_block(T* A, int lda, int n, T* Linv, T* Ui
             if (up ? i >= c : i > c) M[base + s * (i * LU_LD + c)] = x[i];
     }
     __syncthreads();
-    lu128_inv_column<true, 2>(M, Tmp, t);
-    lu128_inv_column<true, 1>(M, Tmp, t);
-    lu128_inv_column<true, 0>(M, Tmp, t);
-    lu128_inv_column<false, 1>(M, Tmp, t);
-    lu128_inv_column<false, 2>(M, Tmp, t);
-    lu128_inv_column<false, 3>(M, Tmp, t);
+    lu128_inv_column<true, 2, T>(M, Tmp, t);
+    lu128_inv_column<true, 1, T>(M, Tmp, t);
+    lu128_inv_column<true, 0, T>(M, Tmp, t);
+    lu128_inv_column<false, 1, T>(M, Tmp, t);
+    lu128_inv_column<false, 2, T>(M, Tmp, t);
+    lu128_inv_column<false, 3, T>(M, Tmp, t);
 #pragma unroll 1
     for (int m0 = 0; m0 < NB / 2; m0 += 16) {  // column-major outputs, consecutive rows a wave
-        double v[16];
+        T v[16];
 #pragma unroll
         for (int m = 0; m < 16; ++m) v[m] = M[ri * LU_LD + rk + 2 * (m0 + m)];
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
             const int c = rk + 2 * (m0 + m);
-            Linv[ri + (size_t)c * NB] = (T)(ri > c ? v[m] : (ri == c ? 1.0 : 0.0));
-            Uinv[ri + (size_t)c * NB] = (T)(ri <= c ? v[m] : 0.0);
+            Linv[ri + (size_t)c * NB] = ri > c ? v[m] : (ri == c ? T(1) : T(0));
+            Uinv[ri + (size_t)c * NB] = ri <= c ? v[m] : T(0);
         }
     }
     if (factor && t == 0 && bad) atomicOr(flag, 1);
