@@ -406,6 +406,33 @@ __global__ void pow2_scale_cols(int k, int n, const float* __restrict__ B, int l
 // B[8(l/32)+0..7][l%32]; accumulator value r of lane l is D[8(r/4) + 4(l/32) + r%4][l%32].
 constexpr int QBK = 64, QPAD = QBK + 8;
 
+// Instruction order of one 16-deep k step of gemm16q_kernel: each of its NM MFMAs is followed
+// by one of the step's memory operations, the next step's fragment reads (NR) first, then this
+// tile's share of the next stage's LDS writes (NW) and of the following tile's global loads
+// (NG); what is left over follows the last MFMA. Only the order changes, never the values.
+template <int NM, int NR, int NW, int NG>
+__device__ __forceinline__ void mfma_interleave() {
+    if constexpr (NM > 0) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+        if constexpr (NR > 0) {
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // one LDS read
+            mfma_interleave<NM - 1, NR - 1, NW, NG>();
+        } else if constexpr (NW > 0) {
+            __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // one LDS write
+            mfma_interleave<NM - 1, 0, NW - 1, NG>();
+        } else if constexpr (NG > 0) {
+            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // one global load
+            mfma_interleave<NM - 1, 0, 0, NG - 1>();
+        } else {
+            mfma_interleave<NM - 1, 0, 0, 0>();
+        }
+    } else {
+        if constexpr (NR > 0) __builtin_amdgcn_sched_group_barrier(0x100, NR, 0);
+        if constexpr (NW > 0) __builtin_amdgcn_sched_group_barrier(0x200, NW, 0);
+        if constexpr (NG > 0) __builtin_amdgcn_sched_group_barrier(0x020, NG, 0);
+    }
+}
+
 // Workgroup tile TM x TN (256 x 256 for large products, 128 x 128 for mid-sized ones), four
 // waves of TM/2 x TN/2, each a grid of 32 x 32 MFMA tiles. PF register sets stage the global
 // loads: with PF = 1 a k tile is loaded one tile before it is stored into LDS, with PF = 2
@@ -469,55 +496,97 @@ __global__ void __launch_bounds__(256) gemm16q_kernel(int m, int n, int kp, floa
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
     const int nk = kp / QBK;
     const int lr = lane & 31, lk = lane >> 5;
-    auto compute = [&](int st) {
+    // k tile kt: its four 16-deep steps each read the next step's fragments (two fragment
+    // sets), write a quarter of tile kt + 1 into the other LDS stage (last read before the
+    // previous barrier) and load a quarter of tile kt + 1 + PF into the register set just
+    // written out (tile t waits in set t % PF); mfma_interleave places one of these operations
+    // after each MFMA.
+    constexpr int QA = CA / 4, QB = CB / 4;
+    auto step = [&](auto store_tag, auto load_tag, auto set_tag, int kt) {
+        constexpr bool ST = decltype(store_tag)::value, LD = decltype(load_tag)::value;
+        constexpr int SET = decltype(set_tag)::value;
+        u4* ra_ = SET == 0 ? ra : ra1;
+        u4* rb_ = SET == 0 ? rb : rb1;
+        const int st = kt & 1;
+        v8 af[2][IM], bf[2][JN];
+        auto frags = [&](int s2, v8* a_, v8* b_) {
 #pragma unroll
-        for (int s2 = 0; s2 < QBK / 16; ++s2) {
-            v8 af[IM], bf[JN];
+            for (int i = 0; i < IM; ++i) a_[i] = *(const v8*)&As_[st][wm + i * 32 + lr][16 * s2 + 8 * lk];
 #pragma unroll
-            for (int i = 0; i < IM; ++i) af[i] = *(const v8*)&As_[st][wm + i * 32 + lr][16 * s2 + 8 * lk];
+            for (int j = 0; j < JN; ++j) b_[j] = *(const v8*)&Bs_[st][wn + j * 32 + lr][16 * s2 + 8 * lk];
+        };
+        frags(0, af[0], bf[0]);
+        auto sub = [&](auto s2_tag) {
+            constexpr int s2 = decltype(s2_tag)::value;
+            if constexpr (s2 < 3) frags(s2 + 1, af[(s2 + 1) & 1], bf[(s2 + 1) & 1]);
+            if constexpr (ST) {
 #pragma unroll
-            for (int j = 0; j < JN; ++j) bf[j] = *(const v8*)&Bs_[st][wn + j * 32 + lr][16 * s2 + 8 * lk];
+                for (int qq = 0; qq < QA; ++qq) {
+                    const int q = s2 * QA + qq, c = tid + 256 * q, row = c >> 3, kc = c & 7;
+                    *(u4*)&As_[st ^ 1][row][8 * kc] = ra_[q];
+                }
+#pragma unroll
+                for (int qq = 0; qq < QB; ++qq) {
+                    const int q = s2 * QB + qq, c = tid + 256 * q, row = c >> 3, kc = c & 7;
+                    *(u4*)&Bs_[st ^ 1][row][8 * kc] = rb_[q];
+                }
+            }
+            if constexpr (LD) {
+                const int k0 = (kt + 1 + PF) * QBK;
+#pragma unroll
+                for (int qq = 0; qq < QA; ++qq) {
+                    const int q = s2 * QA + qq, c = tid + 256 * q, row = c >> 3, kc = c & 7;
+                    ra_[q] = *(const u4*)(Ah + (size_t)(m0 + row) * kp + k0 + 8 * kc);
+                }
+#pragma unroll
+                for (int qq = 0; qq < QB; ++qq) {
+                    const int q = s2 * QB + qq, c = tid + 256 * q, row = c >> 3, kc = c & 7;
+                    rb_[q] = *(const u4*)(Bh + (size_t)(n0 + row) * kp + k0 + 8 * kc);
+                }
+            }
 #pragma unroll
             for (int i = 0; i < IM; ++i)
 #pragma unroll
-                for (int j = 0; j < JN; ++j) acc[i][j] = H::mma32(af[i], bf[j], acc[i][j]);
-        }
+                for (int j = 0; j < JN; ++j) acc[i][j] = H::mma32(af[s2 & 1][i], bf[s2 & 1][j], acc[i][j]);
+            mfma_interleave<IM * JN, s2 < 3 ? IM + JN : 0, ST ? QA + QB : 0, LD ? QA + QB : 0>();
+        };
+        sub(std::integral_constant<int, 0>{});
+        sub(std::integral_constant<int, 1>{});
+        sub(std::integral_constant<int, 2>{});
+        sub(std::integral_constant<int, 3>{});
+        __syncthreads();
     };
+    using T_ = std::true_type;
+    using F_ = std::false_type;
+    using S0 = std::integral_constant<int, 0>;
+    using S1 = std::integral_constant<int, PF == 2 ? 1 : 0>;
+    load_into(0, ra, rb);
+    store_from(0, ra, rb);
+    if (nk > 1) load_into(QBK, PF == 2 ? ra1 : ra, PF == 2 ? rb1 : rb);
+    if (PF == 2 && nk > 2) load_into(2 * QBK, ra, rb);
+    __syncthreads();
+    int kt = 0;
     if constexpr (PF == 1) {
-        load_into(0, ra, rb);
-        store_from(0, ra, rb);
-        if (nk > 1) load_into(QBK, ra, rb);
-        __syncthreads();
-        for (int kt = 0; kt < nk; ++kt) {
-            compute(kt & 1);
-            if (kt + 1 < nk) {
-                store_from((kt + 1) & 1, ra, rb);  // the other stage: last read before the previous barrier
-                if (kt + 2 < nk) load_into((kt + 2) * QBK, ra, rb);
-            }
-            __syncthreads();
+        for (; kt + 2 < nk; ++kt) step(T_{}, T_{}, S0{}, kt);
+        if (kt + 1 < nk) {
+            step(T_{}, F_{}, S0{}, kt);
+            ++kt;
         }
+        step(F_{}, F_{}, S0{}, kt);
     } else {
-        // tile t waits in register set t & 1 and is computed from LDS stage t & 1
-        load_into(0, ra, rb);
-        store_from(0, ra, rb);
-        if (nk > 1) load_into(QBK, ra1, rb1);
-        if (nk > 2) load_into(2 * QBK, ra, rb);
-        __syncthreads();
-        for (int kt = 0; kt < nk; kt += 2) {
-            compute(0);
-            if (kt + 1 < nk) {
-                store_from(1, ra1, rb1);
-                if (kt + 3 < nk) load_into((kt + 3) * QBK, ra1, rb1);
-            }
-            __syncthreads();
-            if (kt + 1 >= nk) break;
-            compute(1);
-            if (kt + 2 < nk) {
-                store_from(0, ra, rb);
-                if (kt + 4 < nk) load_into((kt + 4) * QBK, ra, rb);
-            }
-            __syncthreads();
+        for (; kt + 4 < nk; kt += 2) {  // tiles kt + 1 (set 1) and kt + 2 (set 0) stored
+            step(T_{}, T_{}, S1{}, kt);
+            step(T_{}, T_{}, S0{}, kt + 1);
         }
+        if (kt + 3 < nk) {  // kt even: the last load, tile kt + 3, into set 0 after storing set 1
+            step(T_{}, T_{}, S1{}, kt);
+            ++kt;
+        }
+        for (; kt + 1 < nk; ++kt) {  // stores only
+            if ((kt + 1) & 1) step(T_{}, F_{}, S1{}, kt);
+            else step(T_{}, F_{}, S0{}, kt);
+        }
+        step(F_{}, F_{}, S0{}, kt);
     }
     // epilogue, one 32-row band of tiles at a time: scales and C values are read together
     // from clamped addresses, then the results stored under the bounds test
