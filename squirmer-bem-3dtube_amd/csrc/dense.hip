@@ -100,8 +100,10 @@ __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const
     using Acc = typename Op::acc;
     constexpr int KB = GemmK<T>::KB, PAD = GemmK<T>::PAD, NQ = KB / 2, JS = 256 / KB, NQB = TN * KB / 256;
     constexpr int JT = TN / 32;  // 16-column MFMA tiles per wave
-    __shared__ T As[2][GBM][PAD];
-    __shared__ T Bs[2][TN][PAD];
+    // one LDS block: the two operand stages during the k loop, the C tile in the epilogue
+    __shared__ __attribute__((aligned(16))) T smem[2 * GBM * PAD + 2 * TN * PAD];
+    T(*As)[GBM][PAD] = reinterpret_cast<T(*)[GBM][PAD]>(smem);
+    T(*Bs)[TN][PAD] = reinterpret_cast<T(*)[TN][PAD]>(smem + 2 * GBM * PAD);
 
     const int m0 = tm * GBM, n0 = tn * TN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -164,22 +166,45 @@ __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const
         __syncthreads();
     }
 
+    // epilogue through LDS (the operand stages are free once the k loop has ended): the
+    // accumulators are written into a column-major tile of CW columns at a time, then every
+    // thread updates C down columns, consecutive rows on consecutive lanes, with all of its
+    // C values read before any result is stored
+    constexpr int LDC = GBM + 4, CW = (int)(sizeof(smem) / (LDC * sizeof(T))) >= TN ? TN : TN / 2;
+    static_assert(CW * LDC * sizeof(T) <= sizeof(smem), "epilogue tile exceeds the LDS block");
+    T* Ct = smem;
+    const int er = tid & (GBM - 1), ec = tid >> 7;  // row, first column (then every second)
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int c0 = 0; c0 < TN; c0 += CW) {
+        // the LDS may still be read by the k loop's last step (c0 = 0) or by the previous
+        // chunk's readers
+        __syncthreads();
 #pragma unroll
         for (int j = 0; j < JT; ++j) {
-            const int col = n0 + wn + j * 16 + lr;
-            if (col >= n) continue;
+            const int cl = wn + j * 16 + lr - c0;
+            if (cl < 0 || cl >= CW) continue;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wm + i * 16 + Op::row(lk, r);
-                if (row < m) {
-                    T* c = C + row + (size_t)col * ldc;
-                    T v = alpha * acc[i][j][r];
-                    *c = beta == T(0) ? v : v + beta * *c;
-                }
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) Ct[cl * LDC + wm + i * 16 + Op::row(lk, r)] = acc[i][j][r];
+        }
+        __syncthreads();
+        const int row = m0 + er;
+        T cv[CW / 2];
+        if (beta != T(0)) {
+#pragma unroll
+            for (int q = 0; q < CW / 2; ++q)
+                cv[q] = C[min(row, m - 1) + (size_t)min(n0 + c0 + ec + 2 * q, n - 1) * ldc];
+        }
+#pragma unroll
+        for (int q = 0; q < CW / 2; ++q) {
+            const int col = n0 + c0 + ec + 2 * q;
+            if (row < m && col < n) {
+                const T v = alpha * Ct[(ec + 2 * q) * LDC + er];
+                C[row + (size_t)col * ldc] = beta == T(0) ? v : v + beta * cv[q];
             }
         }
+    }
 }
 
 template <typename T, int TN = GBN>
