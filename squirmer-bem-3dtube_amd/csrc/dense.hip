@@ -358,33 +358,39 @@ __global__ void __launch_bounds__(256) gemm16_nn_kernel(int m, int n, int k, flo
         __syncthreads();
     }
 
-    // epilogue: scales and C values of a row of tiles are read together from clamped
-    // addresses, then the results stored under the bounds test
-    float ucol[4];
+    // epilogue through LDS, 64 columns (the tiles of waves 0-1, then of waves 2-3) at a
+    // time in the A operand stages: column-major tile, then C updated down columns,
+    // consecutive rows on consecutive lanes, the chunk's C values read before any store
+    constexpr int LDC = GBM + 1, CW = 64, NQ = CW / 2;
+    static_assert(CW * LDC * sizeof(float) <= sizeof(As), "epilogue tile exceeds the A stages");
+    float* Ct = reinterpret_cast<float*>(&As[0][0][0]);
+    const int er = tid & (GBM - 1), ec = tid >> 7;
+    const int row = m0 + er;
+    const float ur = rs ? 1.0f / rs[min(row, m - 1)] : 1.0f;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) ucol[j] = cs ? 1.0f / cs[min(n0 + wn + j * 16 + lr, n - 1)] : 1.0f;
+    for (int c0 = 0; c0 < GBN; c0 += CW) {
+        __syncthreads();
+        if (wn == c0) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        float urow[4], cv[4][4];
+            for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) urow[r] = rs ? 1.0f / rs[min(m0 + wm + i * 16 + 4 * lk + r, m - 1)] : 1.0f;
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) Ct[(j * 16 + lr) * LDC + wm + i * 16 + 4 * lk + r] = acc[i][j][r];
+        }
+        __syncthreads();
+        float cv[NQ];
         if (beta != 0.0f) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    cv[j][r] = C[min(m0 + wm + i * 16 + 4 * lk + r, m - 1) + (size_t)min(n0 + wn + j * 16 + lr, n - 1) * ldc];
+            for (int q = 0; q < NQ; ++q) cv[q] = C[min(row, m - 1) + (size_t)min(n0 + c0 + ec + 2 * q, n - 1) * ldc];
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int col = n0 + wn + j * 16 + lr;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wm + i * 16 + 4 * lk + r;
-                if (row < m && col < n) {
-                    const float v = alpha * (acc[i][j][r] * urow[r] * ucol[j]);
-                    C[row + (size_t)col * ldc] = beta == 0.0f ? v : v + beta * cv[j][r];
-                }
+        for (int q = 0; q < NQ; ++q) {
+            const int cl = ec + 2 * q, col = n0 + c0 + cl;
+            if (row < m && col < n) {
+                const float uc = cs ? 1.0f / cs[col] : 1.0f;
+                const float v = alpha * (Ct[cl * LDC + er] * ur * uc);
+                C[row + (size_t)col * ldc] = beta == 0.0f ? v : v + beta * cv[q];
             }
         }
     }
@@ -613,32 +619,45 @@ __global__ void __launch_bounds__(256) gemm16q_kernel(int m, int n, int kp, floa
         }
         step(F_{}, F_{}, S0{}, kt);
     }
-    // epilogue, one 32-row band of tiles at a time: scales and C values are read together
-    // from clamped addresses, then the results stored under the bounds test
-    float ucol[JN];
+    // epilogue through LDS, 64 columns at a time in the A operand stages (free once the k
+    // loop has ended): the chunk's accumulators are written as a column-major tile, then
+    // every thread updates C down columns, consecutive rows on consecutive lanes, with the
+    // chunk's C values read before any result is stored (unscaling in the same order)
+    constexpr int LDC = TM + 1, CW = 64, CS = 256 / TM, NQ = CW / CS, QG = NQ < 32 ? NQ : 32;
+    static_assert(CW * LDC * sizeof(float) <= sizeof(As_), "epilogue tile exceeds the A stages");
+    float* Ct = reinterpret_cast<float*>(&As_[0][0][0]);
+    const int er = tid % TM, ec = tid / TM;  // row; first column, then every CS-th
+    const int row = m0 + er;
+    const float ur = rs ? 1.0f / rs[min(row, m - 1)] : 1.0f;
 #pragma unroll
-    for (int j = 0; j < JN; ++j) ucol[j] = cs ? 1.0f / cs[min(n0 + wn + j * 32 + lr, n - 1)] : 1.0f;
-#pragma unroll
-    for (int i = 0; i < IM; ++i) {
-        float urow[16];
-#pragma unroll
-        for (int r = 0; r < 16; ++r)
-            urow[r] = rs ? 1.0f / rs[min(m0 + wm + i * 32 + 8 * (r >> 2) + 4 * lk + (r & 3), m - 1)] : 1.0f;
+    for (int c0 = 0; c0 < TN; c0 += CW) {
+        __syncthreads();  // the previous chunk's readers are done
 #pragma unroll
         for (int j = 0; j < JN; ++j) {
-            const int col = n0 + wn + j * 32 + lr, colc = min(col, n - 1);
-            float cv[16];
-            if (beta != 0.0f) {
+            const int cl = wn + j * 32 - c0;  // tile j lies wholly inside or outside the chunk
+            if (cl < 0 || cl >= CW) continue;
+#pragma unroll
+            for (int i = 0; i < IM; ++i)
 #pragma unroll
                 for (int r = 0; r < 16; ++r)
-                    cv[r] = C[min(m0 + wm + i * 32 + 8 * (r >> 2) + 4 * lk + (r & 3), m - 1) + (size_t)colc * ldc];
+                    Ct[(cl + lr) * LDC + wm + i * 32 + 8 * (r >> 2) + 4 * lk + (r & 3)] = acc[i][j][r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q0 = 0; q0 < NQ; q0 += QG) {  // QG columns' C values in flight at a time
+            float cv[QG];
+            if (beta != 0.0f) {
+#pragma unroll
+                for (int q = 0; q < QG; ++q)
+                    cv[q] = C[min(row, m - 1) + (size_t)min(n0 + c0 + ec + CS * (q0 + q), n - 1) * ldc];
             }
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int row = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * lk + (r & 3);
+            for (int q = 0; q < QG; ++q) {
+                const int cl = ec + CS * (q0 + q), col = n0 + c0 + cl;
                 if (row < m && col < n) {
-                    const float v = alpha * (acc[i][j][r] * urow[r] * ucol[j]);
-                    C[row + (size_t)col * ldc] = beta == 0.0f ? v : v + beta * cv[r];
+                    const float uc = cs ? 1.0f / cs[col] : 1.0f;
+                    const float v = alpha * (Ct[cl * LDC + er] * ur * uc);
+                    C[row + (size_t)col * ldc] = beta == 0.0f ? v : v + beta * cv[q];
                 }
             }
         }
