@@ -54,6 +54,43 @@ __device__ __forceinline__ void tile_of(int bid, int nblocks, int tiles_m, int& 
     tn = r / gsz;
 }
 
+// Instruction order of one k step of a pipelined GEMM main loop: each of its NM MFMAs is
+// followed by one of the step's memory operations, the next step's fragment reads (NR) first,
+// then the step's share of the next stage's LDS writes (NW) and of a later tile's global
+// loads (NG); what is left over follows the last MFMA. Only the order changes, never the
+// values.
+template <int NM, int NR, int NW, int NG>
+__device__ __forceinline__ void mfma_interleave() {
+    if constexpr (NM > 0) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+        if constexpr (NR > 0) {
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // one LDS read
+            mfma_interleave<NM - 1, NR - 1, NW, NG>();
+        } else if constexpr (NW > 0) {
+            __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // one LDS write
+            mfma_interleave<NM - 1, 0, NW - 1, NG>();
+        } else if constexpr (NG > 0) {
+            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // one global load
+            mfma_interleave<NM - 1, 0, 0, NG - 1>();
+        } else {
+            mfma_interleave<NM - 1, 0, 0, 0>();
+        }
+    } else {
+        if constexpr (NR > 0) __builtin_amdgcn_sched_group_barrier(0x100, NR, 0);
+        if constexpr (NW > 0) __builtin_amdgcn_sched_group_barrier(0x200, NW, 0);
+        if constexpr (NG > 0) __builtin_amdgcn_sched_group_barrier(0x020, NG, 0);
+    }
+}
+
+// f(integral_constant<int, 0>), ..., f(integral_constant<int, N - 1>) in order
+template <int N, int I = 0, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<N, I + 1>(f);
+    }
+}
+
 // ------------------------------------------------------------------ FP64 / FP32 GEMM
 template <typename T>
 struct MfmaOp;
@@ -110,30 +147,22 @@ __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const
     const int wm = (wave & 1) * 64, wn = (wave >> 1) * (TN / 2);
 
     // global -> register staging: A column kk has 128 contiguous rows; B column j has KB
-    // contiguous k values.
+    // contiguous k values. Loads read clamped (always valid) addresses and are masked when
+    // written into LDS, so no instruction waits for a load before the LDS write needs it.
     const int a_i = tid & 127, a_k0 = tid >> 7;     // rows a_i, k = a_k0 + 2q
     const int b_k = tid % KB, b_j0 = tid / KB;      // k = b_k, cols b_j0 + JS q
+    const int gi = m0 + a_i, gic = min(gi, m - 1);
     T ra[NQ], rb[NQB];
-
-    auto load = [&](int k0) {
-        const int gi = m0 + a_i;
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            int gk = k0 + a_k0 + 2 * q;
-            ra[q] = (gi < m && gk < k) ? A[gi + (size_t)gk * lda] : T(0);
-        }
-        const int gk = k0 + b_k;
-#pragma unroll
-        for (int q = 0; q < NQB; ++q) {
-            int gj = n0 + b_j0 + JS * q;
-            rb[q] = (gk < k && gj < n) ? B[gk + (size_t)gj * ldb] : T(0);
-        }
+    auto load_a = [&](int k0, int q) { ra[q] = A[gic + (size_t)min(k0 + a_k0 + 2 * q, k - 1) * lda]; };
+    auto load_b = [&](int k0, int q) {
+        rb[q] = B[min(k0 + b_k, k - 1) + (size_t)min(n0 + b_j0 + JS * q, n - 1) * ldb];
     };
-    auto store = [&](int buf) {
+    auto store = [&](int buf, int k0) {
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) As[buf][a_i][a_k0 + 2 * q] = ra[q];
+        for (int q = 0; q < NQ; ++q) As[buf][a_i][a_k0 + 2 * q] = (gi < m && k0 + a_k0 + 2 * q < k) ? ra[q] : T(0);
 #pragma unroll
-        for (int q = 0; q < NQB; ++q) Bs[buf][b_j0 + JS * q][b_k] = rb[q];
+        for (int q = 0; q < NQB; ++q)
+            Bs[buf][b_j0 + JS * q][b_k] = (k0 + b_k < k && n0 + b_j0 + JS * q < n) ? rb[q] : T(0);
     };
 
     Acc acc[4][JT];
@@ -143,28 +172,51 @@ __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const
         for (int j = 0; j < JT; ++j) acc[i][j] = Acc{0, 0, 0, 0};
 
     const int nk = (k + KB - 1) / KB;
-    load(0);
-    store(0);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) load_a(0, q);
+#pragma unroll
+    for (int q = 0; q < NQB; ++q) load_b(0, q);
+    store(0, 0);
     __syncthreads();
     const int lr = lane & 15, lk = lane >> 4;
-    for (int kt = 0; kt < nk; ++kt) {
-        const int buf = kt & 1;
-        if (kt + 1 < nk) load((kt + 1) * KB);
+    // k tile kt in S steps of 4: each step reads the next step's fragments (two fragment
+    // sets) and issues 1/S of tile kt + 1's global loads; tile kt + 1 goes into the other
+    // LDS stage after the last step
+    constexpr int S = KB / 4, QA = NQ / S, QB = NQB / S;
+    static_assert(QA * S == NQ && QB * S == NQB, "loads do not divide over the k steps");
+    auto step = [&](auto load_tag, int kt) {
+        constexpr bool LD = decltype(load_tag)::value;
+        const int buf = kt & 1, k1 = (kt + 1) * KB;
+        T af[2][4], bf[2][JT];
+        auto frags = [&](int s, T* a_, T* b_) {
 #pragma unroll
-        for (int s = 0; s < KB / 4; ++s) {
-            T af[4], bf[JT];
+            for (int i = 0; i < 4; ++i) a_[i] = As[buf][wm + i * 16 + lr][4 * s + lk];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) af[i] = As[buf][wm + i * 16 + lr][4 * s + lk];
+            for (int j = 0; j < JT; ++j) b_[j] = Bs[buf][wn + j * 16 + lr][4 * s + lk];
+        };
+        frags(0, af[0], bf[0]);
+        auto sub = [&](auto s_tag) {
+            constexpr int s = decltype(s_tag)::value;
+            if constexpr (s + 1 < S) frags(s + 1, af[(s + 1) & 1], bf[(s + 1) & 1]);
+            if constexpr (LD) {
 #pragma unroll
-            for (int j = 0; j < JT; ++j) bf[j] = Bs[buf][wn + j * 16 + lr][4 * s + lk];
+                for (int q = 0; q < QA; ++q) load_a(k1, s * QA + q);
+#pragma unroll
+                for (int q = 0; q < QB; ++q) load_b(k1, s * QB + q);
+            }
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < JT; ++j) acc[i][j] = Op::mma(af[i], bf[j], acc[i][j]);
-        }
-        if (kt + 1 < nk) store(buf ^ 1);
+                for (int j = 0; j < JT; ++j) acc[i][j] = Op::mma(af[s & 1][i], bf[s & 1][j], acc[i][j]);
+            mfma_interleave<4 * JT, s + 1 < S ? 4 + JT : 0, 0, LD ? QA + QB : 0>();
+        };
+        static_for<S>(sub);
+        if constexpr (LD) store(buf ^ 1, k1);
         __syncthreads();
-    }
+    };
+    int kt = 0;
+    for (; kt + 1 < nk; ++kt) step(std::true_type{}, kt);
+    step(std::false_type{}, kt);
 
     // epilogue through LDS (the operand stages are free once the k loop has ended): the
     // accumulators are written into a column-major tile of CW columns at a time, then every
@@ -436,33 +488,6 @@ __global__ void pow2_scale_cols(int k, int n, const float* __restrict__ B, int l
 // MFMA 32x32x16 layout (measured on gfx950): lane l supplies A[l%32][8(l/32)+0..7] and
 // B[8(l/32)+0..7][l%32]; accumulator value r of lane l is D[8(r/4) + 4(l/32) + r%4][l%32].
 constexpr int QBK = 64, QPAD = QBK + 8;
-
-// Instruction order of one 16-deep k step of gemm16q_kernel: each of its NM MFMAs is followed
-// by one of the step's memory operations, the next step's fragment reads (NR) first, then this
-// tile's share of the next stage's LDS writes (NW) and of the following tile's global loads
-// (NG); what is left over follows the last MFMA. Only the order changes, never the values.
-template <int NM, int NR, int NW, int NG>
-__device__ __forceinline__ void mfma_interleave() {
-    if constexpr (NM > 0) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
-        if constexpr (NR > 0) {
-            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // one LDS read
-            mfma_interleave<NM - 1, NR - 1, NW, NG>();
-        } else if constexpr (NW > 0) {
-            __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // one LDS write
-            mfma_interleave<NM - 1, 0, NW - 1, NG>();
-        } else if constexpr (NG > 0) {
-            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // one global load
-            mfma_interleave<NM - 1, 0, 0, NG - 1>();
-        } else {
-            mfma_interleave<NM - 1, 0, 0, 0>();
-        }
-    } else {
-        if constexpr (NR > 0) __builtin_amdgcn_sched_group_barrier(0x100, NR, 0);
-        if constexpr (NW > 0) __builtin_amdgcn_sched_group_barrier(0x200, NW, 0);
-        if constexpr (NG > 0) __builtin_amdgcn_sched_group_barrier(0x020, NG, 0);
-    }
-}
 
 // Workgroup tile TM x TN (256 x 256 for large products, 128 x 128 for mid-sized ones), four
 // waves of TM/2 x TN/2, each a grid of 32 x 32 MFMA tiles. PF register sets stage the global
