@@ -2250,6 +2250,11 @@ __device__ __forceinline__ void trsv_inv_body(int n, int nrhs, const T* __restri
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = 0.0;
     const bool rowok = row < rb;
+    // this block's right-hand sides, read now: only this workgroup writes these rows of B,
+    // so the read leaves the chain's critical path
+    T bown[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) bown[r] = (hf == 0 && r < nrhs && rowok) ? B[(r0 + row) + (size_t)r * ldb] : T(0);
     // tiles of this block row, contiguous in the tiled copy (zero beyond n)
     const T* Lrow = Lt + (size_t)blk * nb * NB * NB + (size_t)(hf * 64) * NB + row;
     unsigned long long seen = base;
@@ -2316,8 +2321,7 @@ __device__ __forceinline__ void trsv_inv_body(int n, int nrhs, const T* __restri
     if (hf == 0) {
 #pragma unroll
         for (int r = 0; r < NR; ++r)
-            rv[r][row] = (r < nrhs && rowok) ? (double)B[(r0 + row) + (size_t)r * ldb] - (part[0][r][row] + part[1][r][row])
-                                             : 0.0;
+            rv[r][row] = (r < nrhs && rowok) ? (double)bown[r] - (part[0][r][row] + part[1][r][row]) : 0.0;
     }
     __syncthreads();
     // x = Dinv r: thread (row, hf) sums over the columns of its half
