@@ -115,10 +115,20 @@ __global__ void gemv_batch_part(const GvJob* __restrict__ jobs, double* __restri
     const T* A2 = (const T*)g.A2;
     const T* x2 = (const T*)g.x2;
     double acc = 0.0;
-    for (int j = ja; j < jb; ++j) {
-        if (j < g.n1) acc += (double)A1[row + (size_t)j * g.lda1] * (double)x1[j];
-        else acc += (double)A2[row + (size_t)(j - g.n1) * g.lda2] * (double)x2[j - g.n1];
-    }
+    // columns j0 .. j1-1 of one matrix in order, eight loads in flight a round
+    auto run = [&](const T* A, long lda, const T* x, int j0, int j1) {
+        int j = j0;
+        for (; j + 8 <= j1; j += 8) {
+            double a[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) a[u] = (double)A[row + (size_t)(j + u) * lda];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc += a[u] * (double)x[j + u];
+        }
+        for (; j < j1; ++j) acc += (double)A[row + (size_t)j * lda] * (double)x[j];
+    };
+    run(A1, g.lda1, x1, ja, min(jb, g.n1));
+    run(A2, g.lda2, x2, max(ja, g.n1) - g.n1, jb - g.n1);
     part[g.poff + (size_t)blockIdx.y * g.m + row] = acc;
 }
 

@@ -157,7 +157,20 @@ __global__ void gemv_part_kernel(int m, int n, int nrhs, const T* __restrict__ A
     int per = (n + GV_CHUNKS - 1) / GV_CHUNKS;
     int ja = chunk * per, jb = min(n, ja + per);
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int j = ja; j < jb; ++j) {
+    int j = ja;
+    // eight columns a round, their loads issued together (HBM needs many bytes in flight);
+    // every row still sums its columns in order
+    for (; j + 8 <= jb; j += 8) {
+        double a[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] = (double)A[row + (size_t)(j + u) * lda];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+                if (r < nrhs) acc[r] += a[u] * (double)X[j + u + (size_t)r * ldx];
+    }
+    for (; j < jb; ++j) {
         const double a = (double)A[row + (size_t)j * lda];
 #pragma unroll
         for (int r = 0; r < 8; ++r)
