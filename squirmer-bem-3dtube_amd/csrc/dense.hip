@@ -1757,6 +1757,33 @@ void gemm_exact<float>(int m, int n, int k, float alpha, const float* A, int lda
     sgemm(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
+// The bottom two levels of trsm_inv in one launch, for 128 < n <= 256 with 128 x 128
+// inverted diagonal blocks: workgroup j takes columns 64 j .. 64 j + 63 of B through that
+// level's three products (lower: X1 = inv1 B1, B2 -= L21 X1, X2 = inv2 B2; upper: the
+// mirror image), each one tile of the workgroup in the storage precision. Every product
+// reads its operands completely before it writes its tile (in place where C is B), and the
+// barrier between two products orders the workgroup's global writes before its next reads.
+constexpr int TRSM2_B = 128, TRSM2_TN = 64;
+template <typename T, bool LOWER>
+__global__ void __launch_bounds__(256) trsm2_kernel(int n, int nrhs, const T* L, int ldl, const T* inv, T* B,
+                                                    int ldb) {
+    const int n1 = TRSM2_B, n2 = n - TRSM2_B, tn = blockIdx.x;
+    const T* inv2 = inv + (size_t)TRSM2_B * TRSM2_B;
+    if (LOWER) {
+        gemm_nn_tile<T, TRSM2_TN>(n1, nrhs, n1, T(1), inv, TRSM2_B, B, ldb, T(0), B, ldb, 0, tn);
+        __syncthreads();
+        gemm_nn_tile<T, TRSM2_TN>(n2, nrhs, n1, T(-1), L + n1, ldl, B, ldb, T(1), B + n1, ldb, 0, tn);
+        __syncthreads();
+        gemm_nn_tile<T, TRSM2_TN>(n2, nrhs, n2, T(1), inv2, TRSM2_B, B + n1, ldb, T(0), B + n1, ldb, 0, tn);
+    } else {
+        gemm_nn_tile<T, TRSM2_TN>(n2, nrhs, n2, T(1), inv2, TRSM2_B, B + n1, ldb, T(0), B + n1, ldb, 0, tn);
+        __syncthreads();
+        gemm_nn_tile<T, TRSM2_TN>(n1, nrhs, n2, T(-1), L + (size_t)n1 * ldl, ldl, B + n1, ldb, T(1), B, ldb, 0, tn);
+        __syncthreads();
+        gemm_nn_tile<T, TRSM2_TN>(n1, nrhs, n1, T(1), inv, TRSM2_B, B, ldb, T(0), B, ldb, 0, tn);
+    }
+}
+
 // Triangular solve through inverted BS x BS diagonal blocks (consecutive in inv, leading
 // dimension BS): recursive split, each leaf one in-place GEMM X = inv_kk B_k in the exact
 // (storage) precision, the off-diagonal updates in the precision of kind.
@@ -1767,6 +1794,15 @@ static void trsm_inv(int kind, int n, int nrhs, const T* L, int ldl, const T* in
         ProfScope prof(PROF_TRSM, 2.0 * n * n * nrhs, sizeof(T) * (2.0 * n * nrhs + (double)n * n));
         gemm_exact<T>(n, nrhs, n, T(1), inv, BS, B, ldb, T(0), B, ldb);
         return;
+    }
+    if constexpr (BS == TRSM2_B) {
+        if (n <= 2 * BS) {  // split_point gives n1 = BS here
+            ProfScope prof(PROF_TRSM, 2.0 * n * n * nrhs, sizeof(T) * (2.0 * n * nrhs + (double)n * n));
+            hipLaunchKernelGGL((trsm2_kernel<T, LOWER>), dim3((unsigned)ceil_div(nrhs, TRSM2_TN)), dim3(256), 0,
+                               ctx().s(), n, nrhs, L, ldl, inv, B, ldb);
+            check_launch("trsm2");
+            return;
+        }
     }
     const int n1 = split_point(n, BS), n2 = n - n1;
     const T* inv2 = inv + (size_t)(n1 / BS) * BS * BS;
