@@ -1784,6 +1784,117 @@ __global__ void __launch_bounds__(256) trsm2_kernel(int n, int nrhs, const T* L,
     }
 }
 
+// FP32 form of trsm2_kernel with the workgroup's 64-column panel of B (n <= 256 rows) held
+// in LDS through the three products: the panel is read once and written once, only the
+// operands inv1, L21 (U12) and inv2 stream through two LDS stages. The products are the
+// same MFMA sums in the same order as gemm_nn_tile's (k tiles of 32, steps of 4), so the
+// results equal trsm2_kernel's.
+template <bool LOWER>
+__global__ void __launch_bounds__(256) trsm2p_kernel(int n, int nrhs, const float* L, int ldl, const float* inv,
+                                                     float* B, int ldb) {
+    using Op = MfmaOp<float>;
+    using Acc = typename Op::acc;
+    constexpr int KB = GemmK<float>::KB, PAD = GemmK<float>::PAD, NQ = KB / 2, S = KB / 4;
+    constexpr int TN = TRSM2_TN, JT = TN / 32, NP = 2 * TRSM2_B + 34;  // panel column stride (34 mod 64 words)
+    __shared__ __attribute__((aligned(16))) float Pn[TN * NP];        // Pn[c * NP + r] = B(r, c)
+    __shared__ __attribute__((aligned(16))) float As[2][GBM][PAD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * (TN / 2);
+    const int lr = lane & 15, lk = lane >> 4;
+    const int n0 = blockIdx.x * TN, n1 = TRSM2_B, n2 = n - TRSM2_B;
+    // the panel: thread t moves row t of every column (zero beyond n and nrhs)
+    {
+        float v[TN];
+#pragma unroll
+        for (int c = 0; c < TN; ++c) v[c] = B[min(tid, n - 1) + (size_t)min(n0 + c, nrhs - 1) * ldb];
+#pragma unroll
+        for (int c = 0; c < TN; ++c) Pn[c * NP + tid] = (tid < n && n0 + c < nrhs) ? v[c] : 0.0f;
+    }
+    __syncthreads();
+    // acc = A(0:mr, 0:kd) P(ko:ko+kd, :), A column-major at lda; A streamed in k tiles of KB
+    const int a_i = tid & 127, a_k0 = tid >> 7;
+    Acc acc[4][JT];
+    auto product = [&](const float* A, int lda, int mr, int kd, int ko) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < JT; ++j) acc[i][j] = Acc{0, 0, 0, 0};
+        const int ai = min(a_i, mr - 1);
+        float ra[NQ];
+        auto load = [&](int k0) {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) ra[q] = A[ai + (size_t)min(k0 + a_k0 + 2 * q, kd - 1) * lda];
+        };
+        auto store = [&](int buf, int k0) {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) As[buf][a_i][a_k0 + 2 * q] = (a_i < mr && k0 + a_k0 + 2 * q < kd) ? ra[q] : 0.0f;
+        };
+        const int nk = (kd + KB - 1) / KB;
+        load(0);
+        store(0, 0);
+        __syncthreads();
+        for (int kt = 0; kt < nk; ++kt) {
+            const int buf = kt & 1;
+            if (kt + 1 < nk) load((kt + 1) * KB);
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                float af[4], bf[JT];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) af[i] = As[buf][wm + i * 16 + lr][4 * s + lk];
+#pragma unroll
+                for (int j = 0; j < JT; ++j) bf[j] = Pn[(wn + j * 16 + lr) * NP + ko + kt * KB + 4 * s + lk];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < JT; ++j) acc[i][j] = Op::mma(af[i], bf[j], acc[i][j]);
+            }
+            if (kt + 1 < nk) store(buf ^ 1, (kt + 1) * KB);
+            __syncthreads();
+        }
+    };
+    // rows ro .. ro + mr - 1 of the panel: = acc (set) or -= acc (sub, as gemm's -acc + c)
+    auto put = [&](int ro, int mr, bool sub) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < JT; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = wm + i * 16 + Op::row(lk, r);
+                    if (row < mr) {
+                        float* p = &Pn[(wn + j * 16 + lr) * NP + ro + row];
+                        *p = sub ? -acc[i][j][r] + *p : acc[i][j][r];
+                    }
+                }
+    };
+    const float* inv2 = inv + (size_t)TRSM2_B * TRSM2_B;
+    if (LOWER) {
+        product(inv, TRSM2_B, n1, n1, 0);  // X1 = inv1 B1 (its reads of rows 0..127 ended at the last barrier)
+        put(0, n1, false);
+        __syncthreads();
+        product(L + n1, ldl, n2, n1, 0);  // B2 -= L21 X1
+        put(n1, n2, true);
+        __syncthreads();
+        product(inv2, TRSM2_B, n2, n2, n1);  // X2 = inv2 B2
+        put(n1, n2, false);
+    } else {
+        product(inv2, TRSM2_B, n2, n2, n1);  // X2 = inv2 B2
+        put(n1, n2, false);
+        __syncthreads();
+        product(L + (size_t)n1 * ldl, ldl, n1, n2, n1);  // B1 -= U12 X2
+        put(0, n1, true);
+        __syncthreads();
+        product(inv, TRSM2_B, n1, n1, 0);  // X1 = inv1 B1
+        put(0, n1, false);
+    }
+    __syncthreads();
+    if (tid < n) {
+#pragma unroll
+        for (int c = 0; c < TN; ++c)
+            if (n0 + c < nrhs) B[tid + (size_t)(n0 + c) * ldb] = Pn[c * NP + tid];
+    }
+}
+
 // Triangular solve through inverted BS x BS diagonal blocks (consecutive in inv, leading
 // dimension BS): recursive split, each leaf one in-place GEMM X = inv_kk B_k in the exact
 // (storage) precision, the off-diagonal updates in the precision of kind.
@@ -1798,8 +1909,12 @@ static void trsm_inv(int kind, int n, int nrhs, const T* L, int ldl, const T* in
     if constexpr (BS == TRSM2_B) {
         if (n <= 2 * BS) {  // split_point gives n1 = BS here
             ProfScope prof(PROF_TRSM, 2.0 * n * n * nrhs, sizeof(T) * (2.0 * n * nrhs + (double)n * n));
-            hipLaunchKernelGGL((trsm2_kernel<T, LOWER>), dim3((unsigned)ceil_div(nrhs, TRSM2_TN)), dim3(256), 0,
-                               ctx().s(), n, nrhs, L, ldl, inv, B, ldb);
+            if constexpr (sizeof(T) == 4)
+                hipLaunchKernelGGL((trsm2p_kernel<LOWER>), dim3((unsigned)ceil_div(nrhs, TRSM2_TN)), dim3(256), 0,
+                                   ctx().s(), n, nrhs, L, ldl, inv, B, ldb);
+            else
+                hipLaunchKernelGGL((trsm2_kernel<T, LOWER>), dim3((unsigned)ceil_div(nrhs, TRSM2_TN)), dim3(256), 0,
+                                   ctx().s(), n, nrhs, L, ldl, inv, B, ldb);
             check_launch("trsm2");
             return;
         }
