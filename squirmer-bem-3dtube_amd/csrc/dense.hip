@@ -2054,27 +2054,28 @@ __device__ __forceinline__ void lu128_update(C* M, int t) {
 }
 
 // Block column J of the inverse of L (S = 2 - J) or of U (S = J - 1), diagonal blocks
-// already inverted; R = 32 (S + 1) rows. Thread (ty, tx) of a 32 x 8 grid owns rows
-// ty + 32 ii and columns tx + 8 jj.
+// already inverted; R = 32 (S + 1) rows. Thread (ty, tx) of a 32 x (NT / 32) grid owns rows
+// ty + 32 ii and columns tx + (NT / 32) jj. Two barriers, whatever NT.
 //   L: Tmp = X(r0.., r0..) L(r0.., cJ), X unit lower;  X(r0.., cJ) = -Tmp X(J, J), r0 = cJ + 32
 //   U: Tmp = X(0..R, 0..R) U(0..R, cJ), X upper;       X(0..R, cJ)  = -Tmp X(J, J), R = cJ
-template <bool LOWER, int J, typename C>
+template <bool LOWER, int J, typename C, int NT = LU_THREADS>
 __device__ __forceinline__ void lu128_inv_column(C* M, C* Tmp, int t) {
     constexpr int cj = 32 * J, R = LOWER ? 96 - cj : cj, RB = R / 32, r0 = LOWER ? cj + 32 : 0;
-    const int ty = t >> 3, tx = t & 7;
-    C acc[RB][4];
+    constexpr int TX = NT / 32, JJ = 32 / TX;  // thread (ty, tx) owns columns tx + TX jj
+    const int ty = t / TX, tx = t % TX;
+    C acc[RB][JJ];
 #pragma unroll
     for (int ii = 0; ii < RB; ++ii)
 #pragma unroll
-        for (int jj = 0; jj < 4; ++jj) acc[ii][jj] = C(0);
+        for (int jj = 0; jj < JJ; ++jj) acc[ii][jj] = C(0);
 #pragma unroll
     for (int kk = 0; kk < RB; ++kk) {
 #pragma unroll 8
         for (int qq = 0; qq < 32; ++qq) {
             const int q = 32 * kk + qq;  // index within the R x R triangle
-            C b[4];
+            C b[JJ];
 #pragma unroll
-            for (int jj = 0; jj < 4; ++jj) b[jj] = M[(r0 + q) * LU_LD + cj + tx + 8 * jj];
+            for (int jj = 0; jj < JJ; ++jj) b[jj] = M[(r0 + q) * LU_LD + cj + tx + TX * jj];
 #pragma unroll
             for (int ii = 0; ii < RB; ++ii) {
                 // X(i, q), i = ty + 32 ii: only chunks kk <= ii (L) or kk >= ii (U) are nonzero;
@@ -2084,29 +2085,29 @@ __device__ __forceinline__ void lu128_inv_column(C* M, C* Tmp, int t) {
                 C a = m;
                 if (kk == ii) a = LOWER ? (qq < ty ? m : (qq == ty ? C(1) : C(0))) : (qq >= ty ? m : C(0));
 #pragma unroll
-                for (int jj = 0; jj < 4; ++jj) acc[ii][jj] = fma(a, b[jj], acc[ii][jj]);
+                for (int jj = 0; jj < JJ; ++jj) acc[ii][jj] = fma(a, b[jj], acc[ii][jj]);
             }
         }
     }
 #pragma unroll
     for (int ii = 0; ii < RB; ++ii)
 #pragma unroll
-        for (int jj = 0; jj < 4; ++jj) Tmp[(ty + 32 * ii) * 33 + tx + 8 * jj] = acc[ii][jj];
+        for (int jj = 0; jj < JJ; ++jj) Tmp[(ty + 32 * ii) * 33 + tx + TX * jj] = acc[ii][jj];
     __syncthreads();
 #pragma unroll
     for (int ii = 0; ii < RB; ++ii)
 #pragma unroll
-        for (int jj = 0; jj < 4; ++jj) acc[ii][jj] = C(0);
+        for (int jj = 0; jj < JJ; ++jj) acc[ii][jj] = C(0);
 #pragma unroll
     for (int q = 0; q < 32; ++q) {
         C tq[RB];
 #pragma unroll
         for (int ii = 0; ii < RB; ++ii) tq[ii] = Tmp[(ty + 32 * ii) * 33 + q];
 #pragma unroll
-        for (int jj = 0; jj < 4; ++jj) {
-            // X(J, J)(q, c), c = tx + 8 jj: unit lower needs q >= c, upper q <= c
-            if (LOWER ? q < 8 * jj : q > 8 * jj + 7) continue;
-            const int c = tx + 8 * jj;
+        for (int jj = 0; jj < JJ; ++jj) {
+            // X(J, J)(q, c), c = tx + TX jj: unit lower needs q >= c, upper q <= c
+            if (LOWER ? q < TX * jj : q > TX * jj + TX - 1) continue;
+            const int c = tx + TX * jj;
             const C m = M[(cj + q) * LU_LD + cj + c];
             const C x = LOWER ? (q > c ? m : (q == c ? C(1) : C(0))) : (q <= c ? m : C(0));
 #pragma unroll
@@ -2116,7 +2117,7 @@ __device__ __forceinline__ void lu128_inv_column(C* M, C* Tmp, int t) {
 #pragma unroll
     for (int ii = 0; ii < RB; ++ii)
 #pragma unroll
-        for (int jj = 0; jj < 4; ++jj) M[(r0 + ty + 32 * ii) * LU_LD + cj + tx + 8 * jj] = -acc[ii][jj];
+        for (int jj = 0; jj < JJ; ++jj) M[(r0 + ty + 32 * ii) * LU_LD + cj + tx + TX * jj] = -acc[ii][jj];
     __syncthreads();
 }
 
@@ -2126,7 +2127,7 @@ __device__ __forceinline__ void lu128_inv_column(C* M, C* Tmp, int t) {
 template <typename T>
 __device__ __forceinline__ void lu128_block(T* A, int lda, int n, T* Linv, T* Uinv, int* flag, int factor, int b) {
     __shared__ T M[NB * LU_LD];  // the block, row-major: M[i * LU_LD + j] (132 KiB in FP64)
-    __shared__ T Tmp[96 * 33];   // products of the inversion (25 KiB in FP64)
+    __shared__ T Tmp[(sizeof(T) == 4 ? 2 : 1) * 96 * 33];  // products of the inversion (one per chain in FP32)
     __shared__ int bad;
     A += (size_t)b * NB * ((size_t)lda + 1);
     Linv += (size_t)b * NB * NB;
@@ -2134,18 +2135,19 @@ __device__ __forceinline__ void lu128_block(T* A, int lda, int n, T* Linv, T* Ui
     const int w = min(NB, n - b * NB);
     const int t = threadIdx.x;
     if (t == 0) bad = 0;
-    // Thread t moves row t % 128 of columns t / 128 + 2 m. Loads are issued 16 at a time
+    // Thread t moves row t % 128 of columns t / 128 + 2 m. Loads are issued 16 (FP32: 32) at a time
     // from clamped (always valid) addresses and masked afterwards: a guarded load would be
     // waited for before the next one is issued.
     const int ri = t & (NB - 1), rk = t >> 7;
     const T* Ai = A + min(ri, w - 1);
+    constexpr int LG = sizeof(T) == 4 ? 32 : 16;  // loads in flight a round
 #pragma unroll 1
-    for (int m0 = 0; m0 < NB / 2; m0 += 16) {
-        T v[16];
+    for (int m0 = 0; m0 < NB / 2; m0 += LG) {
+        T v[LG];
 #pragma unroll
-        for (int m = 0; m < 16; ++m) v[m] = Ai[(size_t)min(rk + 2 * (m0 + m), w - 1) * lda];
+        for (int m = 0; m < LG; ++m) v[m] = Ai[(size_t)min(rk + 2 * (m0 + m), w - 1) * lda];
 #pragma unroll
-        for (int m = 0; m < 16; ++m) {
+        for (int m = 0; m < LG; ++m) {
             const int k = rk + 2 * (m0 + m);
             M[ri * LU_LD + k] = (ri < w && k < w) ? v[m] : (ri == k ? T(1) : T(0));
         }
@@ -2244,12 +2246,26 @@ __device__ __forceinline__ void lu128_block(T* A, int lda, int n, T* Linv, T* Ui
             if (up ? i >= c : i > c) M[base + s * (i * LU_LD + c)] = x[i];
     }
     __syncthreads();
-    lu128_inv_column<true, 2, T>(M, Tmp, t);
-    lu128_inv_column<true, 1, T>(M, Tmp, t);
-    lu128_inv_column<true, 0, T>(M, Tmp, t);
-    lu128_inv_column<false, 1, T>(M, Tmp, t);
-    lu128_inv_column<false, 2, T>(M, Tmp, t);
-    lu128_inv_column<false, 3, T>(M, Tmp, t);
+    if constexpr (sizeof(T) == 4) {
+        // the L and U chains write disjoint parts of M (strictly lower, upper) and read only
+        // their own: waves 0-1 run L's block columns, waves 2-3 U's, two barriers a step each
+        if (t < 128) {
+            lu128_inv_column<true, 2, T, 128>(M, Tmp, t);
+            lu128_inv_column<true, 1, T, 128>(M, Tmp, t);
+            lu128_inv_column<true, 0, T, 128>(M, Tmp, t);
+        } else {
+            lu128_inv_column<false, 1, T, 128>(M, Tmp + 96 * 33, t - 128);
+            lu128_inv_column<false, 2, T, 128>(M, Tmp + 96 * 33, t - 128);
+            lu128_inv_column<false, 3, T, 128>(M, Tmp + 96 * 33, t - 128);
+        }
+    } else {
+        lu128_inv_column<true, 2, T>(M, Tmp, t);
+        lu128_inv_column<true, 1, T>(M, Tmp, t);
+        lu128_inv_column<true, 0, T>(M, Tmp, t);
+        lu128_inv_column<false, 1, T>(M, Tmp, t);
+        lu128_inv_column<false, 2, T>(M, Tmp, t);
+        lu128_inv_column<false, 3, T>(M, Tmp, t);
+    }
 #pragma unroll 1
     for (int m0 = 0; m0 < NB / 2; m0 += 16) {  // column-major outputs, consecutive rows a wave
         T v[16];
