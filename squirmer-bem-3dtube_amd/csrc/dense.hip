@@ -132,7 +132,7 @@ struct GemmK {
 template <typename T, int TN = GBN>
 __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const T* __restrict__ A, int lda,
                                              const T* __restrict__ B, int ldb, T beta, T* __restrict__ C, int ldc,
-                                             int tm, int tn) {
+                                             int tm, int tn, int* unit_flag = nullptr) {
     using Op = MfmaOp<T>;
     using Acc = typename Op::acc;
     constexpr int KB = GemmK<T>::KB, PAD = GemmK<T>::PAD, NQ = KB / 2, JS = 256 / KB, NQB = TN * KB / 256;
@@ -226,6 +226,7 @@ __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const
     static_assert(CW * LDC * sizeof(T) <= sizeof(smem), "epilogue tile exceeds the LDS block");
     T* Ct = smem;
     const int er = tid & (GBM - 1), ec = tid >> 7;  // row, first column (then every second)
+    int bad = 0;
 #pragma unroll
     for (int c0 = 0; c0 < TN; c0 += CW) {
         // the LDS may still be read by the k loop's last step (c0 = 0) or by the previous
@@ -253,19 +254,23 @@ __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const
             const int col = n0 + c0 + ec + 2 * q;
             if (row < m && col < n) {
                 const T v = alpha * Ct[(ec + 2 * q) * LDC + er];
-                C[row + (size_t)col * ldc] = beta == T(0) ? v : v + beta * cv[q];
+                const T out = beta == T(0) ? v : v + beta * cv[q];
+                C[row + (size_t)col * ldc] = out;
+                if (unit_flag && !(fabs((double)out) <= 1.0)) bad = 1;
             }
         }
     }
+    // unit_flag |= (some stored |value| > 1 or not finite): the no-pivot LU's multiplier check
+    if (unit_flag && __syncthreads_or(bad) && tid == 0) atomicOr(unit_flag, 1);
 }
 
 template <typename T, int TN = GBN>
 __global__ void __launch_bounds__(256) gemm_nn_kernel(int m, int n, int k, T alpha, const T* __restrict__ A,
                                                       int lda, const T* __restrict__ B, int ldb, T beta,
-                                                      T* __restrict__ C, int ldc, int tiles_m) {
+                                                      T* __restrict__ C, int ldc, int tiles_m, int* unit_flag) {
     int tm, tn;
     tile_of(blockIdx.x, gridDim.x, tiles_m, tm, tn);
-    gemm_nn_tile<T, TN>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn);
+    gemm_nn_tile<T, TN>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, unit_flag);
 }
 
 // Many independent GEMMs C_j = alpha_j A_j B_j + beta_j C_j in one launch: workgroup
@@ -762,7 +767,7 @@ __global__ void __launch_bounds__(256) convert16_kernel(int m, int n, int k, con
 // ------------------------------------------------------------------ host GEMM wrappers
 template <typename T>
 static void gemm_launch(int m, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb, T beta,
-                        T* C, int ldc) {
+                        T* C, int ldc, int* unit_flag = nullptr) {
     if (m <= 0 || n <= 0) return;
     if (k <= 0 || alpha == T(0)) {
         // C = beta C
@@ -785,11 +790,11 @@ static void gemm_launch(int m, int n, int k, T alpha, const T* A, int lda, const
     if (!overlap && (int)(tiles_m * ceil_div(n, GBN)) < std::max(ctx().cu_count, 1)) {
         const int tiles_n = (int)ceil_div(n, 64);
         hipLaunchKernelGGL((gemm_nn_kernel<T, 64>), dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, k, alpha,
-                           A, lda, B, ldb, beta, C, ldc, tiles_m);
+                           A, lda, B, ldb, beta, C, ldc, tiles_m, unit_flag);
     } else {
         const int tiles_n = (int)ceil_div(n, GBN);
         hipLaunchKernelGGL((gemm_nn_kernel<T, GBN>), dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, k, alpha,
-                           A, lda, B, ldb, beta, C, ldc, tiles_m);
+                           A, lda, B, ldb, beta, C, ldc, tiles_m, unit_flag);
     }
     check_launch("gemm");
 }
@@ -2309,22 +2314,6 @@ static void lu128_launch(int grid, T* A, int lda, int n, T* Linv, T* Uinv, int* 
     check_launch("lu128");
 }
 
-// flag |= (some |x| > 1 or not finite) over an m x n block
-template <typename T>
-__global__ void __launch_bounds__(256) check_unit_bound_kernel(int m, int n, const T* __restrict__ X, int ldx, int* flag) {
-    __shared__ int bad;
-    if (threadIdx.x == 0) bad = 0;
-    __syncthreads();
-    const size_t total = (size_t)m * n;
-    int mine = 0;
-    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
-        const int i = (int)(idx % m), j = (int)(idx / m);
-        if (!(fabs((double)X[i + (size_t)j * ldx]) <= 1.0)) mine = 1;
-    }
-    if (mine) bad = 1;
-    __syncthreads();
-    if (threadIdx.x == 0 && bad) atomicOr(flag, 1);
-}
 
 struct NoPivotWork {
     DBuf<char> inv;     // Linv blocks, then Uinv blocks (NB x NB each, storage type)
@@ -2348,12 +2337,9 @@ static void getrf_np(int kind, T* A, int lda, int m, int w, int base, T* Linv, T
             lu128_launch<T>(1, A, lda, w, Linv + off, Uinv + off, flag, 1);
         }
         if (m > w) {
-            // L21 = A21 U11^-1, in place: one column tile, each row tile read before written
-            gemm_exact<T>(m - w, w, w, T(1), A + w, lda, Uinv + off, NB, T(0), A + w, lda);
-            const size_t tot = (size_t)(m - w) * w;
-            hipLaunchKernelGGL(check_unit_bound_kernel<T>, dim3((unsigned)std::min<size_t>(ceil_div(tot, 256 * 8), 1024)),
-                               dim3(256), 0, ctx().s(), m - w, w, A + w, lda, flag);
-            check_launch("check_unit_bound");
+            // L21 = A21 U11^-1, in place (one column tile, each row tile read before written),
+            // with every multiplier checked against |l| <= 1 as it is stored
+            gemm_launch<T>(m - w, w, w, T(1), A + w, lda, Uinv + off, NB, T(0), A + w, lda, flag);
         }
         return;
     }
