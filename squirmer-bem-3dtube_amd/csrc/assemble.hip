@@ -122,16 +122,18 @@ struct Acc {
 __device__ __forceinline__ void accum_point(Acc& a, double rx, double ry, double rz, double w, double mx, double my,
                                             double mz, bool with_dl) {
     const double r2 = rx * rx + ry * ry + rz * rz;
-    const double rn = sqrt(r2);
-    const double ir = 1.0 / rn;
-    const double ir3 = ir / r2;
+    // 1/|r| by the FP64 reciprocal square root (within an ulp) and its powers by products: a
+    // square root and three divisions per point cost as much as the rest of the point
+    const double ir = rsqrt(r2);
+    const double ir2 = ir * ir;
+    const double ir3 = ir * ir2;
     const double s = w * ir3;
     a.w0 += w * ir;
     const double sx = s * rx, sy = s * ry, sz = s * rz;
     a.s00 += sx * rx; a.s01 += sx * ry; a.s02 += sx * rz;
     a.s11 += sy * ry; a.s12 += sy * rz; a.s22 += sz * rz;
     if (with_dl) {
-        const double dd = -6.0 * w * (rx * mx + ry * my + rz * mz) * ir3 / r2;
+        const double dd = -6.0 * w * (rx * mx + ry * my + rz * mz) * ir3 * ir2;
         const double dx = dd * rx, dy = dd * ry, dz = dd * rz;
         a.d00 += dx * rx; a.d01 += dx * ry; a.d02 += dx * rz;
         a.d11 += dy * ry; a.d12 += dy * rz; a.d22 += dz * rz;
@@ -190,6 +192,7 @@ __device__ __forceinline__ void consume(const AsmArgs& a, const Acc& q, int i, i
                                         double* vel, const double* tn, const SrcTile& t, int s) {
     const double SLm[3][3] = {{q.w0 + q.s00, q.s01, q.s02}, {q.s01, q.w0 + q.s11, q.s12}, {q.s02, q.s12, q.w0 + q.s22}};
     const double DLm[3][3] = {{q.d00, q.d01, q.d02}, {q.d01, q.d11, q.d12}, {q.d02, q.d12, q.d22}};
+    const double isl = 1.0 / a.sl_div, idl = 1.0 / a.dl_div;  // 1/(8 pi mu), 1/(8 pi)
     if (MODE == MODE_BLOCKS) {
         if (a.SL) {
 #pragma unroll
@@ -197,7 +200,7 @@ __device__ __forceinline__ void consume(const AsmArgs& a, const Acc& q, int i, i
                 double* col = a.SL + (size_t)(3 * pcol + b) * a.ld_sl + 3 * (size_t)i;
 #pragma unroll
                 for (int r = 0; r < 3; ++r) {
-                    double v = SLm[r][b] / a.sl_div;
+                    double v = SLm[r][b] * isl;
                     if (a.defl_sc != 0.0) v += a.defl_sc * tn[r] * t.nrm[s][b] * t.area[s];
                     col[r] = v;
                 }
@@ -208,7 +211,7 @@ __device__ __forceinline__ void consume(const AsmArgs& a, const Acc& q, int i, i
             for (int b = 0; b < 3; ++b) {
                 double* col = a.DL + (size_t)(3 * pcol + b) * a.ld_dl + 3 * (size_t)i;
 #pragma unroll
-                for (int r = 0; r < 3; ++r) col[r] = -DLm[r][b] / a.dl_div;
+                for (int r = 0; r < 3; ++r) col[r] = -DLm[r][b] * idl;
             }
         }
         if (dls) {
@@ -224,7 +227,7 @@ __device__ __forceinline__ void consume(const AsmArgs& a, const Acc& q, int i, i
         for (int j = 0; j < 3; ++j) {
             double acc = 0.0;
 #pragma unroll
-            for (int ii = 0; ii < 3; ++ii) acc += SLm[ii][j] * f[ii] / a.sl_div - DLm[ii][j] * u[ii] / a.dl_div;
+            for (int ii = 0; ii < 3; ++ii) acc += SLm[ii][j] * f[ii] * isl - DLm[ii][j] * u[ii] * idl;
             vel[j] += acc;
         }
     }
@@ -417,8 +420,8 @@ __device__ __forceinline__ void dpoint(double* g, double rx, double ry, double r
                                        const double* F, const double* Vv, double mx, double my, double mz,
                                        double a_sl, double a_dl) {
     const double r2 = rx * rx + ry * ry + rz * rz;
-    const double ir = 1.0 / sqrt(r2);
-    const double ir3 = ir / r2, ir5 = ir3 / r2, ir7 = ir5 / r2;
+    const double ir = rsqrt(r2), ir2 = ir * ir;
+    const double ir3 = ir * ir2, ir5 = ir3 * ir2, ir7 = ir5 * ir2;
     const double rL = rx * L[0] + ry * L[1] + rz * L[2];
     const double rF = rx * F[0] + ry * F[1] + rz * F[2];
     const double LF = L[0] * F[0] + L[1] * F[1] + L[2] * F[2];
