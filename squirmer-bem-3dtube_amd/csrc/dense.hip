@@ -1084,20 +1084,28 @@ template void scale_matrix<double>(int, int, double, double*, int);
 template void scale_matrix<float>(int, int, float, float*, int);
 
 template <typename Ti, typename To>
-__global__ void convert_kernel(int m, int n, const Ti* __restrict__ A, int lda, To* __restrict__ B, int ldb) {
-    size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    size_t total = (size_t)m * n;
-    for (; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
-        int i = (int)(idx % m), j = (int)(idx / m);
-        B[i + (size_t)j * ldb] = (To)A[i + (size_t)j * lda];
+__global__ void __launch_bounds__(256) convert_kernel(int m, int n, const Ti* __restrict__ A, int lda,
+                                                      To* __restrict__ B, int ldb) {
+    // workgroup (x, y): rows 1024 x .. 1024 x + 1023 of columns y, y + gridDim.y, ...; a thread
+    // moves rows r, r + 256, r + 512, r + 768 (consecutive rows on consecutive lanes, no
+    // division per element)
+    const int i0 = blockIdx.x * 1024 + threadIdx.x;
+    for (int j = blockIdx.y; j < n; j += gridDim.y) {
+        const Ti* a = A + (size_t)j * lda;
+        To* b = B + (size_t)j * ldb;
+        Ti v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = a[min(i0 + 256 * u, m - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i0 + 256 * u < m) b[i0 + 256 * u] = (To)v[u];
     }
 }
 template <typename Ti, typename To>
 void convert_matrix(int m, int n, const Ti* A, int lda, To* B, int ldb) {
     if (m <= 0 || n <= 0) return;
-    size_t total = (size_t)m * n;
-    unsigned blocks = (unsigned)std::min<size_t>(ceil_div(total, 256), 1 << 20);
-    hipLaunchKernelGGL((convert_kernel<Ti, To>), dim3(blocks), dim3(256), 0, ctx().s(), m, n, A, lda, B, ldb);
+    const dim3 grid((unsigned)ceil_div(m, 1024), (unsigned)std::min(n, 65535));
+    hipLaunchKernelGGL((convert_kernel<Ti, To>), grid, dim3(256), 0, ctx().s(), m, n, A, lda, B, ldb);
     check_launch("convert_matrix");
 }
 template void convert_matrix<double, float>(int, int, const double*, int, float*, int);
