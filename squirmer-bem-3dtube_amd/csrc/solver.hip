@@ -151,32 +151,49 @@ constexpr int GV_CHUNKS = 32;
 template <typename T>
 __global__ void gemv_part_kernel(int m, int n, int nrhs, const T* __restrict__ A, long lda, const T* __restrict__ X,
                                  long ldx, double* __restrict__ part) {
-    int row = blockIdx.x * blockDim.x + threadIdx.x;
+    // rows row0 and row0 + 256 of a 512-row block (a column's 4 KB run per workgroup)
+    const int row0 = blockIdx.x * 512 + threadIdx.x;
     int chunk = blockIdx.y;
-    if (row >= m) return;
+    if (row0 >= m) return;
+    const bool two = row0 + 256 < m;
+    const int row1 = two ? row0 + 256 : row0;
     int per = (n + GV_CHUNKS - 1) / GV_CHUNKS;
     int ja = chunk * per, jb = min(n, ja + per);
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double acc[2][8] = {{0, 0, 0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0, 0, 0}};
     int j = ja;
     // eight columns a round, their loads issued together (HBM needs many bytes in flight);
     // every row still sums its columns in order
     for (; j + 8 <= jb; j += 8) {
-        double a[8];
+        double a0[8], a1[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) a[u] = (double)A[row + (size_t)(j + u) * lda];
+        for (int u = 0; u < 8; ++u) {
+            a0[u] = (double)A[row0 + (size_t)(j + u) * lda];
+            a1[u] = (double)A[row1 + (size_t)(j + u) * lda];
+        }
 #pragma unroll
         for (int u = 0; u < 8; ++u)
 #pragma unroll
             for (int r = 0; r < 8; ++r)
-                if (r < nrhs) acc[r] += a[u] * (double)X[j + u + (size_t)r * ldx];
+                if (r < nrhs) {
+                    const double x = (double)X[j + u + (size_t)r * ldx];
+                    acc[0][r] += a0[u] * x;
+                    acc[1][r] += a1[u] * x;
+                }
     }
     for (; j < jb; ++j) {
-        const double a = (double)A[row + (size_t)j * lda];
+        const double a0 = (double)A[row0 + (size_t)j * lda], a1 = (double)A[row1 + (size_t)j * lda];
 #pragma unroll
         for (int r = 0; r < 8; ++r)
-            if (r < nrhs) acc[r] += a * (double)X[j + (size_t)r * ldx];
+            if (r < nrhs) {
+                const double x = (double)X[j + (size_t)r * ldx];
+                acc[0][r] += a0 * x;
+                acc[1][r] += a1 * x;
+            }
     }
-    for (int r = 0; r < nrhs; ++r) part[((size_t)chunk * nrhs + r) * m + row] = acc[r];
+    for (int r = 0; r < nrhs; ++r) {
+        part[((size_t)chunk * nrhs + r) * m + row0] = acc[0][r];
+        if (two) part[((size_t)chunk * nrhs + r) * m + row1] = acc[1][r];
+    }
 }
 template <typename T>
 __global__ void gemv_reduce_kernel(int m, int nrhs, const double* __restrict__ part, double alpha, double beta, T* Y,
@@ -204,7 +221,7 @@ void gemv(int m, int n, int nrhs, double alpha, const T* A, long lda, const T* X
         return;
     }
     ProfScope prof(PROF_GEMV, 2.0 * m * n * nrhs, sizeof(T) * ((double)m * n + (double)n * nrhs + 2.0 * m * nrhs));
-    hipLaunchKernelGGL(gemv_part_kernel<T>, dim3(ceil_div(m, 256), GV_CHUNKS), dim3(256), 0, ctx().s(), m, n, nrhs, A,
+    hipLaunchKernelGGL(gemv_part_kernel<T>, dim3(ceil_div(m, 512), GV_CHUNKS), dim3(256), 0, ctx().s(), m, n, nrhs, A,
                        lda, X, ldx, g_gv.p);
     hipLaunchKernelGGL(gemv_reduce_kernel<T>, dim3(ceil_div(m, 256)), dim3(256), 0, ctx().s(), m, nrhs, g_gv.p, alpha,
                        beta, Y, ldy);
