@@ -1797,18 +1797,18 @@ __global__ void __launch_bounds__(256) trsm2_kernel(int n, int nrhs, const T* L,
     }
 }
 
-// FP32 form of trsm2_kernel with the workgroup's 64-column panel of B (n <= 256 rows) held
+// FP32 form of trsm2_kernel with the workgroup's TN-column panel of B (n <= 256 rows) held
 // in LDS through the three products: the panel is read once and written once, only the
 // operands inv1, L21 (U12) and inv2 stream through two LDS stages. The products are the
 // same MFMA sums in the same order as gemm_nn_tile's (k tiles of 32, steps of 4), so the
 // results equal trsm2_kernel's.
-template <bool LOWER>
+template <bool LOWER, int TN>
 __global__ void __launch_bounds__(256) trsm2p_kernel(int n, int nrhs, const float* L, int ldl, const float* inv,
                                                      float* B, int ldb) {
     using Op = MfmaOp<float>;
     using Acc = typename Op::acc;
     constexpr int KB = GemmK<float>::KB, PAD = GemmK<float>::PAD, NQ = KB / 2, S = KB / 4;
-    constexpr int TN = TRSM2_TN, JT = TN / 32, NP = 2 * TRSM2_B + 34;  // panel column stride (34 mod 64 words)
+    constexpr int JT = TN / 32, NP = 2 * TRSM2_B + 34;  // panel column stride (34 mod 64 words)
     __shared__ __attribute__((aligned(16))) float Pn[TN * NP];        // Pn[c * NP + r] = B(r, c)
     __shared__ __attribute__((aligned(16))) float As[2][GBM][PAD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1922,9 +1922,16 @@ static void trsm_inv(int kind, int n, int nrhs, const T* L, int ldl, const T* in
     if constexpr (BS == TRSM2_B) {
         if (n <= 2 * BS) {  // split_point gives n1 = BS here
             ProfScope prof(PROF_TRSM, 2.0 * n * n * nrhs, sizeof(T) * (2.0 * n * nrhs + (double)n * n));
-            if constexpr (sizeof(T) == 4)
-                hipLaunchKernelGGL((trsm2p_kernel<LOWER>), dim3((unsigned)ceil_div(nrhs, TRSM2_TN)), dim3(256), 0,
-                                   ctx().s(), n, nrhs, L, ldl, inv, B, ldb);
+            if constexpr (sizeof(T) == 4) {
+                // 32-column panels when 64-column ones would leave CUs idle (narrow right-hand
+                // sides inside the LU): twice the workgroups, half the work each
+                if (ceil_div(nrhs, TRSM2_TN) < std::max(ctx().cu_count, 1))
+                    hipLaunchKernelGGL((trsm2p_kernel<LOWER, 32>), dim3((unsigned)ceil_div(nrhs, 32)), dim3(256), 0,
+                                       ctx().s(), n, nrhs, L, ldl, inv, B, ldb);
+                else
+                    hipLaunchKernelGGL((trsm2p_kernel<LOWER, TRSM2_TN>), dim3((unsigned)ceil_div(nrhs, TRSM2_TN)),
+                                       dim3(256), 0, ctx().s(), n, nrhs, L, ldl, inv, B, ldb);
+            }
             else
                 hipLaunchKernelGGL((trsm2_kernel<T, LOWER>), dim3((unsigned)ceil_div(nrhs, TRSM2_TN)), dim3(256), 0,
                                    ctx().s(), n, nrhs, L, ldl, inv, B, ldb);
