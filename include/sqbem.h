@@ -26,7 +26,11 @@ extern "C" {
 #define SQBEM_ERR_SINGULAR 3 /* zero pivot: LAPACK info > 0 */
 #define SQBEM_ERR_NOMEM 4    /* device allocation failed */
 #define SQBEM_ERR_STATE 5    /* object used in a state that does not allow the call */
-#define SQBEM_ERR_CONVERGE 6 /* iterative refinement did not reach its tolerance */
+#define SQBEM_ERR_CONVERGE 6 /* iterative refinement did not reach SQBEM_IR_TOL */
+
+/* Relative residual |b - A x| / |b| (2-norm per right-hand side, FP64 operator) that every
+ * refined solve reaches, or fails with SQBEM_ERR_CONVERGE after 40 steps. */
+#define SQBEM_IR_TOL 1e-12
 
 /* arithmetic of the dense factorisations (operator assembly is FP64 in every mode) */
 #define SQBEM_FP64 0    /* FP64 everywhere: the drop-in path */
@@ -178,8 +182,10 @@ typedef struct sqbem_duct sqbem_duct;
  * rank-one deflation (src/swimmer.hpp:64-74) on the device and factors it there. */
 sqbem_tube* sqbem_tube_create(const sqbem_mesh* mesh, double mu, int precision);
 int sqbem_tube_info(const sqbem_tube* t, int* Nt, double* t_assembly, double* t_factor);
-/* the tube's factor (Tube::lu, src/swimmer.hpp:38): a view valid while the tube lives
- * (FP64 mode); in the refined modes it solves with the FP32 factors */
+/* the tube's factor (Tube::lu, src/swimmer.hpp:38): a view valid while the tube lives. In
+ * the refined modes sqbem_lu_solve corrects the FP32 factors' solution by iterative
+ * refinement against the FP64 operator to SQBEM_IR_TOL, so it returns FP64 solutions in
+ * every mode; sqbem_lu_get returns the FP32 factors widened to FP64. */
 const sqbem_lu* sqbem_tube_lu(const sqbem_tube* t);
 void sqbem_tube_free(sqbem_tube* t);
 

@@ -49,7 +49,7 @@ struct Tube {
         if (!t) throw std::runtime_error(sqbem_last_error());
         handle_.reset(t, sqbem_tube_free);
         sqbem_check(sqbem_tube_info(t, nullptr, &t_assembly, &t_factor));
-        lu.attach(sqbem_tube_lu(t), 3 * Nt);
+        lu.attach(sqbem_tube_lu(t));
     }
 
     // rank-one deflation of the null mode f = c m on panels [p0, p0+np) of M, on a host Mat

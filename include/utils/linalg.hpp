@@ -41,7 +41,9 @@ inline void solve_dense(Mat A, Mat& B) {
 }
 
 // LU factorisation kept (in HBM) for repeated solves with different right-hand sides.
-// LU and piv are filled on request by download(); factor() and solve() never need them.
+// factor() also copies the packed factors and 1-based pivots back into LU and piv, as the
+// reference's dgetrf_ leaves them (src/utils/linalg.hpp:62-71). A view onto factors owned
+// elsewhere (a Tube's) leaves LU an empty 0 x 0 Mat and piv empty until download().
 struct LUFactor {
     Mat LU;
     std::vector<int> piv;
@@ -52,9 +54,7 @@ struct LUFactor {
         if (!h) throw std::runtime_error(info ? "dgetrf failed" : sqbem_last_error());
         own_.reset(h, sqbem_lu_free);
         view_ = h;
-        LU = Mat();
-        LU.n = LU.m = A.n;
-        piv.clear();
+        download();
     }
     void solve(Mat& B) const {  // B <- A^{-1} B
         if (B.m == 0) return;
@@ -69,11 +69,11 @@ struct LUFactor {
         sqbem_check(sqbem_lu_get(view_, LU.a.data(), piv.data()));
     }
     // non-owning view onto factors owned elsewhere (a Tube)
-    void attach(const sqbem_lu* v, int n) {
+    void attach(const sqbem_lu* v) {
         own_.reset();
         view_ = v;
         LU = Mat();
-        LU.n = LU.m = n;
+        piv.clear();
     }
     const sqbem_lu* handle() const { return view_; }
 

@@ -696,7 +696,7 @@ void run_chunk(SharedTube& ST, std::vector<Inst*>& I, int prec, double mu, Arena
             (void)hipGraphDestroy(graph);
             SQ_REQUIRE(ie == hipSuccess, SQBEM_ERR_HIP, std::string("refinement graph: ") + hipGetErrorString(ie));
         }
-        const double tol = 1e-12;
+        const double tol = SQBEM_IR_TOL;
         int it = 0;
         double worst = 1.0;
         try {
@@ -726,8 +726,9 @@ void run_chunk(SharedTube& ST, std::vector<Inst*>& I, int prec, double mu, Arena
             throw;
         }
         if (exec) (void)hipGraphExecDestroy(exec);
-        SQ_REQUIRE(worst <= 1e-9, SQBEM_ERR_CONVERGE,
-                   "iterative refinement stalled at relative residual " + std::to_string(worst));
+        SQ_REQUIRE(worst <= tol, SQBEM_ERR_CONVERGE,
+                   "iterative refinement did not reach relative residual 1e-12 (" + std::to_string(worst) +
+                       " after " + std::to_string(it) + " steps)");
         SQ_HIP(hipMemcpyAsync(X.data(), Xsc, C * sizeof(double), hipMemcpyDeviceToHost, ctx().s()));
         sync();
         for (int i = 0; i < k; ++i) {

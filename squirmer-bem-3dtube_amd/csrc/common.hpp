@@ -103,6 +103,8 @@ struct DBuf {
 int* device_error_word();
 void mark_device_error_check();
 void check_device_error();
+// zeroes the chained triangular solve's block counters (after an error word was seen)
+void trsv_reset_counters();
 inline void sync() {
     SQ_HIP(hipStreamSynchronize(ctx().s()));
     check_device_error();

@@ -2382,6 +2382,13 @@ struct TrsvState {
 static DBuf<TrsvState> g_trsv;
 static unsigned long long g_trsv_base = 0;
 
+// after a chain reported an error its counters may be out of step: start them over
+void trsv_reset_counters() {
+    if (!g_trsv.p) return;
+    g_trsv.zero();
+    g_trsv_base = 0;
+}
+
 // B (n x nrhs, nrhs <= 8) <- T^-1 B with the inverted 128 x 128 diagonal blocks of T at
 // hand (Dinv: nb blocks, NB x NB each, consecutive): the same ticket-ordered chain as
 // trsv_chain with 128-row blocks, where a block's own solve is the product of its inverse
@@ -2403,11 +2410,17 @@ __device__ __forceinline__ void trsv_inv_body(int n, int nrhs, const T* __restri
     __shared__ unsigned long long s_seen;
     const int tid = threadIdx.x, row = tid & (NB - 1), hf = tid >> 7;
     const int nb = (n + NB - 1) / NB;
-    if (tid == 0)
-        s_k = (int)(__hip_atomic_fetch_add(&st->ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - base);
+    if (tid == 0) {
+        const unsigned long long tk =
+            __hip_atomic_fetch_add(&st->ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // a ticket below the launch's base means the host and device counters disagree: the
+        // block cannot be named, so the call fails instead of touching rows outside B
+        s_k = tk < base ? -1 : (tk - base >= (unsigned long long)nb ? nb : (int)(tk - base));
+        if (s_k < 0) __hip_atomic_store(err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     __syncthreads();
     const int k = s_k;
-    if (k >= nb) return;  // surplus workgroup of a batched launch (whole workgroup)
+    if (k < 0 || k >= nb) return;  // counter mismatch, or surplus workgroup of a batched launch
     const int blk = LOWER ? k : nb - 1 - k;
     const int r0 = blk * NB, rb = min(NB, n - r0);
     {
@@ -2571,7 +2584,6 @@ static void trsv_inv(int n, int nrhs, const T* Lt, const T* Dinv, T* B, int ldb)
     }
     int* err = device_error_word();
     const unsigned long long base = g_trsv_base;
-    g_trsv_base += (unsigned long long)nb;
     mark_device_error_check();
     ProfScope prof(PROF_TRSV, (double)n * n * nrhs, sizeof(T) * 0.5 * n * n);
     if (nrhs == 1) trsv_inv_launch<T, LOWER, 1>(n, nrhs, Lt, Dinv, B, ldb, base, err);
@@ -2579,6 +2591,9 @@ static void trsv_inv(int n, int nrhs, const T* Lt, const T* Dinv, T* B, int ldb)
     else if (nrhs <= 4) trsv_inv_launch<T, LOWER, 4>(n, nrhs, Lt, Dinv, B, ldb, base, err);
     else trsv_inv_launch<T, LOWER, 8>(n, nrhs, Lt, Dinv, B, ldb, base, err);
     check_launch("trsv_inv_chain");
+    // the device counter advances by nb only once the launch is queued: a throw above leaves
+    // host base and device ticket in step
+    g_trsv_base = base + (unsigned long long)nb;
 }
 
 // Copy of an n x n matrix as 128 x 128 tiles, tile (I, J) at (I nb + J) 128^2, column-major

@@ -2,11 +2,14 @@
 // src/utils/linalg.hpp:31-78): operands are staged through HBM, the work runs on the
 // device kernels of dense.hip, and LU factors stay resident in HBM.
 #include <algorithm>
+#include <cmath>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
 #include "dense.hpp"
 #include "lu_handle.hpp"
+#include "solver.hpp"
 
 using namespace sq;
 
@@ -26,12 +29,54 @@ void get(const DBuf<double>& d, int m, int n, double* h, int ld) {
 }
 }  // namespace
 
+namespace {
+// 2-norms of the nrhs columns of a device n x nrhs matrix (leading dimension ld)
+std::vector<double> col_norms(int n, int nrhs, const double* a, int ld) {
+    std::vector<double> h((size_t)n * nrhs), out(nrhs, 0.0);
+    SQ_HIP(hipMemcpy2DAsync(h.data(), sizeof(double) * n, a, sizeof(double) * ld, sizeof(double) * n, nrhs,
+                            hipMemcpyDeviceToHost, ctx().s()));
+    sync();
+    for (int j = 0; j < nrhs; ++j) {
+        double s2 = 0;
+        for (int i = 0; i < n; ++i) s2 += h[(size_t)j * n + i] * h[(size_t)j * n + i];
+        out[j] = std::sqrt(s2);
+    }
+    return out;
+}
+}  // namespace
+
 void sqbem_lu::solve_device(int nrhs, double* B, int ldb) const {
     if (LU32) {
-        DBuf<float> b32((size_t)n * nrhs);
-        convert_matrix<double, float>(n, nrhs, B, ldb, b32.p, n);
-        getrs<float>(kind, 0, n, nrhs, LU32, n, piv, b32.p, n, aux32);
-        convert_matrix<float, double>(n, nrhs, b32.p, n, B, ldb);
+        DBuf<float> d32((size_t)n * nrhs);
+        if (!A64) {  // no FP64 operator at hand: the FP32 factors' solution
+            convert_matrix<double, float>(n, nrhs, B, ldb, d32.p, n);
+            getrs<float>(kind, 0, n, nrhs, LU32, n, piv, d32.p, n, aux32);
+            convert_matrix<float, double>(n, nrhs, d32.p, n, B, ldb);
+            sync();
+            return;
+        }
+        // iterative refinement: x += LU32^-1 r, r = b - A64 x (FP64), to SQBEM_IR_TOL
+        DBuf<double> x((size_t)n * nrhs), r((size_t)n * nrhs), d((size_t)n * nrhs);
+        x.zero();
+        convert_matrix<double, double>(n, nrhs, B, ldb, r.p, n);
+        const std::vector<double> bn = col_norms(n, nrhs, r.p, n);
+        double worst = 1.0;
+        int it = 0;
+        for (; it < 40 && worst > SQBEM_IR_TOL; ++it) {
+            convert_matrix<double, float>(n, nrhs, r.p, n, d32.p, n);
+            getrs<float>(kind, 0, n, nrhs, LU32, n, piv, d32.p, n, aux32);
+            convert_matrix<float, double>(n, nrhs, d32.p, n, d.p, n);
+            axpy((size_t)n * nrhs, 1.0, d.p, x.p);
+            convert_matrix<double, double>(n, nrhs, B, ldb, r.p, n);
+            dgemm(n, nrhs, n, -1.0, A64, n, x.p, n, 1.0, r.p, n);
+            const std::vector<double> rn = col_norms(n, nrhs, r.p, n);
+            worst = 0.0;
+            for (int j = 0; j < nrhs; ++j) worst = std::max(worst, bn[j] > 0 ? rn[j] / bn[j] : rn[j]);
+        }
+        SQ_REQUIRE(worst <= SQBEM_IR_TOL, SQBEM_ERR_CONVERGE,
+                   "iterative refinement did not reach relative residual 1e-12 (" + std::to_string(worst) +
+                       " after " + std::to_string(it) + " steps)");
+        convert_matrix<double, double>(n, nrhs, x.p, n, B, ldb);
         sync();
     } else {
         getrs<double>(SQBEM_FP64, 0, n, nrhs, LU, n, piv, B, ldb, aux);

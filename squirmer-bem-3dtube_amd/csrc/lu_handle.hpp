@@ -11,8 +11,9 @@ struct sqbem_lu {
     const int* piv = nullptr;    // 1-based pivots
     const double* aux = nullptr;   // getrf's solve aid for LU (or null)
     const float* aux32 = nullptr;  // same for LU32
+    const double* A64 = nullptr;   // FP64 operator of FP32 factors (refinement residuals), or null
     sq::DBuf<double> own;
     sq::DBuf<int> own_piv;
-    // B (device, n x nrhs) <- A^-1 B
+    // B (device, n x nrhs) <- A^-1 B; FP32 factors with A64 set are refined to SQBEM_IR_TOL
     void solve_device(int nrhs, double* B, int ldb) const;
 };

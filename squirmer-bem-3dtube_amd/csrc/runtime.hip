@@ -73,8 +73,10 @@ void check_device_error() {
     SQ_HIP(hipMemcpy(&v, g_dev_err, sizeof(int), hipMemcpyDeviceToHost));
     if (v) {
         fill_zero(g_dev_err, 256);
+        trsv_reset_counters();
         SQ_HIP(hipStreamSynchronize(ctx().s()));
-        throw Error(SQBEM_ERR_HIP, "triangular solve: a dependency wait timed out");
+        throw Error(SQBEM_ERR_HIP, v == 2 ? "triangular solve: block counter out of step with the host"
+                                          : "triangular solve: a dependency wait timed out");
     }
 }
 

@@ -233,7 +233,7 @@ __global__ void axpy_kernel(size_t n, double alpha, const double* __restrict__ y
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
         x[e] += alpha * y[e];
 }
-static void axpy(size_t n, double alpha, const double* y, double* x) {
+void axpy(size_t n, double alpha, const double* y, double* x) {
     if (!n) return;
     unsigned blocks = (unsigned)std::min<size_t>(ceil_div(n, 256), 4096);
     hipLaunchKernelGGL(axpy_kernel, dim3(blocks), dim3(256), 0, ctx().s(), n, alpha, y, x);
@@ -365,6 +365,7 @@ void sq::tube_build(sqbem_tube* t) {
     t->view.piv = t->ipiv.p;
     t->view.aux = mixed(t->prec) ? nullptr : t->aux64.p;
     t->view.aux32 = mixed(t->prec) ? t->aux32.p : nullptr;
+    t->view.A64 = mixed(t->prec) ? t->T64.p : nullptr;
     t->t_assembly = t1 - t0;
     t->t_factor = now_s() - t1;
 }
@@ -677,7 +678,7 @@ static void solve_mixed(sqbem_swimmer* s, int k, const double* u_host, bool towe
     std::vector<double> bn, rn;
     col_norms(N, k, b.p, bn);
     convert_matrix<double, double>(N, k, b.p, N, r.p, N);
-    const double tol = 1e-12;
+    const double tol = SQBEM_IR_TOL;
     int it = 0;
     double worst = 1;
     for (; it < 40; ++it) {
@@ -694,8 +695,9 @@ static void solve_mixed(sqbem_swimmer* s, int k, const double* u_host, bool towe
     }
     s->ir_iters = it;
     s->ir_residual = worst;
-    SQ_REQUIRE(worst <= 1e-9, SQBEM_ERR_CONVERGE,
-               "iterative refinement stalled at relative residual " + std::to_string(worst));
+    SQ_REQUIRE(worst <= tol, SQBEM_ERR_CONVERGE,
+               "iterative refinement did not reach relative residual 1e-12 (" + std::to_string(worst) + " after " +
+                   std::to_string(it) + " steps)");
     std::vector<double> hx((size_t)N * k);
     SQ_HIP(hipMemcpyAsync(hx.data(), x.p, hx.size() * sizeof(double), hipMemcpyDeviceToHost, ctx().s()));
     sync();

@@ -36,6 +36,9 @@ template <typename T>
 void gemv(int m, int n, int nrhs, double alpha, const T* A, long lda, const T* X, long ldx, double beta, T* Y,
           long ldy);
 
+// x += alpha y (n doubles, device)
+void axpy(size_t n, double alpha, const double* y, double* x);
+
 // AuBs(3i+a, 3i+b) += dls_i[3a+b] / (8 pi) - minus_identity delta_ab (completed double layer)
 void completion_diag(int Ns, double* A, long lda, const double* dls, double minus_identity);
 

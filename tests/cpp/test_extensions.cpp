@@ -6,7 +6,12 @@
 //     identical copy) equals SwimmerSolver::solve_free on the same meshes, FP64, to 1e-12;
 //  2. the same batch in FP16 MFMA + FP64 refinement agrees with FP64 to 1e-8;
 //  3. the adjoint's dJ/dB2 equals J of the B2-only solution (J is linear in B1, B2), and
-//     dJ/dB1 equals J of the B1-only solution, to 1e-10.
+//     dJ/dB1 equals J of the B1-only solution, to 1e-10;
+//  4. LUFactor::factor leaves the packed factors and 1-based pivots in LU and piv, as
+//     dgetrf_ does (P A = L U to 1e-13 on a matrix that needs row interchanges); a Tube's
+//     lu is a 0 x 0 Mat until download();
+//  5. Tube::lu.solve of an FP32-factored tube returns the FP64 solution (refined against
+//     the FP64 operator) to 1e-10 of the FP64 tube's.
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -81,6 +86,57 @@ int main() {
         eb = std::max(eb, std::fabs(gr.d[j].dB2 - J2[j]) / sc);
     }
     check(eb <= 1e-10, "adjoint dJ/dB1, dJ/dB2 = J of the single-mode solutions", eb);
+
+    // LUFactor fills LU and piv (src/utils/linalg.hpp:62-71)
+    {
+        const int n = 7;
+        bem::Mat A(n, n);
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) A(i, j) = std::sin(1.0 + i * 0.7 + j * j * 0.3) + (i == n - 1 - j ? 0.5 : 0.0);
+        A(0, 0) = 1e-3;  // forces an interchange in the first column
+        bem::LUFactor lu;
+        lu.factor(A);
+        bool shape = lu.LU.n == n && lu.LU.m == n && lu.LU.a.size() == (size_t)n * n && lu.piv.size() == (size_t)n;
+        double err = shape ? 0.0 : 1.0;
+        if (shape) {
+            bem::Mat PA = A;  // apply the interchanges in order, as dlaswp does
+            for (int i = 0; i < n; ++i) {
+                const int p = lu.piv[i] - 1;
+                if (p < 0 || p >= n) { err = 1.0; break; }
+                for (int j = 0; j < n; ++j) std::swap(PA(i, j), PA(p, j));
+            }
+            double amax = 0;
+            for (double v : A.a) amax = std::max(amax, std::fabs(v));
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j) {
+                    double s = 0;
+                    for (int k = 0; k <= std::min(i, j); ++k) s += (k == i ? 1.0 : lu.LU(i, k)) * lu.LU(k, j);
+                    err = std::max(err, std::fabs(s - PA(i, j)) / amax);
+                }
+        }
+        check(shape && lu.piv[0] != 1, "LUFactor::factor fills LU (n x n) and piv (interchange)", shape ? lu.piv[0] : -1);
+        check(err <= 1e-13, "LUFactor: P A = L U from the returned LU and piv", err);
+        check(T.lu.LU.n == 0 && T.lu.LU.m == 0 && T.lu.LU.a.empty() && T.lu.piv.empty(),
+              "Tube::lu is an empty 0 x 0 Mat until download()", (double)T.lu.LU.a.size());
+    }
+    // Tube::lu.solve of a refined-mode tube gives the FP64 solution
+    {
+        const bem::Tube T32(R, L, 0.3, nth, nr, 1.0, SQBEM_FP32_IR);
+        const int n = 3 * T.Nt;
+        bem::Mat b64(n, 2);
+        for (int j = 0; j < 2; ++j)
+            for (int i = 0; i < n; ++i) b64(i, j) = std::cos(0.37 * i + 1.3 * j);
+        bem::Mat b32 = b64;
+        T.lu.solve(b64);
+        T32.lu.solve(b32);
+        double num = 0, den = 0;
+        for (size_t e = 0; e < b64.a.size(); ++e) {
+            num += (b32.a[e] - b64.a[e]) * (b32.a[e] - b64.a[e]);
+            den += b64.a[e] * b64.a[e];
+        }
+        const double d = std::sqrt(num / den);
+        check(d <= 1e-10, "Tube::lu.solve (FP32 factors, refined) = FP64 tube", d);
+    }
 
     std::printf("%s\n", failures ? "some checks failed" : "all checks passed");
     return failures ? 1 : 0;

@@ -311,3 +311,25 @@ def test_host_linalg_api(sq):
     assert np.allclose(Y, X, rtol=1e-13, atol=1e-13)
     with pytest.raises(RuntimeError, match="gemm: dimension mismatch"):
         sq.gemm(1.0, A, B, 0.0, np.zeros((5, 5), order="F"))
+
+
+@pytest.mark.parametrize("n,nrhs", [(256, 16400), (256, 1000), (200, 16384)])
+def test_sgetrs_fused_bottom_levels_both_panel_widths(sq, n, nrhs):
+    """FP32 factors of order 128 < n <= 256 solve many right-hand sides in one fused launch
+    of the triangular solves' bottom two levels: 64-column panels once ceil(nrhs / 64) fills
+    every CU (nrhs >= 16,321 on 256 CUs), 32-column panels below that. Both widths against
+    the FP32 backward-error bound."""
+    rng = np.random.default_rng(n + nrhs)
+    A = rng.standard_normal((n, n)) + n ** 0.5 * np.eye(n)
+    X = rng.standard_normal((n, nrhs))
+    Bm = A @ X
+    dA = dev(A.astype(np.float32))
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    assert sq.sgetrf(1, dA, ipiv) == 0
+    dB = dev(Bm.astype(np.float32))
+    sq.sgetrs(dA, ipiv, dB)
+    sq.sync()
+    Xg = host(dB).astype(np.float64)
+    r = np.linalg.norm(A @ Xg - Bm, axis=0) / (np.linalg.norm(A, 2) * np.linalg.norm(Xg, axis=0))
+    assert r.max() < 30 * n * 1.2e-7, r.max()
