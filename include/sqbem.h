@@ -107,6 +107,20 @@ int sqbem_sgemm(int m, int n, int k, float alpha, const float* A, int lda, const
  * two, undone exactly in the epilogue */
 int sqbem_gemm16(int kind, int m, int n, int k, float alpha, const float* A, int lda,
                  const float* B, int ldb, float beta, float* C, int ldc);
+/* No counterpart in the reference: sqbem_gemm16 that allocates nothing, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Its workspace (converted operands, FP16 scales) is a
+ * fixed scratch region of SQBEM_GEMM16_SCRATCH_BYTES per device inside the library, set up
+ * by sqbem_init (which must have been called; this call never initialises). Products whose
+ * workspace (sqbem_gemm16_noalloc_bytes) exceeds it are rejected with SQBEM_ERR_ARG, e.g.
+ * m = n = k = 4096 needs 64 MiB + scales. Every call on a device shares that device's
+ * region: calls that overlap in time on one device (two streams, two host threads) race.
+ * The tile is the autotuner's decided one for the shape class, else the fixed rule; the
+ * call neither explores nor records timing events. Same results as sqbem_gemm16. */
+#define SQBEM_GEMM16_SCRATCH_BYTES (96u << 20)
+int sqbem_gemm16_noalloc(int kind, int m, int n, int k, float alpha, const float* A, int lda,
+                         const float* B, int ldb, float beta, float* C, int ldc);
+unsigned long sqbem_gemm16_noalloc_bytes(int kind, int m, int n, int k);
 /* P A = L U with partial pivoting, dgetrf_, src/utils/linalg.hpp:65-71. ipiv is a device
  * array of n 1-based pivot rows; *info (host) is 0 or the first zero-pivot column. */
 int sqbem_dgetrf(int n, double* A, int lda, int* ipiv, int* info);

@@ -45,6 +45,9 @@ struct Context {
 Context& ctx();          // initialises device 0 on first use
 void ensure_init();      // throws on failure
 void upload_tables();    // quadrature tables to constant memory (assemble.hip)
+// device address of the no-allocation GEMM's scratch on this device (dense.hip), looked up
+// once per device with that device current
+void gemm16_scratch_setup(int device);
 // Zero device bytes with a kernel on the library stream (no memset nodes, so the
 // sequence stays safe inside captured graphs).
 void fill_zero(void* p, size_t bytes);

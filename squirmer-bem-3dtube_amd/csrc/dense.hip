@@ -23,6 +23,8 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <memory>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -895,6 +897,14 @@ struct Probe {
     }
 };
 
+// the tile of a decided class, else the rule; looks up without adding a class or events
+static int decided(int kind, int m, int n, int k, int rule) {
+    if (!g_mode) return rule;
+    auto it = g_table.find((unsigned)kind << 24 | (unsigned)lg(m) << 16 | (unsigned)lg(n) << 8 | (unsigned)lg(k));
+    if (it == g_table.end() || it->second.best < 0) return rule;
+    return it->second.best;
+}
+
 static void reset(int mode) {
     SQ_HIP(hipStreamSynchronize(ctx().s()));
     for (auto& kv : g_table)
@@ -970,14 +980,44 @@ static void info(int* classes, int* decided, int* counts) {
 }
 }  // namespace tune
 
+// Workspace of one 16-bit GEMM: the converted, zero-padded operands (large shapes) and,
+// for FP16, the power-of-two row and column scales with the row absmax words. ws == null:
+// the library's own growing buffers, the tile autotuner and kernel timing; otherwise the
+// caller's region of ws_bytes (no allocation, no events: a decided tile or the fixed rule).
+struct G16Work {
+    unsigned char* ws = nullptr;
+    size_t ws_bytes = 0;
+};
+
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// bytes of a caller workspace for gemm16(kind, m, n, k): operands padded to 256 rows and
+// columns (every tile candidate fits) and 64-deep k, then the FP16 scales
+size_t gemm16_work_bytes(int kind, int m, int n, int k) {
+    if (m <= 0 || n <= 0 || k <= 0) return 0;
+    size_t b = 0;
+    if (m >= 256 && n >= 256 && k >= 128) {
+        const size_t mp = (size_t)(m + 255) / 256 * 256, np = (size_t)(n + 255) / 256 * 256,
+                     kp = (size_t)(k + QBK - 1) / QBK * QBK;
+        b = align256((mp + np) * kp * 2);
+    }
+    if (kind != SQBEM_BF16_IR) b += align256(sizeof(float) * ((size_t)2 * m + n));
+    return b;
+}
+
 template <bool BF16, int TM, int TN, int PF>
 static void gemm16q(int m, int n, int k, float alpha, const float* A, int lda, const float* B, int ldb, float beta,
-                    float* C, int ldc, float* rs, float* cs) {
+                    float* C, int ldc, float* rs, float* cs, const G16Work& w) {
     using ht = typename Half<BF16>::t;
     const int mp = (m + TM - 1) / TM * TM, np = (n + TN - 1) / TN * TN, kp = (k + QBK - 1) / QBK * QBK;
     const size_t need = ((size_t)mp + np) * kp * sizeof(ht);
-    if (g_q16.n < need) g_q16.alloc(need);
-    ht* Ah = (ht*)g_q16.p;
+    ht* Ah;
+    if (w.ws) {
+        Ah = (ht*)w.ws;  // gemm16_work_bytes reserved the 256-padded operands at its start
+    } else {
+        if (g_q16.n < need) g_q16.alloc(need);
+        Ah = (ht*)g_q16.p;
+    }
     ht* Bh = Ah + (size_t)mp * kp;
     const size_t nrb = (size_t)(mp / 64) * (kp / 64), chunks = (size_t)np * kp / 8;
     if (chunks >= ((size_t)1 << 32) || nrb + ceil_div(chunks, 256) >= ((size_t)1 << 31))
@@ -988,70 +1028,80 @@ static void gemm16q(int m, int n, int k, float alpha, const float* A, int lda, c
     check_launch("convert16");
     const int tiles_m = mp / TM, tiles_n = np / TN;
     const double fl = 2.0 * m * n * k, by = 2.0 * ((double)m * k + (double)k * n) + 4.0 * (beta != 0.0f ? 2.0 : 1.0) * m * n;
-    ProfScope prof(PROF_GEMM16Q, fl, by);
     constexpr int cand = PF == 2 ? (TM == 128 ? (TN == 128 ? 4 : 5) : 6)
                                  : (TM == 128 ? (TN == 128 ? 0 : 1) : (TN == 128 ? 2 : 3));
-    ProfScope prof_tile(PROF_G16Q_C0 + cand, fl, by);
+    ProfScope prof(w.ws ? -1 : PROF_GEMM16Q, fl, by);
+    ProfScope prof_tile(w.ws ? -1 : PROF_G16Q_C0 + cand, fl, by);
     hipLaunchKernelGGL((gemm16q_kernel<BF16, TM, TN, PF>), dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, kp,
                        alpha, Ah, Bh, beta, C, ldc, tiles_m, rs, cs);
     check_launch("gemm16q");
 }
 
-void gemm16(int kind, int m, int n, int k, float alpha, const float* A, int lda, const float* B, int ldb,
-            float beta, float* C, int ldc) {
+// FP16 operand scales (powers of two per row of A and per column of B) into sp[0, 2m + n)
+static void fp16_scales(int m, int n, int k, const float* A, int lda, const float* B, int ldb, float* sp, float** rs,
+                        float** cs) {
+    *rs = sp;
+    *cs = sp + m;
+    unsigned* mx = (unsigned*)sp + m + n;
+    fill_zero(mx, sizeof(unsigned) * m);
+    hipLaunchKernelGGL(row_absmax_kernel, dim3(ceil_div(m, 64), ceil_div(k, 64)), dim3(256), 0, ctx().s(), m, k, A,
+                       lda, mx);
+    hipLaunchKernelGGL(bits_to_pow2_scale, dim3(ceil_div(m, 256)), dim3(256), 0, ctx().s(), m, mx, *rs);
+    hipLaunchKernelGGL(pow2_scale_cols, dim3(ceil_div(n, 4)), dim3(256), 0, ctx().s(), k, n, B, ldb, *cs);
+    check_launch("fp16 scales");
+}
+
+static void gemm16_impl(int kind, int m, int n, int k, float alpha, const float* A, int lda, const float* B, int ldb,
+                        float beta, float* C, int ldc, const G16Work& w) {
     if (m <= 0 || n <= 0) return;
     if (k <= 0 || alpha == 0.0f) {
         if (beta != 1.0f) scale_matrix(m, n, beta, C, ldc);
         return;
     }
     const bool bf = kind == SQBEM_BF16_IR;
-    if (m >= 256 && n >= 256 && k >= 128) {
-        float *rs = nullptr, *cs = nullptr;
-        if (!bf) {
-            float* sp = g_scale.get((size_t)2 * m + n);
-            rs = sp;
-            cs = sp + m;
-            unsigned* mx = (unsigned*)g_scale.get((size_t)2 * m + n) + m + n;
-            fill_zero(mx, sizeof(unsigned) * m);
-            hipLaunchKernelGGL(row_absmax_kernel, dim3(ceil_div(m, 64), ceil_div(k, 64)), dim3(256), 0, ctx().s(), m,
-                               k, A, lda, mx);
-            hipLaunchKernelGGL(bits_to_pow2_scale, dim3(ceil_div(m, 256)), dim3(256), 0, ctx().s(), m, mx, rs);
-            hipLaunchKernelGGL(pow2_scale_cols, dim3(ceil_div(n, 4)), dim3(256), 0, ctx().s(), k, n, B, ldb, cs);
+    const bool big = m >= 256 && n >= 256 && k >= 128;
+    float *rs = nullptr, *cs = nullptr;
+    if (!bf) {
+        float* sp;
+        if (w.ws) {
+            const size_t mp = (size_t)(m + 255) / 256 * 256, np = (size_t)(n + 255) / 256 * 256,
+                         kp = (size_t)(k + QBK - 1) / QBK * QBK;
+            sp = (float*)(w.ws + (big ? align256((mp + np) * kp * 2) : 0));
+        } else {
+            sp = g_scale.get((size_t)2 * m + n);
         }
+        fp16_scales(m, n, k, A, lda, B, ldb, sp, &rs, &cs);
+    }
+    if (big) {
         // fixed rule: 256 x 256 tiles once they fill the device, 128 x 128 below that;
         // the autotuner replaces it per shape class once it has timed every tile
         const int rule = ceil_div(m, 256) * ceil_div(n, 256) >= 192 ? 3 : 0;
-        tune::Probe probe(bf ? 1 : 0, m, n, k, rule);
+        int cand = rule;
+        std::unique_ptr<tune::Probe> probe;
+        if (w.ws) {
+            cand = tune::decided(bf ? 1 : 0, m, n, k, rule);
+        } else {
+            probe.reset(new tune::Probe(bf ? 1 : 0, m, n, k, rule));
+            cand = probe->cand;
+        }
         auto run = [&](auto tag_bf) {
             constexpr bool BFv = decltype(tag_bf)::value;
-            switch (probe.cand) {
-                case 0: gemm16q<BFv, 128, 128, 1>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs); break;
-                case 1: gemm16q<BFv, 128, 256, 1>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs); break;
-                case 2: gemm16q<BFv, 256, 128, 1>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs); break;
-                case 4: gemm16q<BFv, 128, 128, 2>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs); break;
-                case 5: gemm16q<BFv, 128, 256, 2>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs); break;
-                case 6: gemm16q<BFv, 256, 128, 2>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs); break;
-                default: gemm16q<BFv, 256, 256, 1>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs); break;
+            switch (cand) {
+                case 0: gemm16q<BFv, 128, 128, 1>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs, w); break;
+                case 1: gemm16q<BFv, 128, 256, 1>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs, w); break;
+                case 2: gemm16q<BFv, 256, 128, 1>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs, w); break;
+                case 4: gemm16q<BFv, 128, 128, 2>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs, w); break;
+                case 5: gemm16q<BFv, 128, 256, 2>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs, w); break;
+                case 6: gemm16q<BFv, 256, 128, 2>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs, w); break;
+                default: gemm16q<BFv, 256, 256, 1>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs, w); break;
             }
         };
         if (bf) run(std::true_type{});
         else run(std::false_type{});
         return;
     }
-    float *rs = nullptr, *cs = nullptr;
-    if (!bf) {
-        float* sp = g_scale.get((size_t)2 * m + n);
-        rs = sp;
-        cs = sp + m;
-        unsigned* mx = (unsigned*)g_scale.get((size_t)2 * m + n) + m + n;
-        fill_zero(mx, sizeof(unsigned) * m);
-        hipLaunchKernelGGL(row_absmax_kernel, dim3(ceil_div(m, 64), ceil_div(k, 64)), dim3(256), 0, ctx().s(), m, k, A,
-                           lda, mx);
-        hipLaunchKernelGGL(bits_to_pow2_scale, dim3(ceil_div(m, 256)), dim3(256), 0, ctx().s(), m, mx, rs);
-        hipLaunchKernelGGL(pow2_scale_cols, dim3(ceil_div(n, 4)), dim3(256), 0, ctx().s(), k, n, B, ldb, cs);
-    }
     int tiles_m = (int)ceil_div(m, GBM), tiles_n = (int)ceil_div(n, GBN);
-    ProfScope prof(PROF_GEMM16, 2.0 * m * n * k,
+    ProfScope prof(w.ws ? -1 : PROF_GEMM16, 2.0 * m * n * k,
                    4.0 * ((double)m * k + (double)k * n + (beta != 0.0f ? 2.0 : 1.0) * m * n));
     if (bf)
         hipLaunchKernelGGL(gemm16_nn_kernel<true>, dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, k,
@@ -1060,6 +1110,44 @@ void gemm16(int kind, int m, int n, int k, float alpha, const float* A, int lda,
         hipLaunchKernelGGL(gemm16_nn_kernel<false>, dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, k,
                            alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, rs, cs);
     check_launch("gemm16");
+}
+
+void gemm16(int kind, int m, int n, int k, float alpha, const float* A, int lda, const float* B, int ldb,
+            float beta, float* C, int ldc) {
+    gemm16_impl(kind, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, G16Work{});
+}
+
+// ------------------------------------------------------------------ no-allocation 16-bit GEMM
+// Scratch of sqbem_gemm16_noalloc: one fixed region per device, in the code object's
+// device globals (each device holds its own copy until the process ends). Host code
+// reaches it only through the address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[SQBEM_GEMM16_SCRATCH_BYTES];
+constexpr int kMaxDevices = 64;
+static unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+void gemm16_scratch_setup(int device) {
+    SQ_REQUIRE(device >= 0 && device < kMaxDevices, SQBEM_ERR_ARG, "gemm16 scratch: device id out of range");
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    SQ_REQUIRE(e == hipSuccess && p, SQBEM_ERR_HIP,
+               std::string("gemm16 scratch: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = (unsigned char*)p;
+}
+
+void gemm16_noalloc(int kind, int m, int n, int k, float alpha, const float* A, int lda, const float* B, int ldb,
+                    float beta, float* C, int ldc) {
+    const size_t need = gemm16_work_bytes(kind, m, n, k);
+    SQ_REQUIRE(need <= (size_t)SQBEM_GEMM16_SCRATCH_BYTES, SQBEM_ERR_ARG,
+               "gemm16_noalloc: the product needs " + std::to_string(need) + " bytes of scratch, more than the " +
+                   std::to_string((size_t)SQBEM_GEMM16_SCRATCH_BYTES) + " the library holds");
+    const int dev = ctx().device;
+    SQ_REQUIRE(dev >= 0 && dev < kMaxDevices && g_scratch_ptr[dev], SQBEM_ERR_STATE,
+               "gemm16_noalloc: the library is not initialised on this device (call sqbem_init first)");
+    G16Work w;
+    w.ws = g_scratch_ptr[dev];
+    w.ws_bytes = SQBEM_GEMM16_SCRATCH_BYTES;
+    gemm16_impl(kind, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, w);
 }
 
 // ------------------------------------------------------------------ elementwise helpers
@@ -3105,6 +3193,31 @@ int sqbem_gemm16(int kind, int m, int n, int k, float alpha, const float* A, int
         gemm16(kind, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
     });
 }
+
+int sqbem_gemm16_noalloc(int kind, int m, int n, int k, float alpha, const float* A, int lda, const float* B,
+                         int ldb, float beta, float* C, int ldc) {
+    // no implicit initialisation here (it allocates): the caller has called sqbem_init
+    try {
+        SQ_REQUIRE(ctx().device >= 0, SQBEM_ERR_STATE, "gemm16_noalloc: call sqbem_init first");
+        int cur = -1;
+        SQ_HIP(hipGetDevice(&cur));
+        SQ_REQUIRE(cur == ctx().device, SQBEM_ERR_STATE, "gemm16_noalloc: another device is current");
+        SQ_REQUIRE(kind == SQBEM_FP16_IR || kind == SQBEM_BF16_IR, SQBEM_ERR_ARG, "gemm16: kind must be FP16 or BF16");
+        SQ_REQUIRE(m >= 0 && n >= 0 && k >= 0 && lda >= std::max(1, m) && ldb >= std::max(1, k) &&
+                       ldc >= std::max(1, m),
+                   SQBEM_ERR_ARG, "gemm: dimension mismatch");
+        gemm16_noalloc(kind, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+        return SQBEM_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return SQBEM_ERR_ARG;
+    }
+}
+
+unsigned long sqbem_gemm16_noalloc_bytes(int kind, int m, int n, int k) { return (unsigned long)gemm16_work_bytes(kind, m, n, k); }
 
 int sqbem_dgetrf(int n, double* A, int lda, int* ipiv, int* info) {
     return guarded([&] {

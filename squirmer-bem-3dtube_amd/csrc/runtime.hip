@@ -38,6 +38,7 @@ static void init_device(int device) {
     std::snprintf(g_ctx.name, sizeof g_ctx.name, "%s (%s)", prop.name, prop.gcnArchName);
     if (!g_ctx.stream) SQ_HIP(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
     upload_tables();
+    gemm16_scratch_setup(device);
 }
 
 void ensure_init() {
@@ -207,7 +208,7 @@ void resolve_pending() {
 bool prof_enabled() { return g_prof; }
 
 ProfScope::ProfScope(int c, double f, double b) : cls(c), flops(f), bytes(b) {
-    if (!g_prof || !(g_prof_mask & (1u << c))) return;
+    if (c < 0 || !g_prof || !(g_prof_mask & (1u << c))) return;  // c < 0: never timed
     e0 = take_event();
     e1 = take_event();
     SQ_HIP(hipEventRecord(e0, ctx().s()));

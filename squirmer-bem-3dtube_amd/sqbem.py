@@ -52,6 +52,8 @@ _SIGS = {
     "sqbem_dgemm": (_i, [_i, _i, _i, _d, _vp, _i, _vp, _i, _d, _vp, _i]),
     "sqbem_sgemm": (_i, [_i, _i, _i, _f, _vp, _i, _vp, _i, _f, _vp, _i]),
     "sqbem_gemm16": (_i, [_i, _i, _i, _i, _f, _vp, _i, _vp, _i, _f, _vp, _i]),
+    "sqbem_gemm16_noalloc": (_i, [_i, _i, _i, _i, _f, _vp, _i, _vp, _i, _f, _vp, _i]),
+    "sqbem_gemm16_noalloc_bytes": (ctypes.c_ulong, [_i, _i, _i, _i]),
     "sqbem_dgetrf": (_i, [_i, _vp, _i, _vp, _vp]),
     "sqbem_dgetrs": (_i, [_i, _i, _vp, _i, _vp, _vp, _i]),
     "sqbem_dgetrs_trans": (_i, [_i, _i, _vp, _i, _vp, _vp, _i]),
@@ -340,6 +342,13 @@ def gemm16(kind, alpha, A, B, beta, C):
     m, k = A.shape
     n = B.shape[1]
     check(lib().sqbem_gemm16(kind, m, n, k, alpha, _ptr(A), _ld(A), _ptr(B), _ld(B), beta, _ptr(C), _ld(C)))
+
+
+def gemm16_noalloc(kind, alpha, A, B, beta, C):
+    """sqbem_gemm16 with the library's fixed per-device scratch (allocates nothing)."""
+    m, k = A.shape
+    n = B.shape[1]
+    check(lib().sqbem_gemm16_noalloc(kind, m, n, k, alpha, _ptr(A), _ld(A), _ptr(B), _ld(B), beta, _ptr(C), _ld(C)))
 
 
 def dgetrf(A, ipiv):
