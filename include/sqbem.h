@@ -208,6 +208,20 @@ void sqbem_tube_free(sqbem_tube* t);
  * (unbounded fluid). The sphere mesh is copied. */
 sqbem_swimmer* sqbem_swimmer_create(const sqbem_tube* tube, const sqbem_mesh* sphere,
                                     const double* center, double mu, int precision);
+/* Sphere template (an addition; the reference reassembles these per solver): the
+ * translation-invariant sphere-sphere blocks of SwimmerSolver::build (src/swimmer.hpp:
+ * 209-222: E, the double-layer part of AuBs and the sphere panels' stresslet row sums) of a
+ * sphere mesh centred at center, assembled once on the device. sqbem_swimmer_create_moved
+ * builds the solver of that sphere translated by shift[3] (centre center + shift, mesh
+ * translated as Mesh::translate does, src/utils/mesh.hpp:82-85), copying those blocks and
+ * assembling only the tube-sphere couplings; the result equals sqbem_swimmer_create on the
+ * translated mesh to rounding (the sphere blocks are computed in the template's position).
+ * The template must outlive the solvers made from it and share the tube's viscosity. */
+typedef struct sqbem_sphere sqbem_sphere;
+sqbem_sphere* sqbem_sphere_create(const sqbem_mesh* sphere, const double* center, double mu);
+void sqbem_sphere_free(sqbem_sphere* t);
+sqbem_swimmer* sqbem_swimmer_create_moved(const sqbem_tube* tube, const sqbem_sphere* sphere,
+                                          const double* shift, int precision);
 /* solve_free (src/swimmer.hpp:107-139), one solution per slip field. slips: host,
  * nslip*Ns*3; outputs host: U[3*nslip], Om[3*nslip], f_sphere[nslip*Ns*3],
  * f_tube[nslip*Nt*3] (outputs may be NULL) */

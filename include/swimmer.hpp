@@ -16,6 +16,7 @@
 // precision and recovers FP64 accuracy by iterative refinement against the FP64 operator.
 #pragma once
 #include <array>
+#include <cmath>
 #include <memory>
 #include <stdexcept>
 #include <vector>
@@ -52,24 +53,56 @@ struct Tube {
         lu.attach(sqbem_tube_lu(t));
     }
 
-    // rank-one deflation of the null mode f = c m on panels [p0, p0+np) of M, on a host Mat
-    // (the device solver applies the same term inside its assembly)
+    // Wielandt deflation of the null mode f = c m of panels [p0, p0 + np) of M, applied to
+    // the host matrix A at row/column offset off: A += u v^T with u = m / (8 pi mu sqrt(At))
+    // and v = m A_p (the device solver adds the same term inside its assembly)
     static void deflate(const Mesh& M, int p0, int np, Mat& A, int off, double mu) {
         double At = 0;
         for (int p = p0; p < p0 + np; ++p) At += M.area[p];
-        const double sc = 1.0 / (EIGHT_PI * mu * std::sqrt(At));
-        for (int i = 0; i < np; ++i)
-            for (int p = 0; p < np; ++p)
-                for (int a = 0; a < 3; ++a)
-                    for (int b = 0; b < 3; ++b)
-                        A(off + 3 * i + a, off + 3 * p + b) +=
-                            sc * M.normal[p0 + i][a] * M.normal[p0 + p][b] * M.area[p0 + p];
+        const double scale = 1.0 / (EIGHT_PI * mu * std::sqrt(At));
+        std::vector<double> u(3 * (size_t)np), v(3 * (size_t)np);
+        for (int p = 0; p < np; ++p)
+            for (int c = 0; c < 3; ++c) {
+                u[3 * p + c] = scale * M.normal[p0 + p][c];
+                v[3 * p + c] = M.normal[p0 + p][c] * M.area[p0 + p];
+            }
+        for (int j = 0; j < 3 * np; ++j) {
+            double* col = &A(off, off + j);
+            for (int i = 0; i < 3 * np; ++i) col[i] += u[i] * v[j];
+        }
     }
 
     const sqbem_tube* handle() const { return handle_.get(); }
 
 private:
     std::shared_ptr<sqbem_tube> handle_;
+};
+
+// ---------------------------------------------------------------- sphere template
+// An addition: the translation-invariant sphere-sphere blocks of a sphere mesh centred at
+// `center`, assembled once on the device (sqbem_sphere_create). A SwimmerSolver built from
+// it with a shift solves the same sphere translated by that shift without reassembling
+// them (a trajectory moves one sphere through a fixed tube).
+class SphereBlocks {
+public:
+    SphereBlocks(const Mesh& sphere, V3 center, double mu = 1.0) : S_(sphere), c_(center), mu_(mu) {
+        auto m = detail::to_sqbem(S_);
+        const double c[3] = {c_.x, c_.y, c_.z};
+        sqbem_sphere* h = sqbem_sphere_create(m.get(), c, mu_);
+        if (!h) throw std::runtime_error(sqbem_last_error());
+        h_.reset(h, sqbem_sphere_free);
+    }
+    const Mesh& mesh() const { return S_; }
+    V3 center() const { return c_; }
+    double mu() const { return mu_; }
+    const sqbem_sphere* handle() const { return h_.get(); }
+    std::shared_ptr<sqbem_sphere> share() const { return h_; }
+
+private:
+    Mesh S_;
+    V3 c_;
+    double mu_;
+    std::shared_ptr<sqbem_sphere> h_;
 };
 
 // ---------------------------------------------------------------- swimmer
@@ -99,6 +132,22 @@ public:
         sqbem_swimmer* s = sqbem_swimmer_create(tube ? tube->handle() : nullptr, m.get(), c, mu, prec);
         if (!s) throw std::runtime_error(sqbem_last_error());
         h_.reset(s, sqbem_swimmer_free);
+        times();
+    }
+
+    // the sphere of `blocks` translated by `shift` (an addition; equal to the constructor
+    // above on the translated mesh, to rounding)
+    SwimmerSolver(const Tube* t, const SphereBlocks& blocks, V3 shift, int precision = -1)
+        : tube(t), S(blocks.mesh()), xc(blocks.center() + shift), mu(blocks.mu()) {
+        S.translate(shift);
+        Ns = S.P();
+        Nt = tube ? tube->Nt : 0;
+        const int prec = precision >= 0 ? precision : (tube ? tube->precision : SQBEM_FP64);
+        const double d[3] = {shift.x, shift.y, shift.z};
+        sqbem_swimmer* s = sqbem_swimmer_create_moved(tube ? tube->handle() : nullptr, blocks.handle(), d, prec);
+        if (!s) throw std::runtime_error(sqbem_last_error());
+        h_.reset(s, sqbem_swimmer_free);
+        keep_ = blocks.share();
         times();
     }
 
@@ -137,24 +186,25 @@ public:
         return unpack(u, w, fs, ft, 0);
     }
 
-    // hydrodynamic force ON the sphere: F = -sum f dA (m points into it)
+    // hydrodynamic force ON the sphere (its normals point into it): F = -sum_p A_p f_p
     V3 force(const SwimmerSolution& s) const {
-        V3 F{};
-        for (int p = 0; p < Ns; ++p) F = F - S.area[p] * s.f_sphere[p];
-        return F;
+        double F[3] = {0, 0, 0};
+        for (int p = 0; p < Ns; ++p)
+            for (int c = 0; c < 3; ++c) F[c] -= S.area[p] * s.f_sphere[p][c];
+        return V3{F[0], F[1], F[2]};
     }
 
-    // velocity at points in the fluid (lab frame)
+    // velocity at points in the fluid (lab frame), from the boundary data on the merged
+    // surface: tube panels first (zero velocity, tube tractions), then the sphere's panels
+    // (rigid motion plus slip, sphere tractions)
     std::vector<V3> velocity(const SwimmerSolution& s, const std::vector<V3>& slip_field,
                              const std::vector<V3>& pts) const {
-        Mesh all = tube ? merge(tube->mesh, S) : S;
-        std::vector<V3> u(all.P()), f(all.P());
-        for (int p = 0; p < Nt; ++p) f[p] = s.f_tube[p];
-        for (int p = 0; p < Ns; ++p) {
-            u[Nt + p] = s.U + cross(s.Om, S.centroid[p] - xc) + slip_field[p];
-            f[Nt + p] = s.f_sphere[p];
-        }
-        return velocity_at(all, u, f, mu, pts);
+        std::vector<V3> f(s.f_tube.begin(), s.f_tube.begin() + Nt);
+        f.insert(f.end(), s.f_sphere.begin(), s.f_sphere.end());
+        std::vector<V3> u(Nt, V3{});
+        u.reserve(Nt + Ns);
+        for (int p = 0; p < Ns; ++p) u.push_back(s.U + cross(s.Om, S.centroid[p] - xc) + slip_field[p]);
+        return velocity_at(tube ? merge(tube->mesh, S) : S, u, f, mu, pts);
     }
 
     // Adjoint gradients (an addition; the reference has no derivatives). For each weight
@@ -193,6 +243,7 @@ public:
     const sqbem_swimmer* handle() const { return h_.get(); }
 
 private:
+    std::shared_ptr<sqbem_sphere> keep_;  // a moved solver's template (declared first: outlives h_)
     std::shared_ptr<sqbem_swimmer> h_;
 
     void times() { sqbem_check(sqbem_swimmer_times(h_.get(), &t_assembly, &t_schur, &ir_iters, &ir_residual)); }

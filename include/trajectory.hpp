@@ -16,6 +16,7 @@
 #include <fstream>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "swimmer.hpp"
 #include "utils/io.hpp"
@@ -39,113 +40,191 @@ struct TrajectoryParams {
     int precision = SQBEM_FP64;
 };
 
-// Runs (or resumes) a trajectory: writes <out_prefix>.csv, keeps <out_prefix>.ckpt while
-// paused. Returns "finished", "wall contact" or "paused".
-inline std::string run_trajectory(const TrajectoryParams& P, const std::string& out_prefix, bool verbose = true) {
-    using State = std::array<double, 6>;  // x, y, z, ex, ey, ez
-    Timer clock;
-    const double a = 1, R = a / P.a_over_R;
-    Tube tube(R, P.L_over_R * R, a, P.n_theta, P.n_r, 1.0, P.precision);
-    const Mesh sph0 = icosphere(a, P.sphere_level);
-    int nevals = 0;
+namespace detail {
 
-    struct Eval {
-        State d;
+using PathState = std::array<double, 6>;  // centre (x, y, z) and orientation (ex, ey, ez)
+
+// The right-hand side of the path equations, dX/dt = U, de/dt = Omega x e, at one state. In
+// the moving frame the sphere sits at (x, y, 0): the tube (assembled and factored once) and
+// the sphere template (its translation-invariant blocks, assembled once) stay on the device,
+// and each evaluation builds only the tube-sphere couplings and the Schur complement of the
+// translated sphere. Both squirming modes are solved together (U = U[B1=1] + alpha U[B2=1]).
+class PathRhs {
+public:
+    struct Value {
+        PathState rate{};
         V3 U, Om;
     };
-    auto evaluate = [&](const State& y) {
-        const V3 c{y[0], y[1], 0}, e = unit(V3{y[3], y[4], y[5]});
-        Mesh S = sph0;
-        S.translate(c);
-        SwimmerSolver s(&tube, S, c);
-        auto k = s.solve_free({s.slip(e, 1, 0), s.slip(e, 0, 1)});
-        Eval r;
-        r.U = k[0].U + P.alpha * k[1].U;
-        r.Om = k[0].Om + P.alpha * k[1].Om;
-        const V3 de = cross(r.Om, e);
-        r.d = {r.U.x, r.U.y, r.U.z, de.x, de.y, de.z};
-        ++nevals;
-        return r;
-    };
-    auto beta_of = [&](const State& y) { return std::hypot(y[0], y[1]) / (R - a); };
 
-    // ---- initialise or restart
-    const std::string ck = out_prefix + ".ckpt", csvp = out_prefix + ".csv";
-    State y{};
+    explicit PathRhs(const TrajectoryParams& P)
+        : a_(1.0),
+          R_(a_ / P.a_over_R),
+          alpha_(P.alpha),
+          tube_(R_, P.L_over_R * R_, a_, P.n_theta, P.n_r, 1.0, P.precision),
+          sphere_(icosphere(a_, P.sphere_level), V3{0, 0, 0}, 1.0) {}
+
+    // the solver at state y (kept by the caller when the adjoint needs it)
+    std::unique_ptr<SwimmerSolver> solver(const PathState& y) const {
+        return std::make_unique<SwimmerSolver>(&tube_, sphere_, V3{y[0], y[1], 0.0});
+    }
+    Value operator()(const PathState& y, std::unique_ptr<SwimmerSolver>* keep = nullptr) {
+        auto s = solver(y);
+        const V3 e = unit(V3{y[3], y[4], y[5]});
+        const auto modes = s->solve_free({s->slip(e, 1, 0), s->slip(e, 0, 1)});
+        Value v;
+        v.U = modes[0].U + alpha_ * modes[1].U;
+        v.Om = modes[0].Om + alpha_ * modes[1].Om;
+        const V3 turn = cross(v.Om, e);
+        v.rate = {v.U.x, v.U.y, v.U.z, turn.x, turn.y, turn.z};
+        ++calls_;
+        if (keep) *keep = std::move(s);
+        return v;
+    }
+    // radial offset in units of the free gap R - a
+    double beta(const PathState& y) const { return std::hypot(y[0], y[1]) / (R_ - a_); }
+    double gap() const { return R_ - a_; }
+    int calls() const { return calls_; }
+
+private:
+    double a_, R_, alpha_;
+    Tube tube_;
+    SphereBlocks sphere_;
+    int calls_ = 0;
+};
+
+// state at t = 0: offset beta0 (R - a) toward -x, e tilted by pitch toward +x and yaw toward +y
+inline PathState initial_state(const TrajectoryParams& P, double gap) {
+    const double b = P.beta0 * gap, pitch = P.pitch0_deg * PI / 180, yaw = P.yaw0_deg * PI / 180;
+    return {-b, 0.0, 0.0, std::cos(yaw) * std::sin(pitch), std::sin(yaw), std::cos(yaw) * std::cos(pitch)};
+}
+
+// y + h d
+inline PathState shifted(const PathState& y, double h, const PathState& d) {
+    PathState out;
+    for (int i = 0; i < 6; ++i) out[i] = y[i] + h * d[i];
+    return out;
+}
+
+// one step from y with f = rate(y): classical RK4 while fewer than three earlier rates are
+// known (the start-up), then Adams-Bashforth 4 over f and the three earlier rates (history,
+// oldest first), coefficients (55, -59, 37, -9) / 24. The orientation is not renormalised here.
+template <typename Rate>
+PathState integrate_step(const PathState& y, const PathState& f, const std::vector<PathState>& history, double dt,
+                         Rate&& rate) {
+    PathState z = y;
+    if (history.size() < 3) {
+        const PathState k2 = rate(shifted(y, dt / 2, f));
+        const PathState k3 = rate(shifted(y, dt / 2, k2));
+        const PathState k4 = rate(shifted(y, dt, k3));
+        for (int i = 0; i < 6; ++i) z[i] += dt / 6 * (f[i] + 2 * k2[i] + 2 * k3[i] + k4[i]);
+    } else {
+        const PathState &f1 = history[2], &f2 = history[1], &f3 = history[0];
+        for (int i = 0; i < 6; ++i) z[i] += dt / 24 * (55 * f[i] - 59 * f1[i] + 37 * f2[i] - 9 * f3[i]);
+    }
+    return z;
+}
+
+// keep |e| = 1
+inline void renormalise(PathState& y) {
+    const double n = std::sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
+    for (int i = 3; i < 6; ++i) y[i] /= n;
+}
+
+// Restart file of run_trajectory (src/trajectory.hpp:69-104): "t step n" on the first line,
+// the state on the second, then the n (<= 3) most recent rates, oldest first, each value
+// with 17 significant digits.
+struct PathCheckpoint {
     double t = 0;
     int step = 0;
-    std::vector<State> hist;  // up to 3 previous derivatives, oldest first
-    std::ifstream in(ck);
-    const bool restart = (bool)in;
-    if (restart) {
-        size_t nh;
-        in >> t >> step >> nh;
-        for (auto& v : y) in >> v;
-        hist.resize(nh);
-        for (auto& h : hist)
-            for (auto& v : h) in >> v;
-        if (verbose) std::printf("restart at t = %.3f (step %d)\n", t, step);
-    } else {
-        const double b = P.beta0 * (R - a), xi = P.pitch0_deg * PI / 180, chi = P.yaw0_deg * PI / 180;
-        y = {-b, 0, 0, std::cos(chi) * std::sin(xi), std::sin(chi), std::cos(chi) * std::cos(xi)};
+    PathState y{};
+    std::vector<PathState> history;
+
+    bool read(const std::string& path) {
+        std::ifstream in(path);
+        if (!in) return false;
+        size_t n = 0;
+        in >> t >> step >> n;
+        for (double& v : y) in >> v;
+        history.assign(n, PathState{});
+        for (PathState& h : history)
+            for (double& v : h) in >> v;
+        return true;
     }
-    char info[256];
-    std::snprintf(info, sizeof info,
+    void write(const std::string& path) const {
+        std::FILE* f = std::fopen(path.c_str(), "w");
+        if (!f) return;
+        std::fprintf(f, "%.17g %d %zu\n", t, step, history.size());
+        for (double v : y) std::fprintf(f, "%.17g ", v);
+        std::fprintf(f, "\n");
+        for (const PathState& h : history) {
+            for (double v : h) std::fprintf(f, "%.17g ", v);
+            std::fprintf(f, "\n");
+        }
+        std::fclose(f);
+    }
+};
+
+}  // namespace detail
+
+// Runs (or resumes) a trajectory: writes <out_prefix>.csv, keeps <out_prefix>.ckpt while
+// paused. Returns "finished", "wall contact" or "paused". Behaviour of src/trajectory.hpp:
+// 42-151: the CSV row of a step holds the state at its start and the velocity there; the
+// run stops before a step from a state with beta > beta_stop, or (budget_s > 0) before a
+// step whose projected end, the elapsed time plus 1.3 times the mean time per evaluation
+// for each evaluation of that step (4 during the RK4 start-up, else 1), passes the budget.
+inline std::string run_trajectory(const TrajectoryParams& P, const std::string& out_prefix, bool verbose = true) {
+    using detail::PathState;
+    Timer clock;
+    detail::PathRhs rhs(P);
+    const std::string ckpt_path = out_prefix + ".ckpt";
+    detail::PathCheckpoint run;
+    const bool resumed = run.read(ckpt_path);
+    if (resumed) {
+        if (verbose) std::printf("restart at t = %.3f (step %d)\n", run.t, run.step);
+    } else {
+        run.y = detail::initial_state(P, rhs.gap());
+    }
+    char params[256];
+    std::snprintf(params, sizeof params,
                   "alpha=%g a_over_R=%g beta0=%g pitch0_deg=%g yaw0_deg=%g dt=%g n_theta=%d sphere_level=%d", P.alpha,
                   P.a_over_R, P.beta0, P.pitch0_deg, P.yaw0_deg, P.dt, P.n_theta, P.sphere_level);
-    Csv out(csvp, "t,x,y,z,ex,ey,ez,Ux,Uy,Uz,Omx,Omy,Omz,beta", {info}, restart);
-    auto save = [&]() {
-        std::ofstream o(ck);
-        o.precision(17);
-        o << t << " " << step << " " << hist.size() << "\n";
-        for (double v : y) o << v << " ";
-        o << "\n";
-        for (auto& h : hist) {
-            for (double v : h) o << v << " ";
-            o << "\n";
-        }
-    };
+    Csv csv(out_prefix + ".csv", "t,x,y,z,ex,ey,ez,Ux,Uy,Uz,Omx,Omy,Omz,beta", {params}, resumed);
 
-    // ---- time loop
+    auto rate_of = [&](const PathState& y) { return rhs(y).rate; };
     std::string status = "finished";
-    while (t < P.t_max - 1e-9) {
-        if (beta_of(y) > P.beta_stop) {
+    for (; run.t < P.t_max - 1e-9; ++run.step, run.t += P.dt) {
+        const double beta = rhs.beta(run.y);
+        if (beta > P.beta_stop) {
             status = "wall contact";
             break;
         }
-        const double el = clock.seconds();
-        if (P.budget_s > 0 && nevals > 0 && el + 1.3 * el / nevals * (hist.size() < 3 ? 4 : 1) > P.budget_s) {
-            status = "paused";
-            break;
+        if (P.budget_s > 0 && rhs.calls() > 0) {
+            const double spent = clock.seconds(), per_call = spent / rhs.calls();
+            const int calls_next = run.history.size() < 3 ? 4 : 1;
+            if (spent + 1.3 * per_call * calls_next > P.budget_s) {
+                status = "paused";
+                break;
+            }
         }
-        const Eval f = evaluate(y);
-        out.row({t, y[0], y[1], y[2], y[3], y[4], y[5], f.U.x, f.U.y, f.U.z, f.Om.x, f.Om.y, f.Om.z, beta_of(y)});
-        if (hist.size() < 3) {  // RK4 start-up
-            auto axpy = [](const State& s, double h, const State& d) {
-                State o;
-                for (int i = 0; i < 6; ++i) o[i] = s[i] + h * d[i];
-                return o;
-            };
-            const Eval k2 = evaluate(axpy(y, P.dt / 2, f.d));
-            const Eval k3 = evaluate(axpy(y, P.dt / 2, k2.d));
-            const Eval k4 = evaluate(axpy(y, P.dt, k3.d));
-            for (int i = 0; i < 6; ++i) y[i] += P.dt / 6 * (f.d[i] + 2 * k2.d[i] + 2 * k3.d[i] + k4.d[i]);
-        } else {  // Adams-Bashforth 4
-            const State &f1 = hist[2], &f2 = hist[1], &f3 = hist[0];
-            for (int i = 0; i < 6; ++i) y[i] += P.dt / 24 * (55 * f.d[i] - 59 * f1[i] + 37 * f2[i] - 9 * f3[i]);
-        }
-        const double n = std::sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
-        for (int i = 3; i < 6; ++i) y[i] /= n;  // keep |e| = 1
-        hist.push_back(f.d);
-        if (hist.size() > 3) hist.erase(hist.begin());
-        t += P.dt;
-        ++step;
-        save();
+        const auto here = rhs(run.y);
+        const PathState& y = run.y;
+        csv.row({run.t, y[0], y[1], y[2], y[3], y[4], y[5], here.U.x, here.U.y, here.U.z, here.Om.x, here.Om.y,
+                 here.Om.z, beta});
+        run.y = detail::integrate_step(run.y, here.rate, run.history, P.dt, rate_of);
+        detail::renormalise(run.y);
+        run.history.push_back(here.rate);
+        if (run.history.size() > 3) run.history.erase(run.history.begin());
+        detail::PathCheckpoint next = run;  // the checkpoint after the step
+        next.t += P.dt;
+        ++next.step;
+        next.write(ckpt_path);
     }
-    if (verbose)
-        std::printf("%s: t = %.2f, beta = %.3f, z = %.2f  (%d solves, %.1f s, %.2f s/solve)\n", status.c_str(), t,
-                    beta_of(y), y[2], nevals, clock.seconds(), nevals ? clock.seconds() / nevals : 0.0);
-    if (status != "paused") std::remove(ck.c_str());
+    if (verbose) {
+        const int n = rhs.calls();
+        std::printf("%s: t = %.2f, beta = %.3f, z = %.2f  (%d solves, %.1f s, %.2f s/solve)\n", status.c_str(), run.t,
+                    rhs.beta(run.y), run.y[2], n, clock.seconds(), n ? clock.seconds() / n : 0.0);
+    }
+    if (status != "paused") std::remove(ckpt_path.c_str());
     return status;
 }
 
@@ -171,30 +250,21 @@ inline TrajectoryGradient trajectory_gradient(const TrajectoryParams& P, int n_s
     if (P.precision != SQBEM_FP64) throw std::runtime_error("trajectory_gradient: FP64 only");
     if (n_steps < 1 || ckpt_every < 1) throw std::runtime_error("trajectory_gradient: bad step counts");
     const double a = 1, R = a / P.a_over_R;
-    Tube tube(R, P.L_over_R * R, a, P.n_theta, P.n_r, 1.0, SQBEM_FP64);
-    const Mesh sph0 = icosphere(a, P.sphere_level);
-    int nevals = 0;
+    detail::PathRhs rhs(P);
 
     struct Eval {
         State s{}, d{};
         V3 U, Om;
         std::unique_ptr<SwimmerSolver> sw;  // kept for the adjoint sweep
     };
-    // the evaluation of run_trajectory (same arithmetic)
+    // the evaluation of run_trajectory (same right-hand side)
     auto evaluate = [&](const State& y, bool keep) {
         Eval r;
         r.s = y;
-        const V3 c{y[0], y[1], 0}, e = unit(V3{y[3], y[4], y[5]});
-        Mesh S = sph0;
-        S.translate(c);
-        auto sw = std::make_unique<SwimmerSolver>(&tube, S, c);
-        auto k = sw->solve_free({sw->slip(e, 1, 0), sw->slip(e, 0, 1)});
-        r.U = k[0].U + P.alpha * k[1].U;
-        r.Om = k[0].Om + P.alpha * k[1].Om;
-        const V3 de = cross(r.Om, e);
-        r.d = {r.U.x, r.U.y, r.U.z, de.x, de.y, de.z};
-        if (keep) r.sw = std::move(sw);
-        ++nevals;
+        const auto v = rhs(y, keep ? &r.sw : nullptr);
+        r.U = v.U;
+        r.Om = v.Om;
+        r.d = v.rate;
         return r;
     };
     // abar . df/ds at an evaluation, f(s) = (U, Omega x e), e = s_e / |s_e|, centre (s0, s1, 0):
@@ -285,7 +355,7 @@ inline TrajectoryGradient trajectory_gradient(const TrajectoryParams& P, int n_s
     G.beta_final = hyp / (R - a);
     G.tilt_final_deg = std::atan2(rho, ez) * 180 / PI;
     if (!adjoint) {
-        G.evaluations = nevals;
+        G.evaluations = rhs.calls();
         return G;
     }
     if (hyp == 0 || rho == 0) throw std::runtime_error("trajectory_gradient: final state on the axis");
@@ -356,7 +426,7 @@ inline TrajectoryGradient trajectory_gradient(const TrajectoryParams& P, int n_s
     G.dtilt_dbeta0 = -(R - a) * yb[1][0];
     G.dbeta_dpitch0 = dxi * std::cos(chi) * (std::cos(xi) * yb[0][3] - std::sin(xi) * yb[0][5]);
     G.dtilt_dpitch0 = dxi * std::cos(chi) * (std::cos(xi) * yb[1][3] - std::sin(xi) * yb[1][5]);
-    G.evaluations = nevals;
+    G.evaluations = rhs.calls();
     return G;
 }
 

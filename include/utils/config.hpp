@@ -1,10 +1,16 @@
-// config.hpp -- "key = value" input files with '#' comments; list values split on
-// whitespace or commas; command-line key=value overrides use the same line parser.
-// API of src/utils/config.hpp:19-70 (same error texts).
+// config.hpp -- "key = value" input files and command-line overrides.
+//
+// Interface of src/utils/config.hpp:19-70 (Config::load, parse_line, has, str, num, integer,
+// list, all), with its error texts ("cannot open input file: ...", "missing required
+// input: ..."). Semantics kept: text after '#' is a comment; a line without '=' is ignored;
+// key and value are trimmed of blanks; a later assignment of a key replaces an earlier one;
+// numbers are read from the start of the value (std::stod / std::stoi); a list is the run of
+// numbers at the start of the value, separated by blanks or commas.
 #pragma once
+#include <cctype>
+#include <cstdlib>
 #include <fstream>
 #include <map>
-#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -16,47 +22,69 @@ public:
     Config() = default;
 
     static Config load(const std::string& path) {
-        std::ifstream in(path);
-        if (!in) throw std::runtime_error("cannot open input file: " + path);
-        Config c;
-        std::string line;
-        while (std::getline(in, line)) c.parse_line(line);
-        return c;
+        std::ifstream file(path);
+        if (!file) throw std::runtime_error("cannot open input file: " + path);
+        Config cfg;
+        for (std::string text; std::getline(file, text);) cfg.parse_line(text);
+        return cfg;
     }
-    void parse_line(std::string line) {
-        const auto hash = line.find('#');
-        if (hash != std::string::npos) line.erase(hash);
-        const auto eq = line.find('=');
+
+    // one "key = value  # comment" assignment (an input-file line or a command-line override)
+    void parse_line(std::string text) {
+        text = text.substr(0, text.find('#'));
+        const size_t eq = text.find('=');
         if (eq == std::string::npos) return;
-        const std::string k = trim(line.substr(0, eq)), v = trim(line.substr(eq + 1));
-        if (!k.empty()) kv_[k] = v;
+        std::string key = blank_trimmed(text.substr(0, eq));
+        if (key.empty()) return;
+        values_[key] = blank_trimmed(text.substr(eq + 1));
     }
-    bool has(const std::string& k) const { return kv_.count(k) > 0; }
-    std::string str(const std::string& k, const std::string& def = "") const { return has(k) ? kv_.at(k) : def; }
-    double num(const std::string& k, double def) const { return has(k) ? std::stod(kv_.at(k)) : def; }
-    double num(const std::string& k) const {
-        if (!has(k)) throw std::runtime_error("missing required input: " + k);
-        return std::stod(kv_.at(k));
+
+    bool has(const std::string& key) const { return values_.find(key) != values_.end(); }
+
+    std::string str(const std::string& key, const std::string& fallback = "") const {
+        const auto it = values_.find(key);
+        return it == values_.end() ? fallback : it->second;
     }
-    int integer(const std::string& k, int def) const { return has(k) ? std::stoi(kv_.at(k)) : def; }
-    std::vector<double> list(const std::string& k, std::vector<double> def = {}) const {
-        if (!has(k)) return def;
-        std::string s = kv_.at(k);
-        for (char& ch : s)
-            if (ch == ',') ch = ' ';
-        std::istringstream is(s);
-        std::vector<double> v;
-        double x;
-        while (is >> x) v.push_back(x);
-        return v;
+    double num(const std::string& key, double fallback) const {
+        const auto it = values_.find(key);
+        return it == values_.end() ? fallback : std::stod(it->second);
     }
-    const std::map<std::string, std::string>& all() const { return kv_; }
+    double num(const std::string& key) const {
+        const auto it = values_.find(key);
+        if (it == values_.end()) throw std::runtime_error("missing required input: " + key);
+        return std::stod(it->second);
+    }
+    int integer(const std::string& key, int fallback) const {
+        const auto it = values_.find(key);
+        return it == values_.end() ? fallback : std::stoi(it->second);
+    }
+    std::vector<double> list(const std::string& key, std::vector<double> fallback = {}) const {
+        const auto it = values_.find(key);
+        if (it == values_.end()) return fallback;
+        std::vector<double> out;
+        const char* p = it->second.c_str();
+        for (;;) {
+            while (*p == ',' || std::isspace((unsigned char)*p)) ++p;
+            if (*p == '\0') break;
+            char* end = nullptr;
+            const double v = std::strtod(p, &end);
+            if (end == p) break;  // the numbers end at the first token that is not one
+            out.push_back(v);
+            p = end;
+        }
+        return out;
+    }
+    const std::map<std::string, std::string>& all() const { return values_; }
 
 private:
-    std::map<std::string, std::string> kv_;
-    static std::string trim(const std::string& s) {
-        const auto b = s.find_first_not_of(" \t\r\n"), e = s.find_last_not_of(" \t\r\n");
-        return b == std::string::npos ? "" : s.substr(b, e - b + 1);
+    std::map<std::string, std::string> values_;
+
+    static bool blank(char c) { return c == ' ' || c == '\t' || c == '\r' || c == '\n'; }
+    static std::string blank_trimmed(const std::string& s) {
+        size_t lo = 0, hi = s.size();
+        while (lo < hi && blank(s[lo])) ++lo;
+        while (hi > lo && blank(s[hi - 1])) --hi;
+        return s.substr(lo, hi - lo);
     }
 };
 

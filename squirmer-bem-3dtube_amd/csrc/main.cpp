@@ -1,19 +1,30 @@
-// main.cpp -- squirmer_bem: command-line driver over libsqbem (GPU).
+// main.cpp -- squirmer_bem, the command-line front end of libsqbem (one MI355X).
 //
 //   squirmer_bem <input-file> [key=value ...]
 //
-// Modes (input key `mode`), keys, defaults and output CSV schemas follow the reference
-// driver (src/main.cpp:27-201):
-//   duct        pressure-driven flow in a tube         -> flow_rate.csv, panels.csv, profile.csv
-//   kinematics  U, Omega per (a/R, beta) + towed drag   -> kinematics.csv
-//   field       velocity field in the plane y = 0       -> field_B1.csv, field_B2.csv, field_info.csv
-//   trajectory  time-integrated swimmer path            -> <name>.csv (+ <name>.ckpt while paused)
-// Extension key: precision = fp64 | fp32 | fp16 | bf16 (factor precision; refined modes
-// recover FP64 accuracy by iterative refinement). Default fp64.
+// The input file (and the key=value overrides after it, applied in order) selects a mode
+// and its parameters. Modes, keys, defaults, output files and their CSV schemas are those
+// of the reference driver (src/main.cpp:27-201); the console messages are this driver's.
+//
+//   duct        pressure-driven tube flow            flow_rate.csv, panels.csv, profile.csv
+//   kinematics  U, Omega per (a/R, beta), towed drag kinematics.csv
+//   field       velocity in the plane y = 0          field_B1.csv, field_B2.csv, field_info.csv
+//   trajectory  time-integrated path                 <name>.csv (<name>.ckpt while paused)
+//
+// Keys beyond the reference's: precision = fp64 | fp32 | fp16 | bf16 (factor precision;
+// the refined modes return FP64-accurate results by iterative refinement) and n_r (rings
+// per tube cap of the swimmer modes; the reference's Tube default is 4).
+//
+// Kinematics sweeps go through the batched solver: every configuration without a towed
+// problem is solved in one sqbem_batch_solve call (tubes shared by content, both squirming
+// modes as two instances of one configuration); the on-axis configurations whose towed drag
+// is asked for are solved one at a time, since the towed problem needs their solver.
 #include <cmath>
 #include <cstdio>
+#include <functional>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "duct_flow.hpp"
 #include "swimmer.hpp"
@@ -23,154 +34,224 @@
 
 using namespace bem;
 
-static int precision_of(const Config& c) {
-    const std::string p = c.str("precision", "fp64");
-    if (p == "fp64") return SQBEM_FP64;
-    if (p == "fp32") return SQBEM_FP32_IR;
-    if (p == "fp16") return SQBEM_FP16_IR;
-    if (p == "bf16") return SQBEM_BF16_IR;
-    throw std::runtime_error("unknown precision '" + p + "' (fp64|fp32|fp16|bf16)");
+namespace {
+
+int factor_precision(const Config& cfg) {
+    const std::string name = cfg.str("precision", "fp64");
+    const std::pair<const char*, int> known[] = {
+        {"fp64", SQBEM_FP64}, {"fp32", SQBEM_FP32_IR}, {"fp16", SQBEM_FP16_IR}, {"bf16", SQBEM_BF16_IR}};
+    for (const auto& k : known)
+        if (name == k.first) return k.second;
+    throw std::runtime_error("unknown precision '" + name + "' (fp64|fp32|fp16|bf16)");
 }
 
-static void run_duct(const Config& c, const std::string& out) {
-    const double R = c.num("R", 1.0), L = c.num("L", 4.0), mu = c.num("mu", 1.0);
-    const int nth = c.integer("n_theta", 24), nz = c.integer("n_z", 24), nr = c.integer("n_r", 6);
-    const double p_out = c.num("p_out", 0.0), dP_detail = c.num("dP_detail", 1.0);
-    const auto dPs = c.list("dP_list", {0.5, 1, 2, 4});
+// ---------------------------------------------------------------- duct
+void duct_mode(const Config& cfg, const std::string& out) {
+    const double R = cfg.num("R", 1.0), L = cfg.num("L", 4.0), mu = cfg.num("mu", 1.0);
+    const double p_out = cfg.num("p_out", 0.0), dP_detail = cfg.num("dP_detail", 1.0);
+    const int n_profile = cfg.integer("n_profile", 20);
     Timer clock;
-    DuctFlow duct(R, L, nth, nz, nr, mu);
-    std::printf("duct flow: %d panels, %d unknowns, assembled + factorised in %.1f s\n", duct.mesh.P(),
-                3 * duct.mesh.P(), clock.seconds());
-    Csv q(out + "/flow_rate.csv", "dP,Q,Q_exact,rel_err");
-    for (double dP : dPs) {
-        const auto s = duct.solve(p_out + dP, p_out);
-        const double Qe = duct.exact_Q(dP);
-        q.row({dP, s.Q, Qe, std::fabs(s.Q - Qe) / Qe});
-        std::printf("  dP = %-5g  Q = %.6f   exact %.6f   rel. error %.2e\n", dP, s.Q, Qe, std::fabs(s.Q - Qe) / Qe);
+    const DuctFlow duct(R, L, cfg.integer("n_theta", 24), cfg.integer("n_z", 24), cfg.integer("n_r", 6), mu);
+    std::printf("[duct] %d panels (%d unknowns) assembled and factored: %.2f s\n", duct.mesh.P(), 3 * duct.mesh.P(),
+                clock.seconds());
+
+    Csv rates(out + "/flow_rate.csv", "dP,Q,Q_exact,rel_err");
+    for (const double dP : cfg.list("dP_list", {0.5, 1, 2, 4})) {
+        const double Q = duct.solve(p_out + dP, p_out).Q, exact = duct.exact_Q(dP);
+        const double err = std::fabs(Q - exact) / exact;
+        rates.row({dP, Q, exact, err});
+        std::printf("[duct] dP %g: Q %.6f (Poiseuille %.6f, error %.1e)\n", dP, Q, exact, err);
     }
-    const auto s = duct.solve(p_out + dP_detail, p_out);
-    write_panels(out + "/panels.csv", duct.mesh, s.u, s.f);
-    const int nprof = c.integer("n_profile", 20);
+
+    const DuctFlow::Solution detail = duct.solve(p_out + dP_detail, p_out);
+    write_panels(out + "/panels.csv", duct.mesh, detail.u, detail.f);
+    // radial lines at three azimuths through the mid-plane z = L/2, out to 0.95 R
     std::vector<V3> pts;
-    for (double ang : {0.0, PI / 5, PI / 2})
-        for (int k = 0; k < nprof; ++k) {
-            const double r = 0.95 * R * k / (nprof - 1);
-            pts.push_back({r * std::cos(ang), r * std::sin(ang), L / 2});
+    for (const double phi : {0.0, PI / 5, PI / 2})
+        for (int k = 0; k < n_profile; ++k) {
+            const double r = 0.95 * R * k / (n_profile - 1);
+            pts.push_back(V3{r * std::cos(phi), r * std::sin(phi), L / 2});
         }
-    const auto v = duct.velocity(s, pts);
-    Csv pr(out + "/profile.csv", "x,y,z,r,ux,uy,uz,uz_exact",
-           {"interior velocity at z = L/2, dP = " + std::to_string(dP_detail)});
+    const std::vector<V3> vel = duct.velocity(detail, pts);
+    Csv profile(out + "/profile.csv", "x,y,z,r,ux,uy,uz,uz_exact",
+                {"interior velocity at z = L/2, dP = " + std::to_string(dP_detail)});
     for (size_t k = 0; k < pts.size(); ++k) {
         const double r = std::hypot(pts[k].x, pts[k].y);
-        pr.row({pts[k].x, pts[k].y, pts[k].z, r, v[k].x, v[k].y, v[k].z, duct.exact_uz(r, dP_detail)});
+        profile.row({pts[k].x, pts[k].y, pts[k].z, r, vel[k].x, vel[k].y, vel[k].z, duct.exact_uz(r, dP_detail)});
     }
-    std::printf("wrote %s/{flow_rate,panels,profile}.csv  (%.1f s)\n", out.c_str(), clock.seconds());
+    std::printf("[duct] %s/flow_rate.csv, panels.csv, profile.csv: %.2f s\n", out.c_str(), clock.seconds());
 }
 
-static void run_kinematics(const Config& c, const std::string& out) {
-    const double a = 1.0, U0 = 2.0 / 3;
-    const auto lams = c.list("a_over_R", {0.3});
-    const auto betas = c.list("beta", {0.0});
-    const int nth = c.integer("n_theta", 24), lev = c.integer("sphere_level", 3);
-    const double LoR = c.num("L_over_R", 4.0);
-    const bool towed = c.integer("towed_drag", 1) != 0;
-    const int nr = c.integer("n_r", 4);
-    const int prec = precision_of(c);
-    Csv k(out + "/kinematics.csv",
-          "a_over_R,beta,panels,UB1x,UB1y,UB1z,OB1x,OB1y,OB1z,UB2x,UB2y,UB2z,OB2x,OB2y,OB2z,drag_K,seconds",
-          {"U, Omega for B1=1 (neutral mode) and B2=1 (dipole mode); a=1, mu=1, e=z. "
-           "Any squirmer: U = U_B1 + alpha*U_B2. drag_K = axial drag/(6 pi mu a U), sphere on axis only. "
-           "a_over_R=0 means unbounded fluid."});
-    const V3 e{0, 0, 1};
-    for (double lam : lams) {
-        std::unique_ptr<Tube> tube;
-        if (lam > 0) tube = std::make_unique<Tube>(a / lam, LoR * a / lam, a, nth, nr, 1.0, prec);
-        for (double beta : betas) {
-            Timer clock;
-            const double b = lam > 0 ? beta * (a / lam - a) : 0.0;
-            SwimmerSolver s(tube.get(), icosphere(a, lev, {b, 0, 0}), {b, 0, 0}, 1.0, prec);
-            const auto m = s.solve_free({s.slip(e, 1, 0), s.slip(e, 0, 1)});
-            double K = NAN;
-            if (towed && std::fabs(b) < 1e-12) K = -s.force(s.solve_prescribed({0, 0, 1}, {})).z / (6 * PI);
-            k.row({lam, beta, (double)(s.Ns + s.Nt), m[0].U.x, m[0].U.y, m[0].U.z, m[0].Om.x, m[0].Om.y, m[0].Om.z,
-                   m[1].U.x, m[1].U.y, m[1].U.z, m[1].Om.x, m[1].Om.y, m[1].Om.z, K, clock.seconds()});
-            std::printf("a/R=%.2f beta=%.2f: U_B1/U0=(%+.4f,%+.4f,%.4f) Om_B1=(%+.4f,%+.4f,%+.4f) "
-                        "U_B2/U0=(%+.4f,%+.4f,%+.4f) K=%.4f [%.1fs]\n",
-                        lam, beta, m[0].U.x / U0, m[0].U.y / U0, m[0].U.z / U0, m[0].Om.x, m[0].Om.y, m[0].Om.z,
-                        m[1].U.x / U0, m[1].U.y / U0, m[1].U.z / U0, K, clock.seconds());
+// ---------------------------------------------------------------- kinematics
+struct KinematicsRow {
+    double a_over_R, beta;
+    int panels = 0;
+    V3 U[2], Om[2];  // squirming modes B1 = 1 and B2 = 1
+    double drag_K = NAN, seconds = 0;
+};
+
+void kinematics_mode(const Config& cfg, const std::string& out) {
+    const double a = 1.0;  // sphere radius: lengths in units of a, R = a / (a/R)
+    const std::vector<double> lams = cfg.list("a_over_R", {0.3}), betas = cfg.list("beta", {0.0});
+    const int n_theta = cfg.integer("n_theta", 24), level = cfg.integer("sphere_level", 3);
+    const int n_r = cfg.integer("n_r", 4), prec = factor_precision(cfg);
+    const double L_over_R = cfg.num("L_over_R", 4.0);
+    const bool want_drag = cfg.integer("towed_drag", 1) != 0;
+    const V3 ez{0, 0, 1};
+
+    std::vector<KinematicsRow> rows;
+    // batched configurations: meshes kept alive for the call
+    std::vector<std::unique_ptr<Mesh>> meshes;
+    std::vector<BatchConfig> batch;
+    std::vector<size_t> batch_row;  // row of each batched configuration (two entries per row)
+    for (const double lam : lams) {
+        const double R = lam > 0 ? a / lam : 0.0;
+        std::unique_ptr<Tube> tube;        // for the rows solved one at a time
+        const Mesh* tube_mesh = nullptr;   // for the batched rows
+        for (const double beta : betas) {
+            KinematicsRow row{lam, beta};
+            const double b = lam > 0 ? beta * (R - a) : 0.0;
+            const V3 c{b, 0, 0};
+            const bool towed = want_drag && std::fabs(b) < 1e-12;
+            if (lam > 0 && !towed) {
+                if (!tube_mesh) {
+                    meshes.push_back(std::make_unique<Mesh>(swimmer_tube_mesh(R, L_over_R * R, a, n_theta, n_r)));
+                    tube_mesh = meshes.back().get();
+                }
+                meshes.push_back(std::make_unique<Mesh>(icosphere(a, level, c)));
+                const Mesh* sphere = meshes.back().get();
+                row.panels = tube_mesh->P() + sphere->P();
+                for (int m = 0; m < 2; ++m) {
+                    batch.push_back({tube_mesh, sphere, c, ez, m == 0 ? 1.0 : 0.0, m == 0 ? 0.0 : 1.0});
+                    batch_row.push_back(rows.size());
+                }
+            } else {  // unbounded, or the towed problem is wanted: one solver for the row
+                Timer clock;
+                if (lam > 0 && !tube) tube = std::make_unique<Tube>(R, L_over_R * R, a, n_theta, n_r, 1.0, prec);
+                SwimmerSolver s(tube.get(), icosphere(a, level, c), c, 1.0, prec);
+                const auto m = s.solve_free({s.slip(ez, 1, 0), s.slip(ez, 0, 1)});
+                for (int k = 0; k < 2; ++k) {
+                    row.U[k] = m[k].U;
+                    row.Om[k] = m[k].Om;
+                }
+                if (towed) row.drag_K = -s.force(s.solve_prescribed(ez, V3{})).z / (6 * PI);
+                row.panels = s.Ns + s.Nt;
+                row.seconds = clock.seconds();
+            }
+            rows.push_back(row);
         }
     }
+    if (!batch.empty()) {
+        Timer clock;
+        const BatchResult r = batch_solve(batch, 1.0, prec);
+        const double share = clock.seconds() / (double)(batch.size() / 2);  // per configuration
+        for (size_t j = 0; j < batch.size(); ++j) {
+            KinematicsRow& row = rows[batch_row[j]];
+            const int m = batch[j].B1 != 0 ? 0 : 1;
+            row.U[m] = r.U[j];
+            row.Om[m] = r.Om[j];
+            row.seconds = share;
+        }
+        std::printf("[kinematics] %zu configurations in one batched solve: %.2f s\n", batch.size() / 2,
+                    clock.seconds());
+    }
+
+    Csv table(out + "/kinematics.csv",
+              "a_over_R,beta,panels,UB1x,UB1y,UB1z,OB1x,OB1y,OB1z,UB2x,UB2y,UB2z,OB2x,OB2y,OB2z,drag_K,seconds",
+              {"U, Omega for B1=1 (neutral mode) and B2=1 (dipole mode); a=1, mu=1, e=z. "
+               "Any squirmer: U = U_B1 + alpha*U_B2. drag_K = axial drag/(6 pi mu a U), sphere on axis only. "
+               "a_over_R=0 means unbounded fluid."});
+    const double U0 = 2.0 / 3.0;  // free-space speed of the B1 = 1 squirmer
+    for (const KinematicsRow& w : rows) {
+        table.row({w.a_over_R, w.beta, (double)w.panels, w.U[0].x, w.U[0].y, w.U[0].z, w.Om[0].x, w.Om[0].y,
+                   w.Om[0].z, w.U[1].x, w.U[1].y, w.U[1].z, w.Om[1].x, w.Om[1].y, w.Om[1].z, w.drag_K, w.seconds});
+        std::printf("[kinematics] a/R %.2f beta %.2f: U_B1/U0 %.5f, Om_B1 (%+.2e, %+.2e, %+.2e), U_B2/U0 %+.5f, "
+                    "K %.4f\n",
+                    w.a_over_R, w.beta, w.U[0].z / U0, w.Om[0].x, w.Om[0].y, w.Om[0].z, w.U[1].z / U0, w.drag_K);
+    }
 }
 
-static void run_field(const Config& c, const std::string& out) {
+// ---------------------------------------------------------------- field
+void field_mode(const Config& cfg, const std::string& out) {
     const double a = 1.0;
-    const double lam = c.num("a_over_R", 0.3), beta = c.num("beta", 0.0);
-    const int nth = c.integer("n_theta", 24), lev = c.integer("sphere_level", 3);
-    const int nx = c.integer("nx", 25), nz = c.integer("nz", 57);
-    const double zext = c.num("z_extent", 7.0);
-    const int nr = c.integer("n_r", 4);
-    const int prec = precision_of(c);
+    const double lam = cfg.num("a_over_R", 0.3), beta = cfg.num("beta", 0.0);
+    const int nx = cfg.integer("nx", 25), nz = cfg.integer("nz", 57);
+    const double z_extent = cfg.num("z_extent", 7.0);  // the grid spans |z| <= z_extent (units of a)
+    const int prec = factor_precision(cfg);
     Timer clock;
+    // confined: R = a / (a/R); unbounded: R only sets the grid's half-width x_extent / 0.97
+    const double R = lam > 0 ? a / lam : cfg.num("x_extent", 4.0) / 0.97;
     std::unique_ptr<Tube> tube;
-    const double R = lam > 0 ? a / lam : c.num("x_extent", 4.0) / 0.97;
-    if (lam > 0) tube = std::make_unique<Tube>(R, c.num("L_over_R", 4.0) * R, a, nth, nr, 1.0, prec);
+    if (lam > 0)
+        tube = std::make_unique<Tube>(R, cfg.num("L_over_R", 4.0) * R, a, cfg.integer("n_theta", 24),
+                                      cfg.integer("n_r", 4), 1.0, prec);
     const double b = lam > 0 ? beta * (R - a) : 0.0;
-    const V3 e{0, 0, 1};
-    SwimmerSolver s(tube.get(), icosphere(a, lev, {b, 0, 0}), {b, 0, 0}, 1.0, prec);
-    const auto sl1 = s.slip(e, 1, 0), sl2 = s.slip(e, 0, 1);
-    const auto m = s.solve_free({sl1, sl2});
-    std::vector<V3> pts, eval;
-    std::vector<int> inside;
+    const V3 c{b, 0, 0}, ez{0, 0, 1};
+    SwimmerSolver s(tube.get(), icosphere(a, cfg.integer("sphere_level", 3), c), c, 1.0, prec);
+    const std::vector<V3> slips[2] = {s.slip(ez, 1, 0), s.slip(ez, 0, 1)};
+    const auto modes = s.solve_free({slips[0], slips[1]});
+
+    // grid in the plane y = 0; points within 1.03 a of the centre are masked (NaN rows)
+    std::vector<V3> grid, fluid;
+    std::vector<bool> masked;
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < nz; ++j) {
-            const V3 p{-0.97 * R + 1.94 * R * i / (nx - 1), 0, -zext + 2 * zext * j / (nz - 1)};
-            inside.push_back(norm(p - V3{b, 0, 0}) < 1.03 * a ? 1 : 0);
-            pts.push_back(p);
+            const V3 p{-0.97 * R + 1.94 * R * i / (nx - 1), 0, -z_extent + 2 * z_extent * j / (nz - 1)};
+            const bool in = norm(p - c) < 1.03 * a;
+            grid.push_back(p);
+            masked.push_back(in);
+            if (!in) fluid.push_back(p);
         }
-    for (size_t k = 0; k < pts.size(); ++k)
-        if (!inside[k]) eval.push_back(pts[k]);
-    for (int mode = 0; mode < 2; ++mode) {
-        const auto v = s.velocity(m[mode], mode == 0 ? sl1 : sl2, eval);
-        Csv f(out + (mode == 0 ? "/field_B1.csv" : "/field_B2.csv"), "x,z,ux,uy,uz",
-              {"lab-frame velocity in the plane y=0 for B" + std::to_string(mode + 1) +
+    for (int m = 0; m < 2; ++m) {
+        const std::vector<V3> vel = s.velocity(modes[m], slips[m], fluid);
+        const V3& U = modes[m].U;
+        Csv f(out + (m == 0 ? "/field_B1.csv" : "/field_B2.csv"), "x,z,ux,uy,uz",
+              {"lab-frame velocity in the plane y=0 for B" + std::to_string(m + 1) +
                    "=1; points inside the sphere are NaN. a_over_R=" + std::to_string(lam) +
                    " beta=" + std::to_string(beta),
-               "swimmer U=(" + std::to_string(m[mode].U.x) + "," + std::to_string(m[mode].U.y) + "," +
-                   std::to_string(m[mode].U.z) + ")"});
-        size_t q = 0;
-        for (size_t k = 0; k < pts.size(); ++k) {
-            if (inside[k]) {
-                f.row({pts[k].x, pts[k].z, NAN, NAN, NAN});
+               "swimmer U=(" + std::to_string(U.x) + "," + std::to_string(U.y) + "," + std::to_string(U.z) + ")"});
+        size_t next = 0;
+        for (size_t k = 0; k < grid.size(); ++k) {
+            if (masked[k]) {
+                f.row({grid[k].x, grid[k].z, NAN, NAN, NAN});
             } else {
-                f.row({pts[k].x, pts[k].z, v[q].x, v[q].y, v[q].z});
-                ++q;
+                const V3& v = vel[next++];
+                f.row({grid[k].x, grid[k].z, v.x, v.y, v.z});
             }
         }
     }
-    Csv k(out + "/field_info.csv", "a_over_R,beta,R,UB1z,UB2z,OB1y");
-    k.row({lam, beta, lam > 0 ? R : 0.0, m[0].U.z, m[1].U.z, m[0].Om.y});
-    std::printf("field: %zu points x 2 modes, U_B1 = %.5f  (%.1f s)\n", eval.size(), m[0].U.z, clock.seconds());
+    Csv info(out + "/field_info.csv", "a_over_R,beta,R,UB1z,UB2z,OB1y");
+    info.row({lam, beta, lam > 0 ? R : 0.0, modes[0].U.z, modes[1].U.z, modes[0].Om.y});
+    std::printf("[field] %zu fluid points, 2 modes, U_B1 = %.5f: %.2f s\n", fluid.size(), modes[0].U.z,
+                clock.seconds());
 }
 
-static void run_traj(const Config& c, const std::string& out) {
-    TrajectoryParams P;
-    P.alpha = c.num("alpha", 0);
-    P.a_over_R = c.num("a_over_R", 0.3);
-    P.beta0 = c.num("beta0", 0.5);
-    P.pitch0_deg = c.num("pitch0_deg", 0);
-    P.yaw0_deg = c.num("yaw0_deg", 0);
-    P.dt = c.num("dt", 0.5);
-    P.t_max = c.num("t_max", 100);
-    P.L_over_R = c.num("L_over_R", 4);
-    P.n_theta = c.integer("n_theta", 30);
-    P.sphere_level = c.integer("sphere_level", 2);
-    P.beta_stop = c.num("beta_stop", 0.95);
-    P.budget_s = c.num("budget_s", 0);
-    P.n_r = c.integer("n_r", 4);
-    P.precision = precision_of(c);
+// ---------------------------------------------------------------- trajectory
+void trajectory_mode(const Config& cfg, const std::string& out) {
+    TrajectoryParams P;  // defaults of src/trajectory.hpp:23-36
+    P.alpha = cfg.num("alpha", P.alpha);
+    P.a_over_R = cfg.num("a_over_R", P.a_over_R);
+    P.beta0 = cfg.num("beta0", P.beta0);
+    P.pitch0_deg = cfg.num("pitch0_deg", P.pitch0_deg);
+    P.yaw0_deg = cfg.num("yaw0_deg", P.yaw0_deg);
+    P.dt = cfg.num("dt", P.dt);
+    P.t_max = cfg.num("t_max", P.t_max);
+    P.L_over_R = cfg.num("L_over_R", P.L_over_R);
+    P.n_theta = cfg.integer("n_theta", P.n_theta);
+    P.sphere_level = cfg.integer("sphere_level", P.sphere_level);
+    P.beta_stop = cfg.num("beta_stop", P.beta_stop);
+    P.budget_s = cfg.num("budget_s", P.budget_s);
+    P.n_r = cfg.integer("n_r", P.n_r);
+    P.precision = factor_precision(cfg);
     ensure_dir(out);
-    run_trajectory(P, out + "/" + c.str("name", "trajectory"));
+    run_trajectory(P, out + "/" + cfg.str("name", "trajectory"));
 }
+
+using ModeFn = std::function<void(const Config&, const std::string&)>;
+const std::pair<const char*, ModeFn> kModes[] = {
+    {"duct", duct_mode}, {"kinematics", kinematics_mode}, {"field", field_mode}, {"trajectory", trajectory_mode}};
+
+}  // namespace
 
 int main(int argc, char** argv) {
     if (argc < 2) {
@@ -178,19 +259,19 @@ int main(int argc, char** argv) {
         return 1;
     }
     try {
-        Config c = Config::load(argv[1]);
-        for (int i = 2; i < argc; ++i) c.parse_line(argv[i]);  // command-line overrides
-        const std::string mode = c.str("mode");
-        const std::string out = c.str("output_dir", "outputs/" + mode);
+        Config cfg = Config::load(argv[1]);
+        for (int i = 2; i < argc; ++i) cfg.parse_line(argv[i]);
+        const std::string mode = cfg.str("mode");
+        const std::string out = cfg.str("output_dir", "outputs/" + mode);
         ensure_dir(out);
-        if (mode == "duct") run_duct(c, out);
-        else if (mode == "kinematics") run_kinematics(c, out);
-        else if (mode == "field") run_field(c, out);
-        else if (mode == "trajectory") run_traj(c, out);
-        else throw std::runtime_error("unknown mode '" + mode + "' (duct|kinematics|field|trajectory)");
-    } catch (const std::exception& ex) {
-        std::fprintf(stderr, "error: %s\n", ex.what());
+        for (const auto& m : kModes)
+            if (mode == m.first) {
+                m.second(cfg, out);
+                return 0;
+            }
+        throw std::runtime_error("unknown mode '" + mode + "' (duct|kinematics|field|trajectory)");
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
         return 1;
     }
-    return 0;
 }

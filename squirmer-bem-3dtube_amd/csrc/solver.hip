@@ -370,9 +370,44 @@ void sq::tube_build(sqbem_tube* t) {
     t->t_factor = now_s() - t1;
 }
 
+// ------------------------------------------------------------------ sphere template
+// The sphere-sphere blocks of SwimmerSolver::build (src/swimmer.hpp:209-222) depend only on
+// the sphere's shape, not on where it sits: E (deflated SL/(8 pi mu)), the DL part of AuBs,
+// and the sphere panels' share of each row's stresslet sum. A trajectory moves one sphere
+// through a fixed tube, so these are assembled once and copied into every solver; only the
+// tube-sphere couplings, the completion diagonal and the Schur complement are rebuilt.
+struct sqbem_sphere {
+    bem::Mesh S;
+    double xc[3] = {0, 0, 0};
+    double mu = 1;
+    int Ns = 0, ns = 0;
+    DBuf<double> E, AuBs, dlsum;  // ns x ns, ns x ns, 9 Ns
+};
+
+static void sphere_template_build(sqbem_sphere* t) {
+    t->Ns = t->S.P();
+    t->ns = 3 * t->Ns;
+    const int Ns = t->Ns, ns = t->ns;
+    const double e8 = bem::EIGHT_PI;
+    DMesh dm;
+    dm.upload(t->S);
+    DBuf<int> self;
+    device_iota(self, Ns);
+    double Asum = 0;
+    for (int p = 0; p < Ns; ++p) Asum += t->S.area[p];
+    t->E.alloc((size_t)ns * ns);
+    t->AuBs.alloc((size_t)ns * ns);
+    t->dlsum.alloc((size_t)9 * Ns);
+    t->dlsum.zero();
+    assemble(dm, 0, Ns, dm.cen_xyz.p, Ns, self.p, e8 * t->mu, t->E.p, ns, e8, t->AuBs.p, ns, t->dlsum.p,
+             1.0 / (e8 * t->mu * std::sqrt(Asum)), dm.nrm_xyz.p);
+    sync();
+}
+
 // ------------------------------------------------------------------ swimmer
 struct sqbem_swimmer {
     const sqbem_tube* tube = nullptr;
+    const sqbem_sphere* tmpl = nullptr;  // sphere-sphere blocks to copy instead of assembling
     bem::Mesh S;
     DMesh dm;
     double xc[3] = {0, 0, 0};
@@ -431,9 +466,6 @@ static void swimmer_build(sqbem_swimmer* s) {
                  nullptr);
     }
     // sphere rows <- sphere sources: E (deflated, src/swimmer.hpp:222), AuBs, DL sums
-    double Asum = 0;
-    for (int p = 0; p < Ns; ++p) Asum += s->S.area[p];
-    const double defl = 1.0 / (e8 * mu * std::sqrt(Asum));
     s->AuBs.alloc((size_t)ns * ns);
     double* Edst;
     if (mx) {
@@ -443,8 +475,17 @@ static void swimmer_build(sqbem_swimmer* s) {
         s->G.alloc((size_t)ns * (ns + 6));
         Edst = s->G.p;
     }
-    assemble(s->dm, 0, Ns, s->dm.cen_xyz.p, Ns, self.p, e8 * mu, Edst, ns, e8, s->AuBs.p, ns, dlsum.p, defl,
-             s->dm.nrm_xyz.p);
+    if (s->tmpl) {  // translation-invariant: copied from the sphere's template
+        convert_matrix<double, double>(ns, ns, s->tmpl->E.p, ns, Edst, ns);
+        convert_matrix<double, double>(ns, ns, s->tmpl->AuBs.p, ns, s->AuBs.p, ns);
+        axpy((size_t)9 * Ns, 1.0, s->tmpl->dlsum.p, dlsum.p);
+    } else {
+        double Asum = 0;
+        for (int p = 0; p < Ns; ++p) Asum += s->S.area[p];
+        const double defl = 1.0 / (e8 * mu * std::sqrt(Asum));
+        assemble(s->dm, 0, Ns, s->dm.cen_xyz.p, Ns, self.p, e8 * mu, Edst, ns, e8, s->AuBs.p, ns, dlsum.p, defl,
+                 s->dm.nrm_xyz.p);
+    }
     // completed double layer (src/swimmer.hpp:218-220)
     hipLaunchKernelGGL(completion_diag_kernel, dim3(ceil_div(Ns, 256)), dim3(256), 0, ctx().s(), Ns, s->AuBs.p,
                        (long)ns, dlsum.p, e8, tb ? 0.0 : 1.0);
@@ -950,6 +991,53 @@ sqbem_swimmer* sqbem_swimmer_create(const sqbem_tube* tube, const sqbem_mesh* sp
             s->S = sphere->m;
             for (int c = 0; c < 3; ++c) s->xc[c] = center[c];
             s->mu = mu;
+            s->prec = precision;
+            swimmer_build(s);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        return s;
+    });
+}
+
+sqbem_sphere* sqbem_sphere_create(const sqbem_mesh* sphere, const double* center, double mu) {
+    return guarded_ptr<sqbem_sphere>([&] {
+        SQ_REQUIRE(sphere && sphere->m.P() > 0 && center, SQBEM_ERR_ARG, "sqbem_sphere_create: bad arguments");
+        auto* t = new sqbem_sphere;
+        try {
+            t->S = sphere->m;
+            for (int c = 0; c < 3; ++c) t->xc[c] = center[c];
+            t->mu = mu;
+            sphere_template_build(t);
+        } catch (...) {
+            delete t;
+            throw;
+        }
+        return t;
+    });
+}
+
+void sqbem_sphere_free(sqbem_sphere* t) { delete t; }
+
+sqbem_swimmer* sqbem_swimmer_create_moved(const sqbem_tube* tube, const sqbem_sphere* sph, const double* shift,
+                                          int precision) {
+    return guarded_ptr<sqbem_swimmer>([&] {
+        SQ_REQUIRE(sph && shift, SQBEM_ERR_ARG, "sqbem_swimmer_create_moved: bad arguments");
+        SQ_REQUIRE(precision >= SQBEM_FP64 && precision <= SQBEM_BF16_IR, SQBEM_ERR_ARG, "unknown precision");
+        SQ_REQUIRE(!tube || tube->prec == precision || !mixed(precision), SQBEM_ERR_ARG,
+                   "swimmer precision must match the tube's");
+        SQ_REQUIRE(!tube || mixed(tube->prec) == mixed(precision), SQBEM_ERR_ARG,
+                   "swimmer precision must match the tube's (FP64 vs refined)");
+        SQ_REQUIRE(!tube || tube->mu == sph->mu, SQBEM_ERR_ARG, "sphere template and tube differ in viscosity");
+        auto* s = new sqbem_swimmer;
+        try {
+            s->tube = tube;
+            s->tmpl = sph;
+            s->S = sph->S;
+            s->S.translate(bem::V3{shift[0], shift[1], shift[2]});
+            for (int c = 0; c < 3; ++c) s->xc[c] = sph->xc[c] + shift[c];
+            s->mu = sph->mu;
             s->prec = precision;
             swimmer_build(s);
         } catch (...) {

@@ -76,6 +76,9 @@ _SIGS = {
     "sqbem_tube_lu": (_vp, [_vp]),
     "sqbem_tube_free": (None, [_vp]),
     "sqbem_swimmer_create": (_vp, [_vp, _vp, _vp, _d, _i]),
+    "sqbem_sphere_create": (_vp, [_vp, _vp, _d]),
+    "sqbem_sphere_free": (None, [_vp]),
+    "sqbem_swimmer_create_moved": (_vp, [_vp, _vp, _vp, _i]),
     "sqbem_swimmer_solve_free": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
     "sqbem_swimmer_solve_prescribed": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "sqbem_swimmer_times": (_i, [_vp, _vp, _vp, _vp, _vp]),
@@ -475,22 +478,59 @@ def squirmer_slip(x, center, e, B1, B2):
     return (B1 + B2 * ct) * (ct * rh - e)
 
 
+class SphereBlocks:
+    """Translation-invariant sphere-sphere blocks of a sphere mesh centred at `center`,
+    assembled once on the device (an addition; see SwimmerSolver.moved)."""
+
+    def __init__(self, sphere, center, mu=1.0):
+        self.S = sphere
+        self.xc = np.asarray(center, dtype=np.float64).copy()
+        self.mu = mu
+        c = np.ascontiguousarray(self.xc)
+        self.h_ = _obj(lib().sqbem_sphere_create(sphere.h_, _ptr(c), mu))
+
+    def __del__(self):
+        try:
+            if getattr(self, "h_", None) and _lib is not None:
+                _lib.sqbem_sphere_free(self.h_)
+        except Exception:
+            pass
+
+
 class SwimmerSolver:
     """SwimmerSolver (src/swimmer.hpp:83-289) on the device: couplings, Schur complement
     against the tube factors, force- and torque-free bordered solve."""
 
-    def __init__(self, tube, sphere, center, mu=1.0, precision=None):
+    def __init__(self, tube, sphere, center, mu=1.0, precision=None, _moved=None):
         self.tube = tube
-        self.S = sphere
         self.xc = np.asarray(center, dtype=np.float64).copy()
         self.mu = mu
         self.precision = precision if precision is not None else (tube.precision if tube is not None else FP64)
-        self.Ns = sphere.P()
         self.Nt = tube.Nt if tube is not None else 0
-        c = np.ascontiguousarray(self.xc)
-        self.h_ = _obj(lib().sqbem_swimmer_create(tube.h_ if tube is not None else None, sphere.h_, _ptr(c), mu,
-                                                  self.precision))
+        if _moved is None:
+            self.S = sphere
+            c = np.ascontiguousarray(self.xc)
+            self.h_ = _obj(lib().sqbem_swimmer_create(tube.h_ if tube is not None else None, sphere.h_, _ptr(c), mu,
+                                                      self.precision))
+        else:
+            blocks, shift = _moved
+            self._blocks = blocks  # the template outlives the solver built from it
+            self.S = sphere
+            d = np.ascontiguousarray(shift, dtype=np.float64)
+            self.h_ = _obj(lib().sqbem_swimmer_create_moved(tube.h_ if tube is not None else None, blocks.h_, _ptr(d),
+                                                            self.precision))
+        self.Ns = self.S.P()
         self._times()
+
+    @classmethod
+    def moved(cls, tube, blocks, shift, precision=None):
+        """The sphere of `blocks` translated by `shift`, its sphere-sphere blocks copied from
+        the template (an addition; equals the plain constructor on the translated mesh to
+        rounding)."""
+        shift = np.asarray(shift, dtype=np.float64)
+        S = blocks.S.copy()
+        S.translate(shift)
+        return cls(tube, S, blocks.xc + shift, blocks.mu, precision, _moved=(blocks, shift))
 
     def _times(self):
         ta, ts, it, res = ctypes.c_double(), ctypes.c_double(), ctypes.c_int(), ctypes.c_double()

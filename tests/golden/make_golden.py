@@ -2,7 +2,9 @@
 reference checkout and its reference build exist):
 
   * reference output data (CSV files the reference repository holds, produced by its own
-    runs): copied verbatim, trajectories truncated to their first rows;
+    runs): copied verbatim, including all six trajectories and the second revision's run of
+    the neutral trajectory (outputs/trajectories/neutral.csv, as neutral_rev2.csv), which
+    shows how closely two runs of the reference agree;
   * vectors from the reference's own geometry/quadrature/kernel headers, compiled by
     oracle/Makefile into oracle/_ref/libref.so: meshes, rule levels and panel integrals for
     selected pairs of every rule, stored as .npz (no pickles).
@@ -31,7 +33,7 @@ CSV_FULL = [
     "outputs/duct_flow/panels.csv",
     "outputs/duct_flow/profile.csv",
 ]
-CSV_HEAD = [
+CSV_FULL += [
     "outputs/reference/trajectories/neutral.csv",
     "outputs/reference/trajectories/pusher.csv",
     "outputs/reference/trajectories/puller_a3.csv",
@@ -39,8 +41,8 @@ CSV_HEAD = [
     "outputs/reference/traj3d/neutral_yaw15.csv",
     "outputs/reference/traj3d/puller_a3_yaw15.csv",
 ]
+CSV_RENAMED = {"outputs/trajectories/neutral.csv": "neutral_rev2.csv"}
 INPUTS = "inputs"  # the reference's key=value input files, copied verbatim
-HEAD_ROWS = 12
 
 
 def copy_csvs():
@@ -48,18 +50,8 @@ def copy_csvs():
     os.makedirs(out, exist_ok=True)
     for rel in CSV_FULL:
         shutil.copyfile(os.path.join(REF, rel), os.path.join(out, os.path.basename(rel)))
-    for rel in CSV_HEAD:
-        with open(os.path.join(REF, rel)) as f:
-            lines = f.readlines()
-        keep, data = [], 0
-        for ln in lines:
-            if ln.startswith("#") or ln.startswith("t,"):
-                keep.append(ln)
-            elif data < HEAD_ROWS:
-                keep.append(ln)
-                data += 1
-        with open(os.path.join(out, os.path.basename(rel)), "w") as f:
-            f.writelines(keep)
+    for rel, name in CSV_RENAMED.items():
+        shutil.copyfile(os.path.join(REF, rel), os.path.join(out, name))
 
 
 def copy_inputs():

@@ -237,57 +237,72 @@ def test_cli_field_matches_field_csv(tmp_path):
 
 
 # ------------------------------------------------------------------ trajectories (trajectory.hpp)
-TRAJ = [  # golden file, input file, dt
-    ("neutral.csv", "trajectory_neutral.in", 0.5),
-    ("pusher.csv", "trajectory_pusher.in", 0.5),
-    ("puller_a3.csv", "trajectory_puller_a3.in", 1.0),
-    ("puller_a5.csv", "trajectory_puller_a5.in", 1.0),
-    ("neutral_yaw15.csv", "trajectory_neutral_3d.in", 1.0),
-    ("puller_a3_yaw15.csv", "trajectory_puller_a3_3d.in", 1.0),
+TRAJ = [  # golden file, input file, status the run ends with
+    ("neutral.csv", "trajectory_neutral.in", "finished"),
+    ("pusher.csv", "trajectory_pusher.in", "wall contact"),
+    ("puller_a3.csv", "trajectory_puller_a3.in", "finished"),
+    ("puller_a5.csv", "trajectory_puller_a5.in", "finished"),
+    ("neutral_yaw15.csv", "trajectory_neutral_3d.in", "finished"),
+    ("puller_a3_yaw15.csv", "trajectory_puller_a3_3d.in", "finished"),
 ]
 
 
-def _check_traj(head, got, g):
-    assert got.shape[1] == g.shape[1] and got.shape[0] >= g.shape[0]
-    got = got[: g.shape[0]]
-    assert np.allclose(got[:, 0], g[:, 0])
-    groups = {"x": ["x", "y", "z"], "e": ["ex", "ey", "ez"], "U": ["Ux", "Uy", "Uz"],
-              "Om": ["Omx", "Omy", "Omz"], "beta": ["beta"]}
-    for gname, cols in groups.items():
-        idx = [head.index(c) for c in cols]
-        scale = np.abs(g[:, idx]).max()
-        err = np.abs(got[:, idx] - g[:, idx])
-        # 10 printed digits; the integration carries the round-off of every earlier step
-        assert np.all(err <= 2 * RTOL * (np.abs(g[:, idx]) + scale)), (gname, (err / scale).max())
-
-
-@pytest.mark.parametrize("gname,inp,dt", TRAJ[:3])
-def test_trajectory_cli_reference_inputs(tmp_path, gname, inp, dt):
-    head, g = golden(gname)
-    out = tmp_path / "traj"
-    _run_cli([os.path.join(GOLDEN, "inputs", inp), f"output_dir={out}", f"t_max={dt * len(g)!r}"], tmp_path)
-    name = [ln.split("=")[1].split("#")[0].strip() for ln in open(os.path.join(GOLDEN, "inputs", inp))
-            if ln.strip().startswith("name")][0]
-    h2, got = read_csv(out / f"{name}.csv")
-    assert h2 == head
-    _check_traj(head, got, g)
-
-
-@pytest.mark.parametrize("gname,inp,dt", TRAJ[3:])
-def test_trajectory_python_binding(sq, tmp_path, gname, inp, dt):
-    head, g = golden(gname)
+def _input_keys(inp):
     cfg = {}
     for ln in open(os.path.join(GOLDEN, "inputs", inp)):
         ln = ln.split("#")[0].strip()
         if "=" in ln:
             k, v = (x.strip() for x in ln.split("=", 1))
             cfg[k] = v
+    return cfg
+
+
+def _check_traj(head, got, g):
+    """Every row of a whole reference trajectory. The file holds 10 digits (5e-10 of each
+    value); the reference's own two code revisions agree to 1.7e-10 of a column's largest
+    value over 300 steps (neutral.csv vs neutral_rev2.csv); rounding differences of the
+    evaluations (1e-13) are carried along the path by the integration, allowed to grow by
+    2e-12 of the column scale per step."""
+    assert got.shape == g.shape, (got.shape, g.shape)
+    assert np.allclose(got[:, 0], g[:, 0], rtol=0, atol=1e-9)
+    steps = np.arange(len(g))[:, None]
+    worst = {}
+    groups = {"x": ["x", "y", "z"], "e": ["ex", "ey", "ez"], "U": ["Ux", "Uy", "Uz"],
+              "Om": ["Omx", "Omy", "Omz"], "beta": ["beta"]}
+    for gname, cols in groups.items():
+        idx = [head.index(c) for c in cols]
+        scale = np.abs(g[:, idx]).max()
+        err = np.abs(got[:, idx] - g[:, idx])
+        tol = RTOL * np.abs(g[:, idx]) + (1e-9 + 2e-12 * steps) * scale
+        worst[gname] = float((err / scale).max())
+        assert np.all(err <= tol), (gname, int(np.argmax((err - tol).max(axis=1))), (err / scale).max())
+    print("largest difference / column scale:", worst, flush=True)
+
+
+@pytest.mark.parametrize("gname,inp,status", TRAJ[:3])
+def test_trajectory_cli_reference_inputs_whole_run(tmp_path, gname, inp, status):
+    """The reference's input files through squirmer_bem to their end (t_max or wall contact)."""
+    head, g = golden(gname)
+    out = tmp_path / "traj"
+    stdout = _run_cli([os.path.join(GOLDEN, "inputs", inp), f"output_dir={out}"], tmp_path)
+    assert f"{status}: t = " in stdout, stdout[-500:]
+    h2, got = read_csv(out / f"{_input_keys(inp)['name']}.csv")
+    assert h2 == head
+    _check_traj(head, got, g)
+    assert not (out / f"{_input_keys(inp)['name']}.ckpt").exists()
+
+
+@pytest.mark.parametrize("gname,inp,status", TRAJ[3:])
+def test_trajectory_python_binding_whole_run(sq, tmp_path, gname, inp, status):
+    head, g = golden(gname)
+    cfg = _input_keys(inp)
     P = sq.TrajectoryParams(alpha=float(cfg["alpha"]), a_over_R=float(cfg["a_over_R"]), beta0=float(cfg["beta0"]),
-                            pitch0_deg=float(cfg["pitch0_deg"]), yaw0_deg=float(cfg["yaw0_deg"]), dt=dt,
-                            t_max=dt * len(g), L_over_R=float(cfg["L_over_R"]), n_theta=int(cfg["n_theta"]),
-                            sphere_level=int(cfg["sphere_level"]), beta_stop=float(cfg["beta_stop"]))
-    status = sq.run_trajectory(P, str(tmp_path / "t"))
-    assert status == "finished"
+                            pitch0_deg=float(cfg["pitch0_deg"]), yaw0_deg=float(cfg["yaw0_deg"]),
+                            dt=float(cfg["dt"]), t_max=float(cfg["t_max"]), L_over_R=float(cfg["L_over_R"]),
+                            n_theta=int(cfg["n_theta"]), sphere_level=int(cfg["sphere_level"]),
+                            beta_stop=float(cfg["beta_stop"]))
+    assert sq.run_trajectory(P, str(tmp_path / "t")) == status
     h2, got = read_csv(tmp_path / "t.csv")
     assert h2 == head
     _check_traj(head, got, g)
+
