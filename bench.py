@@ -3,8 +3,9 @@
 
 One step = one cold solve of the whole hot path on the device: tube assembly + deflation
 + LU of T, coupling assembly, Schur complement, force/torque-free bordered solve for the
-two squirming modes, with FP64 iterative refinement in the mixed-precision modes. Inputs
-(meshes) are generated on the host before the timed region; all matrices live in HBM.
+two squirming modes, with FP64 iterative refinement in the mixed-precision modes. The
+host-side mesh generation of each solve is inside its timed step (it takes a few ms); all
+matrices live in HBM.
 
 Workload (46,470 DOF): tube R=1, L/R=4, n_theta=64, n_r=16, axially graded; icosphere level
 4 (5120 panels), a/R=0.4, centre offset beta=0.25 (b = beta (R - a)), B2/B1 = +3, e = +z.
@@ -13,6 +14,10 @@ Beside it (detail.gradient): the adjoint gradient of (U_z, Omega_x) with respect
 sphere centre, orientation and B2/B1 on the 9,750-DOF adjoint workload, in ms per solve,
 checked against central differences; and (detail.large_field) the 57,222-DOF solve in
 FP32 + FP64 refinement and in FP64 with the velocity on a masked 41 x 41 x 201 grid.
+
+Beside those (detail.small_configs): the 5,430- and 13,254-DOF configurations in FP64 and
+in the refined mode, next to the reference's own CPU solve of the same configurations
+(cpu_baseline: the reference built from source, timed on this host's cores).
 
 Prints one JSON line (rank 0). --gpus N > 1 exits with a message: this build runs on
 one GPU.
@@ -271,56 +276,118 @@ def batch_bench(pkg, n=256, seed=0, modes=(("fp32+ir", 1), ("fp16+ir", 2)), chec
     return out
 
 
-def cpu_baseline(W, pkg_mesh):
-    """Oracle (FP64, scipy/OpenBLAS) timed on a bounded sample of the same solve: the panel
-    integrals of a subset of rows of every block on the real 46k meshes (single thread, as
-    the reference assembles), and LAPACK/BLAS rates at n = 6000 applied to the algorithmic
-    flop counts of the dense phases."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-    import scipy.linalg as sla
+# the reference CPU path's own configurations beside the headline mesh (R = 1): the plain
+# FP64 kinematics case and the mixed-precision single solve's configuration
+SMALL_CFGS = {
+    "cfg1": dict(R=1.0, L=4.0, a=0.3, nth=24, nr=6, lev=2, center=(0.0, 0.0, 0.0), pitch=0.0, alpha=0.0),
+    "cfg2": dict(R=1.0, L=6.0, a=0.5, nth=32, nr=8, lev=3, center=(0.15, 0.0, 0.0), pitch=30.0, alpha=-3.0),
+}
+CPU_PROFILE = os.path.join(ROOT, "profiles", "cpu_reference_46k.json")
 
+
+def host_cores():
+    """Host threads this process may use: its CPU affinity, capped by OMP_NUM_THREADS."""
     try:
-        from threadpoolctl import threadpool_info
-        cores = max([p.get("num_threads", 1) for p in threadpool_info() if p.get("internal_api") == "openblas"] or [1])
-    except Exception:
-        cores = os.cpu_count() or 1
-    tm = O.swimmer_tube_mesh(W["R"], W["L"], W["a"], W["nth"], W["nr"])
-    sm = O.icosphere(W["a"], W["lev"], W["center"])
-    Nt, Ns = tm.P(), sm.P()
-    nt, ns = 3 * Nt, 3 * Ns
-    rows = 48
-    asm = 0.0
-    for (M, tgt, n_tgt, self_) in [(tm, tm, Nt, True), (sm, tm, Nt, False), (tm, sm, Ns, False), (sm, sm, Ns, True)]:
-        idx = np.linspace(0, n_tgt - 1, rows).astype(int)
-        x0 = tgt.centroid[idx]
-        ss = idx if self_ else None
-        t0 = time.perf_counter()
-        O.block(M, x0, self_src=ss, sl_div=O.EIGHT_PI, dl_div=O.EIGHT_PI if M is sm else None)
-        asm += (time.perf_counter() - t0) * n_tgt / rows
-    n = 6000
-    rng = np.random.default_rng(0)
-    A = rng.standard_normal((n, n)) + n * np.eye(n)
-    t0 = time.perf_counter()
-    lu = sla.lu_factor(A, check_finite=False)
-    r_lu = (2.0 / 3.0) * n ** 3 / (time.perf_counter() - t0)
-    Bm = rng.standard_normal((n, 3000))
-    t0 = time.perf_counter()
-    sla.lu_solve(lu, Bm, check_finite=False)
-    r_trsm = 2.0 * n * n * 3000 / (time.perf_counter() - t0)
-    X = rng.standard_normal((n, n))
-    t0 = time.perf_counter()
-    X @ A
-    r_gemm = 2.0 * n ** 3 / (time.perf_counter() - t0)
-    m = ns + 6
-    dense = ((2.0 / 3.0) * nt ** 3 + (2.0 / 3.0) * m ** 3) / r_lu + 2.0 * nt * nt * m / r_trsm + 2.0 * ns * nt * m / r_gemm
-    total = asm + dense
-    return {"value": 1.0 / total, "unit": "solves/s", "cores": int(cores), "kind": "port",
-            "sample": (f"oracle FP64 on the same meshes: panel integrals of {rows} rows per block extrapolated "
-                       f"over all rows (1 thread), OpenBLAS dgetrf/dgetrs/dgemm rates at n={n} "
-                       f"({r_lu / 1e9:.0f}/{r_trsm / 1e9:.0f}/{r_gemm / 1e9:.0f} GFLOP/s) applied to the "
-                       f"dense flop counts; est. {asm:.1f} s assembly + {dense:.1f} s dense per solve"),
-            "est_seconds_per_solve": total}
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    lim = os.environ.get("OMP_NUM_THREADS", "")
+    if lim.isdigit() and int(lim) > 0:
+        n = min(n, int(lim))
+    return max(n, 1)
+
+
+def reference_cold_solves(cfgs, progress=False):
+    """Cold FP64 solves by the reference's own code (its solver headers built from source,
+    oracle/_ref/libref_solver.so; OpenBLAS threads = host cores, assembly single-threaded
+    as the reference's is), phase by phase."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ref_solver as RS
+
+    cores = host_cores()
+    RS.blas_threads(cores)
+    runs = {}
+    for name, c in cfgs.items():
+        say = (lambda msg, n=name: print(f"[cpu reference {n}] {msg}", file=sys.stderr, flush=True)) if progress \
+            else None
+        runs[name] = RS.timed_cold_solve(c["R"], c["L"], c["a"], c["nth"], c["nr"], c["lev"], c["center"],
+                                         orient(math.radians(c["pitch"]), 0.0), progress=say)
+    return cores, RS.blas_config(), runs
+
+
+def cpu_full(W):
+    """The reference's full cold FP64 solve of the benchmark mesh and of the two small
+    configurations, timed phase by phase on this host (minutes): the record behind the
+    default run's CPU baseline (profiles/cpu_reference_46k.json)."""
+    cfgs = dict(SMALL_CFGS)
+    cfgs["46k"] = dict(R=W["R"], L=W["L"], a=W["a"], nth=W["nth"], nr=W["nr"], lev=W["lev"], center=W["center"],
+                       pitch=0.0, alpha=W["alpha"])
+    cores, blas, runs = reference_cold_solves(cfgs, progress=True)
+    return {"what": "reference FP64 cold solves (src/swimmer.hpp, built from source), phase by phase",
+            "cores": cores, "blas": blas, "runs": runs}
+
+
+def cpu_baseline(W, quick=False):
+    """The reference's CPU path timed on this host: one full cold FP64 solve of the workload
+    (46,470 DOF; about 70 s on a 16-thread MI355X host, 12 s of it the two single-threaded
+    assembly phases) and of the two small configurations, by the reference's own code, phase
+    by phase. quick: only the small configurations are timed now, and the 46k figure is the
+    recorded full solve (profiles/cpu_reference_46k.json, from bench.py --cpu-full) rescaled
+    by how fast this host runs the 13,254-DOF solve compared with that recording."""
+    cfgs = dict(SMALL_CFGS)
+    if not quick:
+        cfgs["46k"] = dict(R=W["R"], L=W["L"], a=W["a"], nth=W["nth"], nr=W["nr"], lev=W["lev"],
+                           center=W["center"], pitch=0.0, alpha=W["alpha"])
+    cores, blas, runs = reference_cold_solves(cfgs, progress=True)
+    out = {"unit": "solves/s", "cores": cores, "kind": "reference", "blas": blas,
+           "measured_now": {k: {kk: v for kk, v in r.items() if kk not in ("U", "Om")} for k, r in runs.items()}}
+    if not quick:
+        full = runs["46k"]
+        out.update(value=1.0 / full["total_s"], seconds_per_solve=full["total_s"],
+                   sample=(f"one full cold FP64 solve of the 46,470-DOF workload by the reference's own code "
+                           f"(src/swimmer.hpp built from source, OpenBLAS on {cores} threads, assembly on one), "
+                           f"timed now: {full['total_s']:.1f} s"))
+        return out
+    try:
+        prof = json.load(open(CPU_PROFILE))
+        full, rec2 = prof["runs"]["46k"], prof["runs"]["cfg2"]
+    except (OSError, ValueError, KeyError):
+        out.update(value=None, sample="no recorded 46k reference solve (run bench.py --cpu-full)")
+        return out
+    host = runs["cfg2"]["total_s"] / rec2["total_s"]
+    secs = full["total_s"] * host
+    out.update(value=1.0 / secs, seconds_per_solve=secs, host_speed_vs_record=host,
+               recorded_46k_phases={k: v for k, v in full.items() if k.endswith("_s")},
+               sample=(f"reference FP64 solves timed now: cfg1 {runs['cfg1']['total_s']:.2f} s, cfg2 "
+                       f"{runs['cfg2']['total_s']:.2f} s; the 46,470-DOF solve recorded at {full['total_s']:.1f} s "
+                       f"(bench.py --cpu-full), x {host:.3f} for this host"))
+    return out
+
+
+def small_cfg_gpu(pkg, prec, reps=3):
+    """The two small configurations on the GPU, cold solves (best of reps), in FP64 and in
+    the headline's refined mode, beside the reference's CPU times of the same solves."""
+    out = {}
+    for name, c in SMALL_CFGS.items():
+        W = dict(R=c["R"], L=c["L"], a=c["a"], nth=c["nth"], nr=c["nr"], lev=c["lev"], center=c["center"],
+                 alpha=c["alpha"])
+        e = orient(math.radians(c["pitch"]), 0.0)
+        res = {}
+        for mode, p in (("fp64", 0), ("refined", prec)):
+            best = None
+            for _ in range(reps):
+                U, Om, fs, ph = cold_solve(pkg, W, p, e)
+                if best is None or ph["total_s"] < best[3]["total_s"]:
+                    best = (U, Om, fs, ph)
+            res[mode] = best
+        ref = np.r_[res["fp64"][0], res["fp64"][1]]
+        got = np.r_[res["refined"][0], res["refined"][1]]
+        out[name] = {"dof": 3 * (res["fp64"][3]["Nt"] + res["fp64"][3]["Ns"]) + 6,
+                     "fp64_s": res["fp64"][3]["total_s"], "refined_s": res["refined"][3]["total_s"],
+                     "refined_ir_iters": res["refined"][3]["ir_iters"],
+                     # (U, Omega) together: the on-axis case's Omega vanishes by symmetry
+                     "rel_err_U_Omega": float(np.linalg.norm(got - ref) / np.linalg.norm(ref))}
+    return out
 
 
 def main():
@@ -350,6 +417,12 @@ def main():
     ap.add_argument("--tune-save", default="", metavar="PATH", help="write the tile table after the run")
     ap.add_argument("--calib-copy-gib", type=float, default=0.0, metavar="GIB",
                     help="also run one plain device-to-device copy of GIB GiB (reference kernel for HBM counters)")
+    ap.add_argument("--cpu-quick", action="store_true",
+                    help="CPU baseline from the recorded full reference solve, rescaled by a 13,254-DOF solve "
+                         "timed now (seconds instead of about a minute)")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="only time the reference's full CPU solve of the workload and the two small "
+                         "configurations (minutes; the record behind the CPU baseline) and print it")
     ap.add_argument("--traj-grad", type=int, default=0, metavar="STEPS",
                     help="also run the trajectory adjoint over STEPS steps (2000 for the full case; minutes)")
     args = ap.parse_args()
@@ -357,6 +430,10 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is described in DESIGN.md, not implemented)"}))
         sys.exit(2)
+
+    if args.cpu_full:
+        print(json.dumps(cpu_full(workload(args))), flush=True)
+        return
 
     import __graft_entry__ as ge
 
@@ -475,10 +552,18 @@ def main():
         except Exception as ex:
             detail["large_field"] = {"error": str(ex)}
 
+    try:
+        detail["small_configs"] = small_cfg_gpu(pkg, prec)
+    except Exception as ex:
+        detail["small_configs"] = {"error": str(ex)}
+
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(W, None)
+            cpu = cpu_baseline(W, quick=args.cpu_quick)
+            for name, g in detail.get("small_configs", {}).items():
+                if isinstance(g, dict) and name in cpu.get("measured_now", {}):
+                    g["cpu_reference_s"] = cpu["measured_now"][name]["total_s"]
         except Exception as ex:  # the baseline must not hide the GPU result
             cpu = {"error": str(ex)}
 

@@ -179,3 +179,28 @@ def duct(R, L, n_theta, n_z, n_r, mu, p_in, p_out):
     _check(lib().ref_duct(R, L, n_theta, n_z, n_r, mu, p_in, p_out, _p(u), _p(f), ctypes.byref(Q), ctypes.byref(npan)))
     assert npan.value == P
     return u, f, Q.value
+
+
+def timed_cold_solve(R, L, a, n_theta, n_r, level, center, e, modes=((1.0, 0.0), (0.0, 1.0)), progress=None):
+    """One cold FP64 solve by the reference's own code, phase by phase as its t_assembly /
+    t_schur split them (src/swimmer.hpp:181, 235, 250) plus the tube's two halves: seconds of
+    tube assembly, tube dgetrf, coupling assembly, Schur complement (dgetrs with 3 Ns + 6
+    right-hand sides and D Y), bordered solve (solve_free: dgesv of the (3 Ns + 6) system
+    and the unpacking), and the U, Omega of each mode."""
+    import time
+
+    say = progress or (lambda msg: None)
+    t0 = time.perf_counter()
+    tube = Tube(R, L, a, n_theta, n_r, split=True)
+    say(f"tube {tube.Nt} panels: assembly {tube.t_assembly:.1f} s, dgetrf {tube.t_factor:.1f} s")
+    s = SwimmerSolver(tube, a, level, center)
+    say(f"swimmer {s.Ns} panels: coupling assembly {s.t_assembly:.1f} s, Schur {s.t_schur:.1f} s")
+    sol = s.solve_free(e, list(modes))
+    total = time.perf_counter() - t0
+    say(f"bordered solve {s.t_solve:.1f} s, total {total:.1f} s")
+    out = {"Nt": tube.Nt, "Ns": s.Ns, "dof": 3 * (tube.Nt + s.Ns) + 6,
+           "tube_assembly_s": tube.t_assembly, "tube_factor_s": tube.t_factor,
+           "coupling_assembly_s": s.t_assembly, "schur_s": s.t_schur, "bordered_solve_s": s.t_solve,
+           "total_s": total, "U": [q.U.tolist() for q in sol], "Om": [q.Om.tolist() for q in sol]}
+    del s, tube
+    return out

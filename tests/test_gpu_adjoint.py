@@ -213,3 +213,53 @@ def test_trajectory_gradient_fd(sq):
     for i in range(2):
         scale = np.max(np.abs(fd[i]))
         assert np.max(np.abs(adj[i] - fd[i])) <= RTOL * scale, (i, adj[i], fd[i])
+
+
+def reference_value_fn(kind, R, L, a, nth, nr, level):
+    """(x_c, pitch, yaw, alpha) -> (U_z, Omega_x) by the reference's own FP64 solver (built
+    from source, oracle/_ref/libref_solver.so) or by the oracle's restatement of it."""
+    if kind == "reference":
+        import ref_solver as RS
+
+        if not RS.available():
+            pytest.skip("reference solver build absent (make -C oracle refsolver)")
+        tube = RS.Tube(R, L, a, nth, nr) if nth else None
+
+        def value(p):
+            s = RS.SwimmerSolver(tube, a, level, p[:3])
+            sol = s.solve_free(orient(p[3], p[4]), [(1.0, p[5])])[0]
+            return np.array([sol.U[2], sol.Om[0]])
+    else:
+        import oracle as O
+
+        tube = O.Tube(R, L, a, nth, nr) if nth else None
+
+        def value(p):
+            s = O.SwimmerSolver(tube, O.icosphere(a, level, p[:3]), p[:3])
+            sol = s.solve_free([s.slip(orient(p[3], p[4]), 1.0, p[5])])[0]
+            return np.array([sol.U[2], sol.Om[0]])
+    return value
+
+
+@pytest.mark.parametrize("kind", ["reference", "oracle"])
+@pytest.mark.parametrize("confined", [True, False])
+def test_swimmer_gradient_vs_fd_of_reference_solver(sq, kind, confined):
+    """The GPU adjoint against central differences (h = 1e-4) of the reference's FP64 solver
+    (the reference's own headers built from source) and of the oracle: confined with far
+    tube-sphere pairs only (no rule can change within the step), and unbounded."""
+    R, L, a, nth, nr, level = 1.0, 4.0, 0.15, (24 if confined else 0), 4, 1
+    tube = sq.Tube(R, L, a, nth, nr) if confined else None
+    model = Model(sq, tube, a, level)
+    p = np.array([0.09, -0.05, 0.07, 0.35, 0.2, -1.5] if confined else [0.0, 0.0, 0.0, 0.5, -0.2, 1.0])
+    assert model.no_level_change(p[:3])
+    _, jac = model.value_and_jacobian(p)
+    value = reference_value_fn(kind, R, L, a, nth, nr, level)
+    fd = np.zeros((2, 6))
+    for k in range(6):
+        dp = np.zeros(6)
+        dp[k] = H
+        fd[:, k] = (value(p + dp) - value(p - dp)) / (2 * H)
+    print(kind, "adjoint\n", jac, "\nfinite differences\n", fd, flush=True)
+    for i in range(2):
+        scale = np.max(np.abs(fd[i]))
+        assert np.max(np.abs(jac[i] - fd[i])) <= RTOL * scale + (0 if confined else 1e-10), (i, jac[i], fd[i])

@@ -216,6 +216,29 @@ def test_cli_kinematics_reference_input(tmp_path):
     assert np.isfinite(K[0]) and np.all(np.isnan(K[1:]))
 
 
+@pytest.mark.parametrize("towed", [1, 0])
+def test_cli_kinematics_axis_sweep(tmp_path, towed):
+    """The a/R sweep of axis.csv (level 3, n_theta = 24, L = 4R, on the axis) through
+    squirmer_bem's kinematics mode. towed_drag = 1: every row needs its towed problem, so
+    each configuration is solved by its own SwimmerSolver; towed_drag = 0: all six are one
+    batched solve (sqbem_batch_solve, both modes as instances), drag_K NaN."""
+    out = tmp_path / "kin"
+    _run_cli([os.path.join(GOLDEN, "inputs", "squirmer_kinematics.in"), f"output_dir={out}",
+              "a_over_R=0.1 0.2 0.3 0.4 0.5 0.6", "beta=0", "n_theta=24", f"towed_drag={towed}"], tmp_path)
+    hk, k = read_csv(out / "kinematics.csv")
+    head, g = golden("axis.csv")
+    assert np.allclose(col(hk, k, "a_over_R"), col(head, g, "a_over_R"))
+    ok, d = close(col(hk, k, "UB1z"), col(head, g, "U_B1"))
+    assert ok, d
+    UB1 = np.abs(col(head, g, "U_B1"))
+    assert np.all(np.abs(col(hk, k, "UB2z") - col(head, g, "U_B2")) <= 2e-8 * UB1)
+    if towed:
+        ok, d = close(col(hk, k, "drag_K"), col(head, g, "drag_K"))
+        assert ok, d
+    else:
+        assert np.all(np.isnan(col(hk, k, "drag_K")))
+
+
 def test_cli_field_matches_field_csv(tmp_path):
     """inputs/squirmer_field.in with the grid of field.csv (z_extent = 2.2 R)."""
     out = tmp_path / "field"

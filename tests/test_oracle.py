@@ -299,3 +299,51 @@ def test_duct_flow_golden(O):
     ur = np.array([[r["ux"], r["uy"], r["uz"]] for r in rows])
     assert np.abs(f - fr).max() <= 2e-9 * np.abs(fr).max()
     assert np.abs(u - ur).max() <= 2e-9 * np.abs(ur).max()
+
+
+# the benchmark configurations' meshes (R = 1): (tube args, sphere level, a, centre)
+BENCH_MESHES = {
+    "cfg1": ((1.0, 4.0, 0.3, 24, 6), 2, 0.3, (0.0, 0.0, 0.0)),
+    "cfg2": ((1.0, 6.0, 0.5, 32, 8), 3, 0.5, (0.15, 0.0, 0.0)),
+    "cfg4": ((1.0, 8.0, 0.3, 24, 6), 3, 0.3, (0.14, 0.0, 0.0)),
+    "46k": ((1.0, 4.0, 0.4, 64, 16), 4, 0.4, (0.15, 0.0, 0.0)),
+    "cfg5": ((1.0, 8.0, 0.4, 64, 16), 4, 0.4, (0.15, 0.0, 0.0)),
+}
+
+
+def min_threshold_margin(O, M, targets, self_src=None, chunk=512):
+    """Smallest |d / threshold - 1| over all (target, panel) pairs, d = |c_p - x0| / h_p,
+    thresholds 0.75, 1.5, 3 (src/utils/kernels.hpp:74-75); self pairs excluded."""
+    best = np.inf
+    for i0 in range(0, len(targets), chunk):
+        x0 = targets[i0:i0 + chunk]
+        ss = None if self_src is None else self_src[i0:i0 + chunk]
+        _, dist = O.levels(M, x0, self_src=ss)
+        if self_src is not None:
+            dist[np.arange(len(x0)), ss] = np.inf
+        for thr in (0.75, 1.5, 3.0):
+            best = min(best, float(np.abs(dist / thr - 1).min()))
+    return best
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("name", list(BENCH_MESHES))
+def test_rule_threshold_margins_on_benchmark_meshes(O, name):
+    """The rule choice is discontinuous, and the reference's build contracts the distance
+    |c_p - x0| into FMAs while the GPU evaluates it without contraction. On every block the
+    benchmark configurations assemble (tube-tube, tube rows x sphere panels, sphere rows x
+    tube panels, sphere-sphere), every pair's distance ratio sits farther from each threshold
+    than rounding can move it (1e-12 relative; a rounding difference is ~1e-16), so both
+    builds pick the same rule for every pair."""
+    targs, lev, a, c = BENCH_MESHES[name]
+    tm = O.swimmer_tube_mesh(*targs)
+    sm = O.icosphere(a, lev, c)
+    margins = {
+        "tube-tube": min_threshold_margin(O, tm, tm.centroid, self_src=np.arange(tm.P(), dtype=np.int32)),
+        "tube rows x sphere panels": min_threshold_margin(O, sm, tm.centroid),
+        "sphere rows x tube panels": min_threshold_margin(O, tm, sm.centroid),
+        "sphere-sphere": min_threshold_margin(O, sm, sm.centroid, self_src=np.arange(sm.P(), dtype=np.int32)),
+    }
+    print(name, margins)
+    for block, m in margins.items():
+        assert m > 1e-12, (block, m)
