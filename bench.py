@@ -44,6 +44,13 @@ for _c, (_tm, _tn, _pf) in enumerate([(128, 128, 1), (128, 256, 1), (256, 128, 1
     _k = "gemm16q_" + ["128x128", "128x256", "256x128", "256x256", "128x128_pf2", "128x256_pf2", "256x128_pf2"][_c]
     PEAKS[_k] = 2.5e15
     KERNEL_NAMES[_k] = "gemm16q_kernel<{t}, %d, %d, %d>" % (_tm, _tn, _pf)
+for _name, (_tm, _tn, _kt, _ns) in [("256x256_dma32x4", (256, 256, 32, 4)), ("256x128_dma32x4", (256, 128, 32, 4)),
+                                    ("128x128_dma64x4", (128, 128, 64, 4))]:
+    PEAKS["gemm16q_" + _name] = 2.5e15
+    KERNEL_NAMES["gemm16q_" + _name] = "gemm16d_kernel<{t}, %d, %d, %d, %d>" % (_tm, _tn, _kt, _ns)
+for _a in range(4):
+    PEAKS["gemm16q_hipblaslt%d" % _a] = 2.5e15
+    KERNEL_NAMES["gemm16q_hipblaslt%d" % _a] = "hipBLASLt BF16 GEMM, heuristic algorithm %d" % _a
 TIMED = list(KERNEL_NAMES)
 MODES = {"fp64": 0, "fp32": 1, "fp16": 2, "bf16": 3}
 
@@ -481,6 +488,7 @@ def main():
     pkg.sync()
     profiled_ms = 1000.0 * (time.perf_counter() - t1)
     prof = {k: pkg.profile_get(k) for k in pkg.PROF_CLASSES}
+    named = pkg.profile_named()
     pkg.profile(False)
     U, Om, fs, phases = runs[-1]
     ms_per_step = 1000.0 * elapsed / args.steps
@@ -500,16 +508,22 @@ def main():
     detail["kernels"] = {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
                              "tflops": (v["flops"] / (v["ms"] * 1e-3) / 1e12) if v["ms"] > 0 else 0.0}
                          for k, v in prof.items() if v["launches"]}
+    detail["library_gemm_kernels"] = {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
+                                          "tflops": (v["flops"] / (v["ms"] * 1e-3) / 1e12) if v["ms"] > 0 else 0.0}
+                                      for k, v in named.items()}
 
-    # dominant kernel: the single kernel (one template instance, as rocprofv3 lists it) with
-    # the most time in the timed steps
-    kname = max((k for k in TIMED if k != "gemm16q"), key=lambda k: prof[k]["ms"])
-    kp = prof[kname]
+    # dominant kernel: the single kernel (one template instance, or one of the library GEMM's
+    # kernels, as rocprofv3 lists them) with the most time in the profiled step
+    tname = "true" if prec == 3 else "false"
+    cands = {KERNEL_NAMES[k].format(t=tname): (prof[k], PEAKS[k]) for k in TIMED
+             if k != "gemm16q" and not k.startswith("gemm16q_hipblaslt")}
+    cands.update({name: (v, 2.5e15) for name, v in named.items()})
+    kernel = max(cands, key=lambda k: cands[k][0]["ms"])
+    kp, peak = cands[kernel]
     achieved = kp["flops"] / (kp["ms"] * 1e-3) if kp["ms"] > 0 else 0.0
     nl = max(kp["launches"], 1)
-    tname = "true" if prec == 3 else "false"
-    roofline = {"bound": "mfma", "kernel": KERNEL_NAMES[kname].format(t=tname), "achieved": achieved / 1e12,
-                "peak": PEAKS[kname] / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAKS[kname], "traffic": None,
+    roofline = {"bound": "mfma", "kernel": kernel, "achieved": achieved / 1e12,
+                "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
                 "launches": kp["launches"], "avg_launch_ms": kp["ms"] / nl, "flops_per_launch": kp["flops"] / nl,
                 "algorithmic_bytes_per_launch": kp["bytes"] / nl}
     traffic = hbm_traffic(roofline["kernel"])

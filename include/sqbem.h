@@ -62,12 +62,20 @@ int sqbem_trim(double* released_gb);
 #define SQBEM_PROF_LASWP 7
 #define SQBEM_PROF_GEMM16Q 8 /* 16-bit GEMM on pre-converted, padded operands (large shapes) */
 #define SQBEM_PROF_TRSV 9    /* triangular solves with at most 8 right-hand sides (one launch each) */
-/* gemm16q launches by autotuner candidate c = 0..6 (each also counted in SQBEM_PROF_GEMM16Q):
+/* gemm16q launches by autotuner candidate c = 0..13 (each also counted in SQBEM_PROF_GEMM16Q):
  * workgroup tiles 128x128, 128x256, 256x128, 256x256 with loads staged one k tile ahead
- * (c = 0..3), and 128x128, 128x256, 256x128 staged two k tiles ahead (c = 4..6) */
+ * (c = 0..3), 128x128, 128x256, 256x128 staged two k tiles ahead (c = 4..6), and operand
+ * tiles copied into LDS by DMA: 256x256 (7) and 256x128 (8) with four 32-deep stages,
+ * 128x128 with four 64-deep stages (9), and hipBLASLt's BF16 GEMM on the same operands
+ * with the first four algorithms of its heuristic for the shape (10-13; BF16 only, FP16 runs
+ * candidate 3 in their place) */
 #define SQBEM_PROF_G16Q(c) (10 + (c))
 int sqbem_profile(int mask);
 int sqbem_profile_get(int kernel_class, long* launches, double* flops, double* bytes, double* ms);
+/* the same totals per kernel name for launches whose kernel the library names (the library
+ * GEMM candidate, whose kernels hipBLASLt chooses per shape): one line per kernel,
+ * "name\tlaunches\tflops\tbytes\tms\n", into buf (len bytes) */
+int sqbem_profile_named(char* buf, int len);
 
 /* ------------------------------------------------------------------ meshes (host)
  * Generators with the reference's node/panel order, src/utils/mesh.hpp:45-247. */
@@ -136,11 +144,11 @@ int sqbem_sgetrf(int kind, int n, float* A, int lda, int* ipiv, int* info);
 int sqbem_sgetrs(int n, int nrhs, const float* LU, int lda, const int* ipiv, float* B, int ldb);
 
 /* Tile autotuner of the 16-bit GEMMs (no reference counterpart): mode 1 (the default)
- * learns the fastest of seven candidates (the SQBEM_PROF_G16Q list: workgroup tile and
+ * learns the fastest of fourteen candidates (the SQBEM_PROF_G16Q list: workgroup tile and
  * load staging depth) per shape class (operand type, binary exponents of m, n, k) by
  * timing each candidate once on live calls; mode 0 uses the fixed rule (256 x 256 tiles
  * when they fill the device, else 128 x 128). Either call clears what was learned.
- * sqbem_tune_info: number of shape classes seen, how many are decided, and counts[7] of
+ * sqbem_tune_info: number of shape classes seen, how many are decided, and counts[14] of
  * decided classes per candidate; any pointer may be NULL. */
 int sqbem_tune(int mode);
 int sqbem_tune_info(int* classes, int* decided, int* counts);

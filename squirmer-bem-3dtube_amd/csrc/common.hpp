@@ -117,15 +117,17 @@ inline void sync() {
 // one launch on the library stream together with its algorithmic flops and bytes.
 enum ProfClass { PROF_DGEMM = 0, PROF_SGEMM, PROF_GEMM16, PROF_PANEL, PROF_TRSM, PROF_ASSEMBLE, PROF_GEMV, PROF_LASWP,
                  PROF_GEMM16Q, PROF_TRSV,
-                 // gemm16q by autotuner candidate 0..6 (each launch also counts in PROF_GEMM16Q)
+                 // gemm16q by autotuner candidate 0..13 (each launch also counts in PROF_GEMM16Q)
                  PROF_G16Q_C0, PROF_G16Q_C1, PROF_G16Q_C2, PROF_G16Q_C3, PROF_G16Q_C4, PROF_G16Q_C5, PROF_G16Q_C6,
-                 PROF_NCLASS };
+                 PROF_G16Q_C7, PROF_G16Q_C8, PROF_G16Q_C9, PROF_G16Q_C10, PROF_G16Q_C11, PROF_G16Q_C12,
+                 PROF_G16Q_C13, PROF_NCLASS };
 bool prof_enabled();
 struct ProfScope {
     int cls;
     double flops, bytes;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    ProfScope(int c, double f, double b);
+    const char* name;  // kernel name (a string that outlives the process's profiling), or null
+    ProfScope(int c, double f, double b, const char* kernel = nullptr);
     ~ProfScope();
 };
 

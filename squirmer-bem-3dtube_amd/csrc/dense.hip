@@ -16,6 +16,7 @@
 //  * TRSM: recursive split with 64-row leaf kernels (one thread per right-hand side,
 //    the triangle in LDS), GEMM updates in between.
 #include <algorithm>
+#include <array>
 #include <type_traits>
 #include <cmath>
 #include <cstdio>
@@ -26,6 +27,9 @@
 #include <memory>
 #include <string>
 #include <vector>
+
+#include <hipblaslt/hipblaslt-ext.hpp>
+#include <hipblaslt/hipblaslt.h>
 
 #include "common.hpp"
 #include "dense.hpp"
@@ -696,6 +700,195 @@ __global__ void __launch_bounds__(256) gemm16q_kernel(int m, int n, int kp, floa
     }
 }
 
+// ------------------------------------------------------------------ 16-bit GEMM, operands DMA'd into LDS
+// The same product as gemm16q_kernel on the same converted operands, with the k tiles
+// copied from global memory into LDS by the CU's load-to-LDS path (global_load_lds_dwordx4:
+// one wave instruction fills one contiguous 1 KiB LDS block, lane l at byte 16 l). No VGPRs
+// stage the operands, so NS stages can be in flight: the loads of tile t + NS - 1 are
+// issued as tile t starts. A stage holds TM + TN rows of KT 16-bit values, KT / 8 chunks of
+// 16 bytes a row; chunk kc of row r sits in slot kc ^ swz(r), which puts the lanes of a
+// fragment read (consecutive rows, one chunk each) on different 16-byte bank groups.
+// Each stage is its own LDS array, so the compiler's own bookkeeping of the DMA writes can
+// tell the stages apart and never waits on the loads of a later stage before reading one.
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+// s_waitcnt immediate that waits until at most n vector-memory operations are outstanding
+// (gfx9 encoding: vmcnt in bits 3:0 and 15:14; expcnt 7 and lgkmcnt 15 mean no wait)
+constexpr unsigned vmcnt_imm(unsigned n) { return (n & 15u) | ((n >> 4) << 14) | (7u << 4) | (15u << 8); }
+
+template <bool BF16, int TM, int TN, int KT, int NS>
+__global__ void __launch_bounds__(256) gemm16d_kernel(int m, int n, int kp, float alpha,
+                                                      const typename Half<BF16>::t* __restrict__ Ah,
+                                                      const typename Half<BF16>::t* __restrict__ Bh, float beta,
+                                                      float* __restrict__ C, int ldc, int tiles_m,
+                                                      const float* __restrict__ rs, const float* __restrict__ cs) {
+    using H = Half<BF16>;
+    using ht = typename H::t;
+    using v8 = typename H::v8;
+    constexpr int IM = TM / 64, JN = TN / 64;  // 32 x 32 MFMA tiles a wave (2 x 2 waves)
+    constexpr int CPR = KT / 8;                 // 16-byte chunks per row of a stage
+    constexpr int RPB = 64 / CPR;               // rows per 1 KiB DMA block
+    constexpr int BA = TM / RPB, BB = TN / RPB; // DMA blocks of A and of B per stage
+    constexpr int LPW = (BA + BB) / 4;          // DMA instructions per wave (per thread) per stage
+    constexpr int SB = (TM + TN) * KT;          // 16-bit values per stage
+    static_assert((BA + BB) % 4 == 0 && KT % 16 == 0 && NS >= 3 && NS <= 4, "gemm16d shape");
+    static_assert(NS * SB * 2 <= 160 * 1024, "gemm16d stages exceed the LDS");
+    constexpr int LDC = TM + 1, CW = 32;  // epilogue tile: CW columns of TM (+1 pad) FP32 values
+    constexpr int S0N = SB > CW * LDC * 2 ? SB : CW * LDC * 2;
+    __shared__ __attribute__((aligned(1024))) ht S0[S0N];
+    __shared__ __attribute__((aligned(1024))) ht S1[SB];
+    __shared__ __attribute__((aligned(1024))) ht S2[SB];
+    __shared__ __attribute__((aligned(1024))) ht S3[NS > 3 ? SB : 8];
+    auto stage = [&](auto s_) -> ht* {
+        constexpr int s = decltype(s_)::value;
+        if constexpr (s == 0) return S0;
+        else if constexpr (s == 1) return S1;
+        else if constexpr (s == 2) return S2;
+        else return S3;
+    };
+    // conflict-free for both ways the LDS may serve a 16-byte read: 8 lanes on 32 banks of
+    // 4 bytes, or 16 lanes on 64 banks (checked over every 8- and 16-row group)
+    auto swz = [](int row) { return CPR == 4 ? ((row >> 1) ^ (row >> 3)) & 3 : (row ^ (row >> 3)) & 7; };
+
+    int tm, tn;
+    tile_of(blockIdx.x, gridDim.x, tiles_m, tm, tn);
+    const int m0 = tm * TM, n0 = tn * TN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave & 1) * (TM / 2), wn = (wave >> 1) * (TN / 2);
+
+    // this thread's source chunk of each of its LPW DMA instructions (block w LPW + q of the
+    // stage: A blocks first, then B), and the wave-uniform destination of each in a stage
+    const ht* src[LPW];
+    int dst[LPW];
+#pragma unroll
+    for (int q = 0; q < LPW; ++q) {
+        const int blk = wave * LPW + q;
+        const bool isA = blk < BA;
+        const int b = isA ? blk : blk - BA;
+        const int row = b * RPB + lane / CPR, kc = (lane % CPR) ^ swz(row);
+        src[q] = (isA ? Ah + (size_t)(m0 + row) * kp : Bh + (size_t)(n0 + row) * kp) + 8 * kc;
+        dst[q] = (isA ? 0 : TM * KT) + b * (RPB * KT);
+    }
+    auto issue = [&](auto s_, int k0) {
+        ht* S = stage(s_);
+#pragma unroll
+        for (int q = 0; q < LPW; ++q)
+            __builtin_amdgcn_global_load_lds((const void*)(src[q] + k0), (lds_void_t*)(S + dst[q]), 16, 0, 0);
+    };
+    auto barrier = [&]() {
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+    };
+
+    f16x acc[IM][JN];
+#pragma unroll
+    for (int i = 0; i < IM; ++i)
+#pragma unroll
+        for (int j = 0; j < JN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+    const int lr = lane & 31, lk = lane >> 5;
+    auto frags = [&](const ht* S, int ks, v8* a_, v8* b_) {
+        const int kc = 2 * ks + lk;
+#pragma unroll
+        for (int i = 0; i < IM; ++i) {
+            const int row = wm + i * 32 + lr;
+            a_[i] = *(const v8*)(S + row * KT + 8 * (kc ^ swz(row)));
+        }
+#pragma unroll
+        for (int j = 0; j < JN; ++j) {
+            const int row = wn + j * 32 + lr;
+            b_[j] = *(const v8*)(S + TM * KT + row * KT + 8 * (kc ^ swz(row)));
+        }
+    };
+    constexpr int KS = KT / 16;  // MFMA k steps a tile (even: the next tile's first step reuses set 0)
+    static_assert(KS % 2 == 0, "gemm16d: an even number of k steps a tile");
+    const int nk = kp / KT;
+    v8 af[2][IM], bf[2][JN];
+    // Tile t in stage s = t % NS, its first fragments already in set 0. Each k step reads the
+    // next step's fragments while its MFMAs run; before the last step the wave waits for its
+    // own loads of tile t + 1 and meets the others at the barrier, after which tile t + 1 is
+    // in LDS for every wave and every wave is done reading stage (t - 1) % NS. The last step
+    // then issues tile t + NS - 1 into that stage (past the last tile: the last tile again,
+    // into a stage nobody reads, so every tile waits the same count) and reads tile t + 1's
+    // first fragments.
+    auto tile = [&](auto s_, int t) {
+        constexpr int s = decltype(s_)::value;
+        const ht* S = stage(s_);
+        static_for<KS>([&](auto ks_) {
+            constexpr int ks = decltype(ks_)::value;
+            if constexpr (ks + 1 < KS) {
+                frags(S, ks + 1, af[(ks + 1) & 1], bf[(ks + 1) & 1]);
+            } else {
+                __builtin_amdgcn_s_waitcnt(vmcnt_imm((NS - 3) * LPW));
+                barrier();
+                issue(std::integral_constant<int, (s + NS - 1) % NS>{}, min(t + NS - 1, nk - 1) * KT);
+                frags(stage(std::integral_constant<int, (s + 1) % NS>{}), 0, af[0], bf[0]);
+            }
+#pragma unroll
+            for (int i = 0; i < IM; ++i)
+#pragma unroll
+                for (int j = 0; j < JN; ++j) acc[i][j] = H::mma32(af[ks & 1][i], bf[ks & 1][j], acc[i][j]);
+            mfma_interleave<IM * JN, IM + JN, 0, ks + 1 < KS ? 0 : LPW>();
+        });
+    };
+
+    static_for<NS - 1>([&](auto p_) { issue(p_, min((int)decltype(p_)::value, nk - 1) * KT); });
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm((NS - 2) * LPW));  // tile 0
+    barrier();
+    frags(S0, 0, af[0], bf[0]);
+    int t0 = 0;
+    for (; t0 + NS <= nk; t0 += NS) static_for<NS>([&](auto s_) { tile(s_, t0 + decltype(s_)::value); });
+    static_for<NS - 1>([&](auto s_) {  // the last nk % NS tiles
+        if (t0 + decltype(s_)::value < nk) tile(s_, t0 + decltype(s_)::value);
+    });
+
+    // epilogue through LDS, 32 columns (one MFMA tile column) at a time in stage 0, which is
+    // sized for it (every DMA has landed and every wave is past its last fragment read): the
+    // chunk's accumulators as a column-major tile, then C updated down columns, consecutive
+    // rows on consecutive lanes, the chunk's C values read before any result is stored
+    // (unscaling in the same order as gemm16q)
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+    constexpr int CS = 256 / TM, NQ = CW / CS, QG = NQ < 16 ? NQ : 16;
+    float* Ct = reinterpret_cast<float*>(S0);
+    const int er = tid % TM, ec = tid / TM;
+    const int row = m0 + er;
+    const float ur = rs ? 1.0f / rs[min(row, m - 1)] : 1.0f;
+#pragma unroll
+    for (int c0 = 0; c0 < TN; c0 += CW) {
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < JN; ++j) {
+            if (wn + j * 32 != c0) continue;  // wave-uniform
+#pragma unroll
+            for (int i = 0; i < IM; ++i)
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    Ct[lr * LDC + wm + i * 32 + 8 * (r >> 2) + 4 * lk + (r & 3)] = acc[i][j][r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q0 = 0; q0 < NQ; q0 += QG) {
+            float cv[QG];
+            if (beta != 0.0f) {
+#pragma unroll
+                for (int q = 0; q < QG; ++q)
+                    cv[q] = C[min(row, m - 1) + (size_t)min(n0 + c0 + ec + CS * (q0 + q), n - 1) * ldc];
+            }
+#pragma unroll
+            for (int q = 0; q < QG; ++q) {
+                const int cl = ec + CS * (q0 + q), col = n0 + c0 + cl;
+                if (row < m && col < n) {
+                    const float uc = cs ? 1.0f / cs[col] : 1.0f;
+                    const float v = alpha * (Ct[cl * LDC + er] * ur * uc);
+                    C[row + (size_t)col * ldc] = beta == 0.0f ? v : v + beta * cv[q];
+                }
+            }
+        }
+    }
+}
+
 // Operand conversion of gemm16q_kernel, both operands in one launch (zero beyond m, n, k):
 //   workgroups [0, nrb):  Ah[i * kp + k] = A(i, k) rs[i], one 64 x 64 tile each, transposed
 //                         through LDS; a thread issues its 16 loads together and writes 16
@@ -832,8 +1025,11 @@ static DBuf<char> g_q16;
 // the fixed rule.
 namespace tune {
 // 0: 128 x 128, 1: 128 x 256, 2: 256 x 128, 3: 256 x 256 (loads staged one k tile ahead);
-// 4: 128 x 128, 5: 128 x 256, 6: 256 x 128 (two k tiles ahead)
-constexpr int NC = 7;
+// 4: 128 x 128, 5: 128 x 256, 6: 256 x 128 (two k tiles ahead); operand tiles DMA'd into
+// LDS: 7: 256 x 256 and 8: 256 x 128 with four 32-deep stages, 9: 128 x 128 with four
+// 64-deep stages; 10-13: hipBLASLt's BF16 GEMM on the same operands with the first four
+// algorithms its heuristic offers for the shape (FP16 runs 3 in their place)
+constexpr int NC = 14;
 struct Cls {
     int tried = 0, best = -1;
     float ms[NC] = {};
@@ -1037,6 +1233,136 @@ static void gemm16q(int m, int n, int k, float alpha, const float* A, int lda, c
     check_launch("gemm16q");
 }
 
+// the same product with gemm16d_kernel (operand tiles DMA'd into LDS, NS stages of KT)
+template <bool BF16, int TM, int TN, int KT, int NS>
+static void gemm16d(int m, int n, int k, float alpha, const float* A, int lda, const float* B, int ldb, float beta,
+                    float* C, int ldc, float* rs, float* cs, const G16Work& w, int cand) {
+    using ht = typename Half<BF16>::t;
+    const int mp = (m + TM - 1) / TM * TM, np = (n + TN - 1) / TN * TN, kp = (k + QBK - 1) / QBK * QBK;
+    const size_t need = ((size_t)mp + np) * kp * sizeof(ht);
+    ht* Ah;
+    if (w.ws) {
+        Ah = (ht*)w.ws;
+    } else {
+        if (g_q16.n < need) g_q16.alloc(need);
+        Ah = (ht*)g_q16.p;
+    }
+    ht* Bh = Ah + (size_t)mp * kp;
+    const size_t nrb = (size_t)(mp / 64) * (kp / 64), chunks = (size_t)np * kp / 8;
+    if (chunks >= ((size_t)1 << 32) || nrb + ceil_div(chunks, 256) >= ((size_t)1 << 31))
+        throw std::runtime_error("gemm16: operand too large for the 16-bit conversion");
+    const int b_vec = ((uintptr_t)B % 16 == 0 && ldb % 4 == 0) ? 1 : 0;
+    hipLaunchKernelGGL(convert16_kernel<BF16>, dim3((unsigned)(nrb + ceil_div(chunks, 256))), dim3(256), 0,
+                       ctx().s(), m, n, k, A, lda, rs, B, ldb, cs, Ah, Bh, kp, np, (int)nrb, b_vec);
+    check_launch("convert16");
+    const int tiles_m = mp / TM, tiles_n = np / TN;
+    const double fl = 2.0 * m * n * k, by = 2.0 * ((double)m * k + (double)k * n) + 4.0 * (beta != 0.0f ? 2.0 : 1.0) * m * n;
+    ProfScope prof(w.ws ? -1 : PROF_GEMM16Q, fl, by);
+    ProfScope prof_tile(w.ws ? -1 : PROF_G16Q_C0 + cand, fl, by);
+    hipLaunchKernelGGL((gemm16d_kernel<BF16, TM, TN, KT, NS>), dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n,
+                       kp, alpha, Ah, Bh, beta, C, ldc, tiles_m, rs, cs);
+    check_launch("gemm16d");
+}
+
+// ------------------------------------------------------------------ library candidate (BF16)
+// hipBLASLt's BF16 GEMM with FP32 accumulation and FP32 C/D, on the same converted operands
+// (a plain GEMM: Ah holds A^T as kp x mp column-major, Bh holds B as kp x np column-major,
+// so D = alpha Ah^T Bh + beta C is its "TN" form). One plan (descriptors and the heuristic's
+// first algorithm) per shape, kept for the process; the workspace is the library's own.
+namespace lt {
+constexpr int kAlgos = 4;  // the heuristic's first kAlgos algorithms are autotuner candidates
+struct Plan {
+    hipblasLtMatmulDesc_t desc = nullptr;
+    hipblasLtMatrixLayout_t a = nullptr, b = nullptr, c = nullptr;
+    hipblasLtMatmulAlgo_t algo[kAlgos]{};
+    std::string kernel[kAlgos];  // the library's name for each algorithm's kernel (profiling)
+    int nalgo = 0;
+    bool ok = false;
+};
+static hipblasLtHandle_t g_handle = nullptr;
+static bool g_broken = false;  // the library could not be set up: the candidate falls back
+static std::map<std::array<long long, 6>, Plan> g_plans;
+static DBuf<char> g_ws;
+constexpr size_t kWorkspace = size_t(64) << 20;
+
+static bool good(hipblasStatus_t st) { return st == HIPBLAS_STATUS_SUCCESS; }
+
+// the plan of D(m x n, ldc) = alpha A^T B + beta C with A^T, B k-deep at leading dimension kp
+static const Plan* plan(int m, int n, int k, int kp, int ldc, bool beta0) {
+    if (g_broken) return nullptr;
+    if (!g_handle) {
+        if (!good(hipblasLtCreate(&g_handle))) {
+            g_broken = true;
+            return nullptr;
+        }
+        g_ws.alloc(kWorkspace);
+    }
+    const std::array<long long, 6> key{m, n, k, kp, ldc, beta0 ? 1 : 0};
+    auto it = g_plans.find(key);
+    if (it != g_plans.end()) return it->second.ok ? &it->second : nullptr;
+    Plan& P = g_plans[key];
+    const hipblasOperation_t tA = HIPBLAS_OP_T, tB = HIPBLAS_OP_N;
+    bool ok = good(hipblasLtMatmulDescCreate(&P.desc, HIPBLAS_COMPUTE_32F, HIP_R_32F)) &&
+              good(hipblasLtMatmulDescSetAttribute(P.desc, HIPBLASLT_MATMUL_DESC_TRANSA, &tA, sizeof tA)) &&
+              good(hipblasLtMatmulDescSetAttribute(P.desc, HIPBLASLT_MATMUL_DESC_TRANSB, &tB, sizeof tB)) &&
+              good(hipblasLtMatrixLayoutCreate(&P.a, HIP_R_16BF, k, m, kp)) &&
+              good(hipblasLtMatrixLayoutCreate(&P.b, HIP_R_16BF, k, n, kp)) &&
+              good(hipblasLtMatrixLayoutCreate(&P.c, HIP_R_32F, m, n, ldc));
+    if (ok) {
+        hipblasLtMatmulPreference_t pref = nullptr;
+        uint64_t ws = kWorkspace;
+        hipblasLtMatmulHeuristicResult_t res[kAlgos];
+        int found = 0;
+        ok = good(hipblasLtMatmulPreferenceCreate(&pref)) &&
+             good(hipblasLtMatmulPreferenceSetAttribute(pref, HIPBLASLT_MATMUL_PREF_MAX_WORKSPACE_BYTES, &ws,
+                                                        sizeof ws)) &&
+             good(hipblasLtMatmulAlgoGetHeuristic(g_handle, P.desc, P.a, P.b, P.c, P.c, pref, kAlgos, res, &found));
+        for (int i = 0; ok && i < found && i < kAlgos; ++i) {
+            if (!good(res[i].state) || res[i].workspaceSize > kWorkspace) continue;
+            P.algo[P.nalgo] = res[i].algo;
+            try {
+                P.kernel[P.nalgo] = hipblaslt_ext::getKernelNameFromAlgo(g_handle, res[i].algo);
+            } catch (...) {
+            }
+            if (P.kernel[P.nalgo].empty()) P.kernel[P.nalgo] = "hipblaslt_gemm_" + std::to_string(i);
+            ++P.nalgo;
+        }
+        ok = ok && P.nalgo > 0;
+        if (pref) (void)hipblasLtMatmulPreferenceDestroy(pref);
+    }
+    P.ok = ok;
+    return ok ? &P : nullptr;
+}
+}  // namespace lt
+
+// candidate 10: the library GEMM on the converted operands (BF16 only; false when the
+// library has no plan for the shape, and the caller then runs an own kernel)
+static bool gemm16lt(int m, int n, int k, float alpha, const float* A, int lda, const float* B, int ldb, float beta,
+                     float* C, int ldc, int which) {
+    const int mp = (m + 63) / 64 * 64, np = (n + 63) / 64 * 64, kp = (k + QBK - 1) / QBK * QBK;
+    const lt::Plan* P = lt::plan(m, n, k, kp, ldc, beta == 0.0f);
+    if (!P) return false;
+    const size_t need = ((size_t)mp + np) * kp * sizeof(__bf16);
+    if (g_q16.n < need) g_q16.alloc(need);
+    __bf16* Ah = (__bf16*)g_q16.p;
+    __bf16* Bh = Ah + (size_t)mp * kp;
+    const size_t nrb = (size_t)(mp / 64) * (kp / 64), chunks = (size_t)np * kp / 8;
+    if (chunks >= ((size_t)1 << 32) || nrb + ceil_div(chunks, 256) >= ((size_t)1 << 31))
+        throw std::runtime_error("gemm16: operand too large for the 16-bit conversion");
+    const int b_vec = ((uintptr_t)B % 16 == 0 && ldb % 4 == 0) ? 1 : 0;
+    hipLaunchKernelGGL(convert16_kernel<true>, dim3((unsigned)(nrb + ceil_div(chunks, 256))), dim3(256), 0,
+                       ctx().s(), m, n, k, A, lda, nullptr, B, ldb, nullptr, Ah, Bh, kp, np, (int)nrb, b_vec);
+    check_launch("convert16");
+    const double fl = 2.0 * m * n * k, by = 2.0 * ((double)m * k + (double)k * n) + 4.0 * (beta != 0.0f ? 2.0 : 1.0) * m * n;
+    const int a = which < P->nalgo ? which : 0;
+    ProfScope prof(PROF_GEMM16Q, fl, by);
+    ProfScope prof_tile(PROF_G16Q_C0 + 10 + a, fl, by, P->kernel[a].c_str());
+    const hipblasStatus_t st = hipblasLtMatmul(lt::g_handle, P->desc, &alpha, Ah, P->a, Bh, P->b, &beta, C, P->c, C,
+                                               P->c, &P->algo[a], lt::g_ws.p, lt::kWorkspace, ctx().s());
+    SQ_REQUIRE(st == HIPBLAS_STATUS_SUCCESS, SQBEM_ERR_HIP, "gemm16: hipblasLtMatmul failed");
+    return true;
+}
+
 // FP16 operand scales (powers of two per row of A and per column of B) into sp[0, 2m + n)
 static void fp16_scales(int m, int n, int k, const float* A, int lda, const float* B, int ldb, float* sp, float** rs,
                         float** cs) {
@@ -1080,9 +1406,14 @@ static void gemm16_impl(int kind, int m, int n, int k, float alpha, const float*
         std::unique_ptr<tune::Probe> probe;
         if (w.ws) {
             cand = tune::decided(bf ? 1 : 0, m, n, k, rule);
+            if (cand >= 10) cand = 3;  // the library may allocate: an own kernel instead
         } else {
             probe.reset(new tune::Probe(bf ? 1 : 0, m, n, k, rule));
             cand = probe->cand;
+        }
+        if (cand >= 10) {
+            if (bf && gemm16lt(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, cand - 10)) return;
+            cand = 3;  // FP16 (scaled operands), or no library plan: the 256 x 256 kernel
         }
         auto run = [&](auto tag_bf) {
             constexpr bool BFv = decltype(tag_bf)::value;
@@ -1093,6 +1424,9 @@ static void gemm16_impl(int kind, int m, int n, int k, float alpha, const float*
                 case 4: gemm16q<BFv, 128, 128, 2>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs, w); break;
                 case 5: gemm16q<BFv, 128, 256, 2>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs, w); break;
                 case 6: gemm16q<BFv, 256, 128, 2>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs, w); break;
+                case 7: gemm16d<BFv, 256, 256, 32, 4>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs, w, 7); break;
+                case 8: gemm16d<BFv, 256, 128, 32, 4>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs, w, 8); break;
+                case 9: gemm16d<BFv, 128, 128, 64, 4>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs, w, 9); break;
                 default: gemm16q<BFv, 256, 256, 1>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rs, cs, w); break;
             }
         };

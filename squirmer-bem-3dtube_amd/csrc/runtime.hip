@@ -1,6 +1,7 @@
 // runtime.hip -- device selection, library stream, error reporting, small utility kernels.
 #include <cstring>
 #include <map>
+#include <string>
 #include <mutex>
 #include <unordered_map>
 #include <vector>
@@ -167,6 +168,7 @@ struct ProfRec {
     int cls;
     double flops, bytes;
     hipEvent_t e0, e1;
+    const char* name;
 };
 struct ProfTotals {
     long launches = 0;
@@ -177,6 +179,7 @@ unsigned g_prof_mask = 0;
 std::vector<ProfRec> g_pending;
 std::vector<hipEvent_t> g_event_pool;
 ProfTotals g_tot[PROF_NCLASS];
+std::map<std::string, ProfTotals> g_named;  // per kernel name, for launches that give one
 
 hipEvent_t take_event() {
     if (!g_event_pool.empty()) {
@@ -193,11 +196,13 @@ void resolve_pending() {
         float ms = 0;
         SQ_HIP(hipEventSynchronize(r.e1));
         SQ_HIP(hipEventElapsedTime(&ms, r.e0, r.e1));
-        auto& t = g_tot[r.cls];
-        t.launches += 1;
-        t.flops += r.flops;
-        t.bytes += r.bytes;
-        t.ms += ms;
+        for (ProfTotals* t : {&g_tot[r.cls], r.name ? &g_named[r.name] : nullptr}) {
+            if (!t) continue;
+            t->launches += 1;
+            t->flops += r.flops;
+            t->bytes += r.bytes;
+            t->ms += ms;
+        }
         g_event_pool.push_back(r.e0);
         g_event_pool.push_back(r.e1);
     }
@@ -207,7 +212,7 @@ void resolve_pending() {
 
 bool prof_enabled() { return g_prof; }
 
-ProfScope::ProfScope(int c, double f, double b) : cls(c), flops(f), bytes(b) {
+ProfScope::ProfScope(int c, double f, double b, const char* kernel) : cls(c), flops(f), bytes(b), name(kernel) {
     if (c < 0 || !g_prof || !(g_prof_mask & (1u << c))) return;  // c < 0: never timed
     e0 = take_event();
     e1 = take_event();
@@ -216,7 +221,7 @@ ProfScope::ProfScope(int c, double f, double b) : cls(c), flops(f), bytes(b) {
 
 ProfScope::~ProfScope() {
     if (!e0) return;
-    if (hipEventRecord(e1, ctx().s()) == hipSuccess) g_pending.push_back({cls, flops, bytes, e0, e1});
+    if (hipEventRecord(e1, ctx().s()) == hipSuccess) g_pending.push_back({cls, flops, bytes, e0, e1, name});
     if (g_pending.size() > 4096) {
         try {
             resolve_pending();
@@ -235,6 +240,7 @@ int sqbem_profile(int mask) {
     return guarded([&] {
         resolve_pending();
         for (auto& t : g_tot) t = ProfTotals{};
+        g_named.clear();
         g_prof = mask != 0;
         g_prof_mask = mask == -1 ? 0xffffffffu : (unsigned)mask;
     });
@@ -249,6 +255,22 @@ int sqbem_profile_get(int cls, long* launches, double* flops, double* bytes, dou
         if (flops) *flops = t.flops;
         if (bytes) *bytes = t.bytes;
         if (ms) *ms = t.ms;
+    });
+}
+
+int sqbem_profile_named(char* buf, int len) {
+    return guarded([&] {
+        SQ_REQUIRE(buf && len > 0, SQBEM_ERR_ARG, "sqbem_profile_named: no buffer");
+        resolve_pending();
+        std::string out;
+        char line[96];
+        for (const auto& kv : g_named) {
+            std::snprintf(line, sizeof line, "\t%ld\t%.6e\t%.6e\t%.6f\n", kv.second.launches, kv.second.flops,
+                          kv.second.bytes, kv.second.ms);
+            out += kv.first + line;
+        }
+        SQ_REQUIRE((int)out.size() < len, SQBEM_ERR_ARG, "sqbem_profile_named: buffer too small");
+        std::memcpy(buf, out.c_str(), out.size() + 1);
     });
 }
 

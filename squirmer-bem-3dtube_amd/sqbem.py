@@ -53,6 +53,7 @@ _SIGS = {
     "sqbem_sgemm": (_i, [_i, _i, _i, _f, _vp, _i, _vp, _i, _f, _vp, _i]),
     "sqbem_gemm16": (_i, [_i, _i, _i, _i, _f, _vp, _i, _vp, _i, _f, _vp, _i]),
     "sqbem_gemm16_noalloc": (_i, [_i, _i, _i, _i, _f, _vp, _i, _vp, _i, _f, _vp, _i]),
+    "sqbem_profile_named": (_i, [ctypes.c_char_p, _i]),
     "sqbem_gemm16_noalloc_bytes": (ctypes.c_ulong, [_i, _i, _i, _i]),
     "sqbem_dgetrf": (_i, [_i, _vp, _i, _vp, _vp]),
     "sqbem_dgetrs": (_i, [_i, _i, _vp, _i, _vp, _vp, _i]),
@@ -159,7 +160,9 @@ def device_info():
     return name.value.decode(), cu.value, gb.value
 
 
-TUNE_TILES = ["128x128", "128x256", "256x128", "256x256", "128x128/pf2", "128x256/pf2", "256x128/pf2"]
+TUNE_TILES = ["128x128", "128x256", "256x128", "256x256", "128x128/pf2", "128x256/pf2", "256x128/pf2",
+              "256x256/dma32x4", "256x128/dma32x4", "128x128/dma64x4", "hipblaslt0", "hipblaslt1",
+              "hipblaslt2", "hipblaslt3"]
 PROF_CLASSES = {"dgemm": 0, "sgemm": 1, "gemm16": 2, "panel": 3, "trsm": 4, "assemble": 5, "gemv": 6, "laswp": 7,
                 "gemm16q": 8, "trsv": 9}
 PROF_CLASSES.update({"gemm16q_" + t.replace("/", "_"): 10 + c for c, t in enumerate(TUNE_TILES)})
@@ -177,6 +180,17 @@ def profile(enable=True, classes=None):
         for c in classes:
             mask |= 1 << PROF_CLASSES[c]
     check(lib().sqbem_profile(mask))
+
+
+def profile_named():
+    """Kernel-time totals per named kernel (the library GEMM's kernels): {name: dict}."""
+    buf = ctypes.create_string_buffer(1 << 16)
+    check(lib().sqbem_profile_named(buf, len(buf)))
+    out = {}
+    for ln in buf.value.decode().splitlines():
+        name, l, f, b, ms = ln.split("\t")
+        out[name] = {"launches": int(l), "flops": float(f), "bytes": float(b), "ms": float(ms)}
+    return out
 
 
 def profile_get(name):

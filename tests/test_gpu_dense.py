@@ -93,12 +93,14 @@ def test_gemm16(sq, kind, tol, m, n, k):
     assert err.max() < tol, err.max()
 
 
+@pytest.mark.parametrize("m,n,k", [(700, 555, 300), (300, 290, 128), (1100, 1030, 4160)])
 @pytest.mark.parametrize("kind,tol", [(2, 2.5e-3), (3, 2e-2)])
-def test_gemm16_autotuner_every_tile(sq, kind, tol):
-    """The tile autotuner runs each of its seven candidates once on a fresh shape class
+def test_gemm16_autotuner_every_tile(sq, kind, tol, m, n, k):
+    """The tile autotuner runs each of its fourteen candidates once on a fresh shape class
     (odd sizes, beta != 0) before settling: every call's product must be right, and the
-    class must end decided."""
-    m, n, k = 700, 555, 300
+    class must end decided. The k values give the load-to-LDS candidates (32- and 64-deep
+    tiles, four stages) a partial last group of tiles (k = 300, 4160) and fewer tiles than
+    stages (k = 128)."""
     rng = np.random.default_rng(7 + kind)
     A = (rng.standard_normal((m, k)) * np.logspace(-3, 2, m)[:, None]).astype(np.float32)
     B = rng.standard_normal((k, n)).astype(np.float32)
@@ -107,7 +109,7 @@ def test_gemm16_autotuner_every_tile(sq, kind, tol):
     scale = 0.5 * (np.abs(A).astype(np.float64) @ np.abs(B).astype(np.float64)) + 2.0 * np.abs(C0)
     sq.tune(1)
     try:
-        for call in range(9):
+        for call in range(16):
             dA, dB, dC = dev(A), dev(B), dev(C0)
             torch.cuda.synchronize()
             sq.gemm16(kind, 0.5, dA, dB, -2.0, dC)
@@ -129,7 +131,7 @@ def test_gemm16_tune_table_save_and_load(sq, tmp_path):
     ref = A.astype(np.float64) @ B.astype(np.float64)
     sq.tune(1)
     try:
-        for _ in range(8):
+        for _ in range(15):  # fourteen candidates explored, then the decided one
             dA, dB, dC = dev(A), dev(B), dev(np.zeros((m, n), dtype=np.float32))
             torch.cuda.synchronize()
             sq.gemm16(3, 1.0, dA, dB, 0.0, dC)
