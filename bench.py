@@ -242,6 +242,44 @@ def hbm_traffic(kernel):
     return {"bytes_per_launch": e["hbm_bytes_per_launch"], "source": "profiles/hbm_traffic_46k_bf16.json"}
 
 
+def calib_copy(nbytes):
+    """One plain device-to-device copy of nbytes, as hipMemcpy2D of 1 MiB rows (the HIP
+    runtime's copy kernel; a flat hipMemcpy of this size goes to a DMA engine, which the
+    counters do not see), through the HIP runtime libsqbem is linked against (the loaded
+    soname resolves to it). Not through torch: importing torch first would bind libsqbem's
+    libhipblaslt.so.1 to torch's bundled copy, whose GEMM kernels differ from the system
+    library's."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    hip.hipMemcpy2D.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
+                                ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    row = 1 << 20
+    nbytes = nbytes // row * row
+
+    def ok(rc, what):
+        if rc != 0:
+            raise RuntimeError(f"calibration copy: {what} failed with hip error {rc}")
+
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    ok(hip.hipMalloc(ctypes.byref(src), nbytes), "hipMalloc")
+    try:
+        ok(hip.hipMalloc(ctypes.byref(dst), nbytes), "hipMalloc")
+        try:
+            ok(hip.hipMemset(src, 0x3f, nbytes), "hipMemset")
+            ok(hip.hipDeviceSynchronize(), "hipDeviceSynchronize")
+            ok(hip.hipMemcpy2D(dst, row, src, row, row, nbytes // row, 3), "hipMemcpy2D")
+            ok(hip.hipDeviceSynchronize(), "hipDeviceSynchronize")
+        finally:
+            hip.hipFree(dst)
+    finally:
+        hip.hipFree(src)
+    return nbytes
+
+
 def batch_bench(pkg, n=256, seed=0, modes=(("fp32+ir", 1), ("fp16+ir", 2)), check=4):
     """Batched uneven solves: n configurations drawn from seed (a/R ~ U[0.1, 0.7], b/R ~
     U[0, 0.8 (1 - a/R)], B2/B1 ~ U[-5, 5], random e, sphere level {2, 3}, tube n_theta
@@ -416,6 +454,7 @@ def main():
     ap.add_argument("--no-grad", action="store_true", help="skip the adjoint-gradient timing")
     ap.add_argument("--no-field", action="store_true", help="skip the 57k-DOF solve + field grid")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-configuration batched solve")
+    ap.add_argument("--no-small", action="store_true", help="skip the 5,430- and 13,254-DOF configurations")
     ap.add_argument("--batch-only", default="", metavar="MODES",
                     help="run only the 256-configuration batch in these modes (e.g. fp16,fp32) and print its line")
     ap.add_argument("--tune-table", default="shipped",
@@ -444,10 +483,6 @@ def main():
 
     import __graft_entry__ as ge
 
-    if args.calib_copy_gib > 0:
-        import torch  # its HIP runtime has to see the device before libsqbem's does
-
-        torch.cuda.init()
     pkg = ge.load_package()
     pkg.init(0)
     tune_loaded = 0
@@ -534,16 +569,7 @@ def main():
     if args.calib_copy_gib > 0:
         # a plain copy of known size: what FETCH_SIZE / WRITE_SIZE read for it calibrates
         # the same counters on the solver's kernels
-        import torch
-
-        nel = int(args.calib_copy_gib * 2 ** 30) // 8
-        src_t = torch.ones(nel, dtype=torch.float64, device="cuda")
-        dst_t = torch.empty_like(src_t)
-        torch.cuda.synchronize()
-        dst_t.copy_(src_t)
-        torch.cuda.synchronize()
-        detail["calib_copy_bytes"] = nel * 8
-        del src_t, dst_t
+        detail["calib_copy_bytes"] = calib_copy(int(args.calib_copy_gib * 2 ** 30))
 
     if not args.no_grad:
         try:
@@ -566,10 +592,11 @@ def main():
         except Exception as ex:
             detail["large_field"] = {"error": str(ex)}
 
-    try:
-        detail["small_configs"] = small_cfg_gpu(pkg, prec)
-    except Exception as ex:
-        detail["small_configs"] = {"error": str(ex)}
+    if not args.no_small:
+        try:
+            detail["small_configs"] = small_cfg_gpu(pkg, prec)
+        except Exception as ex:
+            detail["small_configs"] = {"error": str(ex)}
 
     cpu = None
     if not args.no_cpu:
