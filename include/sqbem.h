@@ -267,14 +267,18 @@ int sqbem_swimmer_gradient(sqbem_swimmer* s, const double* e, double B1, double 
  * the shared tube factors for all their Schur right-hand sides, and (refined modes) a
  * refinement of all their full systems together. Outputs (host): U[3n], Om[3n]; iters[n]
  * and resid[n] (refinement steps and final relative residual; 0 in FP64 mode; may be
- * NULL); stats[11] (may be NULL): seconds in tube assembly + factorisation (+ inverse),
- * coupling assembly, Schur + bordered factors, solves; unique tubes, chunks, most
- * refinement steps, worst final residual, bordered matrices that needed pivoting, bytes
- * of the chunk workspace, refinement steps replayed from a captured HIP graph. Every
- * instance needs a tube (tubes[i] != NULL). */
-/* Batched-solve options (no reference counterpart): use_graph != 0 (the default) captures
- * each chunk's refinement step (a fixed launch sequence) as a HIP graph once and replays
- * it every step; 0 launches the same kernels one by one. */
+ * NULL); stats[12] (may be NULL): seconds in tube assembly + factorisation (+ inverse),
+ * coupling assembly (host preparation and device time), Schur + bordered factors, solves;
+ * unique tubes, chunks, most refinement steps, worst final residual, bordered matrices that
+ * needed pivoting, bytes of the chunk workspace, refinement steps replayed from a captured
+ * HIP graph, build stages run as captured HIP graphs. Every instance needs a tube
+ * (tubes[i] != NULL). */
+/* Batched-solve options (no reference counterpart): use_graph != 0 (the default) runs each
+ * chunk's build (coupling assembly and right-hand sides; Schur complement and bordered
+ * factors: two stages, no host round trip inside either) as two captured HIP graphs, and
+ * captures its refinement step once and replays it every step; a stage whose capture
+ * records anything but kernel launches runs its launches directly instead. 0 launches the
+ * same kernels one by one. The results are bit-identical either way. */
 int sqbem_batch_options(int use_graph);
 int sqbem_batch_solve(int n, const sqbem_mesh* const* tubes, const sqbem_mesh* const* spheres,
                       const double* centers, const double* e, const double* B, double mu, int precision,

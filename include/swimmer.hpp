@@ -284,7 +284,7 @@ struct BatchResult {
     std::vector<V3> U, Om;
     std::vector<int> iters;       // refinement steps per configuration (0 in FP64 mode)
     std::vector<double> resid;    // final relative residual of the full system
-    std::array<double, 11> stats; // see sqbem_batch_solve
+    std::array<double, 12> stats; // see sqbem_batch_solve
 };
 inline BatchResult batch_solve(const std::vector<BatchConfig>& cfg, double mu = 1, int precision = SQBEM_FP64,
                                double mem_budget_gb = 0) {

@@ -586,15 +586,10 @@ __global__ void reduce_dls_batch_kernel(const DlsReduce* __restrict__ jobs) {
     }
 }
 
-static DBuf<AsmArgs> g_jobs;
-static DBuf<long> g_job_start;
-static DBuf<unsigned long long> g_queue;
-static DBuf<DlsReduce> g_dls_jobs;
-
-void assemble_batch(const std::vector<AsmJob>& jobs) {
+void assemble_batch_prepare(const std::vector<AsmJob>& jobs, AsmBatch& plan) {
     std::vector<AsmArgs> args;
     std::vector<long> start(1, 0);
-    double flops = 0, bytes = 0;
+    plan.flops = plan.bytes = 0;
     for (const AsmJob& j : jobs) {
         if (j.nt <= 0 || j.np <= 0) continue;
         SQ_REQUIRE(j.p0 >= 0 && j.p0 + j.np <= j.src->np, SQBEM_ERR_ARG, "assemble: source panel range out of bounds");
@@ -615,34 +610,58 @@ void assemble_batch(const std::vector<AsmJob>& jobs) {
         a.defl_sc = j.defl_sc; a.defl_tn = j.defl_tn;
         args.push_back(a);
         start.push_back(start.back() + (long)ceil_div(j.nt, TT) * asm_tiles(j.np));
-        flops += 72.0 * 7.0 * j.nt * j.np;
-        bytes += 8.0 * 9.0 * j.nt * j.np * ((j.SL ? 1 : 0) + (j.DL ? 1 : 0));
+        plan.flops += 72.0 * 7.0 * j.nt * j.np;
+        plan.bytes += 8.0 * 9.0 * j.nt * j.np * ((j.SL ? 1 : 0) + (j.DL ? 1 : 0));
     }
+    plan.njobs = (int)args.size();
+    plan.total = start.back();
     if (args.empty()) return;
-    if (g_jobs.n < args.size()) g_jobs.alloc(std::max<size_t>(args.size(), 256));
-    if (g_job_start.n < start.size()) g_job_start.alloc(std::max<size_t>(start.size(), 257));
-    if (!g_queue.p) g_queue.alloc(1);
-    g_jobs.upload(args.data(), args.size());
-    g_job_start.upload(start.data(), start.size());
-    g_queue.zero();
-    const long total = start.back();
-    const unsigned grid = (unsigned)std::min<long>(total, 8L * std::max(ctx().cu_count, 1));
-    ProfScope prof(PROF_ASSEMBLE, flops, bytes);
-    hipLaunchKernelGGL(assemble_batch_kernel, dim3(grid), dim3(TT), 0, ctx().s(), g_jobs.p, g_job_start.p,
-                       (int)args.size(), g_queue.p);
+    plan.jobs.alloc(args.size());
+    plan.start.alloc(start.size());
+    if (!plan.queue.p) plan.queue.alloc(1);
+    plan.jobs.upload(args.data(), args.size());
+    plan.start.upload(start.data(), start.size());
+    sync();  // args and start are host staging
+}
+
+void assemble_batch_launch(const AsmBatch& plan) {
+    if (!plan.njobs) return;
+    fill_zero(plan.queue.p, sizeof(unsigned long long));  // a kernel: the launch may be captured
+    const unsigned grid = (unsigned)std::min<long>(plan.total, 8L * std::max(ctx().cu_count, 1));
+    ProfScope prof(PROF_ASSEMBLE, plan.flops, plan.bytes);
+    hipLaunchKernelGGL(assemble_batch_kernel, dim3(grid), dim3(TT), 0, ctx().s(), plan.jobs.p, plan.start.p,
+                       plan.njobs, plan.queue.p);
     check_launch("assemble_batch");
-    sync();  // the job tables are host staging for the next call
+}
+
+void assemble_batch(const std::vector<AsmJob>& jobs) {
+    AsmBatch plan;
+    assemble_batch_prepare(jobs, plan);
+    assemble_batch_launch(plan);
+    sync();  // the plan's tables are released on return
+}
+
+void reduce_dls_prepare(const std::vector<DlsReduce>& jobs, DlsBatch& plan) {
+    plan.n = (int)jobs.size();
+    plan.ntmax = 1;
+    if (jobs.empty()) return;
+    for (const DlsReduce& r : jobs) plan.ntmax = std::max(plan.ntmax, r.nt);
+    plan.jobs.alloc(jobs.size());
+    plan.jobs.upload(jobs.data(), jobs.size());
+    sync();  // jobs is host staging
+}
+
+void reduce_dls_launch(const DlsBatch& plan) {
+    if (!plan.n) return;
+    hipLaunchKernelGGL(reduce_dls_batch_kernel, dim3(ceil_div(plan.ntmax, 256), (unsigned)plan.n), dim3(256), 0,
+                       ctx().s(), plan.jobs.p);
+    check_launch("reduce_dls_batch");
 }
 
 void reduce_dls_batch(const std::vector<DlsReduce>& jobs) {
-    if (jobs.empty()) return;
-    int mx = 1;
-    for (const DlsReduce& r : jobs) mx = std::max(mx, r.nt);
-    if (g_dls_jobs.n < jobs.size()) g_dls_jobs.alloc(std::max<size_t>(jobs.size(), 256));
-    g_dls_jobs.upload(jobs.data(), jobs.size());
-    hipLaunchKernelGGL(reduce_dls_batch_kernel, dim3(ceil_div(mx, 256), (unsigned)jobs.size()), dim3(256), 0, ctx().s(),
-                       g_dls_jobs.p);
-    check_launch("reduce_dls_batch");
+    DlsBatch plan;
+    reduce_dls_prepare(jobs, plan);
+    reduce_dls_launch(plan);
     sync();
 }
 

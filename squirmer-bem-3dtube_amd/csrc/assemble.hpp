@@ -75,6 +75,19 @@ struct AsmJob {
 int asm_tiles(int np);
 // all jobs in one persistent launch (work queue over every job's workgroup tiles)
 void assemble_batch(const std::vector<AsmJob>& jobs);
+// the same in two halves: the job tables uploaded once (prepare, which synchronises), then
+// launches only (launch: the work-queue counter is cleared by a kernel, so the launch can
+// be captured into a graph while the plan lives)
+struct AsmBatch {
+    DBuf<AsmArgs> jobs;
+    DBuf<long> start;
+    DBuf<unsigned long long> queue;
+    int njobs = 0;
+    long total = 0;
+    double flops = 0, bytes = 0;
+};
+void assemble_batch_prepare(const std::vector<AsmJob>& jobs, AsmBatch& plan);
+void assemble_batch_launch(const AsmBatch& plan);
 // out[9i + k] = (sum over tiles of part1) + (sum over tiles of part2), tiles in order (the
 // row sums of two consecutive assemble() calls into one zeroed dlsum); part2 may be null
 struct DlsReduce {
@@ -86,6 +99,12 @@ struct DlsReduce {
     double* out;
 };
 void reduce_dls_batch(const std::vector<DlsReduce>& jobs);
+struct DlsBatch {
+    DBuf<DlsReduce> jobs;
+    int n = 0, ntmax = 1;
+};
+void reduce_dls_prepare(const std::vector<DlsReduce>& jobs, DlsBatch& plan);
+void reduce_dls_launch(const DlsBatch& plan);
 
 // velocity at npts points (device, xyz interleaved) from panel velocities u and
 // tractions f (device, 3 per panel): out (device, 3 npts)

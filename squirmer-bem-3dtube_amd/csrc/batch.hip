@@ -149,16 +149,18 @@ struct CJob {
     const double* fs;
     const double* cen;
     const double* area;
-    double sc, xc[3];
+    const double* sc;  // device: the instance's bordered scale
+    double xc[3];
     double* out;
 };
 __global__ void constraint_batch_kernel(const CJob* __restrict__ jobs) {
     const CJob& j = jobs[blockIdx.x];
     double acc[6] = {0, 0, 0, 0, 0, 0};
+    const double sc = *j.sc;
     for (int p = threadIdx.x; p < j.Ns; p += blockDim.x) {
         const double rx = j.cen[3 * (size_t)p] - j.xc[0], ry = j.cen[3 * (size_t)p + 1] - j.xc[1],
                      rz = j.cen[3 * (size_t)p + 2] - j.xc[2];
-        const double A = j.area[p] * j.sc;
+        const double A = j.area[p] * sc;
         const double f0 = j.fs[3 * (size_t)p], f1 = j.fs[3 * (size_t)p + 1], f2 = j.fs[3 * (size_t)p + 2];
         acc[0] += A * f0; acc[1] += A * f1; acc[2] += A * f2;
         acc[3] += -A * rz * f1 + A * ry * f2;
@@ -195,6 +197,13 @@ __global__ void batch_norms_kernel(int nt, const double* __restrict__ Rt, const 
         __syncthreads();
     }
     if (threadIdx.x == 0) out[i] = sqrt(red[0]);
+}
+
+// sc_i = max |G_i| / Amax_i, the maximum as the bits of a non-negative double
+__global__ void border_scale_kernel(int k, const unsigned long long* __restrict__ amax, const double* __restrict__ Amax,
+                                    double* sc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < k) sc[i] = __longlong_as_double((long long)amax[i]) / Amax[i];
 }
 
 __global__ void batch_iota_kernel(int n, int* a) {
@@ -281,11 +290,11 @@ size_t gemv_part_elems(int nt, const std::vector<int>& ns, const std::vector<int
 
 struct Stats {
     double t_tubes = 0, t_assembly = 0, t_schur = 0, t_solve = 0;
-    int unique_tubes = 0, chunks = 0, max_iters = 0, pivoted = 0, graph_replays = 0;
+    int unique_tubes = 0, chunks = 0, max_iters = 0, pivoted = 0, graph_replays = 0, build_graphs = 0;
     double worst = 0;
 };
 
-bool g_use_graph = true;  // capture the refinement step once per chunk and replay it
+bool g_use_graph = true;  // capture each chunk's build stages and its refinement step as graphs
 
 // A tube shared by instances: its factors, and for the refined modes its FP32 inverse.
 struct SharedTube {
@@ -434,12 +443,13 @@ void run_chunk(SharedTube& ST, std::vector<Inst*>& I, int prec, double mu, Arena
     double *Xt = w.Xt, *Xsc = w.Xsc, *Rt = w.Rt, *Rsc = w.Rsc, *Dt = w.Dt, *Dsc = w.Dsc;
     float *Z32 = w.Z32, *Zo32 = w.Zo32, *V32 = w.V32;
 
-    // ---- coupling assembly (src/swimmer.hpp:188-234), one persistent launch
+    // ---- preparation on the host: every job table and upload the chunk's build needs,
+    // then one wait; the build itself (coupling assembly, right-hand sides, Schur
+    // complement, bordered factors) is launches only, two captured graphs when it records
+    // nothing else, with one read-back at its end (the no-pivot verdicts and the norms)
     double t0 = now_s();
     std::vector<DMesh> dm(k);
     for (int i = 0; i < k; ++i) dm[i].upload(I[i]->S);
-    hipLaunchKernelGGL(batch_iota_kernel, dim3(ceil_div(maxNs, 256)), dim3(256), 0, ctx().s(), maxNs, self);
-    check_launch("iota");
     std::vector<AsmJob> jobs;
     std::vector<DlsReduce> red;
     for (int i = 0; i < k; ++i) {
@@ -447,29 +457,24 @@ void run_chunk(SharedTube& ST, std::vector<Inst*>& I, int prec, double mu, Arena
         double Asum = 0;
         for (int p = 0; p < q.Ns; ++p) Asum += q.S.area[p];
         const double defl = 1.0 / (e8 * mu * std::sqrt(Asum));
-        // tube rows <- sphere sources: B = SL/(8 pi mu), AuBt = -DL/(8 pi)
+        // tube rows <- sphere sources: B = SL/(8 pi mu), AuBt = -DL/(8 pi)   (src/swimmer.hpp:188-198)
         jobs.push_back({&dm[i], 0, q.Ns, tb->dm.cen_xyz.p, Nt, nullptr, e8 * mu, BC + (size_t)coff[i] * nt, nt, e8,
                         AuBt + (size_t)soff[i] * nt, nt, nullptr, 0.0, nullptr});
-        // sphere rows <- tube sources: D = SL/(8 pi mu), stresslet row sums
+        // sphere rows <- tube sources: D = SL/(8 pi mu), stresslet row sums (src/swimmer.hpp:201-207)
         jobs.push_back({&tb->dm, 0, Nt, dm[i].cen_xyz.p, q.Ns, nullptr, e8 * mu, D + oD[i], q.ns, 1.0, nullptr, 0,
                         part + oP2[i], 0.0, nullptr});
-        // sphere rows <- sphere sources: E (deflated), AuBs, stresslet row sums
+        // sphere rows <- sphere sources: E (deflated), AuBs, stresslet row sums (src/swimmer.hpp:210-222)
         jobs.push_back({&dm[i], 0, q.Ns, dm[i].cen_xyz.p, q.Ns, self, e8 * mu, GE + oG[i], q.ns, e8, AuBs + oA[i], q.ns,
                         part + oP3[i], defl, dm[i].nrm_xyz.p});
         red.push_back({q.Ns, asm_tiles(Nt), part + oP2[i], asm_tiles(q.Ns), part + oP3[i], dls + 9 * (size_t)Noff[i]});
     }
-    assemble_batch(jobs);
-    reduce_dls_batch(red);
-    for (int i = 0; i < k; ++i) {
-        const Inst& q = *I[i];
-        completion_diag(q.Ns, AuBs + oA[i], q.ns, dls + 9 * (size_t)Noff[i], 0.0);
-        // Ct = AuBt K beside B_i, Cs = AuBs K beside E_i (src/swimmer.hpp:223-234)
-        kproj_async(nt, q.Ns, AuBt + (size_t)soff[i] * nt, nt, dm[i].cen_xyz.p, q.xc, BC + ((size_t)coff[i] + q.ns) * nt,
-                    nt, kp);
-        kproj_async(q.ns, q.Ns, AuBs + oA[i], q.ns, dm[i].cen_xyz.p, q.xc, GE + oG[i] + (size_t)q.ns * q.ns, q.ns, kp);
-    }
+    AsmBatch asm_plan;
+    assemble_batch_prepare(jobs, asm_plan);
+    DlsBatch dls_plan;
+    reduce_dls_prepare(red, dls_plan);
     // right-hand sides of the full bordered system for each instance's slip:
     // b_t = -AuBt u, b_s = -AuBs u, b_c = 0 (src/swimmer.hpp:256-269)
+    GvPlan rhs_plan;
     {
         std::vector<double> hu(S);
         for (int i = 0; i < k; ++i) {
@@ -483,8 +488,6 @@ void run_chunk(SharedTube& ST, std::vector<Inst*>& I, int prec, double mu, Arena
             }
         }
         SQ_HIP(hipMemcpyAsync(u, hu.data(), S * sizeof(double), hipMemcpyHostToDevice, ctx().s()));
-        fill_zero(Bt, sizeof(double) * (size_t)nt * k);
-        fill_zero(Bsc, sizeof(double) * C);
         std::vector<GvJob> g;
         for (int i = 0; i < k; ++i) {
             const Inst& q = *I[i];
@@ -492,71 +495,131 @@ void run_chunk(SharedTube& ST, std::vector<Inst*>& I, int prec, double mu, Arena
                          Bt + (size_t)i * nt, -1.0, 0});
             g.push_back({q.ns, q.ns, 0, AuBs + oA[i], q.ns, u + soff[i], nullptr, 0, nullptr, Bsc + coff[i], -1.0, 0});
         }
-        gemv_batch<double>(g, gvp);  // synchronises (the slip upload is complete)
+        gemv_prepare<double>(rhs_plan, g);  // synchronises: hu is uploaded
     }
-    double t1 = now_s();
-    st.t_assembly += t1 - t0;
-
-    // ---- Schur complement (src/swimmer.hpp:235-249) and bordered factors
-    std::vector<double> Amax(k, 0.0), sc(k);
-    for (int i = 0; i < k; ++i)
-        for (int p = 0; p < I[i]->Ns; ++p) Amax[i] = std::max(Amax[i], 5.0 * I[i]->S.area[p]);
-    std::vector<unsigned long long> abits(k);
-    fill_zero(amax, sizeof(unsigned long long) * k);
-    if (!mx) {
-        getrs<double>(SQBEM_FP64, 0, nt, (int)C, tb->T64.p, nt, tb->ipiv.p, BC, nt, tb->aux64.p);  // Y in place
-        for (int i = 0; i < k; ++i) {
-            const Inst& q = *I[i];
-            dgemm(q.ns, q.m, nt, -1.0, D + oD[i], q.ns, BC + (size_t)coff[i] * nt, nt, 1.0, GE + oG[i], q.ns);
-            absmax_async<double>(q.ns, q.ns, GE + oG[i], q.ns, amax + i);
-        }
-    } else {
-        // Y = T^-1 [B | Ct] for every instance: one GEMM through the shared inverse
-        convert_matrix<double, float>(nt, (int)C, BC, nt, B32, nt);
-        if (prec == SQBEM_FP32_IR)
-            sgemm(nt, (int)C, nt, 1.0f, ST.inv.p, nt, B32, nt, 0.0f, Y32, nt);
-        else
-            gemm16(prec, nt, (int)C, nt, 1.0f, ST.inv.p, nt, B32, nt, 0.0f, Y32, nt);
-        for (int i = 0; i < k; ++i) {
-            const Inst& q = *I[i];
-            convert_matrix<double, float>(q.ns, nt, D + oD[i], q.ns, D32 + oD[i], q.ns);
-            convert_matrix<double, float>(q.ns, q.m, GE + oG[i], q.ns, G32 + oG[i], q.ns);
-            const float* Yi = Y32 + (size_t)coff[i] * nt;
-            if (prec == SQBEM_FP32_IR)
-                sgemm(q.ns, q.m, nt, -1.0f, D32 + oD[i], q.ns, Yi, nt, 1.0f, G32 + oG[i], q.ns);
-            else
-                gemm16(prec, q.ns, q.m, nt, -1.0f, D32 + oD[i], q.ns, Yi, nt, 1.0f, G32 + oG[i], q.ns);
-            absmax_async<float>(q.ns, q.ns, G32 + oG[i], q.ns, amax + i);
-        }
+    // the bordered scale sc_i = max |G_i| / (5 max area_i) (the max |G_i| found on the
+    // device), Amax_i from the host mesh
+    DBuf<double> Amax_d(k), sc_d(k);
+    {
+        std::vector<double> Amax(k, 0.0);
+        for (int i = 0; i < k; ++i)
+            for (int p = 0; p < I[i]->Ns; ++p) Amax[i] = std::max(Amax[i], 5.0 * I[i]->S.area[p]);
+        Amax_d.upload(Amax.data(), k);
     }
-    SQ_HIP(hipMemcpyAsync(abits.data(), amax, sizeof(unsigned long long) * k, hipMemcpyDeviceToHost, ctx().s()));
-    sync();
-    for (int i = 0; i < k; ++i) {
-        double a;
-        std::memcpy(&a, &abits[i], sizeof a);
-        sc[i] = a / Amax[i];
-    }
-    // bordered matrices, factored together without pivoting; any that partial pivoting
-    // would interchange are refactored alone with pivoting
-    std::vector<char> pivoted(k, 0);
-    auto border = [&](int i) {
-        const Inst& q = *I[i];
-        if (mx) border_build<float>(q.Ns, G32 + oG[i], M32 + oM[i], dm[i].cen_xyz.p, dm[i].area.p, sc[i], q.xc);
-        else border_build<double>(q.Ns, GE + oG[i], M64 + oM[i], dm[i].cen_xyz.p, dm[i].area.p, sc[i], q.xc);
-    };
-    for (int i = 0; i < k; ++i) border(i);
-    std::vector<int> hflag(k, 0);
+    BatchLuPlan<float> lu32;
+    BatchLuPlan<double> lu64;
     if (mx) {
         std::vector<BatchMat<float>> bm;
         for (int i = 0; i < k; ++i) bm.push_back({M32 + oM[i], I[i]->m, I[i]->m, X32 + oX[i], flags + i});
-        getrf_np_batch<float>(bm);
+        getrf_np_batch_prepare<float>(bm, lu32);
     } else {
         std::vector<BatchMat<double>> bm;
         for (int i = 0; i < k; ++i) bm.push_back({M64 + oM[i], I[i]->m, I[i]->m, X64 + oX[i], flags + i});
-        getrf_np_batch<double>(bm);
+        getrf_np_batch_prepare<double>(bm, lu64);
     }
-    SQ_HIP(hipMemcpyAsync(hflag.data(), flags, sizeof(int) * k, hipMemcpyDeviceToHost, ctx().s()));
-    sync();
+    if (mx) {
+        std::vector<int> hc(coff.begin(), coff.begin() + k);
+        SQ_HIP(hipMemcpyAsync(dcoff, hc.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx().s()));
+        SQ_HIP(hipMemcpyAsync(dmlen, mv.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx().s()));
+    }
+    sync();  // every host staging buffer above is consumed
+    const double t_prep = now_s() - t0;
+
+    auto border = [&](int i) {
+        const Inst& q = *I[i];
+        if (mx) border_build<float>(q.Ns, G32 + oG[i], M32 + oM[i], dm[i].cen_xyz.p, dm[i].area.p, 0.0, q.xc, sc_d.p + i);
+        else border_build<double>(q.Ns, GE + oG[i], M64 + oM[i], dm[i].cen_xyz.p, dm[i].area.p, 0.0, q.xc, sc_d.p + i);
+    };
+    // stage 1: coupling assembly (src/swimmer.hpp:188-234) and the right-hand sides
+    auto stage_assembly = [&]() {
+        hipLaunchKernelGGL(batch_iota_kernel, dim3(ceil_div(maxNs, 256)), dim3(256), 0, ctx().s(), maxNs, self);
+        check_launch("iota");
+        assemble_batch_launch(asm_plan);
+        reduce_dls_launch(dls_plan);
+        for (int i = 0; i < k; ++i) {
+            const Inst& q = *I[i];
+            completion_diag(q.Ns, AuBs + oA[i], q.ns, dls + 9 * (size_t)Noff[i], 0.0);
+            // Ct = AuBt K beside B_i, Cs = AuBs K beside E_i (src/swimmer.hpp:223-234)
+            kproj_async(nt, q.Ns, AuBt + (size_t)soff[i] * nt, nt, dm[i].cen_xyz.p, q.xc,
+                        BC + ((size_t)coff[i] + q.ns) * nt, nt, kp);
+            kproj_async(q.ns, q.Ns, AuBs + oA[i], q.ns, dm[i].cen_xyz.p, q.xc, GE + oG[i] + (size_t)q.ns * q.ns, q.ns,
+                        kp);
+        }
+        fill_zero(Bt, sizeof(double) * (size_t)nt * k);
+        fill_zero(Bsc, sizeof(double) * C);
+        gemv_launch<double>(rhs_plan, gvp);
+    };
+    // stage 2: Schur complement (src/swimmer.hpp:235-249), bordered matrices and their
+    // factors without pivoting (each verified against the partial-pivoting condition)
+    auto stage_schur = [&]() {
+        fill_zero(amax, sizeof(unsigned long long) * k);
+        if (!mx) {
+            getrs<double>(SQBEM_FP64, 0, nt, (int)C, tb->T64.p, nt, tb->ipiv.p, BC, nt, tb->aux64.p);  // Y in place
+            for (int i = 0; i < k; ++i) {
+                const Inst& q = *I[i];
+                dgemm(q.ns, q.m, nt, -1.0, D + oD[i], q.ns, BC + (size_t)coff[i] * nt, nt, 1.0, GE + oG[i], q.ns);
+                absmax_async<double>(q.ns, q.ns, GE + oG[i], q.ns, amax + i);
+            }
+        } else {
+            // Y = T^-1 [B | Ct] for every instance: one GEMM through the shared inverse
+            convert_matrix<double, float>(nt, (int)C, BC, nt, B32, nt);
+            if (prec == SQBEM_FP32_IR)
+                sgemm(nt, (int)C, nt, 1.0f, ST.inv.p, nt, B32, nt, 0.0f, Y32, nt);
+            else
+                gemm16(prec, nt, (int)C, nt, 1.0f, ST.inv.p, nt, B32, nt, 0.0f, Y32, nt);
+            for (int i = 0; i < k; ++i) {
+                const Inst& q = *I[i];
+                convert_matrix<double, float>(q.ns, nt, D + oD[i], q.ns, D32 + oD[i], q.ns);
+                convert_matrix<double, float>(q.ns, q.m, GE + oG[i], q.ns, G32 + oG[i], q.ns);
+                const float* Yi = Y32 + (size_t)coff[i] * nt;
+                if (prec == SQBEM_FP32_IR)
+                    sgemm(q.ns, q.m, nt, -1.0f, D32 + oD[i], q.ns, Yi, nt, 1.0f, G32 + oG[i], q.ns);
+                else
+                    gemm16(prec, q.ns, q.m, nt, -1.0f, D32 + oD[i], q.ns, Yi, nt, 1.0f, G32 + oG[i], q.ns);
+                absmax_async<float>(q.ns, q.ns, G32 + oG[i], q.ns, amax + i);
+            }
+        }
+        hipLaunchKernelGGL(border_scale_kernel, dim3(ceil_div(k, 256)), dim3(256), 0, ctx().s(), k, amax, Amax_d.p,
+                           sc_d.p);
+        check_launch("border_scale");
+        for (int i = 0; i < k; ++i) border(i);
+        if (mx) getrf_np_batch_launch<float>(lu32);
+        else getrf_np_batch_launch<double>(lu64);
+        if (mx) {
+            hipLaunchKernelGGL(batch_norms_kernel, dim3(k), dim3(256), 0, ctx().s(), nt, Bt, Bsc, dcoff, dmlen, nrm);
+            check_launch("batch_norms");
+        }
+    };
+    const bool use_graph = g_use_graph && !prof_enabled();
+    hipEvent_t ev[3];
+    for (auto& x : ev) SQ_HIP(hipEventCreate(&x));
+    std::vector<int> hflag(k, 0);
+    std::vector<double> bn(k, 0.0);
+    try {
+        GraphExec g1, g2;
+        SQ_HIP(hipEventRecord(ev[0], ctx().s()));
+        if (use_graph) st.build_graphs += launch_as_graph(stage_assembly, &g1.x);
+        else stage_assembly();
+        SQ_HIP(hipEventRecord(ev[1], ctx().s()));
+        if (use_graph) st.build_graphs += launch_as_graph(stage_schur, &g2.x);
+        else stage_schur();
+        SQ_HIP(hipEventRecord(ev[2], ctx().s()));
+        SQ_HIP(hipMemcpyAsync(hflag.data(), flags, sizeof(int) * k, hipMemcpyDeviceToHost, ctx().s()));
+        if (mx) SQ_HIP(hipMemcpyAsync(bn.data(), nrm, sizeof(double) * k, hipMemcpyDeviceToHost, ctx().s()));
+        sync();
+    } catch (...) {
+        for (auto& x : ev) (void)hipEventDestroy(x);
+        throw;
+    }
+    float ms_asm = 0, ms_schur = 0;
+    (void)hipEventElapsedTime(&ms_asm, ev[0], ev[1]);
+    (void)hipEventElapsedTime(&ms_schur, ev[1], ev[2]);
+    for (auto& x : ev) (void)hipEventDestroy(x);
+    st.t_assembly += t_prep + 1e-3 * ms_asm;
+    double t1 = now_s();
+    // any bordered matrix that partial pivoting would interchange is refactored alone with
+    // pivoting (src/utils/linalg.hpp:42-77)
+    std::vector<char> pivoted(k, 0);
     for (int i = 0; i < k; ++i) {
         if (!hflag[i]) continue;
         pivoted[i] = 1;
@@ -577,7 +640,7 @@ void run_chunk(SharedTube& ST, std::vector<Inst*>& I, int prec, double mu, Arena
     }
     sync();
     double t2 = now_s();
-    st.t_schur += t2 - t1;
+    st.t_schur += 1e-3 * ms_schur + (t2 - t1);
 
     // bordered solves of all instances, B_i <- M_i^-1 B_i: one batched chained solve (the
     // pivoted ones gather their rows through the permutation first)
@@ -614,22 +677,12 @@ void run_chunk(SharedTube& ST, std::vector<Inst*>& I, int prec, double mu, Arena
         // iterative refinement of every instance's full bordered system
         //   [T B Ct; D E Cs; 0 scF 0] [f_t; f_s; w] = [b_t; b_s; 0]
         // residuals from the FP64 operator, corrections from the low-precision Schur pieces
-        {
-            std::vector<int> hc(coff.begin(), coff.begin() + k);
-            SQ_HIP(hipMemcpyAsync(dcoff, hc.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx().s()));
-            SQ_HIP(hipMemcpyAsync(dmlen, mv.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx().s()));
-            sync();  // hc is staging
-        }
         fill_zero(Xt, sizeof(double) * (size_t)nt * k);
         fill_zero(Xsc, sizeof(double) * C);
         convert_matrix<double, double>(nt, k, Bt, nt, Rt, nt);
         convert_matrix<double, double>((int)C, 1, Bsc, (int)C, Rsc, (int)C);
-        std::vector<double> bn(k), rn(k), rel(k, 1.0);
+        std::vector<double> rn(k), rel(k, 1.0);  // bn: the right-hand-side norms, read back with the verdicts
         std::vector<int> itc(k, 0);
-        hipLaunchKernelGGL(batch_norms_kernel, dim3(k), dim3(256), 0, ctx().s(), nt, Bt, Bsc, dcoff, dmlen, nrm);
-        check_launch("batch_norms");
-        SQ_HIP(hipMemcpyAsync(bn.data(), nrm, sizeof(double) * k, hipMemcpyDeviceToHost, ctx().s()));
-        sync();
         // job lists of the per-instance products (fixed for the whole refinement)
         std::vector<GvJob> gv_s, gv_t, gv_r;
         std::vector<CJob> cj;
@@ -644,7 +697,7 @@ void run_chunk(SharedTube& ST, std::vector<Inst*>& I, int prec, double mu, Arena
                             -1.0, 0});
             gv_r.push_back({q.ns, nt, q.m, D + oD[i], q.ns, Xt + (size_t)i * nt, GE + oG[i], q.ns, xs, Rsc + coff[i],
                             -1.0, 0});
-            cj.push_back({q.Ns, xs, dm[i].cen_xyz.p, dm[i].area.p, sc[i], {q.xc[0], q.xc[1], q.xc[2]},
+            cj.push_back({q.Ns, xs, dm[i].cen_xyz.p, dm[i].area.p, sc_d.p + i, {q.xc[0], q.xc[1], q.xc[2]},
                           Rsc + coff[i] + q.ns});
         }
         GvPlan ps, pt, pr;
@@ -862,6 +915,7 @@ int sqbem_batch_solve(int n, const sqbem_mesh* const* tubes, const sqbem_mesh* c
             stats[8] = st.pivoted;
             stats[9] = (double)need;
             stats[10] = st.graph_replays;
+            stats[11] = st.build_graphs;
         }
     });
 }

@@ -4,6 +4,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 
@@ -112,6 +113,29 @@ inline void sync() {
     SQ_HIP(hipStreamSynchronize(ctx().s()));
     check_device_error();
 }
+
+// Graph capture of a launch sequence. While capturing() is set, work issued on the library
+// stream is recorded, not run: nothing in it may synchronise, copy from host memory or read
+// results back, and the 16-bit GEMM autotuner takes decided tiles without timing any.
+bool& capturing();
+// Runs fn (launches on the library stream) captured into a HIP graph, launched once; *exec
+// receives the executable graph, which the caller destroys once the stream has passed it.
+// When the capture fails or records anything but kernel launches (a memset or copy node, as
+// a library call may add), the graph is discarded and fn runs directly: returns false, *exec
+// null. The results are the same either way.
+bool launch_as_graph(const std::function<void()>& fn, hipGraphExec_t* exec);
+struct GraphExec {
+    hipGraphExec_t x = nullptr;
+    GraphExec() = default;
+    GraphExec(const GraphExec&) = delete;
+    GraphExec& operator=(const GraphExec&) = delete;
+    ~GraphExec() {
+        if (x) {
+            (void)hipStreamSynchronize(ctx().s());
+            (void)hipGraphExecDestroy(x);
+        }
+    }
+};
 
 // Kernel timing for roofline reports: when enabled, a ProfScope records HIP events around
 // one launch on the library stream together with its algorithmic flops and bytes.

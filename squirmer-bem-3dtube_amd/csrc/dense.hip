@@ -1068,6 +1068,14 @@ static void poll(Cls& c) {
     }
 }
 
+// the tile of a decided class, else the rule; looks up without adding a class or events
+static int decided(int kind, int m, int n, int k, int rule) {
+    if (!g_mode) return rule;
+    auto it = g_table.find((unsigned)kind << 24 | (unsigned)lg(m) << 16 | (unsigned)lg(n) << 8 | (unsigned)lg(k));
+    if (it == g_table.end() || it->second.best < 0) return rule;
+    return it->second.best;
+}
+
 // Candidate for one call, with its timing events around the call when it explores.
 struct Probe {
     int cand;
@@ -1075,6 +1083,10 @@ struct Probe {
     int slot = -1;
     Probe(int kind, int m, int n, int k, int rule) : cand(rule) {
         if (!g_mode) return;
+        if (capturing()) {  // recorded, not run: no timing events, and the class stays as it is
+            cand = decided(kind, m, n, k, rule);
+            return;
+        }
         Cls& c = g_table[(unsigned)kind << 24 | (unsigned)lg(m) << 16 | (unsigned)lg(n) << 8 | (unsigned)lg(k)];
         poll(c);
         if (c.best >= 0) {
@@ -1093,13 +1105,6 @@ struct Probe {
     }
 };
 
-// the tile of a decided class, else the rule; looks up without adding a class or events
-static int decided(int kind, int m, int n, int k, int rule) {
-    if (!g_mode) return rule;
-    auto it = g_table.find((unsigned)kind << 24 | (unsigned)lg(m) << 16 | (unsigned)lg(n) << 8 | (unsigned)lg(k));
-    if (it == g_table.end() || it->second.best < 0) return rule;
-    return it->second.best;
-}
 
 static void reset(int mode) {
     SQ_HIP(hipStreamSynchronize(ctx().s()));
@@ -3218,7 +3223,6 @@ struct Plan {
         return off;
     }
 };
-DBuf<unsigned char> g_plan_dev;
 
 template <typename T>
 struct NpPlanner {
@@ -3301,53 +3305,87 @@ struct NpPlanner {
     }
 };
 
+}  // namespace
+
+template <typename T>
+void getrf_np_batch_launch(const BatchLuPlan<T>& bp);
+
+namespace {
+// a plan run once: tables uploaded, launches issued, then a wait (the tables are staging)
 template <typename T>
 void run_plan(const Plan& P) {
     if (P.steps.empty()) return;
-    if (g_plan_dev.n < P.bytes.size()) g_plan_dev.alloc(std::max<size_t>(P.bytes.size(), 1 << 20));
-    SQ_HIP(hipMemcpyAsync(g_plan_dev.p, P.bytes.data(), P.bytes.size(), hipMemcpyHostToDevice, ctx().s()));
-    for (const PlanStep& s : P.steps) {
-        const void* tab = g_plan_dev.p + s.off;
-        if (s.kind == PK_LU128) {
-            hipLaunchKernelGGL(lu128_batch_kernel<T>, dim3(s.count), dim3(LU_THREADS), 0, ctx().s(),
-                               (const Lu128Job<T>*)tab);
-        } else if (s.kind == PK_GEMM) {
-            hipLaunchKernelGGL(gemm_nn_batch_kernel<T>, dim3(s.gridx, s.count), dim3(256), 0, ctx().s(),
-                               (const GemmJob<T>*)tab);
-        } else {
-            hipLaunchKernelGGL(check_unit_bound_batch_kernel<T>, dim3(s.gridx, s.count), dim3(256), 0, ctx().s(),
-                               (const GemmJob<T>*)tab);
-        }
-        check_launch("batched factorisation step");
-    }
-    sync();  // the host plan is staging for the next batch
+    BatchLuPlan<T> bp;
+    for (const PlanStep& st : P.steps) bp.steps.push_back({st.kind, st.off, st.count, st.gridx});
+    bp.tab.alloc(P.bytes.size());
+    bp.tab.upload(P.bytes.data(), P.bytes.size());
+    getrf_np_batch_launch<T>(bp);  // no matrices: the steps only
+    sync();
 }
 }  // namespace
 
 template <typename T>
-void getrf_np_batch(const std::vector<BatchMat<T>>& mats) {
+void getrf_np_batch_prepare(const std::vector<BatchMat<T>>& mats, BatchLuPlan<T>& bp) {
     int nmax = 0;
     for (const auto& m : mats) {
         SQ_REQUIRE(m.n >= 0 && m.lda >= std::max(1, m.n) && m.A && m.aux && m.flag, SQBEM_ERR_ARG,
                    "getrf_np_batch: bad matrix");
         nmax = std::max(nmax, m.n);
     }
+    bp.mats = mats;
+    bp.steps.clear();
+    bp.flops = 0;
     if (!nmax) return;
-    for (const auto& m : mats) fill_zero(m.flag, sizeof(int));
     Plan P;
     NpPlanner<T> pl{P, mats};
     pl.rec(0, nmax);
-    double flops = 0;
-    for (const auto& m : mats) flops += (2.0 / 3.0) * m.n * (double)m.n * m.n;
-    {
-        ProfScope prof(PROF_PANEL, flops, 0.0);
-        run_plan<T>(P);
+    for (const PlanStep& st : P.steps) bp.steps.push_back({st.kind, st.off, st.count, st.gridx});
+    for (const auto& m : mats) bp.flops += (2.0 / 3.0) * m.n * (double)m.n * m.n;
+    if (!P.bytes.empty()) {
+        bp.tab.alloc(P.bytes.size());
+        bp.tab.upload(P.bytes.data(), P.bytes.size());
+        sync();  // the host plan is staging
     }
-    for (const auto& m : mats)
+}
+
+template <typename T>
+void getrf_np_batch_launch(const BatchLuPlan<T>& bp) {
+    if (bp.steps.empty()) return;
+    for (const auto& m : bp.mats) fill_zero(m.flag, sizeof(int));
+    {
+        ProfScope prof(bp.mats.empty() ? -1 : PROF_PANEL, bp.flops, 0.0);  // no matrices: a plain GEMM plan
+        for (const BatchLuStep& st : bp.steps) {
+            const void* tab = bp.tab.p + st.off;
+            if (st.kind == PK_LU128) {
+                hipLaunchKernelGGL(lu128_batch_kernel<T>, dim3(st.count), dim3(LU_THREADS), 0, ctx().s(),
+                                   (const Lu128Job<T>*)tab);
+            } else if (st.kind == PK_GEMM) {
+                hipLaunchKernelGGL(gemm_nn_batch_kernel<T>, dim3(st.gridx, st.count), dim3(256), 0, ctx().s(),
+                                   (const GemmJob<T>*)tab);
+            } else {
+                hipLaunchKernelGGL(check_unit_bound_batch_kernel<T>, dim3(st.gridx, st.count), dim3(256), 0, ctx().s(),
+                                   (const GemmJob<T>*)tab);
+            }
+            check_launch("batched factorisation step");
+        }
+    }
+    for (const auto& m : bp.mats)
         if (m.n > 0) tile_copy<T>(m.n, m.A, m.lda, m.aux + 2 * NpPlanner<T>::nbk(m.n) * NB * NB);
+}
+
+template <typename T>
+void getrf_np_batch(const std::vector<BatchMat<T>>& mats) {
+    BatchLuPlan<T> bp;
+    getrf_np_batch_prepare<T>(mats, bp);
+    getrf_np_batch_launch<T>(bp);
+    sync();  // the plan's table is released on return
 }
 template void getrf_np_batch<float>(const std::vector<BatchMat<float>>&);
 template void getrf_np_batch<double>(const std::vector<BatchMat<double>>&);
+template void getrf_np_batch_prepare<float>(const std::vector<BatchMat<float>>&, BatchLuPlan<float>&);
+template void getrf_np_batch_prepare<double>(const std::vector<BatchMat<double>>&, BatchLuPlan<double>&);
+template void getrf_np_batch_launch<float>(const BatchLuPlan<float>&);
+template void getrf_np_batch_launch<double>(const BatchLuPlan<double>&);
 
 // rows of a right-hand side gathered through a permutation: tmp = v[perm], then v = tmp
 template <typename T>

@@ -66,6 +66,25 @@ struct BatchMat {
 };
 template <typename T>
 void getrf_np_batch(const std::vector<BatchMat<T>>& mats);
+// the same in two halves: prepare plans the launches and uploads their job tables (and
+// synchronises); launch only issues kernels (the flags are cleared by a kernel), so it may
+// be captured into a graph while the plan lives
+struct BatchLuStep {
+    int kind;
+    size_t off;
+    int count, gridx;
+};
+template <typename T>
+struct BatchLuPlan {
+    std::vector<BatchMat<T>> mats;
+    std::vector<BatchLuStep> steps;
+    DBuf<unsigned char> tab;
+    double flops = 0;
+};
+template <typename T>
+void getrf_np_batch_prepare(const std::vector<BatchMat<T>>& mats, BatchLuPlan<T>& plan);
+template <typename T>
+void getrf_np_batch_launch(const BatchLuPlan<T>& plan);
 // B_j <- A_j^-1 B_j, one right-hand side each, from factors with getrf's solve aid (aux):
 // rows first gathered through perm (device, n entries: B <- B[perm], the row interchanges
 // of a pivoted factorisation as one permutation) unless perm is null

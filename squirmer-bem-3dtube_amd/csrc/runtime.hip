@@ -212,6 +212,58 @@ void resolve_pending() {
 
 bool prof_enabled() { return g_prof; }
 
+bool& capturing() {
+    static thread_local bool c = false;
+    return c;
+}
+
+bool launch_as_graph(const std::function<void()>& fn, hipGraphExec_t* exec) {
+    *exec = nullptr;
+    hipGraph_t graph = nullptr;
+    bool ok = hipStreamBeginCapture(ctx().s(), hipStreamCaptureModeRelaxed) == hipSuccess;
+    if (ok) {
+        capturing() = true;
+        try {
+            fn();
+        } catch (...) {
+            ok = false;
+        }
+        capturing() = false;
+        if (hipStreamEndCapture(ctx().s(), &graph) != hipSuccess) ok = false;
+    }
+    (void)hipGetLastError();
+    if (ok && graph) {
+        size_t n = 0;
+        ok = hipGraphGetNodes(graph, nullptr, &n) == hipSuccess;
+        std::vector<hipGraphNode_t> nodes(n);
+        if (ok && n) ok = hipGraphGetNodes(graph, nodes.data(), &n) == hipSuccess;
+        for (size_t i = 0; ok && i < n; ++i) {
+            hipGraphNodeType t;
+            ok = hipGraphNodeGetType(nodes[i], &t) == hipSuccess &&
+                 (t == hipGraphNodeTypeKernel || t == hipGraphNodeTypeEmpty);
+        }
+    }
+    if (ok && graph && hipGraphInstantiate(exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        *exec = nullptr;
+        ok = false;
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    if (!ok || !*exec) {
+        if (*exec) (void)hipGraphExecDestroy(*exec);
+        *exec = nullptr;
+        fn();
+        return false;
+    }
+    const hipError_t le = hipGraphLaunch(*exec, ctx().s());
+    if (le != hipSuccess) {
+        (void)hipGraphExecDestroy(*exec);
+        *exec = nullptr;
+        SQ_HIP(le);
+    }
+    return true;
+}
+
 ProfScope::ProfScope(int c, double f, double b, const char* kernel) : cls(c), flops(f), bytes(b), name(kernel) {
     if (c < 0 || !g_prof || !(g_prof_mask & (1u << c))) return;  // c < 0: never timed
     e0 = take_event();

@@ -106,12 +106,12 @@ __global__ void border_copy_kernel(int ns, const T* __restrict__ G, T* M) {
 }
 template <typename T>
 __global__ void border_rows_kernel(int Ns, T* M, const double* __restrict__ cen, const double* __restrict__ area,
-                                   double sc, double xcx, double xcy, double xcz) {
+                                   double sc, const double* scp, double xcx, double xcy, double xcz) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= Ns) return;
     const size_t ns = 3 * (size_t)Ns, n = ns + 6;
     const double rx = cen[3 * (size_t)p] - xcx, ry = cen[3 * (size_t)p + 1] - xcy, rz = cen[3 * (size_t)p + 2] - xcz;
-    const double A = area[p] * sc;
+    const double A = area[p] * (scp ? *scp : sc);  // the scale from device memory when given
     auto at = [&](size_t r, size_t c) -> T& { return M[r + c * n]; };
     for (int a = 0; a < 3; ++a) at(ns + a, 3 * p + a) = (T)A;
     at(ns + 3, 3 * p + 1) = (T)(-A * rz); at(ns + 3, 3 * p + 2) = (T)(A * ry);
@@ -304,16 +304,19 @@ template void absmax_async<double>(int, int, const double*, long, unsigned long 
 template void absmax_async<float>(int, int, const float*, long, unsigned long long*);
 
 template <typename T>
-void border_build(int Ns, const T* G, T* M, const double* cen_xyz, const double* area, double sc, const double* xc) {
+void border_build(int Ns, const T* G, T* M, const double* cen_xyz, const double* area, double sc, const double* xc,
+                  const double* sc_dev) {
     const int ns = 3 * Ns, n = ns + 6;
     hipLaunchKernelGGL(border_copy_kernel<T>, dim3(std::min<unsigned>(ceil_div((size_t)n * n, 256), 65535)), dim3(256),
                        0, ctx().s(), ns, G, M);
     hipLaunchKernelGGL(border_rows_kernel<T>, dim3(ceil_div(Ns, 256)), dim3(256), 0, ctx().s(), Ns, M, cen_xyz, area, sc,
-                       xc[0], xc[1], xc[2]);
+                       sc_dev, xc[0], xc[1], xc[2]);
     check_launch("border");
 }
-template void border_build<double>(int, const double*, double*, const double*, const double*, double, const double*);
-template void border_build<float>(int, const float*, float*, const double*, const double*, double, const double*);
+template void border_build<double>(int, const double*, double*, const double*, const double*, double, const double*,
+                                   const double*);
+template void border_build<float>(int, const float*, float*, const double*, const double*, double, const double*,
+                                  const double*);
 
 void constraint_apply(int Ns, int nrhs, const double* fs, long ldf, const double* cen_xyz, const double* area,
                       double sc, const double* xc, double* out, long ldo) {
@@ -538,7 +541,7 @@ static void swimmer_build(sqbem_swimmer* s) {
         hipLaunchKernelGGL(border_copy_kernel<float>, dim3(std::min<unsigned>(ceil_div((size_t)n * n, 256), 65535)),
                            dim3(256), 0, ctx().s(), ns, s->G32.p, s->M32.p);
         hipLaunchKernelGGL(border_rows_kernel<float>, dim3(ceil_div(Ns, 256)), dim3(256), 0, ctx().s(), Ns, s->M32.p,
-                           s->dm.cen_xyz.p, s->dm.area.p, s->sc32, s->xc[0], s->xc[1], s->xc[2]);
+                           s->dm.cen_xyz.p, s->dm.area.p, s->sc32, nullptr, s->xc[0], s->xc[1], s->xc[2]);
         check_launch("border");
         s->Mpiv.alloc(n);
         s->Maux.alloc(getrf_aux_elems(n));
@@ -573,7 +576,7 @@ static void bordered_factor_fp64(sqbem_swimmer* s, DBuf<double>& M, DBuf<int>& p
     hipLaunchKernelGGL(border_copy_kernel<double>, dim3(std::min<unsigned>(ceil_div((size_t)n * n, 256), 65535)),
                        dim3(256), 0, ctx().s(), ns, s->G.p, M.p);
     hipLaunchKernelGGL(border_rows_kernel<double>, dim3(ceil_div(Ns, 256)), dim3(256), 0, ctx().s(), Ns, M.p,
-                       s->dm.cen_xyz.p, s->dm.area.p, sc, s->xc[0], s->xc[1], s->xc[2]);
+                       s->dm.cen_xyz.p, s->dm.area.p, sc, nullptr, s->xc[0], s->xc[1], s->xc[2]);
     check_launch("border");
     piv.alloc(n);
     int info = getrf<double>(SQBEM_FP64, n, M.p, n, piv.p);

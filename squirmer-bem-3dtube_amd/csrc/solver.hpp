@@ -55,7 +55,8 @@ void absmax_async(int m, int n, const T* A, long lda, unsigned long long* out);
 // bordered matrix M ((ns + 6)^2) of solve_free from G = [Gs | Gc] (ns x (ns + 6)) and the
 // force/torque rows scaled by sc (src/swimmer.hpp:107-125)
 template <typename T>
-void border_build(int Ns, const T* G, T* M, const double* cen_xyz, const double* area, double sc, const double* xc);
+void border_build(int Ns, const T* G, T* M, const double* cen_xyz, const double* area, double sc, const double* xc,
+                  const double* sc_dev = nullptr);  // sc_dev (device, one double): the scale, read there
 
 // out(0..5, k) = -sc F f_s(:, k): the constraint rows applied to sphere tractions
 void constraint_apply(int Ns, int nrhs, const double* fs, long ldf, const double* cen_xyz, const double* area,

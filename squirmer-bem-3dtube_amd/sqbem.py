@@ -656,18 +656,19 @@ def batch_solve(instances, mu=1.0, precision=FP64, mem_budget_gb=0.0) -> BatchRe
     Om = np.zeros((n, 3))
     it = np.zeros(n, dtype=np.int32)
     res = np.zeros(n)
-    st = np.zeros(11)
+    st = np.zeros(12)
     check(lib().sqbem_batch_solve(n, ctypes.cast(tubes, ctypes.c_void_p), ctypes.cast(spheres, ctypes.c_void_p),
                                   _ptr(c), _ptr(e), _ptr(B), mu, precision, float(mem_budget_gb), _ptr(U), _ptr(Om),
                                   _ptr(it), _ptr(res), _ptr(st)))
     stats = {"tube_s": st[0], "assembly_s": st[1], "schur_s": st[2], "solve_s": st[3], "unique_tubes": int(st[4]),
              "chunks": int(st[5]), "max_iters": int(st[6]), "worst_residual": st[7], "pivoted": int(st[8]),
-             "workspace_gb": st[9] / 1e9, "graph_replays": int(st[10])}
+             "workspace_gb": st[9] / 1e9, "graph_replays": int(st[10]), "build_graphs": int(st[11])}
     return BatchResult(U, Om, it, res, stats)
 
 
 def batch_options(use_graph=True):
-    """use_graph: capture each chunk's refinement step as a HIP graph and replay it."""
+    """use_graph: run each chunk's two build stages as captured HIP graphs, and capture its
+    refinement step as a graph and replay it."""
     check(lib().sqbem_batch_options(1 if use_graph else 0))
 
 
