@@ -142,6 +142,13 @@ int sqbem_dgesv(int n, int nrhs, double* A, int lda, int* ipiv, double* B, int l
 /* FP32 factorisation; kind selects the trailing-update MFMA (SQBEM_FP32_IR/FP16_IR/BF16_IR) */
 int sqbem_sgetrf(int kind, int n, float* A, int lda, int* ipiv, int* info);
 int sqbem_sgetrs(int n, int nrhs, const float* LU, int lda, const int* ipiv, float* B, int ldb);
+/* How the calling thread's most recent LU factorisation went (a diagnostic, no reference
+ * counterpart; dgetrf_ always pivots): 0 partial pivoting throughout (orders below 512),
+ * 1 factored without row interchanges, every multiplier verified |l| <= 1 (exactly the
+ * factors partial pivoting gives), 2 that attempt was rejected and the matrix refactored
+ * with partial pivoting. A swimmer's construction factors the tube, then the bordered
+ * Schur matrix: afterwards this tells about the bordered matrix. */
+int sqbem_getrf_path(void);
 
 /* Tile autotuner of the 16-bit GEMMs (no reference counterpart): mode 1 (the default)
  * learns the fastest of fourteen candidates (the SQBEM_PROF_G16Q list: workgroup tile and

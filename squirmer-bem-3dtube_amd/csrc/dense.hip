@@ -3077,22 +3077,23 @@ __global__ void iota_pivots_kernel(int n, int* ipiv) {
 
 // orders at least this large try the no-pivot factorisation first
 constexpr int NP_MIN = 512;
-static int g_last_getrf_path = 0;  // 0 pivoting, 1 no-pivot verified, 2 no-pivot rejected
+static thread_local int g_last_getrf_path = 0;  // 0 pivoting, 1 no-pivot verified, 2 no-pivot rejected
 
 // aux (optional, getrf_aux_elems(n) elements): receives the solve aid for getrs: the
 // inverted 128 x 128 diagonal blocks of L then of U, then the tiled copy of the factors.
 template <typename T>
-int getrf(int kind, int n, T* A, int lda, int* ipiv, T* aux) {
+int getrf(int kind, int n, T* A, int lda, int* ipiv, T* aux, const std::function<void()>* restore) {
     T* inv = aux;
     if (n <= 0) return 0;
     SQ_REQUIRE(lda >= n, SQBEM_ERR_ARG, "getrf: lda < n");
     g_last_getrf_path = 0;
     const size_t nb = (size_t)(n + NB - 1) / NB;
     if (n >= NP_MIN) {
-        g_np.ensure(n, sizeof(T), true);
+        g_np.ensure(n, sizeof(T), !restore);
         T* backup = (T*)g_np.backup.p;
-        SQ_HIP(hipMemcpy2DAsync(backup, (size_t)n * sizeof(T), A, (size_t)lda * sizeof(T), (size_t)n * sizeof(T), n,
-                                hipMemcpyDeviceToDevice, ctx().s()));
+        if (!restore)
+            SQ_HIP(hipMemcpy2DAsync(backup, (size_t)n * sizeof(T), A, (size_t)lda * sizeof(T), (size_t)n * sizeof(T),
+                                    n, hipMemcpyDeviceToDevice, ctx().s()));
         g_np.flag.zero();
         T* Linv = inv ? inv : (T*)g_np.inv.p;
         getrf_np<T>(kind, A, lda, n, n, 0, Linv, Linv + nb * NB * NB, g_np.flag.p);
@@ -3105,8 +3106,11 @@ int getrf(int kind, int n, T* A, int lda, int* ipiv, T* aux) {
             g_last_getrf_path = 1;
             return 0;
         }
-        SQ_HIP(hipMemcpy2DAsync(A, (size_t)lda * sizeof(T), backup, (size_t)n * sizeof(T), (size_t)n * sizeof(T), n,
-                                hipMemcpyDeviceToDevice, ctx().s()));
+        if (restore)
+            (*restore)();
+        else
+            SQ_HIP(hipMemcpy2DAsync(A, (size_t)lda * sizeof(T), backup, (size_t)n * sizeof(T), (size_t)n * sizeof(T),
+                                    n, hipMemcpyDeviceToDevice, ctx().s()));
         g_last_getrf_path = 2;
     }
     g_panel.ensure(256, n, sizeof(T));
@@ -3126,8 +3130,8 @@ int getrf(int kind, int n, T* A, int lda, int* ipiv, T* aux) {
     }
     return st[0];
 }
-template int getrf<double>(int, int, double*, int, int*, double*);
-template int getrf<float>(int, int, float*, int, int*, float*);
+template int getrf<double>(int, int, double*, int, int*, double*, const std::function<void()>*);
+template int getrf<float>(int, int, float*, int, int*, float*, const std::function<void()>*);
 
 // number of panels that needed row interchanges in the last getrf (0 = diagonal pivots)
 int last_getrf_pivoted_panels(int n) {
@@ -3644,6 +3648,8 @@ int sqbem_tune(int mode) {
         tune::reset(mode);
     });
 }
+
+int sqbem_getrf_path(void) { return g_last_getrf_path; }
 
 int sqbem_tune_info(int* classes, int* decided, int* counts) {
     return guarded([&] { tune::info(classes, decided, counts); });

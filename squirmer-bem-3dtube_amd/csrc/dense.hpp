@@ -1,6 +1,7 @@
 // dense.hpp -- internal interface of the dense linear algebra (dense.hip).
 #pragma once
 #include <cstddef>
+#include <functional>
 #include <vector>
 
 #include "common.hpp"
@@ -26,8 +27,12 @@ void convert_matrix(int m, int n, const Ti* A, int lda, To* B, int ldb);
 // kind selects the GEMM of the FP32 variant (SQBEM_FP32_IR / FP16_IR / BF16_IR).
 // aux (optional, getrf_aux_elems(n) elements): receives a solve aid for getrs (inverted
 // 128 x 128 diagonal blocks of L and U, and a tiled copy of the factors).
+// restore (optional): rebuilds A as it was on entry (e.g. from the matrix it was converted
+// from); given, the first attempt without row interchanges keeps no copy of A and calls it
+// instead when it has to fall back to partial pivoting.
 template <typename T>
-int getrf(int kind, int n, T* A, int lda, int* ipiv, T* aux = nullptr);
+int getrf(int kind, int n, T* A, int lda, int* ipiv, T* aux = nullptr,
+          const std::function<void()>* restore = nullptr);
 inline size_t getrf_aux_elems(int n) {
     const size_t nb = (size_t)((n + 127) / 128);
     return (2 * nb + nb * nb) * 128 * 128;

@@ -353,9 +353,12 @@ void sq::tube_build(sqbem_tube* t) {
     int info;
     if (mixed(t->prec)) {
         t->T32.alloc((size_t)nt * nt);
-        convert_matrix<double, float>(nt, nt, t->T64.p, nt, t->T32.p, nt);
+        const std::function<void()> convert = [&] {
+            convert_matrix<double, float>(nt, nt, t->T64.p, nt, t->T32.p, nt);
+        };
+        convert();
         t->aux32.alloc(getrf_aux_elems(nt));
-        info = getrf<float>(t->prec, nt, t->T32.p, nt, t->ipiv.p, t->aux32.p);
+        info = getrf<float>(t->prec, nt, t->T32.p, nt, t->ipiv.p, t->aux32.p, &convert);
     } else {
         t->aux64.alloc(getrf_aux_elems(nt));
         info = getrf<double>(SQBEM_FP64, nt, t->T64.p, nt, t->ipiv.p, t->aux64.p);
@@ -538,14 +541,13 @@ static void swimmer_build(sqbem_swimmer* s) {
         for (int p = 0; p < Ns; ++p) Amax = std::max(Amax, 5.0 * s->S.area[p]);
         s->sc32 = amax / Amax;
         s->M32.alloc((size_t)n * n);
-        hipLaunchKernelGGL(border_copy_kernel<float>, dim3(std::min<unsigned>(ceil_div((size_t)n * n, 256), 65535)),
-                           dim3(256), 0, ctx().s(), ns, s->G32.p, s->M32.p);
-        hipLaunchKernelGGL(border_rows_kernel<float>, dim3(ceil_div(Ns, 256)), dim3(256), 0, ctx().s(), Ns, s->M32.p,
-                           s->dm.cen_xyz.p, s->dm.area.p, s->sc32, nullptr, s->xc[0], s->xc[1], s->xc[2]);
-        check_launch("border");
+        const std::function<void()> border = [&] {
+            border_build<float>(Ns, s->G32.p, s->M32.p, s->dm.cen_xyz.p, s->dm.area.p, s->sc32, s->xc);
+        };
+        border();
         s->Mpiv.alloc(n);
         s->Maux.alloc(getrf_aux_elems(n));
-        int info = getrf<float>(kind, n, s->M32.p, n, s->Mpiv.p, s->Maux.p);
+        int info = getrf<float>(kind, n, s->M32.p, n, s->Mpiv.p, s->Maux.p, &border);
         SQ_REQUIRE(info == 0, SQBEM_ERR_SINGULAR, "dgesv failed");
     }
     sync();
@@ -708,10 +710,13 @@ static void solve_mixed(sqbem_swimmer* s, int k, const double* u_host, bool towe
         // Gs factor for the towed corrections (built on first use)
         if (!s->Gs32.p) {
             s->Gs32.alloc((size_t)ns * ns);
-            convert_matrix<float, float>(ns, ns, s->G32.p, ns, s->Gs32.p, ns);
+            const std::function<void()> copy = [&] {
+                convert_matrix<float, float>(ns, ns, s->G32.p, ns, s->Gs32.p, ns);
+            };
+            copy();
             s->Gspiv.alloc(ns);
             s->Gsaux.alloc(getrf_aux_elems(ns));
-            int info = getrf<float>(s->prec, ns, s->Gs32.p, ns, s->Gspiv.p, s->Gsaux.p);
+            int info = getrf<float>(s->prec, ns, s->Gs32.p, ns, s->Gspiv.p, s->Gsaux.p, &copy);
             SQ_REQUIRE(info == 0, SQBEM_ERR_SINGULAR, "dgesv failed");
         }
         DBuf<double> w(6);

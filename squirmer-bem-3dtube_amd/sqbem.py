@@ -88,6 +88,7 @@ _SIGS = {
     "sqbem_trajectory_gradient": (_i, [_d, _d, _d, _d, _d, _d, _i, _i, _d, _i, _i, _i, _i, _i, _vp]),
     "sqbem_batch_solve": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _d, _i, _d, _vp, _vp, _vp, _vp, _vp]),
     "sqbem_batch_options": (_i, [_i]),
+    "sqbem_getrf_path": (_i, []),
     "sqbem_velocity_at": (_i, [_vp, _vp, _vp, _d, _vp, _i, _vp]),
     "sqbem_duct_create": (_vp, [_vp, _d]),
     "sqbem_duct_solve": (_i, [_vp, _d, _d, _vp, _vp, _vp]),
@@ -664,6 +665,12 @@ def batch_solve(instances, mu=1.0, precision=FP64, mem_budget_gb=0.0) -> BatchRe
              "chunks": int(st[5]), "max_iters": int(st[6]), "worst_residual": st[7], "pivoted": int(st[8]),
              "workspace_gb": st[9] / 1e9, "graph_replays": int(st[10]), "build_graphs": int(st[11])}
     return BatchResult(U, Om, it, res, stats)
+
+
+def getrf_path():
+    """How the most recent LU factorisation went: 0 partial pivoting throughout (orders below
+    512), 1 without row interchanges (verified), 2 that attempt rejected, then pivoted."""
+    return int(lib().sqbem_getrf_path())
 
 
 def batch_options(use_graph=True):
