@@ -208,8 +208,15 @@ typedef struct sqbem_swimmer sqbem_swimmer;
 typedef struct sqbem_duct sqbem_duct;
 
 /* Tube::Tube + Tube::factor (src/swimmer.hpp:40-60): assembles T = SL/(8 pi mu) with the
- * rank-one deflation (src/swimmer.hpp:64-74) on the device and factors it there. */
+ * rank-one deflation (src/swimmer.hpp:64-74) on the device and factors it there. Returns
+ * once the factorisation is issued: the device goes on factoring while the caller carries
+ * on (a swimmer created on the tube meanwhile assembles its coupling blocks beside it), and
+ * every use of the factors (a swimmer on the tube, the factor view's solve and get,
+ * t_factor below) first waits for it to finish. A singular tube matrix is reported there
+ * ("dgetrf failed"), not here. */
 sqbem_tube* sqbem_tube_create(const sqbem_mesh* mesh, double mu, int precision);
+/* Nt, and the seconds spent assembling and factoring; asking for t_factor waits for the
+ * factorisation (any pointer may be NULL). */
 int sqbem_tube_info(const sqbem_tube* t, int* Nt, double* t_assembly, double* t_factor);
 /* the tube's factor (Tube::lu, src/swimmer.hpp:38): a view valid while the tube lives. In
  * the refined modes sqbem_lu_solve corrects the FP32 factors' solution by iterative

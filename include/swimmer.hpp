@@ -37,7 +37,7 @@ struct Tube {
     double R = 1, L = 4, mu = 1;
     LUFactor lu;  // view onto the factors held in HBM
     int precision = SQBEM_FP64;
-    double t_assembly = 0, t_factor = 0;
+    double t_assembly = 0, t_factor = 0;  // seconds, filled by times()
 
     Tube() = default;
     Tube(double R_, double L_, double a, int n_theta, int n_r = 4, double mu_ = 1, int precision_ = SQBEM_FP64)
@@ -49,9 +49,11 @@ struct Tube {
         sqbem_tube* t = sqbem_tube_create(m.get(), mu, precision);
         if (!t) throw std::runtime_error(sqbem_last_error());
         handle_.reset(t, sqbem_tube_free);
-        sqbem_check(sqbem_tube_info(t, nullptr, &t_assembly, &t_factor));
+        // the device may still be factoring: the factors are waited for where first used
         lu.attach(sqbem_tube_lu(t));
     }
+    // assembly and factorisation seconds (waits for the factorisation to finish)
+    void times() { sqbem_check(sqbem_tube_info(handle_.get(), nullptr, &t_assembly, &t_factor)); }
 
     // Wielandt deflation of the null mode f = c m of panels [p0, p0 + np) of M, applied to
     // the host matrix A at row/column offset off: A += u v^T with u = m / (8 pi mu sqrt(At))

@@ -308,6 +308,7 @@ void build_shared_tube(SharedTube& S, const bem::Mesh& mesh, double mu, int prec
     S.t->mu = mu;
     S.t->prec = prec;
     tube_build(S.t.get());
+    tube_ready(S.t.get());
     if (mixed(prec)) {
         const int nt = S.t->nt;
         S.inv.alloc((size_t)nt * nt);

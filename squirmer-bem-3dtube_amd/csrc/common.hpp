@@ -37,13 +37,26 @@ struct Context {
     int device = -1;
     hipStream_t stream = nullptr;      // library-owned stream
     hipStream_t user_stream = nullptr; // caller override (sqbem_set_stream)
+    hipStream_t side = nullptr;        // second library stream (default priority)
+    hipEvent_t join_ev = nullptr;
+    hipStream_t redirect = nullptr;    // set inside a SideScope: launches go to side
     int cu_count = 0;
     size_t hbm_bytes = 0;
     char name[256] = {0};
-    hipStream_t s() const { return user_stream ? user_stream : stream; }
+    hipStream_t s() const { return redirect ? redirect : (user_stream ? user_stream : stream); }
 };
 
 Context& ctx();          // initialises device 0 on first use
+
+// Work issued inside this scope goes to the side stream, beside what the library stream
+// holds (it must not depend on that work); when the scope ends the library stream waits
+// for it. Not nested.
+struct SideScope {
+    SideScope();
+    ~SideScope();
+    SideScope(const SideScope&) = delete;
+    SideScope& operator=(const SideScope&) = delete;
+};
 void ensure_init();      // throws on failure
 void upload_tables();    // quadrature tables to constant memory (assemble.hip)
 // device address of the no-allocation GEMM's scratch on this device (dense.hip), looked up

@@ -2433,17 +2433,23 @@ __device__ __forceinline__ void lu128_panel(C* M, int t, int& mybad) {
     C a[32];
 #pragma unroll
     for (int c = 0; c < 32; ++c) a[c] = Mr[c];
+    // Branch-free (selects, no divergent branches): the whole unrolled panel is one basic
+    // block, so the scheduler overlaps each step's pivot reciprocal with the previous step's
+    // updates. The values are those of the guarded form.
+    int bad = 0;
 #pragma unroll
     for (int j = 0; j < 32; ++j) {
         const C piv = readlane_c(a[j], j);  // lane j holds row p0 + j in every wave
-        if (lane == 0 && !(piv != C(0) && fabs(piv) <= std::numeric_limits<C>::max())) mybad = 1;
+        bad |= (int)!(piv != C(0) && fabs(piv) <= std::numeric_limits<C>::max());
+        const C rp = C(1) / piv;
         const bool below = valid && (!top || lane > j);
-        const C lm = below ? a[j] * (C(1) / piv) : C(0);
-        if (below && !(fabs(lm) <= C(1))) mybad = 1;
+        const C lm = below ? a[j] * rp : C(0);
+        bad |= (int)(below & !(fabs(lm) <= C(1)));
         a[j] = below ? lm : a[j];
 #pragma unroll
         for (int c = j + 1; c < 32; ++c) a[c] = fma(-lm, readlane_c(a[c], j), a[c]);  // lm = 0 above
     }
+    mybad |= bad;
     if (valid && (wv == 0 || !top)) {
 #pragma unroll
         for (int c = 0; c < 32; ++c) Mr[c] = a[c];
@@ -2757,18 +2763,6 @@ static void lu128_launch(int grid, T* A, int lda, int n, T* Linv, T* Uinv, int* 
 }
 
 
-struct NoPivotWork {
-    DBuf<char> inv;     // Linv blocks, then Uinv blocks (NB x NB each, storage type)
-    DBuf<char> backup;  // copy of the matrix for the pivoting fallback
-    DBuf<int> flag;
-    void ensure(int n, size_t elem, bool need_backup) {
-        const size_t nb = (size_t)(n + NB - 1) / NB;
-        if (inv.n < 2 * nb * NB * NB * elem) inv.alloc(2 * nb * NB * NB * elem);
-        if (need_backup && backup.n < (size_t)n * n * elem) backup.alloc((size_t)n * n * elem);
-        if (!flag.p) flag.alloc(1);
-    }
-};
-static NoPivotWork g_np;
 
 template <typename T>
 static void getrf_np(int kind, T* A, int lda, int m, int w, int base, T* Linv, T* Uinv, int* flag) {
@@ -3082,35 +3076,56 @@ static thread_local int g_last_getrf_path = 0;  // 0 pivoting, 1 no-pivot verifi
 // aux (optional, getrf_aux_elems(n) elements): receives the solve aid for getrs: the
 // inverted 128 x 128 diagonal blocks of L then of U, then the tiled copy of the factors.
 template <typename T>
-int getrf(int kind, int n, T* A, int lda, int* ipiv, T* aux, const std::function<void()>* restore) {
-    T* inv = aux;
+void getrf_begin(GetrfJob<T>& j) {
+    j.started = true;
+    if (j.n <= 0) return;
+    SQ_REQUIRE(j.lda >= j.n, SQBEM_ERR_ARG, "getrf: lda < n");
+    if (j.n < NP_MIN) return;
+    const int n = j.n;
+    const size_t nb = (size_t)(n + NB - 1) / NB;
+    if (!j.restore) {
+        j.backup.alloc((size_t)n * n);
+        SQ_HIP(hipMemcpy2DAsync(j.backup.p, (size_t)n * sizeof(T), j.A, (size_t)j.lda * sizeof(T),
+                                (size_t)n * sizeof(T), n, hipMemcpyDeviceToDevice, ctx().s()));
+    }
+    j.flag.alloc(1);
+    j.flag.zero();
+    T* Linv = j.aux;
+    if (!Linv) {
+        j.inv.alloc(2 * nb * NB * NB);
+        Linv = j.inv.p;
+    }
+    getrf_np<T>(j.kind, j.A, j.lda, n, n, 0, Linv, Linv + nb * NB * NB, j.flag.p);
+}
+
+template <typename T>
+int getrf_end(GetrfJob<T>& j) {
+    SQ_REQUIRE(j.started, SQBEM_ERR_STATE, "getrf: end without begin");
+    j.started = false;
+    const int n = j.n, lda = j.lda;
+    T* A = j.A;
+    T* aux = j.aux;
+    int* ipiv = j.ipiv;
     if (n <= 0) return 0;
-    SQ_REQUIRE(lda >= n, SQBEM_ERR_ARG, "getrf: lda < n");
     g_last_getrf_path = 0;
     const size_t nb = (size_t)(n + NB - 1) / NB;
     if (n >= NP_MIN) {
-        g_np.ensure(n, sizeof(T), !restore);
-        T* backup = (T*)g_np.backup.p;
-        if (!restore)
-            SQ_HIP(hipMemcpy2DAsync(backup, (size_t)n * sizeof(T), A, (size_t)lda * sizeof(T), (size_t)n * sizeof(T),
-                                    n, hipMemcpyDeviceToDevice, ctx().s()));
-        g_np.flag.zero();
-        T* Linv = inv ? inv : (T*)g_np.inv.p;
-        getrf_np<T>(kind, A, lda, n, n, 0, Linv, Linv + nb * NB * NB, g_np.flag.p);
         int bad = 0;
-        g_np.flag.download(&bad, 1);
+        j.flag.download(&bad, 1);
+        j.inv.release();
         if (!bad) {
             hipLaunchKernelGGL(iota_pivots_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx().s(), n, ipiv);
             check_launch("iota_pivots");
             if (aux) tile_copy<T>(n, A, lda, aux + 2 * nb * NB * NB);
+            j.backup.release();  // stream-ordered reuse only
             g_last_getrf_path = 1;
             return 0;
         }
-        if (restore)
-            (*restore)();
+        if (j.restore)
+            j.restore();
         else
-            SQ_HIP(hipMemcpy2DAsync(A, (size_t)lda * sizeof(T), backup, (size_t)n * sizeof(T), (size_t)n * sizeof(T),
-                                    n, hipMemcpyDeviceToDevice, ctx().s()));
+            SQ_HIP(hipMemcpy2DAsync(A, (size_t)lda * sizeof(T), j.backup.p, (size_t)n * sizeof(T),
+                                    (size_t)n * sizeof(T), n, hipMemcpyDeviceToDevice, ctx().s()));
         g_last_getrf_path = 2;
     }
     g_panel.ensure(256, n, sizeof(T));
@@ -3118,20 +3133,38 @@ int getrf(int kind, int n, T* A, int lda, int* ipiv, T* aux, const std::function
     g_panel.status.zero();
     g_panel.flags.zero();
     g_bar_parity = 0;
-    getrf_rec<T>(kind, A, lda, n, n, ipiv, 0);
+    getrf_rec<T>(j.kind, A, lda, n, n, ipiv, 0);
     hipLaunchKernelGGL(ipiv_to_one_based<T>, dim3(ceil_div(n, 256)), dim3(256), 0, ctx().s(), n, ipiv, 1);
     check_launch("ipiv");
     int st[2];
     g_panel.status.download(st, 2);
     SQ_REQUIRE(st[1] == 0, SQBEM_ERR_HIP, "getrf: panel barrier timed out");
     if (aux && st[0] == 0) {
-        lu128_launch<T>((int)nb, A, lda, n, inv, inv + nb * NB * NB, nullptr, 0);
+        lu128_launch<T>((int)nb, A, lda, n, aux, aux + nb * NB * NB, nullptr, 0);
         tile_copy<T>(n, A, lda, aux + 2 * nb * NB * NB);
     }
     return st[0];
 }
+
+template <typename T>
+int getrf(int kind, int n, T* A, int lda, int* ipiv, T* aux, const std::function<void()>* restore) {
+    GetrfJob<T> j;
+    j.kind = kind;
+    j.n = n;
+    j.lda = lda;
+    j.A = A;
+    j.ipiv = ipiv;
+    j.aux = aux;
+    if (restore) j.restore = *restore;
+    getrf_begin<T>(j);
+    return getrf_end<T>(j);
+}
 template int getrf<double>(int, int, double*, int, int*, double*, const std::function<void()>*);
 template int getrf<float>(int, int, float*, int, int*, float*, const std::function<void()>*);
+template void getrf_begin<double>(GetrfJob<double>&);
+template void getrf_begin<float>(GetrfJob<float>&);
+template int getrf_end<double>(GetrfJob<double>&);
+template int getrf_end<float>(GetrfJob<float>&);
 
 // number of panels that needed row interchanges in the last getrf (0 = diagonal pivots)
 int last_getrf_pivoted_panels(int n) {

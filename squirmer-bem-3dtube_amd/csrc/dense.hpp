@@ -33,6 +33,26 @@ void convert_matrix(int m, int n, const Ti* A, int lda, To* B, int ldb);
 template <typename T>
 int getrf(int kind, int n, T* A, int lda, int* ipiv, T* aux = nullptr,
           const std::function<void()>* restore = nullptr);
+// The same in two halves (getrf = begin, then end): begin issues the factorisation without
+// row interchanges (orders of 512 and more) and returns without waiting; end waits for its
+// verdict, finishes, or refactors with partial pivoting, and returns the LAPACK info. In
+// between the stream may carry other work that leaves A, ipiv and aux alone. The job owns
+// the verdict word and, without restore, the copy of A (and, without aux, the inverses).
+template <typename T>
+struct GetrfJob {
+    int kind = 0, n = 0, lda = 0;
+    T* A = nullptr;
+    int* ipiv = nullptr;
+    T* aux = nullptr;
+    std::function<void()> restore;
+    DBuf<int> flag;
+    DBuf<T> backup, inv;
+    bool started = false;
+};
+template <typename T>
+void getrf_begin(GetrfJob<T>& job);
+template <typename T>
+int getrf_end(GetrfJob<T>& job);
 inline size_t getrf_aux_elems(int n) {
     const size_t nb = (size_t)((n + 127) / 128);
     return (2 * nb + nb * nb) * 128 * 128;

@@ -46,6 +46,7 @@ std::vector<double> col_norms(int n, int nrhs, const double* a, int ld) {
 }  // namespace
 
 void sqbem_lu::solve_device(int nrhs, double* B, int ldb) const {
+    if (owner) tube_ready(owner);
     if (LU32) {
         DBuf<float> d32((size_t)n * nrhs);
         if (!A64) {  // no FP64 operator at hand: the FP32 factors' solution
@@ -155,6 +156,7 @@ int sqbem_lu_solve(const sqbem_lu* lu, int nrhs, double* B, int ldb) {
 int sqbem_lu_get(const sqbem_lu* lu, double* LUh, int* piv) {
     return guarded([&] {
         SQ_REQUIRE(lu, SQBEM_ERR_ARG, "null factor");
+        if (lu->owner) tube_ready(lu->owner);
         const size_t nn = (size_t)lu->n * lu->n;
         if (LUh) {
             if (lu->LU32) {

@@ -3,7 +3,14 @@
 #pragma once
 #include "common.hpp"
 
+struct sqbem_tube;
+namespace sq {
+// finishes the tube's factorisation if it is still running (solver.hip)
+void tube_ready(const sqbem_tube* t);
+}  // namespace sq
+
 struct sqbem_lu {
+    const sqbem_tube* owner = nullptr;  // a view: the tube whose factors these are
     int n = 0;
     int kind = SQBEM_FP64;       // trailing-update kind of an FP32 factor
     const double* LU = nullptr;  // FP64 factors, or

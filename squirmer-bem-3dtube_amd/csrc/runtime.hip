@@ -38,6 +38,8 @@ static void init_device(int device) {
     g_ctx.hbm_bytes = prop.totalGlobalMem;
     std::snprintf(g_ctx.name, sizeof g_ctx.name, "%s (%s)", prop.name, prop.gcnArchName);
     if (!g_ctx.stream) SQ_HIP(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+    if (!g_ctx.side) SQ_HIP(hipStreamCreateWithFlags(&g_ctx.side, hipStreamNonBlocking));
+    if (!g_ctx.join_ev) SQ_HIP(hipEventCreateWithFlags(&g_ctx.join_ev, hipEventDisableTiming));
     upload_tables();
     gemm16_scratch_setup(device);
 }
@@ -54,6 +56,16 @@ void ensure_init() {
 }
 
 Context& ctx() { return g_ctx; }
+
+SideScope::SideScope() {
+    SQ_REQUIRE(!g_ctx.redirect, SQBEM_ERR_STATE, "side-stream scopes do not nest");
+    g_ctx.redirect = g_ctx.side;
+}
+SideScope::~SideScope() {
+    (void)hipEventRecord(g_ctx.join_ev, g_ctx.side);
+    g_ctx.redirect = nullptr;
+    (void)hipStreamWaitEvent(g_ctx.s(), g_ctx.join_ev, 0);
+}
 
 static int* g_dev_err = nullptr;
 static bool g_dev_err_check = false;

@@ -350,20 +350,34 @@ void sq::tube_build(sqbem_tube* t) {
     sync();
     double t1 = now_s();
     t->ipiv.alloc(nt);
-    int info;
+    // the factorisation is issued here and finished by tube_ready (its first use)
     if (mixed(t->prec)) {
         t->T32.alloc((size_t)nt * nt);
-        const std::function<void()> convert = [&] {
-            convert_matrix<double, float>(nt, nt, t->T64.p, nt, t->T32.p, nt);
-        };
-        convert();
+        convert_matrix<double, float>(nt, nt, t->T64.p, nt, t->T32.p, nt);
         t->aux32.alloc(getrf_aux_elems(nt));
-        info = getrf<float>(t->prec, nt, t->T32.p, nt, t->ipiv.p, t->aux32.p, &convert);
+        t->job32.reset(new GetrfJob<float>);
+        GetrfJob<float>& j = *t->job32;
+        j.kind = t->prec;
+        j.n = j.lda = nt;
+        j.A = t->T32.p;
+        j.ipiv = t->ipiv.p;
+        j.aux = t->aux32.p;
+        // the pivoting fallback rebuilds the FP32 matrix from the FP64 one (no copy is kept)
+        j.restore = [t, nt] { convert_matrix<double, float>(nt, nt, t->T64.p, nt, t->T32.p, nt); };
+        getrf_begin<float>(j);
     } else {
         t->aux64.alloc(getrf_aux_elems(nt));
-        info = getrf<double>(SQBEM_FP64, nt, t->T64.p, nt, t->ipiv.p, t->aux64.p);
+        t->job64.reset(new GetrfJob<double>);
+        GetrfJob<double>& j = *t->job64;
+        j.kind = SQBEM_FP64;
+        j.n = j.lda = nt;
+        j.A = t->T64.p;
+        j.ipiv = t->ipiv.p;
+        j.aux = t->aux64.p;
+        getrf_begin<double>(j);
     }
-    SQ_REQUIRE(info == 0, SQBEM_ERR_SINGULAR, "dgetrf failed");
+    t->t_factor0 = t1;
+    t->view.owner = t;
     t->view.n = nt;
     t->view.kind = t->prec;
     t->view.LU = mixed(t->prec) ? nullptr : t->T64.p;
@@ -373,7 +387,24 @@ void sq::tube_build(sqbem_tube* t) {
     t->view.aux32 = mixed(t->prec) ? t->aux32.p : nullptr;
     t->view.A64 = mixed(t->prec) ? t->T64.p : nullptr;
     t->t_assembly = t1 - t0;
-    t->t_factor = now_s() - t1;
+    t->t_factor = 0;
+}
+
+void sq::tube_ready(const sqbem_tube* tc) {
+    if (!tc || (!tc->job32 && !tc->job64)) return;
+    sqbem_tube* t = const_cast<sqbem_tube*>(tc);
+    int info = 0;
+    try {
+        info = t->job32 ? getrf_end<float>(*t->job32) : getrf_end<double>(*t->job64);
+    } catch (...) {
+        t->job32.reset();
+        t->job64.reset();
+        throw;
+    }
+    t->job32.reset();
+    t->job64.reset();
+    t->t_factor = now_s() - t->t_factor0;
+    SQ_REQUIRE(info == 0, SQBEM_ERR_SINGULAR, "dgetrf failed");
 }
 
 // ------------------------------------------------------------------ sphere template
@@ -446,12 +477,18 @@ static void swimmer_build(sqbem_swimmer* s) {
     const int Ns = s->Ns, Nt = s->Nt, ns = s->ns, nt = s->nt;
     const double mu = s->mu, e8 = bem::EIGHT_PI;
     const bool mx = mixed(s->prec);
+    // The coupling blocks need the meshes only, not the tube's factors: they are assembled
+    // on the side stream while the tube's factorisation (issued when the tube was built) may
+    // still be running on the library stream, into the CUs its narrow steps leave idle.
+    DBuf<int> self;
+    DBuf<double> dlsum;
+    {
+    SideScope side;
     s->dm.upload(s->S);
     s->xc_d.alloc(3);
     s->xc_d.upload(s->xc, 3);
-    DBuf<int> self;
     device_iota(self, Ns);
-    DBuf<double> dlsum((size_t)9 * Ns);
+    dlsum.alloc((size_t)9 * Ns);
     dlsum.zero();
     // tube rows <- sphere sources: B = SL/(8 pi mu), AuBt = -DL/(8 pi)   (src/swimmer.hpp:188-198)
     if (tb) {
@@ -508,9 +545,26 @@ static void swimmer_build(sqbem_swimmer* s) {
         if (tb) kproj(nt, Ns, s->AuBt.p, nt, s->dm.cen_xyz.p, s->xc, s->Y.p + (size_t)nt * ns, nt);
         kproj(ns, Ns, s->AuBs.p, ns, s->dm.cen_xyz.p, s->xc, s->G.p + (size_t)ns * ns, ns);
     }
+    if (mx) {
+        // the low-precision copies the Schur complement starts from (still beside the tube's
+        // factorisation): G32 = [E | Cs], Y32 = [B | Ct] (solved in place), D32 = D
+        s->G32.alloc((size_t)ns * (ns + 6));
+        convert_matrix<double, float>(ns, ns, s->E.p, ns, s->G32.p, ns);
+        convert_matrix<double, float>(ns, 6, s->Cs.p, ns, s->G32.p + (size_t)ns * ns, ns);
+        if (tb) {
+            s->Y32.alloc((size_t)nt * (ns + 6));
+            convert_matrix<double, float>(nt, ns, s->B.p, nt, s->Y32.p, nt);
+            convert_matrix<double, float>(nt, 6, s->Ct.p, nt, s->Y32.p + (size_t)nt * ns, nt);
+            s->D32.alloc((size_t)ns * nt);
+            convert_matrix<double, float>(ns, nt, s->D.p, ns, s->D32.p, ns);
+        }
+    }
     sync();
+    }  // the library stream waits for the side stream from here
     double t1 = now_s();
     s->t_assembly = t1 - t0;
+    tube_ready(tb);  // the factors the Schur complement uses
+    t1 = now_s();
     // Schur complement (src/swimmer.hpp:235-249): Y = T^-1 [B | Ct]; [Gs | Gc] -= D Y
     if (!mx) {
         if (tb) {
@@ -519,16 +573,8 @@ static void swimmer_build(sqbem_swimmer* s) {
         }
     } else {
         const int kind = s->prec;
-        s->G32.alloc((size_t)ns * (ns + 6));
-        convert_matrix<double, float>(ns, ns, s->E.p, ns, s->G32.p, ns);
-        convert_matrix<double, float>(ns, 6, s->Cs.p, ns, s->G32.p + (size_t)ns * ns, ns);
         if (tb) {
-            s->Y32.alloc((size_t)nt * (ns + 6));
-            convert_matrix<double, float>(nt, ns, s->B.p, nt, s->Y32.p, nt);
-            convert_matrix<double, float>(nt, 6, s->Ct.p, nt, s->Y32.p + (size_t)nt * ns, nt);
             getrs<float>(kind, 0, nt, ns + 6, tb->T32.p, nt, tb->ipiv.p, s->Y32.p, nt, tb->aux32.p);
-            s->D32.alloc((size_t)ns * nt);
-            convert_matrix<double, float>(ns, nt, s->D.p, ns, s->D32.p, ns);
             if (kind == SQBEM_FP32_IR)
                 sgemm(ns, ns + 6, nt, -1.0f, s->D32.p, ns, s->Y32.p, nt, 1.0f, s->G32.p, ns);
             else
@@ -974,6 +1020,7 @@ sqbem_tube* sqbem_tube_create(const sqbem_mesh* mesh, double mu, int precision) 
 int sqbem_tube_info(const sqbem_tube* t, int* Nt, double* ta, double* tf) {
     return guarded([&] {
         SQ_REQUIRE(t, SQBEM_ERR_ARG, "null tube");
+        if (tf) tube_ready(t);  // the factorisation time is known once it has finished
         if (Nt) *Nt = t->Nt;
         if (ta) *ta = t->t_assembly;
         if (tf) *tf = t->t_factor;

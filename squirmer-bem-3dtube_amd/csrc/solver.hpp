@@ -2,9 +2,11 @@
 // batched solver (batch.hip): the tube object and the small device helpers of the Schur
 // complement and of the bordered force/torque-free system.
 #pragma once
+#include <memory>
 #include "../../include/utils/mesh.hpp"
 #include "assemble.hpp"
 #include "common.hpp"
+#include "dense.hpp"
 #include "lu_handle.hpp"
 
 // Tube::Tube + Tube::factor (src/swimmer.hpp:40-60): FP64 operator and its factors in HBM.
@@ -21,6 +23,16 @@ struct sqbem_tube {
     sq::DBuf<float> aux32;
     sqbem_lu view;  // non-owning handle onto the factors (Tube::lu)
     double t_assembly = 0, t_factor = 0;
+    // The factorisation is issued by tube_build and finished by tube_ready, when the factors
+    // are first needed: in between the device factors while the host goes on (a swimmer's
+    // coupling blocks are assembled on the side stream meanwhile).
+    mutable std::unique_ptr<sq::GetrfJob<float>> job32;
+    mutable std::unique_ptr<sq::GetrfJob<double>> job64;
+    mutable double t_factor0 = 0;
+    ~sqbem_tube() {
+        // a factorisation still running writes into the buffers released below
+        if (job32 || job64) (void)hipStreamSynchronize(sq::ctx().s());
+    }
 };
 
 namespace sq {
@@ -30,6 +42,10 @@ inline bool mixed(int prec) { return prec == SQBEM_FP32_IR || prec == SQBEM_FP16
 
 // assembles, deflates and factors t->mesh (t->mu, t->prec set by the caller)
 void tube_build(sqbem_tube* t);
+// waits for the factorisation tube_build issued, finishes it (the pivoting fallback if
+// needed) and completes t->view and t->t_factor; throws if the matrix is singular. Cheap
+// once done.
+void tube_ready(const sqbem_tube* t);
 
 // Y = alpha A X + beta Y for at most 8 right-hand sides (FP64 sums)
 template <typename T>

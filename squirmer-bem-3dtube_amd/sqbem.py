@@ -460,11 +460,28 @@ class Tube:
     def __init__(self, R, L, a, n_theta, n_r=4, mu=1.0, precision=FP64, mesh=None):
         self.R, self.L, self.mu, self.precision = R, L, mu, precision
         self.mesh = mesh if mesh is not None else swimmer_tube_mesh(R, L, a, n_theta, n_r)
+        # returns while the device may still be factoring: the factors are waited for where
+        # first used (a swimmer on this tube, lu, t_factor)
         self.h_ = _obj(lib().sqbem_tube_create(self.mesh.h_, mu, precision))
-        Nt, ta, tf = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
-        check(lib().sqbem_tube_info(self.h_, ctypes.byref(Nt), ctypes.byref(ta), ctypes.byref(tf)))
-        self.Nt, self.t_assembly, self.t_factor = Nt.value, ta.value, tf.value
+        Nt = ctypes.c_int()
+        check(lib().sqbem_tube_info(self.h_, ctypes.byref(Nt), None, None))
+        self.Nt = Nt.value
         self.lu = LUFactor(lib().sqbem_tube_lu(self.h_), owner=self)
+
+    def _times(self):
+        ta, tf = ctypes.c_double(), ctypes.c_double()
+        check(lib().sqbem_tube_info(self.h_, None, ctypes.byref(ta), ctypes.byref(tf)))
+        return ta.value, tf.value
+
+    @property
+    def t_assembly(self):
+        """Seconds assembling the tube-tube block."""
+        return self._times()[0]
+
+    @property
+    def t_factor(self):
+        """Seconds from issuing the factorisation to its end (waits for it)."""
+        return self._times()[1]
 
     def __del__(self):
         try:
