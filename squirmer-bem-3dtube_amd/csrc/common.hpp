@@ -38,7 +38,7 @@ struct Context {
     hipStream_t stream = nullptr;      // library-owned stream
     hipStream_t user_stream = nullptr; // caller override (sqbem_set_stream)
     hipStream_t side = nullptr;        // second library stream (default priority)
-    hipEvent_t join_ev = nullptr;
+    hipEvent_t join_ev = nullptr, fork_ev = nullptr;
     hipStream_t redirect = nullptr;    // set inside a SideScope: launches go to side
     int cu_count = 0;
     size_t hbm_bytes = 0;
@@ -47,6 +47,14 @@ struct Context {
 };
 
 Context& ctx();          // initialises device 0 on first use
+
+// Work left on the side stream for later (tiled factor copies): side_defer starts the side
+// stream after what the library stream holds and marks it pending; side_join makes the
+// library stream wait for it if pending; side_drain waits on the host (before memory the
+// side stream may still touch is released).
+void side_defer();
+void side_join();
+void side_drain();
 
 // Work issued inside this scope goes to the side stream, beside what the library stream
 // holds (it must not depend on that work); when the scope ends the library stream waits

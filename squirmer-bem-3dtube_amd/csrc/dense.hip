@@ -3006,6 +3006,7 @@ static void trsv_inv(int n, int nrhs, const T* Lt, const T* Dinv, T* B, int ldb)
     int* err = device_error_word();
     const unsigned long long base = g_trsv_base;
     mark_device_error_check();
+    side_join();  // a tiled factor copy may still be in the making on the side stream
     ProfScope prof(PROF_TRSV, (double)n * n * nrhs, sizeof(T) * 0.5 * n * n);
     if (nrhs == 1) trsv_inv_launch<T, LOWER, 1>(n, nrhs, Lt, Dinv, B, ldb, base, err);
     else if (nrhs == 2) trsv_inv_launch<T, LOWER, 2>(n, nrhs, Lt, Dinv, B, ldb, base, err);
@@ -3034,9 +3035,10 @@ __global__ void __launch_bounds__(256) tile_copy_kernel(int n, const T* __restri
 }
 
 template <typename T>
-static void tile_copy(int n, const T* A, int lda, T* At) {
+static void tile_copy(int n, const T* A, int lda, T* At, hipStream_t stream = nullptr) {
     const int nb = (n + NB - 1) / NB;
-    hipLaunchKernelGGL(tile_copy_kernel<T>, dim3((unsigned)nb * nb), dim3(256), 0, ctx().s(), n, A, lda, At);
+    hipLaunchKernelGGL(tile_copy_kernel<T>, dim3((unsigned)nb * nb), dim3(256), 0, stream ? stream : ctx().s(), n, A,
+                       lda, At);
     check_launch("tile_copy");
 }
 
@@ -3116,7 +3118,12 @@ int getrf_end(GetrfJob<T>& j) {
         if (!bad) {
             hipLaunchKernelGGL(iota_pivots_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx().s(), n, ipiv);
             check_launch("iota_pivots");
-            if (aux) tile_copy<T>(n, A, lda, aux + 2 * nb * NB * NB);
+            if (aux) {
+                // the tiled copy is read only by the chained solves (which wait for it):
+                // made on the side stream, beside what follows on the library stream
+                side_defer();
+                tile_copy<T>(n, A, lda, aux + 2 * nb * NB * NB, ctx().side);
+            }
             j.backup.release();  // stream-ordered reuse only
             g_last_getrf_path = 1;
             return 0;

@@ -40,6 +40,7 @@ static void init_device(int device) {
     if (!g_ctx.stream) SQ_HIP(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
     if (!g_ctx.side) SQ_HIP(hipStreamCreateWithFlags(&g_ctx.side, hipStreamNonBlocking));
     if (!g_ctx.join_ev) SQ_HIP(hipEventCreateWithFlags(&g_ctx.join_ev, hipEventDisableTiming));
+    if (!g_ctx.fork_ev) SQ_HIP(hipEventCreateWithFlags(&g_ctx.fork_ev, hipEventDisableTiming));
     upload_tables();
     gemm16_scratch_setup(device);
 }
@@ -56,6 +57,23 @@ void ensure_init() {
 }
 
 Context& ctx() { return g_ctx; }
+
+static bool g_side_pending = false;
+
+void side_defer() {
+    SQ_HIP(hipEventRecord(g_ctx.fork_ev, g_ctx.s()));
+    SQ_HIP(hipStreamWaitEvent(g_ctx.side, g_ctx.fork_ev, 0));
+    g_side_pending = true;
+}
+void side_join() {
+    if (!g_side_pending) return;
+    g_side_pending = false;
+    SQ_HIP(hipEventRecord(g_ctx.join_ev, g_ctx.side));
+    SQ_HIP(hipStreamWaitEvent(g_ctx.s(), g_ctx.join_ev, 0));
+}
+void side_drain() {
+    if (g_ctx.side) (void)hipStreamSynchronize(g_ctx.side);
+}
 
 SideScope::SideScope() {
     SQ_REQUIRE(!g_ctx.redirect, SQBEM_ERR_STATE, "side-stream scopes do not nest");

@@ -465,6 +465,7 @@ struct sqbem_swimmer {
     double t_assembly = 0, t_schur = 0;
     int ir_iters = 0;
     double ir_residual = 0;
+    ~sqbem_swimmer() { side_drain(); }  // the bordered factors' tiled copy may still be written
 };
 
 static void swimmer_build(sqbem_swimmer* s) {

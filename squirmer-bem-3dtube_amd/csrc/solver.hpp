@@ -32,6 +32,7 @@ struct sqbem_tube {
     ~sqbem_tube() {
         // a factorisation still running writes into the buffers released below
         if (job32 || job64) (void)hipStreamSynchronize(sq::ctx().s());
+        sq::side_drain();  // a tiled copy of the factors may still be written there
     }
 };
 
