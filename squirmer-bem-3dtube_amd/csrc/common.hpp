@@ -76,11 +76,15 @@ void fill_zero(void* p, size_t bytes);
 
 // Device memory through a cache: a released block is kept (by its rounded size) and handed
 // to the next request of that size instead of going back to the driver, so repeated solves
-// of one size allocate nothing after the first. All library work is ordered on one stream,
-// which makes a block's reuse safe without synchronisation. A failed allocation empties
-// the cache and retries once.
+// of one size allocate nothing after the first. Library work is ordered on one stream,
+// which makes a block's reuse safe without synchronisation; work issued on the side stream
+// (SideScope) only receives blocks released before the library stream last drained, since
+// kernels still queued there may use a block the host has released. A failed allocation
+// empties the cache and retries once.
 void* dev_alloc(size_t bytes);  // nullptr when the device is out of memory
 void dev_free(void* p);
+// the library stream has drained: every block released so far may go to side-stream work
+void dev_cache_settled();
 void dev_cache_release();       // return every cached block to the driver
 size_t dev_cache_bytes();
 
@@ -132,6 +136,7 @@ void check_device_error();
 void trsv_reset_counters();
 inline void sync() {
     SQ_HIP(hipStreamSynchronize(ctx().s()));
+    if (!ctx().redirect) dev_cache_settled();
     check_device_error();
 }
 

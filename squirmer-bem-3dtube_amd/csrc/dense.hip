@@ -2802,11 +2802,16 @@ struct TrsvState {
 };
 static DBuf<TrsvState> g_trsv;
 static unsigned long long g_trsv_base = 0;
+// the tagged words of the single-system chain (trsv_tag_body), and the base past which the
+// counters and words are started over, far below where a 32-bit tag would wrap
+static DBuf<unsigned long long> g_trsv_words;
+constexpr unsigned long long kTagLimit = 0xF0000000ull;
 
-// after a chain reported an error its counters may be out of step: start them over
+// after a chain reported an error its counters may be out of step: start them over (and
+// clear the tagged words, whose tags restart with the counters)
 void trsv_reset_counters() {
-    if (!g_trsv.p) return;
-    g_trsv.zero();
+    if (g_trsv.p) g_trsv.zero();
+    g_trsv_words.zero();
     g_trsv_base = 0;
 }
 
@@ -2948,12 +2953,200 @@ __device__ __forceinline__ void trsv_inv_body(int n, int nrhs, const T* __restri
         __hip_atomic_fetch_max(&st->done, base + (unsigned long long)k + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The chained solve of one system (trsv_inv) hands solved values from block to block in
+// tagged words instead of through a completion counter: every value travels as 32-bit words
+// (one for FP32, two for FP64), each in the low half of a 64-bit word whose high half is the
+// tag of the block that wrote it (the launch's base + its position in solve order + 1).
+// Writer and readers use relaxed device-scope 64-bit atomics, so a reader that sees its tag
+// has that launch's value: no fence, counter or cache invalidation sits on the chain. Tags
+// grow from launch to launch (the host restarts them, and clears the words, long before
+// they wrap), so a word left by an earlier launch never carries a tag a reader waits for.
+// Workgroup k forms y_k = Dinv_k (b_k - sum_{q < k-1} T_kq x_q) before it waits for block
+// k-1 (x_q of earlier blocks arrive while it streams its block row), then only
+// x_k = y_k - M_k x_{k-1} remains, with M_k = Dinv_k T_{k,k-1} formed once per
+// factorisation (chain_premul_kernel). Waits are bounded; a timeout, or a ticket below the
+// launch's base, is reported through the device error word.
+template <typename T, bool LOWER, int NR>
+__device__ __forceinline__ void trsv_tag_body(int n, int nrhs, const T* __restrict__ Lt, const T* __restrict__ Dinv,
+                                              const T* __restrict__ Mk, T* B, int ldb, unsigned long long* XT,
+                                              TrsvState* st, unsigned long long base, int* err,
+                                              unsigned char* tv_smem) {
+    constexpr int GQ = sizeof(T) == 4 ? 2 : 1;
+    constexpr int W = sizeof(T) / 4;  // 32-bit words a value
+    constexpr int DLD = NB + 1;
+    T* Dv = reinterpret_cast<T*>(tv_smem);                                              // [NB][DLD]
+    T(*Xs)[NR][NB] = reinterpret_cast<T(*)[NR][NB]>(tv_smem + sizeof(T) * NB * DLD);  // [GQ][NR][NB]
+    // the partial sums of the two column halves, and the right-hand side of the block solve;
+    // part shares its bytes with Xs
+    double(*part)[NR][NB] = reinterpret_cast<double(*)[NR][NB]>(tv_smem + sizeof(T) * NB * DLD);
+    double(*rv)[NB] = reinterpret_cast<double(*)[NB]>(tv_smem + sizeof(T) * NB * DLD + sizeof(double) * 2 * NR * NB);
+    __shared__ int s_k;
+    const int tid = threadIdx.x, row = tid & (NB - 1), hf = tid >> 7;
+    const int nb = (n + NB - 1) / NB;
+    if (tid == 0) {
+        const unsigned long long tk =
+            __hip_atomic_fetch_add(&st->ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_k = tk < base ? -1 : (tk - base >= (unsigned long long)nb ? nb : (int)(tk - base));
+        if (s_k < 0) __hip_atomic_store(err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    const int k = s_k;
+    if (k < 0 || k >= nb) return;
+    const int blk = LOWER ? k : nb - 1 - k;
+    const int r0 = blk * NB, rb = min(NB, n - r0);
+    {
+        const T* D = Dinv + (size_t)blk * NB * NB;
+        for (int e = tid; e < NB * NB; e += 256) Dv[(e % NB) * DLD + e / NB] = D[e];
+    }
+    const bool rowok = row < rb;
+    T bown[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) bown[r] = (hf == 0 && r < nrhs && rowok) ? B[(r0 + row) + (size_t)r * ldb] : T(0);
+    // row `row`, columns of half hf, of M_k (column-major 128 x 128 like the tiles)
+    T mk[64];
+    if (k > 0) {
+        const T* M = Mk + (size_t)blk * NB * NB + (size_t)(hf * 64) * NB + row;
+#pragma unroll
+        for (int c = 0; c < 64; ++c) mk[c] = M[(size_t)c * NB];
+    }
+    int failed = 0;
+    // x of solve-order blocks q0 .. q0+nq-1 into Xs: each thread polls its own words
+    auto fetch = [&](int q0, int nq) {
+        for (int e = tid; e < nq * NR * NB; e += 256) {
+            const int g = e / (NR * NB), r = (e / NB) % NR, i = e % NB;
+            const int q = q0 + g, jb = LOWER ? q : nb - 1 - q;
+            const int rr = jb * NB + i;
+            T xv = T(0);
+            if (r < nrhs && rr < n) {
+                const unsigned tag = (unsigned)(base + (unsigned long long)q + 1);
+                unsigned bits[W];
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    const unsigned long long* word = XT + ((size_t)r * n + rr) * W + w;
+                    unsigned long long v = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    unsigned spins = 0;
+                    while ((unsigned)(v >> 32) != tag && !failed) {
+                        if (++spins > (1u << 22)) {
+                            failed = 1;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                        v = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    bits[w] = (unsigned)v;
+                }
+                if constexpr (W == 1) xv = __uint_as_float(bits[0]);
+                else xv = __hiloint2double((int)bits[1], (int)bits[0]);
+            }
+            Xs[g][r][i] = xv;
+        }
+    };
+    // tiles of this block row, contiguous in the tiled copy; blocks before k-1 are summed
+    // before block k-1 is waited for
+    const T* Lrow = Lt + (size_t)blk * nb * NB * NB + (size_t)(hf * 64) * NB + row;
+    double acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = 0.0;
+    const int kp = k > 0 ? k - 1 : 0;
+    for (int q0 = 0; q0 < kp; q0 += GQ) {
+        const int nq = min(GQ, kp - q0);
+        T l[GQ][64];
+        // unconditional loads (a tile past the group's end re-reads its last one, unused)
+#pragma unroll
+        for (int g = 0; g < GQ; ++g) {
+            const int qq = q0 + min(g, nq - 1);
+            const int jb = LOWER ? qq : nb - 1 - qq;
+            const T* tile = Lrow + (size_t)jb * NB * NB;
+#pragma unroll
+            for (int c = 0; c < 64; ++c) l[g][c] = tile[(size_t)c * NB];
+        }
+        fetch(q0, nq);
+        __syncthreads();
+#pragma unroll
+        for (int g = 0; g < GQ; ++g) {
+            if (g < nq) {
+#pragma unroll
+                for (int c = 0; c < 64; ++c) {
+                    const double lv = (double)l[g][c];
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) acc[r] += lv * (double)Xs[g][r][hf * 64 + c];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // y = Dinv (b - sum), thread (row, hf) summing over the columns of its half
+#pragma unroll
+    for (int r = 0; r < NR; ++r) part[hf][r][row] = acc[r];
+    __syncthreads();
+    if (hf == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+            rv[r][row] = (r < nrhs && rowok) ? (double)bown[r] - (part[0][r][row] + part[1][r][row]) : 0.0;
+    }
+    __syncthreads();
+    double y[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) y[r] = 0.0;
+    for (int c = 0; c < 64; ++c) {
+        const int cc = hf * 64 + c;
+        const double dv = (double)Dv[row * DLD + cc];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) y[r] += dv * rv[r][cc];
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) part[hf][r][row] = y[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NR; ++r) y[r] = part[0][r][row] + part[1][r][row];
+    __syncthreads();  // part is read by every thread before Xs (its bytes) is written
+    if (k > 0) {
+        fetch(k - 1, 1);
+        __syncthreads();
+        double s2[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) s2[r] = 0.0;
+#pragma unroll
+        for (int c = 0; c < 64; ++c) {
+            const double mv = (double)mk[c];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) s2[r] += mv * (double)Xs[0][r][hf * 64 + c];
+        }
+        __syncthreads();  // Xs read by every thread before part (its bytes) is written
+#pragma unroll
+        for (int r = 0; r < NR; ++r) part[hf][r][row] = s2[r];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < NR; ++r) y[r] -= part[0][r][row] + part[1][r][row];
+    }
+    if (failed) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (hf == 0 && rowok) {
+        const unsigned long long tag = (unsigned long long)(unsigned)(base + (unsigned long long)k + 1) << 32;
+        for (int r = 0; r < nrhs; ++r) {
+            const T x = (T)y[r];
+            B[(r0 + row) + (size_t)r * ldb] = x;
+            unsigned bits[W];
+            if constexpr (W == 1) {
+                bits[0] = __float_as_uint(x);
+            } else {
+                bits[0] = (unsigned)__double2loint(x);
+                bits[1] = (unsigned)__double2hiint(x);
+            }
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                __hip_atomic_store(XT + ((size_t)r * n + r0 + row) * W + w, tag | bits[w], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
 template <typename T, bool LOWER, int NR>
 __global__ void __launch_bounds__(256) trsv_inv_chain(int n, int nrhs, const T* __restrict__ Lt,
-                                                      const T* __restrict__ Dinv, T* B, int ldb, TrsvState* st,
+                                                      const T* __restrict__ Dinv, const T* __restrict__ Mk, T* B,
+                                                      int ldb, unsigned long long* XT, TrsvState* st,
                                                       unsigned long long base, int* err) {
     extern __shared__ __attribute__((aligned(16))) unsigned char tv_smem[];
-    trsv_inv_body<T, LOWER, NR>(n, nrhs, Lt, Dinv, B, ldb, st, base, err, tv_smem);
+    trsv_tag_body<T, LOWER, NR>(n, nrhs, Lt, Dinv, Mk, B, ldb, XT, st, base, err, tv_smem);
 }
 
 // Independent chained solves, one right-hand side each, in one launch: row y of the grid
@@ -2975,8 +3168,8 @@ __global__ void __launch_bounds__(256) trsv_inv_batch_kernel(const TrsvJob<T>* _
 }
 
 template <typename T, bool LOWER, int NR>
-static void trsv_inv_launch(int n, int nrhs, const T* Lt, const T* Dinv, T* B, int ldb, unsigned long long base,
-                            int* err) {
+static void trsv_inv_launch(int n, int nrhs, const T* Lt, const T* Dinv, const T* Mk, T* B, int ldb,
+                            unsigned long long base, int* err) {
     constexpr int GQ = sizeof(T) == 4 ? 2 : 1;
     constexpr size_t xs = sizeof(T) * GQ * NR * NB, red = sizeof(double) * 3 * NR * NB;
     constexpr size_t lds = sizeof(T) * NB * (NB + 1) + (xs > red ? xs : red);
@@ -2988,30 +3181,35 @@ static void trsv_inv_launch(int n, int nrhs, const T* Lt, const T* Dinv, T* B, i
         lds_set = true;
     }
     const int nb = (n + NB - 1) / NB;
-    hipLaunchKernelGGL((trsv_inv_chain<T, LOWER, NR>), dim3(nb), dim3(256), lds, ctx().s(), n, nrhs, Lt, Dinv, B, ldb,
-                       g_trsv.p, base, err);
+    hipLaunchKernelGGL((trsv_inv_chain<T, LOWER, NR>), dim3(nb), dim3(256), lds, ctx().s(), n, nrhs, Lt, Dinv, Mk, B,
+                       ldb, g_trsv_words.p, g_trsv.p, base, err);
 }
 
 // one launch of the chained solve through inverted diagonal blocks; Lt is the tiled copy
-// of the factors (tile_copy), Dinv the inverted diagonal blocks of the triangle used
+// of the factors (tile_copy), Dinv the inverted diagonal blocks of the triangle used and Mk
+// the products M_k = Dinv_k T_{k,k-1} (in solve order) of chain_premul
 template <typename T, bool LOWER>
-static void trsv_inv(int n, int nrhs, const T* Lt, const T* Dinv, T* B, int ldb) {
+static void trsv_inv(int n, int nrhs, const T* Lt, const T* Dinv, const T* Mk, T* B, int ldb) {
     SQ_REQUIRE(nrhs >= 1 && nrhs <= 8, SQBEM_ERR_ARG, "trsv: 1..8 right-hand sides");
+    // the launch's tags come from a host counter, which a captured graph would freeze
+    SQ_REQUIRE(!capturing(), SQBEM_ERR_STATE, "trsv: the chained solve cannot be captured in a graph");
     const int nb = (n + NB - 1) / NB;
-    if (!g_trsv.p) {
-        g_trsv.alloc(1);
-        g_trsv.zero();
-        g_trsv_base = 0;
+    const size_t words = (size_t)n * 8 * 2;  // up to 8 right-hand sides of two words
+    if (!g_trsv.p || g_trsv_words.n < words || g_trsv_base + (unsigned long long)nb > kTagLimit) {
+        // fresh words carry tag 0, below every tag a launch waits for
+        if (g_trsv_words.n < words) g_trsv_words.alloc(words);
+        if (!g_trsv.p) g_trsv.alloc(1);
+        trsv_reset_counters();
     }
     int* err = device_error_word();
     const unsigned long long base = g_trsv_base;
     mark_device_error_check();
     side_join();  // a tiled factor copy may still be in the making on the side stream
     ProfScope prof(PROF_TRSV, (double)n * n * nrhs, sizeof(T) * 0.5 * n * n);
-    if (nrhs == 1) trsv_inv_launch<T, LOWER, 1>(n, nrhs, Lt, Dinv, B, ldb, base, err);
-    else if (nrhs == 2) trsv_inv_launch<T, LOWER, 2>(n, nrhs, Lt, Dinv, B, ldb, base, err);
-    else if (nrhs <= 4) trsv_inv_launch<T, LOWER, 4>(n, nrhs, Lt, Dinv, B, ldb, base, err);
-    else trsv_inv_launch<T, LOWER, 8>(n, nrhs, Lt, Dinv, B, ldb, base, err);
+    if (nrhs == 1) trsv_inv_launch<T, LOWER, 1>(n, nrhs, Lt, Dinv, Mk, B, ldb, base, err);
+    else if (nrhs == 2) trsv_inv_launch<T, LOWER, 2>(n, nrhs, Lt, Dinv, Mk, B, ldb, base, err);
+    else if (nrhs <= 4) trsv_inv_launch<T, LOWER, 4>(n, nrhs, Lt, Dinv, Mk, B, ldb, base, err);
+    else trsv_inv_launch<T, LOWER, 8>(n, nrhs, Lt, Dinv, Mk, B, ldb, base, err);
     check_launch("trsv_inv_chain");
     // the device counter advances by nb only once the launch is queued: a throw above leaves
     // host base and device ticket in step
@@ -3040,6 +3238,71 @@ static void tile_copy(int n, const T* A, int lda, T* At, hipStream_t stream = nu
     hipLaunchKernelGGL(tile_copy_kernel<T>, dim3((unsigned)nb * nb), dim3(256), 0, stream ? stream : ctx().s(), n, A,
                        lda, At);
     check_launch("tile_copy");
+}
+
+// The products the single-system chain applies after its one wait, one 128 x 128 x 128
+// product per workgroup (grid nb x 2): M^L_k = Linv_k L_{k,k-1} (k >= 1) and
+// M^U_k = Uinv_k U_{k,k+1} (k <= nb - 2), from the inverted diagonal blocks and the tiled
+// copy, column-major 128 x 128 at ML + k 128^2 and MU + k 128^2. Thread (ty, tx) of a
+// 16 x 16 grid owns rows ty + 16 a and columns tx + 16 b; the k dimension is staged through
+// LDS in KC-deep slices; FP64 sums for FP64 blocks, FP32 for FP32 ones.
+template <typename T>
+__global__ void __launch_bounds__(256) chain_premul_kernel(int nb, const T* __restrict__ Linv,
+                                                           const T* __restrict__ Uinv, const T* __restrict__ Lt,
+                                                           T* __restrict__ ML, T* __restrict__ MU) {
+    constexpr int KC = sizeof(T) == 4 ? 32 : 16;
+    __shared__ T As[NB][KC + 1];
+    __shared__ T Bs[KC][NB + 1];
+    const int k = blockIdx.x;
+    const bool up = blockIdx.y != 0;
+    if (up ? k >= nb - 1 : k == 0) return;
+    const T* A = (up ? Uinv : Linv) + (size_t)k * NB * NB;
+    const T* Bt = Lt + ((size_t)k * nb + (up ? k + 1 : k - 1)) * NB * NB;
+    T* C = (up ? MU : ML) + (size_t)k * NB * NB;
+    const int t = threadIdx.x, ty = t >> 4, tx = t & 15;
+    T acc[8][8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) acc[a][b] = T(0);
+    for (int k0 = 0; k0 < NB; k0 += KC) {
+        for (int e = t; e < NB * KC; e += 256) {
+            As[e % NB][e / NB] = A[(size_t)(k0 + e / NB) * NB + e % NB];  // A(i, k0 + kk)
+            Bs[e % KC][e / KC] = Bt[(size_t)(e / KC) * NB + k0 + e % KC];  // B(k0 + kk, j)
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < KC; ++kk) {
+            T av[8], bv[8];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) av[a] = As[ty + 16 * a][kk];
+#pragma unroll
+            for (int b = 0; b < 8; ++b) bv[b] = Bs[kk][tx + 16 * b];
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+#pragma unroll
+                for (int b = 0; b < 8; ++b) acc[a][b] = fma(av[a], bv[b], acc[a][b]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) C[(size_t)(tx + 16 * b) * NB + ty + 16 * a] = acc[a][b];
+}
+
+// what the single-system chained solves read besides the inverted diagonal blocks (aux =
+// getrf's solve aid): the tiled copy of the factors, then the chain products
+template <typename T>
+static void chain_prepare(int n, const T* A, int lda, T* aux, hipStream_t stream = nullptr) {
+    const int nb = (n + NB - 1) / NB;
+    const size_t blk = (size_t)NB * NB;
+    T* Lt = aux + 2 * nb * blk;
+    tile_copy<T>(n, A, lda, Lt, stream);
+    T* ML = Lt + (size_t)nb * nb * blk;
+    hipLaunchKernelGGL(chain_premul_kernel<T>, dim3((unsigned)nb, 2), dim3(256), 0, stream ? stream : ctx().s(), nb,
+                       aux, aux + nb * blk, Lt, ML, ML + nb * blk);
+    check_launch("chain_premul");
 }
 
 // ------------------------------------------------------------------ recursive LU
@@ -3076,7 +3339,8 @@ constexpr int NP_MIN = 512;
 static thread_local int g_last_getrf_path = 0;  // 0 pivoting, 1 no-pivot verified, 2 no-pivot rejected
 
 // aux (optional, getrf_aux_elems(n) elements): receives the solve aid for getrs: the
-// inverted 128 x 128 diagonal blocks of L then of U, then the tiled copy of the factors.
+// inverted 128 x 128 diagonal blocks of L then of U, the tiled copy of the factors, then the
+// chained solves' products M^L_k and M^U_k (chain_premul_kernel).
 template <typename T>
 void getrf_begin(GetrfJob<T>& j) {
     j.started = true;
@@ -3122,7 +3386,7 @@ int getrf_end(GetrfJob<T>& j) {
                 // the tiled copy is read only by the chained solves (which wait for it):
                 // made on the side stream, beside what follows on the library stream
                 side_defer();
-                tile_copy<T>(n, A, lda, aux + 2 * nb * NB * NB, ctx().side);
+                chain_prepare<T>(n, A, lda, aux, ctx().side);
             }
             j.backup.release();  // stream-ordered reuse only
             g_last_getrf_path = 1;
@@ -3148,7 +3412,7 @@ int getrf_end(GetrfJob<T>& j) {
     SQ_REQUIRE(st[1] == 0, SQBEM_ERR_HIP, "getrf: panel barrier timed out");
     if (aux && st[0] == 0) {
         lu128_launch<T>((int)nb, A, lda, n, aux, aux + nb * NB * NB, nullptr, 0);
-        tile_copy<T>(n, A, lda, aux + 2 * nb * NB * NB);
+        chain_prepare<T>(n, A, lda, aux);
     }
     return st[0];
 }
@@ -3198,18 +3462,20 @@ void getrs(int kind, int trans, int n, int nrhs, const T* LU, int lda, const int
         const int nb = (n + NB - 1) / NB;
         const size_t ninv = 2 * (size_t)nb * NB * NB;
         if (!aux) {
-            const size_t need = (ninv + (nrhs <= 8 ? (size_t)nb * nb * NB * NB : 0)) * sizeof(T);
+            const size_t need = (nrhs <= 8 ? getrf_aux_elems(n) : ninv) * sizeof(T);
             if (g_getrs_inv.n < need) g_getrs_inv.alloc(need);
             T* w = (T*)g_getrs_inv.p;
             lu128_launch<T>(nb, const_cast<T*>(LU), lda, n, w, w + (size_t)nb * NB * NB, nullptr, 0);
-            if (nrhs <= 8) tile_copy<T>(n, LU, lda, w + ninv);
+            if (nrhs <= 8) chain_prepare<T>(n, LU, lda, w);
             aux = w;
         }
         const T* Linv = aux;
         const T* Uinv = aux + (size_t)nb * NB * NB;
         if (nrhs <= 8) {
-            trsv_inv<T, true>(n, nrhs, aux + ninv, Linv, B, ldb);
-            trsv_inv<T, false>(n, nrhs, aux + ninv, Uinv, B, ldb);
+            const T* Lt = aux + ninv;
+            const T* ML = Lt + (size_t)nb * nb * NB * NB;
+            trsv_inv<T, true>(n, nrhs, Lt, Linv, ML, B, ldb);
+            trsv_inv<T, false>(n, nrhs, Lt, Uinv, ML + (size_t)nb * NB * NB, B, ldb);
             return;
         }
         // recursive solves whose leaves are single in-place GEMMs with the inverses

@@ -26,7 +26,8 @@ void convert_matrix(int m, int n, const Ti* A, int lda, To* B, int ldb);
 // LU with partial pivoting; ipiv (device, n) 1-based on return; returns LAPACK info.
 // kind selects the GEMM of the FP32 variant (SQBEM_FP32_IR / FP16_IR / BF16_IR).
 // aux (optional, getrf_aux_elems(n) elements): receives a solve aid for getrs (inverted
-// 128 x 128 diagonal blocks of L and U, and a tiled copy of the factors).
+// 128 x 128 diagonal blocks of L and U, a tiled copy of the factors, and the products the
+// chained solves apply after their one wait).
 // restore (optional): rebuilds A as it was on entry (e.g. from the matrix it was converted
 // from); given, the first attempt without row interchanges keeps no copy of A and calls it
 // instead when it has to fall back to partial pivoting.
@@ -55,7 +56,7 @@ template <typename T>
 int getrf_end(GetrfJob<T>& job);
 inline size_t getrf_aux_elems(int n) {
     const size_t nb = (size_t)((n + 127) / 128);
-    return (2 * nb + nb * nb) * 128 * 128;
+    return (4 * nb + nb * nb) * 128 * 128;
 }
 // B <- A^-1 B (trans = 0) or A^-T B (trans = 1)
 // aux (optional): getrf's solve aid for these factors; formed here when absent

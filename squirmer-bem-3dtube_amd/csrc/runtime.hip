@@ -114,10 +114,16 @@ void check_device_error() {
 
 // ------------------------------------------------------------------ device memory cache
 namespace {
+struct FreeBlock {
+    void* p;
+    unsigned long long seq;  // when it was released (count of releases so far)
+};
 struct MemCache {
-    std::multimap<size_t, void*> free_blocks;  // rounded size -> block
-    std::unordered_map<void*, size_t> live;    // block -> rounded size
+    std::multimap<size_t, FreeBlock> free_blocks;  // rounded size -> block
+    std::unordered_map<void*, size_t> live;        // block -> rounded size
     size_t cached = 0;
+    unsigned long long released = 0;  // releases so far
+    unsigned long long settled = 0;   // releases made before the library stream last drained
 };
 // never destroyed: buffers with static storage release into it during process exit
 MemCache& mem() {
@@ -136,9 +142,14 @@ size_t round_bytes(size_t b) {
 void* dev_alloc(size_t bytes) {
     MemCache& M = mem();
     const size_t r = round_bytes(bytes);
-    auto it = M.free_blocks.find(r);
-    if (it != M.free_blocks.end()) {
-        void* p = it->second;
+    // Work on the side stream runs beside the library stream, whose queued kernels may still
+    // use a block the host has already released: such a block is handed to side work only
+    // once the library stream has drained after its release.
+    const bool side = ctx().redirect != nullptr;
+    auto range = M.free_blocks.equal_range(r);
+    for (auto it = range.first; it != range.second; ++it) {
+        if (side && it->second.seq > M.settled) continue;
+        void* p = it->second.p;
         M.free_blocks.erase(it);
         M.cached -= r;
         M.live[p] = r;
@@ -166,16 +177,21 @@ void dev_free(void* p) {
         (void)hipFree(p);
         return;
     }
-    M.free_blocks.emplace(it->second, p);
+    M.free_blocks.emplace(it->second, FreeBlock{p, ++M.released});
     M.cached += it->second;
     M.live.erase(it);
+}
+
+void dev_cache_settled() {
+    MemCache& M = mem();
+    M.settled = M.released;
 }
 
 void dev_cache_release() {
     MemCache& M = mem();
     if (M.free_blocks.empty()) return;
     (void)hipDeviceSynchronize();
-    for (auto& kv : M.free_blocks) (void)hipFree(kv.second);
+    for (auto& kv : M.free_blocks) (void)hipFree(kv.second.p);
     M.free_blocks.clear();
     M.cached = 0;
 }
