@@ -2226,7 +2226,10 @@ __global__ void __launch_bounds__(256) trsm2_kernel(int n, int nrhs, const T* L,
 
 // FP32 form of trsm2_kernel with the workgroup's TN-column panel of B (n <= 256 rows) held
 // in LDS through the three products: the panel is read once and written once, only the
-// operands inv1, L21 (U12) and inv2 stream through two LDS stages. The products are the
+// operands inv1, L21 (U12) and inv2 pass through two LDS stages. The operands do not depend
+// on the panel, so each is loaded whole into registers before the product that needs it
+// (the first two at the start, the third while the first product runs): a workgroup waits
+// for global memory about once instead of once per 32-deep k tile. The products are the
 // same MFMA sums in the same order as gemm_nn_tile's (k tiles of 32, steps of 4), so the
 // results equal trsm2_kernel's.
 template <bool LOWER, int TN>
@@ -2235,6 +2238,7 @@ __global__ void __launch_bounds__(256) trsm2p_kernel(int n, int nrhs, const floa
     using Op = MfmaOp<float>;
     using Acc = typename Op::acc;
     constexpr int KB = GemmK<float>::KB, PAD = GemmK<float>::PAD, NQ = KB / 2, S = KB / 4;
+    constexpr int NKT = TRSM2_B / KB;  // k tiles of a product (k <= 128)
     constexpr int JT = TN / 32, NP = 2 * TRSM2_B + 34;  // panel column stride (34 mod 64 words)
     __shared__ __attribute__((aligned(16))) float Pn[TN * NP];        // Pn[c * NP + r] = B(r, c)
     __shared__ __attribute__((aligned(16))) float As[2][GBM][PAD];
@@ -2242,6 +2246,32 @@ __global__ void __launch_bounds__(256) trsm2p_kernel(int n, int nrhs, const floa
     const int wm = (wave & 1) * 64, wn = (wave >> 1) * (TN / 2);
     const int lr = lane & 15, lk = lane >> 4;
     const int n0 = blockIdx.x * TN, n1 = TRSM2_B, n2 = n - TRSM2_B;
+    const float* inv2 = inv + (size_t)TRSM2_B * TRSM2_B;
+    // the three operands in the order used: A (mr x kd, column-major at lda)
+    const float* opA[3];
+    int opld[3], opm[3], opk[3];
+    if (LOWER) {
+        opA[0] = inv;    opld[0] = TRSM2_B; opm[0] = n1; opk[0] = n1;  // X1 = inv1 B1
+        opA[1] = L + n1; opld[1] = ldl;     opm[1] = n2; opk[1] = n1;  // B2 -= L21 X1
+        opA[2] = inv2;   opld[2] = TRSM2_B; opm[2] = n2; opk[2] = n2;  // X2 = inv2 B2
+    } else {
+        opA[0] = inv2;                    opld[0] = TRSM2_B; opm[0] = n2; opk[0] = n2;  // X2 = inv2 B2
+        opA[1] = L + (size_t)n1 * ldl;    opld[1] = ldl;     opm[1] = n1; opk[1] = n2;  // B1 -= U12 X2
+        opA[2] = inv;                     opld[2] = TRSM2_B; opm[2] = n1; opk[2] = n1;  // X1 = inv1 B1
+    }
+    // thread t holds row t % 128 and k values t / 128 + 2 q of every k tile of an operand
+    const int a_i = tid & 127, a_k0 = tid >> 7;
+    float ra[NKT][NQ], rb[NKT][NQ];
+    auto fetch = [&](float (&r)[NKT][NQ], int o) {
+        const float* A = opA[o];
+        const int lda = opld[o], ai = min(a_i, opm[o] - 1), kd = opk[o];
+#pragma unroll
+        for (int kt = 0; kt < NKT; ++kt)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) r[kt][q] = A[ai + (size_t)min(kt * KB + a_k0 + 2 * q, kd - 1) * lda];
+    };
+    fetch(ra, 0);
+    fetch(rb, 1);
     // the panel: thread t moves row t of every column (zero beyond n and nrhs)
     {
         float v[TN];
@@ -2250,46 +2280,41 @@ __global__ void __launch_bounds__(256) trsm2p_kernel(int n, int nrhs, const floa
 #pragma unroll
         for (int c = 0; c < TN; ++c) Pn[c * NP + tid] = (tid < n && n0 + c < nrhs) ? v[c] : 0.0f;
     }
-    __syncthreads();
-    // acc = A(0:mr, 0:kd) P(ko:ko+kd, :), A column-major at lda; A streamed in k tiles of KB
-    const int a_i = tid & 127, a_k0 = tid >> 7;
+    // acc = A(0:mr, 0:kd) P(ko:ko+kd, :) with A held in r
     Acc acc[4][JT];
-    auto product = [&](const float* A, int lda, int mr, int kd, int ko) {
+    auto product = [&](const float (&r)[NKT][NQ], int o, int ko) {
+        const int mr = opm[o], kd = opk[o];
+        const int nk = (kd + KB - 1) / KB;
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < JT; ++j) acc[i][j] = Acc{0, 0, 0, 0};
-        const int ai = min(a_i, mr - 1);
-        float ra[NQ];
-        auto load = [&](int k0) {
+        auto store = [&](int buf, int kt) {
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) ra[q] = A[ai + (size_t)min(k0 + a_k0 + 2 * q, kd - 1) * lda];
+            for (int q = 0; q < NQ; ++q)
+                As[buf][a_i][a_k0 + 2 * q] = (a_i < mr && kt * KB + a_k0 + 2 * q < kd) ? r[kt][q] : 0.0f;
         };
-        auto store = [&](int buf, int k0) {
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) As[buf][a_i][a_k0 + 2 * q] = (a_i < mr && k0 + a_k0 + 2 * q < kd) ? ra[q] : 0.0f;
-        };
-        const int nk = (kd + KB - 1) / KB;
-        load(0);
         store(0, 0);
-        __syncthreads();
-        for (int kt = 0; kt < nk; ++kt) {
-            const int buf = kt & 1;
-            if (kt + 1 < nk) load((kt + 1) * KB);
+        __syncthreads();  // also orders the panel's earlier writes before its reads below
 #pragma unroll
-            for (int s = 0; s < S; ++s) {
-                float af[4], bf[JT];
+        for (int kt = 0; kt < NKT; ++kt) {
+            if (kt < nk) {
+                const int buf = kt & 1;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) af[i] = As[buf][wm + i * 16 + lr][4 * s + lk];
+                for (int s = 0; s < S; ++s) {
+                    float af[4], bf[JT];
 #pragma unroll
-                for (int j = 0; j < JT; ++j) bf[j] = Pn[(wn + j * 16 + lr) * NP + ko + kt * KB + 4 * s + lk];
+                    for (int i = 0; i < 4; ++i) af[i] = As[buf][wm + i * 16 + lr][4 * s + lk];
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
+                    for (int j = 0; j < JT; ++j) bf[j] = Pn[(wn + j * 16 + lr) * NP + ko + kt * KB + 4 * s + lk];
 #pragma unroll
-                    for (int j = 0; j < JT; ++j) acc[i][j] = Op::mma(af[i], bf[j], acc[i][j]);
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int j = 0; j < JT; ++j) acc[i][j] = Op::mma(af[i], bf[j], acc[i][j]);
+                }
+                if (kt + 1 < nk) store(buf ^ 1, kt + 1);
+                __syncthreads();
             }
-            if (kt + 1 < nk) store(buf ^ 1, (kt + 1) * KB);
-            __syncthreads();
         }
     };
     // rows ro .. ro + mr - 1 of the panel: = acc (set) or -= acc (sub, as gemm's -acc + c)
@@ -2307,26 +2332,18 @@ __global__ void __launch_bounds__(256) trsm2p_kernel(int n, int nrhs, const floa
                     }
                 }
     };
-    const float* inv2 = inv + (size_t)TRSM2_B * TRSM2_B;
-    if (LOWER) {
-        product(inv, TRSM2_B, n1, n1, 0);  // X1 = inv1 B1 (its reads of rows 0..127 ended at the last barrier)
-        put(0, n1, false);
-        __syncthreads();
-        product(L + n1, ldl, n2, n1, 0);  // B2 -= L21 X1
-        put(n1, n2, true);
-        __syncthreads();
-        product(inv2, TRSM2_B, n2, n2, n1);  // X2 = inv2 B2
-        put(n1, n2, false);
-    } else {
-        product(inv2, TRSM2_B, n2, n2, n1);  // X2 = inv2 B2
-        put(n1, n2, false);
-        __syncthreads();
-        product(L + (size_t)n1 * ldl, ldl, n1, n2, n1);  // B1 -= U12 X2
-        put(0, n1, true);
-        __syncthreads();
-        product(inv, TRSM2_B, n1, n1, 0);  // X1 = inv1 B1
-        put(0, n1, false);
-    }
+    // panel rows each product reads and writes: lower 0 | 0 -> n1 | n1; upper n1 | n1 -> 0 | 0
+    const int ko0 = LOWER ? 0 : n1, ko1 = LOWER ? 0 : n1, ko2 = LOWER ? n1 : 0;
+    const int ro0 = LOWER ? 0 : n1, ro1 = LOWER ? n1 : 0, ro2 = LOWER ? n1 : 0;
+    product(ra, 0, ko0);  // its reads of the panel ended at its last barrier
+    fetch(ra, 2);         // the third operand, in flight during the second product
+    put(ro0, opm[0], false);
+    __syncthreads();
+    product(rb, 1, ko1);
+    put(ro1, opm[1], true);
+    __syncthreads();
+    product(ra, 2, ko2);
+    put(ro2, opm[2], false);
     __syncthreads();
     if (tid < n) {
 #pragma unroll
@@ -2574,6 +2591,110 @@ __device__ __forceinline__ void lu128_inv_column(C* M, C* Tmp, int t) {
     __syncthreads();
 }
 
+// FP32 blocks: the trailing updates and the inverses' block columns as MFMA products
+// (mfma_f32_16x16x4f32: lane l supplies A(l%16, l/16) and B(l/16, l%16) of a 16 x 4 x 16
+// step and holds D(4 (l/16) + r, l%16), r = 0..3), one 16 x 16 output tile per wave at a
+// time, operands read from the block in LDS. The sums are the same as the VALU forms',
+// four terms at a time.
+__device__ __forceinline__ f4v lu_mma(float a, float b, f4v c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// A22 -= L21 U12 after panel P (rows and columns 32P+32 .., inner dimension 32): wave w
+// takes PW consecutive 16 x 16 tiles of A22 and runs them as independent accumulators, the
+// operand reads of all its tiles issued before the products.
+template <int P>
+__device__ __forceinline__ void lu128_update_mfma(float* M, int t) {
+    constexpr int p0 = 32 * P, r0 = p0 + 32, NT = (96 - 32 * P) / 16, PW = (NT * NT + 3) / 4;
+    const int lane = t & 63, wv = t >> 6, lr = lane & 15, lk = lane >> 4;
+    f4v acc[PW];
+    float a[PW][8], b[PW][8];
+#pragma unroll
+    for (int u = 0; u < PW; ++u) {
+        const int tile = min(wv * PW + u, NT * NT - 1);  // a surplus tile repeats the last, unstored
+        const int i0 = r0 + 16 * (tile / NT), j0 = r0 + 16 * (tile % NT);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[u][r] = M[(i0 + 4 * lk + r) * LU_LD + j0 + lr];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            a[u][q] = -M[(i0 + lr) * LU_LD + p0 + 4 * q + lk];
+            b[u][q] = M[(p0 + 4 * q + lk) * LU_LD + j0 + lr];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+#pragma unroll
+        for (int u = 0; u < PW; ++u) acc[u] = lu_mma(a[u][q], b[u][q], acc[u]);
+#pragma unroll
+    for (int u = 0; u < PW; ++u) {
+        const int tile = wv * PW + u;
+        if (tile < NT * NT) {
+            const int i0 = r0 + 16 * (tile / NT), j0 = r0 + 16 * (tile % NT);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) M[(i0 + 4 * lk + r) * LU_LD + j0 + lr] = acc[u][r];
+        }
+    }
+}
+
+// Block column J of the inverse of L (unit lower) or of U, as lu128_inv_column, by the two
+// waves of its chain: wave cw owns the 16-column half cw of the block column and runs its
+// R / 16 row tiles as independent accumulators; Tmp is the chain's R x 32 product (row
+// stride 33).
+//   L: Tmp = X(r0.., r0..) L(r0.., cJ);  X(r0.., cJ) = -Tmp X(J, J)   (r0 = cJ + 32, R = 96 - cJ)
+//   U: Tmp = X(0..R, 0..R) U(0..R, cJ);  X(0..R, cJ) = -Tmp X(J, J)   (R = cJ)
+// k steps where a tile's triangular operand is zero are skipped at compile time; inside
+// the remaining ones the triangle (and the unit diagonal of L's inverse) is applied by
+// selects.
+template <bool LOWER, int J>
+__device__ __forceinline__ void lu128_inv_column_mfma(float* M, float* Tmp, int cw, int lane) {
+    constexpr int cj = 32 * J, R = LOWER ? 96 - cj : cj, r0 = LOWER ? cj + 32 : 0, NTI = R / 16, KS = R / 4;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int c = 16 * cw + lr;  // this lane's column of the block column (B operand, output)
+    f4v acc[NTI];
+#pragma unroll
+    for (int ti = 0; ti < NTI; ++ti) acc[ti] = f4v{0.f, 0.f, 0.f, 0.f};
+    // Tmp = X(r0.., r0..) B, X(i, q) nonzero for q <= i (L) or q >= i (U): row tile ti needs
+    // k steps s < 4 ti + 4 (L) or s >= 4 ti (U)
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        const int q = 4 * s + lk;
+        const float bv = M[(r0 + q) * LU_LD + cj + c];
+#pragma unroll
+        for (int ti = 0; ti < NTI; ++ti) {
+            if (LOWER ? s < 4 * ti + 4 : s >= 4 * ti) {
+                const int i = 16 * ti + lr;
+                const float m = M[(r0 + i) * LU_LD + r0 + q];
+                const float av = LOWER ? (q < i ? m : (q == i ? 1.f : 0.f)) : (q >= i ? m : 0.f);
+                acc[ti] = lu_mma(av, bv, acc[ti]);
+            }
+        }
+    }
+#pragma unroll
+    for (int ti = 0; ti < NTI; ++ti)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Tmp[(16 * ti + 4 * lk + r) * 33 + c] = acc[ti][r];
+    __syncthreads();
+    // X(r0.., cJ) = -Tmp X(J, J), X(J, J)(q, c) nonzero for q >= c (L) or q <= c (U): this
+    // wave's column half needs k steps s >= 4 cw (L) or s < 4 cw + 4 (U)
+#pragma unroll
+    for (int ti = 0; ti < NTI; ++ti) acc[ti] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        if (LOWER ? s >= 4 * cw : s < 4 * cw + 4) {
+            const int q = 4 * s + lk;
+            const float m = M[(cj + q) * LU_LD + cj + c];
+            const float bv = LOWER ? (q > c ? m : (q == c ? 1.f : 0.f)) : (q <= c ? m : 0.f);
+#pragma unroll
+            for (int ti = 0; ti < NTI; ++ti) acc[ti] = lu_mma(Tmp[(16 * ti + lr) * 33 + q], bv, acc[ti]);
+        }
+    }
+#pragma unroll
+    for (int ti = 0; ti < NTI; ++ti)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) M[(r0 + 16 * ti + 4 * lk + r) * LU_LD + cj + c] = -acc[ti][r];
+    __syncthreads();
+}
+
 // factor = 1: factor the block at A (grid of one), check it, write L\U back, and the
 // inverses. factor = 0: the blocks are already factored; workgroup b inverts diagonal
 // block b (A advanced by b NB (lda + 1), inverses by b NB^2).
@@ -2612,19 +2733,22 @@ __device__ __forceinline__ void lu128_block(T* A, int lda, int n, T* Linv, T* Ui
         __syncthreads();
         lu128_u12<0, T>(M, t);
         __syncthreads();
-        lu128_update<0, T>(M, t);
+        if constexpr (sizeof(T) == 4) lu128_update_mfma<0>(M, t);
+        else lu128_update<0, T>(M, t);
         __syncthreads();
         lu128_panel<1, T>(M, t, mybad);
         __syncthreads();
         lu128_u12<1, T>(M, t);
         __syncthreads();
-        lu128_update<1, T>(M, t);
+        if constexpr (sizeof(T) == 4) lu128_update_mfma<1>(M, t);
+        else lu128_update<1, T>(M, t);
         __syncthreads();
         lu128_panel<2, T>(M, t, mybad);
         __syncthreads();
         lu128_u12<2, T>(M, t);
         __syncthreads();
-        lu128_update<2, T>(M, t);
+        if constexpr (sizeof(T) == 4) lu128_update_mfma<2>(M, t);
+        else lu128_update<2, T>(M, t);
         __syncthreads();
         lu128_panel<3, T>(M, t, mybad);
         if (mybad) bad = 1;
@@ -2702,14 +2826,15 @@ __device__ __forceinline__ void lu128_block(T* A, int lda, int n, T* Linv, T* Ui
     if constexpr (sizeof(T) == 4) {
         // the L and U chains write disjoint parts of M (strictly lower, upper) and read only
         // their own: waves 0-1 run L's block columns, waves 2-3 U's, two barriers a step each
+        const int cw = (t >> 6) & 1, lane = t & 63;
         if (t < 128) {
-            lu128_inv_column<true, 2, T, 128>(M, Tmp, t);
-            lu128_inv_column<true, 1, T, 128>(M, Tmp, t);
-            lu128_inv_column<true, 0, T, 128>(M, Tmp, t);
+            lu128_inv_column_mfma<true, 2>(M, Tmp, cw, lane);
+            lu128_inv_column_mfma<true, 1>(M, Tmp, cw, lane);
+            lu128_inv_column_mfma<true, 0>(M, Tmp, cw, lane);
         } else {
-            lu128_inv_column<false, 1, T, 128>(M, Tmp + 96 * 33, t - 128);
-            lu128_inv_column<false, 2, T, 128>(M, Tmp + 96 * 33, t - 128);
-            lu128_inv_column<false, 3, T, 128>(M, Tmp + 96 * 33, t - 128);
+            lu128_inv_column_mfma<false, 1>(M, Tmp + 96 * 33, cw, lane);
+            lu128_inv_column_mfma<false, 2>(M, Tmp + 96 * 33, cw, lane);
+            lu128_inv_column_mfma<false, 3>(M, Tmp + 96 * 33, cw, lane);
         }
     } else {
         lu128_inv_column<true, 2, T>(M, Tmp, t);
