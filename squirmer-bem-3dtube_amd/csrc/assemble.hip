@@ -198,11 +198,13 @@ __device__ __forceinline__ void consume(const AsmArgs& a, const Acc& q, int i, i
 #pragma unroll
             for (int b = 0; b < 3; ++b) {
                 double* col = a.SL + (size_t)(3 * pcol + b) * a.ld_sl + 3 * (size_t)i;
+                float* col32 = a.SL32 ? a.SL32 + (size_t)(3 * pcol + b) * a.ld_sl32 + 3 * (size_t)i : nullptr;
 #pragma unroll
                 for (int r = 0; r < 3; ++r) {
                     double v = SLm[r][b] * isl;
                     if (a.defl_sc != 0.0) v += a.defl_sc * tn[r] * t.nrm[s][b] * t.area[s];
                     col[r] = v;
+                    if (col32) col32[r] = (float)v;
                 }
             }
         }
@@ -535,11 +537,12 @@ static double* part_buffer(size_t n) {
 
 void assemble(const DMesh& src, int p0, int np, const double* x0, int nt, const int* self_src, double sl_div,
               double* SL, long ld_sl, double dl_div, double* DL, long ld_dl, double* dlsum, double defl_sc,
-              const double* defl_tn) {
+              const double* defl_tn, float* SL32, long ld_sl32) {
     if (nt <= 0 || np <= 0) return;
     SQ_REQUIRE(p0 >= 0 && p0 + np <= src.np, SQBEM_ERR_ARG, "assemble: source panel range out of bounds");
     SQ_REQUIRE(!SL || ld_sl >= 3L * nt, SQBEM_ERR_ARG, "assemble: SL leading dimension too small");
     SQ_REQUIRE(!DL || ld_dl >= 3L * nt, SQBEM_ERR_ARG, "assemble: DL leading dimension too small");
+    SQ_REQUIRE(!SL32 || (SL && ld_sl32 >= 3L * nt), SQBEM_ERR_ARG, "assemble: SL32 without SL, or its leading dimension too small");
     AsmArgs a{};
     a.nt = nt;
     a.x0 = x0;
@@ -550,6 +553,7 @@ void assemble(const DMesh& src, int p0, int np, const double* x0, int nt, const 
     a.V = src.V.p; a.nrm = src.nrm.p; a.cen = src.cen.p; a.area = src.area.p; a.h = src.h.p; a.q7 = src.q7.p;
     a.sl_div = sl_div; a.dl_div = dl_div;
     a.SL = SL; a.ld_sl = ld_sl; a.DL = DL; a.ld_dl = ld_dl;
+    a.SL32 = SL32; a.ld_sl32 = ld_sl32;
     a.defl_sc = defl_sc; a.defl_tn = defl_tn;
     const int ntiles = (int)ceil_div(np, TS);
     if (dlsum) a.dls_part = part_buffer((size_t)ntiles * 9 * nt);

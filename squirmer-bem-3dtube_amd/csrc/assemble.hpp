@@ -37,6 +37,8 @@ struct AsmArgs {
     double sl_div, dl_div;
     double* SL;
     long ld_sl;
+    float* SL32;  // optional FP32 copy of SL, written with it
+    long ld_sl32;
     double* DL;
     long ld_dl;
     double* dls_part;
@@ -48,10 +50,11 @@ struct AsmArgs {
 };
 
 // SL(3i+a, 3p+b) = SL_ab/sl_div (+ defl_sc tn_i[a] n_p[b] A_p), DL(...) = -DL_ab/dl_div,
-// dlsum[9i+k] += DL_k. Device pointers; any output may be null.
+// dlsum[9i+k] += DL_k; SL32 (optional) receives SL rounded to FP32. Device pointers; any
+// output may be null.
 void assemble(const DMesh& src, int p0, int np, const double* x0, int nt, const int* self_src, double sl_div,
               double* SL, long ld_sl, double dl_div, double* DL, long ld_dl, double* dlsum, double defl_sc,
-              const double* defl_tn);
+              const double* defl_tn, float* SL32 = nullptr, long ld_sl32 = 0);
 
 // One job of a batched assembly: the arguments of assemble(), except that the stresslet
 // row sums stay per source tile in dls_part (asm_tiles(np) x 9 x nt doubles, or null) for

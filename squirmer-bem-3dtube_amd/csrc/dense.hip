@@ -1963,39 +1963,52 @@ __global__ void laswp_kernel(T* A, int lda, int ncols, const int* __restrict__ i
     }
 }
 
-// Long pivot ranges: one workgroup first compacts the interchanges that move a row into an
-// ordered list (ballot + prefix count per chunk), then the swap kernel walks only that list,
-// so identity pivots (the no-pivot factorisation) cost one short pass over ipiv.
+// Long pivot ranges: the interchanges that move a row are first compacted into an ordered
+// list, then the swap kernel walks only that list, so identity pivots (the no-pivot
+// factorisation) cost one short pass over ipiv. Workgroup c takes pivots 1024 c .. 1024 c +
+// 1023 of the range (coalesced), places its moves in order in its own slot region (ballot
+// and per-wave counts) and records their number; the last workgroup to finish (a device-scope
+// counter, which it resets for the next launch) lays the regions out one after another.
 __global__ void __launch_bounds__(1024) piv_compact_kernel(const int* __restrict__ ipiv, int ibase, int vbase, int j0,
-                                                           int j1, int* list, int* count) {
+                                                           int j1, int* list, int* count, int* ccount, int* slots,
+                                                           unsigned* done) {
     __shared__ int wsum[16];
-    __shared__ int s_total;
+    __shared__ int s_last;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    int total = 0;
-    for (int b = j0; b < j1; b += 1024) {
-        const int j = b + tid;
-        const int p = j < j1 ? ipiv[ibase + j] - vbase : j;
-        const bool mv = j < j1 && p != j;
-        const unsigned long long m = __ballot(mv);
-        const int before = __popcll(m & ((1ull << lane) - 1ull));
-        if (lane == 0) wsum[wv] = __popcll(m);
-        __syncthreads();
-        int off = total;
-        for (int q = 0; q < wv; ++q) off += wsum[q];
-        if (mv) {
-            list[2 * (off + before)] = j;
-            list[2 * (off + before) + 1] = p;
-        }
-        if (tid == 0) {
-            int t = 0;
-            for (int q = 0; q < 16; ++q) t += wsum[q];
-            s_total = total + t;
-        }
-        __syncthreads();
-        total = s_total;
-        __syncthreads();
+    const int c = blockIdx.x, nch = gridDim.x;
+    const int j = j0 + c * 1024 + tid;
+    const int p = j < j1 ? ipiv[ibase + j] - vbase : j;
+    const bool mv = j < j1 && p != j;
+    const unsigned long long m = __ballot(mv);
+    if (lane == 0) wsum[wv] = __popcll(m);
+    __syncthreads();
+    if (mv) {
+        int k = __popcll(m & ((1ull << lane) - 1ull));
+        for (int q = 0; q < wv; ++q) k += wsum[q];
+        slots[c * 2048 + 2 * k] = j;
+        slots[c * 2048 + 2 * k + 1] = p;
     }
-    if (tid == 0) *count = total;
+    if (tid == 0) {
+        int tot = 0;
+        for (int q = 0; q < 16; ++q) tot += wsum[q];
+        ccount[c] = tot;
+    }
+    __threadfence();  // this chunk's slots and count, device-wide, before it is counted
+    __syncthreads();
+    if (tid == 0) s_last = atomicAdd(done, 1u) == (unsigned)(nch - 1);
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();  // every chunk's writes, seen by the last workgroup
+    int base = 0;
+    for (int cc = 0; cc < nch; ++cc) {
+        const int nc = __hip_atomic_load(&ccount[cc], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int e = tid; e < 2 * nc; e += 1024) list[2 * base + e] = slots[cc * 2048 + e];
+        base += nc;
+    }
+    if (tid == 0) {
+        *count = base;
+        *done = 0u;
+    }
 }
 
 template <typename T>
@@ -2014,19 +2027,27 @@ __global__ void laswp_list_kernel(T* A, int lda, int ncols, const int* __restric
     }
 }
 
-static DBuf<int> g_swaps;  // [0] count, then (row, pivot row) pairs
+// [0] count, [1] the compaction's completion counter (zero between launches), then the
+// (row, pivot row) pairs, the per-chunk counts and the per-chunk slot regions
+static DBuf<int> g_swaps;
 
 template <typename T>
 static void laswp(T* A, int lda, int ncols, const int* ipiv, int ibase, int vbase, int j0, int j1, int dir = 1) {
     if (ncols <= 0 || j1 <= j0) return;
     ProfScope prof(PROF_LASWP, 0.0, 0.0);
     if (j1 - j0 > 256) {
-        const size_t need = 1 + 2 * (size_t)(j1 - j0);
-        if (g_swaps.n < need) g_swaps.alloc(need);
-        hipLaunchKernelGGL(piv_compact_kernel, dim3(1), dim3(1024), 0, ctx().s(), ipiv, ibase, vbase, j0, j1,
-                           g_swaps.p + 1, g_swaps.p);
+        const size_t len = (size_t)(j1 - j0), nch = (len + 1023) / 1024;
+        const size_t need = 2 + 2 * len + nch + nch * 2048;
+        if (g_swaps.n < need) {
+            g_swaps.alloc(need);
+            g_swaps.zero();
+        }
+        int* list = g_swaps.p + 2;
+        int* ccount = list + 2 * len;
+        hipLaunchKernelGGL(piv_compact_kernel, dim3((unsigned)nch), dim3(1024), 0, ctx().s(), ipiv, ibase, vbase, j0,
+                           j1, list, g_swaps.p, ccount, ccount + nch, (unsigned*)(g_swaps.p + 1));
         hipLaunchKernelGGL(laswp_list_kernel<T>, dim3(ceil_div(ncols, 256)), dim3(256), 0, ctx().s(), A, lda, ncols,
-                           g_swaps.p + 1, g_swaps.p, dir);
+                           list, g_swaps.p, dir);
         check_launch("laswp");
         return;
     }
@@ -2233,17 +2254,20 @@ __global__ void __launch_bounds__(256) trsm2_kernel(int n, int nrhs, const T* L,
 // same MFMA sums in the same order as gemm_nn_tile's (k tiles of 32, steps of 4), so the
 // results equal trsm2_kernel's.
 template <bool LOWER, int TN>
-__global__ void __launch_bounds__(256) trsm2p_kernel(int n, int nrhs, const float* L, int ldl, const float* inv,
-                                                     float* B, int ldb) {
+__global__ void __launch_bounds__(256, TN <= 32 ? 2 : 1) trsm2p_kernel(int n, int nrhs, const float* L, int ldl,
+                                                                       const float* inv, float* B, int ldb) {
     using Op = MfmaOp<float>;
     using Acc = typename Op::acc;
     constexpr int KB = GemmK<float>::KB, PAD = GemmK<float>::PAD, NQ = KB / 2, S = KB / 4;
     constexpr int NKT = TRSM2_B / KB;  // k tiles of a product (k <= 128)
-    constexpr int JT = TN / 32, NP = 2 * TRSM2_B + 34;  // panel column stride (34 mod 64 words)
+    // waves of 64 rows x TN/2 columns (2 x 2), or, for 16-column panels, of 32 rows x 16
+    // columns (4 x 1): MI 16-row and JT 16-column MFMA tiles each
+    constexpr int MI = TN == 16 ? 2 : 4, JT = TN == 16 ? 1 : TN / 32;
+    constexpr int NP = 2 * TRSM2_B + 34;  // panel column stride (34 mod 64 words)
     __shared__ __attribute__((aligned(16))) float Pn[TN * NP];        // Pn[c * NP + r] = B(r, c)
     __shared__ __attribute__((aligned(16))) float As[2][GBM][PAD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave & 1) * 64, wn = (wave >> 1) * (TN / 2);
+    const int wm = TN == 16 ? wave * 32 : (wave & 1) * 64, wn = TN == 16 ? 0 : (wave >> 1) * (TN / 2);
     const int lr = lane & 15, lk = lane >> 4;
     const int n0 = blockIdx.x * TN, n1 = TRSM2_B, n2 = n - TRSM2_B;
     const float* inv2 = inv + (size_t)TRSM2_B * TRSM2_B;
@@ -2281,12 +2305,12 @@ __global__ void __launch_bounds__(256) trsm2p_kernel(int n, int nrhs, const floa
         for (int c = 0; c < TN; ++c) Pn[c * NP + tid] = (tid < n && n0 + c < nrhs) ? v[c] : 0.0f;
     }
     // acc = A(0:mr, 0:kd) P(ko:ko+kd, :) with A held in r
-    Acc acc[4][JT];
+    Acc acc[MI][JT];
     auto product = [&](const float (&r)[NKT][NQ], int o, int ko) {
         const int mr = opm[o], kd = opk[o];
         const int nk = (kd + KB - 1) / KB;
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < MI; ++i)
 #pragma unroll
             for (int j = 0; j < JT; ++j) acc[i][j] = Acc{0, 0, 0, 0};
         auto store = [&](int buf, int kt) {
@@ -2302,13 +2326,13 @@ __global__ void __launch_bounds__(256) trsm2p_kernel(int n, int nrhs, const floa
                 const int buf = kt & 1;
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
-                    float af[4], bf[JT];
+                    float af[MI], bf[JT];
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) af[i] = As[buf][wm + i * 16 + lr][4 * s + lk];
+                    for (int i = 0; i < MI; ++i) af[i] = As[buf][wm + i * 16 + lr][4 * s + lk];
 #pragma unroll
                     for (int j = 0; j < JT; ++j) bf[j] = Pn[(wn + j * 16 + lr) * NP + ko + kt * KB + 4 * s + lk];
 #pragma unroll
-                    for (int i = 0; i < 4; ++i)
+                    for (int i = 0; i < MI; ++i)
 #pragma unroll
                         for (int j = 0; j < JT; ++j) acc[i][j] = Op::mma(af[i], bf[j], acc[i][j]);
                 }
@@ -2320,7 +2344,7 @@ __global__ void __launch_bounds__(256) trsm2p_kernel(int n, int nrhs, const floa
     // rows ro .. ro + mr - 1 of the panel: = acc (set) or -= acc (sub, as gemm's -acc + c)
     auto put = [&](int ro, int mr, bool sub) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < MI; ++i)
 #pragma unroll
             for (int j = 0; j < JT; ++j)
 #pragma unroll
@@ -2367,9 +2391,14 @@ static void trsm_inv(int kind, int n, int nrhs, const T* L, int ldl, const T* in
         if (n <= 2 * BS) {  // split_point gives n1 = BS here
             ProfScope prof(PROF_TRSM, 2.0 * n * n * nrhs, sizeof(T) * (2.0 * n * nrhs + (double)n * n));
             if constexpr (sizeof(T) == 4) {
-                // 32-column panels when 64-column ones would leave CUs idle (narrow right-hand
-                // sides inside the LU): twice the workgroups, half the work each
-                if (ceil_div(nrhs, TRSM2_TN) < std::max(ctx().cu_count, 1))
+                // narrower panels when 64-column ones would leave CUs idle (narrow right-hand
+                // sides inside the LU): 32 columns, or 16 when even those fill under half the
+                // CUs; more workgroups, less work each
+                const int cus = std::max(ctx().cu_count, 1);
+                if (ceil_div(nrhs, 32) < cus / 2)
+                    hipLaunchKernelGGL((trsm2p_kernel<LOWER, 16>), dim3((unsigned)ceil_div(nrhs, 16)), dim3(256), 0,
+                                       ctx().s(), n, nrhs, L, ldl, inv, B, ldb);
+                else if (ceil_div(nrhs, TRSM2_TN) < cus)
                     hipLaunchKernelGGL((trsm2p_kernel<LOWER, 32>), dim3((unsigned)ceil_div(nrhs, 32)), dim3(256), 0,
                                        ctx().s(), n, nrhs, L, ldl, inv, B, ldb);
                 else
@@ -2938,6 +2967,7 @@ void trsv_reset_counters() {
     if (g_trsv.p) g_trsv.zero();
     g_trsv_words.zero();
     g_trsv_base = 0;
+    g_swaps.zero();  // the pivot compaction's completion counter
 }
 
 // B (n x nrhs, nrhs <= 8) <- T^-1 B with the inverted 128 x 128 diagonal blocks of T at

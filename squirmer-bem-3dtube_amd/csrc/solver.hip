@@ -340,20 +340,20 @@ void sq::tube_build(sqbem_tube* t) {
     DBuf<int> self;
     device_iota(self, Nt);
     t->T64.alloc((size_t)nt * nt);
+    // the refined modes factor an FP32 copy, written by the assembly beside the FP64 matrix
+    if (mixed(t->prec)) t->T32.alloc((size_t)nt * nt);
     // Wielandt deflation of the f = c m null mode, fused into the assembly
     // (Tube::deflate, src/swimmer.hpp:64-74): += sc n_i[a] n_p[b] A_p, sc = 1/(8 pi mu sqrt(At))
     double At = 0;
     for (int p = 0; p < Nt; ++p) At += t->mesh.area[p];
     const double sc = 1.0 / (bem::EIGHT_PI * t->mu * std::sqrt(At));
     assemble(t->dm, 0, Nt, t->dm.cen_xyz.p, Nt, self.p, bem::EIGHT_PI * t->mu, t->T64.p, nt, 1.0, nullptr, 0, nullptr,
-             sc, t->dm.nrm_xyz.p);
+             sc, t->dm.nrm_xyz.p, t->T32.p, nt);
     sync();
     double t1 = now_s();
     t->ipiv.alloc(nt);
     // the factorisation is issued here and finished by tube_ready (its first use)
     if (mixed(t->prec)) {
-        t->T32.alloc((size_t)nt * nt);
-        convert_matrix<double, float>(nt, nt, t->T64.p, nt, t->T32.p, nt);
         t->aux32.alloc(getrf_aux_elems(nt));
         t->job32.reset(new GetrfJob<float>);
         GetrfJob<float>& j = *t->job32;

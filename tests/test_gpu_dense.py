@@ -315,12 +315,13 @@ def test_host_linalg_api(sq):
         sq.gemm(1.0, A, B, 0.0, np.zeros((5, 5), order="F"))
 
 
-@pytest.mark.parametrize("n,nrhs", [(256, 16400), (256, 1000), (200, 16384)])
-def test_sgetrs_fused_bottom_levels_both_panel_widths(sq, n, nrhs):
+@pytest.mark.parametrize("n,nrhs", [(256, 16400), (200, 16384), (256, 6000), (200, 5000), (256, 1000), (160, 40)])
+def test_sgetrs_fused_bottom_levels_all_panel_widths(sq, n, nrhs):
     """FP32 factors of order 128 < n <= 256 solve many right-hand sides in one fused launch
     of the triangular solves' bottom two levels: 64-column panels once ceil(nrhs / 64) fills
-    every CU (nrhs >= 16,321 on 256 CUs), 32-column panels below that. Both widths against
-    the FP32 backward-error bound."""
+    every CU (nrhs >= 16,321 on 256 CUs), 32-column panels while ceil(nrhs / 32) fills at
+    least half of them (nrhs >= 4,065), 16-column panels below that. Every width against the
+    FP32 backward-error bound."""
     rng = np.random.default_rng(n + nrhs)
     A = rng.standard_normal((n, n)) + n ** 0.5 * np.eye(n)
     X = rng.standard_normal((n, nrhs))
