@@ -133,39 +133,42 @@ struct GemmK {
 
 // MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16] and B[l/16][l%16]; the four
 // accumulator values of lane l are D[Op::row(l/16, r)][l%16].
-// Workgroup tile 128 x TN (TN = 128, or 64 for products whose 128 x 128 tiles would leave
-// CUs idle): four waves of 64 x TN/2.
-template <typename T, int TN = GBN>
+// Workgroup tile TM x TN (TM = 128: TN = 128, or 64 for products whose 128 x 128 tiles would
+// leave CUs idle, four waves of 64 x TN/2; TM = 64: four waves of 64 x TN/4, for products
+// whose 128-row tiles would give the CUs under two workgroups each).
+template <typename T, int TN = GBN, int TM = GBM>
 __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const T* __restrict__ A, int lda,
                                              const T* __restrict__ B, int ldb, T beta, T* __restrict__ C, int ldc,
                                              int tm, int tn, int* unit_flag = nullptr) {
     using Op = MfmaOp<T>;
     using Acc = typename Op::acc;
-    constexpr int KB = GemmK<T>::KB, PAD = GemmK<T>::PAD, NQ = KB / 2, JS = 256 / KB, NQB = TN * KB / 256;
-    constexpr int JT = TN / 32;  // 16-column MFMA tiles per wave
+    static_assert(TM == 128 || TM == 64, "row tile of 64 or 128");
+    constexpr int KB = GemmK<T>::KB, PAD = GemmK<T>::PAD, AS = 256 / TM, NQ = TM * KB / 256, JS = 256 / KB,
+                  NQB = TN * KB / 256;
+    constexpr int JT = TM == 128 ? TN / 32 : TN / 64;  // 16-column MFMA tiles per wave
     // one LDS block: the two operand stages during the k loop, the C tile in the epilogue
-    __shared__ __attribute__((aligned(16))) T smem[2 * GBM * PAD + 2 * TN * PAD];
-    T(*As)[GBM][PAD] = reinterpret_cast<T(*)[GBM][PAD]>(smem);
-    T(*Bs)[TN][PAD] = reinterpret_cast<T(*)[TN][PAD]>(smem + 2 * GBM * PAD);
+    __shared__ __attribute__((aligned(16))) T smem[2 * TM * PAD + 2 * TN * PAD];
+    T(*As)[TM][PAD] = reinterpret_cast<T(*)[TM][PAD]>(smem);
+    T(*Bs)[TN][PAD] = reinterpret_cast<T(*)[TN][PAD]>(smem + 2 * TM * PAD);
 
-    const int m0 = tm * GBM, n0 = tn * TN;
+    const int m0 = tm * TM, n0 = tn * TN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave & 1) * 64, wn = (wave >> 1) * (TN / 2);
+    const int wm = TM == 128 ? (wave & 1) * 64 : 0, wn = TM == 128 ? (wave >> 1) * (TN / 2) : wave * (TN / 4);
 
-    // global -> register staging: A column kk has 128 contiguous rows; B column j has KB
+    // global -> register staging: A column kk has TM contiguous rows; B column j has KB
     // contiguous k values. Loads read clamped (always valid) addresses and are masked when
     // written into LDS, so no instruction waits for a load before the LDS write needs it.
-    const int a_i = tid & 127, a_k0 = tid >> 7;     // rows a_i, k = a_k0 + 2q
+    const int a_i = tid % TM, a_k0 = tid / TM;     // rows a_i, k = a_k0 + AS q
     const int b_k = tid % KB, b_j0 = tid / KB;      // k = b_k, cols b_j0 + JS q
     const int gi = m0 + a_i, gic = min(gi, m - 1);
     T ra[NQ], rb[NQB];
-    auto load_a = [&](int k0, int q) { ra[q] = A[gic + (size_t)min(k0 + a_k0 + 2 * q, k - 1) * lda]; };
+    auto load_a = [&](int k0, int q) { ra[q] = A[gic + (size_t)min(k0 + a_k0 + AS * q, k - 1) * lda]; };
     auto load_b = [&](int k0, int q) {
         rb[q] = B[min(k0 + b_k, k - 1) + (size_t)min(n0 + b_j0 + JS * q, n - 1) * ldb];
     };
     auto store = [&](int buf, int k0) {
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) As[buf][a_i][a_k0 + 2 * q] = (gi < m && k0 + a_k0 + 2 * q < k) ? ra[q] : T(0);
+        for (int q = 0; q < NQ; ++q) As[buf][a_i][a_k0 + AS * q] = (gi < m && k0 + a_k0 + AS * q < k) ? ra[q] : T(0);
 #pragma unroll
         for (int q = 0; q < NQB; ++q)
             Bs[buf][b_j0 + JS * q][b_k] = (k0 + b_k < k && n0 + b_j0 + JS * q < n) ? rb[q] : T(0);
@@ -228,10 +231,10 @@ __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const
     // accumulators are written into a column-major tile of CW columns at a time, then every
     // thread updates C down columns, consecutive rows on consecutive lanes, with all of its
     // C values read before any result is stored
-    constexpr int LDC = GBM + 4, CW = (int)(sizeof(smem) / (LDC * sizeof(T))) >= TN ? TN : TN / 2;
+    constexpr int LDC = TM + 4, CW = (int)(sizeof(smem) / (LDC * sizeof(T))) >= TN ? TN : TN / 2;
     static_assert(CW * LDC * sizeof(T) <= sizeof(smem), "epilogue tile exceeds the LDS block");
     T* Ct = smem;
-    const int er = tid & (GBM - 1), ec = tid >> 7;  // row, first column (then every second)
+    const int er = tid % TM, ec = tid / TM;  // row, first column (then every AS-th)
     int bad = 0;
 #pragma unroll
     for (int c0 = 0; c0 < TN; c0 += CW) {
@@ -249,17 +252,17 @@ __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const
         }
         __syncthreads();
         const int row = m0 + er;
-        T cv[CW / 2];
+        T cv[CW / AS];
         if (beta != T(0)) {
 #pragma unroll
-            for (int q = 0; q < CW / 2; ++q)
-                cv[q] = C[min(row, m - 1) + (size_t)min(n0 + c0 + ec + 2 * q, n - 1) * ldc];
+            for (int q = 0; q < CW / AS; ++q)
+                cv[q] = C[min(row, m - 1) + (size_t)min(n0 + c0 + ec + AS * q, n - 1) * ldc];
         }
 #pragma unroll
-        for (int q = 0; q < CW / 2; ++q) {
-            const int col = n0 + c0 + ec + 2 * q;
+        for (int q = 0; q < CW / AS; ++q) {
+            const int col = n0 + c0 + ec + AS * q;
             if (row < m && col < n) {
-                const T v = alpha * Ct[(ec + 2 * q) * LDC + er];
+                const T v = alpha * Ct[(ec + AS * q) * LDC + er];
                 const T out = beta == T(0) ? v : v + beta * cv[q];
                 C[row + (size_t)col * ldc] = out;
                 if (unit_flag && !(fabs((double)out) <= 1.0)) bad = 1;
@@ -270,13 +273,13 @@ __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const
     if (unit_flag && __syncthreads_or(bad) && tid == 0) atomicOr(unit_flag, 1);
 }
 
-template <typename T, int TN = GBN>
+template <typename T, int TN = GBN, int TM = GBM>
 __global__ void __launch_bounds__(256) gemm_nn_kernel(int m, int n, int k, T alpha, const T* __restrict__ A,
                                                       int lda, const T* __restrict__ B, int ldb, T beta,
                                                       T* __restrict__ C, int ldc, int tiles_m, int* unit_flag) {
     int tm, tn;
     tile_of(blockIdx.x, gridDim.x, tiles_m, tm, tn);
-    gemm_nn_tile<T, TN>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, unit_flag);
+    gemm_nn_tile<T, TN, TM>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, unit_flag);
 }
 
 // Many independent GEMMs C_j = alpha_j A_j B_j + beta_j C_j in one launch: workgroup
@@ -973,16 +976,28 @@ static void gemm_launch(int m, int n, int k, T alpha, const T* A, int lda, const
     const int tiles_m = (int)ceil_div(m, GBM);
     ProfScope prof(sizeof(T) == 8 ? PROF_DGEMM : PROF_SGEMM, 2.0 * m * n * k,
                    (double)sizeof(T) * ((double)m * k + (double)k * n + (beta != T(0) ? 2.0 : 1.0) * m * n));
-    // 128 x 64 tiles when 128 x 128 ones would leave CUs idle (a one-row-tile product with
-    // many columns, e.g. the inverse-block leaves of the triangular solves). Not when A and C
-    // overlap: an in-place product (L21 = A21 U11^-1) relies on one workgroup holding all
-    // columns of its rows, which it reads before it writes them.
+    // Smaller tiles when 128 x 128 ones would leave the CUs short of work. An in-place
+    // product relies on one workgroup reading all of C's rows and columns that its own
+    // results overwrite before it writes them: with A overlapping C (L21 = A21 U11^-1) a
+    // workgroup must hold every column of its rows, with B overlapping C (X = inv B) every
+    // row of its columns.
     const char* a0 = (const char*)A;
     const char* a1 = (const char*)(A + (size_t)(k - 1) * lda + m);
+    const char* b0 = (const char*)B;
+    const char* b1 = (const char*)(B + (size_t)(n - 1) * ldb + k);
     const char* c0 = (const char*)C;
     const char* c1 = (const char*)(C + (size_t)(n - 1) * ldc + m);
-    const bool overlap = a0 < c1 && c0 < a1;
-    if (!overlap && (int)(tiles_m * ceil_div(n, GBN)) < std::max(ctx().cu_count, 1)) {
+    const bool overlap = a0 < c1 && c0 < a1, overlap_b = b0 < c1 && c0 < b1;
+    const int cus = std::max(ctx().cu_count, 1);
+    const int tiles128 = (int)(tiles_m * ceil_div(n, GBN));
+    if (!overlap_b && n <= GBN && tiles128 < 2 * cus) {
+        // 64 x 128 tiles: twice the workgroups, every one holding all columns of its rows
+        const int tm64 = (int)ceil_div(m, 64);
+        hipLaunchKernelGGL((gemm_nn_kernel<T, GBN, 64>), dim3(tm64), dim3(256), 0, ctx().s(), m, n, k, alpha, A, lda,
+                           B, ldb, beta, C, ldc, tm64, unit_flag);
+    } else if (!overlap && tiles128 < cus) {
+        // 128 x 64 tiles (a one-row-tile product with many columns, e.g. the inverse-block
+        // leaves of the triangular solves)
         const int tiles_n = (int)ceil_div(n, 64);
         hipLaunchKernelGGL((gemm_nn_kernel<T, 64>), dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, k, alpha,
                            A, lda, B, ldb, beta, C, ldc, tiles_m, unit_flag);
