@@ -149,6 +149,11 @@ int sqbem_sgetrs(int n, int nrhs, const float* LU, int lda, const int* ipiv, flo
  * with partial pivoting. A swimmer's construction factors the tube, then the bordered
  * Schur matrix: afterwards this tells about the bordered matrix. */
 int sqbem_getrf_path(void);
+/* Cooperative launches of the pivoted LU's panel kernel (a test switch, no reference
+ * counterpart; dgetrf_ has no such choice): 1 (the default) uses them while the runtime
+ * accepts them; 0 makes every pivoted panel take the barrier-free path that the library
+ * otherwise uses only once the runtime has refused a cooperative launch. */
+int sqbem_coop(int enable);
 
 /* Tile autotuner of the 16-bit GEMMs (no reference counterpart): mode 1 (the default)
  * learns the fastest of fourteen candidates (the SQBEM_PROF_G16Q list: workgroup tile and

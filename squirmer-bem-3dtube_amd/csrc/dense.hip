@@ -1561,6 +1561,10 @@ constexpr int PANEL_WG = 256;  // threads per workgroup
 constexpr int IB = LEAF;       // diagonal blocks inverted for the triangular solves
 constexpr int IBP = IB + 1;
 
+// cleared once the runtime refuses a cooperative launch (or by sqbem_coop(0)): the pivoted
+// panels then take the barrier-free path
+static bool g_coop_ok = true;
+
 // Workspace of one factorisation: candidates of the cooperative pivot search, a saved
 // copy of the current panel, one verification flag per panel, and the inverses of the
 // IB x IB diagonal blocks of L (unit lower) used by the triangular solves inside getrf.
@@ -1572,6 +1576,8 @@ struct PanelWork {
     DBuf<int> flags;    // per panel: 1 when row interchanges were needed
     DBuf<char> backup;  // m x LEAF copy of the panel in its storage type
     DBuf<char> inv;     // Linv blocks then Uinv blocks, IB x IB each, storage type
+    DBuf<double> serial; // m x LEAF working panel of the barrier-free fallback (FP64)
+    DBuf<int> pivrow;   // the fallback's pivot row of the current column
     int gmax = 0;
     void ensure(int g, int n, size_t elem) {
         if (gmax < g || !cand.p) {
@@ -1585,11 +1591,12 @@ struct PanelWork {
         if (flags.n < nb + 1) flags.alloc(nb + 1);
         if (backup.n < (size_t)n * LEAF * elem) backup.alloc((size_t)n * LEAF * elem);
         if (inv.n < 2 * nb * IB * IB * elem) inv.alloc(2 * nb * IB * IB * elem);
+        if (!g_coop_ok && serial.n < (size_t)n * LEAF) serial.alloc((size_t)n * LEAF);
+        if (!pivrow.p) pivrow.alloc(1);
     }
 };
 static PanelWork g_panel;
 static int g_bar_parity = 0;
-static bool g_coop_ok = true;  // cleared once the runtime refuses a cooperative launch
 
 // Inverses of one IB x IB diagonal block B (LDS, row-major; strict lower part = L with a unit
 // diagonal, upper part = U), padded to IB with the identity beyond w. Threads 0..IB-1 solve
@@ -1907,6 +1914,111 @@ __global__ void __launch_bounds__(PANEL_WG) getf2_coop(T* A, int lda, int m, int
     }
 }
 
+// Barrier-free form of getf2_coop, for when the runtime refuses a cooperative launch (whose
+// residency guarantee the grid barrier needs): the same arithmetic (the panel in FP64, the
+// first largest |a| of each column as pivot, rows swapped, multipliers l = a * (1 / pivot),
+// rank-one updates) on a working copy in global memory, one column at a time in three
+// launches (pivot search, swap, update), then the factors stored and the new L11 inverted.
+// Every kernel returns at once when the speculative pass verified its panel (flag 0).
+template <typename T>
+__global__ void pser_load(const int* flag, const T* __restrict__ backup, size_t mw, double* P) {
+    if (*flag == 0) return;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < mw; e += (size_t)gridDim.x * blockDim.x)
+        P[e] = (double)backup[e];
+}
+__global__ void __launch_bounds__(1024) pser_pivot(const int* flag, const double* __restrict__ P, int m, int j,
+                                                   int* pivrow) {
+    if (*flag == 0) return;
+    __shared__ double rv[1024];
+    __shared__ int ri[1024];
+    const int tid = threadIdx.x;
+    double best = -1.0;
+    int brow = 0x7fffffff;
+    for (int r = j + tid; r < m; r += 1024) {
+        const double v = fabs(P[r + (size_t)j * m]);
+        if (v > best || (v == best && r < brow)) { best = v; brow = r; }
+    }
+    rv[tid] = best;
+    ri[tid] = brow;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (tid < off) {
+            const double v = rv[tid + off];
+            const int r = ri[tid + off];
+            if (v > rv[tid] || (v == rv[tid] && r < ri[tid])) { rv[tid] = v; ri[tid] = r; }
+        }
+        __syncthreads();
+    }
+    // no finite candidate (column all NaN): keep row j, a zero pivot is reported by the swap
+    if (tid == 0) *pivrow = rv[0] >= 0.0 ? ri[0] : j;
+}
+__global__ void pser_swap(const int* flag, double* P, int m, int w, int j, const int* pivrow, int* ipiv, int base,
+                          int* status) {
+    if (*flag == 0) return;
+    const int pr = *pivrow, c = threadIdx.x;
+    if (c < w && pr != j) {
+        const double t = P[j + (size_t)c * m];
+        P[j + (size_t)c * m] = P[pr + (size_t)c * m];
+        P[pr + (size_t)c * m] = t;
+    }
+    if (c == 0) {
+        ipiv[j] = base + pr;
+        const double pv = P[pr + (size_t)j * m];  // the value moved to row j
+        if (!(pv != 0.0 && pv == pv) && status[0] == 0) status[0] = base + j + 1;
+    }
+}
+__global__ void pser_update(const int* flag, double* P, int m, int w, int j) {
+    if (*flag == 0) return;
+    const int r = j + 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= m) return;
+    const double pv = P[j + (size_t)j * m];
+    const double rinv = pv != 0.0 ? 1.0 / pv : 0.0;
+    const double l = P[r + (size_t)j * m] * rinv;
+    P[r + (size_t)j * m] = l;
+    for (int c = j + 1; c < w; ++c) P[r + (size_t)c * m] -= l * P[j + (size_t)c * m];
+}
+template <typename T>
+__global__ void pser_store(const int* flag, const double* __restrict__ P, int m, int w, T* A, int lda) {
+    if (*flag == 0) return;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < (size_t)m * w;
+         e += (size_t)gridDim.x * blockDim.x)
+        A[(e % m) + (e / m) * (size_t)lda] = (T)P[e];
+}
+template <typename T>
+__global__ void __launch_bounds__(PANEL_WG) pser_invert(const int* flag, const double* __restrict__ P, int m, int w,
+                                                        T* Linv) {
+    if (*flag == 0) return;
+    __shared__ double TBs[IB][IBP];
+    __shared__ double X[2 * IB * IBP];
+    for (int e = threadIdx.x; e < IB * IB; e += PANEL_WG) {
+        const int i = e % IB, j = e / IB;
+        TBs[i][j] = (i < w && j < w) ? P[i + (size_t)j * m] : 0.0;
+    }
+    __syncthreads();
+    invert_block<T>(TBs, w, (double(*)[IBP])X, (double(*)[IBP])(X + IB * IBP), Linv, nullptr);
+}
+
+template <typename T>
+static void panel_leaf_serial(T* A, int lda, int m, int w, int* ipiv, int base, const T* backup, const int* flag,
+                              T* Linv) {
+    double* P = g_panel.serial.p;
+    int* pr = g_panel.pivrow.p;
+    int* status = g_panel.status.p;
+    const size_t mw = (size_t)m * w;
+    const unsigned gl = (unsigned)std::min<size_t>(std::max<size_t>(ceil_div(mw, 256), 1), 4096);
+    hipLaunchKernelGGL(pser_load<T>, dim3(gl), dim3(256), 0, ctx().s(), flag, backup, mw, P);
+    for (int j = 0; j < w; ++j) {
+        hipLaunchKernelGGL(pser_pivot, dim3(1), dim3(1024), 0, ctx().s(), flag, P, m, j, pr);
+        hipLaunchKernelGGL(pser_swap, dim3(1), dim3(64), 0, ctx().s(), flag, P, m, w, j, pr, ipiv, base, status);
+        if (m - j - 1 > 0)
+            hipLaunchKernelGGL(pser_update, dim3((unsigned)ceil_div(m - j - 1, 256)), dim3(256), 0, ctx().s(), flag, P,
+                               m, w, j);
+    }
+    hipLaunchKernelGGL(pser_store<T>, dim3(gl), dim3(256), 0, ctx().s(), flag, P, m, w, A, lda);
+    hipLaunchKernelGGL(pser_invert<T>, dim3(1), dim3(PANEL_WG), 0, ctx().s(), flag, P, m, w, Linv);
+    check_launch("panel (barrier-free)");
+}
+
 template <typename T>
 static void panel_leaf(T* A, int lda, int m, int w, int* ipiv, int base) {
     // one workgroup per >= 128 rows, at most one per CU (the fallback is cooperative)
@@ -1939,21 +2051,19 @@ static void panel_leaf(T* A, int lda, int m, int w, int* ipiv, int base) {
     void* args[] = {&A, &lda, &m, &w, &rpw, &ipiv, &base, &cand, &rowj, &bar, &bar_next, &status, &cflag, &cbackup,
                     &Linv};
     ProfScope prof(PROF_PANEL, 0.0, 0.0);
-    // The cooperative launch guarantees that all G <= CU-count workgroups are resident. The
-    // runtime refuses it when another process on the device holds the cooperative (global
-    // wave sync) queue; then an ordinary launch of the same grid is used: one workgroup
-    // per CU of an otherwise idle device is resident as well, and if one ever is not, the
-    // bounded barrier times out and getrf reports the error instead of waiting.
+    // The cooperative launch guarantees that all G <= CU-count workgroups are resident, as
+    // the grid barrier needs. The runtime refuses it when another process on the device holds
+    // the cooperative (global wave sync) queue; then, and from then on, the panel takes the
+    // barrier-free path, which needs no residency.
     if (g_coop_ok) {
         const hipError_t e = hipLaunchCooperativeKernel((const void*)getf2_coop<T>, dim3(G), dim3(PANEL_WG), args,
                                                         (unsigned)shmem, ctx().s());
         if (e == hipSuccess) return;
         (void)hipGetLastError();  // launch refused (nothing was enqueued): clear it
         g_coop_ok = false;
+        if (g_panel.serial.n < (size_t)m * LEAF) g_panel.serial.alloc((size_t)m * LEAF);
     }
-    hipLaunchKernelGGL(getf2_coop<T>, dim3(G), dim3(PANEL_WG), shmem, ctx().s(), A, lda, m, w, rpw, ipiv, base, cand,
-                       rowj, bar, bar_next, status, cflag, cbackup, Linv);
-    check_launch("getf2_coop");
+    panel_leaf_serial<T>(A, lda, m, w, ipiv, base, cbackup, cflag, Linv);
 }
 
 // ------------------------------------------------------------------ row interchanges
@@ -4126,6 +4236,13 @@ int sqbem_tune(int mode) {
 }
 
 int sqbem_getrf_path(void) { return g_last_getrf_path; }
+
+int sqbem_coop(int enable) {
+    return guarded([&] {
+        SQ_REQUIRE(enable == 0 || enable == 1, SQBEM_ERR_ARG, "sqbem_coop: enable must be 0 or 1");
+        g_coop_ok = enable != 0;
+    });
+}
 
 int sqbem_tune_info(int* classes, int* decided, int* counts) {
     return guarded([&] { tune::info(classes, decided, counts); });

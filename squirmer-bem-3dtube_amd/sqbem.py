@@ -89,6 +89,7 @@ _SIGS = {
     "sqbem_batch_solve": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _d, _i, _d, _vp, _vp, _vp, _vp, _vp]),
     "sqbem_batch_options": (_i, [_i]),
     "sqbem_getrf_path": (_i, []),
+    "sqbem_coop": (_i, [_i]),
     "sqbem_velocity_at": (_i, [_vp, _vp, _vp, _d, _vp, _i, _vp]),
     "sqbem_duct_create": (_vp, [_vp, _d]),
     "sqbem_duct_solve": (_i, [_vp, _d, _d, _vp, _vp, _vp]),
@@ -688,6 +689,12 @@ def getrf_path():
     """How the most recent LU factorisation went: 0 partial pivoting throughout (orders below
     512), 1 without row interchanges (verified), 2 that attempt rejected, then pivoted."""
     return int(lib().sqbem_getrf_path())
+
+
+def coop(enable):
+    """1: the pivoted LU's panels use cooperative launches while the runtime accepts them
+    (the default); 0: they take the barrier-free path (a test switch)."""
+    check(lib().sqbem_coop(int(enable)))
 
 
 def batch_options(use_graph=True):

@@ -336,3 +336,42 @@ def test_sgetrs_fused_bottom_levels_all_panel_widths(sq, n, nrhs):
     Xg = host(dB).astype(np.float64)
     r = np.linalg.norm(A @ Xg - Bm, axis=0) / (np.linalg.norm(A, 2) * np.linalg.norm(Xg, axis=0))
     assert r.max() < 30 * n * 1.2e-7, r.max()
+
+
+@pytest.mark.parametrize("n", [5, 33, 65, 257, 1000, 1500])
+def test_pivoted_panels_without_cooperative_launch(sq, n):
+    """The pivoted LU's panels, made to take the barrier-free path (the one the library
+    uses once the runtime refuses a cooperative launch): LAPACK's pivot rows and factors,
+    FP64 and FP32, for random matrices (pivots throughout) and, at n >= 512, a column-
+    dominant matrix with two rows exchanged (the no-pivot attempt rejected first)."""
+    rng = np.random.default_rng(100 + n)
+    mats = [rng.standard_normal((n, n))]
+    if n >= 512:
+        B = rng.standard_normal((n, n)) + 2.0 * n ** 0.5 * np.eye(n)
+        B += np.diag(np.abs(B).sum(axis=0))
+        B[[300, 310]] = B[[310, 300]]
+        mats.append(B)
+    sq.coop(0)
+    try:
+        for A in mats:
+            lu, piv = sla.lu_factor(A)
+            for dt in (np.float64, np.float32):
+                dA = dev(A.astype(dt))
+                ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+                torch.cuda.synchronize()
+                info = sq.dgetrf(dA, ipiv) if dt == np.float64 else sq.sgetrf(1, dA, ipiv)
+                assert info == 0
+                if dt == np.float64:
+                    # FP32 factors of a random matrix may legitimately choose other pivots on
+                    # near-ties; FP64 ones must be LAPACK's
+                    assert np.array_equal(host(ipiv) - 1, piv), "pivot rows differ from LAPACK"
+                    assert np.allclose(host(dA), lu, rtol=1e-10, atol=1e-11 * np.abs(lu).max())
+                X = rng.standard_normal((n, 3))
+                dB = dev((A @ X).astype(dt))
+                (sq.dgetrs if dt == np.float64 else sq.sgetrs)(dA, ipiv, dB)
+                sq.sync()
+                Xg = host(dB).astype(np.float64)
+                r = np.linalg.norm(A @ Xg - A @ X, axis=0) / (np.linalg.norm(A, 2) * np.linalg.norm(Xg, axis=0))
+                assert r.max() < 30 * n * (2.2e-16 if dt == np.float64 else 1.2e-7), (dt, r.max())
+    finally:
+        sq.coop(1)
