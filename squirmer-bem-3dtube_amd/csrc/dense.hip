@@ -139,7 +139,8 @@ struct GemmK {
 template <typename T, int TN = GBN, int TM = GBM>
 __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const T* __restrict__ A, int lda,
                                              const T* __restrict__ B, int ldb, T beta, T* __restrict__ C, int ldc,
-                                             int tm, int tn, int* unit_flag = nullptr) {
+                                             int tm, int tn, int* unit_flag = nullptr, __bf16* sh_col = nullptr,
+                                             int sh_cld = 0, __bf16* sh_row = nullptr, int sh_rld = 0) {
     using Op = MfmaOp<T>;
     using Acc = typename Op::acc;
     static_assert(TM == 128 || TM == 64, "row tile of 64 or 128");
@@ -265,7 +266,16 @@ __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const
                 const T v = alpha * Ct[(ec + AS * q) * LDC + er];
                 const T out = beta == T(0) ? v : v + beta * cv[q];
                 C[row + (size_t)col * ldc] = out;
+                if (sh_col) sh_col[row + (size_t)col * sh_cld] = (__bf16)out;
                 if (unit_flag && !(fabs((double)out) <= 1.0)) bad = 1;
+            }
+        }
+        if (sh_row) {
+            // the row-major BF16 copy (beta = 0: the stored values are alpha times the tile),
+            // consecutive columns on consecutive lanes
+            for (int e = tid; e < TM * CW; e += 256) {
+                const int r = e / CW, c = e % CW, row = m0 + r, col = n0 + c0 + c;
+                if (row < m && col < n) sh_row[(size_t)row * sh_rld + col] = (__bf16)(alpha * Ct[c * LDC + r]);
             }
         }
     }
@@ -276,10 +286,12 @@ __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const
 template <typename T, int TN = GBN, int TM = GBM>
 __global__ void __launch_bounds__(256) gemm_nn_kernel(int m, int n, int k, T alpha, const T* __restrict__ A,
                                                       int lda, const T* __restrict__ B, int ldb, T beta,
-                                                      T* __restrict__ C, int ldc, int tiles_m, int* unit_flag) {
+                                                      T* __restrict__ C, int ldc, int tiles_m, int* unit_flag,
+                                                      __bf16* sh_col, int sh_cld, __bf16* sh_row, int sh_rld) {
     int tm, tn;
     tile_of(blockIdx.x, gridDim.x, tiles_m, tm, tn);
-    gemm_nn_tile<T, TN, TM>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, unit_flag);
+    gemm_nn_tile<T, TN, TM>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, unit_flag, sh_col, sh_cld, sh_row,
+                            sh_rld);
 }
 
 // Many independent GEMMs C_j = alpha_j A_j B_j + beta_j C_j in one launch: workgroup
@@ -962,10 +974,103 @@ __global__ void __launch_bounds__(256) convert16_kernel(int m, int n, int k, con
     *(v8*)(Bh + (size_t)j * kp + k0) = o;
 }
 
+// ------------------------------------------------------------------ BF16 shadows
+// Registered FP32 matrices with BF16 copies of their final values. A factor registration
+// holds the part below the 128 x 128 diagonal blocks row-major (lo) and the part above them
+// column-major (up): every 16-bit product of the factorisation and of its triangular solves
+// takes L blocks as the A operand (k-contiguous rows) and U blocks or solved right-hand
+// sides as the B operand (k-contiguous columns); only the U-side solves take U blocks as A
+// (column-major). A dense registration holds a whole matrix, column-major or row-major.
+namespace {
+struct ShadowMap {
+    const float* base;
+    int ld, rows, cols;
+    __bf16* lo;  // (i, j) at lo[i lds + j]
+    __bf16* up;  // (i, j) at up[i + j lds]
+    int lds;
+    bool factor;
+    int id;
+};
+std::vector<ShadowMap> g_shadows;
+int g_shadow_next = 1;
+constexpr int SHB = 128;  // diagonal block of a registered factor
+
+// the registration holding p (same leading dimension) and p's row and column in it
+const ShadowMap* shadow_at(const float* p, int ld, int rows, int cols, int& r0, int& c0) {
+    for (auto it = g_shadows.rbegin(); it != g_shadows.rend(); ++it) {
+        const ShadowMap& m = *it;
+        if (m.ld != ld || p < m.base) continue;
+        const size_t off = (size_t)(p - m.base), c = off / (size_t)ld, r = off % (size_t)ld;
+        if (c + (size_t)cols > (size_t)m.cols || r + (size_t)rows > (size_t)m.rows) continue;
+        r0 = (int)r;
+        c0 = (int)c;
+        return &m;
+    }
+    return nullptr;
+}
+
+// an operand's BF16 copy: trans = stored as its transpose (a row-major copy)
+struct Sh16 {
+    const __bf16* p;
+    int ld;
+    bool trans;
+};
+// the copy of the rows x cols operand at (p, ld), if registered and aligned for 16-byte loads
+bool shadow_operand(const float* p, int ld, int rows, int cols, Sh16& o) {
+    if (g_shadows.empty() || rows <= 0 || cols <= 0) return false;
+    int r0, c0;
+    const ShadowMap* m = shadow_at(p, ld, rows, cols, r0, c0);
+    if (!m) return false;
+    const bool lower = r0 / SHB > (c0 + cols - 1) / SHB, upper = c0 / SHB > (r0 + rows - 1) / SHB;
+    if (m->lo && (!m->factor || lower))
+        o = {m->lo + (size_t)r0 * m->lds + c0, m->lds, true};
+    else if (m->up && (!m->factor || upper))
+        o = {m->up + r0 + (size_t)c0 * m->lds, m->lds, false};
+    else
+        return false;
+    return (uintptr_t)o.p % 16 == 0 && o.ld % 8 == 0;
+}
+// where a kernel that finalises the rows x cols block at (p, ld) writes its copy:
+// column-major (the up part of a factor, or a dense column-major registration) or row-major
+// (the lo part of a factor); null when the block has none
+__bf16* shadow_out(const float* p, int ld, int rows, int cols, bool row_major, int* sld) {
+    if (g_shadows.empty() || rows <= 0 || cols <= 0) return nullptr;
+    int r0, c0;
+    const ShadowMap* m = shadow_at(p, ld, rows, cols, r0, c0);
+    if (!m) return nullptr;
+    *sld = m->lds;
+    if (row_major) {
+        const bool lower = r0 / SHB > (c0 + cols - 1) / SHB;
+        return m->lo && m->factor && lower ? m->lo + (size_t)r0 * m->lds + c0 : nullptr;
+    }
+    const bool upper = c0 / SHB > (r0 + rows - 1) / SHB;
+    return m->up && (!m->factor || upper) ? m->up + r0 + (size_t)c0 * m->lds : nullptr;
+}
+}  // namespace
+
+ShadowScope::ShadowScope(const float* base, int ld, int rows, int cols, __bf16* lo, __bf16* up, int lds, bool factor)
+    : id(0) {
+    if (!base || (!lo && !up)) return;
+    id = g_shadow_next++;
+    g_shadows.push_back(ShadowMap{base, ld, rows, cols, lo, up, lds, factor, id});
+}
+ShadowScope::~ShadowScope() {
+    if (!id) return;
+    for (auto it = g_shadows.begin(); it != g_shadows.end(); ++it)
+        if (it->id == id) {
+            g_shadows.erase(it);
+            break;
+        }
+}
+
 // ------------------------------------------------------------------ host GEMM wrappers
+// sh_col / sh_row (optional): BF16 copies of the stored C, column-major at leading
+// dimension sh_cld / row-major at sh_rld (beta = 0), addressed like C
 template <typename T>
 static void gemm_launch(int m, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb, T beta,
-                        T* C, int ldc, int* unit_flag = nullptr) {
+                        T* C, int ldc, int* unit_flag = nullptr, __bf16* sh_col = nullptr, int sh_cld = 0,
+                        __bf16* sh_row = nullptr, int sh_rld = 0) {
+    SQ_REQUIRE(!sh_row || beta == T(0), SQBEM_ERR_ARG, "gemm: a row-major copy needs beta = 0");
     if (m <= 0 || n <= 0) return;
     if (k <= 0 || alpha == T(0)) {
         // C = beta C
@@ -994,17 +1099,17 @@ static void gemm_launch(int m, int n, int k, T alpha, const T* A, int lda, const
         // 64 x 128 tiles: twice the workgroups, every one holding all columns of its rows
         const int tm64 = (int)ceil_div(m, 64);
         hipLaunchKernelGGL((gemm_nn_kernel<T, GBN, 64>), dim3(tm64), dim3(256), 0, ctx().s(), m, n, k, alpha, A, lda,
-                           B, ldb, beta, C, ldc, tm64, unit_flag);
+                           B, ldb, beta, C, ldc, tm64, unit_flag, sh_col, sh_cld, sh_row, sh_rld);
     } else if (!overlap && tiles128 < cus) {
         // 128 x 64 tiles (a one-row-tile product with many columns, e.g. the inverse-block
         // leaves of the triangular solves)
         const int tiles_n = (int)ceil_div(n, 64);
         hipLaunchKernelGGL((gemm_nn_kernel<T, 64>), dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, k, alpha,
-                           A, lda, B, ldb, beta, C, ldc, tiles_m, unit_flag);
+                           A, lda, B, ldb, beta, C, ldc, tiles_m, unit_flag, sh_col, sh_cld, sh_row, sh_rld);
     } else {
         const int tiles_n = (int)ceil_div(n, GBN);
         hipLaunchKernelGGL((gemm_nn_kernel<T, GBN>), dim3(tiles_m * tiles_n), dim3(256), 0, ctx().s(), m, n, k, alpha,
-                           A, lda, B, ldb, beta, C, ldc, tiles_m, unit_flag);
+                           A, lda, B, ldb, beta, C, ldc, tiles_m, unit_flag, sh_col, sh_cld, sh_row, sh_rld);
     }
     check_launch("gemm");
 }
@@ -1301,14 +1406,16 @@ struct Plan {
 };
 static hipblasLtHandle_t g_handle = nullptr;
 static bool g_broken = false;  // the library could not be set up: the candidate falls back
-static std::map<std::array<long long, 6>, Plan> g_plans;
+static std::map<std::array<long long, 9>, Plan> g_plans;
 static DBuf<char> g_ws;
 constexpr size_t kWorkspace = size_t(64) << 20;
 
 static bool good(hipblasStatus_t st) { return st == HIPBLAS_STATUS_SUCCESS; }
 
-// the plan of D(m x n, ldc) = alpha A^T B + beta C with A^T, B k-deep at leading dimension kp
-static const Plan* plan(int m, int n, int k, int kp, int ldc, bool beta0) {
+// the plan of D(m x n, ldc) = alpha op(A) op(B) + beta C, BF16 A and B: ta = A stored as
+// its transpose (k x m at lda), else m x k; tb = B stored as its transpose (n x k at ldb),
+// else k x n
+static const Plan* plan(int m, int n, int k, bool ta, int lda, bool tb, int ldb, int ldc, bool beta0) {
     if (g_broken) return nullptr;
     if (!g_handle) {
         if (!good(hipblasLtCreate(&g_handle))) {
@@ -1317,16 +1424,16 @@ static const Plan* plan(int m, int n, int k, int kp, int ldc, bool beta0) {
         }
         g_ws.alloc(kWorkspace);
     }
-    const std::array<long long, 6> key{m, n, k, kp, ldc, beta0 ? 1 : 0};
+    const std::array<long long, 9> key{m, n, k, ta ? 1 : 0, lda, tb ? 1 : 0, ldb, ldc, beta0 ? 1 : 0};
     auto it = g_plans.find(key);
     if (it != g_plans.end()) return it->second.ok ? &it->second : nullptr;
     Plan& P = g_plans[key];
-    const hipblasOperation_t tA = HIPBLAS_OP_T, tB = HIPBLAS_OP_N;
+    const hipblasOperation_t tA = ta ? HIPBLAS_OP_T : HIPBLAS_OP_N, tB = tb ? HIPBLAS_OP_T : HIPBLAS_OP_N;
     bool ok = good(hipblasLtMatmulDescCreate(&P.desc, HIPBLAS_COMPUTE_32F, HIP_R_32F)) &&
               good(hipblasLtMatmulDescSetAttribute(P.desc, HIPBLASLT_MATMUL_DESC_TRANSA, &tA, sizeof tA)) &&
               good(hipblasLtMatmulDescSetAttribute(P.desc, HIPBLASLT_MATMUL_DESC_TRANSB, &tB, sizeof tB)) &&
-              good(hipblasLtMatrixLayoutCreate(&P.a, HIP_R_16BF, k, m, kp)) &&
-              good(hipblasLtMatrixLayoutCreate(&P.b, HIP_R_16BF, k, n, kp)) &&
+              good(hipblasLtMatrixLayoutCreate(&P.a, HIP_R_16BF, ta ? k : m, ta ? m : k, lda)) &&
+              good(hipblasLtMatrixLayoutCreate(&P.b, HIP_R_16BF, tb ? n : k, tb ? k : n, ldb)) &&
               good(hipblasLtMatrixLayoutCreate(&P.c, HIP_R_32F, m, n, ldc));
     if (ok) {
         hipblasLtMatmulPreference_t pref = nullptr;
@@ -1360,7 +1467,7 @@ static const Plan* plan(int m, int n, int k, int kp, int ldc, bool beta0) {
 static bool gemm16lt(int m, int n, int k, float alpha, const float* A, int lda, const float* B, int ldb, float beta,
                      float* C, int ldc, int which) {
     const int mp = (m + 63) / 64 * 64, np = (n + 63) / 64 * 64, kp = (k + QBK - 1) / QBK * QBK;
-    const lt::Plan* P = lt::plan(m, n, k, kp, ldc, beta == 0.0f);
+    const lt::Plan* P = lt::plan(m, n, k, true, kp, false, kp, ldc, beta == 0.0f);
     if (!P) return false;
     const size_t need = ((size_t)mp + np) * kp * sizeof(__bf16);
     if (g_q16.n < need) g_q16.alloc(need);
@@ -1381,6 +1488,35 @@ static bool gemm16lt(int m, int n, int k, float alpha, const float* A, int lda, 
                                                P->c, &P->algo[a], lt::g_ws.p, lt::kWorkspace, ctx().s());
     SQ_REQUIRE(st == HIPBLAS_STATUS_SUCCESS, SQBEM_ERR_HIP, "gemm16: hipblasLtMatmul failed");
     return true;
+}
+
+// the library GEMM straight on registered BF16 copies of both operands (no conversion);
+// false when the library has no plan for the shape
+static bool gemm16lt_shadow(int m, int n, int k, float alpha, const Sh16& a, const Sh16& b, float beta, float* C,
+                            int ldc, int which) {
+    const lt::Plan* P = lt::plan(m, n, k, a.trans, a.ld, b.trans, b.ld, ldc, beta == 0.0f);
+    if (!P) return false;
+    const double fl = 2.0 * m * n * k, by = 2.0 * ((double)m * k + (double)k * n) + 4.0 * (beta != 0.0f ? 2.0 : 1.0) * m * n;
+    const int al = which < P->nalgo ? which : 0;
+    ProfScope prof(PROF_GEMM16Q, fl, by);
+    ProfScope prof_tile(PROF_G16Q_C0 + 10 + al, fl, by, P->kernel[al].c_str());
+    const hipblasStatus_t st = hipblasLtMatmul(lt::g_handle, P->desc, &alpha, a.p, P->a, b.p, P->b, &beta, C, P->c, C,
+                                               P->c, &P->algo[al], lt::g_ws.p, lt::kWorkspace, ctx().s());
+    SQ_REQUIRE(st == HIPBLAS_STATUS_SUCCESS, SQBEM_ERR_HIP, "gemm16: hipblasLtMatmul failed");
+    return true;
+}
+
+int shadow_rows_ld(int k) { return (k + 63) / 64 * 64; }
+size_t shadow_rows_elems(int m, int k) { return (size_t)(m + 63) / 64 * 64 * (size_t)shadow_rows_ld(k); }
+// the A operand conversion of the 16-bit products alone: rows of A, k-contiguous, BF16
+void shadow_rows(int m, int k, const float* A, int lda, __bf16* out) {
+    if (m <= 0 || k <= 0) return;
+    const int kp = shadow_rows_ld(k), mp = (m + 63) / 64 * 64;
+    const size_t nrb = (size_t)(mp / 64) * (kp / 64);
+    SQ_REQUIRE(nrb < ((size_t)1 << 31), SQBEM_ERR_ARG, "shadow_rows: matrix too large");
+    hipLaunchKernelGGL(convert16_kernel<true>, dim3((unsigned)nrb), dim3(256), 0, ctx().s(), m, 0, k, A, lda, nullptr,
+                       nullptr, 0, nullptr, out, nullptr, kp, 0, (int)nrb, 0);
+    check_launch("shadow_rows");
 }
 
 // FP16 operand scales (powers of two per row of A and per column of B) into sp[0, 2m + n)
@@ -1406,6 +1542,15 @@ static void gemm16_impl(int kind, int m, int n, int k, float alpha, const float*
     }
     const bool bf = kind == SQBEM_BF16_IR;
     const bool big = m >= 256 && n >= 256 && k >= 128;
+    if (bf && !w.ws) {
+        // both operands have BF16 copies: the library GEMM reads them as they are (the
+        // algorithm the autotuner chose for the class when it chose the library)
+        Sh16 sa, sb;
+        if (shadow_operand(A, lda, m, k, sa) && shadow_operand(B, ldb, k, n, sb)) {
+            const int cand = big ? tune::decided(1, m, n, k, -1) : -1;
+            if (gemm16lt_shadow(m, n, k, alpha, sa, sb, beta, C, ldc, cand >= 10 ? cand - 10 : 0)) return;
+        }
+    }
     float *rs = nullptr, *cs = nullptr;
     if (!bf) {
         float* sp;
@@ -2380,7 +2525,8 @@ __global__ void __launch_bounds__(256) trsm2_kernel(int n, int nrhs, const T* L,
 // results equal trsm2_kernel's.
 template <bool LOWER, int TN>
 __global__ void __launch_bounds__(256, TN <= 32 ? 2 : 1) trsm2p_kernel(int n, int nrhs, const float* L, int ldl,
-                                                                       const float* inv, float* B, int ldb) {
+                                                                       const float* inv, float* B, int ldb,
+                                                                       __bf16* sh, int shld) {
     using Op = MfmaOp<float>;
     using Acc = typename Op::acc;
     constexpr int KB = GemmK<float>::KB, PAD = GemmK<float>::PAD, NQ = KB / 2, S = KB / 4;
@@ -2497,7 +2643,11 @@ __global__ void __launch_bounds__(256, TN <= 32 ? 2 : 1) trsm2p_kernel(int n, in
     if (tid < n) {
 #pragma unroll
         for (int c = 0; c < TN; ++c)
-            if (n0 + c < nrhs) B[tid + (size_t)(n0 + c) * ldb] = Pn[c * NP + tid];
+            if (n0 + c < nrhs) {
+                const float v = Pn[c * NP + tid];
+                B[tid + (size_t)(n0 + c) * ldb] = v;
+                if (sh) sh[tid + (size_t)(n0 + c) * shld] = (__bf16)v;  // BF16 copy (column-major)
+            }
     }
 }
 
@@ -2507,9 +2657,18 @@ __global__ void __launch_bounds__(256, TN <= 32 ? 2 : 1) trsm2p_kernel(int n, in
 template <typename T, bool LOWER, int BS = IB>
 static void trsm_inv(int kind, int n, int nrhs, const T* L, int ldl, const T* inv, T* B, int ldb) {
     if (n <= 0 || nrhs <= 0) return;
+    // a leaf's results are final rows of X: their BF16 copy, where B's matrix has one
+    int shld = 0;
+    __bf16* sh = nullptr;
+    if constexpr (sizeof(T) == 4) {
+        if (n <= 2 * BS) sh = shadow_out((const float*)B, ldb, n, nrhs, false, &shld);
+    }
     if (n <= BS) {
         ProfScope prof(PROF_TRSM, 2.0 * n * n * nrhs, sizeof(T) * (2.0 * n * nrhs + (double)n * n));
-        gemm_exact<T>(n, nrhs, n, T(1), inv, BS, B, ldb, T(0), B, ldb);
+        if (sh)
+            gemm_launch<T>(n, nrhs, n, T(1), inv, BS, B, ldb, T(0), B, ldb, nullptr, sh, shld);
+        else
+            gemm_exact<T>(n, nrhs, n, T(1), inv, BS, B, ldb, T(0), B, ldb);
         return;
     }
     if constexpr (BS == TRSM2_B) {
@@ -2522,13 +2681,13 @@ static void trsm_inv(int kind, int n, int nrhs, const T* L, int ldl, const T* in
                 const int cus = std::max(ctx().cu_count, 1);
                 if (ceil_div(nrhs, 32) < cus / 2)
                     hipLaunchKernelGGL((trsm2p_kernel<LOWER, 16>), dim3((unsigned)ceil_div(nrhs, 16)), dim3(256), 0,
-                                       ctx().s(), n, nrhs, L, ldl, inv, B, ldb);
+                                       ctx().s(), n, nrhs, L, ldl, inv, B, ldb, sh, shld);
                 else if (ceil_div(nrhs, TRSM2_TN) < cus)
                     hipLaunchKernelGGL((trsm2p_kernel<LOWER, 32>), dim3((unsigned)ceil_div(nrhs, 32)), dim3(256), 0,
-                                       ctx().s(), n, nrhs, L, ldl, inv, B, ldb);
+                                       ctx().s(), n, nrhs, L, ldl, inv, B, ldb, sh, shld);
                 else
                     hipLaunchKernelGGL((trsm2p_kernel<LOWER, TRSM2_TN>), dim3((unsigned)ceil_div(nrhs, TRSM2_TN)),
-                                       dim3(256), 0, ctx().s(), n, nrhs, L, ldl, inv, B, ldb);
+                                       dim3(256), 0, ctx().s(), n, nrhs, L, ldl, inv, B, ldb, sh, shld);
             }
             else
                 hipLaunchKernelGGL((trsm2_kernel<T, LOWER>), dim3((unsigned)ceil_div(nrhs, TRSM2_TN)), dim3(256), 0,
@@ -3053,8 +3212,13 @@ static void getrf_np(int kind, T* A, int lda, int m, int w, int base, T* Linv, T
         }
         if (m > w) {
             // L21 = A21 U11^-1, in place (one column tile, each row tile read before written),
-            // with every multiplier checked against |l| <= 1 as it is stored
-            gemm_launch<T>(m - w, w, w, T(1), A + w, lda, Uinv + off, NB, T(0), A + w, lda, flag);
+            // with every multiplier checked against |l| <= 1 as it is stored (and its BF16
+            // copy written row-major where the factor has one)
+            int sld = 0;
+            __bf16* sr = nullptr;
+            if constexpr (sizeof(T) == 4) sr = shadow_out((const float*)(A + w), lda, m - w, w, true, &sld);
+            gemm_launch<T>(m - w, w, w, T(1), A + w, lda, Uinv + off, NB, T(0), A + w, lda, flag, nullptr, 0, sr,
+                           sld);
         }
         return;
     }
@@ -3502,21 +3666,33 @@ static void trsv_inv(int n, int nrhs, const T* Lt, const T* Dinv, const T* Mk, T
 template <typename T>
 __global__ void __launch_bounds__(256) tile_copy_kernel(int n, const T* __restrict__ A, int lda, T* __restrict__ At) {
     const int nb = (n + NB - 1) / NB;
-    const int I = blockIdx.x % nb, J = blockIdx.x / nb;
-    T* dst = At + ((size_t)I * nb + J) * NB * NB;
     const int r = threadIdx.x & 127, c1 = threadIdx.x >> 7;
-    const int gi = I * NB + r;
-    for (int c = c1; c < NB; c += 2) {
-        const int gj = J * NB + c;
-        dst[(size_t)c * NB + r] = (gi < n && gj < n) ? A[gi + (size_t)gj * lda] : T(0);
+    for (int t = blockIdx.x; t < nb * nb; t += gridDim.x) {
+        const int I = t % nb, J = t / nb;
+        T* dst = At + ((size_t)I * nb + J) * NB * NB;
+        const int gi = I * NB + r;
+        for (int c0 = c1; c0 < NB; c0 += 32) {  // 16 loads in flight, then their stores
+            T v[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int gj = J * NB + c0 + 2 * q;
+                v[q] = (gi < n && gj < n) ? A[gi + (size_t)gj * lda] : T(0);
+            }
+#pragma unroll
+            for (int q = 0; q < 16; ++q) dst[(size_t)(c0 + 2 * q) * NB + r] = v[q];
+        }
     }
 }
 
+// On the side stream the copy runs beside the library stream's work with one workgroup on
+// at most half the CUs, so the kernels queued there keep getting CUs (a workgroup per tile
+// would fill the device and hold them off for the whole copy).
 template <typename T>
 static void tile_copy(int n, const T* A, int lda, T* At, hipStream_t stream = nullptr) {
     const int nb = (n + NB - 1) / NB;
-    hipLaunchKernelGGL(tile_copy_kernel<T>, dim3((unsigned)nb * nb), dim3(256), 0, stream ? stream : ctx().s(), n, A,
-                       lda, At);
+    const unsigned tiles = (unsigned)nb * nb;
+    const unsigned grid = stream ? std::min(tiles, (unsigned)std::max(ctx().cu_count / 2, 1)) : tiles;
+    hipLaunchKernelGGL(tile_copy_kernel<T>, dim3(grid), dim3(256), 0, stream ? stream : ctx().s(), n, A, lda, At);
     check_launch("tile_copy");
 }
 
@@ -3641,6 +3817,13 @@ void getrf_begin(GetrfJob<T>& j) {
         j.inv.alloc(2 * nb * NB * NB);
         Linv = j.inv.p;
     }
+    // BF16 copies of the factors, written as the no-pivot recursion finalises them and read
+    // by its 16-bit products (registered while its launches are issued)
+    std::unique_ptr<ShadowScope> sh;
+    if constexpr (sizeof(T) == 4) {
+        if (j.kind == SQBEM_BF16_IR && j.sh_lo && j.sh_up)
+            sh.reset(new ShadowScope((const float*)j.A, j.lda, n, n, j.sh_lo, j.sh_up, j.sh_ld ? j.sh_ld : n, true));
+    }
     getrf_np<T>(j.kind, j.A, j.lda, n, n, 0, Linv, Linv + nb * NB * NB, j.flag.p);
 }
 
@@ -3652,6 +3835,7 @@ int getrf_end(GetrfJob<T>& j) {
     T* A = j.A;
     T* aux = j.aux;
     int* ipiv = j.ipiv;
+    j.path = 0;
     if (n <= 0) return 0;
     g_last_getrf_path = 0;
     const size_t nb = (size_t)(n + NB - 1) / NB;
@@ -3670,6 +3854,7 @@ int getrf_end(GetrfJob<T>& j) {
             }
             j.backup.release();  // stream-ordered reuse only
             g_last_getrf_path = 1;
+            j.path = 1;
             return 0;
         }
         if (j.restore)
@@ -3678,6 +3863,7 @@ int getrf_end(GetrfJob<T>& j) {
             SQ_HIP(hipMemcpy2DAsync(A, (size_t)lda * sizeof(T), j.backup.p, (size_t)n * sizeof(T),
                                     (size_t)n * sizeof(T), n, hipMemcpyDeviceToDevice, ctx().s()));
         g_last_getrf_path = 2;
+        j.path = 2;
     }
     g_panel.ensure(256, n, sizeof(T));
     g_panel.bar.zero();
@@ -3698,7 +3884,8 @@ int getrf_end(GetrfJob<T>& j) {
 }
 
 template <typename T>
-int getrf(int kind, int n, T* A, int lda, int* ipiv, T* aux, const std::function<void()>* restore) {
+int getrf(int kind, int n, T* A, int lda, int* ipiv, T* aux, const std::function<void()>* restore, __bf16* sh_lo,
+          __bf16* sh_up, int sh_ld) {
     GetrfJob<T> j;
     j.kind = kind;
     j.n = n;
@@ -3706,12 +3893,16 @@ int getrf(int kind, int n, T* A, int lda, int* ipiv, T* aux, const std::function
     j.A = A;
     j.ipiv = ipiv;
     j.aux = aux;
+    j.sh_lo = sh_lo;
+    j.sh_up = sh_up;
+    j.sh_ld = sh_ld;
     if (restore) j.restore = *restore;
     getrf_begin<T>(j);
     return getrf_end<T>(j);
 }
-template int getrf<double>(int, int, double*, int, int*, double*, const std::function<void()>*);
-template int getrf<float>(int, int, float*, int, int*, float*, const std::function<void()>*);
+template int getrf<double>(int, int, double*, int, int*, double*, const std::function<void()>*, __bf16*, __bf16*,
+                           int);
+template int getrf<float>(int, int, float*, int, int*, float*, const std::function<void()>*, __bf16*, __bf16*, int);
 template void getrf_begin<double>(GetrfJob<double>&);
 template void getrf_begin<float>(GetrfJob<float>&);
 template int getrf_end<double>(GetrfJob<double>&);

@@ -33,12 +33,19 @@ void convert_matrix(int m, int n, const Ti* A, int lda, To* B, int ldb);
 // instead when it has to fall back to partial pivoting.
 template <typename T>
 int getrf(int kind, int n, T* A, int lda, int* ipiv, T* aux = nullptr,
-          const std::function<void()>* restore = nullptr);
+          const std::function<void()>* restore = nullptr, __bf16* sh_lo = nullptr, __bf16* sh_up = nullptr,
+          int sh_ld = 0);
 // The same in two halves (getrf = begin, then end): begin issues the factorisation without
 // row interchanges (orders of 512 and more) and returns without waiting; end waits for its
 // verdict, finishes, or refactors with partial pivoting, and returns the LAPACK info. In
 // between the stream may carry other work that leaves A, ipiv and aux alone. The job owns
 // the verdict word and, without restore, the copy of A (and, without aux, the inverses).
+// sh_lo, sh_up (optional, BF16 kind, n x sh_ld each, sh_ld >= n, 0 = n): receive BF16
+// copies of the factors as the no-pivot factorisation finalises them (sh_lo: the part below
+// the 128 x 128 diagonal blocks, row-major, (i, j) at sh_lo[i sh_ld + j]; sh_up: the part
+// above them, column-major), and its 16-bit products read their operands from there instead
+// of converting them. path: 1 when the factors came from the verified no-pivot
+// factorisation (the copies are then whole), 2 when it fell back to pivoting.
 template <typename T>
 struct GetrfJob {
     int kind = 0, n = 0, lda = 0;
@@ -46,6 +53,10 @@ struct GetrfJob {
     int* ipiv = nullptr;
     T* aux = nullptr;
     std::function<void()> restore;
+    __bf16* sh_lo = nullptr;
+    __bf16* sh_up = nullptr;
+    int sh_ld = 0;
+    int path = 0;
     DBuf<int> flag;
     DBuf<T> backup, inv;
     bool started = false;
@@ -58,6 +69,26 @@ inline size_t getrf_aux_elems(int n) {
     const size_t nb = (size_t)((n + 127) / 128);
     return (4 * nb + nb * nb) * 128 * 128;
 }
+// ---- BF16 copies of final values ("shadows") for the 16-bit products (BF16 kind)
+// While a scope is alive, a 16-bit product whose operands lie in a registered FP32 matrix
+// reads them from its BF16 copy (no conversion pass), and the kernels that finalise values
+// of the matrix (the L21 products and the triangular-solve leaves) write the copy as well.
+// A factor registration covers the parts below (lo, row-major) and above (up,
+// column-major) the 128 x 128 diagonal blocks; a dense one the whole matrix, column-major
+// (up) or row-major (lo). The caller registers only copies whose values are, or will be
+// made, final before any product reads them.
+struct ShadowScope {
+    ShadowScope(const float* base, int ld, int rows, int cols, __bf16* lo, __bf16* up, int lds, bool factor);
+    ~ShadowScope();
+    ShadowScope(const ShadowScope&) = delete;
+    ShadowScope& operator=(const ShadowScope&) = delete;
+    int id;
+};
+// row-major BF16 copy of an m x k FP32 matrix (leading dimension shadow_rows_ld(k))
+size_t shadow_rows_elems(int m, int k);
+int shadow_rows_ld(int k);
+void shadow_rows(int m, int k, const float* A, int lda, __bf16* out);
+
 // B <- A^-1 B (trans = 0) or A^-T B (trans = 1)
 // aux (optional): getrf's solve aid for these factors; formed here when absent
 template <typename T>

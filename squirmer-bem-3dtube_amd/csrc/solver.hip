@@ -364,6 +364,16 @@ void sq::tube_build(sqbem_tube* t) {
         j.aux = t->aux32.p;
         // the pivoting fallback rebuilds the FP32 matrix from the FP64 one (no copy is kept)
         j.restore = [t, nt] { convert_matrix<double, float>(nt, nt, t->T64.p, nt, t->T32.p, nt); };
+        if (t->prec == SQBEM_BF16_IR && nt >= 512) {
+            // BF16 copies of the factors for the 16-bit products of the factorisation and of
+            // the Schur complement's solves
+            t->sh_ld = (nt + 63) / 64 * 64;
+            t->SL.alloc((size_t)nt * t->sh_ld);
+            t->SU.alloc((size_t)nt * t->sh_ld);
+            j.sh_lo = t->SL.p;
+            j.sh_up = t->SU.p;
+            j.sh_ld = t->sh_ld;
+        }
         getrf_begin<float>(j);
     } else {
         t->aux64.alloc(getrf_aux_elems(nt));
@@ -396,6 +406,7 @@ void sq::tube_ready(const sqbem_tube* tc) {
     int info = 0;
     try {
         info = t->job32 ? getrf_end<float>(*t->job32) : getrf_end<double>(*t->job64);
+        t->np_path = t->job32 ? t->job32->path : t->job64->path;
     } catch (...) {
         t->job32.reset();
         t->job64.reset();
@@ -456,6 +467,8 @@ struct sqbem_swimmer {
     DBuf<double> AuBt, AuBs, D, Y, G;  // Y = T^-1 [B | Ct] (FP64 mode); G = [Gs | Gc]
     DBuf<double> B, Ct, E, Cs;         // mixed: kept in FP64 for the refinement residual
     DBuf<float> Y32, G32, D32, M32;    // mixed: low-precision Schur pieces and bordered factors
+    DBuf<__bf16> D16;                  // BF16 mode: D row-major in BF16 (the Schur product's A)
+    int D16ld = 0;
     DBuf<int> Mpiv;
     DBuf<float> Maux;                  // getrf's solve aid for the M32 factors
     DBuf<float> Gs32;                  // mixed, towed problem: factors of Gs alone
@@ -558,6 +571,11 @@ static void swimmer_build(sqbem_swimmer* s) {
             convert_matrix<double, float>(nt, 6, s->Ct.p, nt, s->Y32.p + (size_t)nt * ns, nt);
             s->D32.alloc((size_t)ns * nt);
             convert_matrix<double, float>(ns, nt, s->D.p, ns, s->D32.p, ns);
+            if (s->prec == SQBEM_BF16_IR) {
+                s->D16ld = shadow_rows_ld(nt);
+                s->D16.alloc(shadow_rows_elems(ns, nt));
+                shadow_rows(ns, nt, s->D32.p, ns, s->D16.p);
+            }
         }
     }
     sync();
@@ -575,11 +593,23 @@ static void swimmer_build(sqbem_swimmer* s) {
     } else {
         const int kind = s->prec;
         if (tb) {
-            getrs<float>(kind, 0, nt, ns + 6, tb->T32.p, nt, tb->ipiv.p, s->Y32.p, nt, tb->aux32.p);
-            if (kind == SQBEM_FP32_IR)
-                sgemm(ns, ns + 6, nt, -1.0f, s->D32.p, ns, s->Y32.p, nt, 1.0f, s->G32.p, ns);
-            else
-                gemm16(kind, ns, ns + 6, nt, -1.0f, s->D32.p, ns, s->Y32.p, nt, 1.0f, s->G32.p, ns);
+            // BF16: the 16-bit products of the solves and of D Y read BF16 copies of their
+            // operands (the tube's factors, the solved right-hand sides as the solves'
+            // leaves finalise them, D) instead of converting them
+            const bool sh = kind == SQBEM_BF16_IR && tb->np_path == 1 && tb->SL.p;
+            DBuf<__bf16> Yh;
+            const int yld = (nt + 63) / 64 * 64;
+            if (sh) Yh.alloc((size_t)yld * (ns + 6));
+            {
+                ShadowScope f(sh ? tb->T32.p : nullptr, nt, nt, nt, tb->SL.p, tb->SU.p, tb->sh_ld, true);
+                ShadowScope y(sh ? s->Y32.p : nullptr, nt, nt, ns + 6, nullptr, Yh.p, yld, false);
+                ShadowScope d(sh ? s->D32.p : nullptr, ns, ns, nt, s->D16.p, nullptr, s->D16ld, false);
+                getrs<float>(kind, 0, nt, ns + 6, tb->T32.p, nt, tb->ipiv.p, s->Y32.p, nt, tb->aux32.p);
+                if (kind == SQBEM_FP32_IR)
+                    sgemm(ns, ns + 6, nt, -1.0f, s->D32.p, ns, s->Y32.p, nt, 1.0f, s->G32.p, ns);
+                else
+                    gemm16(kind, ns, ns + 6, nt, -1.0f, s->D32.p, ns, s->Y32.p, nt, 1.0f, s->G32.p, ns);
+            }
         }
         // bordered low-precision factors, reused by every refinement step
         const int n = ns + 6;
@@ -594,7 +624,11 @@ static void swimmer_build(sqbem_swimmer* s) {
         border();
         s->Mpiv.alloc(n);
         s->Maux.alloc(getrf_aux_elems(n));
-        int info = getrf<float>(kind, n, s->M32.p, n, s->Mpiv.p, s->Maux.p, &border);
+        DBuf<__bf16> Msh;  // BF16 copies of the bordered factors for its 16-bit products
+        const int mld = (n + 63) / 64 * 64;
+        if (kind == SQBEM_BF16_IR && n >= 512) Msh.alloc((size_t)2 * n * mld);
+        int info = getrf<float>(kind, n, s->M32.p, n, s->Mpiv.p, s->Maux.p, &border, Msh.p,
+                                Msh.p ? Msh.p + (size_t)n * mld : nullptr, mld);
         SQ_REQUIRE(info == 0, SQBEM_ERR_SINGULAR, "dgesv failed");
     }
     sync();
