@@ -1130,9 +1130,12 @@ struct ScaleWork {
         return buf.p;
     }
 };
-static ScaleWork g_scale;
-
-static DBuf<char> g_q16;
+// the 16-bit products' working buffers, one set per stream: work issued on the side stream
+// runs beside the library stream's products
+static ScaleWork g_scale_main, g_scale_side;
+static DBuf<char> g_q16_main, g_q16_side;
+static ScaleWork& scale_work() { return ctx().redirect ? g_scale_side : g_scale_main; }
+static DBuf<char>& q16_work() { return ctx().redirect ? g_q16_side : g_q16_main; }
 
 // ------------------------------------------------------------------ tile autotuner
 // The 16-bit GEMM on converted operands comes in four workgroup tiles (128 or 256 rows by
@@ -1336,8 +1339,9 @@ static void gemm16q(int m, int n, int k, float alpha, const float* A, int lda, c
     if (w.ws) {
         Ah = (ht*)w.ws;  // gemm16_work_bytes reserved the 256-padded operands at its start
     } else {
-        if (g_q16.n < need) g_q16.alloc(need);
-        Ah = (ht*)g_q16.p;
+        DBuf<char>& q16 = q16_work();
+        if (q16.n < need) q16.alloc(need);
+        Ah = (ht*)q16.p;
     }
     ht* Bh = Ah + (size_t)mp * kp;
     const size_t nrb = (size_t)(mp / 64) * (kp / 64), chunks = (size_t)np * kp / 8;
@@ -1369,8 +1373,9 @@ static void gemm16d(int m, int n, int k, float alpha, const float* A, int lda, c
     if (w.ws) {
         Ah = (ht*)w.ws;
     } else {
-        if (g_q16.n < need) g_q16.alloc(need);
-        Ah = (ht*)g_q16.p;
+        DBuf<char>& q16 = q16_work();
+        if (q16.n < need) q16.alloc(need);
+        Ah = (ht*)q16.p;
     }
     ht* Bh = Ah + (size_t)mp * kp;
     const size_t nrb = (size_t)(mp / 64) * (kp / 64), chunks = (size_t)np * kp / 8;
@@ -1407,8 +1412,13 @@ struct Plan {
 static hipblasLtHandle_t g_handle = nullptr;
 static bool g_broken = false;  // the library could not be set up: the candidate falls back
 static std::map<std::array<long long, 9>, Plan> g_plans;
-static DBuf<char> g_ws;
+static DBuf<char> g_ws_main, g_ws_side;  // the library's workspace, one per stream
 constexpr size_t kWorkspace = size_t(64) << 20;
+static void* ws() {
+    DBuf<char>& w = ctx().redirect ? g_ws_side : g_ws_main;
+    if (!w.p) w.alloc(kWorkspace);
+    return w.p;
+}
 
 static bool good(hipblasStatus_t st) { return st == HIPBLAS_STATUS_SUCCESS; }
 
@@ -1422,7 +1432,6 @@ static const Plan* plan(int m, int n, int k, bool ta, int lda, bool tb, int ldb,
             g_broken = true;
             return nullptr;
         }
-        g_ws.alloc(kWorkspace);
     }
     const std::array<long long, 9> key{m, n, k, ta ? 1 : 0, lda, tb ? 1 : 0, ldb, ldc, beta0 ? 1 : 0};
     auto it = g_plans.find(key);
@@ -1470,8 +1479,9 @@ static bool gemm16lt(int m, int n, int k, float alpha, const float* A, int lda, 
     const lt::Plan* P = lt::plan(m, n, k, true, kp, false, kp, ldc, beta == 0.0f);
     if (!P) return false;
     const size_t need = ((size_t)mp + np) * kp * sizeof(__bf16);
-    if (g_q16.n < need) g_q16.alloc(need);
-    __bf16* Ah = (__bf16*)g_q16.p;
+    DBuf<char>& q16 = q16_work();
+    if (q16.n < need) q16.alloc(need);
+    __bf16* Ah = (__bf16*)q16.p;
     __bf16* Bh = Ah + (size_t)mp * kp;
     const size_t nrb = (size_t)(mp / 64) * (kp / 64), chunks = (size_t)np * kp / 8;
     if (chunks >= ((size_t)1 << 32) || nrb + ceil_div(chunks, 256) >= ((size_t)1 << 31))
@@ -1485,7 +1495,7 @@ static bool gemm16lt(int m, int n, int k, float alpha, const float* A, int lda, 
     ProfScope prof(PROF_GEMM16Q, fl, by);
     ProfScope prof_tile(PROF_G16Q_C0 + 10 + a, fl, by, P->kernel[a].c_str());
     const hipblasStatus_t st = hipblasLtMatmul(lt::g_handle, P->desc, &alpha, Ah, P->a, Bh, P->b, &beta, C, P->c, C,
-                                               P->c, &P->algo[a], lt::g_ws.p, lt::kWorkspace, ctx().s());
+                                               P->c, &P->algo[a], lt::ws(), lt::kWorkspace, ctx().s());
     SQ_REQUIRE(st == HIPBLAS_STATUS_SUCCESS, SQBEM_ERR_HIP, "gemm16: hipblasLtMatmul failed");
     return true;
 }
@@ -1501,7 +1511,7 @@ static bool gemm16lt_shadow(int m, int n, int k, float alpha, const Sh16& a, con
     ProfScope prof(PROF_GEMM16Q, fl, by);
     ProfScope prof_tile(PROF_G16Q_C0 + 10 + al, fl, by, P->kernel[al].c_str());
     const hipblasStatus_t st = hipblasLtMatmul(lt::g_handle, P->desc, &alpha, a.p, P->a, b.p, P->b, &beta, C, P->c, C,
-                                               P->c, &P->algo[al], lt::g_ws.p, lt::kWorkspace, ctx().s());
+                                               P->c, &P->algo[al], lt::ws(), lt::kWorkspace, ctx().s());
     SQ_REQUIRE(st == HIPBLAS_STATUS_SUCCESS, SQBEM_ERR_HIP, "gemm16: hipblasLtMatmul failed");
     return true;
 }
@@ -1542,7 +1552,15 @@ static void gemm16_impl(int kind, int m, int n, int k, float alpha, const float*
     }
     const bool bf = kind == SQBEM_BF16_IR;
     const bool big = m >= 256 && n >= 256 && k >= 128;
-    if (bf && !w.ws) {
+    // The library's GEMM kernels are stream-K kernels: one persistent workgroup per CU (or
+    // two), some of which wait for others' partial tiles, so they assume the whole grid is
+    // resident. Two of them running at once on different streams can each hold CUs that the
+    // other's waiting workgroups need. Library kernels therefore run on the library stream
+    // only; products issued on the side stream use this file's kernels, whose workgroups
+    // never wait on one another (beside those, a stream-K kernel's missing workgroups start
+    // as soon as CUs free up).
+    const bool side = ctx().redirect != nullptr;
+    if (bf && !w.ws && !side) {
         // both operands have BF16 copies: the library GEMM reads them as they are (the
         // algorithm the autotuner chose for the class when it chose the library)
         Sh16 sa, sb;
@@ -1559,7 +1577,7 @@ static void gemm16_impl(int kind, int m, int n, int k, float alpha, const float*
                          kp = (size_t)(k + QBK - 1) / QBK * QBK;
             sp = (float*)(w.ws + (big ? align256((mp + np) * kp * 2) : 0));
         } else {
-            sp = g_scale.get((size_t)2 * m + n);
+            sp = scale_work().get((size_t)2 * m + n);
         }
         fp16_scales(m, n, k, A, lda, B, ldb, sp, &rs, &cs);
     }
@@ -1569,16 +1587,18 @@ static void gemm16_impl(int kind, int m, int n, int k, float alpha, const float*
         const int rule = ceil_div(m, 256) * ceil_div(n, 256) >= 192 ? 3 : 0;
         int cand = rule;
         std::unique_ptr<tune::Probe> probe;
-        if (w.ws) {
+        if (w.ws || side) {
+            // no timing: the decided tile, and an own kernel where the library was chosen
+            // (it may allocate; beside the library stream it must not run)
             cand = tune::decided(bf ? 1 : 0, m, n, k, rule);
-            if (cand >= 10) cand = 3;  // the library may allocate: an own kernel instead
+            if (cand >= 10) cand = 3;
         } else {
             probe.reset(new tune::Probe(bf ? 1 : 0, m, n, k, rule));
             cand = probe->cand;
         }
         if (cand >= 10) {
-            if (bf && gemm16lt(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, cand - 10)) return;
-            cand = 3;  // FP16 (scaled operands), or no library plan: the 256 x 256 kernel
+            if (bf && !side && gemm16lt(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, cand - 10)) return;
+            cand = 3;  // FP16 (scaled operands), side stream, or no library plan: the 256 x 256 kernel
         }
         auto run = [&](auto tag_bf) {
             constexpr bool BFv = decltype(tag_bf)::value;
