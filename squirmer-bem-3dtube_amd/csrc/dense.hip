@@ -136,6 +136,14 @@ struct GemmK {
 // Workgroup tile TM x TN (TM = 128: TN = 128, or 64 for products whose 128 x 128 tiles would
 // leave CUs idle, four waves of 64 x TN/2; TM = 64: four waves of 64 x TN/4, for products
 // whose 128-row tiles would give the CUs under two workgroups each).
+// The LDS block of gemm_nn_tile, one per (element type, size): tiles of equal size (128 x 64
+// and 64 x 128) used by one kernel share it.
+template <typename T, int ELEMS>
+__device__ __forceinline__ T* gemm_lds() {
+    __shared__ __attribute__((aligned(16))) T block[ELEMS];
+    return block;
+}
+
 template <typename T, int TN = GBN, int TM = GBM>
 __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const T* __restrict__ A, int lda,
                                              const T* __restrict__ B, int ldb, T beta, T* __restrict__ C, int ldc,
@@ -148,7 +156,8 @@ __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const
                   NQB = TN * KB / 256;
     constexpr int JT = TM == 128 ? TN / 32 : TN / 64;  // 16-column MFMA tiles per wave
     // one LDS block: the two operand stages during the k loop, the C tile in the epilogue
-    __shared__ __attribute__((aligned(16))) T smem[2 * TM * PAD + 2 * TN * PAD];
+    constexpr int SMEM = 2 * TM * PAD + 2 * TN * PAD;
+    T* smem = gemm_lds<T, SMEM>();
     T(*As)[TM][PAD] = reinterpret_cast<T(*)[TM][PAD]>(smem);
     T(*Bs)[TN][PAD] = reinterpret_cast<T(*)[TN][PAD]>(smem + 2 * TM * PAD);
 
@@ -232,8 +241,8 @@ __device__ __forceinline__ void gemm_nn_tile(int m, int n, int k, T alpha, const
     // accumulators are written into a column-major tile of CW columns at a time, then every
     // thread updates C down columns, consecutive rows on consecutive lanes, with all of its
     // C values read before any result is stored
-    constexpr int LDC = TM + 4, CW = (int)(sizeof(smem) / (LDC * sizeof(T))) >= TN ? TN : TN / 2;
-    static_assert(CW * LDC * sizeof(T) <= sizeof(smem), "epilogue tile exceeds the LDS block");
+    constexpr int LDC = TM + 4, CW = SMEM / LDC >= TN ? TN : TN / 2;
+    static_assert(CW * LDC <= SMEM, "epilogue tile exceeds the LDS block");
     T* Ct = smem;
     const int er = tid % TM, ec = tid / TM;  // row, first column (then every AS-th)
     int bad = 0;
@@ -3222,8 +3231,55 @@ static void lu128_launch(int grid, T* A, int lda, int n, T* Linv, T* Uinv, int* 
 
 
 
+// The two products that follow the first diagonal block of a pair of leaves (w = 128 + w2
+// columns, w2 <= 128) in one launch, as they depend on that block alone: workgroups below
+// tiles_l form L21 = A21 U11^-1 in 64-row tiles (in place, multipliers checked, the BF16
+// copy row-major where the factor has one), the others U12 = L11^-1 A12 in 64-column tiles
+// (in place, each tile holding all 128 rows of its columns; the BF16 copy column-major).
+// Both tile shapes have the same LDS block.
+template <typename T>
+__global__ void __launch_bounds__(256) l21_u12_kernel(int m, int w2, T* A, int lda, const T* __restrict__ Linv,
+                                                      const T* __restrict__ Uinv, int* flag, int tiles_l,
+                                                      __bf16* sh_row, int sh_rld, __bf16* sh_col, int sh_cld) {
+    const int b = blockIdx.x;
+    if (b < tiles_l)
+        gemm_nn_tile<T, NB, 64>(m - NB, NB, NB, T(1), A + NB, lda, Uinv, NB, T(0), A + NB, lda, b, 0, flag, nullptr,
+                                0, sh_row, sh_rld);
+    else
+        gemm_nn_tile<T, 64, NB>(NB, w2, NB, T(1), Linv, NB, A + (size_t)NB * lda, lda, T(0),
+                                A + (size_t)NB * lda, lda, 0, b - tiles_l, nullptr, sh_col, sh_cld);
+}
+
 template <typename T>
 static void getrf_np(int kind, T* A, int lda, int m, int w, int base, T* Linv, T* Uinv, int* flag) {
+    if (w > NB && w <= 2 * NB && m > NB) {
+        // a pair of leaves: the first diagonal block, its L21 and U12 together, the update,
+        // then the second leaf
+        const int w2 = w - NB;
+        const size_t off = (size_t)(base / NB) * NB * NB;
+        {
+            ProfScope prof(PROF_PANEL, (2.0 / 3.0) * NB * NB * NB, 2.0 * sizeof(T) * NB * NB);
+            lu128_launch<T>(1, A, lda, NB, Linv + off, Uinv + off, flag, 1);
+        }
+        int srl = 0, scl = 0;
+        __bf16 *sr = nullptr, *sc = nullptr;
+        if constexpr (sizeof(T) == 4) {
+            sr = shadow_out((const float*)(A + NB), lda, m - NB, NB, true, &srl);
+            sc = shadow_out((const float*)(A + (size_t)NB * lda), lda, NB, w2, false, &scl);
+        }
+        T* A12 = A + (size_t)NB * lda;
+        const int tiles_l = (int)ceil_div(m - NB, 64), tiles_u = (int)ceil_div(w2, 64);
+        {
+            ProfScope prof(sizeof(T) == 8 ? PROF_DGEMM : PROF_SGEMM, 2.0 * (m - NB) * NB * NB + 2.0 * NB * NB * w2,
+                           sizeof(T) * (2.0 * (m - NB) * NB + 2.0 * NB * w2 + 2.0 * NB * NB));
+            hipLaunchKernelGGL(l21_u12_kernel<T>, dim3((unsigned)(tiles_l + tiles_u)), dim3(256), 0, ctx().s(), m, w2,
+                               A, lda, Linv + off, Uinv + off, flag, tiles_l, sr, srl, sc, scl);
+            check_launch("l21_u12");
+        }
+        gemm_kind<T>(kind, m - NB, w2, NB, T(-1), A + NB, lda, A12, lda, T(1), A12 + NB, lda);
+        getrf_np<T>(kind, A12 + NB, lda, m - NB, w2, base + NB, Linv, Uinv, flag);
+        return;
+    }
     if (w <= NB) {
         const size_t off = (size_t)(base / NB) * NB * NB;
         {
