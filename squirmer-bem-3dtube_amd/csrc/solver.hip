@@ -504,6 +504,16 @@ static void swimmer_build(sqbem_swimmer* s) {
     device_iota(self, Ns);
     dlsum.alloc((size_t)9 * Ns);
     dlsum.zero();
+    // the refined modes' low-precision copies the Schur complement starts from, written by
+    // the assembly beside the FP64 blocks: G32 = [E | Cs], Y32 = [B | Ct] (solved in place),
+    // D32 = D
+    if (mx) {
+        s->G32.alloc((size_t)ns * (ns + 6));
+        if (tb) {
+            s->Y32.alloc((size_t)nt * (ns + 6));
+            s->D32.alloc((size_t)ns * nt);
+        }
+    }
     // tube rows <- sphere sources: B = SL/(8 pi mu), AuBt = -DL/(8 pi)   (src/swimmer.hpp:188-198)
     if (tb) {
         s->AuBt.alloc((size_t)nt * ns);
@@ -516,11 +526,11 @@ static void swimmer_build(sqbem_swimmer* s) {
             Bdst = s->Y.p;
         }
         assemble(s->dm, 0, Ns, tb->dm.cen_xyz.p, Nt, nullptr, e8 * mu, Bdst, nt, e8, s->AuBt.p, nt, nullptr, 0.0,
-                 nullptr);
+                 nullptr, mx ? s->Y32.p : nullptr, nt);
         // sphere rows <- tube sources: D = SL/(8 pi mu), DL summed (src/swimmer.hpp:201-207)
         s->D.alloc((size_t)ns * nt);
         assemble(tb->dm, 0, Nt, s->dm.cen_xyz.p, Ns, nullptr, e8 * mu, s->D.p, ns, 1.0, nullptr, 0, dlsum.p, 0.0,
-                 nullptr);
+                 nullptr, mx ? s->D32.p : nullptr, ns);
     }
     // sphere rows <- sphere sources: E (deflated, src/swimmer.hpp:222), AuBs, DL sums
     s->AuBs.alloc((size_t)ns * ns);
@@ -541,7 +551,7 @@ static void swimmer_build(sqbem_swimmer* s) {
         for (int p = 0; p < Ns; ++p) Asum += s->S.area[p];
         const double defl = 1.0 / (e8 * mu * std::sqrt(Asum));
         assemble(s->dm, 0, Ns, s->dm.cen_xyz.p, Ns, self.p, e8 * mu, Edst, ns, e8, s->AuBs.p, ns, dlsum.p, defl,
-                 s->dm.nrm_xyz.p);
+                 s->dm.nrm_xyz.p, mx ? s->G32.p : nullptr, ns);
     }
     // completed double layer (src/swimmer.hpp:218-220)
     hipLaunchKernelGGL(completion_diag_kernel, dim3(ceil_div(Ns, 256)), dim3(256), 0, ctx().s(), Ns, s->AuBs.p,
@@ -560,17 +570,12 @@ static void swimmer_build(sqbem_swimmer* s) {
         kproj(ns, Ns, s->AuBs.p, ns, s->dm.cen_xyz.p, s->xc, s->G.p + (size_t)ns * ns, ns);
     }
     if (mx) {
-        // the low-precision copies the Schur complement starts from (still beside the tube's
-        // factorisation): G32 = [E | Cs], Y32 = [B | Ct] (solved in place), D32 = D
-        s->G32.alloc((size_t)ns * (ns + 6));
-        convert_matrix<double, float>(ns, ns, s->E.p, ns, s->G32.p, ns);
+        // the rigid-body columns' low-precision copies (and E's, when it came from the
+        // sphere's template), still beside the tube's factorisation
+        if (s->tmpl) convert_matrix<double, float>(ns, ns, s->E.p, ns, s->G32.p, ns);
         convert_matrix<double, float>(ns, 6, s->Cs.p, ns, s->G32.p + (size_t)ns * ns, ns);
         if (tb) {
-            s->Y32.alloc((size_t)nt * (ns + 6));
-            convert_matrix<double, float>(nt, ns, s->B.p, nt, s->Y32.p, nt);
             convert_matrix<double, float>(nt, 6, s->Ct.p, nt, s->Y32.p + (size_t)nt * ns, nt);
-            s->D32.alloc((size_t)ns * nt);
-            convert_matrix<double, float>(ns, nt, s->D.p, ns, s->D32.p, ns);
             if (s->prec == SQBEM_BF16_IR) {
                 s->D16ld = shadow_rows_ld(nt);
                 s->D16.alloc(shadow_rows_elems(ns, nt));
