@@ -38,7 +38,21 @@ static void init_device(int device) {
     g_ctx.hbm_bytes = prop.totalGlobalMem;
     std::snprintf(g_ctx.name, sizeof g_ctx.name, "%s (%s)", prop.name, prop.gcnArchName);
     if (!g_ctx.stream) SQ_HIP(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
-    if (!g_ctx.side) SQ_HIP(hipStreamCreateWithFlags(&g_ctx.side, hipStreamNonBlocking));
+    if (!g_ctx.side) {
+        // The side stream (a swimmer's coupling assembly beside the tube's factorisation,
+        // tiled factor copies) runs on three quarters of the CUs. The factorisation's
+        // diagonal-block kernel takes a CU's whole register file, so it waits for a CU to
+        // empty; with the side stream's workgroups on every CU it would wait at each of its
+        // hundreds of launches. Without a CU mask the side stream is an ordinary one.
+        const int cus = g_ctx.cu_count;
+        std::vector<uint32_t> mask((size_t)(cus + 31) / 32, 0u);
+        for (int i = 0; i < cus - cus / 4; ++i) mask[(size_t)i / 32] |= 1u << (i % 32);
+        if (cus < 16 || hipExtStreamCreateWithCUMask(&g_ctx.side, (uint32_t)mask.size(), mask.data()) != hipSuccess) {
+            (void)hipGetLastError();
+            g_ctx.side = nullptr;
+            SQ_HIP(hipStreamCreateWithFlags(&g_ctx.side, hipStreamNonBlocking));
+        }
+    }
     if (!g_ctx.join_ev) SQ_HIP(hipEventCreateWithFlags(&g_ctx.join_ev, hipEventDisableTiming));
     if (!g_ctx.fork_ev) SQ_HIP(hipEventCreateWithFlags(&g_ctx.fork_ev, hipEventDisableTiming));
     upload_tables();
@@ -102,7 +116,8 @@ void check_device_error() {
     if (!g_dev_err_check || !g_dev_err) return;
     g_dev_err_check = false;
     int v = 0;
-    SQ_HIP(hipMemcpy(&v, g_dev_err, sizeof(int), hipMemcpyDeviceToHost));
+    SQ_HIP(hipMemcpyAsync(&v, g_dev_err, sizeof(int), hipMemcpyDeviceToHost, ctx().s()));
+    SQ_HIP(hipStreamSynchronize(ctx().s()));
     if (v) {
         fill_zero(g_dev_err, 256);
         trsv_reset_counters();
