@@ -994,11 +994,12 @@ namespace {
 struct ShadowMap {
     const float* base;
     int ld, rows, cols;
-    __bf16* lo;  // (i, j) at lo[i lds + j]
-    __bf16* up;  // (i, j) at up[i + j lds]
+    __bf16* lo;   // (i, j) at lo[i lds + j]
+    __bf16* up;   // (i, j) at up[i + j lds]
     int lds;
     bool factor;
     int id;
+    __bf16* upr;  // a factor's part above the diagonal blocks again, row-major (optional)
 };
 std::vector<ShadowMap> g_shadows;
 int g_shadow_next = 1;
@@ -1024,44 +1025,53 @@ struct Sh16 {
     int ld;
     bool trans;
 };
-// the copy of the rows x cols operand at (p, ld), if registered and aligned for 16-byte loads
-bool shadow_operand(const float* p, int ld, int rows, int cols, Sh16& o) {
+// the copy of the rows x cols operand at (p, ld), if registered and aligned for 16-byte
+// loads; a_operand: the product's A (k-contiguous rows preferred) or its B (k-contiguous
+// columns preferred)
+bool shadow_operand(const float* p, int ld, int rows, int cols, bool a_operand, Sh16& o) {
     if (g_shadows.empty() || rows <= 0 || cols <= 0) return false;
     int r0, c0;
     const ShadowMap* m = shadow_at(p, ld, rows, cols, r0, c0);
     if (!m) return false;
     const bool lower = r0 / SHB > (c0 + cols - 1) / SHB, upper = c0 / SHB > (r0 + rows - 1) / SHB;
-    if (m->lo && (!m->factor || lower))
-        o = {m->lo + (size_t)r0 * m->lds + c0, m->lds, true};
-    else if (m->up && (!m->factor || upper))
-        o = {m->up + r0 + (size_t)c0 * m->lds, m->lds, false};
+    const Sh16 lo_rows = {m->lo ? m->lo + (size_t)r0 * m->lds + c0 : nullptr, m->lds, true};
+    const Sh16 up_rows = {m->upr ? m->upr + (size_t)r0 * m->lds + c0 : nullptr, m->lds, true};
+    const Sh16 up_cols = {m->up ? m->up + r0 + (size_t)c0 * m->lds : nullptr, m->lds, false};
+    if (!m->factor)
+        o = m->lo ? lo_rows : up_cols;
+    else if (lower && m->lo)
+        o = lo_rows;
+    else if (upper && (m->up || m->upr))
+        o = (a_operand && m->upr) || !m->up ? up_rows : up_cols;
     else
         return false;
     return (uintptr_t)o.p % 16 == 0 && o.ld % 8 == 0;
 }
 // where a kernel that finalises the rows x cols block at (p, ld) writes its copy:
 // column-major (the up part of a factor, or a dense column-major registration) or row-major
-// (the lo part of a factor); null when the block has none
+// (the lo or upr part of a factor); null when the block has none
 __bf16* shadow_out(const float* p, int ld, int rows, int cols, bool row_major, int* sld) {
     if (g_shadows.empty() || rows <= 0 || cols <= 0) return nullptr;
     int r0, c0;
     const ShadowMap* m = shadow_at(p, ld, rows, cols, r0, c0);
     if (!m) return nullptr;
     *sld = m->lds;
+    const bool lower = r0 / SHB > (c0 + cols - 1) / SHB, upper = c0 / SHB > (r0 + rows - 1) / SHB;
     if (row_major) {
-        const bool lower = r0 / SHB > (c0 + cols - 1) / SHB;
-        return m->lo && m->factor && lower ? m->lo + (size_t)r0 * m->lds + c0 : nullptr;
+        if (!m->factor) return nullptr;
+        if (lower) return m->lo ? m->lo + (size_t)r0 * m->lds + c0 : nullptr;
+        return upper && m->upr ? m->upr + (size_t)r0 * m->lds + c0 : nullptr;
     }
-    const bool upper = c0 / SHB > (r0 + rows - 1) / SHB;
     return m->up && (!m->factor || upper) ? m->up + r0 + (size_t)c0 * m->lds : nullptr;
 }
 }  // namespace
 
-ShadowScope::ShadowScope(const float* base, int ld, int rows, int cols, __bf16* lo, __bf16* up, int lds, bool factor)
+ShadowScope::ShadowScope(const float* base, int ld, int rows, int cols, __bf16* lo, __bf16* up, int lds, bool factor,
+                         __bf16* upr)
     : id(0) {
-    if (!base || (!lo && !up)) return;
+    if (!base || (!lo && !up && !upr)) return;
     id = g_shadow_next++;
-    g_shadows.push_back(ShadowMap{base, ld, rows, cols, lo, up, lds, factor, id});
+    g_shadows.push_back(ShadowMap{base, ld, rows, cols, lo, up, lds, factor, id, upr});
 }
 ShadowScope::~ShadowScope() {
     if (!id) return;
@@ -1573,7 +1583,7 @@ static void gemm16_impl(int kind, int m, int n, int k, float alpha, const float*
         // both operands have BF16 copies: the library GEMM reads them as they are (the
         // algorithm the autotuner chose for the class when it chose the library)
         Sh16 sa, sb;
-        if (shadow_operand(A, lda, m, k, sa) && shadow_operand(B, ldb, k, n, sb)) {
+        if (shadow_operand(A, lda, m, k, true, sa) && shadow_operand(B, ldb, k, n, false, sb)) {
             const int cand = big ? tune::decided(1, m, n, k, -1) : -1;
             if (gemm16lt_shadow(m, n, k, alpha, sa, sb, beta, C, ldc, cand >= 10 ? cand - 10 : 0)) return;
         }
@@ -2555,7 +2565,7 @@ __global__ void __launch_bounds__(256) trsm2_kernel(int n, int nrhs, const T* L,
 template <bool LOWER, int TN>
 __global__ void __launch_bounds__(256, TN <= 32 ? 2 : 1) trsm2p_kernel(int n, int nrhs, const float* L, int ldl,
                                                                        const float* inv, float* B, int ldb,
-                                                                       __bf16* sh, int shld) {
+                                                                       __bf16* sh, int shld, __bf16* shr, int shrld) {
     using Op = MfmaOp<float>;
     using Acc = typename Op::acc;
     constexpr int KB = GemmK<float>::KB, PAD = GemmK<float>::PAD, NQ = KB / 2, S = KB / 4;
@@ -2678,6 +2688,13 @@ __global__ void __launch_bounds__(256, TN <= 32 ? 2 : 1) trsm2p_kernel(int n, in
                 if (sh) sh[tid + (size_t)(n0 + c) * shld] = (__bf16)v;  // BF16 copy (column-major)
             }
     }
+    if (shr) {
+        // the row-major BF16 copy: consecutive columns of a row on consecutive lanes
+        for (int e = tid; e < n * TN; e += 256) {
+            const int r = e / TN, c = e % TN;
+            if (n0 + c < nrhs) shr[(size_t)r * shrld + n0 + c] = (__bf16)Pn[c * NP + r];
+        }
+    }
 }
 
 // Triangular solve through inverted BS x BS diagonal blocks (consecutive in inv, leading
@@ -2687,15 +2704,18 @@ template <typename T, bool LOWER, int BS = IB>
 static void trsm_inv(int kind, int n, int nrhs, const T* L, int ldl, const T* inv, T* B, int ldb) {
     if (n <= 0 || nrhs <= 0) return;
     // a leaf's results are final rows of X: their BF16 copy, where B's matrix has one
-    int shld = 0;
-    __bf16* sh = nullptr;
+    int shld = 0, shrld = 0;
+    __bf16 *sh = nullptr, *shr = nullptr;
     if constexpr (sizeof(T) == 4) {
-        if (n <= 2 * BS) sh = shadow_out((const float*)B, ldb, n, nrhs, false, &shld);
+        if (n <= 2 * BS) {
+            sh = shadow_out((const float*)B, ldb, n, nrhs, false, &shld);
+            shr = shadow_out((const float*)B, ldb, n, nrhs, true, &shrld);
+        }
     }
     if (n <= BS) {
         ProfScope prof(PROF_TRSM, 2.0 * n * n * nrhs, sizeof(T) * (2.0 * n * nrhs + (double)n * n));
-        if (sh)
-            gemm_launch<T>(n, nrhs, n, T(1), inv, BS, B, ldb, T(0), B, ldb, nullptr, sh, shld);
+        if (sh || shr)
+            gemm_launch<T>(n, nrhs, n, T(1), inv, BS, B, ldb, T(0), B, ldb, nullptr, sh, shld, shr, shrld);
         else
             gemm_exact<T>(n, nrhs, n, T(1), inv, BS, B, ldb, T(0), B, ldb);
         return;
@@ -2710,13 +2730,13 @@ static void trsm_inv(int kind, int n, int nrhs, const T* L, int ldl, const T* in
                 const int cus = std::max(ctx().cu_count, 1);
                 if (ceil_div(nrhs, 32) < cus / 2)
                     hipLaunchKernelGGL((trsm2p_kernel<LOWER, 16>), dim3((unsigned)ceil_div(nrhs, 16)), dim3(256), 0,
-                                       ctx().s(), n, nrhs, L, ldl, inv, B, ldb, sh, shld);
+                                       ctx().s(), n, nrhs, L, ldl, inv, B, ldb, sh, shld, shr, shrld);
                 else if (ceil_div(nrhs, TRSM2_TN) < cus)
                     hipLaunchKernelGGL((trsm2p_kernel<LOWER, 32>), dim3((unsigned)ceil_div(nrhs, 32)), dim3(256), 0,
-                                       ctx().s(), n, nrhs, L, ldl, inv, B, ldb, sh, shld);
+                                       ctx().s(), n, nrhs, L, ldl, inv, B, ldb, sh, shld, shr, shrld);
                 else
                     hipLaunchKernelGGL((trsm2p_kernel<LOWER, TRSM2_TN>), dim3((unsigned)ceil_div(nrhs, TRSM2_TN)),
-                                       dim3(256), 0, ctx().s(), n, nrhs, L, ldl, inv, B, ldb, sh, shld);
+                                       dim3(256), 0, ctx().s(), n, nrhs, L, ldl, inv, B, ldb, sh, shld, shr, shrld);
             }
             else
                 hipLaunchKernelGGL((trsm2_kernel<T, LOWER>), dim3((unsigned)ceil_div(nrhs, TRSM2_TN)), dim3(256), 0,
@@ -3240,14 +3260,15 @@ static void lu128_launch(int grid, T* A, int lda, int n, T* Linv, T* Uinv, int* 
 template <typename T>
 __global__ void __launch_bounds__(256) l21_u12_kernel(int m, int w2, T* A, int lda, const T* __restrict__ Linv,
                                                       const T* __restrict__ Uinv, int* flag, int tiles_l,
-                                                      __bf16* sh_row, int sh_rld, __bf16* sh_col, int sh_cld) {
+                                                      __bf16* sh_row, int sh_rld, __bf16* sh_col, int sh_cld,
+                                                      __bf16* sh_urow, int sh_urld) {
     const int b = blockIdx.x;
     if (b < tiles_l)
         gemm_nn_tile<T, NB, 64>(m - NB, NB, NB, T(1), A + NB, lda, Uinv, NB, T(0), A + NB, lda, b, 0, flag, nullptr,
                                 0, sh_row, sh_rld);
     else
         gemm_nn_tile<T, 64, NB>(NB, w2, NB, T(1), Linv, NB, A + (size_t)NB * lda, lda, T(0),
-                                A + (size_t)NB * lda, lda, 0, b - tiles_l, nullptr, sh_col, sh_cld);
+                                A + (size_t)NB * lda, lda, 0, b - tiles_l, nullptr, sh_col, sh_cld, sh_urow, sh_urld);
 }
 
 template <typename T>
@@ -3261,11 +3282,12 @@ static void getrf_np(int kind, T* A, int lda, int m, int w, int base, T* Linv, T
             ProfScope prof(PROF_PANEL, (2.0 / 3.0) * NB * NB * NB, 2.0 * sizeof(T) * NB * NB);
             lu128_launch<T>(1, A, lda, NB, Linv + off, Uinv + off, flag, 1);
         }
-        int srl = 0, scl = 0;
-        __bf16 *sr = nullptr, *sc = nullptr;
+        int srl = 0, scl = 0, sul = 0;
+        __bf16 *sr = nullptr, *sc = nullptr, *su = nullptr;
         if constexpr (sizeof(T) == 4) {
             sr = shadow_out((const float*)(A + NB), lda, m - NB, NB, true, &srl);
             sc = shadow_out((const float*)(A + (size_t)NB * lda), lda, NB, w2, false, &scl);
+            su = shadow_out((const float*)(A + (size_t)NB * lda), lda, NB, w2, true, &sul);
         }
         T* A12 = A + (size_t)NB * lda;
         const int tiles_l = (int)ceil_div(m - NB, 64), tiles_u = (int)ceil_div(w2, 64);
@@ -3273,7 +3295,7 @@ static void getrf_np(int kind, T* A, int lda, int m, int w, int base, T* Linv, T
             ProfScope prof(sizeof(T) == 8 ? PROF_DGEMM : PROF_SGEMM, 2.0 * (m - NB) * NB * NB + 2.0 * NB * NB * w2,
                            sizeof(T) * (2.0 * (m - NB) * NB + 2.0 * NB * w2 + 2.0 * NB * NB));
             hipLaunchKernelGGL(l21_u12_kernel<T>, dim3((unsigned)(tiles_l + tiles_u)), dim3(256), 0, ctx().s(), m, w2,
-                               A, lda, Linv + off, Uinv + off, flag, tiles_l, sr, srl, sc, scl);
+                               A, lda, Linv + off, Uinv + off, flag, tiles_l, sr, srl, sc, scl, su, sul);
             check_launch("l21_u12");
         }
         gemm_kind<T>(kind, m - NB, w2, NB, T(-1), A + NB, lda, A12, lda, T(1), A12 + NB, lda);
@@ -3898,7 +3920,8 @@ void getrf_begin(GetrfJob<T>& j) {
     std::unique_ptr<ShadowScope> sh;
     if constexpr (sizeof(T) == 4) {
         if (j.kind == SQBEM_BF16_IR && j.sh_lo && j.sh_up)
-            sh.reset(new ShadowScope((const float*)j.A, j.lda, n, n, j.sh_lo, j.sh_up, j.sh_ld ? j.sh_ld : n, true));
+            sh.reset(new ShadowScope((const float*)j.A, j.lda, n, n, j.sh_lo, j.sh_up, j.sh_ld ? j.sh_ld : n, true,
+                                     j.sh_upr));
     }
     getrf_np<T>(j.kind, j.A, j.lda, n, n, 0, Linv, Linv + nb * NB * NB, j.flag.p);
 }

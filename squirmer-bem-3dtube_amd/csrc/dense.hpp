@@ -55,6 +55,7 @@ struct GetrfJob {
     std::function<void()> restore;
     __bf16* sh_lo = nullptr;
     __bf16* sh_up = nullptr;
+    __bf16* sh_upr = nullptr;  // optional: the part above the diagonal blocks again, row-major
     int sh_ld = 0;
     int path = 0;
     DBuf<int> flag;
@@ -75,10 +76,12 @@ inline size_t getrf_aux_elems(int n) {
 // of the matrix (the L21 products and the triangular-solve leaves) write the copy as well.
 // A factor registration covers the parts below (lo, row-major) and above (up,
 // column-major) the 128 x 128 diagonal blocks; a dense one the whole matrix, column-major
-// (up) or row-major (lo). The caller registers only copies whose values are, or will be
-// made, final before any product reads them.
+// (up) or row-major (lo); a factor may also hold its upper part row-major (upr), which the
+// U-side solves read as their products' A operand. The caller registers only copies whose
+// values are, or will be made, final before any product reads them.
 struct ShadowScope {
-    ShadowScope(const float* base, int ld, int rows, int cols, __bf16* lo, __bf16* up, int lds, bool factor);
+    ShadowScope(const float* base, int ld, int rows, int cols, __bf16* lo, __bf16* up, int lds, bool factor,
+                __bf16* upr = nullptr);
     ~ShadowScope();
     ShadowScope(const ShadowScope&) = delete;
     ShadowScope& operator=(const ShadowScope&) = delete;

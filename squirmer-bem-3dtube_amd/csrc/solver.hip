@@ -370,8 +370,10 @@ void sq::tube_build(sqbem_tube* t) {
             t->sh_ld = (nt + 63) / 64 * 64;
             t->SL.alloc((size_t)nt * t->sh_ld);
             t->SU.alloc((size_t)nt * t->sh_ld);
+            t->SUr.alloc((size_t)nt * t->sh_ld);
             j.sh_lo = t->SL.p;
             j.sh_up = t->SU.p;
+            j.sh_upr = t->SUr.p;
             j.sh_ld = t->sh_ld;
         }
         getrf_begin<float>(j);
@@ -606,7 +608,7 @@ static void swimmer_build(sqbem_swimmer* s) {
             const int yld = (nt + 63) / 64 * 64;
             if (sh) Yh.alloc((size_t)yld * (ns + 6));
             {
-                ShadowScope f(sh ? tb->T32.p : nullptr, nt, nt, nt, tb->SL.p, tb->SU.p, tb->sh_ld, true);
+                ShadowScope f(sh ? tb->T32.p : nullptr, nt, nt, nt, tb->SL.p, tb->SU.p, tb->sh_ld, true, tb->SUr.p);
                 ShadowScope y(sh ? s->Y32.p : nullptr, nt, nt, ns + 6, nullptr, Yh.p, yld, false);
                 ShadowScope d(sh ? s->D32.p : nullptr, ns, ns, nt, s->D16.p, nullptr, s->D16ld, false);
                 getrs<float>(kind, 0, nt, ns + 6, tb->T32.p, nt, tb->ipiv.p, s->Y32.p, nt, tb->aux32.p);

@@ -23,7 +23,7 @@ struct sqbem_tube {
     sq::DBuf<float> aux32;
     // BF16 mode: BF16 copies of the factors below / above the diagonal blocks (row-major /
     // column-major, leading dimension sh_ld), whole when np_path = 1 (no-pivot factors)
-    sq::DBuf<__bf16> SL, SU;
+    sq::DBuf<__bf16> SL, SU, SUr;  // SUr: the part above the diagonal blocks again, row-major
     int sh_ld = 0, np_path = 0;
     sqbem_lu view;  // non-owning handle onto the factors (Tube::lu)
     double t_assembly = 0, t_factor = 0;
