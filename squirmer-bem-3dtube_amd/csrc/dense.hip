@@ -2562,7 +2562,7 @@ __global__ void __launch_bounds__(256) trsm2_kernel(int n, int nrhs, const T* L,
 // for global memory about once instead of once per 32-deep k tile. The products are the
 // same MFMA sums in the same order as gemm_nn_tile's (k tiles of 32, steps of 4), so the
 // results equal trsm2_kernel's.
-template <bool LOWER, int TN>
+template <bool LOWER, int TN, bool MID16 = false>
 __global__ void __launch_bounds__(256, TN <= 32 ? 2 : 1) trsm2p_kernel(int n, int nrhs, const float* L, int ldl,
                                                                        const float* inv, float* B, int ldb,
                                                                        __bf16* sh, int shld, __bf16* shr, int shrld) {
@@ -2651,6 +2651,47 @@ __global__ void __launch_bounds__(256, TN <= 32 ? 2 : 1) trsm2p_kernel(int n, in
             }
         }
     };
+    // MID16: the off-diagonal product (o = 1) with BF16 MFMA (16x16x32: lane (lr, lk) holds
+    // A(row, 8 lk .. 8 lk + 7) and B(8 lk .. 8 lk + 7, col), rounded to BF16 as the 16-bit
+    // products above these levels round their operands), one MFMA step per 32-deep k tile
+    auto product16 = [&](const float (&r)[NKT][NQ], int o, int ko) {
+        const int mr = opm[o], kd = opk[o];
+        const int nk = (kd + KB - 1) / KB;
+#pragma unroll
+        for (int i = 0; i < MI; ++i)
+#pragma unroll
+            for (int j = 0; j < JT; ++j) acc[i][j] = Acc{0, 0, 0, 0};
+        auto store = [&](int buf, int kt) {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+                As[buf][a_i][a_k0 + 2 * q] = (a_i < mr && kt * KB + a_k0 + 2 * q < kd) ? r[kt][q] : 0.0f;
+        };
+        store(0, 0);
+        __syncthreads();
+#pragma unroll
+        for (int kt = 0; kt < NKT; ++kt) {
+            if (kt < nk) {
+                const int buf = kt & 1;
+                b8v af[MI], bv[JT];
+#pragma unroll
+                for (int i = 0; i < MI; ++i)
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) af[i][e] = (__bf16)As[buf][wm + i * 16 + lr][8 * lk + e];
+#pragma unroll
+                for (int j = 0; j < JT; ++j)
+#pragma unroll
+                    for (int e = 0; e < 8; ++e)
+                        bv[j][e] = (__bf16)Pn[(wn + j * 16 + lr) * NP + ko + kt * KB + 8 * lk + e];
+#pragma unroll
+                for (int i = 0; i < MI; ++i)
+#pragma unroll
+                    for (int j = 0; j < JT; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bv[j], acc[i][j], 0, 0, 0);
+                if (kt + 1 < nk) store(buf ^ 1, kt + 1);
+                __syncthreads();
+            }
+        }
+    };
     // rows ro .. ro + mr - 1 of the panel: = acc (set) or -= acc (sub, as gemm's -acc + c)
     auto put = [&](int ro, int mr, bool sub) {
 #pragma unroll
@@ -2673,7 +2714,10 @@ __global__ void __launch_bounds__(256, TN <= 32 ? 2 : 1) trsm2p_kernel(int n, in
     fetch(ra, 2);         // the third operand, in flight during the second product
     put(ro0, opm[0], false);
     __syncthreads();
-    product(rb, 1, ko1);
+    if constexpr (MID16)
+        product16(rb, 1, ko1);
+    else
+        product(rb, 1, ko1);
     put(ro1, opm[1], true);
     __syncthreads();
     product(ra, 2, ko2);
@@ -2727,16 +2771,27 @@ static void trsm_inv(int kind, int n, int nrhs, const T* L, int ldl, const T* in
                 // narrower panels when 64-column ones would leave CUs idle (narrow right-hand
                 // sides inside the LU): 32 columns, or 16 when even those fill under half the
                 // CUs; more workgroups, less work each
+                // BF16 mode: the off-diagonal product in BF16, like the updates above it
                 const int cus = std::max(ctx().cu_count, 1);
-                if (ceil_div(nrhs, 32) < cus / 2)
-                    hipLaunchKernelGGL((trsm2p_kernel<LOWER, 16>), dim3((unsigned)ceil_div(nrhs, 16)), dim3(256), 0,
-                                       ctx().s(), n, nrhs, L, ldl, inv, B, ldb, sh, shld, shr, shrld);
-                else if (ceil_div(nrhs, TRSM2_TN) < cus)
-                    hipLaunchKernelGGL((trsm2p_kernel<LOWER, 32>), dim3((unsigned)ceil_div(nrhs, 32)), dim3(256), 0,
-                                       ctx().s(), n, nrhs, L, ldl, inv, B, ldb, sh, shld, shr, shrld);
+                auto go = [&](auto mid_tag) {
+                    constexpr bool M16 = decltype(mid_tag)::value;
+                    if (ceil_div(nrhs, 32) < cus / 2)
+                        hipLaunchKernelGGL((trsm2p_kernel<LOWER, 16, M16>), dim3((unsigned)ceil_div(nrhs, 16)),
+                                           dim3(256), 0, ctx().s(), n, nrhs, L, ldl, inv, B, ldb, sh, shld, shr,
+                                           shrld);
+                    else if (ceil_div(nrhs, TRSM2_TN) < cus)
+                        hipLaunchKernelGGL((trsm2p_kernel<LOWER, 32, M16>), dim3((unsigned)ceil_div(nrhs, 32)),
+                                           dim3(256), 0, ctx().s(), n, nrhs, L, ldl, inv, B, ldb, sh, shld, shr,
+                                           shrld);
+                    else
+                        hipLaunchKernelGGL((trsm2p_kernel<LOWER, TRSM2_TN, M16>),
+                                           dim3((unsigned)ceil_div(nrhs, TRSM2_TN)), dim3(256), 0, ctx().s(), n, nrhs,
+                                           L, ldl, inv, B, ldb, sh, shld, shr, shrld);
+                };
+                if (kind == SQBEM_BF16_IR)
+                    go(std::true_type{});
                 else
-                    hipLaunchKernelGGL((trsm2p_kernel<LOWER, TRSM2_TN>), dim3((unsigned)ceil_div(nrhs, TRSM2_TN)),
-                                       dim3(256), 0, ctx().s(), n, nrhs, L, ldl, inv, B, ldb, sh, shld, shr, shrld);
+                    go(std::false_type{});
             }
             else
                 hipLaunchKernelGGL((trsm2_kernel<T, LOWER>), dim3((unsigned)ceil_div(nrhs, TRSM2_TN)), dim3(256), 0,
