@@ -820,6 +820,8 @@ int sqbem_batch_solve(int n, const sqbem_mesh* const* tubes, const sqbem_mesh* c
         SQ_REQUIRE(n >= 0 && (n == 0 || (tubes && spheres && centers && e && B && U && Om)), SQBEM_ERR_ARG,
                    "sqbem_batch_solve: bad arguments");
         SQ_REQUIRE(precision >= SQBEM_FP64 && precision <= SQBEM_BF16_IR, SQBEM_ERR_ARG, "unknown precision");
+        // the chunks' stages run eagerly or as captured graphs: the same kernels either way
+        LibraryGemmOff own_kernels;
         std::vector<int> it_local(std::max(n, 1));
         std::vector<double> res_local(std::max(n, 1));
         if (!iters) iters = it_local.data();

@@ -70,6 +70,16 @@ inline size_t getrf_aux_elems(int n) {
     const size_t nb = (size_t)((n + 127) / 128);
     return (4 * nb + nb * nb) * 128 * 128;
 }
+// While one is alive, plain FP64/FP32 products use this library's kernels instead of
+// rocBLAS: work that is also captured into graphs (where rocBLAS is never used) then runs
+// the same kernels eagerly and in a graph.
+struct LibraryGemmOff {
+    LibraryGemmOff();
+    ~LibraryGemmOff();
+    LibraryGemmOff(const LibraryGemmOff&) = delete;
+    LibraryGemmOff& operator=(const LibraryGemmOff&) = delete;
+};
+
 // ---- BF16 copies of final values ("shadows") for the 16-bit products (BF16 kind)
 // While a scope is alive, a 16-bit product whose operands lie in a registered FP32 matrix
 // reads them from its BF16 copy (no conversion pass), and the kernels that finalise values

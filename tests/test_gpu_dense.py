@@ -27,7 +27,9 @@ def host(t):
     return t.cpu().numpy()
 
 
-SHAPES = [(1, 1, 1), (37, 53, 19), (128, 128, 16), (129, 130, 17), (300, 257, 129), (1000, 2, 700), (64, 960, 3)]
+# the last shape is large enough for the library (rocBLAS) path of plain FP64/FP32 products
+SHAPES = [(1, 1, 1), (37, 53, 19), (128, 128, 16), (129, 130, 17), (300, 257, 129), (1000, 2, 700), (64, 960, 3),
+          (1024, 1100, 700)]
 
 
 @pytest.mark.parametrize("m,n,k", SHAPES)
@@ -43,6 +45,22 @@ def test_dgemm(sq, m, n, k):
     scale = np.abs(A) @ np.abs(B)
     err = np.abs(host(dC) - ref) / (np.abs(alpha) * scale + np.abs(beta * C) + 1e-300)
     assert err.max() < 4 * k * 2.2e-16 + 1e-15, err.max()
+
+
+def test_dgemm_library_path_submatrix(sq):
+    """A large plain product (rocBLAS) on sub-blocks of one matrix: leading dimensions larger
+    than the rows, beta = 0 with C starting as NaN, and C beside (not overlapping) A and B."""
+    rng = np.random.default_rng(5)
+    big = rng.standard_normal((1700, 1800))
+    dbig = dev(big)
+    A = dbig[3:803, 0:600]
+    B = dbig[900:1500, 1000:1700]
+    C = torch.full((800, 700), float("nan"), dtype=torch.float64, device="cuda").t().contiguous().t()
+    torch.cuda.synchronize()
+    sq.dgemm(1.0, A, B, 0.0, C)
+    sq.sync()
+    ref = big[3:803, 0:600] @ big[900:1500, 1000:1700]
+    assert np.allclose(host(C), ref, rtol=1e-12, atol=1e-11)
 
 
 def test_dgemm_submatrix_and_beta_zero(sq):
