@@ -44,6 +44,8 @@ struct Context {
     size_t hbm_bytes = 0;
     char name[256] = {0};
     hipStream_t s() const { return redirect ? redirect : (user_stream ? user_stream : stream); }
+    // the library stream, also inside a SideScope
+    hipStream_t lib() const { return user_stream ? user_stream : stream; }
 };
 
 Context& ctx();          // initialises device 0 on first use

@@ -115,13 +115,18 @@ void mark_device_error_check() { g_dev_err_check = true; }
 void check_device_error() {
     if (!g_dev_err_check || !g_dev_err) return;
     g_dev_err_check = false;
+    // the chained solves that write the word run on the library stream: read it there, also
+    // when called inside a side-stream scope
     int v = 0;
-    SQ_HIP(hipMemcpyAsync(&v, g_dev_err, sizeof(int), hipMemcpyDeviceToHost, ctx().s()));
-    SQ_HIP(hipStreamSynchronize(ctx().s()));
+    SQ_HIP(hipMemcpyAsync(&v, g_dev_err, sizeof(int), hipMemcpyDeviceToHost, ctx().lib()));
+    SQ_HIP(hipStreamSynchronize(ctx().lib()));
     if (v) {
+        const hipStream_t saved = ctx().redirect;
+        ctx().redirect = nullptr;  // the clearing kernels go to the library stream as well
         fill_zero(g_dev_err, 256);
         trsv_reset_counters();
-        SQ_HIP(hipStreamSynchronize(ctx().s()));
+        ctx().redirect = saved;
+        SQ_HIP(hipStreamSynchronize(ctx().lib()));
         throw Error(SQBEM_ERR_HIP, v == 2 ? "triangular solve: block counter out of step with the host"
                                           : "triangular solve: a dependency wait timed out");
     }
