@@ -208,13 +208,15 @@ __global__ void gemv_reduce_kernel(int m, int nrhs, const double* __restrict__ p
     }
 }
 
-static DBuf<double> g_gv;
+// partial sums of the column chunks, one buffer per stream (products may run on both at once)
+static DBuf<double> g_gv_main, g_gv_side;
 template <typename T>
 void gemv(int m, int n, int nrhs, double alpha, const T* A, long lda, const T* X, long ldx, double beta, T* Y,
                  long ldy) {
     if (m <= 0 || nrhs <= 0) return;
     SQ_REQUIRE(nrhs <= 8, SQBEM_ERR_ARG, "gemv: at most 8 right-hand sides");
     size_t need = (size_t)GV_CHUNKS * nrhs * m;
+    DBuf<double>& g_gv = ctx().redirect ? g_gv_side : g_gv_main;
     if (g_gv.n < need) g_gv.alloc(std::max<size_t>(need, 1 << 20));
     if (n <= 0) {
         scale_matrix<T>(m, nrhs, (T)beta, Y, (int)ldy);
@@ -716,38 +718,93 @@ static void solve_fp64(sqbem_swimmer* s, int k, const double* u_host, bool towed
 // refined solution is the reference's solution to the refinement tolerance.
 //
 // x, b, r: device, N = nt + ns + 6 rows per right-hand side, blocks [t; s; c].
-static void full_residual(sqbem_swimmer* s, int k, const double* x, const double* b, double* r, bool towed) {
+// Rows of the tube (r_t): T x_t + B x_s + Ct w, the larger part of the FP64 operator.
+static void residual_tube_rows(sqbem_swimmer* s, int k, const double* x, const double* b, double* r, bool towed) {
     const int ns = s->ns, nt = s->nt, N = nt + ns + 6;
-    convert_matrix<double, double>(N, k, b, N, r, N);
+    if (!s->tube) return;
+    convert_matrix<double, double>(nt, k, b, N, r, N);
+    gemv<double>(nt, nt, k, -1.0, s->tube->T64.p, nt, x, N, 1.0, r, N);
+    gemv<double>(nt, ns, k, -1.0, s->B.p, nt, x + nt, N, 1.0, r, N);
+    if (!towed) gemv<double>(nt, 6, k, -1.0, s->Ct.p, nt, x + nt + ns, N, 1.0, r, N);
+}
+// Rows of the sphere (r_s: D x_t + E x_s + Cs w) and of the constraints (r_c); c: 6 k
+// values of scratch that outlive the launches.
+static void residual_sphere_rows(sqbem_swimmer* s, int k, const double* x, const double* b, double* r, bool towed,
+                                 double* c) {
+    const int ns = s->ns, nt = s->nt, N = nt + ns + 6;
     const double* xt = x;
     const double* xs = x + nt;
     const double* xw = xs + ns;
-    double* rt = r;
-    double* rs = rt + nt;
+    double* rs = r + nt;
     double* rc = rs + ns;
-    // all right-hand sides per pass: the FP64 operator is read once per refinement step
-    if (s->tube) {
-        gemv<double>(nt, nt, k, -1.0, s->tube->T64.p, nt, xt, N, 1.0, rt, N);
-        gemv<double>(nt, ns, k, -1.0, s->B.p, nt, xs, N, 1.0, rt, N);
-        if (!towed) gemv<double>(nt, 6, k, -1.0, s->Ct.p, nt, xw, N, 1.0, rt, N);
-        gemv<double>(ns, nt, k, -1.0, s->D.p, ns, xt, N, 1.0, rs, N);
-    }
+    convert_matrix<double, double>(ns + 6, k, b + nt, N, rs, N);
+    if (s->tube) gemv<double>(ns, nt, k, -1.0, s->D.p, ns, xt, N, 1.0, rs, N);
     gemv<double>(ns, ns, k, -1.0, s->E.p, ns, xs, N, 1.0, rs, N);
     if (!towed) {
         gemv<double>(ns, 6, k, -1.0, s->Cs.p, ns, xw, N, 1.0, rs, N);
-        DBuf<double> c((size_t)6 * k);
         hipLaunchKernelGGL(constraint_apply_kernel, dim3(k), dim3(256), 0, ctx().s(), s->Ns, k, xs, (long)N,
-                           s->dm.cen_xyz.p, s->dm.area.p, s->sc32, s->xc[0], s->xc[1], s->xc[2], 1.0, c.p, 6L);
+                           s->dm.cen_xyz.p, s->dm.area.p, s->sc32, s->xc[0], s->xc[1], s->xc[2], 1.0, c, 6L);
         check_launch("constraint_apply");
-        for (int j = 0; j < k; ++j) axpy(6, -1.0, c.p + 6 * j, rc + (size_t)j * N);
-        // c goes back to the memory cache on return: the next library-stream work that takes
-        // it runs after these kernels, and side-stream work only takes blocks released
-        // before the library stream last drained
+        for (int j = 0; j < k; ++j) axpy(6, -1.0, c + 6 * j, rc + (size_t)j * N);
     }
 }
 
+// The sphere and constraint rows of a refinement residual run on the side stream: the next
+// correction reads them only after its tube solves (chains of latency-bound hand-overs,
+// which leave most of the HBM bandwidth to this pass) on the library stream. Nothing is
+// allocated or released on the side stream here (the partial sums have a buffer of its
+// own, the scratch comes from the caller), and the library stream waits for the pass
+// before anything reads those rows and before the caller's buffers are released.
+struct ResidualFork {
+    hipEvent_t fork = nullptr, join = nullptr;
+    bool pending = false;
+    ResidualFork() {
+        SQ_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        SQ_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+    }
+    ~ResidualFork() {
+        wait();
+        (void)hipEventDestroy(fork);
+        (void)hipEventDestroy(join);
+    }
+    ResidualFork(const ResidualFork&) = delete;
+    ResidualFork& operator=(const ResidualFork&) = delete;
+    void wait() {
+        if (!pending) return;
+        pending = false;
+        (void)hipStreamWaitEvent(ctx().s(), join, 0);
+    }
+};
+
+// r = b - A x in FP64 (every right-hand side a pass: the FP64 operator is read once per
+// refinement step). With rf, the sphere and constraint rows go to the side stream and rf
+// holds their completion until rf->wait().
+static void full_residual(sqbem_swimmer* s, int k, const double* x, const double* b, double* r, bool towed,
+                          double* c, ResidualFork* rf) {
+    if (rf && ctx().side && !ctx().redirect && !capturing()) {
+        SQ_HIP(hipEventRecord(rf->fork, ctx().s()));
+        SQ_HIP(hipStreamWaitEvent(ctx().side, rf->fork, 0));
+        ctx().redirect = ctx().side;
+        try {
+            residual_sphere_rows(s, k, x, b, r, towed, c);
+        } catch (...) {
+            ctx().redirect = nullptr;
+            (void)hipEventRecord(rf->join, ctx().side);
+            rf->pending = true;
+            throw;
+        }
+        ctx().redirect = nullptr;
+        SQ_HIP(hipEventRecord(rf->join, ctx().side));
+        rf->pending = true;
+    } else {
+        residual_sphere_rows(s, k, x, b, r, towed, c);
+    }
+    residual_tube_rows(s, k, x, b, r, towed);
+}
+
 // d = approx A^-1 r from the low-precision factors
-static void approx_solve(sqbem_swimmer* s, int k, const double* r, double* d, bool towed) {
+// rf: a residual whose sphere rows may still be in the making on the side stream
+static void approx_solve(sqbem_swimmer* s, int k, const double* r, double* d, bool towed, ResidualFork& rf) {
     const int ns = s->ns, nt = s->nt, N = nt + ns + 6, n = ns + 6;
     const int kind = s->prec;
     const int nv = towed ? ns : n;  // unknowns of the reduced system
@@ -757,6 +814,7 @@ static void approx_solve(sqbem_swimmer* s, int k, const double* r, double* d, bo
         convert_matrix<double, float>(nt, k, r, N, z32.p, nt);
         getrs<float>(kind, 0, nt, k, s->tube->T32.p, nt, s->tube->ipiv.p, z32.p, nt, s->tube->aux32.p);
     }
+    rf.wait();
     convert_matrix<double, float>(nv, k, r + nt, N, v32.p, n);
     if (nt) gemv<float>(ns, nt, k, -1.0, s->D32.p, ns, z32.p, nt, 1.0, v32.p, n);
     if (towed)
@@ -798,7 +856,7 @@ static void col_norms(int N, int k, const double* a, std::vector<double>& out) {
 static void solve_mixed(sqbem_swimmer* s, int k, const double* u_host, bool towed, const double* w_host,
                         double* X_host, double* ft_host) {
     const int ns = s->ns, nt = s->nt, N = nt + ns + 6, n = ns + 6;
-    DBuf<double> b((size_t)N * k), x((size_t)N * k), r((size_t)N * k), d((size_t)N * k);
+    DBuf<double> b((size_t)N * k), x((size_t)N * k), r((size_t)N * k), d((size_t)N * k), c((size_t)6 * k);
     b.zero();
     x.zero();
     if (!towed) {
@@ -838,11 +896,13 @@ static void solve_mixed(sqbem_swimmer* s, int k, const double* u_host, bool towe
     const int min_check = s->prec == SQBEM_BF16_IR ? 3 : 2;
     int it = 0;
     double worst = 1;
+    ResidualFork rf;  // after the buffers: waited for before they are released
     for (; it < 40; ++it) {
-        approx_solve(s, k, r.p, d.p, towed);
+        approx_solve(s, k, r.p, d.p, towed, rf);
         axpy((size_t)N * k, 1.0, d.p, x.p);
-        full_residual(s, k, x.p, b.p, r.p, towed);
+        full_residual(s, k, x.p, b.p, r.p, towed, c.p, &rf);
         if (it + 1 < min_check) continue;
+        rf.wait();
         col_norms(N, k, r.p, rn);
         worst = 0;
         for (int j = 0; j < k; ++j) worst = std::max(worst, bn[j] > 0 ? rn[j] / bn[j] : rn[j]);
