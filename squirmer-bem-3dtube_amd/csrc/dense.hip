@@ -3769,13 +3769,31 @@ __device__ __forceinline__ void trsv_tag_body(int n, int nrhs, const T* __restri
         fetch(k - 1, 1);
         __syncthreads();
         double s2[NR];
+        if constexpr (sizeof(T) == 4) {
+            // FP32 factors: this product, the only work left on the chain's critical path,
+            // accumulates in FP32 (twice the FP64 rate) over four independent partial sums;
+            // its rounding stays far below that of the FP32 factors themselves
+            float f2[NR][4];
 #pragma unroll
-        for (int r = 0; r < NR; ++r) s2[r] = 0.0;
+            for (int r = 0; r < NR; ++r)
 #pragma unroll
-        for (int c = 0; c < 64; ++c) {
-            const double mv = (double)mk[c];
+                for (int u = 0; u < 4; ++u) f2[r][u] = 0.0f;
 #pragma unroll
-            for (int r = 0; r < NR; ++r) s2[r] += mv * (double)Xs[0][r][hf * 64 + c];
+            for (int c = 0; c < 64; ++c) {
+#pragma unroll
+                for (int r = 0; r < NR; ++r) f2[r][c & 3] = fmaf(mk[c], Xs[0][r][hf * 64 + c], f2[r][c & 3]);
+            }
+#pragma unroll
+            for (int r = 0; r < NR; ++r) s2[r] = (double)((f2[r][0] + f2[r][1]) + (f2[r][2] + f2[r][3]));
+        } else {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) s2[r] = 0.0;
+#pragma unroll
+            for (int c = 0; c < 64; ++c) {
+                const double mv = (double)mk[c];
+#pragma unroll
+                for (int r = 0; r < NR; ++r) s2[r] += mv * (double)Xs[0][r][hf * 64 + c];
+            }
         }
         __syncthreads();  // Xs read by every thread before part (its bytes) is written
 #pragma unroll
