@@ -93,15 +93,35 @@ __global__ void absmax_kernel_t(int m, int n, const T* __restrict__ A, long lda,
     if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(mx));
 }
 
-// bordered matrix of solve_free, src/swimmer.hpp:107-125: copy [Gs | Gc] into the first
-// ns rows, zero the last 6 rows, then the scaled force/torque rows
+// bordered matrix of solve_free, src/swimmer.hpp:107-125: copy [Gs | Gc] (ns x (ns + 6))
+// into the first ns rows, zero the last 6 rows, then the scaled force/torque rows. The copy
+// can also find max |Gs| (mx, as absmax_kernel_t), which sets the rows' scale: one pass over
+// G instead of two. Workgroups stride over columns, threads over rows, four loads in flight.
 template <typename T>
-__global__ void border_copy_kernel(int ns, const T* __restrict__ G, T* M) {
+__global__ void border_copy_kernel(int ns, const T* __restrict__ G, T* __restrict__ M, unsigned long long* mx) {
     const int n = ns + 6;
-    size_t total = (size_t)n * n;
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        size_t i = e % n, j = e / n;
-        M[e] = i < (size_t)ns ? G[i + j * ns] : T(0);
+    double m = 0.0;
+    for (int j = blockIdx.x; j < n; j += gridDim.x) {
+        const T* g = G + (size_t)j * ns;
+        T* out = M + (size_t)j * n;
+        for (int i0 = threadIdx.x; i0 < n; i0 += 4 * 256) {
+            T v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int i = i0 + 256 * u;
+                v[u] = i < ns ? g[i] : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int i = i0 + 256 * u;
+                if (i < n) out[i] = v[u];
+                if (j < ns) m = fmax(m, fabs((double)v[u]));
+            }
+        }
+    }
+    if (mx) {
+        for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+        if ((threadIdx.x & 63) == 0) atomicMax(mx, (unsigned long long)__double_as_longlong(m));
     }
 }
 template <typename T>
@@ -242,20 +262,6 @@ void axpy(size_t n, double alpha, const double* y, double* x) {
     check_launch("axpy");
 }
 
-template <typename T>
-static double absmax(int m, int n, const T* A, long lda) {
-    DBuf<unsigned long long> r(1);
-    r.zero();
-    unsigned blocks = (unsigned)std::max(1, std::min(n, 4096));
-    hipLaunchKernelGGL(absmax_kernel_t<T>, dim3(blocks), dim3(256), 0, ctx().s(), m, n, A, lda, r.p);
-    check_launch("absmax");
-    unsigned long long bits = 0;
-    r.download(&bits, 1);
-    double v;
-    std::memcpy(&v, &bits, sizeof v);
-    return v;
-}
-
 static void device_iota(DBuf<int>& b, int n) {
     b.alloc(n);
     hipLaunchKernelGGL(iota_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx().s(), n, b.p);
@@ -306,14 +312,35 @@ template void absmax_async<double>(int, int, const double*, long, unsigned long 
 template void absmax_async<float>(int, int, const float*, long, unsigned long long*);
 
 template <typename T>
-void border_build(int Ns, const T* G, T* M, const double* cen_xyz, const double* area, double sc, const double* xc,
-                  const double* sc_dev) {
-    const int ns = 3 * Ns, n = ns + 6;
-    hipLaunchKernelGGL(border_copy_kernel<T>, dim3(std::min<unsigned>(ceil_div((size_t)n * n, 256), 65535)), dim3(256),
-                       0, ctx().s(), ns, G, M);
+static void border_copy(int ns, const T* G, T* M, unsigned long long* mx) {
+    hipLaunchKernelGGL(border_copy_kernel<T>, dim3((unsigned)std::min(ns + 6, 4096)), dim3(256), 0, ctx().s(), ns, G, M,
+                       mx);
+    check_launch("border_copy");
+}
+template <typename T>
+static void border_rows(int Ns, T* M, const double* cen_xyz, const double* area, double sc, const double* xc,
+                        const double* sc_dev) {
     hipLaunchKernelGGL(border_rows_kernel<T>, dim3(ceil_div(Ns, 256)), dim3(256), 0, ctx().s(), Ns, M, cen_xyz, area, sc,
                        sc_dev, xc[0], xc[1], xc[2]);
-    check_launch("border");
+    check_launch("border_rows");
+}
+template <typename T>
+void border_build(int Ns, const T* G, T* M, const double* cen_xyz, const double* area, double sc, const double* xc,
+                  const double* sc_dev) {
+    border_copy<T>(3 * Ns, G, M, nullptr);
+    border_rows<T>(Ns, M, cen_xyz, area, sc, xc, sc_dev);
+}
+// [Gs | Gc] into the bordered matrix M (n = ns + 6) and max |Gs|, on the host, from the same pass
+template <typename T>
+static double border_copy_absmax(int ns, const T* G, T* M) {
+    DBuf<unsigned long long> r(1);
+    r.zero();
+    border_copy<T>(ns, G, M, r.p);
+    unsigned long long bits = 0;
+    r.download(&bits, 1);
+    double v;
+    std::memcpy(&v, &bits, sizeof v);
+    return v;
 }
 template void border_build<double>(int, const double*, double*, const double*, const double*, double, const double*,
                                    const double*);
@@ -622,15 +649,16 @@ static void swimmer_build(sqbem_swimmer* s) {
         }
         // bordered low-precision factors, reused by every refinement step
         const int n = ns + 6;
-        double amax = absmax<float>(ns, ns, s->G32.p, ns);
+        s->M32.alloc((size_t)n * n);
+        const double amax = border_copy_absmax<float>(ns, s->G32.p, s->M32.p);
         double Amax = 0;
         for (int p = 0; p < Ns; ++p) Amax = std::max(Amax, 5.0 * s->S.area[p]);
         s->sc32 = amax / Amax;
-        s->M32.alloc((size_t)n * n);
+        border_rows<float>(Ns, s->M32.p, s->dm.cen_xyz.p, s->dm.area.p, s->sc32, s->xc, nullptr);
+        // rebuilds M from G (the pivoting fallback)
         const std::function<void()> border = [&] {
             border_build<float>(Ns, s->G32.p, s->M32.p, s->dm.cen_xyz.p, s->dm.area.p, s->sc32, s->xc);
         };
-        border();
         s->Mpiv.alloc(n);
         s->Maux.alloc(getrf_aux_elems(n));
         DBuf<__bf16> Msh;  // BF16 copies of the bordered factors for its 16-bit products
@@ -660,16 +688,12 @@ static void reduced_rhs(sqbem_swimmer* s, int k, const double* u, double* rs, lo
 // force/torque rows scaled by max|Gs| / max(5 A), FP64
 static void bordered_factor_fp64(sqbem_swimmer* s, DBuf<double>& M, DBuf<int>& piv) {
     const int Ns = s->Ns, ns = s->ns, n = ns + 6;
-    double amax = absmax<double>(ns, ns, s->G.p, ns);
+    M.alloc((size_t)n * n);
+    const double amax = border_copy_absmax<double>(ns, s->G.p, M.p);
     double Amax = 0;
     for (int p = 0; p < Ns; ++p) Amax = std::max(Amax, 5.0 * s->S.area[p]);
     const double sc = amax / Amax;
-    M.alloc((size_t)n * n);
-    hipLaunchKernelGGL(border_copy_kernel<double>, dim3(std::min<unsigned>(ceil_div((size_t)n * n, 256), 65535)),
-                       dim3(256), 0, ctx().s(), ns, s->G.p, M.p);
-    hipLaunchKernelGGL(border_rows_kernel<double>, dim3(ceil_div(Ns, 256)), dim3(256), 0, ctx().s(), Ns, M.p,
-                       s->dm.cen_xyz.p, s->dm.area.p, sc, nullptr, s->xc[0], s->xc[1], s->xc[2]);
-    check_launch("border");
+    border_rows<double>(Ns, M.p, s->dm.cen_xyz.p, s->dm.area.p, sc, s->xc, nullptr);
     piv.alloc(n);
     int info = getrf<double>(SQBEM_FP64, n, M.p, n, piv.p);
     SQ_REQUIRE(info == 0, SQBEM_ERR_SINGULAR, "dgesv failed");
