@@ -166,10 +166,15 @@ __global__ void constraint_apply_kernel(int Ns, int nrhs, const double* __restri
     if (threadIdx.x < 6) out[threadIdx.x + ldo * k] = sign * red[threadIdx.x][0];
 }
 
+// exact widening of a matrix entry to FP64 (BF16 through FP32)
+__device__ __forceinline__ double widen(double v) { return v; }
+__device__ __forceinline__ double widen(float v) { return (double)v; }
+__device__ __forceinline__ double widen(__bf16 v) { return (double)(float)v; }
+
 // y = alpha A x + beta y for a few right-hand sides (nrhs <= 8), split over column chunks
 constexpr int GV_CHUNKS = 32;
-template <typename T>
-__global__ void gemv_part_kernel(int m, int n, int nrhs, const T* __restrict__ A, long lda, const T* __restrict__ X,
+template <typename T, typename TX = T>
+__global__ void gemv_part_kernel(int m, int n, int nrhs, const T* __restrict__ A, long lda, const TX* __restrict__ X,
                                  long ldx, double* __restrict__ part) {
     // rows row0 and row0 + 256 of a 512-row block (a column's 4 KB run per workgroup)
     const int row0 = blockIdx.x * 512 + threadIdx.x;
@@ -187,8 +192,8 @@ __global__ void gemv_part_kernel(int m, int n, int nrhs, const T* __restrict__ A
         double a0[8], a1[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            a0[u] = (double)A[row0 + (size_t)(j + u) * lda];
-            a1[u] = (double)A[row1 + (size_t)(j + u) * lda];
+            a0[u] = widen(A[row0 + (size_t)(j + u) * lda]);
+            a1[u] = widen(A[row1 + (size_t)(j + u) * lda]);
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u)
@@ -201,7 +206,7 @@ __global__ void gemv_part_kernel(int m, int n, int nrhs, const T* __restrict__ A
                 }
     }
     for (; j < jb; ++j) {
-        const double a0 = (double)A[row0 + (size_t)j * lda], a1 = (double)A[row1 + (size_t)j * lda];
+        const double a0 = widen(A[row0 + (size_t)j * lda]), a1 = widen(A[row1 + (size_t)j * lda]);
 #pragma unroll
         for (int r = 0; r < 8; ++r)
             if (r < nrhs) {
@@ -248,6 +253,81 @@ void gemv(int m, int n, int nrhs, double alpha, const T* A, long lda, const T* X
     hipLaunchKernelGGL(gemv_reduce_kernel<T>, dim3(ceil_div(m, 256)), dim3(256), 0, ctx().s(), m, nrhs, g_gv.p, alpha,
                        beta, Y, ldy);
     check_launch("gemv");
+}
+
+// The same product with a BF16 A (column-major) and FP32 x, y: the refinement's corrections
+// multiply by BF16 copies of D and Y (half the bytes of their FP32 forms; the corrections'
+// accuracy is set by the 16-bit factorisations, not by these products).
+void gemv_cols16(int m, int n, int nrhs, double alpha, const __bf16* A, long lda, const float* X, long ldx, double beta,
+                 float* Y, long ldy) {
+    if (m <= 0 || nrhs <= 0) return;
+    SQ_REQUIRE(nrhs <= 8 && n > 0, SQBEM_ERR_ARG, "gemv_cols16: 1 to 8 right-hand sides, n > 0");
+    size_t need = (size_t)GV_CHUNKS * nrhs * m;
+    DBuf<double>& g_gv = ctx().redirect ? g_gv_side : g_gv_main;
+    if (g_gv.n < need) g_gv.alloc(std::max<size_t>(need, 1 << 20));
+    ProfScope prof(PROF_GEMV, 2.0 * m * n * nrhs, 2.0 * m * n + 4.0 * ((double)n * nrhs + 2.0 * m * nrhs));
+    hipLaunchKernelGGL((gemv_part_kernel<__bf16, float>), dim3(ceil_div(m, 512), GV_CHUNKS), dim3(256), 0, ctx().s(),
+                       m, n, nrhs, A, lda, X, ldx, g_gv.p);
+    hipLaunchKernelGGL(gemv_reduce_kernel<float>, dim3(ceil_div(m, 256)), dim3(256), 0, ctx().s(), m, nrhs, g_gv.p,
+                       alpha, beta, Y, ldy);
+    check_launch("gemv_cols16");
+}
+
+// y(i, r) = alpha sum_j A(i, j) x(j, r) + beta y(i, r) for a row-major BF16 A (row i at
+// A + i lda, lda a multiple of 8, A 16-byte aligned): one wave per row, 16-byte loads of 8
+// entries a lane, FP64 sums (nrhs <= 8)
+__global__ void __launch_bounds__(256) gemv_rows16_kernel(int m, int n, int nrhs, const __bf16* __restrict__ A,
+                                                          long lda, const float* __restrict__ X, long ldx,
+                                                          double alpha, double beta, float* Y, long ldy) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (row >= m) return;  // whole waves
+    const __bf16* a = A + (size_t)row * lda;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int n8 = n & ~7;
+    for (int j = lane * 8; j < n8; j += 512) {
+        const uint4 w = *reinterpret_cast<const uint4*>(a + j);
+        const unsigned u[4] = {w.x, w.y, w.z, w.w};
+        float av[8];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            av[2 * e] = __uint_as_float(u[e] << 16);
+            av[2 * e + 1] = __uint_as_float(u[e] & 0xffff0000u);
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+            if (r < nrhs) {
+                const float* x = X + j + (size_t)r * ldx;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) acc[r] += (double)av[e] * (double)x[e];
+            }
+    }
+    for (int j = n8 + lane; j < n; j += 64) {
+        const double av = (double)(float)a[j];
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+            if (r < nrhs) acc[r] += av * (double)X[j + (size_t)r * ldx];
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        if (r >= nrhs) break;
+        double v = acc[r];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) {
+            float* y = Y + row + (size_t)r * ldy;
+            *y = (float)(beta == 0.0 ? alpha * v : alpha * v + beta * (double)*y);
+        }
+    }
+}
+void gemv_rows16(int m, int n, int nrhs, double alpha, const __bf16* A, long lda, const float* X, long ldx,
+                 double beta, float* Y, long ldy) {
+    if (m <= 0 || nrhs <= 0) return;
+    SQ_REQUIRE(nrhs <= 8 && n > 0 && lda % 8 == 0 && ((uintptr_t)A & 15) == 0, SQBEM_ERR_ARG,
+               "gemv_rows16: 1 to 8 right-hand sides, n > 0, 16-byte aligned rows");
+    ProfScope prof(PROF_GEMV, 2.0 * m * n * nrhs, 2.0 * m * n + 4.0 * ((double)n * nrhs + 2.0 * m * nrhs));
+    hipLaunchKernelGGL(gemv_rows16_kernel, dim3(ceil_div(m, 4)), dim3(256), 0, ctx().s(), m, n, nrhs, A, lda, X, ldx,
+                       alpha, beta, Y, ldy);
+    check_launch("gemv_rows16");
 }
 
 // Z = X + alpha Y elementwise over n values
@@ -500,6 +580,8 @@ struct sqbem_swimmer {
     DBuf<float> Y32, G32, D32, M32;    // mixed: low-precision Schur pieces and bordered factors
     DBuf<__bf16> D16;                  // BF16 mode: D row-major in BF16 (the Schur product's A)
     int D16ld = 0;
+    DBuf<__bf16> Y16;                  // BF16 mode: Y column-major in BF16, as the solves finalise it
+    int Y16ld = 0;
     DBuf<int> Mpiv;
     DBuf<float> Maux;                  // getrf's solve aid for the M32 factors
     DBuf<float> Gs32;                  // mixed, towed problem: factors of Gs alone
@@ -632,13 +714,18 @@ static void swimmer_build(sqbem_swimmer* s) {
             // BF16: the 16-bit products of the solves and of D Y read BF16 copies of their
             // operands (the tube's factors, the solved right-hand sides as the solves'
             // leaves finalise them, D) instead of converting them
+            // (Y's copy is kept: the refinement's corrections multiply by it)
             const bool sh = kind == SQBEM_BF16_IR && tb->np_path == 1 && tb->SL.p;
-            DBuf<__bf16> Yh;
             const int yld = (nt + 63) / 64 * 64;
-            if (sh) Yh.alloc((size_t)yld * (ns + 6));
+            s->Y16.release();
+            s->Y16ld = 0;
+            if (sh) {
+                s->Y16.alloc((size_t)yld * (ns + 6));
+                s->Y16ld = yld;
+            }
             {
                 ShadowScope f(sh ? tb->T32.p : nullptr, nt, nt, nt, tb->SL.p, tb->SU.p, tb->sh_ld, true, tb->SUr.p);
-                ShadowScope y(sh ? s->Y32.p : nullptr, nt, nt, ns + 6, nullptr, Yh.p, yld, false);
+                ShadowScope y(sh ? s->Y32.p : nullptr, nt, nt, ns + 6, nullptr, s->Y16.p, yld, false);
                 ShadowScope d(sh ? s->D32.p : nullptr, ns, ns, nt, s->D16.p, nullptr, s->D16ld, false);
                 getrs<float>(kind, 0, nt, ns + 6, tb->T32.p, nt, tb->ipiv.p, s->Y32.p, nt, tb->aux32.p);
                 if (kind == SQBEM_FP32_IR)
@@ -840,13 +927,23 @@ static void approx_solve(sqbem_swimmer* s, int k, const double* r, double* d, bo
     }
     rf.wait();
     convert_matrix<double, float>(nv, k, r + nt, N, v32.p, n);
-    if (nt) gemv<float>(ns, nt, k, -1.0, s->D32.p, ns, z32.p, nt, 1.0, v32.p, n);
+    if (nt) {
+        if (s->D16.p)
+            gemv_rows16(ns, nt, k, -1.0, s->D16.p, s->D16ld, z32.p, nt, 1.0, v32.p, n);
+        else
+            gemv<float>(ns, nt, k, -1.0, s->D32.p, ns, z32.p, nt, 1.0, v32.p, n);
+    }
     if (towed)
         getrs<float>(kind, 0, ns, k, s->Gs32.p, ns, s->Gspiv.p, v32.p, n, s->Gsaux.p);
     else
         getrs<float>(kind, 0, n, k, s->M32.p, n, s->Mpiv.p, v32.p, n, s->Maux.p);
     // f_t = z - Y [f_s; w]   (towed: w is not an unknown, only the f_s columns of Y)
-    if (nt) gemv<float>(nt, nv, k, -1.0, s->Y32.p, nt, v32.p, n, 1.0, z32.p, nt);
+    if (nt) {
+        if (s->Y16.p)
+            gemv_cols16(nt, nv, k, -1.0, s->Y16.p, s->Y16ld, v32.p, n, 1.0, z32.p, nt);
+        else
+            gemv<float>(nt, nv, k, -1.0, s->Y32.p, nt, v32.p, n, 1.0, z32.p, nt);
+    }
     fill_zero(d, sizeof(double) * (size_t)N * k);
     if (nt) convert_matrix<float, double>(nt, k, z32.p, nt, d, N);
     convert_matrix<float, double>(nv, k, v32.p, n, d + nt, N);
