@@ -964,12 +964,15 @@ __global__ void __launch_bounds__(256) col_norm2_kernel(int N, const double* __r
 }
 
 // the 2-norms of the k columns of a, on the device; only k values cross to the host
-static void col_norms(int N, int k, const double* a, std::vector<double>& out) {
-    DBuf<double> d(k);
-    hipLaunchKernelGGL(col_norm2_kernel, dim3(k), dim3(256), 0, ctx().s(), N, a, d.p);
+// the squared 2-norms of the k columns of a into sq (device, k values), without waiting
+static void col_norms_launch(int N, int k, const double* a, double* sq) {
+    hipLaunchKernelGGL(col_norm2_kernel, dim3(k), dim3(256), 0, ctx().s(), N, a, sq);
     check_launch("col_norm2");
-    out.assign(k, 0.0);
-    d.download(out.data(), k);
+}
+// the first cnt of them on the host, as norms
+static void col_norms_read(const DBuf<double>& sq, int cnt, std::vector<double>& out) {
+    out.assign(cnt, 0.0);
+    sq.download(out.data(), cnt);
     sync();
     for (double& v : out) v = std::sqrt(v);
 }
@@ -983,9 +986,13 @@ static void solve_mixed(sqbem_swimmer* s, int k, const double* u_host, bool towe
     if (!towed) {
         DBuf<double> u((size_t)ns * k);
         u.upload(u_host, (size_t)ns * k);
-        for (int j = 0; j < k; ++j) {
-            if (nt) gemv<double>(nt, ns, 1, -1.0, s->AuBt.p, nt, u.p + (size_t)j * ns, ns, 0.0, b.p + (size_t)j * N, N);
-            gemv<double>(ns, ns, 1, -1.0, s->AuBs.p, ns, u.p + (size_t)j * ns, ns, 0.0, b.p + (size_t)j * N + nt, N);
+        // every slip field in one pass over AuBt and AuBs
+        for (int j0 = 0; j0 < k; j0 += 8) {
+            const int kk = std::min(8, k - j0);
+            const double* uj = u.p + (size_t)j0 * ns;
+            double* bj = b.p + (size_t)j0 * N;
+            if (nt) gemv<double>(nt, ns, kk, -1.0, s->AuBt.p, nt, uj, ns, 0.0, bj, N);
+            gemv<double>(ns, ns, kk, -1.0, s->AuBs.p, ns, uj, ns, 0.0, bj + nt, N);
         }
     } else {
         SQ_REQUIRE(k == 1, SQBEM_ERR_ARG, "towed solve takes one right-hand side");
@@ -1006,8 +1013,10 @@ static void solve_mixed(sqbem_swimmer* s, int k, const double* u_host, bool towe
         if (nt) gemv<double>(nt, 6, 1, -1.0, s->Ct.p, nt, w.p, 6, 0.0, b.p, N);
         gemv<double>(ns, 6, 1, -1.0, s->Cs.p, ns, w.p, 6, 0.0, b.p + nt, N);
     }
-    std::vector<double> bn, rn;
-    col_norms(N, k, b.p, bn);
+    // norms of b (first k) and of the residual (next k): b's are read with the first check's
+    std::vector<double> nrm;
+    DBuf<double> nrm_d((size_t)2 * k);
+    col_norms_launch(N, k, b.p, nrm_d.p);
     convert_matrix<double, double>(N, k, b.p, N, r.p, N);
     const double tol = SQBEM_IR_TOL;
     // The residual is first read back after min_check steps, so the steps before run without
@@ -1024,9 +1033,10 @@ static void solve_mixed(sqbem_swimmer* s, int k, const double* u_host, bool towe
         full_residual(s, k, x.p, b.p, r.p, towed, c.p, &rf);
         if (it + 1 < min_check) continue;
         rf.wait();
-        col_norms(N, k, r.p, rn);
+        col_norms_launch(N, k, r.p, nrm_d.p + k);
+        col_norms_read(nrm_d, 2 * k, nrm);
         worst = 0;
-        for (int j = 0; j < k; ++j) worst = std::max(worst, bn[j] > 0 ? rn[j] / bn[j] : rn[j]);
+        for (int j = 0; j < k; ++j) worst = std::max(worst, nrm[j] > 0 ? nrm[k + j] / nrm[j] : nrm[k + j]);
         if (worst <= tol) {
             ++it;
             break;
