@@ -309,6 +309,9 @@ void build_shared_tube(SharedTube& S, const bem::Mesh& mesh, double mu, int prec
     S.t->prec = prec;
     tube_build(S.t.get());
     tube_ready(S.t.get());
+    // the tiled copy for the chained solves, beside what follows
+    chain_tiles_issue(S.t->aux32.p);
+    chain_tiles_issue(S.t->aux64.p);
     if (mixed(prec)) {
         const int nt = S.t->nt;
         S.inv.alloc((size_t)nt * nt);

@@ -4000,6 +4000,18 @@ static void chain_prepare(int n, const T* A, int lda, T* aux, hipStream_t stream
     check_launch("chain_premul");
 }
 
+namespace {
+std::map<const void*, std::function<void()>> g_pending_tiles;
+}
+void chain_tiles_issue(const void* aux) {
+    auto it = g_pending_tiles.find(aux);
+    if (it == g_pending_tiles.end()) return;
+    const std::function<void()> fn = std::move(it->second);
+    g_pending_tiles.erase(it);
+    fn();
+}
+void chain_tiles_forget(const void* aux) { g_pending_tiles.erase(aux); }
+
 // ------------------------------------------------------------------ recursive LU
 template <typename T>
 static void getrf_rec(int kind, T* A, int lda, int m, int w, int* ipiv, int base) {
@@ -4088,9 +4100,17 @@ int getrf_end(GetrfJob<T>& j) {
             check_launch("iota_pivots");
             if (aux) {
                 // the tiled copy is read only by the chained solves (which wait for it):
-                // made on the side stream, beside what follows on the library stream
-                side_defer();
-                chain_prepare<T>(n, A, lda, aux, ctx().side);
+                // made on the side stream, beside what follows on the library stream (now,
+                // or when the caller issues it)
+                if (j.defer_tiles) {
+                    g_pending_tiles[aux] = [n, A, lda, aux] {
+                        side_defer();
+                        chain_prepare<T>(n, A, lda, aux, ctx().side);
+                    };
+                } else {
+                    side_defer();
+                    chain_prepare<T>(n, A, lda, aux, ctx().side);
+                }
             }
             j.backup.release();  // stream-ordered reuse only
             g_last_getrf_path = 1;
@@ -4183,6 +4203,7 @@ void getrs(int kind, int trans, int n, int nrhs, const T* LU, int lda, const int
         const T* Linv = aux;
         const T* Uinv = aux + (size_t)nb * NB * NB;
         if (nrhs <= 8) {
+            chain_tiles_issue(aux);  // a tiled copy left pending by the factorisation
             const T* Lt = aux + ninv;
             const T* ML = Lt + (size_t)nb * nb * NB * NB;
             trsv_inv<T, true>(n, nrhs, Lt, Linv, ML, B, ldb);

@@ -58,6 +58,9 @@ struct GetrfJob {
     __bf16* sh_upr = nullptr;  // optional: the part above the diagonal blocks again, row-major
     int sh_ld = 0;
     int path = 0;
+    // the tiled copy of the factors for the chained solves is not made by getrf_end but left
+    // pending (chain_tiles_issue), so that it can run beside later latency-bound work
+    bool defer_tiles = false;
     DBuf<int> flag;
     DBuf<T> backup, inv;
     bool started = false;
@@ -66,6 +69,11 @@ template <typename T>
 void getrf_begin(GetrfJob<T>& job);
 template <typename T>
 int getrf_end(GetrfJob<T>& job);
+// A pending tiled copy (GetrfJob::defer_tiles) of the factors whose solve aid is aux: issue
+// starts it on the side stream (no-op when none is pending; getrs issues it itself before a
+// chained solve needs it); forget drops it, before the factors or aux are released.
+void chain_tiles_issue(const void* aux);
+void chain_tiles_forget(const void* aux);
 inline size_t getrf_aux_elems(int n) {
     const size_t nb = (size_t)((n + 127) / 128);
     return (4 * nb + nb * nb) * 128 * 128;

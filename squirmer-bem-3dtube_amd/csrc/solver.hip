@@ -471,6 +471,9 @@ void sq::tube_build(sqbem_tube* t) {
         j.A = t->T32.p;
         j.ipiv = t->ipiv.p;
         j.aux = t->aux32.p;
+        // the chained solves' tiled copy is made when a swimmer's Schur complement is done
+        // (beside its bordered factorisation) or when a solve first needs it
+        j.defer_tiles = true;
         // the pivoting fallback rebuilds the FP32 matrix from the FP64 one (no copy is kept)
         j.restore = [t, nt] { convert_matrix<double, float>(nt, nt, t->T64.p, nt, t->T32.p, nt); };
         if (t->prec == SQBEM_BF16_IR && nt >= 512) {
@@ -495,6 +498,7 @@ void sq::tube_build(sqbem_tube* t) {
         j.A = t->T64.p;
         j.ipiv = t->ipiv.p;
         j.aux = t->aux64.p;
+        j.defer_tiles = true;
         getrf_begin<double>(j);
     }
     t->t_factor0 = t1;
@@ -707,6 +711,7 @@ static void swimmer_build(sqbem_swimmer* s) {
         if (tb) {
             getrs<double>(SQBEM_FP64, 0, nt, ns + 6, tb->T64.p, nt, tb->ipiv.p, s->Y.p, nt, tb->aux64.p);
             dgemm(ns, ns + 6, nt, -1.0, s->D.p, ns, s->Y.p, nt, 1.0, s->G.p, ns);
+            chain_tiles_issue(tb->aux64.p);  // for the solves, after the Schur complement's products
         }
     } else {
         const int kind = s->prec;
@@ -742,6 +747,9 @@ static void swimmer_build(sqbem_swimmer* s) {
         for (int p = 0; p < Ns; ++p) Amax = std::max(Amax, 5.0 * s->S.area[p]);
         s->sc32 = amax / Amax;
         border_rows<float>(Ns, s->M32.p, s->dm.cen_xyz.p, s->dm.area.p, s->sc32, s->xc, nullptr);
+        // the tube factors' tiled copy for the refinement's solves: on the side stream beside the
+        // bordered factorisation (latency-bound), after the Schur complement's products
+        if (tb) chain_tiles_issue(tb->aux32.p);
         // rebuilds M from G (the pivoting fallback)
         const std::function<void()> border = [&] {
             border_build<float>(Ns, s->G32.p, s->M32.p, s->dm.cen_xyz.p, s->dm.area.p, s->sc32, s->xc);

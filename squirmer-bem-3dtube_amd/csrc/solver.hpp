@@ -36,6 +36,8 @@ struct sqbem_tube {
     ~sqbem_tube() {
         // a factorisation still running writes into the buffers released below
         if (job32 || job64) (void)hipStreamSynchronize(sq::ctx().s());
+        sq::chain_tiles_forget(aux32.p);  // a tiled copy never issued
+        sq::chain_tiles_forget(aux64.p);
         sq::side_drain();  // a tiled copy of the factors may still be written there
     }
 };
