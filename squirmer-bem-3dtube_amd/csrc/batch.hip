@@ -137,8 +137,13 @@ __global__ void gemv_batch_reduce(const GvJob* __restrict__ jobs, const double* 
     const GvJob& g = jobs[blockIdx.y];
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= g.m) return;
+    // every chunk's load issued before the (in-order) sum
+    double v[GV_CH];
+#pragma unroll
+    for (int c = 0; c < GV_CH; ++c) v[c] = part[g.poff + (size_t)c * g.m + row];
     double s = 0.0;
-    for (int c = 0; c < GV_CH; ++c) s += part[g.poff + (size_t)c * g.m + row];
+#pragma unroll
+    for (int c = 0; c < GV_CH; ++c) s += v[c];
     T* y = (T*)g.y;
     y[row] = (T)((double)y[row] + g.alpha * s);
 }

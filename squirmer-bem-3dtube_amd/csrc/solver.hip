@@ -220,17 +220,21 @@ __global__ void gemv_part_kernel(int m, int n, int nrhs, const T* __restrict__ A
         if (two) part[((size_t)chunk * nrhs + r) * m + row1] = acc[1][r];
     }
 }
+// right-hand side blockIdx.y: the chunks' partial sums of a row, all loads issued before the
+// (in-order) sum
 template <typename T>
 __global__ void gemv_reduce_kernel(int m, int nrhs, const double* __restrict__ part, double alpha, double beta, T* Y,
                                    long ldy) {
-    int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= m) return;
-    for (int r = 0; r < nrhs; ++r) {
-        double s = 0;
-        for (int k = 0; k < GV_CHUNKS; ++k) s += part[((size_t)k * nrhs + r) * m + row];
-        T* y = Y + row + (size_t)r * ldy;
-        *y = (T)(beta == 0.0 ? alpha * s : alpha * s + beta * (double)*y);
-    }
+    const int row = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+    if (row >= m || r >= nrhs) return;
+    double v[GV_CHUNKS];
+#pragma unroll
+    for (int k = 0; k < GV_CHUNKS; ++k) v[k] = part[((size_t)k * nrhs + r) * m + row];
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < GV_CHUNKS; ++k) s += v[k];
+    T* y = Y + row + (size_t)r * ldy;
+    *y = (T)(beta == 0.0 ? alpha * s : alpha * s + beta * (double)*y);
 }
 
 // partial sums of the column chunks, one buffer per stream (products may run on both at once)
@@ -250,7 +254,7 @@ void gemv(int m, int n, int nrhs, double alpha, const T* A, long lda, const T* X
     ProfScope prof(PROF_GEMV, 2.0 * m * n * nrhs, sizeof(T) * ((double)m * n + (double)n * nrhs + 2.0 * m * nrhs));
     hipLaunchKernelGGL(gemv_part_kernel<T>, dim3(ceil_div(m, 512), GV_CHUNKS), dim3(256), 0, ctx().s(), m, n, nrhs, A,
                        lda, X, ldx, g_gv.p);
-    hipLaunchKernelGGL(gemv_reduce_kernel<T>, dim3(ceil_div(m, 256)), dim3(256), 0, ctx().s(), m, nrhs, g_gv.p, alpha,
+    hipLaunchKernelGGL(gemv_reduce_kernel<T>, dim3(ceil_div(m, 256), nrhs), dim3(256), 0, ctx().s(), m, nrhs, g_gv.p, alpha,
                        beta, Y, ldy);
     check_launch("gemv");
 }
@@ -268,7 +272,7 @@ void gemv_cols16(int m, int n, int nrhs, double alpha, const __bf16* A, long lda
     ProfScope prof(PROF_GEMV, 2.0 * m * n * nrhs, 2.0 * m * n + 4.0 * ((double)n * nrhs + 2.0 * m * nrhs));
     hipLaunchKernelGGL((gemv_part_kernel<__bf16, float>), dim3(ceil_div(m, 512), GV_CHUNKS), dim3(256), 0, ctx().s(),
                        m, n, nrhs, A, lda, X, ldx, g_gv.p);
-    hipLaunchKernelGGL(gemv_reduce_kernel<float>, dim3(ceil_div(m, 256)), dim3(256), 0, ctx().s(), m, nrhs, g_gv.p,
+    hipLaunchKernelGGL(gemv_reduce_kernel<float>, dim3(ceil_div(m, 256), nrhs), dim3(256), 0, ctx().s(), m, nrhs, g_gv.p,
                        alpha, beta, Y, ldy);
     check_launch("gemv_cols16");
 }
