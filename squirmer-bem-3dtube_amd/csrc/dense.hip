@@ -23,6 +23,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <climits>
 #include <map>
 #include <memory>
 #include <string>
@@ -2875,6 +2876,19 @@ static void trsm_inv(int kind, int n, int nrhs, const T* L, int ldl, const T* in
     }
 }
 
+// Columns g_cols_from and beyond of the matrix being factored are written by work on another
+// stream that completes at g_cols_ev: the no-pivot recursion waits for it before it first
+// touches such a column (GetrfJob::cols_event).
+static hipEvent_t g_cols_ev = nullptr;
+static long g_cols_from = LONG_MAX;
+static void cols_needed(long upto) {
+    if (g_cols_from >= upto) return;
+    const hipEvent_t ev = g_cols_ev;
+    g_cols_ev = nullptr;
+    g_cols_from = LONG_MAX;
+    SQ_HIP(hipStreamWaitEvent(ctx().s(), ev, 0));
+}
+
 // ------------------------------------------------------------------ no-pivot blocked LU
 // Matrices of this code's operators never need row interchanges (the multipliers of
 // partial pivoting stay below ~0.7), so getrf first factors without pivoting in
@@ -3396,6 +3410,7 @@ __global__ void __launch_bounds__(256) l21_u12_kernel(int m, int w2, T* A, int l
 
 template <typename T>
 static void getrf_np(int kind, T* A, int lda, int m, int w, int base, T* Linv, T* Uinv, int* flag) {
+    if (w <= 2 * NB) cols_needed((long)base + w);  // a leaf or a pair of leaves: its own columns
     if (w > NB && w <= 2 * NB && m > NB) {
         // a pair of leaves: the first diagonal block, its L21 and U12 together, the update,
         // then the second leaf
@@ -3446,6 +3461,7 @@ static void getrf_np(int kind, T* A, int lda, int m, int w, int base, T* Linv, T
     const int w1 = split_point(w, NB), w2 = w - w1;
     getrf_np<T>(kind, A, lda, m, w1, base, Linv, Uinv, flag);
     T* A12 = A + (size_t)w1 * lda;
+    cols_needed((long)base + w);  // the right part's columns from here on
     trsm_inv<T, true, NB>(kind, w1, w2, A, lda, Linv + (size_t)(base / NB) * NB * NB, A12, lda);
     gemm_kind<T>(kind, m - w1, w2, w1, T(-1), A + w1, lda, A12, lda, T(1), A12 + w1, lda);
     getrf_np<T>(kind, A12 + w1, lda, m - w1, w2, base + w1, Linv, Uinv, flag);
@@ -4053,6 +4069,12 @@ void getrf_begin(GetrfJob<T>& j) {
     j.started = true;
     if (j.n <= 0) return;
     SQ_REQUIRE(j.lda >= j.n, SQBEM_ERR_ARG, "getrf: lda < n");
+    if (j.cols_event) {
+        g_cols_ev = j.cols_event;
+        g_cols_from = j.cols_from;
+        // the pivoted path, and the copy kept for it, read every column at once
+        if (j.n < NP_MIN || !j.restore) cols_needed(LONG_MAX);
+    }
     if (j.n < NP_MIN) return;
     const int n = j.n;
     const size_t nb = (size_t)(n + NB - 1) / NB;
@@ -4077,7 +4099,9 @@ void getrf_begin(GetrfJob<T>& j) {
                                      j.sh_upr));
     }
     getrf_np<T>(j.kind, j.A, j.lda, n, n, 0, Linv, Linv + nb * NB * NB, j.flag.p);
+    cols_needed(LONG_MAX);  // nothing later in the stream may run before those columns are whole
 }
+int getrf_top_split(int n) { return split_point(n, NB); }
 
 template <typename T>
 int getrf_end(GetrfJob<T>& j) {

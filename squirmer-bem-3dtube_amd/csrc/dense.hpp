@@ -61,12 +61,19 @@ struct GetrfJob {
     // the tiled copy of the factors for the chained solves is not made by getrf_end but left
     // pending (chain_tiles_issue), so that it can run beside later latency-bound work
     bool defer_tiles = false;
+    // columns cols_from .. n-1 of A are still being written by work that completes at
+    // cols_event (another stream): the factorisation waits for it before it first touches them
+    hipEvent_t cols_event = nullptr;
+    int cols_from = 0;
     DBuf<int> flag;
     DBuf<T> backup, inv;
     bool started = false;
 };
 template <typename T>
 void getrf_begin(GetrfJob<T>& job);
+// the first split of the no-pivot factorisation of order n (its left part, columns
+// 0 .. split-1, is factored before any later column is touched)
+int getrf_top_split(int n);
 template <typename T>
 int getrf_end(GetrfJob<T>& job);
 // A pending tiled copy (GetrfJob::defer_tiles) of the factors whose solve aid is aux: issue

@@ -450,8 +450,7 @@ void sq::tube_build(sqbem_tube* t) {
     t->nt = 3 * t->Nt;
     const int Nt = t->Nt, nt = t->nt;
     t->dm.upload(t->mesh);
-    DBuf<int> self;
-    device_iota(self, Nt);
+    device_iota(t->self, Nt);
     t->T64.alloc((size_t)nt * nt);
     // the refined modes factor an FP32 copy, written by the assembly beside the FP64 matrix
     if (mixed(t->prec)) t->T32.alloc((size_t)nt * nt);
@@ -460,8 +459,33 @@ void sq::tube_build(sqbem_tube* t) {
     double At = 0;
     for (int p = 0; p < Nt; ++p) At += t->mesh.area[p];
     const double sc = 1.0 / (bem::EIGHT_PI * t->mu * std::sqrt(At));
-    assemble(t->dm, 0, Nt, t->dm.cen_xyz.p, Nt, self.p, bem::EIGHT_PI * t->mu, t->T64.p, nt, 1.0, nullptr, 0, nullptr,
-             sc, t->dm.nrm_xyz.p, t->T32.p, nt);
+    // Large refined tubes: the panels whose columns lie beyond the no-pivot factorisation's
+    // first split are assembled on the side stream, beside the factorisation of the left half
+    // (which reads and writes only its own columns); the factorisation waits for them before
+    // it first touches one. The assembly allocates nothing, so the side stream holds no block
+    // the library stream could take.
+    const bool split = mixed(t->prec) && nt >= 8192 && ctx().side && !ctx().redirect && !capturing();
+    const int Ps = split ? std::min(Nt, (getrf_top_split(nt) + 2) / 3) : Nt;
+    assemble(t->dm, 0, Ps, t->dm.cen_xyz.p, Nt, t->self.p, bem::EIGHT_PI * t->mu, t->T64.p, nt, 1.0, nullptr, 0,
+             nullptr, sc, t->dm.nrm_xyz.p, t->T32.p, nt);
+    if (split && Ps < Nt) {
+        if (!t->fork_ev) SQ_HIP(hipEventCreateWithFlags(&t->fork_ev, hipEventDisableTiming));
+        if (!t->asm_ev) SQ_HIP(hipEventCreateWithFlags(&t->asm_ev, hipEventDisableTiming));
+        SQ_HIP(hipEventRecord(t->fork_ev, ctx().s()));  // after the uploads
+        SQ_HIP(hipStreamWaitEvent(ctx().side, t->fork_ev, 0));
+        ctx().redirect = ctx().side;
+        try {
+            assemble(t->dm, Ps, Nt - Ps, t->dm.cen_xyz.p, Nt, t->self.p, bem::EIGHT_PI * t->mu,
+                     t->T64.p + (size_t)3 * Ps * nt, nt, 1.0, nullptr, 0, nullptr, sc, t->dm.nrm_xyz.p,
+                     t->T32.p + (size_t)3 * Ps * nt, nt);
+        } catch (...) {
+            ctx().redirect = nullptr;
+            side_drain();
+            throw;
+        }
+        ctx().redirect = nullptr;
+        SQ_HIP(hipEventRecord(t->asm_ev, ctx().side));
+    }
     sync();
     double t1 = now_s();
     t->ipiv.alloc(nt);
@@ -475,6 +499,10 @@ void sq::tube_build(sqbem_tube* t) {
         j.A = t->T32.p;
         j.ipiv = t->ipiv.p;
         j.aux = t->aux32.p;
+        if (Ps < Nt) {  // columns 3 Ps .. nt-1 come from the side stream's assembly
+            j.cols_event = t->asm_ev;
+            j.cols_from = 3 * Ps;
+        }
         // the chained solves' tiled copy is made when a swimmer's Schur complement is done
         // (beside its bordered factorisation) or when a solve first needs it
         j.defer_tiles = true;

@@ -19,6 +19,10 @@ struct sqbem_tube {
     sq::DBuf<double> T64;  // FP64 mode: LU factors; mixed: the FP64 operator
     sq::DBuf<float> T32;   // mixed: FP32 LU factors
     sq::DBuf<int> ipiv;
+    sq::DBuf<int> self;      // panel index of each collocation point (the assembly's self pairs)
+    // large refined tubes: the columns beyond the factorisation's first split are assembled on
+    // the side stream (fork_ev: its start, asm_ev: its end), beside the left half's factorisation
+    hipEvent_t fork_ev = nullptr, asm_ev = nullptr;
     sq::DBuf<double> aux64;  // getrf's solve aid for the factors (inverted blocks, tiled copy)
     sq::DBuf<float> aux32;
     // BF16 mode: BF16 copies of the factors below / above the diagonal blocks (row-major /
@@ -38,7 +42,9 @@ struct sqbem_tube {
         if (job32 || job64) (void)hipStreamSynchronize(sq::ctx().s());
         sq::chain_tiles_forget(aux32.p);  // a tiled copy never issued
         sq::chain_tiles_forget(aux64.p);
-        sq::side_drain();  // a tiled copy of the factors may still be written there
+        sq::side_drain();  // a tiled copy of the factors (or an assembly) may still run there
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+        if (asm_ev) (void)hipEventDestroy(asm_ev);
     }
 };
 
