@@ -52,24 +52,28 @@ struct Mesh {
         centroid.assign(n, {});
         area.assign(n, 0.0);
         h.assign(n, 0.0);
-        for (int p = 0; p < n; ++p) {
-            const auto& t = tris[p];
-            V[p] = {nodes[t[0]], nodes[t[1]], nodes[t[2]]};
-            const V3 twice = cross(V[p][1] - V[p][0], V[p][2] - V[p][0]);
-            const double A = 0.5 * norm(twice);
-            area[p] = A;
-            normal[p] = (1.0 / (2 * A)) * twice;
-            centroid[p] = (1.0 / 3) * (V[p][0] + V[p][1] + V[p][2]);
-            h[p] = std::sqrt(2 * A);
-        }
+        for (int p = 0; p < n; ++p) panel_geometry(p);
+    }
+    void panel_geometry(int p) {
+        const auto& t = tris[p];
+        V[p] = {nodes[t[0]], nodes[t[1]], nodes[t[2]]};
+        const V3 twice = cross(V[p][1] - V[p][0], V[p][2] - V[p][0]);
+        const double A = 0.5 * norm(twice);
+        area[p] = A;
+        normal[p] = (1.0 / (2 * A)) * twice;
+        centroid[p] = (1.0 / 3) * (V[p][0] + V[p][1] + V[p][2]);
+        h[p] = std::sqrt(2 * A);
     }
 
-    // reverse the panels whose normal points against outward(centroid)
+    // reverse the panels whose normal points against outward(centroid); the geometry of the
+    // reversed panels is formed again (the same values a full pass would give)
     void orient(const std::function<V3(V3)>& outward) {
         geometry();
         for (int p = 0; p < P(); ++p)
-            if (dot(normal[p], outward(centroid[p])) < 0) std::swap(tris[p][1], tris[p][2]);
-        geometry();
+            if (dot(normal[p], outward(centroid[p])) < 0) {
+                std::swap(tris[p][1], tris[p][2]);
+                panel_geometry(p);
+            }
     }
 
     void translate(V3 d) {
@@ -84,7 +88,7 @@ struct Mesh {
     }
 };
 
-// b appended after a (node indices of b shifted)
+// b appended after a (node indices of b shifted), geometry formed for the result
 inline Mesh merge(const Mesh& a, const Mesh& b) {
     Mesh m = a;
     const int shift = (int)a.nodes.size();
@@ -94,6 +98,24 @@ inline Mesh merge(const Mesh& a, const Mesh& b) {
     m.geometry();
     return m;
 }
+
+namespace detail {
+// merge() of two parts whose geometry is formed and current: their panels' geometry is
+// carried over (the values a new pass over the same node coordinates would give)
+inline Mesh merge_formed(const Mesh& a, const Mesh& b) {
+    Mesh m = a;
+    const int shift = (int)a.nodes.size();
+    m.nodes.insert(m.nodes.end(), b.nodes.begin(), b.nodes.end());
+    for (const auto& t : b.tris) m.tris.push_back({t[0] + shift, t[1] + shift, t[2] + shift});
+    m.tags.insert(m.tags.end(), b.tags.begin(), b.tags.end());
+    m.V.insert(m.V.end(), b.V.begin(), b.V.end());
+    m.normal.insert(m.normal.end(), b.normal.begin(), b.normal.end());
+    m.centroid.insert(m.centroid.end(), b.centroid.begin(), b.centroid.end());
+    m.area.insert(m.area.end(), b.area.begin(), b.area.end());
+    m.h.insert(m.h.end(), b.h.begin(), b.h.end());
+    return m;
+}
+}  // namespace detail
 
 // nz + 1 equally spaced axial nodes on [z0, z1]
 inline std::vector<double> uniform_z(double z0, double z1, int nz) {
@@ -173,7 +195,7 @@ inline Mesh tube_mesh(double R, const std::vector<double>& z, int nth, int nr) {
     wall.orient([](V3 x) { return V3{x.x, x.y, 0}; });
     Mesh inlet = detail::tube_cap(z.front(), INLET, -1.0, nth, nr, Rp, al);
     Mesh outlet = detail::tube_cap(z.back(), OUTLET, 1.0, nth, nr, Rp, al);
-    return merge(merge(wall, inlet), outlet);
+    return detail::merge_formed(detail::merge_formed(wall, inlet), outlet);
 }
 
 // swimmer tube: length L centred on z = 0, graded finer within |z| < a + R/2
