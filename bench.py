@@ -78,8 +78,7 @@ def cold_solve(pkg, W, prec, e):
     U = sol[0].U + W["alpha"] * sol[1].U
     Om = sol[0].Om + W["alpha"] * sol[1].Om
     fs = sol[0].f_sphere + W["alpha"] * sol[1].f_sphere
-    del s, tube
-    gc.collect()
+    del s, tube  # no reference cycles: both are freed here
     return U, Om, fs, phases
 
 

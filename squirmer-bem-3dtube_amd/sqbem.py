@@ -467,7 +467,12 @@ class Tube:
         Nt = ctypes.c_int()
         check(lib().sqbem_tube_info(self.h_, ctypes.byref(Nt), None, None))
         self.Nt = Nt.value
-        self.lu = LUFactor(lib().sqbem_tube_lu(self.h_), owner=self)
+
+    @property
+    def lu(self):
+        """Tube::lu: a view onto the factors that keeps this tube alive (made on each access,
+        so the tube holds no reference to it and is freed as soon as the caller drops it)."""
+        return LUFactor(lib().sqbem_tube_lu(self.h_), owner=self)
 
     def _times(self):
         ta, tf = ctypes.c_double(), ctypes.c_double()
