@@ -2879,14 +2879,16 @@ static void trsm_inv(int kind, int n, int nrhs, const T* L, int ldl, const T* in
 // Columns g_cols_from and beyond of the matrix being factored are written by work on another
 // stream that completes at g_cols_ev: the no-pivot recursion waits for it before it first
 // touches such a column (GetrfJob::cols_event).
-static hipEvent_t g_cols_ev = nullptr;
-static long g_cols_from = LONG_MAX;
+static hipEvent_t g_cols_ev[2] = {nullptr, nullptr};
+static long g_cols_from[2] = {LONG_MAX, LONG_MAX};
 static void cols_needed(long upto) {
-    if (g_cols_from >= upto) return;
-    const hipEvent_t ev = g_cols_ev;
-    g_cols_ev = nullptr;
-    g_cols_from = LONG_MAX;
-    SQ_HIP(hipStreamWaitEvent(ctx().s(), ev, 0));
+    for (int i = 0; i < 2; ++i) {
+        if (g_cols_from[i] >= upto) continue;
+        const hipEvent_t ev = g_cols_ev[i];
+        g_cols_ev[i] = nullptr;
+        g_cols_from[i] = LONG_MAX;
+        SQ_HIP(hipStreamWaitEvent(ctx().s(), ev, 0));
+    }
 }
 
 // ------------------------------------------------------------------ no-pivot blocked LU
@@ -4069,9 +4071,11 @@ void getrf_begin(GetrfJob<T>& j) {
     j.started = true;
     if (j.n <= 0) return;
     SQ_REQUIRE(j.lda >= j.n, SQBEM_ERR_ARG, "getrf: lda < n");
-    if (j.cols_event) {
-        g_cols_ev = j.cols_event;
-        g_cols_from = j.cols_from;
+    if (j.ncols > 0) {
+        for (int i = 0; i < 2; ++i) {
+            g_cols_ev[i] = i < j.ncols ? j.cols_event[i] : nullptr;
+            g_cols_from[i] = i < j.ncols ? j.cols_from[i] : LONG_MAX;
+        }
         // the pivoted path, and the copy kept for it, read every column at once
         if (j.n < NP_MIN || !j.restore) cols_needed(LONG_MAX);
     }

@@ -61,10 +61,12 @@ struct GetrfJob {
     // the tiled copy of the factors for the chained solves is not made by getrf_end but left
     // pending (chain_tiles_issue), so that it can run beside later latency-bound work
     bool defer_tiles = false;
-    // columns cols_from .. n-1 of A are still being written by work that completes at
-    // cols_event (another stream): the factorisation waits for it before it first touches them
-    hipEvent_t cols_event = nullptr;
-    int cols_from = 0;
+    // columns cols_from[i] .. n-1 of A are still being written by work that completes at
+    // cols_event[i] (another stream; ncols ranges, cols_from increasing): the factorisation
+    // waits for each before it first touches one of its columns
+    hipEvent_t cols_event[2] = {nullptr, nullptr};
+    int cols_from[2] = {0, 0};
+    int ncols = 0;
     DBuf<int> flag;
     DBuf<T> backup, inv;
     bool started = false;

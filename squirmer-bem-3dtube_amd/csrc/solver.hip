@@ -465,26 +465,36 @@ void sq::tube_build(sqbem_tube* t) {
     // it first touches one. The assembly allocates nothing, so the side stream holds no block
     // the library stream could take.
     const bool split = mixed(t->prec) && nt >= 8192 && ctx().side && !ctx().redirect && !capturing();
-    const int Ps = split ? std::min(Nt, (getrf_top_split(nt) + 2) / 3) : Nt;
-    assemble(t->dm, 0, Ps, t->dm.cen_xyz.p, Nt, t->self.p, bem::EIGHT_PI * t->mu, t->T64.p, nt, 1.0, nullptr, 0,
+    // panel boundaries at or past the factorisation's first split and its left part's split
+    const int w1 = split ? getrf_top_split(nt) : nt;
+    const int P2 = split ? std::min(Nt, (w1 + 2) / 3) : Nt;
+    const int P1 = split ? std::min(P2, (getrf_top_split(w1) + 2) / 3) : Nt;
+    assemble(t->dm, 0, P1, t->dm.cen_xyz.p, Nt, t->self.p, bem::EIGHT_PI * t->mu, t->T64.p, nt, 1.0, nullptr, 0,
              nullptr, sc, t->dm.nrm_xyz.p, t->T32.p, nt);
-    if (split && Ps < Nt) {
+    int nparts = 0;
+    if (split && P1 < Nt) {
         if (!t->fork_ev) SQ_HIP(hipEventCreateWithFlags(&t->fork_ev, hipEventDisableTiming));
-        if (!t->asm_ev) SQ_HIP(hipEventCreateWithFlags(&t->asm_ev, hipEventDisableTiming));
+        for (hipEvent_t& e : t->asm_ev)
+            if (!e) SQ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         SQ_HIP(hipEventRecord(t->fork_ev, ctx().s()));  // after the uploads
         SQ_HIP(hipStreamWaitEvent(ctx().side, t->fork_ev, 0));
+        const int bounds[3] = {P1, P2, Nt};
         ctx().redirect = ctx().side;
         try {
-            assemble(t->dm, Ps, Nt - Ps, t->dm.cen_xyz.p, Nt, t->self.p, bem::EIGHT_PI * t->mu,
-                     t->T64.p + (size_t)3 * Ps * nt, nt, 1.0, nullptr, 0, nullptr, sc, t->dm.nrm_xyz.p,
-                     t->T32.p + (size_t)3 * Ps * nt, nt);
+            for (int q = 0; q < 2; ++q) {
+                if (bounds[q + 1] <= bounds[q]) continue;
+                assemble(t->dm, bounds[q], bounds[q + 1] - bounds[q], t->dm.cen_xyz.p, Nt, t->self.p,
+                         bem::EIGHT_PI * t->mu, t->T64.p + (size_t)3 * bounds[q] * nt, nt, 1.0, nullptr, 0, nullptr,
+                         sc, t->dm.nrm_xyz.p, t->T32.p + (size_t)3 * bounds[q] * nt, nt);
+                SQ_HIP(hipEventRecord(t->asm_ev[nparts], ctx().side));
+                t->asm_from[nparts++] = 3 * bounds[q];
+            }
         } catch (...) {
             ctx().redirect = nullptr;
             side_drain();
             throw;
         }
         ctx().redirect = nullptr;
-        SQ_HIP(hipEventRecord(t->asm_ev, ctx().side));
     }
     sync();
     double t1 = now_s();
@@ -499,10 +509,11 @@ void sq::tube_build(sqbem_tube* t) {
         j.A = t->T32.p;
         j.ipiv = t->ipiv.p;
         j.aux = t->aux32.p;
-        if (Ps < Nt) {  // columns 3 Ps .. nt-1 come from the side stream's assembly
-            j.cols_event = t->asm_ev;
-            j.cols_from = 3 * Ps;
+        for (int q = 0; q < nparts; ++q) {  // columns written by the side stream's assembly
+            j.cols_event[q] = t->asm_ev[q];
+            j.cols_from[q] = t->asm_from[q];
         }
+        j.ncols = nparts;
         // the chained solves' tiled copy is made when a swimmer's Schur complement is done
         // (beside its bordered factorisation) or when a solve first needs it
         j.defer_tiles = true;

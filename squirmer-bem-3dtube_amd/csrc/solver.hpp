@@ -22,7 +22,8 @@ struct sqbem_tube {
     sq::DBuf<int> self;      // panel index of each collocation point (the assembly's self pairs)
     // large refined tubes: the columns beyond the factorisation's first split are assembled on
     // the side stream (fork_ev: its start, asm_ev: its end), beside the left half's factorisation
-    hipEvent_t fork_ev = nullptr, asm_ev = nullptr;
+    hipEvent_t fork_ev = nullptr, asm_ev[2] = {nullptr, nullptr};
+    int asm_from[2] = {0, 0};  // first column of each side-stream part
     sq::DBuf<double> aux64;  // getrf's solve aid for the factors (inverted blocks, tiled copy)
     sq::DBuf<float> aux32;
     // BF16 mode: BF16 copies of the factors below / above the diagonal blocks (row-major /
@@ -44,7 +45,8 @@ struct sqbem_tube {
         sq::chain_tiles_forget(aux64.p);
         sq::side_drain();  // a tiled copy of the factors (or an assembly) may still run there
         if (fork_ev) (void)hipEventDestroy(fork_ev);
-        if (asm_ev) (void)hipEventDestroy(asm_ev);
+        for (hipEvent_t e : asm_ev)
+            if (e) (void)hipEventDestroy(e);
     }
 };
 
