@@ -67,7 +67,9 @@ struct Mesh {
 
     // reverse the panels whose normal points against outward(centroid); the geometry of the
     // reversed panels is formed again (the same values a full pass would give)
-    void orient(const std::function<V3(V3)>& outward) {
+    void orient(const std::function<V3(V3)>& outward) { orient_with(outward); }
+    template <typename F>
+    void orient_with(const F& outward) {
         geometry();
         for (int p = 0; p < P(); ++p)
             if (dot(normal[p], outward(centroid[p])) < 0) {
@@ -102,9 +104,11 @@ inline Mesh merge(const Mesh& a, const Mesh& b) {
 namespace detail {
 // merge() of two parts whose geometry is formed and current: their panels' geometry is
 // carried over (the values a new pass over the same node coordinates would give)
-inline Mesh merge_formed(const Mesh& a, const Mesh& b) {
-    Mesh m = a;
-    const int shift = (int)a.nodes.size();
+inline Mesh merge_formed(Mesh a, const Mesh& b) {
+    Mesh m = std::move(a);
+    const int shift = (int)m.nodes.size();
+    m.nodes.reserve(m.nodes.size() + b.nodes.size());
+    m.tris.reserve(m.tris.size() + b.tris.size());
     m.nodes.insert(m.nodes.end(), b.nodes.begin(), b.nodes.end());
     for (const auto& t : b.tris) m.tris.push_back({t[0] + shift, t[1] + shift, t[2] + shift});
     m.tags.insert(m.tags.end(), b.tags.begin(), b.tags.end());
@@ -166,7 +170,7 @@ inline Mesh tube_cap(double z0, int tag, double sg, int nth, int nr, double Rp, 
             m.tris.push_back({a, c, b});
         }
     m.tags.assign(m.tris.size(), tag);
-    m.orient([sg](V3) { return V3{0, 0, sg}; });
+    m.orient_with([sg](V3) { return V3{0, 0, sg}; });
     return m;
 }
 }  // namespace detail
@@ -192,10 +196,10 @@ inline Mesh tube_mesh(double R, const std::vector<double>& z, int nth, int nr) {
             wall.tris.push_back({a, c, d});
         }
     wall.tags.assign(wall.tris.size(), WALL);
-    wall.orient([](V3 x) { return V3{x.x, x.y, 0}; });
+    wall.orient_with([](V3 x) { return V3{x.x, x.y, 0}; });
     Mesh inlet = detail::tube_cap(z.front(), INLET, -1.0, nth, nr, Rp, al);
     Mesh outlet = detail::tube_cap(z.back(), OUTLET, 1.0, nth, nr, Rp, al);
-    return detail::merge_formed(detail::merge_formed(wall, inlet), outlet);
+    return detail::merge_formed(detail::merge_formed(std::move(wall), inlet), outlet);
 }
 
 // swimmer tube: length L centred on z = 0, graded finer within |z| < a + R/2
