@@ -240,9 +240,18 @@ def tune_info() -> dict:
 class Mesh:
     """bem::Mesh (src/utils/mesh.hpp:50-91) owned by libsqbem; geometry read back as numpy."""
 
+    _GEOMETRY = ("nodes", "tris", "tags", "V", "normal", "centroid", "area", "h")
+
     def __init__(self, handle):
         self.h_ = _obj(handle)
-        self._load()
+
+    def __getattr__(self, name):
+        # the geometry arrays are read back from the library on first access (a tube or
+        # swimmer built from the mesh needs only its handle)
+        if name in Mesh._GEOMETRY and "h_" in self.__dict__:
+            self._load()
+            return self.__dict__[name]
+        raise AttributeError(name)
 
     def _load(self):
         nn, npn = ctypes.c_int(), ctypes.c_int()
